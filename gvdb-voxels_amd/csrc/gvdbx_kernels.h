@@ -1,0 +1,171 @@
+// gvdbx_kernels.h -- host-side launchers of the gfx950 kernels.
+#pragma once
+#include "gvdbx_internal.h"
+
+struct U2 {
+    uint32_t a, b;
+};
+
+// ---------------- scan (exclusive prefix over cells) ----------------
+// Number of 4096-element tiles of a scan over n items.
+int64_t scan_tiles(int64_t n);
+
+// ---------------- topology ----------------
+void launch_bbox_init(BBoxAccum* acc, hipStream_t s);
+void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, hipStream_t s);
+
+struct ReachParams {
+    CellGrid g0;
+    int range0;
+    int use_mask;      // 1: 3x3x3 reach mask per cell; 0: no extra bricks (radius < 1)
+    float radius;
+};
+void launch_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz, ReachParams rp, uint32_t* reach0,
+                       hipStream_t s);
+// generic path for radii beyond half a brick: mark every box cell directly
+void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid g0, int range0,
+                       float radius, uint8_t* all0, hipStream_t s);
+// base0 = reach0 != 0; all0 |= base0 and every reached neighbour
+void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, uint8_t* all0, int generic,
+                         hipStream_t s);
+// level l flags from level l-1 flags (children gather)
+void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
+                     uint8_t* base_p, uint8_t* all_p, hipStream_t s);
+
+// ordered scan of (base, extra) flags of one level; partials must hold
+// scan_tiles(n)+1 entries, totals gets (#base, #extra)
+void launch_flag_scan_reduce(const uint8_t* base, const uint8_t* all, int64_t n, U2* partials, U2* totals,
+                             hipStream_t s);
+struct NodeBuildParams {
+    CellGrid g;
+    int lev;
+    int64_t range;
+    int is_leaf;
+    GNode* nodes;
+    int32_t* idx;        // cell -> node index or -1
+};
+void launch_flag_scan_apply(const uint8_t* base, const uint8_t* all, int64_t n, const U2* partials,
+                            const U2* totals, NodeBuildParams np, hipStream_t s);
+void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
+void launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+
+struct LinkParams {
+    int lev;              // child level
+    int64_t nchild;
+    const GNode* child;
+    GNode* childw;
+    const GNode* parent_nodes;
+    uint64_t* parent_clist;
+    int res_p;            // res of level lev+1
+    int64_t range_p;      // range of level lev+1
+    int logres_p;
+    int parent_is_root;
+    CellGrid gp;          // grid of level lev+1 (unused for the root)
+    const int32_t* idx_p;
+    int64_t nparent;
+};
+void launch_link(LinkParams lp, hipStream_t s);
+
+// leaf k -> mValue (closed form of getAtlasPos) and atlas map slot k
+void launch_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int brick_res, int apron,
+                         GAtlasNode* amap, int64_t nmap, hipStream_t s);
+void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6, hipStream_t s);
+
+// ---------------- point binning ----------------
+struct BinParams {
+    CellGrid gb;          // binning grid (leaf grid grown by one cell)
+    int range0;
+    // subcell list length (InsertPointsSubcell out-parameter)
+    int count_len;
+    int sc;
+    float radius;
+    CellGrid g0;
+    const uint32_t* leafbits;   // one bit per g0 cell
+};
+void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
+                      int32_t* pslot, unsigned long long* len, hipStream_t s);
+void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s);
+void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
+                             hipStream_t s);
+void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int range0,
+                        const int32_t* off, const int32_t* pslot, float4* sorted, hipStream_t s);
+void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s);
+// sort each bin's points by original index (stable per-cell order)
+void launch_bin_sort(const int32_t* off, int64_t ncell, float4* sorted, float4* tmp, hipStream_t s);
+
+// ---------------- points -> voxels ----------------
+struct GatherParams {
+    const GNode* leaves;
+    int64_t nleaf;
+    CellGrid gb;
+    const int32_t* off;       // gb.count()+1 offsets into sorted
+    const float4* sorted;
+    float* chan;              // brick-linear atlas channel
+    int brick_res;
+    int apron;
+    int mode;                 // 0 density (max), 1 level set (min)
+    int accum;
+    int sc;
+    float box_radius;         // radius given to InsertPointsSubcell
+    float radius;             // radius given to Gather*
+    float cutoff;             // voxels farther than this cannot change a value
+};
+void launch_gather(GatherParams gp, hipStream_t s);
+
+struct SplatParams {
+    const GNode* leaves;
+    int64_t nleaf;
+    CellGrid gb;
+    const int32_t* off;
+    const float4* sorted;
+    const char* weights;      // may be null
+    int64_t wstride;
+    float* chan;
+    int brick_res;
+    int apron;
+    int accum;
+    float radius;
+};
+void launch_splat_cubic(SplatParams sp, hipStream_t s);
+
+struct ApronParams {
+    const GNode* leaves;
+    int64_t nleaf;
+    CellGrid g0;
+    const int32_t* idx0;
+    int range0;
+    float* chan;
+    int brick_res;
+    float boundval;
+};
+void launch_apron(ApronParams ap, hipStream_t s);
+
+struct InsertParams {
+    CellGrid gb;
+    CellGrid g0;
+    const int32_t* idx0;
+    const int32_t* off;
+    const float4* sorted;
+    int range0;
+    int32_t* pnode;
+    int32_t* poff;
+    int64_t n;
+};
+void launch_insert_points(InsertParams ip, hipStream_t s);
+
+struct ScatterParams {
+    const GNode* leaves;
+    int64_t nleaf;
+    CellGrid gb;
+    const int32_t* off;
+    const float4* sorted;
+    float* chan;
+    int brick_res;
+    int apron;
+    int res0;
+    float radius;
+    int expand;
+};
+void launch_scatter_density(ScatterParams sp, hipStream_t s);

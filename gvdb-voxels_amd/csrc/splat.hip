@@ -1,0 +1,412 @@
+// splat.hip -- gfx950 kernels that turn binned points into voxel values.
+//
+// Every kernel works on one 8x8x8 tile of a leaf brick per wavefront: lane
+// (x,y) owns the z-column of the tile and keeps its eight voxels in
+// registers.  Candidate points come from the 27 neighbouring bins of the
+// brick (points sorted by home brick), are filtered to those whose support
+// reaches the tile, and are staged in LDS.  Results are written once to the
+// brick-linear atlas (leaf k at k*brick_res^3 floats).  No float atomics are
+// used anywhere, so every result is deterministic.
+#include "gvdbx_kernels.h"
+
+namespace {
+
+constexpr int CAND = 512;   // staged candidates per wavefront
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float rflf(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+__device__ __forceinline__ int tile_dim(int res0) { return res0 < 8 ? res0 : 8; }
+
+// Position of tile t inside its brick.
+__device__ __forceinline__ void tile_origin(int res0, int64_t t, int& ox, int& oy, int& oz) {
+    int td = tile_dim(res0);
+    int per = res0 / td;
+    ox = (int)(t % per) * td;
+    oy = (int)((t / per) % per) * td;
+    oz = (int)(t / ((int64_t)per * per)) * td;
+}
+
+// ---------------------------------------------------------------------------
+// GatherDensity / GatherLevelSet (cuda_gvdb_particles.cuh:696-773) in scatter
+// form.  Semantics: a point belongs to the list of subcell s when s lies in
+// the point's subcell box (gvdbInsertSubcell, cuda_gvdb_particles.cuh:241-270);
+// a voxel takes min (level set) or max (density) over the members of its
+// subcell.  Because sqrt and the subtraction of r are monotone, the min of
+// |p-v|-r equals sqrt(min |p-v|^2) - r bit for bit, so the kernel keeps one
+// min of squared distance per voxel.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int tiles_per_brick) {
+    __shared__ float4 cand[CAND];
+    const int lane = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int64_t k = tile / tiles_per_brick;
+    if (k >= gp.nleaf) return;
+    const int td = tile_dim(res0);
+    int tox, toy, toz;
+    tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
+    const GNode nd = gp.leaves[k];
+    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    const int lx = lane % td, ly = lane / td;
+    const bool lane_ok = ly < td;
+    const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
+    float vz[8];
+    float m[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        vz[z] = (float)(T0z + z);
+        m[z] = __builtin_inff();
+    }
+    const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
+    const float R = gp.box_radius;
+    const int sc = gp.sc;
+    const float cut = gp.cutoff;
+    int cnt = 0;
+
+    auto process = [&](int count) {
+        __syncthreads();
+        for (int j = 0; j < count; j++) {
+            float4 c = cand[j];
+            uint32_t pk = rfl(__float_as_uint(c.w));
+            float px = rflf(c.x), py = rflf(c.y), pz = rflf(c.z);
+            int xlo = pk & 7, xhi = (pk >> 3) & 7, ylo = (pk >> 6) & 7, yhi = (pk >> 9) & 7;
+            int zlo = (pk >> 12) & 7, zhi = (pk >> 15) & 7;
+            if (lane_ok && lx >= xlo && lx <= xhi && ly >= ylo && ly <= yhi) {
+                float dx = vx - px, dy = vy - py;
+                float dxy = __builtin_fmaf(dy, dy, dx * dx);
+#pragma unroll
+                for (int z = 0; z < 8; z++) {
+                    if (z >= zlo && z <= zhi) {
+                        float dz = vz[z] - pz;
+                        m[z] = fminf(m[z], __builtin_fmaf(dz, dz, dxy));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+
+    for (int dcx = -1; dcx <= 1; dcx++)
+        for (int dcy = -1; dcy <= 1; dcy++)
+            for (int dcz = -1; dcz <= 1; dcz++) {
+                int bx = cx + dcx, by = cy + dcy, bz = cz + dcz;
+                if (!gp.gb.contains(bx, by, bz)) continue;
+                int64_t ci = gp.gb.index(bx, by, bz);
+                int beg = gp.off[ci], end = gp.off[ci + 1];
+                for (int j0 = beg; j0 < end; j0 += 64) {
+                    int j = j0 + lane;
+                    bool ok = j < end;
+                    float4 p = make_float4(0, 0, 0, 0);
+                    uint32_t pk = 0;
+                    if (ok) {
+                        p = gp.sorted[j];
+                        float pw[3] = {p.x, p.y, p.z};
+                        int T0[3] = {T0x, T0y, T0z};
+                        int lo_r[3], hi_r[3];
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            // subcell box (cuda_gvdb_particles.cuh:237-242) as a voxel range
+                            int lo = ((int)(pw[a] - R) / sc) * sc;
+                            int hi = ((int)(pw[a] + 1 + R) / sc) * sc + sc - 1;
+                            lo = max(lo, (int)ceilf(pw[a] - cut));
+                            hi = min(hi, (int)floorf(pw[a] + cut));
+                            lo_r[a] = max(lo - T0[a], 0);
+                            hi_r[a] = min(hi - T0[a], td - 1);
+                            ok = ok && lo_r[a] <= hi_r[a];
+                        }
+                        pk = (uint32_t)lo_r[0] | ((uint32_t)hi_r[0] << 3) | ((uint32_t)lo_r[1] << 6) |
+                             ((uint32_t)hi_r[1] << 9) | ((uint32_t)lo_r[2] << 12) | ((uint32_t)hi_r[2] << 15);
+                    }
+                    uint64_t bal = __ballot(ok);
+                    if (ok) {
+                        int pos = cnt + (int)__popcll(bal & ((1ull << lane) - 1ull));
+                        cand[pos] = make_float4(p.x, p.y, p.z, __uint_as_float(pk));
+                    }
+                    cnt += (int)__popcll(bal);
+                    if (cnt > CAND - 64) {
+                        process(cnt);
+                        cnt = 0;
+                    }
+                }
+            }
+    if (cnt > 0) process(cnt);
+
+    if (!lane_ok) return;
+    const int br = gp.brick_res, a = gp.apron;
+    float* brick = gp.chan + (int64_t)k * br * br * br;
+    const float r = gp.radius;
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        if (z < td) {
+            int64_t vi = ((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a);
+            float init = gp.accum ? brick[vi] : (gp.mode == 1 ? 3.0f : 0.0f);
+            float c = sqrtf(m[z]);
+            float val = gp.mode == 1 ? fminf(init, c - r) : fmaxf(init, r - c);
+            brick[vi] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weighted cubic splat: rho(v) = sum_p w_p * max(0, 1 - |v-p|^2/r^2)^3 at the
+// integer voxel coordinates of every leaf (extension of distFunc,
+// cuda_gvdb_particles.cuh:516-523).  Sums are taken in a fixed order
+// (neighbour bins in a fixed order, then ascending point index within a
+// bin), so results are deterministic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, int tiles_per_brick) {
+    __shared__ float4 cand[CAND];
+    const int lane = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int64_t k = tile / tiles_per_brick;
+    if (k >= sp.nleaf) return;
+    const int td = tile_dim(res0);
+    int tox, toy, toz;
+    tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
+    const GNode nd = sp.leaves[k];
+    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    const int lx = lane % td, ly = lane / td;
+    const bool lane_ok = ly < td;
+    const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
+    const int br = sp.brick_res, a = sp.apron;
+    float* brick = sp.chan + (int64_t)k * br * br * br;
+    float vz[8], acc[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        vz[z] = (float)(T0z + z);
+        acc[z] = 0.0f;
+        if (sp.accum && lane_ok && z < td)
+            acc[z] = brick[((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a)];
+    }
+    const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
+    const float r = sp.radius;
+    const float r2 = r * r;
+    int cnt = 0;
+
+    auto process = [&](int count) {
+        __syncthreads();
+        for (int j = 0; j < count; j++) {
+            float4 c = cand[j];
+            float px = rflf(c.x), py = rflf(c.y), pz = rflf(c.z), wt = rflf(c.w);
+            int zlo = max((int)ceilf(pz - r) - T0z, 0), zhi = min((int)floorf(pz + r) - T0z, td - 1);
+            if (lane_ok) {
+                float dx = vx - px, dy = vy - py;
+                float dxy = __builtin_fmaf(dy, dy, dx * dx);
+                if (dxy < r2) {
+#pragma unroll
+                    for (int z = 0; z < 8; z++) {
+                        if (z >= zlo && z <= zhi) {
+                            float dz = vz[z] - pz;
+                            float d2 = __builtin_fmaf(dz, dz, dxy);
+                            if (d2 < r2) {
+                                float t = 1.0f - d2 / r2;
+                                float kk = (t * t) * t;
+                                acc[z] = acc[z] + wt * kk;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+
+    for (int dcx = -1; dcx <= 1; dcx++)
+        for (int dcy = -1; dcy <= 1; dcy++)
+            for (int dcz = -1; dcz <= 1; dcz++) {
+                int bx = cx + dcx, by = cy + dcy, bz = cz + dcz;
+                if (!sp.gb.contains(bx, by, bz)) continue;
+                int64_t ci = sp.gb.index(bx, by, bz);
+                int beg = sp.off[ci], end = sp.off[ci + 1];
+                for (int j0 = beg; j0 < end; j0 += 64) {
+                    int j = j0 + lane;
+                    bool ok = j < end;
+                    float4 p = make_float4(0, 0, 0, 0);
+                    float wt = 1.0f;
+                    if (ok) {
+                        p = sp.sorted[j];
+                        float pw[3] = {p.x, p.y, p.z};
+                        int T0[3] = {T0x, T0y, T0z};
+#pragma unroll
+                        for (int a2 = 0; a2 < 3; a2++) {
+                            int lo = (int)ceilf(pw[a2] - r) - T0[a2];
+                            int hi = (int)floorf(pw[a2] + r) - T0[a2];
+                            ok = ok && hi >= 0 && lo <= td - 1;
+                        }
+                        if (sp.weights) wt = *(const float*)(sp.weights + (int64_t)__float_as_int(p.w) * sp.wstride);
+                    }
+                    uint64_t bal = __ballot(ok);
+                    if (ok) {
+                        int pos = cnt + (int)__popcll(bal & ((1ull << lane) - 1ull));
+                        cand[pos] = make_float4(p.x, p.y, p.z, wt);
+                    }
+                    cnt += (int)__popcll(bal);
+                    if (cnt > CAND - 64) {
+                        process(cnt);
+                        cnt = 0;
+                    }
+                }
+            }
+    if (cnt > 0) process(cnt);
+    if (!lane_ok) return;
+#pragma unroll
+    for (int z = 0; z < 8; z++)
+        if (z < td) brick[((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a)] = acc[z];
+}
+
+// ---------------------------------------------------------------------------
+// UpdateApron (cuda_gvdb_operators.cuh:73-126) as a neighbour-brick gather:
+// apron voxel (x,y,z) of brick k holds the value of world voxel
+// mPos + (x,y,z) - 1 in whichever leaf contains it, else boundval.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
+    const int64_t k = blockIdx.x;
+    if (k >= ap.nleaf) return;
+    const int br = ap.brick_res;
+    const int r0 = ap.range0;
+    const int a = (br - r0) / 2;
+    const GNode nd = ap.leaves[k];
+    float* brick = ap.chan + k * (int64_t)br * br * br;
+    const int nv = br * br * br;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        int x = i % br, y = (i / br) % br, z = i / (br * br);
+        bool interior = x >= a && y >= a && z >= a && x < a + r0 && y < a + r0 && z < a + r0;
+        if (interior) continue;
+        int ux = nd.mPos[0] + x - a, uy = nd.mPos[1] + y - a, uz = nd.mPos[2] + z - a;
+        float v = ap.boundval;
+        if (ux >= 0 && uy >= 0 && uz >= 0) {
+            int cx = ux / r0, cy = uy / r0, cz = uz / r0;
+            if (ap.g0.contains(cx, cy, cz)) {
+                int32_t leaf = ap.idx0[ap.g0.index(cx, cy, cz)];
+                if (leaf >= 0 && leaf < ap.nleaf) {
+                    const int32_t* lp = ap.leaves[leaf].mPos;
+                    v = ap.chan[(int64_t)leaf * nv + ((int64_t)(uz - lp[2] + a) * br + (uy - lp[1] + a)) * br +
+                                (ux - lp[0] + a)];
+                }
+            }
+        }
+        brick[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// InsertPoints (cuda_gvdb_particles.cuh:19-36): point -> leaf index and its
+// rank among the leaf's points in ascending point order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_insert_points(InsertParams ip) {
+    int64_t total = ip.off[ip.gb.count()];
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
+        float4 p = ip.sorted[j];
+        int i = __float_as_int(p.w);
+        int c[3] = {gvdbx_cover(p.x, ip.range0) / ip.range0, gvdbx_cover(p.y, ip.range0) / ip.range0,
+                    gvdbx_cover(p.z, ip.range0) / ip.range0};
+        int32_t leaf = -1;
+        if (ip.g0.contains(c[0], c[1], c[2])) leaf = ip.idx0[ip.g0.index(c[0], c[1], c[2])];
+        if (leaf >= 0) {
+            ip.pnode[i] = leaf;
+            ip.poff[i] = (int32_t)(j - ip.off[ip.gb.index(c[0], c[1], c[2])]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ScatterDensity (cuda_gvdb_particles.cuh:525-573) without the surface race:
+// each point adds the unclamped cubic distFunc to its home voxel and, with
+// expand, to the six face neighbours (which may be apron voxels of the same
+// brick).  Each brick's own points are visited in ascending point order, so
+// every voxel sums in that order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dist_func(float ax, float ay, float az, float bx, float by, float bz, float r) {
+    bx -= ax; by -= ay; bz -= az;
+    float c = gvdbx_d2(bx, by, bz) / (r * r);
+    float t = 3.0f - c;
+    t = c * t;
+    t = -3.0f + t;
+    t = c * t;
+    return 1.0f + t;
+}
+
+__global__ __launch_bounds__(128) void k_scatter_density(ScatterParams sp) {
+    const int64_t k = blockIdx.x;
+    if (k >= sp.nleaf) return;
+    const int br = sp.brick_res, a = sp.apron, r0 = sp.res0;
+    const GNode nd = sp.leaves[k];
+    float* brick = sp.chan + k * (int64_t)br * br * br;
+    int c[3] = {nd.mPos[0] / r0, nd.mPos[1] / r0, nd.mPos[2] / r0};
+    int beg = 0, end = 0;
+    if (sp.gb.contains(c[0], c[1], c[2])) {
+        int64_t ci = sp.gb.index(c[0], c[1], c[2]);
+        beg = sp.off[ci];
+        end = sp.off[ci + 1];
+    }
+    // each thread owns whole z-columns (qx,qy) of the padded brick
+    for (int col = threadIdx.x; col < br * br; col += blockDim.x) {
+        const int qx = col % br, qy = col / br;
+        for (int j = beg; j < end; j++) {
+            float4 w = sp.sorted[j];
+            float p[3] = {w.x - (float)nd.mPos[0], w.y - (float)nd.mPos[1], w.z - (float)nd.mPos[2]};
+            float pi[3] = {(float)(int)p[0], (float)(int)p[1], (float)(int)p[2]};
+            if (pi[0] < 0 || pi[1] < 0 || pi[2] < 0 || pi[0] >= r0 || pi[1] >= r0 || pi[2] >= r0) continue;
+            int ix = (int)pi[0] + a, iy = (int)pi[1] + a, iz = (int)pi[2] + a;
+            float* colp = brick + (int64_t)qy * br + qx;   // voxel z at colp[z*br*br]
+            const int64_t zs = (int64_t)br * br;
+            if (qx == ix && qy == iy) {
+                colp[iz * zs] = colp[iz * zs] + dist_func(p[0], p[1], p[2], pi[0], pi[1], pi[2], sp.radius);
+                if (sp.expand) {
+                    colp[(iz - 1) * zs] = colp[(iz - 1) * zs] + dist_func(p[0], p[1], p[2], pi[0], pi[1], pi[2] - 1, sp.radius);
+                    colp[(iz + 1) * zs] = colp[(iz + 1) * zs] + dist_func(p[0], p[1], p[2], pi[0], pi[1], pi[2] + 1, sp.radius);
+                }
+            } else if (sp.expand) {
+                float d = 0.0f;
+                bool hit = true;
+                if (qy == iy && qx == ix - 1) d = dist_func(p[0], p[1], p[2], pi[0] - 1, pi[1], pi[2], sp.radius);
+                else if (qy == iy && qx == ix + 1) d = dist_func(p[0], p[1], p[2], pi[0] + 1, pi[1], pi[2], sp.radius);
+                else if (qx == ix && qy == iy - 1) d = dist_func(p[0], p[1], p[2], pi[0], pi[1] - 1, pi[2], sp.radius);
+                else if (qx == ix && qy == iy + 1) d = dist_func(p[0], p[1], p[2], pi[0], pi[1] + 1, pi[2], sp.radius);
+                else hit = false;
+                if (hit) colp[iz * zs] = colp[iz * zs] + d;
+            }
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+inline int res0_of(const CellGrid&, int r0) { return r0; }
+
+}  // namespace
+
+void launch_gather(GatherParams gp, hipStream_t s) {
+    int res0 = gp.brick_res - 2 * gp.apron;
+    int td = res0 < 8 ? res0 : 8;
+    int per = res0 / td;
+    int tiles = per * per * per;
+    int64_t nblk = gp.nleaf * tiles;
+    if (nblk > 0) hipLaunchKernelGGL(k_gather, dim3((unsigned)nblk), dim3(64), 0, s, gp, res0, tiles);
+}
+
+void launch_splat_cubic(SplatParams sp, hipStream_t s) {
+    int res0 = sp.brick_res - 2 * sp.apron;
+    int td = res0 < 8 ? res0 : 8;
+    int per = res0 / td;
+    int tiles = per * per * per;
+    int64_t nblk = sp.nleaf * tiles;
+    if (nblk > 0) hipLaunchKernelGGL(k_splat_cubic, dim3((unsigned)nblk), dim3(64), 0, s, sp, res0, tiles);
+}
+
+void launch_apron(ApronParams ap, hipStream_t s) {
+    if (ap.nleaf > 0) hipLaunchKernelGGL(k_apron, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
+}
+
+void launch_insert_points(InsertParams ip, hipStream_t s) {
+    hipLaunchKernelGGL(k_insert_points, dim3(grid_for(ip.n, 256)), dim3(256), 0, s, ip);
+}
+
+void launch_scatter_density(ScatterParams sp, hipStream_t s) {
+    if (sp.nleaf > 0) hipLaunchKernelGGL(k_scatter_density, dim3((unsigned)sp.nleaf), dim3(128), 0, s, sp);
+}

@@ -1,0 +1,650 @@
+// topology.hip -- gfx950 kernels that rebuild the GVDB tree from points.
+//
+// Design: instead of sorting N*depth 64-bit keys (the reference's Thrust
+// path, VG.cpp:904-928), every level of the tree is a dense cell grid over
+// the point extents.  One atomicOr per point records the cell it falls in
+// plus a 27-bit mask of the neighbouring cells its extra-brick box reaches
+// (gvdbCalcExtraBrickId, cuda_gvdb_particles.cuh:467-512).  Ordered scans
+// over the grids (x-major, then y, then z == the reference key order
+// lev<<48|x<<32|y<<16|z) give each node its pool index directly: base nodes
+// first, extra nodes appended, exactly the order ActivateBricksGPU +
+// ActivateExtraBricksGPU produce (VG.cpp:1288-1331, 966-988).
+#include "gvdbx_kernels.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ U2 operator+(U2 x, U2 y) { return U2{x.a + y.a, x.b + y.b}; }
+__device__ __forceinline__ U2 operator-(U2 x, U2 y) { return U2{x.a - y.a, x.b - y.b}; }
+
+__device__ __forceinline__ uint32_t shfl_up_t(uint32_t v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ U2 shfl_up_t(U2 v, int d) { return U2{__shfl_up(v.a, d, 64), __shfl_up(v.b, d, 64)}; }
+
+template <class T>
+__device__ __forceinline__ T zero_t();
+template <>
+__device__ __forceinline__ uint32_t zero_t<uint32_t>() { return 0u; }
+template <>
+__device__ __forceinline__ U2 zero_t<U2>() { return U2{0u, 0u}; }
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+    int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = shfl_up_t(v, d);
+        if (lane >= d) v = v + o;
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread across the block; NW = waves per block
+template <class T, int NW>
+__device__ __forceinline__ T block_excl_scan(T v, T* lds, T& total) {
+    T inc = wave_incl_scan(v);
+    int w = threadIdx.x >> 6;
+    if (lane_id() == 63) lds[w] = inc;
+    __syncthreads();
+    T wofs = zero_t<T>(), tot = zero_t<T>();
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        if (i < w) wofs = wofs + lds[i];
+        tot = tot + lds[i];
+    }
+    __syncthreads();
+    total = tot;
+    return wofs + inc - v;
+}
+
+// ---- scan over tiles: phase 2 (single block) ----
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_partials(T* partials, int64_t ntiles, T* total) {
+    __shared__ T lds[16];
+    int64_t per = (ntiles + 1023) / 1024;
+    int64_t b = (int64_t)threadIdx.x * per;
+    T s = zero_t<T>();
+    for (int64_t i = 0; i < per; i++)
+        if (b + i < ntiles) s = s + partials[b + i];
+    T tot;
+    T ex = block_excl_scan<T, 16>(s, lds, tot);
+    T run = ex;
+    for (int64_t i = 0; i < per; i++)
+        if (b + i < ntiles) {
+            T x = partials[b + i];
+            partials[b + i] = run;
+            run = run + x;
+        }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+// ---- flag scan (base, extra) ----
+__device__ __forceinline__ U2 flag_val(const uint8_t* base, const uint8_t* all, int64_t i, int64_t n) {
+    if (i >= n) return U2{0, 0};
+    uint32_t b = base[i] ? 1u : 0u;
+    uint32_t e = (!b && all[i]) ? 1u : 0u;
+    return U2{b, e};
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_flag_reduce(const uint8_t* base, const uint8_t* all, int64_t n,
+                                                                U2* partials) {
+    __shared__ U2 lds[4];
+    int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE;
+    U2 s{0, 0};
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) s = s + flag_val(base, all, t0 + k * SCAN_THREADS + threadIdx.x, n);
+    U2 tot;
+    block_excl_scan<U2, 4>(s, lds, tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_flag_apply(const uint8_t* base, const uint8_t* all, int64_t n,
+                                                               const U2* partials, const U2* totals,
+                                                               NodeBuildParams np) {
+    __shared__ U2 lds[4];
+    int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    U2 v[SCAN_ITEMS];
+    U2 s{0, 0};
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = flag_val(base, all, t0 + k, n);
+        s = s + v[k];
+    }
+    U2 tot;
+    U2 run = block_excl_scan<U2, 4>(s, lds, tot) + partials[blockIdx.x];
+    uint32_t nbase = totals->a;
+    const CellGrid& g = np.g;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int64_t i = t0 + k;
+        if (i < n) {
+            int32_t node = -1;
+            if (v[k].a) node = (int32_t)run.a;
+            else if (v[k].b) node = (int32_t)(nbase + run.b);
+            np.idx[i] = node;
+            if (node >= 0) {
+                int64_t yz = (int64_t)g.ny * g.nz;
+                int cx = (int)(i / yz) + g.ox;
+                int cy = (int)((i / g.nz) % g.ny) + g.oy;
+                int cz = (int)(i % g.nz) + g.oz;
+                GNode nd;
+                nd.mLev = (uint8_t)np.lev;
+                nd.mFlags = 1;
+                nd.mPriority = 0;
+                nd.pad = 0;
+                nd.mPos[0] = (int32_t)(cx * np.range);
+                nd.mPos[1] = (int32_t)(cy * np.range);
+                nd.mPos[2] = (int32_t)(cz * np.range);
+                nd.mValue[0] = nd.mValue[1] = nd.mValue[2] = -1;
+                nd.mVRange[0] = nd.mVRange[1] = nd.mVRange[2] = 0.0f;
+                nd.mParent = GVDBX_ID_UNDEFL;
+                nd.mChildList = np.is_leaf ? (uint64_t)GVDBX_ID_UNDEFL
+                                           : (1ull | ((uint64_t)np.lev << 8) | ((uint64_t)node << 16));
+                nd.mMask = 0;
+                np.nodes[node] = nd;
+            }
+        }
+        run = run + v[k];
+    }
+}
+
+// ---- count scan (int32 counts -> int32 exclusive offsets, off[n] = total) ----
+__global__ __launch_bounds__(SCAN_THREADS) void k_count_reduce(const int32_t* cnt, int64_t n, uint32_t* partials) {
+    __shared__ uint32_t lds[4];
+    int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int64_t i = t0 + k * SCAN_THREADS + threadIdx.x;
+        if (i < n) s += (uint32_t)cnt[i];
+    }
+    uint32_t tot;
+    block_excl_scan<uint32_t, 4>(s, lds, tot);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_count_apply(const int32_t* cnt, int64_t n,
+                                                                const uint32_t* partials, int32_t* off) {
+    __shared__ uint32_t lds[4];
+    int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int64_t i = t0 + k;
+        v[k] = i < n ? (uint32_t)cnt[i] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<uint32_t, 4>(s, lds, tot) + partials[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        int64_t i = t0 + k;
+        if (i < n) off[i] = (int32_t)run;
+        if (i == n) off[i] = (int32_t)run;
+        run += v[k];
+    }
+}
+
+// ---- bounding box ----
+__global__ void k_bbox_init(BBoxAccum* acc) {
+    if (threadIdx.x < 3) {
+        acc->mn[threadIdx.x] = 0x7F800000;   // +inf bits
+        acc->mx[threadIdx.x] = -1;
+    }
+    if (threadIdx.x == 0) acc->nvalid = 0;
+}
+
+__global__ __launch_bounds__(256) void k_bbox(PointSrc src, int64_t n, float tx, float ty, float tz,
+                                              BBoxAccum* acc) {
+    float mn0 = __builtin_inff(), mn1 = __builtin_inff(), mn2 = __builtin_inff();
+    float mx0 = -__builtin_inff(), mx1 = -__builtin_inff(), mx2 = -__builtin_inff();
+    unsigned long long cnt = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float x, y, z;
+        if (!gvdbx_point(src, i, tx, ty, tz, x, y, z)) continue;
+        cnt++;
+        mn0 = fminf(mn0, x); mn1 = fminf(mn1, y); mn2 = fminf(mn2, z);
+        mx0 = fmaxf(mx0, x); mx1 = fmaxf(mx1, y); mx2 = fmaxf(mx2, z);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        mn0 = fminf(mn0, __shfl_xor(mn0, d, 64));
+        mn1 = fminf(mn1, __shfl_xor(mn1, d, 64));
+        mn2 = fminf(mn2, __shfl_xor(mn2, d, 64));
+        mx0 = fmaxf(mx0, __shfl_xor(mx0, d, 64));
+        mx1 = fmaxf(mx1, __shfl_xor(mx1, d, 64));
+        mx2 = fmaxf(mx2, __shfl_xor(mx2, d, 64));
+        cnt += __shfl_xor(cnt, d, 64);
+    }
+    if (lane_id() == 0 && cnt > 0) {
+        atomicMin(&acc->mn[0], __float_as_int(mn0));
+        atomicMin(&acc->mn[1], __float_as_int(mn1));
+        atomicMin(&acc->mn[2], __float_as_int(mn2));
+        atomicMax(&acc->mx[0], __float_as_int(mx0));
+        atomicMax(&acc->mx[1], __float_as_int(mx1));
+        atomicMax(&acc->mx[2], __float_as_int(mx2));
+        atomicAdd(&acc->nvalid, cnt);
+    }
+}
+
+// ---- occupancy + reach ----
+__global__ __launch_bounds__(256) void k_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz,
+                                                    ReachParams rp, uint32_t* reach0) {
+    const int r0 = rp.range0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float w[3];
+        if (!gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) continue;
+        int c[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) c[a] = gvdbx_cover(w[a], r0) / r0;
+        if (!rp.g0.contains(c[0], c[1], c[2])) continue;
+        uint32_t m = 1u << 13;
+        if (rp.use_mask) {
+            int lo[3], hi[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                // extra-brick box of gvdbCalcExtraBrickId (cuda_gvdb_particles.cuh:493-494)
+                int ix = (int)w[a];
+                int bmn = (int)((float)ix - rp.radius);
+                int bmx = (int)((float)ix + 2.0f * rp.radius);
+                lo[a] = max(bmn / r0 - c[a], -1);
+                hi[a] = min(bmx / r0 - c[a], 1);
+            }
+            m = 0;
+            for (int dz = lo[2]; dz <= hi[2]; dz++)
+                for (int dy = lo[1]; dy <= hi[1]; dy++)
+                    for (int dx = lo[0]; dx <= hi[0]; dx++) m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+        }
+        atomicOr(&reach0[rp.g0.index(c[0], c[1], c[2])], m);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz,
+                                                    CellGrid g0, int r0, float radius, uint8_t* all0) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float w[3];
+        if (!gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) continue;
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            int ix = (int)w[a];
+            int bmn = (int)((float)ix - radius);
+            int bmx = (int)((float)ix + 2.0f * radius);
+            lo[a] = bmn / r0;
+            hi[a] = bmx / r0;
+        }
+        for (int x = lo[0]; x <= hi[0]; x++)
+            for (int y = lo[1]; y <= hi[1]; y++)
+                for (int z = lo[2]; z <= hi[2]; z++)
+                    if (g0.contains(x, y, z)) all0[g0.index(x, y, z)] = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0,
+                                                      uint8_t* all0, int generic) {
+    int64_t n = g0.count();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = reach0[i];
+        base0[i] = m ? 1 : 0;
+        if (!m) continue;
+        all0[i] = 1;
+        if (generic) continue;
+        int64_t yz = (int64_t)g0.ny * g0.nz;
+        int cx = (int)(i / yz) + g0.ox, cy = (int)((i / g0.nz) % g0.ny) + g0.oy, cz = (int)(i % g0.nz) + g0.oz;
+        m &= ~(1u << 13);
+        while (m) {
+            int b = __ffs(m) - 1;
+            m &= m - 1;
+            int dx = b % 3 - 1, dy = (b / 3) % 3 - 1, dz = b / 9 - 1;
+            if (g0.contains(cx + dx, cy + dy, cz + dz)) all0[g0.index(cx + dx, cy + dy, cz + dz)] = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c,
+                                                  CellGrid gp, int logres, uint8_t* base_p, uint8_t* all_p) {
+    int64_t n = gp.count();
+    int res = 1 << logres;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t yz = (int64_t)gp.ny * gp.nz;
+        int px = (int)(i / yz) + gp.ox, py = (int)((i / gp.nz) % gp.ny) + gp.oy, pz = (int)(i % gp.nz) + gp.oz;
+        int x0 = max(px * res, gc.ox), x1 = min(px * res + res, gc.ox + gc.nx);
+        int y0 = max(py * res, gc.oy), y1 = min(py * res + res, gc.oy + gc.ny);
+        int z0 = max(pz * res, gc.oz), z1 = min(pz * res + res, gc.oz + gc.nz);
+        uint8_t b = 0, a = 0;
+        for (int x = x0; x < x1 && !(b && a); x++)
+            for (int y = y0; y < y1; y++)
+                for (int z = z0; z < z1; z++) {
+                    int64_t ci = gc.index(x, y, z);
+                    b |= base_c[ci];
+                    a |= all_c[ci];
+                }
+        base_p[i] = b;
+        all_p[i] = a | b;
+    }
+}
+
+// ---- fills ----
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T* p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---- link (gvdbLinkBricks cuda_gvdb_nodes.cuh:470-505 for levels below top_lev) ----
+__global__ __launch_bounds__(256) void k_link(LinkParams lp) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= lp.nchild) return;
+    const GNode& c = lp.child[k];
+    int64_t p = 0;
+    if (!lp.parent_is_root) {
+        int pc[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) pc[a] = (int)(c.mPos[a] / lp.range_p);
+        if (!lp.gp.contains(pc[0], pc[1], pc[2])) return;
+        p = lp.idx_p[lp.gp.index(pc[0], pc[1], pc[2])];
+        if (p < 0 || p >= lp.nparent) return;
+    }
+    const GNode& pn = lp.parent_nodes[p];
+    int64_t pin[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) pin[a] = ((int64_t)(c.mPos[a] - pn.mPos[a]) * lp.res_p) / lp.range_p;
+    if (pin[0] < 0 || pin[1] < 0 || pin[2] < 0 || pin[0] >= lp.res_p || pin[1] >= lp.res_p || pin[2] >= lp.res_p) return;
+    int64_t bit = (pin[2] * lp.res_p + pin[1]) * lp.res_p + pin[0];
+    lp.childw[k].mParent = ((uint64_t)p << 16) | ((uint64_t)(lp.lev + 1) << 8);
+    uint64_t cndx = pn.mChildList >> 16;
+    int64_t len = (int64_t)lp.res_p * lp.res_p * lp.res_p;
+    lp.parent_clist[(int64_t)cndx * len + bit] = ((uint64_t)k << 16) | ((uint64_t)lp.lev << 8);
+}
+
+// ---- atlas (UpdateAtlas VG.cpp:2630-2704, getAtlasPos AL.cpp:705-715) ----
+__global__ __launch_bounds__(256) void k_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int br, int apron,
+                                                      GAtlasNode* amap, int64_t nmap) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nmap; k += (int64_t)gridDim.x * blockDim.x) {
+        GAtlasNode an;
+        if (k < nleaf) {
+            int64_t a2 = (int64_t)sx * sy;
+            int64_t z = k / a2, rem = k - z * a2;
+            int64_t y = rem / sx, x = rem - y * sx;
+            GNode& nd = leaves[k];
+            nd.mValue[0] = (int32_t)(x * br + apron);
+            nd.mValue[1] = (int32_t)(y * br + apron);
+            nd.mValue[2] = (int32_t)(z * br + apron);
+            an.mPos[0] = nd.mPos[0]; an.mPos[1] = nd.mPos[1]; an.mPos[2] = nd.mPos[2];
+            an.mLeafID = (int32_t)k;
+        } else {
+            an.mPos[0] = an.mPos[1] = an.mPos[2] = (int32_t)GVDBX_ID_UNDEFL;
+            an.mLeafID = (int32_t)GVDBX_ID_UNDEFL;
+        }
+        amap[k] = an;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6) {
+    int mn[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nleaf; k += (int64_t)gridDim.x * blockDim.x) {
+        if (!leaves[k].mFlags) continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = min(mn[a], leaves[k].mPos[a]);
+            mx[a] = max(mx[a], leaves[k].mPos[a] + range0);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
+            mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
+        }
+    if (lane_id() == 0)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&out6[a], mn[a]);
+            atomicMax(&out6[3 + a], mx[a]);
+        }
+}
+
+// ---- point binning ----
+__global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp,
+                                                   int32_t* cnt, int32_t* pslot, unsigned long long* len) {
+    const int r0 = bp.range0;
+    unsigned long long lsum = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float w[3];
+        int32_t slot = -1;
+        if (gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) {
+            int c[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) c[a] = gvdbx_cover(w[a], r0) / r0;
+            if (bp.gb.contains(c[0], c[1], c[2])) slot = atomicAdd(&cnt[bp.gb.index(c[0], c[1], c[2])], 1);
+            if (bp.count_len) {
+                // subcell box of gvdbCountSubcell (cuda_gvdb_particles.cuh:288-293)
+                int lo[3], hi[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = ((int)(w[a] - bp.radius) / bp.sc) * bp.sc;
+                    hi[a] = ((int)(w[a] + 1 + bp.radius) / bp.sc) * bp.sc;
+                }
+                for (int x = lo[0]; x <= hi[0]; x += bp.sc) {
+                    if (x < 0) continue;
+                    int cx = x / r0;
+                    for (int y = lo[1]; y <= hi[1]; y += bp.sc) {
+                        if (y < 0) continue;
+                        int cy = y / r0;
+                        for (int z = lo[2]; z <= hi[2]; z += bp.sc) {
+                            if (z < 0) continue;
+                            int cz = z / r0;
+                            if (!bp.g0.contains(cx, cy, cz)) continue;
+                            int64_t ci = bp.g0.index(cx, cy, cz);
+                            if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum++;
+                        }
+                    }
+                }
+            }
+        }
+        pslot[i] = slot;
+    }
+    if (bp.count_len) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) lsum += __shfl_xor(lsum, d, 64);
+        if (lane_id() == 0 && lsum) atomicAdd(len, lsum);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz,
+                                                     CellGrid gb, int r0, const int32_t* off, const int32_t* pslot,
+                                                     float4* sorted) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t slot = pslot[i];
+        if (slot < 0) continue;
+        float w[3];
+        gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2]);
+        int c[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) c[a] = gvdbx_cover(w[a], r0) / r0;
+        int64_t dst = (int64_t)off[gb.index(c[0], c[1], c[2])] + slot;
+        sorted[dst] = make_float4(w[0], w[1], w[2], __int_as_float((int)i));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits) {
+    int64_t nw = (ncell + 31) / 32;
+    for (int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < nw; wi += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t b = 0;
+        for (int k = 0; k < 32; k++) {
+            int64_t c = wi * 32 + k;
+            if (c < ncell && idx0[c] >= 0) b |= 1u << k;
+        }
+        bits[wi] = b;
+    }
+}
+
+// per-cell stable order: sort each bin by original point index.  Small bins
+// are sorted in LDS (bitonic); large bins use a rank count in global memory.
+constexpr int BINSORT_LDS = 2048;
+__global__ __launch_bounds__(256) void k_bin_sort(const int32_t* off, int64_t ncell, float4* sorted, float4* tmp) {
+    __shared__ float4 s[BINSORT_LDS];
+    for (int64_t c = blockIdx.x; c < ncell; c += gridDim.x) {
+        int64_t b = off[c], e = off[c + 1];
+        int64_t m = e - b;
+        if (m <= 1) continue;
+        if (m <= BINSORT_LDS) {
+            int p2 = 1;
+            while (p2 < m) p2 <<= 1;
+            for (int i = threadIdx.x; i < p2; i += blockDim.x)
+                s[i] = i < m ? sorted[b + i] : make_float4(0, 0, 0, __int_as_float(0x7FFFFFFF));
+            __syncthreads();
+            for (int k = 2; k <= p2; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = threadIdx.x; i < p2; i += blockDim.x) {
+                        int ixj = i ^ j;
+                        if (ixj > i) {
+                            int ki = __float_as_int(s[i].w), kj = __float_as_int(s[ixj].w);
+                            bool up = (i & k) == 0;
+                            if ((ki > kj) == up) {
+                                float4 t = s[i];
+                                s[i] = s[ixj];
+                                s[ixj] = t;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (int i = threadIdx.x; i < m; i += blockDim.x) sorted[b + i] = s[i];
+            __syncthreads();
+        } else {
+            for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+                float4 v = sorted[b + i];
+                int key = __float_as_int(v.w);
+                int64_t rank = 0;
+                for (int64_t j = 0; j < m; j++) rank += __float_as_int(sorted[b + j].w) < key;
+                tmp[b + rank] = v;
+            }
+            __syncthreads();
+            for (int64_t i = threadIdx.x; i < m; i += blockDim.x) sorted[b + i] = tmp[b + i];
+            __syncthreads();
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void launch_bbox_init(BBoxAccum* acc, hipStream_t s) { hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, acc); }
+
+void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_bbox, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, n, tx, ty, tz, acc);
+}
+
+void launch_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz, ReachParams rp, uint32_t* reach0,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_mark_reach, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, rp, reach0);
+}
+
+void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid g0, int range0,
+                       float radius, uint8_t* all0, hipStream_t s) {
+    hipLaunchKernelGGL(k_mark_boxes, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, g0, range0, radius,
+                       all0);
+}
+
+void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, uint8_t* all0, int generic,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_expand_reach, dim3(grid_for(g0.count(), 256)), dim3(256), 0, s, g0, reach0, base0, all0,
+                       generic);
+}
+
+void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
+                     uint8_t* base_p, uint8_t* all_p, hipStream_t s) {
+    hipLaunchKernelGGL(k_level_up, dim3(grid_for(gp.count(), 256)), dim3(256), 0, s, gc, base_c, all_c, gp, logres_p,
+                       base_p, all_p);
+}
+
+void launch_flag_scan_reduce(const uint8_t* base, const uint8_t* all, int64_t n, U2* partials, U2* totals,
+                             hipStream_t s) {
+    int64_t tiles = scan_tiles(n);
+    if (tiles < 1) tiles = 1;
+    hipLaunchKernelGGL(k_flag_reduce, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, base, all, n, partials);
+    hipLaunchKernelGGL(k_scan_partials<U2>, dim3(1), dim3(1024), 0, s, partials, tiles, totals);
+}
+
+void launch_flag_scan_apply(const uint8_t* base, const uint8_t* all, int64_t n, const U2* partials,
+                            const U2* totals, NodeBuildParams np, hipStream_t s) {
+    int64_t tiles = scan_tiles(n);
+    if (tiles < 1) tiles = 1;
+    hipLaunchKernelGGL(k_flag_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, base, all, n, partials, totals,
+                       np);
+}
+
+void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s) {
+    int64_t tiles = scan_tiles(n + 1);
+    hipLaunchKernelGGL(k_count_reduce, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, cnt, n, partials);
+    hipLaunchKernelGGL(k_scan_partials<uint32_t>, dim3(1), dim3(1024), 0, s, partials, tiles, total);
+}
+
+void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
+                             hipStream_t s) {
+    int64_t tiles = scan_tiles(n + 1);
+    hipLaunchKernelGGL(k_count_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, cnt, n, partials, off);
+}
+
+void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill<uint64_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_f32(float* p, int64_t n, float v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+}
+
+void launch_link(LinkParams lp, hipStream_t s) {
+    if (lp.nchild > 0) hipLaunchKernelGGL(k_link, dim3(grid_for(lp.nchild, 256, 1 << 30)), dim3(256), 0, s, lp);
+}
+
+void launch_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int brick_res, int apron, GAtlasNode* amap,
+                         int64_t nmap, hipStream_t s) {
+    if (nmap > 0)
+        hipLaunchKernelGGL(k_atlas_assign, dim3(grid_for(nmap, 256)), dim3(256), 0, s, leaves, nleaf, sx, sy, brick_res,
+                           apron, amap, nmap);
+}
+
+void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6, hipStream_t s) {
+    hipLaunchKernelGGL(k_leaf_bounds, dim3(grid_for(nleaf, 256, 1024)), dim3(256), 0, s, leaves, nleaf, range0, out6);
+}
+
+void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
+                      int32_t* pslot, unsigned long long* len, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, bp, cnt, pslot, len);
+}
+
+void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int range0,
+                        const int32_t* off, const int32_t* pslot, float4* sorted, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, range0, off,
+                       pslot, sorted);
+}
+
+void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s) {
+    hipLaunchKernelGGL(k_leaf_bits, dim3(grid_for((ncell + 31) / 32, 256)), dim3(256), 0, s, idx0, ncell, bits);
+}
+
+void launch_bin_sort(const int32_t* off, int64_t ncell, float4* sorted, float4* tmp, hipStream_t s) {
+    if (ncell > 0)
+        hipLaunchKernelGGL(k_bin_sort, dim3((unsigned)(ncell < 65536 ? ncell : 65536)), dim3(256), 0, s, off, ncell,
+                           sorted, tmp);
+}

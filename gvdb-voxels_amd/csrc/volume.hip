@@ -1,0 +1,926 @@
+// volume.hip -- host runtime and C-ABI of the MI355X point -> sparse-volume
+// library.  One gvdbx_volume is one reference nvdb::VolumeGVDB restricted to
+// the points path; each exported function names the method it replaces in
+// include/gvdbx/gvdbx_c.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "gvdbx/gvdbx_c.h"
+#include "gvdbx_kernels.h"
+
+static thread_local char g_err[1024] = "";
+
+void gvdbx_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+#define FAIL(code, ...)               \
+    do {                              \
+        gvdbx_set_error(__VA_ARGS__); \
+        return (code);                \
+    } while (0)
+
+#define TRY(expr)                     \
+    do {                              \
+        int _s = (expr);              \
+        if (_s != GVDBX_OK) return _s; \
+    } while (0)
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+constexpr int64_t MAX_DENSE_CELLS = int64_t(1) << 28;
+
+}  // namespace
+
+struct gvdbx_volume {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // configuration (Configure VG.cpp:2380-2434)
+    int levs = 0;
+    int logdim[GVDBX_MAXLEV] = {};
+    int res[GVDBX_MAXLEV] = {};
+    int64_t range[GVDBX_MAXLEV] = {};
+    bool configured = false;
+    // atlas (AtlasCreate AL.cpp:528-560)
+    int def_axis[3] = {16, 16, 1};
+    int subdim[3] = {0, 0, 0};
+    int apron = -1;
+    bool has_chan[GVDBX_MAXCHAN] = {};
+    int chan_filter[GVDBX_MAXCHAN] = {};
+    int chan_border[GVDBX_MAXCHAN] = {};
+    DevBuf chan[GVDBX_MAXCHAN];
+    int64_t atlas_cap = 0;   // bricks per channel buffer
+    // points (SetPoints VG.cpp:4900-4908)
+    bool has_pos = false;
+    PointSrc pos{nullptr, 12};
+    const char* wts = nullptr;
+    int64_t wstride = 4;
+    // topology
+    bool has_topo = false;
+    int depth = 0, top_lev = 1;
+    int root_pos[3] = {0, 0, 0};
+    float pos_min[3] = {}, pos_max[3] = {}, obj_min[3] = {}, obj_max[3] = {};
+    int64_t cnt[GVDBX_MAXLEV] = {};
+    int64_t nbase[GVDBX_MAXLEV] = {};
+    CellGrid grid[GVDBX_MAXLEV] = {};
+    DevBuf nodes[GVDBX_MAXLEV + 1], clist[GVDBX_MAXLEV + 1], idx[GVDBX_MAXLEV], flag_base[GVDBX_MAXLEV],
+        flag_all[GVDBX_MAXLEV];
+    DevBuf reach0, leafbits, partials, totals, bbox, misc;
+    int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
+    // atlas map
+    bool has_atlas = false;
+    DevBuf amap;
+    int64_t amap_cnt = 0;
+    // binning
+    DevBuf bin_cnt, bin_off, pslot, sorted, sorted_tmp;
+    CellGrid gbin = {};
+    bool bins_valid = false;
+    bool bins_stable = false;
+    int64_t bins_n = -1;
+    float bins_trans[3] = {};
+    const char* bins_base = nullptr;
+    int64_t bins_stride = 0;
+    // subcell state (InsertPointsSubcell)
+    int sc_size = 0;
+    float sc_radius = 0.0f;
+    int64_t sc_len = 0;
+    bool sc_valid = false;
+    // InsertPoints output
+    DevBuf pnode, poff;
+    int64_t npnode = 0;
+    // pinned host staging
+    void* host = nullptr;
+    size_t host_cap = 0;
+};
+
+namespace {
+
+int set_dev(gvdbx_volume* v) {
+    hipError_t e = hipSetDevice(v->device);
+    if (e != hipSuccess) FAIL(GVDBX_ERR_HIP, "hipSetDevice(%d): %s", v->device, hipGetErrorString(e));
+    return GVDBX_OK;
+}
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) FAIL(GVDBX_ERR_HIP, "%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return GVDBX_OK;
+}
+
+int sync(gvdbx_volume* v) {
+    hipError_t e = hipStreamSynchronize(v->stream);
+    if (e != hipSuccess) FAIL(GVDBX_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    return GVDBX_OK;
+}
+
+int ensure(gvdbx_volume* v, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return GVDBX_OK;
+    TRY(sync(v));
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4 + 256;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        FAIL(GVDBX_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    return GVDBX_OK;
+}
+
+int ensure_host(gvdbx_volume* v, size_t bytes) {
+    if (v->host_cap >= bytes) return GVDBX_OK;
+    TRY(sync(v));
+    if (v->host) (void)hipHostFree(v->host);
+    v->host = nullptr;
+    v->host_cap = 0;
+    size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&v->host, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        v->host = nullptr;
+        FAIL(GVDBX_ERR_NOMEM, "hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    v->host_cap = want;
+    return GVDBX_OK;
+}
+
+void free_buf(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int brick_res(const gvdbx_volume* v) { return v->res[0] + 2 * std::max(v->apron, 0); }
+int64_t brick_vox(const gvdbx_volume* v) {
+    int64_t br = brick_res(v);
+    return br * br * br;
+}
+
+bool any_chan(const gvdbx_volume* v) {
+    for (int c = 0; c < GVDBX_MAXCHAN; c++)
+        if (v->has_chan[c]) return true;
+    return false;
+}
+
+// Grow every channel to hold `bricks` bricks, preserving contents (AtlasResize AL.cpp:622-650).
+int grow_atlas(gvdbx_volume* v, int64_t bricks) {
+    if (bricks <= v->atlas_cap) return GVDBX_OK;
+    int64_t bv = brick_vox(v);
+    TRY(sync(v));
+    for (int c = 0; c < GVDBX_MAXCHAN; c++) {
+        if (!v->has_chan[c]) continue;
+        void* np = nullptr;
+        size_t bytes = (size_t)(bricks * bv) * sizeof(float);
+        hipError_t e = hipMalloc(&np, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            FAIL(GVDBX_ERR_NOMEM, "atlas growth to %lld bricks failed: %s", (long long)bricks, hipGetErrorString(e));
+        }
+        launch_fill_f32((float*)np, bricks * bv, 0.0f, v->stream);
+        if (v->chan[c].p && v->atlas_cap > 0)
+            GVDBX_CHECK(hipMemcpyAsync(np, v->chan[c].p, (size_t)(v->atlas_cap * bv) * sizeof(float),
+                                       hipMemcpyDeviceToDevice, v->stream));
+        TRY(sync(v));
+        free_buf(v->chan[c]);
+        v->chan[c].p = np;
+        v->chan[c].cap = bytes;
+    }
+    v->atlas_cap = bricks;
+    return GVDBX_OK;
+}
+
+int check_vol(gvdbx_volume* v) {
+    if (!v) FAIL(GVDBX_ERR_ARG, "null volume handle");
+    return set_dev(v);
+}
+
+int check_chan(gvdbx_volume* v, int chan) {
+    if (chan < 0 || chan >= GVDBX_MAXCHAN || !v->has_chan[chan]) FAIL(GVDBX_ERR_ARG, "channel %d does not exist", chan);
+    return GVDBX_OK;
+}
+
+// Host restatement of DetermineDepth (VG.cpp:821-837) over GetCoveringNode (VG.cpp:2915-2931).
+int determine_depth(const gvdbx_volume* v, const float pmin[3], const float pmax[3], int root[3]) {
+    int ipmin[3] = {(int)pmin[0], (int)pmin[1], (int)pmin[2]};
+    int ipmax[3] = {(int)pmax[0], (int)pmax[1], (int)pmax[2]};
+    for (int lev = 0; lev < v->levs; lev++) {
+        int64_t rg = v->range[lev];
+        int64_t a0[3], a1[3];
+        for (int a = 0; a < 3; a++) {
+            a0[a] = ((int64_t)ipmin[a] / rg) * rg;
+            if (ipmin[a] < a0[a]) a0[a] -= rg;
+            a1[a] = ((int64_t)ipmax[a] / rg) * rg;
+            if (ipmax[a] < a1[a]) a1[a] -= rg;
+        }
+        if (a0[0] == a1[0] && a0[1] == a1[1] && a0[2] == a1[2]) {
+            for (int a = 0; a < 3; a++) root[a] = (int)a0[a];
+            return lev;
+        }
+    }
+    return -1;
+}
+
+// Bin the current points by home brick cell (one cell margin around the
+// leaf grid).  stable=true additionally orders each bin by point index.
+int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool count_len, int sc, float radius,
+               int64_t* len_out) {
+    if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
+    bool same = v->bins_valid && v->bins_n == n && v->bins_base == v->pos.base && v->bins_stride == v->pos.stride &&
+                v->bins_trans[0] == t[0] && v->bins_trans[1] == t[1] && v->bins_trans[2] == t[2];
+    const CellGrid& g0 = v->grid[0];
+    CellGrid gb = {g0.ox - 1, g0.oy - 1, g0.oz - 1, g0.nx + 2, g0.ny + 2, g0.nz + 2};
+    int64_t nb = gb.count();
+    hipStream_t s = v->stream;
+    if (!same || count_len) {
+        TRY(ensure(v, v->bin_cnt, (size_t)(nb + 1) * 4));
+        TRY(ensure(v, v->bin_off, (size_t)(nb + 1) * 4));
+        TRY(ensure(v, v->pslot, (size_t)std::max<int64_t>(n, 1) * 4));
+        TRY(ensure(v, v->sorted, (size_t)std::max<int64_t>(n, 1) * 16));
+        TRY(ensure(v, v->partials, (size_t)(scan_tiles(nb + 1) + 2) * 8));
+        TRY(ensure(v, v->misc, 256));
+        BinParams bp{};
+        bp.gb = gb;
+        bp.range0 = v->res[0];
+        bp.count_len = count_len ? 1 : 0;
+        bp.sc = sc;
+        bp.radius = radius;
+        bp.g0 = g0;
+        if (count_len) {
+            TRY(ensure(v, v->leafbits, (size_t)((g0.count() + 31) / 32) * 4));
+            launch_leaf_bits(v->idx[0].as<int32_t>(), g0.count(), v->leafbits.as<uint32_t>(), s);
+            bp.leafbits = v->leafbits.as<uint32_t>();
+        }
+        unsigned long long* len = (unsigned long long*)((char*)v->misc.p + 64);
+        launch_fill_i32(v->bin_cnt.as<int32_t>(), nb + 1, 0, s);
+        launch_fill_u64((uint64_t*)len, 1, 0ull, s);
+        launch_bin_count(v->pos, n, t[0], t[1], t[2], bp, v->bin_cnt.as<int32_t>(), v->pslot.as<int32_t>(), len, s);
+        launch_count_scan_reduce(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(),
+                                 (uint32_t*)((char*)v->misc.p + 96), s);
+        launch_count_scan_apply(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(), v->bin_off.as<int32_t>(), s);
+        launch_bin_scatter(v->pos, n, t[0], t[1], t[2], gb, v->res[0], v->bin_off.as<int32_t>(),
+                           v->pslot.as<int32_t>(), v->sorted.as<float4>(), s);
+        TRY(check_launch("bin points"));
+        v->gbin = gb;
+        v->bins_valid = true;
+        v->bins_stable = false;
+        v->bins_n = n;
+        v->bins_base = v->pos.base;
+        v->bins_stride = v->pos.stride;
+        memcpy(v->bins_trans, t, sizeof(v->bins_trans));
+        if (count_len) {
+            TRY(ensure_host(v, 64));
+            GVDBX_CHECK(hipMemcpyAsync(v->host, len, 8, hipMemcpyDeviceToHost, s));
+            TRY(sync(v));
+            if (len_out) *len_out = (int64_t)(*(unsigned long long*)v->host);
+        }
+    }
+    if (stable && !v->bins_stable) {
+        TRY(ensure(v, v->sorted_tmp, (size_t)std::max<int64_t>(n, 1) * 16));
+        launch_bin_sort(v->bin_off.as<int32_t>(), nb, v->sorted.as<float4>(), v->sorted_tmp.as<float4>(), s);
+        TRY(check_launch("bin sort"));
+        v->bins_stable = true;
+    }
+    return GVDBX_OK;
+}
+
+int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[3]) {
+    if (!v->configured) FAIL(GVDBX_ERR_STATE, "Configure has not been called");
+    if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
+    if (n < 0) FAIL(GVDBX_ERR_ARG, "negative point count");
+    if (!(radius >= 0.0f) || !isfinite(radius)) FAIL(GVDBX_ERR_ARG, "radius must be finite and >= 0");
+    hipStream_t s = v->stream;
+    TRY(ensure(v, v->bbox, sizeof(BBoxAccum)));
+    TRY(ensure_host(v, 4096));
+    launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
+    if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), s);
+    TRY(check_launch("bounding box"));
+    GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    BBoxAccum bb;
+    memcpy(&bb, v->host, sizeof(bb));
+    if (bb.nvalid == 0) FAIL(GVDBX_ERR_NOPOINTS, "RebuildTopology: no valid point");
+    // GetBoundingBox VG.cpp:5452-5463
+    float mn[3], mx[3], pmin[3], pmax[3];
+    for (int a = 0; a < 3; a++) {
+        memcpy(&mn[a], &bb.mn[a], 4);
+        memcpy(&mx[a], &bb.mx[a], 4);
+        pmin[a] = mn[a] - 1;
+        pmax[a] = mx[a] + 2;
+        pmin[a] = 0 > pmin[a] ? 0 : pmin[a];
+        pmax[a] = 0 > pmax[a] ? 0 : pmax[a];
+    }
+    int root[3];
+    int depth = determine_depth(v, pmin, pmax, root);
+    if (depth == 0) FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: point extents smaller than one brick");
+    if (depth < 0 || depth >= v->levs) FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: point extents exceed the configured levels");
+    for (int a = 0; a < 3; a++)
+        if ((int64_t)pmax[a] / v->range[0] > 65535) FAIL(GVDBX_ERR_RANGE, "RebuildTopology: extents exceed 16-bit brick keys");
+
+    const bool extra = radius >= 1.0f;   // VG.cpp:1207-1208
+    const int r0 = v->res[0];
+    const bool mask_ok = 2.0f * radius <= (float)r0 && radius + 1.0f <= (float)r0;
+    int64_t rtop = v->range[depth];
+    int lo_v[3], hi_v[3];
+    for (int a = 0; a < 3; a++) {
+        int fmn = (int)mn[a], fmx = (int)mx[a];
+        if (extra) {
+            lo_v[a] = std::max<int64_t>(root[a], (int64_t)fmn - (int64_t)ceilf(radius) - 1);
+            hi_v[a] = (int)std::min<int64_t>(root[a] + rtop - 1, (int64_t)fmx + (int64_t)ceilf(2.0f * radius) + 1);
+        } else {
+            lo_v[a] = std::max(root[a], fmn);
+            hi_v[a] = (int)std::min<int64_t>(root[a] + rtop - 1, fmx);
+        }
+    }
+    CellGrid grid[GVDBX_MAXLEV] = {};
+    for (int l = 0; l < depth; l++) {
+        int64_t rg = v->range[l];
+        CellGrid& g = grid[l];
+        g.ox = (int)(lo_v[0] / rg); g.oy = (int)(lo_v[1] / rg); g.oz = (int)(lo_v[2] / rg);
+        g.nx = (int)(hi_v[0] / rg) - g.ox + 1;
+        g.ny = (int)(hi_v[1] / rg) - g.oy + 1;
+        g.nz = (int)(hi_v[2] / rg) - g.oz + 1;
+    }
+    if (grid[0].count() > MAX_DENSE_CELLS)
+        FAIL(GVDBX_ERR_RANGE, "RebuildTopology: %lld level-0 cells exceed the dense-grid limit", (long long)grid[0].count());
+
+    // ---- occupancy ----
+    int64_t G0 = grid[0].count();
+    TRY(ensure(v, v->reach0, (size_t)G0 * 4));
+    for (int l = 0; l < depth; l++) {
+        int64_t G = grid[l].count();
+        TRY(ensure(v, v->flag_base[l], (size_t)G));
+        TRY(ensure(v, v->flag_all[l], (size_t)G));
+        TRY(ensure(v, v->idx[l], (size_t)G * 4));
+    }
+    int64_t ptot = 0;
+    int64_t poffs[GVDBX_MAXLEV];
+    for (int l = 0; l < depth; l++) {
+        poffs[l] = ptot;
+        ptot += scan_tiles(grid[l].count()) + 1;
+    }
+    TRY(ensure(v, v->partials, (size_t)(ptot + 2) * sizeof(U2)));
+    TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
+    launch_fill_i32((int32_t*)v->reach0.p, G0, 0, s);
+    launch_fill_u8(v->flag_all[0].as<uint8_t>(), G0, 0, s);
+    ReachParams rp{grid[0], r0, (extra && mask_ok) ? 1 : 0, radius};
+    launch_mark_reach(v->pos, n, orig[0], orig[1], orig[2], rp, v->reach0.as<uint32_t>(), s);
+    if (extra && !mask_ok)
+        launch_mark_boxes(v->pos, n, orig[0], orig[1], orig[2], grid[0], r0, radius, v->flag_all[0].as<uint8_t>(), s);
+    launch_expand_reach(grid[0], v->reach0.as<uint32_t>(), v->flag_base[0].as<uint8_t>(), v->flag_all[0].as<uint8_t>(),
+                        (extra && !mask_ok) ? 1 : 0, s);
+    for (int l = 1; l < depth; l++)
+        launch_level_up(grid[l - 1], v->flag_base[l - 1].as<uint8_t>(), v->flag_all[l - 1].as<uint8_t>(), grid[l],
+                        v->logdim[l], v->flag_base[l].as<uint8_t>(), v->flag_all[l].as<uint8_t>(), s);
+    U2* partials = v->partials.as<U2>();
+    U2* totals = v->totals.as<U2>();
+    for (int l = 0; l < depth; l++)
+        launch_flag_scan_reduce(v->flag_base[l].as<uint8_t>(), v->flag_all[l].as<uint8_t>(), grid[l].count(),
+                                partials + poffs[l], totals + l, s);
+    TRY(check_launch("occupancy"));
+    GVDBX_CHECK(hipMemcpyAsync(v->host, totals, (size_t)depth * sizeof(U2), hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    U2 tot[GVDBX_MAXLEV];
+    memcpy(tot, v->host, (size_t)depth * sizeof(U2));
+
+    // ---- pools (Clear + PoolAlloc, VG.cpp:2487-2503, AL.cpp:163-192) ----
+    int64_t cnt[GVDBX_MAXLEV] = {};
+    for (int l = 0; l < depth; l++) cnt[l] = (int64_t)tot[l].a + tot[l].b;
+    cnt[depth] = 1;
+    for (int l = 0; l <= depth; l++) {
+        TRY(ensure(v, v->nodes[l], (size_t)cnt[l] * sizeof(GNode)));
+        if (l >= 1) {
+            int64_t len = (int64_t)v->res[l] * v->res[l] * v->res[l];
+            TRY(ensure(v, v->clist[l], (size_t)(cnt[l] * len) * 8));
+        }
+    }
+    for (int l = 0; l < depth; l++) {
+        NodeBuildParams np{grid[l], l, v->range[l], l == 0 ? 1 : 0, v->nodes[l].as<GNode>(), v->idx[l].as<int32_t>()};
+        launch_flag_scan_apply(v->flag_base[l].as<uint8_t>(), v->flag_all[l].as<uint8_t>(), grid[l].count(),
+                               partials + poffs[l], totals + l, np, s);
+    }
+    // root node (ActivateBricksGPU VG.cpp:1290-1301)
+    GNode rn;
+    memset(&rn, 0, sizeof(rn));
+    rn.mLev = (uint8_t)depth;
+    rn.mFlags = 1;
+    for (int a = 0; a < 3; a++) {
+        rn.mPos[a] = root[a];
+        rn.mValue[a] = -1;
+    }
+    rn.mParent = GVDBX_ID_UNDEFL;
+    rn.mChildList = 1ull | ((uint64_t)depth << 8);
+    memcpy((char*)v->host + 1024, &rn, sizeof(rn));
+    GVDBX_CHECK(hipMemcpyAsync(v->nodes[depth].p, (char*)v->host + 1024, sizeof(rn), hipMemcpyHostToDevice, s));
+    // top level (PrepareVDBPartially VG.cpp:3996-4012)
+    int tlev = 1;
+    for (int l = v->levs - 1; l >= 0; l--)
+        if (cnt[l] == 1) tlev = l;
+    for (int l = 1; l <= depth; l++) {
+        int64_t len = (int64_t)v->res[l] * v->res[l] * v->res[l];
+        launch_fill_u64(v->clist[l].as<uint64_t>(), cnt[l] * len, GVDBX_ID_UNDEF64, s);
+    }
+    // links (gvdbLinkBricks, levels below top_lev, top-down)
+    for (int l = depth - 1; l >= 0; l--) {
+        if (l >= tlev) continue;
+        LinkParams lp{};
+        lp.lev = l;
+        lp.nchild = cnt[l];
+        lp.child = v->nodes[l].as<GNode>();
+        lp.childw = v->nodes[l].as<GNode>();
+        lp.parent_nodes = v->nodes[l + 1].as<GNode>();
+        lp.parent_clist = v->clist[l + 1].as<uint64_t>();
+        lp.res_p = v->res[l + 1];
+        lp.range_p = v->range[l + 1];
+        lp.logres_p = v->logdim[l + 1];
+        lp.parent_is_root = (l + 1 == depth) ? 1 : 0;
+        if (l + 1 < depth) {
+            lp.gp = grid[l + 1];
+            lp.idx_p = v->idx[l + 1].as<int32_t>();
+        }
+        lp.nparent = cnt[l + 1];
+        launch_link(lp, s);
+    }
+    TRY(check_launch("node pools"));
+    TRY(sync(v));
+
+    v->depth = depth;
+    v->top_lev = tlev;
+    memcpy(v->root_pos, root, sizeof(root));
+    memcpy(v->pos_min, pmin, sizeof(pmin));
+    memcpy(v->pos_max, pmax, sizeof(pmax));
+    for (int l = 0; l < GVDBX_MAXLEV; l++) {
+        v->cnt[l] = l <= depth ? cnt[l] : 0;
+        v->nbase[l] = l < depth ? (int64_t)tot[l].a : (l == depth ? 1 : 0);
+        v->grid[l] = l < depth ? grid[l] : CellGrid{};
+    }
+    v->has_topo = true;
+    v->has_atlas = false;
+    v->bins_valid = false;
+    v->sc_valid = false;
+    v->topo_mode_used = GVDBX_TOPO_DENSE;
+    return GVDBX_OK;
+}
+
+int compute_bounds(gvdbx_volume* v) {
+    // ComputeBounds VG.cpp:1792-1816
+    hipStream_t s = v->stream;
+    TRY(ensure(v, v->misc, 256));
+    TRY(ensure_host(v, 4096));
+    int32_t init[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, INT32_MIN, INT32_MIN, INT32_MIN};
+    memcpy((char*)v->host + 2048, init, sizeof(init));
+    GVDBX_CHECK(hipMemcpyAsync(v->misc.p, (char*)v->host + 2048, sizeof(init), hipMemcpyHostToDevice, s));
+    launch_leaf_bounds(v->nodes[0].as<GNode>(), v->cnt[0], v->res[0], v->misc.as<int32_t>(), s);
+    TRY(check_launch("leaf bounds"));
+    GVDBX_CHECK(hipMemcpyAsync((char*)v->host + 2048, v->misc.p, sizeof(init), hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    int32_t out[6];
+    memcpy(out, (char*)v->host + 2048, sizeof(out));
+    for (int a = 0; a < 3; a++) {
+        v->obj_min[a] = (float)out[a];
+        v->obj_max[a] = (float)out[3 + a];
+    }
+    return GVDBX_OK;
+}
+
+int update_atlas(gvdbx_volume* v) {
+    if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "UpdateAtlas: no topology");
+    if (!any_chan(v)) FAIL(GVDBX_ERR_STATE, "UpdateAtlas: no channel (AddChannel)");
+    int64_t used = v->cnt[0];
+    int64_t amax = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+    if (used > amax) {   // VG.cpp:2645-2651 + AtlasResize AL.cpp:622-650
+        int sz = (int)ceil(used / (float)(v->subdim[0] * v->subdim[1]));
+        if (sz > v->subdim[2]) v->subdim[2] = sz;
+    }
+    int64_t slots = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+    TRY(grow_atlas(v, slots));
+    TRY(ensure(v, v->amap, (size_t)slots * sizeof(GAtlasNode)));
+    launch_atlas_assign(v->nodes[0].as<GNode>(), v->cnt[0], v->subdim[0], v->subdim[1], brick_res(v),
+                        std::max(v->apron, 0), v->amap.as<GAtlasNode>(), slots, v->stream);
+    TRY(check_launch("atlas assign"));
+    v->amap_cnt = slots;
+    v->has_atlas = true;
+    return GVDBX_OK;
+}
+
+int need_atlas(gvdbx_volume* v, const char* what) {
+    if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "%s: no topology", what);
+    if (!v->has_atlas) FAIL(GVDBX_ERR_STATE, "%s: UpdateAtlas has not been called", what);
+    if (v->cnt[0] == 0) FAIL(GVDBX_ERR_STATE, "%s: no bricks", what);
+    return GVDBX_OK;
+}
+
+int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int chan, int accum) {
+    const char* what = mode ? "GatherLevelSet" : "GatherDensity";
+    TRY(need_atlas(v, what));
+    TRY(check_chan(v, chan));
+    if (!v->sc_valid || !v->bins_valid) FAIL(GVDBX_ERR_STATE, "%s: InsertPointsSubcell has not been called", what);
+    if (sc != v->sc_size) FAIL(GVDBX_ERR_ARG, "%s: subcell size %d differs from InsertPointsSubcell (%d)", what, sc, v->sc_size);
+    if (!(radius >= 0.0f) || !isfinite(radius)) FAIL(GVDBX_ERR_ARG, "%s: bad radius", what);
+    if (mode == 0 && sc_len == 0) return GVDBX_OK;   // VG.cpp:5321
+    GatherParams gp{};
+    gp.leaves = v->nodes[0].as<GNode>();
+    gp.nleaf = v->cnt[0];
+    gp.gb = v->gbin;
+    gp.off = v->bin_off.as<int32_t>();
+    gp.sorted = v->sorted.as<float4>();
+    gp.chan = v->chan[chan].as<float>();
+    gp.brick_res = brick_res(v);
+    gp.apron = std::max(v->apron, 0);
+    gp.mode = mode;
+    gp.accum = accum ? 1 : 0;
+    gp.sc = sc;
+    gp.box_radius = v->sc_radius;
+    gp.radius = radius;
+    gp.cutoff = accum ? 1.0e7f : (mode == 1 ? radius + 4.0f : radius + 1.0f);
+    launch_gather(gp, v->stream);
+    return check_launch(what);
+}
+
+}  // namespace
+
+// =====================================================================
+// C-ABI
+// =====================================================================
+extern "C" {
+
+const char* gvdbx_last_error(void) { return g_err; }
+const char* gvdbx_version(void) { return "gvdbx 0.1 (gfx950)"; }
+
+int gvdbx_create(gvdbx_volume** out, int device) {
+    if (!out) FAIL(GVDBX_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) FAIL(GVDBX_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) FAIL(GVDBX_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    gvdbx_volume* v = new gvdbx_volume();
+    v->device = device;
+    int st = set_dev(v);
+    if (st != GVDBX_OK) {
+        delete v;
+        return st;
+    }
+    *out = v;
+    return GVDBX_OK;
+}
+
+int gvdbx_destroy(gvdbx_volume* v) {
+    if (!v) FAIL(GVDBX_ERR_ARG, "null volume handle");
+    (void)hipSetDevice(v->device);
+    (void)hipStreamSynchronize(v->stream);
+    for (int c = 0; c < GVDBX_MAXCHAN; c++) free_buf(v->chan[c]);
+    for (int l = 0; l <= GVDBX_MAXLEV; l++) {
+        free_buf(v->nodes[l]);
+        free_buf(v->clist[l]);
+    }
+    for (int l = 0; l < GVDBX_MAXLEV; l++) {
+        free_buf(v->idx[l]);
+        free_buf(v->flag_base[l]);
+        free_buf(v->flag_all[l]);
+    }
+    DevBuf* bufs[] = {&v->reach0, &v->leafbits, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
+                      &v->bin_cnt, &v->bin_off, &v->pslot, &v->sorted, &v->sorted_tmp, &v->pnode, &v->poff};
+    for (DevBuf* b : bufs) free_buf(*b);
+    if (v->host) (void)hipHostFree(v->host);
+    delete v;
+    return GVDBX_OK;
+}
+
+int gvdbx_set_stream(gvdbx_volume* v, void* stream) {
+    TRY(check_vol(v));
+    v->stream = (hipStream_t)stream;
+    return GVDBX_OK;
+}
+
+int gvdbx_synchronize(gvdbx_volume* v) {
+    TRY(check_vol(v));
+    return sync(v);
+}
+
+int gvdbx_configure(gvdbx_volume* v, int levs, const int* log2dims) {
+    TRY(check_vol(v));
+    if (levs < 2 || levs > GVDBX_MAXLEV || !log2dims) FAIL(GVDBX_ERR_ARG, "Configure: levels must be in [2,%d]", GVDBX_MAXLEV);
+    int ld[GVDBX_MAXLEV];
+    int64_t rg = 1;
+    for (int l = 0; l < levs; l++) {
+        ld[l] = log2dims[l] == 0 ? 1 : log2dims[l];   // VG.cpp:2394
+        if (ld[l] < 1 || ld[l] > 10) FAIL(GVDBX_ERR_ARG, "Configure: log2 dim %d at level %d out of range", ld[l], l);
+        rg *= (int64_t)1 << ld[l];
+        if (rg > (int64_t)1 << 30) FAIL(GVDBX_ERR_ARG, "Configure: tree range exceeds 2^30 voxels");
+    }
+    if ((1 << ld[0]) > 32) FAIL(GVDBX_ERR_ARG, "Configure: bricks larger than 32^3 are not supported");
+    v->levs = levs;
+    for (int l = 0; l < GVDBX_MAXLEV; l++) {
+        v->logdim[l] = l < levs ? ld[l] : 0;
+        v->res[l] = l < levs ? 1 << ld[l] : 0;
+    }
+    for (int l = 0; l < levs; l++) v->range[l] = l == 0 ? v->res[0] : v->range[l - 1] * v->res[l];
+    v->configured = true;
+    v->has_topo = false;
+    v->has_atlas = false;
+    v->bins_valid = false;
+    v->sc_valid = false;
+    return GVDBX_OK;
+}
+
+int gvdbx_set_channel_default(gvdbx_volume* v, int sx, int sy, int sz) {
+    TRY(check_vol(v));
+    if (sx < 1 || sy < 1 || sz < 1) FAIL(GVDBX_ERR_ARG, "SetChannelDefault: counts must be >= 1");
+    v->def_axis[0] = sx;
+    v->def_axis[1] = sy;
+    v->def_axis[2] = sz;
+    return GVDBX_OK;
+}
+
+int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filter, int border) {
+    TRY(check_vol(v));
+    if (!v->configured) FAIL(GVDBX_ERR_STATE, "AddChannel before Configure");
+    if (chan < 0 || chan >= GVDBX_MAXCHAN) FAIL(GVDBX_ERR_ARG, "AddChannel: channel %d out of range", chan);
+    if (dtype != GVDBX_FLOAT) FAIL(GVDBX_ERR_ARG, "AddChannel: only float channels are supported");
+    if (apron < 0 || apron > 1) FAIL(GVDBX_ERR_ARG, "AddChannel: apron must be 0 or 1");
+    if (!any_chan(v)) {
+        for (int a = 0; a < 3; a++) v->subdim[a] = v->def_axis[a];
+        v->apron = apron;
+        v->atlas_cap = 0;
+    } else if (apron != v->apron) {
+        FAIL(GVDBX_ERR_ARG, "AddChannel: all channels share one apron");
+    }
+    int64_t slots = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+    int64_t cap = std::max(slots, v->atlas_cap);
+    size_t bytes = (size_t)(cap * brick_vox(v)) * sizeof(float);
+    free_buf(v->chan[chan]);
+    hipError_t e = hipMalloc(&v->chan[chan].p, bytes);
+    if (e != hipSuccess) {
+        v->chan[chan].p = nullptr;
+        (void)hipGetLastError();
+        FAIL(GVDBX_ERR_NOMEM, "AddChannel: hipMalloc(%zu) failed", bytes);
+    }
+    v->chan[chan].cap = bytes;
+    launch_fill_f32(v->chan[chan].as<float>(), cap * brick_vox(v), 0.0f, v->stream);
+    v->has_chan[chan] = true;
+    v->chan_filter[chan] = filter;
+    v->chan_border[chan] = border;
+    v->atlas_cap = cap;
+    return check_launch("AddChannel");
+}
+
+int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value) {
+    TRY(check_vol(v));
+    TRY(check_chan(v, chan));
+    launch_fill_f32(v->chan[chan].as<float>(), v->atlas_cap * brick_vox(v), value, v->stream);
+    return check_launch("FillChannel");
+}
+
+int gvdbx_set_topology_mode(gvdbx_volume* v, int mode) {
+    TRY(check_vol(v));
+    if (mode < GVDBX_TOPO_AUTO || mode > GVDBX_TOPO_RADIX) FAIL(GVDBX_ERR_ARG, "unknown topology mode %d", mode);
+    if (mode == GVDBX_TOPO_RADIX) FAIL(GVDBX_ERR_ARG, "radix topology mode is not available in this build");
+    v->topo_mode = mode;
+    return GVDBX_OK;
+}
+
+int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_stride, const void* vel_dev,
+                     int vel_off, int vel_stride, const void* clr_dev, int clr_off, int clr_stride) {
+    (void)vel_dev; (void)vel_off; (void)vel_stride; (void)clr_dev; (void)clr_off; (void)clr_stride;
+    TRY(check_vol(v));
+    if (!pos_dev) FAIL(GVDBX_ERR_ARG, "SetPoints: null position pointer");
+    if (pos_stride < 12 || pos_off < 0) FAIL(GVDBX_ERR_ARG, "SetPoints: position stride must be >= 12");
+    if (((uintptr_t)pos_dev + pos_off) % 4 || pos_stride % 4) FAIL(GVDBX_ERR_ARG, "SetPoints: positions must be 4-byte aligned");
+    v->pos.base = (const char*)pos_dev + pos_off;
+    v->pos.stride = pos_stride;
+    v->has_pos = true;
+    v->bins_valid = false;
+    return GVDBX_OK;
+}
+
+int gvdbx_set_point_weights(gvdbx_volume* v, const float* w_dev, int stride) {
+    TRY(check_vol(v));
+    if (w_dev && (stride < 4 || stride % 4)) FAIL(GVDBX_ERR_ARG, "SetPointWeights: bad stride");
+    v->wts = (const char*)w_dev;
+    v->wstride = stride;
+    return GVDBX_OK;
+}
+
+int gvdbx_rebuild_topology(gvdbx_volume* v, int num_pnts, float radius, float ox, float oy, float oz) {
+    TRY(check_vol(v));
+    float o[3] = {ox, oy, oz};
+    return rebuild_topology(v, num_pnts, radius, o);
+}
+
+int gvdbx_finish_topology(gvdbx_volume* v, int commit_pool, int compute_bnds) {
+    (void)commit_pool;   // pools live on the device only
+    TRY(check_vol(v));
+    if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "FinishTopology: no topology");
+    if (compute_bnds) TRY(compute_bounds(v));
+    return GVDBX_OK;
+}
+
+int gvdbx_update_atlas(gvdbx_volume* v) {
+    TRY(check_vol(v));
+    return update_atlas(v);
+}
+
+int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                                int64_t* out_len) {
+    TRY(check_vol(v));
+    TRY(need_atlas(v, "InsertPointsSubcell"));
+    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "InsertPointsSubcell: negative point count");
+    if (sc <= 0 || v->res[0] % sc) FAIL(GVDBX_ERR_ARG, "InsertPointsSubcell: subcell size must divide the brick");
+    if (!(radius >= 0.0f) || radius + (float)sc > (float)v->res[0])
+        FAIL(GVDBX_ERR_RANGE, "InsertPointsSubcell: radius %g + subcell %d exceeds the brick (%d)", radius, sc, v->res[0]);
+    float t[3] = {tx, ty, tz};
+    int64_t len = 0;
+    v->bins_valid = false;
+    TRY(bin_points(v, num_pnts, t, false, true, sc, radius, &len));
+    v->sc_size = sc;
+    v->sc_radius = radius;
+    v->sc_len = len;
+    v->sc_valid = true;
+    if (out_len) *out_len = len;
+    return GVDBX_OK;
+}
+
+int gvdbx_gather_density(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                         int64_t sc_len, int chan_density, int chan_clr, int accum) {
+    (void)num_pnts; (void)tx; (void)ty; (void)tz; (void)chan_clr;
+    TRY(check_vol(v));
+    return gather(v, 0, sc, radius, sc_len, chan_density, accum);
+}
+
+int gvdbx_gather_level_set(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                           int64_t sc_len, int chan_levelset, int chan_clr, int accum) {
+    (void)num_pnts; (void)tx; (void)ty; (void)tz; (void)chan_clr;
+    TRY(check_vol(v));
+    return gather(v, 1, sc, radius, sc_len, chan_levelset, accum);
+}
+
+int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix) {
+    (void)prefix;
+    TRY(check_vol(v));
+    TRY(need_atlas(v, "InsertPoints"));
+    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "InsertPoints: negative point count");
+    float t[3] = {tx, ty, tz};
+    TRY(bin_points(v, num_pnts, t, true, false, 0, 0.0f, nullptr));
+    TRY(ensure(v, v->pnode, (size_t)std::max(num_pnts, 1) * 4));
+    TRY(ensure(v, v->poff, (size_t)std::max(num_pnts, 1) * 4));
+    launch_fill_i32(v->pnode.as<int32_t>(), num_pnts, (int32_t)GVDBX_ID_UNDEFL, v->stream);
+    launch_fill_i32(v->poff.as<int32_t>(), num_pnts, 0, v->stream);
+    InsertParams ip{v->gbin, v->grid[0], v->idx[0].as<int32_t>(), v->bin_off.as<int32_t>(), v->sorted.as<float4>(),
+                    v->res[0], v->pnode.as<int32_t>(), v->poff.as<int32_t>(), num_pnts};
+    launch_insert_points(ip, v->stream);
+    v->npnode = num_pnts;
+    return check_launch("InsertPoints");
+}
+
+int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp, float tx, float ty, float tz,
+                          int expand, int avg_color) {
+    (void)amp; (void)avg_color;   // amp is unused by the reference kernel
+    TRY(check_vol(v));
+    TRY(need_atlas(v, "ScatterDensity"));
+    TRY(check_chan(v, 0));
+    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "ScatterDensity: negative point count");
+    float t[3] = {tx, ty, tz};
+    TRY(bin_points(v, num_pnts, t, true, false, 0, 0.0f, nullptr));
+    ScatterParams sp{v->nodes[0].as<GNode>(), v->cnt[0], v->gbin, v->bin_off.as<int32_t>(), v->sorted.as<float4>(),
+                     v->chan[0].as<float>(), brick_res(v), std::max(v->apron, 0), v->res[0], radius, expand ? 1 : 0};
+    if (sp.apron == 0 && expand) FAIL(GVDBX_ERR_ARG, "ScatterDensity: expand needs an apron");
+    launch_scatter_density(sp, v->stream);
+    return check_launch("ScatterDensity");
+}
+
+int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz, int chan, int accum) {
+    TRY(check_vol(v));
+    TRY(need_atlas(v, "SplatCubic"));
+    TRY(check_chan(v, chan));
+    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "SplatCubic: negative point count");
+    if (!(radius > 0.0f) || radius > (float)v->res[0]) FAIL(GVDBX_ERR_RANGE, "SplatCubic: radius must be in (0, brick]");
+    float t[3] = {tx, ty, tz};
+    TRY(bin_points(v, num_pnts, t, true, false, 0, 0.0f, nullptr));
+    SplatParams sp{};
+    sp.leaves = v->nodes[0].as<GNode>();
+    sp.nleaf = v->cnt[0];
+    sp.gb = v->gbin;
+    sp.off = v->bin_off.as<int32_t>();
+    sp.sorted = v->sorted.as<float4>();
+    sp.weights = v->wts;
+    sp.wstride = v->wstride;
+    sp.chan = v->chan[chan].as<float>();
+    sp.brick_res = brick_res(v);
+    sp.apron = std::max(v->apron, 0);
+    sp.accum = accum ? 1 : 0;
+    sp.radius = radius;
+    launch_splat_cubic(sp, v->stream);
+    return check_launch("SplatCubic");
+}
+
+int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
+    TRY(check_vol(v));
+    TRY(check_chan(v, chan));
+    if (v->apron == 0) return GVDBX_OK;   // VG.cpp:4420
+    TRY(need_atlas(v, "UpdateApron"));
+    ApronParams ap{v->nodes[0].as<GNode>(), v->cnt[0], v->grid[0], v->idx[0].as<int32_t>(), v->res[0],
+                   v->chan[chan].as<float>(), brick_res(v), boundval};
+    launch_apron(ap, v->stream);
+    return check_launch("UpdateApron");
+}
+
+int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out) {
+    TRY(check_vol(v));
+    if (!out) FAIL(GVDBX_ERR_ARG, "null info pointer");
+    memset(out, 0, sizeof(*out));
+    out->levs = v->levs;
+    out->depth = v->depth;
+    out->top_lev = v->top_lev;
+    memcpy(out->root_pos, v->root_pos, sizeof(out->root_pos));
+    for (int l = 0; l < GVDBX_MAXLEV; l++) out->node_cnt[l] = v->cnt[l];
+    memcpy(out->subdim, v->subdim, sizeof(out->subdim));
+    out->brick_res = brick_res(v);
+    out->apron = std::max(v->apron, 0);
+    memcpy(out->pos_min, v->pos_min, sizeof(out->pos_min));
+    memcpy(out->pos_max, v->pos_max, sizeof(out->pos_max));
+    memcpy(out->obj_min, v->obj_min, sizeof(out->obj_min));
+    memcpy(out->obj_max, v->obj_max, sizeof(out->obj_max));
+    out->sc_len = v->sc_len;
+    out->topology_mode_used = v->topo_mode_used;
+    return GVDBX_OK;
+}
+
+int gvdbx_read_nodes(gvdbx_volume* v, int lev, void* out_host, int64_t max_nodes) {
+    TRY(check_vol(v));
+    if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "no topology");
+    if (lev < 0 || lev >= v->levs || !out_host) FAIL(GVDBX_ERR_ARG, "bad level or buffer");
+    int64_t n = v->cnt[lev];
+    if (max_nodes < n) FAIL(GVDBX_ERR_ARG, "buffer holds %lld nodes, need %lld", (long long)max_nodes, (long long)n);
+    if (n == 0) return GVDBX_OK;
+    GVDBX_CHECK(hipMemcpyAsync(out_host, v->nodes[lev].p, (size_t)n * sizeof(GNode), hipMemcpyDeviceToHost, v->stream));
+    return sync(v);
+}
+
+int gvdbx_read_child_lists(gvdbx_volume* v, int lev, uint64_t* out_host, int64_t max_entries) {
+    TRY(check_vol(v));
+    if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "no topology");
+    if (lev < 1 || lev >= v->levs || !out_host) FAIL(GVDBX_ERR_ARG, "bad level or buffer");
+    int64_t len = (int64_t)v->res[lev] * v->res[lev] * v->res[lev];
+    int64_t n = v->cnt[lev] * len;
+    if (max_entries < n) FAIL(GVDBX_ERR_ARG, "buffer too small");
+    if (n == 0) return GVDBX_OK;
+    GVDBX_CHECK(hipMemcpyAsync(out_host, v->clist[lev].p, (size_t)n * 8, hipMemcpyDeviceToHost, v->stream));
+    return sync(v);
+}
+
+int gvdbx_read_atlas_map(gvdbx_volume* v, void* out_host, int64_t max_entries) {
+    TRY(check_vol(v));
+    if (!v->has_atlas) FAIL(GVDBX_ERR_STATE, "no atlas map (UpdateAtlas)");
+    if (!out_host || max_entries < v->amap_cnt) FAIL(GVDBX_ERR_ARG, "buffer too small");
+    GVDBX_CHECK(hipMemcpyAsync(out_host, v->amap.p, (size_t)v->amap_cnt * sizeof(GAtlasNode), hipMemcpyDeviceToHost,
+                               v->stream));
+    return sync(v);
+}
+
+int gvdbx_read_bricks(gvdbx_volume* v, int chan, float* out_host, int64_t max_floats) {
+    TRY(check_vol(v));
+    TRY(check_chan(v, chan));
+    if (!v->has_atlas) FAIL(GVDBX_ERR_STATE, "no atlas (UpdateAtlas)");
+    int64_t n = v->cnt[0] * brick_vox(v);
+    if (!out_host || max_floats < n) FAIL(GVDBX_ERR_ARG, "buffer too small");
+    if (n == 0) return GVDBX_OK;
+    GVDBX_CHECK(hipMemcpyAsync(out_host, v->chan[chan].p, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+    return sync(v);
+}
+
+int gvdbx_channel_device_ptr(gvdbx_volume* v, int chan, float** out_dev) {
+    TRY(check_vol(v));
+    TRY(check_chan(v, chan));
+    if (!out_dev) FAIL(GVDBX_ERR_ARG, "null output");
+    *out_dev = v->chan[chan].as<float>();
+    return GVDBX_OK;
+}
+
+int gvdbx_read_point_nodes(gvdbx_volume* v, int32_t* pnode_host, int32_t* poff_host, int64_t n) {
+    TRY(check_vol(v));
+    if (n > v->npnode) FAIL(GVDBX_ERR_ARG, "only %lld points were inserted", (long long)v->npnode);
+    if (n == 0) return GVDBX_OK;
+    if (pnode_host)
+        GVDBX_CHECK(hipMemcpyAsync(pnode_host, v->pnode.p, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+    if (poff_host) GVDBX_CHECK(hipMemcpyAsync(poff_host, v->poff.p, (size_t)n * 4, hipMemcpyDeviceToHost, v->stream));
+    return sync(v);
+}
+
+}  // extern "C"

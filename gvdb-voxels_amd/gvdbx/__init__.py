@@ -1,0 +1,199 @@
+"""gvdbx: MI355X-native GVDB points -> sparse volume path.
+
+`VolumeGVDB` mirrors the reference class nvdb::VolumeGVDB
+(source/gvdb_library/src/gvdb_volume_gvdb.h:319-901) on the points path:
+Configure, AddChannel, SetPoints, RebuildTopology, FinishTopology,
+UpdateAtlas, InsertPointsSubcell, GatherDensity, GatherLevelSet,
+InsertPoints, ScatterDensity, UpdateApron.  Every call goes through the
+C-ABI of libgvdbx.so; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import GvdbxError, Info, TOPO_AUTO, TOPO_DENSE, TOPO_RADIX, call, exported_names, lib, last_error  # noqa: F401
+
+NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("pad", "u1"),
+                       ("mPos", "<i4", 3), ("mValue", "<i4", 3), ("mVRange", "<f4", 3),
+                       ("mParent", "<u8"), ("mChildList", "<u8"), ("mMask", "<u8")])
+AMAP_DTYPE = np.dtype([("mPos", "<i4", 3), ("mLeafID", "<i4")])
+T_FLOAT = 0
+F_LINEAR, F_CLAMP = 1, 1
+
+
+def _ptr(x):
+    """Device address of a torch tensor (or an int address)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+class VolumeGVDB:
+    """One sparse volume on one GPU (reference: VolumeGVDB)."""
+
+    def __init__(self, device=0, stream=None):
+        self._h = C.c_void_p()
+        call("gvdbx_create", C.byref(self._h), int(device))
+        self._keep = []
+        if stream is not None:
+            self.SetStream(stream)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gvdbx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ setup
+    def SetStream(self, stream):
+        """stream: a torch.cuda.Stream, or a raw hipStream_t address."""
+        handle = getattr(stream, "cuda_stream", stream)
+        call("gvdbx_set_stream", self._h, C.c_void_p(handle))
+
+    def Synchronize(self):
+        call("gvdbx_synchronize", self._h)
+
+    def Configure(self, q4, q3, q2, q1, q0):
+        """Configure(q4,q3,q2,q1,q0) (VG.h:387): level 0 (the brick) gets q0."""
+        self.configure([q0, q1, q2, q3, q4])
+
+    def configure(self, log2dims):
+        arr = (C.c_int * len(log2dims))(*[int(x) for x in log2dims])
+        call("gvdbx_configure", self._h, len(log2dims), arr)
+        self.log2dims = tuple(int(x) for x in log2dims)
+
+    def SetChannelDefault(self, sx, sy, sz):
+        call("gvdbx_set_channel_default", self._h, sx, sy, sz)
+
+    def AddChannel(self, chan, dtype=T_FLOAT, apron=1, filter=F_LINEAR, border=F_CLAMP):
+        call("gvdbx_add_channel", self._h, chan, dtype, apron, filter, border)
+
+    def FillChannel(self, chan, value):
+        call("gvdbx_fill_channel", self._h, chan, float(value))
+
+    def SetTopologyMode(self, mode):
+        call("gvdbx_set_topology_mode", self._h, int(mode))
+
+    # ------------------------------------------------------------ points
+    def SetPoints(self, pos, vel=None, clr=None, pos_off=0, pos_stride=12):
+        """pos: device float32 tensor (N,3) (or an address with offset/stride)."""
+        self._keep = [pos, vel, clr]
+        call("gvdbx_set_points", self._h, C.c_void_p(_ptr(pos)), pos_off, pos_stride,
+             C.c_void_p(_ptr(vel)) if vel is not None else None, 0, 12,
+             C.c_void_p(_ptr(clr)) if clr is not None else None, 0, 4)
+
+    def SetPointWeights(self, w, stride=4):
+        self._keep_w = w
+        call("gvdbx_set_point_weights", self._h, C.c_void_p(_ptr(w)) if w is not None else None, stride)
+
+    # ------------------------------------------------------------ topology
+    def RebuildTopology(self, num_pnts, radius, orig=(0.0, 0.0, 0.0)):
+        call("gvdbx_rebuild_topology", self._h, int(num_pnts), float(radius), *[float(x) for x in orig])
+
+    def FinishTopology(self, commit_pool=False, compute_bounds=True):
+        call("gvdbx_finish_topology", self._h, int(commit_pool), int(compute_bounds))
+
+    def UpdateAtlas(self):
+        call("gvdbx_update_atlas", self._h)
+
+    # ------------------------------------------------------------ points -> voxels
+    def InsertPointsSubcell(self, subcell_size, num_pnts, radius, trans=(0.0, 0.0, 0.0)):
+        out = C.c_int64(0)
+        call("gvdbx_insert_points_subcell", self._h, int(subcell_size), int(num_pnts), float(radius),
+             *[float(x) for x in trans], C.byref(out))
+        return out.value
+
+    def GatherDensity(self, subcell_size, num_pnts, radius, trans, sc_len, chan_density=0, chan_clr=-1, accum=False):
+        call("gvdbx_gather_density", self._h, int(subcell_size), int(num_pnts), float(radius),
+             *[float(x) for x in trans], int(sc_len), int(chan_density), int(chan_clr), int(accum))
+
+    def GatherLevelSet(self, subcell_size, num_pnts, radius, trans, sc_len, chan_levelset=0, chan_clr=-1,
+                       accum=False):
+        call("gvdbx_gather_level_set", self._h, int(subcell_size), int(num_pnts), float(radius),
+             *[float(x) for x in trans], int(sc_len), int(chan_levelset), int(chan_clr), int(accum))
+
+    def InsertPoints(self, num_pnts, trans=(0.0, 0.0, 0.0), prefix=True):
+        call("gvdbx_insert_points", self._h, int(num_pnts), *[float(x) for x in trans], int(prefix))
+
+    def ScatterDensity(self, num_pnts, radius, amp=1.0, trans=(0.0, 0.0, 0.0), expand=True, avg_color=False):
+        call("gvdbx_scatter_density", self._h, int(num_pnts), float(radius), float(amp),
+             *[float(x) for x in trans], int(expand), int(avg_color))
+
+    def SplatCubic(self, num_pnts, radius, trans=(0.0, 0.0, 0.0), chan=0, accum=False):
+        call("gvdbx_splat_cubic", self._h, int(num_pnts), float(radius), *[float(x) for x in trans], int(chan),
+             int(accum))
+
+    def UpdateApron(self, chan=0, boundval=0.0):
+        call("gvdbx_update_apron", self._h, int(chan), float(boundval))
+
+    # ------------------------------------------------------------ readback
+    def info(self):
+        inf = Info()
+        call("gvdbx_get_info", self._h, C.byref(inf))
+        return inf
+
+    def nodes(self, lev):
+        n = self.info().node_cnt[lev]
+        out = np.zeros(n, dtype=NODE_DTYPE)
+        if n:
+            call("gvdbx_read_nodes", self._h, lev, out.ctypes.data, n)
+        return out
+
+    def child_lists(self, lev):
+        inf = self.info()
+        r = 1 << self.log2dims[lev]
+        n = inf.node_cnt[lev]
+        out = np.zeros((n, r ** 3), dtype=np.uint64)
+        if n:
+            call("gvdbx_read_child_lists", self._h, lev, out.ctypes.data, out.size)
+        return out
+
+    def atlas_map(self):
+        inf = self.info()
+        n = inf.subdim[0] * inf.subdim[1] * inf.subdim[2]
+        out = np.zeros(n, dtype=AMAP_DTYPE)
+        call("gvdbx_read_atlas_map", self._h, out.ctypes.data, n)
+        return out
+
+    def bricks(self, chan=0):
+        """All leaf bricks incl. apron, (B, br, br, br) indexed [k, z, y, x]."""
+        inf = self.info()
+        br = inf.brick_res
+        B = inf.node_cnt[0]
+        out = np.zeros((B, br, br, br), dtype=np.float32)
+        call("gvdbx_read_bricks", self._h, chan, out.ctypes.data, out.size)
+        return out
+
+    def channel_ptr(self, chan=0):
+        p = C.c_void_p()
+        call("gvdbx_channel_device_ptr", self._h, chan, C.byref(p))
+        return p.value
+
+    def point_nodes(self, n):
+        pnode = np.zeros(n, dtype=np.int32)
+        poff = np.zeros(n, dtype=np.int32)
+        call("gvdbx_read_point_nodes", self._h, pnode.ctypes.data, poff.ctypes.data, n)
+        return pnode, poff
+
+
+def gen_shell(seed, n, center=(256.0, 256.0, 256.0), r0=180.0, r1=200.0):
+    """Seeded points uniform by volume in a spherical shell (host, float32 (n,3))."""
+    out = np.zeros((n, 3), dtype=np.float32)
+    call("gvdbx_gen_shell", C.c_uint64(seed), n, *[float(c) for c in center], float(r0), float(r1), out.ctypes.data)
+    return out
+
+
+def gen_box(seed, n, lo, hi, jitter=0.0):
+    """Seeded points uniform in the box [lo, hi) plus per-axis jitter U(-j, j)."""
+    out = np.zeros((n, 3), dtype=np.float32)
+    flo = (C.c_float * 3)(*[float(x) for x in lo])
+    fhi = (C.c_float * 3)(*[float(x) for x in hi])
+    call("gvdbx_gen_box", C.c_uint64(seed), n, flo, fhi, float(jitter), out.ctypes.data)
+    return out

@@ -1,0 +1,151 @@
+/*
+ * gvdbx_c.h -- C-ABI of the MI355X-native point -> sparse-volume library.
+ *
+ * This is the drop-in boundary for GVDB Voxels' points path.  Each entry
+ * point replaces one method of nvdb::VolumeGVDB; the citation after each
+ * declaration names the reference method (paths relative to the reference
+ * repository; VG.h = source/gvdb_library/src/gvdb_volume_gvdb.h, VG.cpp =
+ * source/gvdb_library/src/gvdb_volume_gvdb.cpp).
+ *
+ * Conventions
+ *   - every entry point returns a status (GVDBX_OK == 0); the message of the
+ *     last failure on the calling thread is gvdbx_last_error();
+ *   - a failing call leaves the volume as it was before the call;
+ *   - pointers named *_dev are device pointers (HBM); *_host are host memory;
+ *   - the library never frees caller memory;
+ *   - work is issued on the volume's stream (gvdbx_set_stream; default: the
+ *     null stream); calls that return host data synchronise that stream.
+ */
+#ifndef GVDBX_C_H
+#define GVDBX_C_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gvdbx_volume gvdbx_volume;
+
+enum gvdbx_status {
+    GVDBX_OK = 0,
+    GVDBX_ERR_ARG = 1,       /* bad argument or handle */
+    GVDBX_ERR_STATE = 2,     /* call out of order (e.g. no topology / no channel) */
+    GVDBX_ERR_DEPTH = 3,     /* point extents need depth 0 or more levels than configured */
+    GVDBX_ERR_NOPOINTS = 4,  /* no valid point */
+    GVDBX_ERR_RANGE = 5,     /* extents exceed key or grid limits */
+    GVDBX_ERR_HIP = 6,       /* HIP runtime failure */
+    GVDBX_ERR_NOMEM = 7      /* device allocation failed */
+};
+
+enum gvdbx_dtype { GVDBX_FLOAT = 0 };
+
+enum gvdbx_topology_mode {
+    GVDBX_TOPO_AUTO = 0,       /* dense cell grids when they fit, else radix sort */
+    GVDBX_TOPO_DENSE = 1,
+    GVDBX_TOPO_RADIX = 2
+};
+
+/* summary of the current topology, filled by gvdbx_get_info */
+typedef struct {
+    int levs;
+    int depth;                 /* root level (DetermineDepth VG.cpp:821-837) */
+    int top_lev;               /* lowest level with exactly one node (VG.cpp:3996-4012) */
+    int root_pos[3];
+    int64_t node_cnt[10];      /* nodes per level, root level included */
+    int subdim[3];             /* atlas bricks per axis */
+    int brick_res;             /* res0 + 2*apron */
+    int apron;
+    float pos_min[3], pos_max[3];   /* GetBoundingBox result (VG.cpp:5404-5467) */
+    float obj_min[3], obj_max[3];   /* ComputeBounds result (VG.cpp:1792-1816) */
+    int64_t sc_len;                 /* last InsertPointsSubcell list length */
+    int topology_mode_used;         /* gvdbx_topology_mode actually taken */
+} gvdbx_info;
+
+/* ---- lifecycle ---- */
+const char* gvdbx_last_error(void);
+const char* gvdbx_version(void);
+/* VolumeGVDB::VolumeGVDB + SetCudaDevice + Initialize (VG.cpp:240,2320-2361) */
+int gvdbx_create(gvdbx_volume** out, int device);
+int gvdbx_destroy(gvdbx_volume* v);
+/* hipStream_t passed as void* so the ABI carries no HIP types */
+int gvdbx_set_stream(gvdbx_volume* v, void* stream);
+int gvdbx_synchronize(gvdbx_volume* v);
+
+/* ---- configuration ---- */
+/* Configure(levs, r, numcnt) VG.h:388 / VG.cpp:2380-2434; log2dims[0] is
+ * the brick level.  Configure(q4,q3,q2,q1,q0) VG.h:387 == {q0,q1,q2,q3,q4}. */
+int gvdbx_configure(gvdbx_volume* v, int levs, const int* log2dims);
+/* SetChannelDefault VG.h:390 */
+int gvdbx_set_channel_default(gvdbx_volume* v, int sx, int sy, int sz);
+/* AddChannel VG.h:392 / VG.cpp:2437-2453 (float channels; filter/border
+ * are recorded for Render) */
+int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filter, int border);
+/* FillChannel VG.cpp:2456-2470 (float) */
+int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value);
+/* choose the topology builder (no reference counterpart; default AUTO) */
+int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
+
+/* ---- points ---- */
+/* SetPoints(DataPtr pos, vel, clr) VG.h:541 / VG.cpp:4900-4908 with
+ * SetDataGPU VG.cpp:4872-4880: float3 positions at pos_dev + i*stride + off.
+ * vel/clr may be NULL. */
+int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_stride,
+                     const void* vel_dev, int vel_off, int vel_stride,
+                     const void* clr_dev, int clr_off, int clr_stride);
+/* per-point FP32 weights (extension; no reference counterpart) */
+int gvdbx_set_point_weights(gvdbx_volume* v, const float* w_dev, int stride);
+
+/* ---- topology ---- */
+/* RebuildTopology(int, float, Vector3DF) VG.h:429 / VG.cpp:1181-1218 */
+int gvdbx_rebuild_topology(gvdbx_volume* v, int num_pnts, float radius, float ox, float oy, float oz);
+/* FinishTopology(bool, bool) VG.h:405 / VG.cpp:1579-1593 */
+int gvdbx_finish_topology(gvdbx_volume* v, int commit_pool, int compute_bounds);
+/* UpdateAtlas() VG.h:406 / VG.cpp:2630-2704 */
+int gvdbx_update_atlas(gvdbx_volume* v);
+
+/* ---- points -> voxels ---- */
+/* InsertPointsSubcell(int, int, float, Vector3DF, int&) VG.h:551 / VG.cpp:4971-5092 */
+int gvdbx_insert_points_subcell(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
+                                float tx, float ty, float tz, int64_t* out_len);
+/* GatherDensity VG.h:556 / VG.cpp:5318-5344 */
+int gvdbx_gather_density(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
+                         float tx, float ty, float tz, int64_t sc_len, int chan_density, int chan_clr, int accum);
+/* GatherLevelSet VG.h:557 / VG.cpp:5348-5373 */
+int gvdbx_gather_level_set(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
+                           float tx, float ty, float tz, int64_t sc_len, int chan_levelset, int chan_clr, int accum);
+/* InsertPoints(int, Vector3DF, bool) VG.h:542 / VG.cpp:5219-5267 */
+int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix);
+/* ScatterDensity(int, float, float, Vector3DF, bool, bool) VG.h:555 / VG.cpp:5624-5660 */
+int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp,
+                          float tx, float ty, float tz, int expand, int avg_color);
+/* weighted cubic splat rho(v) = sum w_p max(0,1-|v-p|^2/r^2)^3 (extension of
+ * the ScatterDensity kernel cuda_gvdb_particles.cuh:516-523) */
+int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz,
+                      int chan, int accum);
+/* UpdateApron(uchar, float, bool) VG.h:408 / VG.cpp:4418-4457 */
+int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
+
+/* ---- readback (host copies; synchronising) ---- */
+int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out);
+/* pool group 0 of one level: 64-byte records (VG.h getNode / AL.cpp PoolFetch) */
+int gvdbx_read_nodes(gvdbx_volume* v, int lev, void* out_host, int64_t max_nodes);
+/* pool group 1 of one level: res^3 uint64 entries per node */
+int gvdbx_read_child_lists(gvdbx_volume* v, int lev, uint64_t* out_host, int64_t max_entries);
+/* atlas map, 16-byte entries, subdim.x*subdim.y*subdim.z of them */
+int gvdbx_read_atlas_map(gvdbx_volume* v, void* out_host, int64_t max_entries);
+/* all leaf bricks of a channel incl. apron, leaf-major, x fastest */
+int gvdbx_read_bricks(gvdbx_volume* v, int chan, float* out_host, int64_t max_floats);
+/* device pointer of a channel: brick of leaf k at k*brick_res^3 floats */
+int gvdbx_channel_device_ptr(gvdbx_volume* v, int chan, float** out_dev);
+/* point -> leaf index and rank within leaf from the last InsertPoints */
+int gvdbx_read_point_nodes(gvdbx_volume* v, int32_t* pnode_host, int32_t* poff_host, int64_t n);
+
+/* ---- host-side workload generators (no reference counterpart) ---- */
+/* seeded synthetic inputs shared by tests and bench (SplitMix64 -> xoshiro256**) */
+int gvdbx_gen_shell(uint64_t seed, int64_t n, float cx, float cy, float cz, float r0, float r1, float* out_xyz);
+int gvdbx_gen_box(uint64_t seed, int64_t n, const float* lo, const float* hi, float jitter, float* out_xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1053 @@
+/*
+ * gvdb_oracle.c -- TEST INFRASTRUCTURE ONLY (see gvdb_oracle.h).
+ *
+ * A single-threaded C restatement of the reference's point -> sparse-volume
+ * path.  It follows the reference's own algorithm (sort keys, unique, pool
+ * allocation in key order, top-down linking by tree descent, subcell point
+ * lists, per-subcell gather), not the GPU design of the product, so that the
+ * two are independent implementations of one semantics.
+ *
+ * Floating-point conventions (fixed here and in the product kernels):
+ *   - every expression is evaluated in IEEE binary32 with round-to-nearest and
+ *     no contraction (built with -ffp-contract=off);
+ *   - a squared distance is fmaf(dz,dz, fmaf(dy,dy, dx*dx));
+ *   - sqrtf and divisions are IEEE correctly rounded.
+ * The reference builds with --use_fast_math, whose results are not
+ * reproducible off NVIDIA hardware; float results are therefore pinned to
+ * these defined semantics and, separately, to an FP64 evaluation.
+ *
+ * Deliberate deviations (the reference is undefined or racy there):
+ *   D1 invalid points (non-finite, z == NOHIT before translation, or any
+ *      coordinate < 0 after translation) are dropped from every stage,
+ *      including the bounding box (reference: VG.cpp:5404-5467 includes them);
+ *   D2 a child-list read outside the allocated lists returns "empty"
+ *      (reference reads stale pool memory, ND.cuh:115-124);
+ *   D3 the extra-brick existence test is "the node is in the phase-1 node
+ *      set" (reference getPosParent, ND.cuh:303-348, which equals this when
+ *      the descent is defined) and extra nodes outside the root node are not
+ *      emitted (reference: out-of-range child-list reads);
+ *   D4 a point -> leaf lookup that lands on a leaf not containing the
+ *      position is "no leaf" (reference: out-of-range subcell index writes,
+ *      PT.cuh:255-266, or unchecked root when top_lev==0, ND.cuh:199-226);
+ *   D5 list order and poff are by ascending point index (reference: atomic
+ *      order, PT.cuh:35,266);
+ *   D6 ScatterDensity accumulates serially in point order (reference: racy
+ *      surface read-modify-write, PT.cuh:543-551).
+ */
+#include "gvdb_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct or_volume {
+    int levs;
+    int logdim[OR_MAXLEV];
+    int res[OR_MAXLEV];
+    int64_t range[OR_MAXLEV];
+    /* pools: group 0 nodes, group 1 child lists (AL.cpp:55-231) */
+    or_node* nodes[OR_MAXLEV];
+    int64_t ncnt[OR_MAXLEV], ncap[OR_MAXLEV];
+    uint64_t* clist[OR_MAXLEV];
+    int64_t ccnt[OR_MAXLEV], ccap[OR_MAXLEV];
+    int depth, top_lev;
+    int root_pos[3];
+    float pos_min[3], pos_max[3];
+    float obj_min[3], obj_max[3];
+    int configured;
+    /* atlas (AL.cpp:528-715) */
+    int default_axis[3];
+    int subdim[3];
+    int apron;
+    int has_chan[OR_MAXCHAN];
+    float* atlas[OR_MAXCHAN];
+    int64_t atlas_vox;
+    or_atlas_node* amap;
+    int64_t amap_cnt;
+    int64_t atlas_used;
+    /* subcell lists (VG.cpp:4971-5092) */
+    int sc_size, sc_per_brick, sc_dim;
+    int64_t num_sc;
+    int64_t sc_len;
+    int32_t* sc_cnt;
+    int64_t* sc_off;
+    float* sc_pnt;
+    int32_t* sc_nid;
+    int32_t* sc_pos;
+};
+
+/* ------------------------------------------------------------------ */
+/* helpers                                                             */
+/* ------------------------------------------------------------------ */
+
+static int64_t ipow3(int64_t r) { return r * r * r; }
+
+/* getRange VG.h:614-619 */
+static int64_t or_range(const or_volume* v, int lev) {
+    if (lev < 0) return 1;
+    int64_t r = v->res[0];
+    for (int l = 1; l <= lev; l++) r *= v->res[l];
+    return r;
+}
+
+/* D1: validity of a raw input point under translation */
+static int point_valid(const float* p, const float t[3], float w[3]) {
+    if (!isfinite(p[0]) || !isfinite(p[1]) || !isfinite(p[2])) return 0;
+    if (p[2] == OR_NOHIT) return 0;
+    w[0] = p[0] + t[0];
+    w[1] = p[1] + t[1];
+    w[2] = p[2] + t[2];
+    if (!isfinite(w[0]) || !isfinite(w[1]) || !isfinite(w[2])) return 0;
+    if (w[0] < 0 || w[1] < 0 || w[2] < 0) return 0;
+    return 1;
+}
+
+static const float* pt(const float* base, int64_t i, int64_t stride) {
+    return (const float*)((const char*)base + i * stride);
+}
+
+/* PoolAlloc AL.cpp:163-192 (capacity doubling; contents zeroed) */
+static int64_t pool_alloc_node(or_volume* v, int lev) {
+    if (v->ncnt[lev] >= v->ncap[lev]) {
+        int64_t nc = v->ncap[lev] ? v->ncap[lev] * 2 : 4;
+        or_node* nn = (or_node*)calloc((size_t)nc, sizeof(or_node));
+        if (v->nodes[lev]) {
+            memcpy(nn, v->nodes[lev], (size_t)v->ncnt[lev] * sizeof(or_node));
+            free(v->nodes[lev]);
+        }
+        v->nodes[lev] = nn;
+        v->ncap[lev] = nc;
+    }
+    int64_t i = v->ncnt[lev]++;
+    memset(&v->nodes[lev][i], 0, sizeof(or_node));
+    return i;
+}
+
+static int64_t pool_alloc_clist(or_volume* v, int lev) {
+    int64_t len = ipow3(v->res[lev]);
+    if (v->ccnt[lev] >= v->ccap[lev]) {
+        int64_t nc = v->ccap[lev] ? v->ccap[lev] * 2 : 4;
+        uint64_t* nn = (uint64_t*)malloc((size_t)(nc * len) * sizeof(uint64_t));
+        if (v->clist[lev]) {
+            memcpy(nn, v->clist[lev], (size_t)(v->ccnt[lev] * len) * sizeof(uint64_t));
+            free(v->clist[lev]);
+        }
+        v->clist[lev] = nn;
+        v->ccap[lev] = nc;
+    }
+    int64_t i = v->ccnt[lev]++;
+    /* ActivateBricksGPU fills every new child list with 0xFF (VG.cpp:1327-1329) */
+    memset(v->clist[lev] + i * len, 0xFF, (size_t)len * sizeof(uint64_t));
+    return i;
+}
+
+static uint64_t elem(int grp, int lev, int64_t ndx) {
+    /* Elem AL.h:59-62 */
+    return (uint64_t)grp | ((uint64_t)lev << 8) | ((uint64_t)ndx << 16);
+}
+
+/* SetupNode VG.cpp:2507-2520 */
+static void setup_node(or_volume* v, int lev, int64_t idx, int px, int py, int pz) {
+    or_node* n = &v->nodes[lev][idx];
+    n->mLev = (uint8_t)lev;
+    n->mPos[0] = px; n->mPos[1] = py; n->mPos[2] = pz;
+    n->mChildList = OR_ID_UNDEFL;
+    n->mParent = OR_ID_UNDEFL;
+    n->mValue[0] = n->mValue[1] = n->mValue[2] = -1;
+    n->mFlags = 1;
+}
+
+/* getChild ND.cuh:115-124, with D2 for reads outside the allocated lists */
+static int64_t get_child(const or_volume* v, const or_node* nd, int64_t b) {
+    uint64_t listid = nd->mChildList;
+    if (listid == OR_ID_UNDEFL) return -1;
+    int clev = (int)((listid >> 8) & 0xFF);
+    int64_t cndx = (int64_t)(listid >> 16);
+    int64_t len = ipow3(v->res[clev]);
+    int64_t flat = cndx * len + b;
+    if (clev <= 0 || clev >= v->levs) return -1;
+    if (flat < 0 || flat >= v->ccnt[clev] * len) return -1;
+    uint64_t e = v->clist[clev][flat];
+    if (e == OR_ID_UNDEF64) return -1;
+    return (int64_t)(int32_t)(uint32_t)(e >> 16);
+}
+
+/* integer (pos - npos)*res/range with C truncation, ND.cuh:318-323 */
+static void pos_in_node(const int pos[3], const int32_t np[3], int res, int64_t range, int64_t out[3]) {
+    for (int a = 0; a < 3; a++) {
+        int64_t d = (int64_t)pos[a] - np[a];
+        d *= res;
+        out[a] = d / range;
+    }
+}
+
+/* getParent ND.cuh:350-397 (used by LinkBricks) */
+static int64_t get_parent(const or_volume* v, int stop_lev, const int pos[3]) {
+    if (v->ncnt[v->top_lev] < 1) return -1;
+    const or_node* nd = &v->nodes[v->top_lev][0];
+    if (!nd->mFlags) return 0;
+    int l = nd->mLev;
+    if (l == stop_lev) return 0;
+    while (l > stop_lev) {
+        int res = v->res[l];
+        int64_t pin[3];
+        pos_in_node(pos, nd->mPos, res, v->range[l], pin);
+        int64_t bit = (pin[2] * res + pin[1]) * res + pin[0];
+        l--;
+        int64_t c = get_child(v, nd, bit);
+        if (c < 0) return -1;
+        if (l == stop_lev) return c;
+        if (c >= v->ncnt[l]) return -1;
+        nd = &v->nodes[l][c];
+    }
+    return -1;
+}
+
+/* top level selection of PrepareVDBPartially VG.cpp:3996-4012 */
+static void compute_top_lev(or_volume* v) {
+    int tlev = 1;
+    for (int n = v->levs - 1; n >= 0; n--)
+        if (v->ncnt[n] == 1) tlev = n;
+    v->top_lev = tlev;
+}
+
+/* gvdbLinkBricks ND.cuh:470-505, launched top-down (VG.cpp:1394-1402) */
+static void link_level(or_volume* v, int lev) {
+    if (lev + 1 >= v->levs) return;
+    int res = v->res[lev + 1];
+    int64_t range = v->range[lev + 1];
+    int64_t len = ipow3(res);
+    for (int64_t i = 0; i < v->ncnt[lev]; i++) {
+        or_node* node = &v->nodes[lev][i];
+        if (!node->mFlags) continue;
+        int pos[3] = {node->mPos[0], node->mPos[1], node->mPos[2]};
+        int64_t p = get_parent(v, lev + 1, pos);
+        if (p < 0 || p >= v->ncnt[lev + 1]) continue;
+        or_node* pn = &v->nodes[lev + 1][p];
+        int64_t pin[3];
+        pos_in_node(pos, pn->mPos, res, range, pin);
+        int64_t bit = (pin[2] * res + pin[1]) * res + pin[0];
+        if (pin[0] > res || pin[0] < 0 || pin[1] > res || pin[1] < 0 || pin[2] > res || pin[2] < 0) continue;
+        node->mParent = ((uint64_t)p << 16) | ((uint64_t)(lev + 1) << 8);
+        int64_t cndx = (int64_t)(pn->mChildList >> 16);
+        if (cndx >= v->ncnt[lev + 1]) continue;
+        int64_t flat = cndx * len + bit;
+        if (flat < 0 || flat >= v->ccnt[lev + 1] * len) continue; /* D2 */
+        v->clist[lev + 1][flat] = ((uint64_t)i << 16) | ((uint64_t)lev << 8);
+    }
+}
+
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* GetCoveringNode (float version) PT.cuh:88-100 */
+static int cover_axis(float p, int range) {
+    float q = p / (float)range;
+    int np = (int)(ceilf(q) * (float)range);
+    if (p < (float)np) np -= range;
+    return np;
+}
+
+static uint64_t make_key(int lev, int kx, int ky, int kz) {
+    /* u16 quad {z,y,x,lev} read as little-endian u64 (PT.cuh:377-380, VG.cpp:916-923) */
+    return ((uint64_t)(uint16_t)lev << 48) | ((uint64_t)(uint16_t)kx << 32) |
+           ((uint64_t)(uint16_t)ky << 16) | (uint64_t)(uint16_t)kz;
+}
+
+static int64_t unique_u64(uint64_t* a, int64_t n) {
+    /* gvdbFindUnique + gvdbCompactUnique PT.cuh:336-350,934-943 */
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (i == 0 || a[i] != a[i - 1]) a[m++] = a[i];
+    return m;
+}
+
+/* allocate nodes of one sorted-unique key set, per level in key order
+ * (ActivateBricksGPU VG.cpp:1305-1331, ActivateExtraBricksGPU 966-988) */
+static void alloc_from_keys(or_volume* v, const uint64_t* keys, int64_t m) {
+    for (int lev = 0; lev < v->depth; lev++) {
+        int64_t rng = v->range[lev];
+        for (int64_t j = 0; j < m; j++) {
+            uint64_t k = keys[j];
+            if ((int)((k >> 48) & 0xFFFF) != lev) continue;
+            int kx = (int)((k >> 32) & 0xFFFF), ky = (int)((k >> 16) & 0xFFFF), kz = (int)(k & 0xFFFF);
+            int64_t id = pool_alloc_node(v, lev);
+            setup_node(v, lev, id, (int)(kx * rng), (int)(ky * rng), (int)(kz * rng));
+            if (lev > 0) {
+                int64_t c = pool_alloc_clist(v, lev);
+                v->nodes[lev][id].mChildList = elem(1, lev, c);
+            }
+        }
+    }
+}
+
+static int key_in(const uint64_t* keys, int64_t m, uint64_t k) {
+    int64_t lo = 0, hi = m;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (keys[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    return lo < m && keys[lo] == k;
+}
+
+/* ------------------------------------------------------------------ */
+/* lifecycle / configuration                                           */
+/* ------------------------------------------------------------------ */
+
+or_volume* or_create(void) {
+    or_volume* v = (or_volume*)calloc(1, sizeof(or_volume));
+    v->default_axis[0] = 16; v->default_axis[1] = 16; v->default_axis[2] = 1;  /* VG.cpp:2348 */
+    v->top_lev = 1;
+    return v;
+}
+
+static void free_sc(or_volume* v) {
+    free(v->sc_cnt); free(v->sc_off); free(v->sc_pnt); free(v->sc_nid); free(v->sc_pos);
+    v->sc_cnt = NULL; v->sc_off = NULL; v->sc_pnt = NULL; v->sc_nid = NULL; v->sc_pos = NULL;
+    v->num_sc = 0; v->sc_len = 0;
+}
+
+void or_destroy(or_volume* v) {
+    if (!v) return;
+    for (int l = 0; l < OR_MAXLEV; l++) { free(v->nodes[l]); free(v->clist[l]); }
+    for (int c = 0; c < OR_MAXCHAN; c++) free(v->atlas[c]);
+    free(v->amap);
+    free_sc(v);
+    free(v);
+}
+
+int or_configure(or_volume* v, int levs, const int* logdim) {
+    if (!v || levs < 2 || levs > OR_MAXLEV || !logdim) return OR_ERR_ARG;
+    for (int l = 0; l < OR_MAXLEV; l++) {
+        free(v->nodes[l]); free(v->clist[l]);
+        v->nodes[l] = NULL; v->clist[l] = NULL;
+        v->ncnt[l] = v->ncap[l] = v->ccnt[l] = v->ccap[l] = 0;
+    }
+    v->levs = levs;
+    for (int l = 0; l < levs; l++) {
+        v->logdim[l] = logdim[l] == 0 ? 1 : logdim[l];
+        if (v->logdim[l] < 1 || v->logdim[l] > 10) return OR_ERR_ARG;
+        v->res[l] = 1 << v->logdim[l];
+    }
+    for (int l = 0; l < levs; l++) v->range[l] = or_range(v, l);
+    v->configured = 1;
+    v->depth = 0;
+    v->top_lev = 1;
+    return OR_OK;
+}
+
+void or_set_channel_default(or_volume* v, int sx, int sy, int sz) {
+    v->default_axis[0] = sx; v->default_axis[1] = sy; v->default_axis[2] = sz;
+}
+
+static int brick_res(const or_volume* v) { return v->res[0] + 2 * v->apron; }
+
+static void atlas_resize(or_volume* v, int sz) {
+    /* AtlasResize AL.cpp:622-650: grows Z, preserves the old contents */
+    int br = brick_res(v);
+    int64_t nvox = (int64_t)v->subdim[0] * br * (int64_t)v->subdim[1] * br * (int64_t)sz * br;
+    for (int c = 0; c < OR_MAXCHAN; c++) {
+        if (!v->has_chan[c]) continue;
+        float* na = (float*)calloc((size_t)nvox, sizeof(float));
+        if (v->atlas[c]) {
+            int64_t keep = v->atlas_vox < nvox ? v->atlas_vox : nvox;
+            memcpy(na, v->atlas[c], (size_t)keep * sizeof(float));
+            free(v->atlas[c]);
+        }
+        v->atlas[c] = na;
+    }
+    v->subdim[2] = sz;
+    v->atlas_vox = nvox;
+}
+
+int or_add_channel(or_volume* v, int chan, int apron) {
+    if (!v || !v->configured || chan < 0 || chan >= OR_MAXCHAN || apron < 0) return OR_ERR_ARG;
+    int any = 0;
+    for (int c = 0; c < OR_MAXCHAN; c++) any |= v->has_chan[c];
+    if (!any) {
+        v->subdim[0] = v->default_axis[0]; v->subdim[1] = v->default_axis[1]; v->subdim[2] = v->default_axis[2];
+        v->apron = apron;
+        v->atlas_vox = 0;
+    } else if (apron != v->apron) {
+        return OR_ERR_ARG;
+    }
+    v->has_chan[chan] = 1;
+    int br = brick_res(v);
+    int64_t nvox = (int64_t)v->subdim[0] * br * (int64_t)v->subdim[1] * br * (int64_t)v->subdim[2] * br;
+    free(v->atlas[chan]);
+    v->atlas[chan] = (float*)calloc((size_t)nvox, sizeof(float));
+    v->atlas_vox = nvox;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* topology                                                            */
+/* ------------------------------------------------------------------ */
+
+int or_rebuild_topology(or_volume* v, const float* pos, int64_t n, int64_t stride, float radius,
+                        const float orig[3]) {
+    if (!v || !v->configured || (!pos && n > 0) || n < 0) return OR_ERR_ARG;
+    if (stride == 0) stride = 12;
+    /* GetBoundingBox VG.cpp:5404-5467 over valid points (D1) */
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int64_t nvalid = 0;
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), orig, w)) continue;
+        nvalid++;
+        for (int a = 0; a < 3; a++) {
+            if (w[a] < mn[a]) mn[a] = w[a];
+            if (w[a] > mx[a]) mx[a] = w[a];
+        }
+    }
+    if (nvalid == 0) return OR_ERR_NOPOINTS;
+    float pmin[3], pmax[3];
+    for (int a = 0; a < 3; a++) {
+        pmin[a] = mn[a] - 1;
+        pmax[a] = mx[a] + 2;
+        pmin[a] = 0 > pmin[a] ? 0 : pmin[a];
+        pmax[a] = 0 > pmax[a] ? 0 : pmax[a];
+    }
+    /* DetermineDepth VG.cpp:821-837 with GetCoveringNode VG.cpp:2915-2931 */
+    int ipmin[3] = {(int)pmin[0], (int)pmin[1], (int)pmin[2]};
+    int ipmax[3] = {(int)pmax[0], (int)pmax[1], (int)pmax[2]};
+    int depth = -1, rootp[3] = {0, 0, 0};
+    for (int lev = 0; lev < v->levs; lev++) {
+        int64_t rg = v->range[lev];
+        int64_t a0[3], a1[3];
+        for (int a = 0; a < 3; a++) {
+            a0[a] = ((int64_t)ipmin[a] / rg) * rg;
+            if (ipmin[a] < a0[a]) a0[a] -= rg;
+            a1[a] = ((int64_t)ipmax[a] / rg) * rg;
+            if (ipmax[a] < a1[a]) a1[a] -= rg;
+        }
+        if (a0[0] == a1[0] && a0[1] == a1[1] && a0[2] == a1[2]) {
+            depth = lev;
+            rootp[0] = (int)a0[0]; rootp[1] = (int)a0[1]; rootp[2] = (int)a0[2];
+            break;
+        }
+    }
+    if (depth <= 0 || depth >= v->levs) return OR_ERR_DEPTH;  /* VG.cpp:1187-1194 */
+    /* u16 key range (PT.cuh:377-380) */
+    for (int a = 0; a < 3; a++)
+        if ((int64_t)ipmax[a] / v->range[0] > 65535) return OR_ERR_RANGE;
+
+    memcpy(v->pos_min, pmin, sizeof(pmin));
+    memcpy(v->pos_max, pmax, sizeof(pmax));
+    /* Clear VG.cpp:2487-2503 */
+    for (int l = 0; l < OR_MAXLEV; l++) { v->ncnt[l] = 0; v->ccnt[l] = 0; }
+    free_sc(v);
+    v->depth = depth;
+    memcpy(v->root_pos, rootp, sizeof(rootp));
+
+    /* ---- phase 1: ActivateBricksGPU VG.cpp:1229-1413 ---- */
+    uint64_t* keys = (uint64_t*)malloc((size_t)(nvalid * depth + 1) * sizeof(uint64_t));
+    int64_t nk = 0;
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), orig, w)) continue;
+        for (int lev = 0; lev < depth; lev++) {   /* gvdbCalcBrickId PT.cuh:352-382 */
+            int rg = (int)v->range[lev];
+            int bx = cover_axis(w[0], rg), by = cover_axis(w[1], rg), bz = cover_axis(w[2], rg);
+            keys[nk++] = make_key(lev, bx / rg, by / rg, bz / rg);
+        }
+    }
+    qsort(keys, (size_t)nk, sizeof(uint64_t), cmp_u64);
+    int64_t nu = unique_u64(keys, nk);
+
+    int64_t root = pool_alloc_node(v, depth);
+    int64_t rcl = pool_alloc_clist(v, depth);
+    setup_node(v, depth, root, rootp[0], rootp[1], rootp[2]);
+    v->nodes[depth][root].mFlags = 1;
+    v->nodes[depth][root].mChildList = elem(1, depth, rcl);
+    alloc_from_keys(v, keys, nu);
+    compute_top_lev(v);
+    for (int lev = depth - 1; lev >= 0; lev--) link_level(v, lev);
+
+    /* ---- phase 2: ActivateExtraBricksGPU VG.cpp:931-1016 ---- */
+    if (radius >= 1) {
+        int64_t cap = 1024, ne = 0;
+        uint64_t* ek = (uint64_t*)malloc((size_t)cap * sizeof(uint64_t));
+        int64_t rtop = v->range[depth];
+        for (int64_t i = 0; i < n; i++) {
+            float w[3];
+            if (!point_valid(pt(pos, i, stride), orig, w)) continue;
+            /* gvdbCalcExtraBrickId PT.cuh:467-512 */
+            int bmin[3], bmax[3];
+            for (int a = 0; a < 3; a++) {
+                bmin[a] = (int)((float)(int)w[a] - radius);
+                bmax[a] = (int)((float)(int)w[a] + 2.0f * radius);
+            }
+            for (int lev = 0; lev < depth; lev++) {
+                int rres = (int)v->range[lev];
+                int nd[3];
+                for (nd[0] = bmin[0] / rres * rres; nd[0] <= bmax[0] / rres * rres; nd[0] += rres)
+                    for (nd[1] = bmin[1] / rres * rres; nd[1] <= bmax[1] / rres * rres; nd[1] += rres)
+                        for (nd[2] = bmin[2] / rres * rres; nd[2] <= bmax[2] / rres * rres; nd[2] += rres) {
+                            int inside = 1;   /* D3: clip to the root node */
+                            for (int a = 0; a < 3; a++)
+                                if (nd[a] < rootp[a] || (int64_t)nd[a] >= rootp[a] + rtop) inside = 0;
+                            if (!inside) continue;
+                            uint64_t k = make_key(lev, nd[0] / rres, nd[1] / rres, nd[2] / rres);
+                            if (key_in(keys, nu, k)) continue;   /* D3: getPosParent != UNDEF */
+                            if (ne >= cap) {
+                                cap *= 2;
+                                ek = (uint64_t*)realloc(ek, (size_t)cap * sizeof(uint64_t));
+                            }
+                            ek[ne++] = k;
+                        }
+            }
+        }
+        qsort(ek, (size_t)ne, sizeof(uint64_t), cmp_u64);
+        int64_t neu = unique_u64(ek, ne);
+        alloc_from_keys(v, ek, neu);
+        compute_top_lev(v);
+        for (int lev = depth - 1; lev >= 0; lev--) link_level(v, lev);
+        free(ek);
+    }
+    free(keys);
+
+    /* FinishTopology(false,true) -> ComputeBounds VG.cpp:1792-1816 */
+    int64_t rg0 = v->range[0];
+    or_node* c0 = &v->nodes[0][0];
+    float vmin[3] = {(float)c0->mPos[0], (float)c0->mPos[1], (float)c0->mPos[2]};
+    float vmax[3] = {vmin[0], vmin[1], vmin[2]};
+    for (int64_t i = 0; i < v->ncnt[0]; i++) {
+        or_node* c = &v->nodes[0][i];
+        if (!c->mFlags) continue;
+        for (int a = 0; a < 3; a++) {
+            if (c->mPos[a] < vmin[a]) vmin[a] = (float)c->mPos[a];
+            if (c->mPos[a] + rg0 > vmax[a]) vmax[a] = (float)(c->mPos[a] + rg0);
+        }
+    }
+    memcpy(v->obj_min, vmin, sizeof(vmin));
+    memcpy(v->obj_max, vmax, sizeof(vmax));
+    v->atlas_used = 0;
+    return OR_OK;
+}
+
+int or_update_atlas(or_volume* v) {
+    if (!v || !v->configured) return OR_ERR_ARG;
+    int any = 0;
+    for (int c = 0; c < OR_MAXCHAN; c++) any |= v->has_chan[c];
+    if (!any) return OR_ERR_STATE;
+    int64_t used = v->ncnt[0];
+    int64_t amax = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+    if (used > amax) {   /* VG.cpp:2645-2651, AL.cpp:622-650 */
+        int sz = (int)ceil(used / (float)(v->subdim[0] * v->subdim[1]));
+        if (sz > v->subdim[2]) atlas_resize(v, sz);
+    }
+    int br = brick_res(v);
+    int sx = v->subdim[0], sy = v->subdim[1];
+    int64_t id = 0;
+    for (int64_t k = 0; k < v->ncnt[0]; k++) {   /* AtlasAlloc/getAtlasPos AL.cpp:692-715 */
+        or_node* nd = &v->nodes[0][k];
+        if (!nd->mFlags) continue;
+        int64_t a2 = (int64_t)sx * sy;
+        int64_t z = id / a2, rem = id - z * a2;
+        int64_t y = rem / sx, x = rem - y * sx;
+        nd->mValue[0] = (int)(x * br + v->apron);
+        nd->mValue[1] = (int)(y * br + v->apron);
+        nd->mValue[2] = (int)(z * br + v->apron);
+        id++;
+    }
+    v->atlas_used = id;
+    /* AllocateAtlasMap + ClearMapping + AssignMapping VG.cpp:2559-2588,2676-2689 */
+    int64_t mc = (int64_t)sx * sy * v->subdim[2];
+    free(v->amap);
+    v->amap = (or_atlas_node*)malloc((size_t)mc * sizeof(or_atlas_node));
+    v->amap_cnt = mc;
+    for (int64_t i = 0; i < mc; i++) {
+        v->amap[i].mPos[0] = v->amap[i].mPos[1] = v->amap[i].mPos[2] = (int32_t)OR_ID_UNDEFL;
+        v->amap[i].mLeafID = (int32_t)OR_ID_UNDEFL;
+    }
+    for (int64_t k = 0; k < v->ncnt[0]; k++) {
+        or_node* nd = &v->nodes[0][k];
+        if (!nd->mFlags) continue;
+        int64_t ix = nd->mValue[0] / br, iy = nd->mValue[1] / br, iz = nd->mValue[2] / br;
+        int64_t slot = (iz * sy + iy) * sx + ix;
+        memcpy(v->amap[slot].mPos, nd->mPos, sizeof(nd->mPos));
+        v->amap[slot].mLeafID = (int32_t)k;
+    }
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* tree lookups                                                        */
+/* ------------------------------------------------------------------ */
+
+/* getPosLeafParent ND.cuh:255-288 + D4 containment */
+static int64_t pos_leaf(const or_volume* v, const int pos[3]) {
+    if (v->ncnt[v->top_lev] < 1) return -1;
+    const or_node* nd = &v->nodes[v->top_lev][0];
+    int l = nd->mLev;
+    int64_t id = -1;
+    if (l == 0) {
+        id = 0;
+    } else {
+        while (1) {
+            int res = v->res[l];
+            int64_t pin[3];
+            pos_in_node(pos, nd->mPos, res, v->range[l], pin);
+            if (pin[0] > res || pin[1] > res || pin[2] > res) return -1;
+            int64_t bit = (pin[2] * res + pin[1]) * res + pin[0];
+            int64_t c = get_child(v, nd, bit);
+            if (c < 0) return -1;
+            l--;
+            if (c >= v->ncnt[l]) return -1;
+            if (l == 0) { id = c; break; }
+            nd = &v->nodes[l][c];
+        }
+    }
+    const or_node* lf = &v->nodes[0][id];
+    for (int a = 0; a < 3; a++)
+        if (pos[a] < lf->mPos[a] || (int64_t)pos[a] >= lf->mPos[a] + v->res[0]) return -1;
+    return id;
+}
+
+/* getNodeAtPoint ND.cuh:199-226,243-253 (float descent) + D4 */
+static int64_t point_leaf(const or_volume* v, const float pos[3]) {
+    int lev = v->top_lev;
+    if (v->ncnt[lev] < 1) return -1;
+    const or_node* node = &v->nodes[lev][0];
+    int64_t id = 0;
+    float vmin[3] = {(float)node->mPos[0], (float)node->mPos[1], (float)node->mPos[2]};
+    while (lev > 0) {
+        float rg = (float)v->range[lev];
+        if (pos[0] < vmin[0] || pos[1] < vmin[1] || pos[2] < vmin[2] ||
+            pos[0] >= vmin[0] + rg || pos[1] >= vmin[1] + rg || pos[2] >= vmin[2] + rg)
+            return -1;
+        float vdel = (float)v->range[lev] / (float)v->res[lev];
+        int p0 = (int)((pos[0] - vmin[0]) / vdel), p1 = (int)((pos[1] - vmin[1]) / vdel),
+            p2 = (int)((pos[2] - vmin[2]) / vdel);
+        int64_t b = (((int64_t)p2 << v->logdim[lev]) + p1) << v->logdim[lev];
+        b += p0;
+        lev--;
+        int64_t c = get_child(v, node, b);
+        if (c < 0 || c >= v->ncnt[lev]) return -1;
+        id = c;
+        node = &v->nodes[lev][c];
+        vmin[0] = (float)node->mPos[0]; vmin[1] = (float)node->mPos[1]; vmin[2] = (float)node->mPos[2];
+    }
+    float r0 = (float)v->res[0];
+    for (int a = 0; a < 3; a++)
+        if (!(pos[a] >= vmin[a] && pos[a] < vmin[a] + r0)) return -1;
+    return id;
+}
+
+static int64_t atlas_index(const or_volume* v, int64_t x, int64_t y, int64_t z) {
+    int br = brick_res(v);
+    int64_t ax = (int64_t)v->subdim[0] * br, ay = (int64_t)v->subdim[1] * br;
+    return (z * ay + y) * ax + x;
+}
+
+/* ------------------------------------------------------------------ */
+/* points -> subcells -> gather                                        */
+/* ------------------------------------------------------------------ */
+
+int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride,
+                             float radius, const float trans[3], int64_t* out_len) {
+    if (!v || !v->configured || sc <= 0 || (!pos && n > 0)) return OR_ERR_ARG;
+    if (stride == 0) stride = 12;
+    int64_t nb = v->ncnt[0];
+    if (nb == 0) return OR_ERR_STATE;   /* VG.cpp:4973-4977 */
+    if (v->res[0] % sc) return OR_ERR_ARG;
+    free_sc(v);
+    int scdim = v->res[0] / sc;
+    int per = scdim * scdim * scdim;
+    v->sc_size = sc; v->sc_dim = scdim; v->sc_per_brick = per;
+    /* gvdbSetFlagSubcell PT.cuh:109-118 + PrefixSum -> sc_mapping */
+    int32_t* mapping = (int32_t*)malloc((size_t)nb * sizeof(int32_t));
+    int32_t acc = 0;
+    for (int64_t i = 0; i < nb; i++) {
+        mapping[i] = acc;
+        acc += (v->nodes[0][i].mParent == OR_ID_UNDEF64) ? 0 : 1;
+    }
+    int64_t nsc = (int64_t)per * nb;
+    v->num_sc = nsc;
+    v->sc_cnt = (int32_t*)calloc((size_t)nsc, sizeof(int32_t));
+    v->sc_off = (int64_t*)calloc((size_t)nsc + 1, sizeof(int64_t));
+    /* two passes: gvdbCountSubcell PT.cuh:275-315 then gvdbInsertSubcell PT.cuh:218-273 */
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            int64_t run = 0;
+            for (int64_t s = 0; s < nsc; s++) { v->sc_off[s] = run; run += v->sc_cnt[s]; }
+            v->sc_off[nsc] = run;
+            v->sc_len = run;
+            v->sc_pnt = (float*)malloc((size_t)(run * 3 + 3) * sizeof(float));
+            memset(v->sc_cnt, 0, (size_t)nsc * sizeof(int32_t));
+        }
+        for (int64_t i = 0; i < n; i++) {
+            float w[3];
+            if (!point_valid(pt(pos, i, stride), trans, w)) continue;
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                lo[a] = ((int)(w[a] - radius) / sc) * sc;
+                hi[a] = ((int)(w[a] + 1 + radius) / sc) * sc;
+            }
+            int s3[3];
+            for (s3[0] = lo[0]; s3[0] <= hi[0]; s3[0] += sc)
+                for (s3[1] = lo[1]; s3[1] <= hi[1]; s3[1] += sc)
+                    for (s3[2] = lo[2]; s3[2] <= hi[2]; s3[2] += sc) {
+                        int64_t leaf = pos_leaf(v, s3);
+                        if (leaf < 0) continue;
+                        const or_node* lf = &v->nodes[0][leaf];
+                        int px = (s3[0] - lf->mPos[0]) / sc, py = (s3[1] - lf->mPos[1]) / sc,
+                            pz = (s3[2] - lf->mPos[2]) / sc;
+                        int loc = (pz * scdim + py) * scdim + px;
+                        int64_t sidx = (int64_t)mapping[leaf] * per + loc;
+                        if (sidx < 0 || sidx >= nsc) continue;
+                        if (pass == 1) {
+                            int64_t e = v->sc_off[sidx] + v->sc_cnt[sidx];
+                            v->sc_pnt[e * 3 + 0] = w[0];
+                            v->sc_pnt[e * 3 + 1] = w[1];
+                            v->sc_pnt[e * 3 + 2] = w[2];
+                        }
+                        v->sc_cnt[sidx]++;
+                    }
+        }
+    }
+    /* gvdbCalcSubcellPos PT.cuh:677-694 */
+    v->sc_nid = (int32_t*)malloc((size_t)nsc * sizeof(int32_t));
+    v->sc_pos = (int32_t*)malloc((size_t)nsc * 3 * sizeof(int32_t));
+    for (int64_t s = 0; s < nsc; s++) v->sc_nid[s] = -1;
+    for (int64_t i = 0; i < nb; i++) {
+        const or_node* nd = &v->nodes[0][i];
+        if (nd->mParent == OR_ID_UNDEF64) continue;
+        for (int s = 0; s < per; s++) {
+            int64_t idx = (int64_t)mapping[i] * per + s;
+            v->sc_pos[idx * 3 + 0] = nd->mPos[0] + s % scdim * sc;
+            v->sc_pos[idx * 3 + 1] = nd->mPos[1] + (s / scdim) % scdim * sc;
+            v->sc_pos[idx * 3 + 2] = nd->mPos[2] + (s / scdim / scdim) % scdim * sc;
+            v->sc_nid[idx] = (int32_t)i;
+        }
+    }
+    free(mapping);
+    if (out_len) *out_len = v->sc_len;
+    return OR_OK;
+}
+
+static float dist2f(float dx, float dy, float dz) { return fmaf(dz, dz, fmaf(dy, dy, dx * dx)); }
+
+int or_gather(or_volume* v, int mode, float radius, int chan, int accum) {
+    if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (!v->sc_cnt || !v->sc_nid) return OR_ERR_STATE;
+    float* A = v->atlas[chan];
+    int sc = v->sc_size;
+    for (int64_t s = 0; s < v->num_sc; s++) {
+        int32_t nid = v->sc_nid[s];
+        if (nid < 0) continue;
+        const or_node* nd = &v->nodes[0][nid];
+        for (int tz = 0; tz < sc; tz++)
+            for (int ty = 0; ty < sc; ty++)
+                for (int tx = 0; tx < sc; tx++) {
+                    int wp[3] = {v->sc_pos[s * 3] + tx, v->sc_pos[s * 3 + 1] + ty, v->sc_pos[s * 3 + 2] + tz};
+                    int64_t vx = nd->mValue[0] + (wp[0] - nd->mPos[0]);
+                    int64_t vy = nd->mValue[1] + (wp[1] - nd->mPos[1]);
+                    int64_t vz = nd->mValue[2] + (wp[2] - nd->mPos[2]);
+                    int64_t ai = atlas_index(v, vx, vy, vz);
+                    float val;
+                    if (mode == 1) val = accum ? A[ai] : 3.0f;   /* PT.cuh:753-756 */
+                    else val = accum ? A[ai] : 0.0f;              /* PT.cuh:714-717 */
+                    for (int64_t j = v->sc_off[s]; j < v->sc_off[s] + v->sc_cnt[s]; j++) {
+                        float jx = v->sc_pnt[j * 3] - (float)wp[0];
+                        float jy = v->sc_pnt[j * 3 + 1] - (float)wp[1];
+                        float jz = v->sc_pnt[j * 3 + 2] - (float)wp[2];
+                        float c = sqrtf(dist2f(jx, jy, jz));
+                        if (mode == 1) val = fminf(val, c - radius);
+                        else val = fmaxf(val, radius - c);
+                    }
+                    A[ai] = val;
+                }
+    }
+    return OR_OK;
+}
+
+int or_gather_f64(or_volume* v, int mode, float radius, int accum, double* out) {
+    if (!v || !out) return OR_ERR_ARG;
+    if (!v->sc_cnt || !v->sc_nid) return OR_ERR_STATE;
+    if (accum) return OR_ERR_ARG;
+    int sc = v->sc_size;
+    int r0 = v->res[0];
+    for (int64_t s = 0; s < v->num_sc; s++) {
+        int32_t nid = v->sc_nid[s];
+        if (nid < 0) continue;
+        const or_node* nd = &v->nodes[0][nid];
+        for (int tz = 0; tz < sc; tz++)
+            for (int ty = 0; ty < sc; ty++)
+                for (int tx = 0; tx < sc; tx++) {
+                    int wp[3] = {v->sc_pos[s * 3] + tx, v->sc_pos[s * 3 + 1] + ty, v->sc_pos[s * 3 + 2] + tz};
+                    double val = mode == 1 ? 3.0 : 0.0;
+                    for (int64_t j = v->sc_off[s]; j < v->sc_off[s] + v->sc_cnt[s]; j++) {
+                        double jx = (double)v->sc_pnt[j * 3] - wp[0];
+                        double jy = (double)v->sc_pnt[j * 3 + 1] - wp[1];
+                        double jz = (double)v->sc_pnt[j * 3 + 2] - wp[2];
+                        double c = sqrt(jx * jx + jy * jy + jz * jz);
+                        if (mode == 1) val = fmin(val, c - (double)radius);
+                        else val = fmax(val, (double)radius - c);
+                    }
+                    int lx = wp[0] - nd->mPos[0], ly = wp[1] - nd->mPos[1], lz = wp[2] - nd->mPos[2];
+                    out[(int64_t)nid * r0 * r0 * r0 + ((int64_t)lz * r0 + ly) * r0 + lx] = val;
+                }
+    }
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* apron                                                               */
+/* ------------------------------------------------------------------ */
+
+int or_update_apron(or_volume* v, int chan, float boundval) {
+    if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (v->apron == 0) return OR_OK;        /* VG.cpp:4420 */
+    if (v->apron != 1) return OR_ERR_ARG;   /* OP.cuh kernels assume apron 1 (VG.cpp:4450) */
+    if (!v->amap) return OR_ERR_STATE;
+    float* A = v->atlas[chan];
+    int br = brick_res(v);
+    int sx = v->subdim[0], sy = v->subdim[1];
+    for (int64_t b = 0; b < v->ncnt[0]; b++) {   /* UpdateApron<T> OP.cuh:73-126 */
+        const or_node* node = &v->nodes[0][b];
+        for (int side = 0; side < 6; side++)
+            for (int su = 0; su < br; su++)
+                for (int sv = 0; sv < br; sv++) {
+                    int bv[3];
+                    switch (side) {
+                        case 0: bv[0] = 0; bv[1] = su; bv[2] = sv; break;
+                        case 1: bv[0] = su; bv[1] = 0; bv[2] = sv; break;
+                        case 2: bv[0] = su; bv[1] = sv; bv[2] = 0; break;
+                        case 3: bv[0] = br - 1; bv[1] = su; bv[2] = sv; break;
+                        case 4: bv[0] = su; bv[1] = br - 1; bv[2] = sv; break;
+                        default: bv[0] = su; bv[1] = sv; bv[2] = br - 1; break;
+                    }
+                    int64_t av[3];
+                    for (int a = 0; a < 3; a++) av[a] = (int64_t)bv[a] + node->mValue[a] - 1;
+                    if (av[0] < 0 || av[1] < 0 || av[2] < 0) continue;
+                    /* getAtlasToWorld ND.cuh:145-153 */
+                    int64_t bx = av[0] / br, by = av[1] / br, bz = av[2] / br;
+                    int64_t bid = (bz * sy + by) * sx + bx;
+                    if (bid >= v->amap_cnt || v->amap[bid].mLeafID == (int32_t)OR_ID_UNDEFL) continue;
+                    float wpos[3];
+                    for (int a = 0; a < 3; a++)
+                        wpos[a] = (float)v->amap[bid].mPos[a] + (float)(int)(av[a] % br - v->apron) + 0.5f;
+                    int64_t leaf = point_leaf(v, wpos);
+                    float value;
+                    if (leaf < 0) {
+                        value = boundval;
+                    } else {
+                        const or_node* ln = &v->nodes[0][leaf];
+                        float ox = (float)ln->mValue[0] + (wpos[0] - (float)ln->mPos[0]);
+                        float oy = (float)ln->mValue[1] + (wpos[1] - (float)ln->mPos[1]);
+                        float oz = (float)ln->mValue[2] + (wpos[2] - (float)ln->mPos[2]);
+                        value = A[atlas_index(v, (int64_t)(uint32_t)ox, (int64_t)(uint32_t)oy, (int64_t)(uint32_t)oz)];
+                    }
+                    A[atlas_index(v, av[0], av[1], av[2])] = value;
+                }
+    }
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* scatter path                                                        */
+/* ------------------------------------------------------------------ */
+
+int or_insert_points(or_volume* v, const float* pos, int64_t n, int64_t stride, const float trans[3],
+                     int32_t* pnode, int32_t* poff) {
+    if (!v || (!pos && n > 0) || !pnode || !poff) return OR_ERR_ARG;
+    if (stride == 0) stride = 12;
+    int64_t nb = v->ncnt[0];
+    int32_t* gcnt = (int32_t*)calloc((size_t)(nb + 1), sizeof(int32_t));
+    for (int64_t i = 0; i < n; i++) {   /* gvdbInsertPoints PT.cuh:19-36 */
+        float w[3];
+        pnode[i] = (int32_t)OR_ID_UNDEFL;
+        poff[i] = 0;
+        if (!point_valid(pt(pos, i, stride), trans, w)) continue;
+        int64_t leaf = point_leaf(v, w);
+        if (leaf < 0) continue;
+        pnode[i] = (int32_t)leaf;
+        poff[i] = gcnt[leaf]++;
+    }
+    free(gcnt);
+    return OR_OK;
+}
+
+/* distFunc PT.cuh:516-523 */
+static float dist_func(const float a[3], float bx, float by, float bz, float r) {
+    bx -= a[0]; by -= a[1]; bz -= a[2];
+    float c = dist2f(bx, by, bz) / (r * r);
+    float t = 3.0f - c;
+    t = c * t;
+    t = -3.0f + t;
+    t = c * t;
+    return 1.0f + t;
+}
+
+int or_scatter_density(or_volume* v, const float* pos, int64_t n, int64_t stride, float radius, float amp,
+                       const float trans[3], int expand, int chan) {
+    (void)amp;   /* unused by the reference kernel (PT.cuh:525-573) */
+    if (!v || (!pos && n > 0) || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (stride == 0) stride = 12;
+    float* A = v->atlas[chan];
+    int r0 = v->res[0];
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), trans, w)) continue;
+        int64_t leaf = point_leaf(v, w);   /* pnode from InsertPoints */
+        if (leaf < 0) continue;
+        const or_node* nd = &v->nodes[0][leaf];
+        float p[3] = {w[0] - (float)nd->mPos[0], w[1] - (float)nd->mPos[1], w[2] - (float)nd->mPos[2]};
+        float pi[3] = {(float)(int)p[0], (float)(int)p[1], (float)(int)p[2]};
+        if (pi[0] < 0 || pi[1] < 0 || pi[2] < 0 || pi[0] >= r0 || pi[1] >= r0 || pi[2] >= r0) continue;
+        int64_t q[3] = {(int64_t)pi[0] + nd->mValue[0], (int64_t)pi[1] + nd->mValue[1], (int64_t)pi[2] + nd->mValue[2]};
+        static const int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        int cnt = expand ? 7 : 1;
+        for (int k = 0; k < cnt; k++) {
+            int64_t ai = atlas_index(v, q[0] + off[k][0], q[1] + off[k][1], q[2] + off[k][2]);
+            A[ai] = A[ai] + dist_func(p, pi[0] + off[k][0], pi[1] + off[k][1], pi[2] + off[k][2], radius);
+        }
+    }
+    return OR_OK;
+}
+
+/* Leaf lookup for an integer voxel coordinate. */
+static int64_t voxel_leaf(const or_volume* v, int64_t x, int64_t y, int64_t z) {
+    float c[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+    return point_leaf(v, c);
+}
+
+/* Weighted cubic splat, extension mode (iii): rho(v) = sum_p w_p max(0,1-|v-p|^2/r^2)^3
+ * over integer voxel coordinates v of active leaves, summed in ascending point order. */
+int or_splat_cubic(or_volume* v, const float* pos, int64_t n, int64_t stride, const float* weights,
+                   float radius, const float trans[3], int chan, int accum) {
+    if (!v || (!pos && n > 0) || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (!v->amap) return OR_ERR_STATE;
+    if (stride == 0) stride = 12;
+    float* A = v->atlas[chan];
+    int r0 = v->res[0];
+    if (!accum) {
+        for (int64_t b = 0; b < v->ncnt[0]; b++) {
+            const or_node* nd = &v->nodes[0][b];
+            for (int z = 0; z < r0; z++)
+                for (int y = 0; y < r0; y++)
+                    for (int x = 0; x < r0; x++)
+                        A[atlas_index(v, nd->mValue[0] + x, nd->mValue[1] + y, nd->mValue[2] + z)] = 0.0f;
+        }
+    }
+    float r2 = radius * radius;
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), trans, w)) continue;
+        float wt = weights ? weights[i] : 1.0f;
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)floorf(w[a] - radius);
+            hi[a] = (int64_t)ceilf(w[a] + radius);
+            if (lo[a] < 0) lo[a] = 0;
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    float d2 = dist2f((float)x - w[0], (float)y - w[1], (float)z - w[2]);
+                    if (!(d2 < r2)) continue;
+                    int64_t leaf = voxel_leaf(v, x, y, z);
+                    if (leaf < 0) continue;
+                    const or_node* nd = &v->nodes[0][leaf];
+                    float t = 1.0f - d2 / r2;
+                    float k = (t * t) * t;
+                    int64_t ai = atlas_index(v, nd->mValue[0] + (x - nd->mPos[0]), nd->mValue[1] + (y - nd->mPos[1]),
+                                             nd->mValue[2] + (z - nd->mPos[2]));
+                    A[ai] = A[ai] + wt * k;
+                }
+    }
+    return OR_OK;
+}
+
+int or_splat_cubic_f64(or_volume* v, const float* pos, int64_t n, int64_t stride, const float* weights,
+                       float radius, const float trans[3], double* out) {
+    if (!v || (!pos && n > 0) || !out) return OR_ERR_ARG;
+    if (stride == 0) stride = 12;
+    int r0 = v->res[0];
+    int64_t nb = v->ncnt[0];
+    memset(out, 0, (size_t)(nb * r0 * r0 * r0) * sizeof(double));
+    double R = radius, R2 = R * R;
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), trans, w)) continue;
+        double wt = weights ? weights[i] : 1.0;
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)floor((double)w[a] - R);
+            hi[a] = (int64_t)ceil((double)w[a] + R);
+            if (lo[a] < 0) lo[a] = 0;
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    double dx = x - (double)w[0], dy = y - (double)w[1], dz = z - (double)w[2];
+                    double d2 = dx * dx + dy * dy + dz * dz;
+                    if (!(d2 < R2)) continue;
+                    int64_t leaf = voxel_leaf(v, x, y, z);
+                    if (leaf < 0) continue;
+                    const or_node* nd = &v->nodes[0][leaf];
+                    double t = 1.0 - d2 / R2;
+                    int64_t lx = x - nd->mPos[0], ly = y - nd->mPos[1], lz = z - nd->mPos[2];
+                    out[leaf * r0 * r0 * r0 + (lz * r0 + ly) * r0 + lx] += wt * t * t * t;
+                }
+    }
+    return OR_OK;
+}
+
+int or_fill_channel(or_volume* v, int chan, float val) {
+    if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    for (int64_t i = 0; i < v->atlas_vox; i++) v->atlas[chan][i] = val;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* readback                                                            */
+/* ------------------------------------------------------------------ */
+
+int or_get_info(or_volume* v, or_info* out) {
+    if (!v || !out) return OR_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    out->levs = v->levs;
+    out->depth = v->depth;
+    out->top_lev = v->top_lev;
+    memcpy(out->root_pos, v->root_pos, sizeof(v->root_pos));
+    for (int l = 0; l < OR_MAXLEV; l++) out->node_cnt[l] = v->ncnt[l];
+    memcpy(out->subdim, v->subdim, sizeof(v->subdim));
+    out->brick_res = brick_res(v);
+    out->apron = v->apron;
+    memcpy(out->pos_min, v->pos_min, sizeof(v->pos_min));
+    memcpy(out->pos_max, v->pos_max, sizeof(v->pos_max));
+    memcpy(out->obj_min, v->obj_min, sizeof(v->obj_min));
+    memcpy(out->obj_max, v->obj_max, sizeof(v->obj_max));
+    out->sc_len = v->sc_len;
+    return OR_OK;
+}
+
+const or_node* or_nodes(or_volume* v, int lev) { return (v && lev >= 0 && lev < OR_MAXLEV) ? v->nodes[lev] : NULL; }
+const uint64_t* or_child_lists(or_volume* v, int lev) { return (v && lev >= 0 && lev < OR_MAXLEV) ? v->clist[lev] : NULL; }
+const or_atlas_node* or_atlas_map(or_volume* v, int64_t* count) {
+    if (count) *count = v ? v->amap_cnt : 0;
+    return v ? v->amap : NULL;
+}
+const float* or_atlas(or_volume* v, int chan) { return (v && chan >= 0 && chan < OR_MAXCHAN) ? v->atlas[chan] : NULL; }
+
+int or_read_brick(or_volume* v, int chan, int64_t leaf, float* out) {
+    if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan] || leaf < 0 || leaf >= v->ncnt[0]) return OR_ERR_ARG;
+    int br = brick_res(v);
+    const or_node* nd = &v->nodes[0][leaf];
+    int64_t c0 = nd->mValue[0] - v->apron, c1 = nd->mValue[1] - v->apron, c2 = nd->mValue[2] - v->apron;
+    for (int z = 0; z < br; z++)
+        for (int y = 0; y < br; y++)
+            for (int x = 0; x < br; x++)
+                out[((int64_t)z * br + y) * br + x] = v->atlas[chan][atlas_index(v, c0 + x, c1 + y, c2 + z)];
+    return OR_OK;
+}
+
+int or_raycast_ea(or_volume* v, int chan, const or_render_params* rp, int64_t nrays, const float* origins,
+                  const float* dirs, float* out_rgba, double* out_f64) {
+    (void)v; (void)chan; (void)rp; (void)nrays; (void)origins; (void)dirs; (void)out_rgba; (void)out_f64;
+    return OR_ERR_STATE;
+}
