@@ -1,0 +1,184 @@
+"""Benchmark: points -> sparse volume on one MI355X (plain GPU workload).
+
+Workload (the reference's plain GPU configuration): 2^24 particles of a
+dam-break block, uniform in [64,576)x[64,320)x[64,576) with +-0.5 voxel
+jitter, seeded; ten frames advected by a seeded divergence-free ABC flow
+(each velocity component <= 2 voxels/frame).  One step = one frame of the
+hot path on resident positions:
+    RebuildTopology(N, 2r=3) -> FinishTopology -> UpdateAtlas
+    -> InsertPointsSubcell(4, N, 3) -> GatherLevelSet(4, N, r=1.5) -> UpdateApron(0, 3.0)
+The ten frames are generated and advected on the GPU before timing.
+
+Prints one JSON line (contract: metric/value/unit/.../roofline/cpu_baseline).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
+
+METRIC = "Mpoints/s topology+splat (HBM % of peak); fwd+bwd/fwd time ratio"
+HBM_PEAK = 8.0e12
+SEED = 20261203 + 1          # configs[1]
+
+
+def abc_velocity(torch, p, phases, amp=(1.0, 1.0, 1.0)):
+    k = 2.0 * math.pi / 128.0
+    A, B, Cc = amp
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    u = A * torch.sin(k * z + phases[0]) + Cc * torch.cos(k * y + phases[1])
+    v = B * torch.sin(k * x + phases[2]) + A * torch.cos(k * z + phases[3])
+    w = Cc * torch.sin(k * y + phases[4]) + B * torch.cos(k * x + phases[5])
+    return torch.stack([u, v, w], 1)
+
+
+def make_frames(torch, gvdbx, n, nframes, dev):
+    import numpy as np
+    pts = gvdbx.gen_box(SEED, n, (64, 64, 64), (576, 320, 576), 0.5)
+    rng = np.random.default_rng(SEED)
+    phases = [float(x) for x in rng.uniform(0, 2 * math.pi, 6)]
+    frames = [torch.from_numpy(pts).to(dev)]
+    for _ in range(nframes - 1):
+        p = frames[-1]
+        frames.append((p + abc_velocity(torch, p, phases)).contiguous())
+    return frames, pts
+
+
+def frame_bytes_alg(n, nleaf, node_cnt, res, depth, br=10, r0=8, passes=3):
+    """Algorithmic bytes of one frame (N points, B leaves), per the measurement plan:
+    N*(24 + 4 + 16P) + B*(4*br^3 + 4*(br^3 - r0^3) + 64 + 16) + sum_internal(64 + 8*res_l^3)."""
+    b = n * (24 + 4 + 16 * passes)
+    b += nleaf * (4 * br ** 3 + 4 * (br ** 3 - r0 ** 3) + 64 + 16)
+    for l in range(1, depth + 1):
+        b += node_cnt[l] * (64 + 8 * res[l] ** 3)
+    return b
+
+
+def cpu_baseline(pts):
+    """The CPU oracle (single thread) on a full-density slab of frame 0 (x < 96: 1/16 of the points)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sub = pts[pts[:, 0] < 96.0]
+    ov = oracle.OracleVolume((3, 3, 3, 3, 5))
+    ov.add_channel(0, 1)
+    t0 = time.perf_counter()
+    ov.rebuild_topology(sub, 3.0)
+    ov.update_atlas()
+    ov.insert_points_subcell(4, sub, 3.0)
+    ov.gather_level_set(1.5)
+    ov.update_apron(0, 3.0)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(sub) / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+            "sample": f"frame 0 slab x<96 ({len(sub)} points, full density), topology+subcell insert+level set+apron,"
+                      f" {dt:.1f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=1 << 24)
+    ap.add_argument("--frames", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stages", action="store_true", help="also print per-stage times to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the path has no multi-GPU form here); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gvdbx
+
+    dev = torch.device("cuda:0")
+    n = args.points
+    frames, pts0 = make_frames(torch, gvdbx, n, args.frames, dev)
+    stream = torch.cuda.current_stream()
+    gv = gvdbx.VolumeGVDB(0, stream=stream)
+    gv.Configure(5, 3, 3, 3, 3)
+    gv.SetChannelDefault(16, 16, 1)
+    gv.AddChannel(0, gvdbx.T_FLOAT, 1)
+    torch.cuda.synchronize()
+
+    ev = {}
+
+    def mark(name):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        ev.setdefault(name, []).append(e)
+
+    def step(i, timed):
+        p = frames[i % len(frames)]
+        gv.SetPoints(p)
+        if timed:
+            mark("a")
+        gv.RebuildTopology(n, 3.0, (0.0, 0.0, 0.0))
+        gv.FinishTopology(False, True)
+        gv.UpdateAtlas()
+        if timed:
+            mark("b")
+        ln = gv.InsertPointsSubcell(4, n, 3.0, (0.0, 0.0, 0.0))
+        if timed:
+            mark("c")
+        gv.GatherLevelSet(4, n, 1.5, (0.0, 0.0, 0.0), ln, 0)
+        if timed:
+            mark("d")
+        gv.UpdateApron(0, 3.0)
+        if timed:
+            mark("e")
+        return ln
+
+    for i in range(args.warmup):
+        step(i, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(args.warmup + i, True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+
+    def avg(a, b):
+        return sum(x.elapsed_time(y) for x, y in zip(ev[a], ev[b])) / len(ev[a])
+
+    stage = {"topology_ms": avg("a", "b"), "insert_subcell_ms": avg("b", "c"), "gather_levelset_ms": avg("c", "d"),
+             "apron_ms": avg("d", "e")}
+    inf = gv.info()
+    nleaf = inf.node_cnt[0]
+    res = [1 << d for d in gv.log2dims]
+    binned = n   # every valid point is binned once
+    gather_bytes = 16 * binned + 4 * 512 * nleaf + 8 * nleaf
+    gather_s = stage["gather_levelset_ms"] / 1e3
+    achieved = gather_bytes / gather_s
+    fb = frame_bytes_alg(n, nleaf, list(inf.node_cnt), res, inf.depth)
+    value = n / (ms / 1e3) / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "dam-break level set: 2^24 points in [64,576)x[64,320)x[64,576) +-0.5 jitter, "
+                               "tree {3,3,3,3,5} (8^3 bricks, apron 1), r=1.5 (topology/subcell radius 3.0), "
+                               "UpdateApron(3.0), 10 ABC-advected frames, full rebuild per frame",
+                   "n_points": n, "frames": args.frames, "leaf_bricks": int(nleaf), "depth": inf.depth,
+                   "sc_len": int(inf.sc_len)},
+        "roofline": {"bound": "hbm", "kernel": "k_gather (GatherLevelSet)", "achieved": round(achieved / 1e9, 2),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
+                     "traffic": None, "alg_bytes_per_launch": gather_bytes,
+                     "avg_launch_ms": round(stage["gather_levelset_ms"], 4)},
+        "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
+                           "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
+        "stages_ms": {k: round(v, 4) for k, v in stage.items()},
+        "fwd_bwd_over_fwd": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pts0)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,274 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Topology (node records, child lists, atlas map, counts, top level, bounds),
+subcell list length, level-set / density gathers, apron, InsertPoints and
+ScatterDensity are compared bit-exactly.  The cubic splat sums in a fixed
+order that differs from the oracle's ascending point order, so it is checked
+to 1e-5 relative error against the FP64 evaluation (absolute floor
+1e-6 * max|rho|), and bit-exactly against itself across runs.
+"""
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _gvdbx():
+    import gvdbx
+    return gvdbx
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def build_pair(pts, radius, log2dims=(3, 3, 3, 3, 5), orig=(0.0, 0.0, 0.0)):
+    G = _gvdbx()
+    dev = _dev()
+    gv = G.VolumeGVDB(0)
+    gv.configure(log2dims)
+    gv.SetChannelDefault(16, 16, 1)
+    gv.AddChannel(0, G.T_FLOAT, 1)
+    tp = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32)).to(dev)
+    gv.SetPoints(tp)
+    gv.RebuildTopology(len(pts), radius, orig)
+    gv.FinishTopology(False, True)
+    gv.UpdateAtlas()
+    ov = O.OracleVolume(log2dims)
+    ov.add_channel(0, 1)
+    ov.rebuild_topology(pts, radius, orig)
+    ov.update_atlas()
+    return gv, ov, tp
+
+
+def assert_same_topology(gv, ov):
+    gi, oi = gv.info(), ov.info()
+    assert gi.depth == oi.depth
+    assert gi.top_lev == oi.top_lev
+    assert list(gi.root_pos) == list(oi.root_pos)
+    assert list(gi.node_cnt) == list(oi.node_cnt)
+    assert list(gi.pos_min) == list(oi.pos_min) and list(gi.pos_max) == list(oi.pos_max)
+    assert list(gi.obj_min) == list(oi.obj_min) and list(gi.obj_max) == list(oi.obj_max)
+    assert list(gi.subdim) == list(oi.subdim)
+    for l in range(gi.depth + 1):
+        gn, on = gv.nodes(l), ov.nodes(l)
+        for f in ("mLev", "mFlags", "mPos", "mValue", "mParent", "mChildList"):
+            np.testing.assert_array_equal(gn[f], on[f], err_msg=f"level {l} field {f}")
+        if l >= 1:
+            np.testing.assert_array_equal(gv.child_lists(l), ov.child_lists(l), err_msg=f"child lists level {l}")
+    np.testing.assert_array_equal(gv.atlas_map(), ov.atlas_map())
+
+
+def _shell(rng, n, c, r0, r1):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rad = np.cbrt(rng.uniform(r0 ** 3, r1 ** 3, n))
+    return (c + d * rad[:, None]).astype(np.float32)
+
+
+TOPO_CASES = [
+    ("box_r3", lambda r: r.random((20000, 3)) * [120, 64, 100] + [30, 40, 20], 3.0, (3, 3, 3, 3, 5)),
+    ("box_r0.5", lambda r: r.random((20000, 3)) * [120, 64, 100] + [30, 40, 20], 0.5, (3, 3, 3, 3, 5)),
+    ("shell_r4", lambda r: _shell(r, 30000, 120, 80, 96), 4.0, (3, 3, 3, 3, 5)),
+    ("clusters_r2", lambda r: np.abs(np.concatenate([r.normal(c, 5.0, (4000, 3)) for c in
+                                                    ([60, 60, 60], [150, 90, 40], [40, 200, 120])])), 2.0,
+     (3, 3, 3, 3, 5)),
+    ("top_below_root", lambda r: r.random((3000, 3)) * 60.9 + 2, 0.5, (3, 3, 3, 3, 5)),
+    ("cfg2_shape_small", lambda r: r.random((60000, 3)) * [512, 256, 512] + 64 + (r.random((60000, 3)) - 0.5), 3.0,
+     (3, 3, 3, 3, 5)),
+    ("sample_cfg_32_bricks", lambda r: r.random((20000, 3)) * [150, 90, 120] + 3, 3.0, (5, 3, 3, 3, 3)),
+    ("small_bricks_4", lambda r: r.random((5000, 3)) * 30 + 2, 1.5, (2, 2, 2, 2, 3)),
+]
+
+
+@pytest.mark.parametrize("name,gen,radius,log2dims", TOPO_CASES, ids=[c[0] for c in TOPO_CASES])
+def test_topology_parity(name, gen, radius, log2dims):
+    _log(f"[gpu] topology {name}")
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    pts = np.asarray(gen(rng), dtype=np.float32)
+    gv, ov, _ = build_pair(pts, radius, log2dims)
+    assert_same_topology(gv, ov)
+
+
+def test_topology_invalid_points():
+    _log("[gpu] topology invalid points")
+    rng = np.random.default_rng(9)
+    pts = (rng.random((20000, 3)) * 100 + 10).astype(np.float32)
+    pts[::7, 2] = 1.0e10
+    pts[1::11, 0] = -3.0
+    pts[2::13, 1] = np.nan
+    gv, ov, _ = build_pair(pts, 3.0)
+    assert_same_topology(gv, ov)
+
+
+@pytest.mark.parametrize("mode,R,r,boundval", [(1, 3.0, 1.5, 3.0), (0, 4.0, 2.0, 0.0), (1, 2.0, 1.0, 3.0),
+                                               (0, 1.0, 2.0, 0.0)])
+def test_gather_and_apron_parity(mode, R, r, boundval):
+    _log(f"[gpu] gather mode={mode} R={R} r={r}")
+    rng = np.random.default_rng(int(R * 100 + r * 10 + mode))
+    pts = (rng.random((30000, 3)) * [90, 70, 80] + [20, 12, 30]).astype(np.float32)
+    gv, ov, tp = build_pair(pts, R)
+    gl = gv.InsertPointsSubcell(4, len(pts), R, (0, 0, 0))
+    ol = ov.insert_points_subcell(4, pts, R)
+    assert gl == ol
+    if mode == 1:
+        gv.GatherLevelSet(4, len(pts), r, (0, 0, 0), gl, 0)
+        ov.gather_level_set(r)
+    else:
+        gv.GatherDensity(4, len(pts), r, (0, 0, 0), gl, 0)
+        ov.gather_density(r)
+    gb, ob = gv.bricks(), ov.bricks()
+    np.testing.assert_array_equal(gb[:, 1:9, 1:9, 1:9], ob[:, 1:9, 1:9, 1:9])
+    gv.UpdateApron(0, boundval)
+    ov.update_apron(0, boundval)
+    np.testing.assert_array_equal(gv.bricks(), ov.bricks())
+
+
+def test_gather_accumulate_parity():
+    _log("[gpu] gather accumulate")
+    rng = np.random.default_rng(77)
+    pts = (rng.random((20000, 3)) * 80 + 20).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 3.0)
+    gl = gv.InsertPointsSubcell(4, len(pts), 3.0)
+    ov.insert_points_subcell(4, pts, 3.0)
+    gv.FillChannel(0, 2.5)
+    ov.fill_channel(0, 2.5)
+    gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), gl, 0, -1, True)
+    ov.gather_level_set(1.5, accum=True)
+    np.testing.assert_array_equal(gv.bricks()[:, 1:9, 1:9, 1:9], ov.bricks()[:, 1:9, 1:9, 1:9])
+
+
+def test_levelset_fp64_tolerance():
+    """Level set vs the FP64 evaluation: |err| <= 1e-5 * max(|ref|, r).
+
+    phi = |p-v| - r cancels near the surface, so the absolute floor is the
+    scale of the operands of that subtraction (r = 1.5), not of the result."""
+    _log("[gpu] level set fp64")
+    rng = np.random.default_rng(5)
+    pts = (rng.random((20000, 3)) * 60 + 20).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 3.0)
+    gl = gv.InsertPointsSubcell(4, len(pts), 3.0)
+    ov.insert_points_subcell(4, pts, 3.0)
+    gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), gl, 0)
+    ref = ov.gather_f64(1, 1.5)
+    got = gv.bricks()[:, 1:9, 1:9, 1:9].astype(np.float64)
+    bound = 1e-5 * np.maximum(np.abs(ref), 1.5)
+    err = np.abs(got - ref)
+    assert (err <= bound).all(), f"max err {err.max():.3e}, worst ratio {(err / bound).max():.3f}"
+
+
+def test_insert_points_parity():
+    _log("[gpu] insert points")
+    rng = np.random.default_rng(12)
+    pts = (rng.random((20000, 3)) * 70 + 9).astype(np.float32)
+    pts[::17, 2] = 1.0e10
+    gv, ov, tp = build_pair(pts, 0.5)
+    gv.InsertPoints(len(pts), (0, 0, 0))
+    gp, go = gv.point_nodes(len(pts))
+    op, oo = ov.insert_points(pts)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(go, oo)
+
+
+def test_scatter_density_parity():
+    _log("[gpu] scatter density")
+    rng = np.random.default_rng(13)
+    pts = (rng.random((20000, 3)) * 60 + 9).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 1.0)
+    gv.InsertPoints(len(pts))
+    gv.ScatterDensity(len(pts), 2.0, 1.0, (0, 0, 0), True)
+    ov.scatter_density(pts, 2.0, expand=True)
+    np.testing.assert_array_equal(gv.bricks(), ov.bricks())
+
+
+def test_splat_cubic_fp64():
+    _log("[gpu] cubic splat")
+    rng = np.random.default_rng(14)
+    pts = np.abs(np.concatenate([rng.normal(c, 6.0, (8000, 3)) for c in ([50, 50, 50], [90, 70, 40])])
+                 ).astype(np.float32)
+    w = rng.uniform(0.1, 1.0, len(pts)).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 4.0)
+    tw = torch.from_numpy(w).to(tp.device)
+    gv.SetPointWeights(tw)
+    gv.SplatCubic(len(pts), 2.0)
+    got = gv.bricks()[:, 1:9, 1:9, 1:9].astype(np.float64)
+    ref = ov.splat_cubic_f64(pts, 2.0, weights=w)
+    floor = 1e-6 * np.abs(ref).max()
+    err = np.abs(got - ref)
+    assert (err <= 1e-5 * np.abs(ref) + floor).all(), f"max err {err.max():.3e}"
+    ov.splat_cubic(pts, 2.0, weights=w)
+    np.testing.assert_allclose(got, ov.bricks()[:, 1:9, 1:9, 1:9], rtol=1e-5, atol=floor)
+    gv.SplatCubic(len(pts), 2.0)
+    np.testing.assert_array_equal(gv.bricks()[:, 1:9, 1:9, 1:9], got.astype(np.float32))
+
+
+def test_error_paths_leave_volume_unchanged():
+    _log("[gpu] error paths")
+    G = _gvdbx()
+    rng = np.random.default_rng(15)
+    pts = (rng.random((5000, 3)) * 60 + 9).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 3.0)
+    before = gv.nodes(0)
+    tiny = torch.tensor([[1.0, 1.0, 1.0], [2.0, 2.0, 2.0]], device=tp.device)
+    gv.SetPoints(tiny)
+    with pytest.raises(G.GvdbxError) as e:
+        gv.RebuildTopology(2, 3.0)
+    assert e.value.code == 3
+    bad = torch.full((4, 3), float("nan"), device=tp.device)
+    gv.SetPoints(bad)
+    with pytest.raises(G.GvdbxError) as e:
+        gv.RebuildTopology(4, 3.0)
+    assert e.value.code == 4
+    np.testing.assert_array_equal(gv.nodes(0), before)
+    with pytest.raises(G.GvdbxError):
+        gv.GatherLevelSet(4, 4, 1.5, (0, 0, 0), 0, 0)   # no InsertPointsSubcell yet
+    with pytest.raises(G.GvdbxError):
+        gv.InsertPointsSubcell(4, 4, 9.0)                # box radius beyond the brick halo
+
+
+def test_full_size_cfg2_properties():
+    """16M-point frame: determinism, level-set bounds, topology size invariants."""
+    _log("[gpu] full size config-2 frame")
+    G = _gvdbx()
+    dev = _dev()
+    n = 1 << 24
+    pts = G.gen_box(20261205, n, (64, 64, 64), (576, 320, 576), 0.5)
+    tp = torch.from_numpy(pts).to(dev)
+    gv = G.VolumeGVDB(0)
+    gv.configure((3, 3, 3, 3, 5))
+    gv.AddChannel(0, G.T_FLOAT, 1)
+    gv.SetPoints(tp)
+    outs = []
+    for rep in range(2):
+        gv.RebuildTopology(n, 3.0)
+        gv.FinishTopology()
+        gv.UpdateAtlas()
+        ln = gv.InsertPointsSubcell(4, n, 3.0)
+        gv.GatherLevelSet(4, n, 1.5, (0, 0, 0), ln, 0)
+        gv.UpdateApron(0, 3.0)
+        inf = gv.info()
+        nb = inf.node_cnt[0]
+        b = gv.bricks()
+        outs.append((ln, nb, float(b.astype(np.float64).sum()), float(b.min()), float(b.max())))
+        del b
+    assert outs[0] == outs[1]
+    ln, nb, s, mn, mx = outs[0]
+    assert inf.depth == 3
+    assert mx <= 3.0 and mn >= -1.5
+    assert 130000 < nb < 170000
+    # every leaf is the cell of some point or within the extra-brick box of one
+    assert inf.node_cnt[3] == 1
