@@ -85,7 +85,8 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 24)
     ap.add_argument("--frames", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--stages", action="store_true", help="also print per-stage times to stderr")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path has no multi-GPU form here); got --gpus {args.gpus}",
@@ -96,6 +97,14 @@ def main():
     import gvdbx
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4}))
+        return
     n = args.points
     frames, pts0 = make_frames(torch, gvdbx, n, args.frames, dev)
     stream = torch.cuda.current_stream()
