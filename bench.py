@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 24)
     ap.add_argument("--frames", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gather-variant", type=int, default=None,
+                    help="0: LDS-staged candidates, 1: ballot/readlane walk (default)")
+    ap.add_argument("--xcd-remap", type=int, default=None, help="1: contiguous brick runs per XCD (default 0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -112,6 +115,10 @@ def main():
     gv.Configure(5, 3, 3, 3, 3)
     gv.SetChannelDefault(16, 16, 1)
     gv.AddChannel(0, gvdbx.T_FLOAT, 1)
+    if args.gather_variant is not None:
+        gv.SetTuning("gather_variant", args.gather_variant)
+    if args.xcd_remap is not None:
+        gv.SetTuning("xcd_remap", args.xcd_remap)
     torch.cuda.synchronize()
 
     ev = {}
@@ -182,6 +189,7 @@ def main():
         "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
                            "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
+        "gather_variant": gv.GetTuning("gather_variant"), "xcd_remap": gv.GetTuning("xcd_remap"),
         "fwd_bwd_over_fwd": None,
     }
     if not args.no_cpu_baseline:

@@ -72,13 +72,30 @@ __device__ __forceinline__ bool gvdbx_point(const PointSrc& s, int64_t i, float 
     return wx >= 0.0f && wy >= 0.0f && wz >= 0.0f;
 }
 
-// Covering-node corner along one axis as the reference computes it
-// (GetCoveringNode, cuda_gvdb_particles.cuh:88-100).
-__device__ __forceinline__ int gvdbx_cover(float p, int range) {
-    float q = p / (float)range;
-    int np = (int)(ceilf(q) * (float)range);
-    if (p < (float)np) np -= range;
-    return np;
+// Cell index floor(p / 2^logr) of a valid (finite, non-negative) coordinate.
+// Equal to the reference's ceil-then-correct corner divided by the range;
+// the multiply by 2^-logr is exact, so no division is needed.
+__device__ __forceinline__ int gvdbx_cell(float p, int logr) {
+    return (int)floorf(p * __builtin_ldexpf(1.0f, -logr));
+}
+
+// C integer division (truncation toward zero) by 2^s without a divide.
+__device__ __forceinline__ int gvdbx_cdiv(int x, int s) { return x >= 0 ? (x >> s) : -((-x) >> s); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Remapping block b to item (b % 8) * per + b / 8 gives
+// every XCD a contiguous run of items (spatially adjacent bricks, which share
+// neighbour bins), so the data they share stays in one L2.  Launch with
+// gvdbx_xcd_blocks(n) blocks; items >= n return -1.
+constexpr int GVDBX_NUM_XCD = 8;
+__host__ __device__ __forceinline__ int64_t gvdbx_xcd_blocks(int64_t n) {
+    return (n + GVDBX_NUM_XCD - 1) / GVDBX_NUM_XCD * GVDBX_NUM_XCD;
+}
+__device__ __forceinline__ int64_t gvdbx_xcd_item(int64_t b, int64_t n, int remap = 1) {
+    if (!remap) return b < n ? b : -1;
+    int64_t per = (n + GVDBX_NUM_XCD - 1) / GVDBX_NUM_XCD;
+    int64_t t = (b % GVDBX_NUM_XCD) * per + b / GVDBX_NUM_XCD;
+    return t < n ? t : -1;
 }
 
 // Squared distance in the fixed operation order used everywhere.

@@ -17,6 +17,7 @@ void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccu
 struct ReachParams {
     CellGrid g0;
     int range0;
+    int logr0;
     int use_mask;      // 1: 3x3x3 reach mask per cell; 0: no extra bricks (radius < 1)
     float radius;
 };
@@ -77,6 +78,8 @@ void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t*
 struct BinParams {
     CellGrid gb;          // binning grid (leaf grid grown by one cell)
     int range0;
+    int logr0;
+    int sc_shift;
     // subcell list length (InsertPointsSubcell out-parameter)
     int count_len;
     int sc;
@@ -89,7 +92,7 @@ void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, Bin
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s);
 void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
                              hipStream_t s);
-void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int range0,
+void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0,
                         const int32_t* off, const int32_t* pslot, float4* sorted, hipStream_t s);
 void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s);
 // sort each bin's points by original index (stable per-cell order)
@@ -108,9 +111,12 @@ struct GatherParams {
     int mode;                 // 0 density (max), 1 level set (min)
     int accum;
     int sc;
+    int sc_shift;             // log2(sc): sc divides the power-of-two brick
     float box_radius;         // radius given to InsertPointsSubcell
     float radius;             // radius given to Gather*
     float cutoff;             // voxels farther than this cannot change a value
+    int variant;              // 0: LDS-staged candidates, 1: ballot/readlane walk
+    int xcd_remap;            // 1: give each XCD a contiguous run of tiles
 };
 void launch_gather(GatherParams gp, hipStream_t s);
 
@@ -136,6 +142,7 @@ struct ApronParams {
     CellGrid g0;
     const int32_t* idx0;
     int range0;
+    int logr0;
     float* chan;
     int brick_res;
     float boundval;
@@ -148,7 +155,7 @@ struct InsertParams {
     const int32_t* idx0;
     const int32_t* off;
     const float4* sorted;
-    int range0;
+    int logr0;
     int32_t* pnode;
     int32_t* poff;
     int64_t n;

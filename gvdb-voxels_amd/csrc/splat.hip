@@ -36,10 +36,11 @@ __device__ __forceinline__ void tile_origin(int res0, int64_t t, int& ox, int& o
 // |p-v|-r equals sqrt(min |p-v|^2) - r bit for bit, so the kernel keeps one
 // min of squared distance per voxel.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int tiles_per_brick) {
+__global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
     __shared__ float4 cand[CAND];
     const int lane = threadIdx.x;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
+    if (tile < 0) return;
     const int64_t k = tile / tiles_per_brick;
     if (k >= gp.nleaf) return;
     const int td = tile_dim(res0);
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
     }
     const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
     const float R = gp.box_radius;
-    const int sc = gp.sc;
+    const int sc = gp.sc, scs = gp.sc_shift;
     const float cut = gp.cutoff;
     int cnt = 0;
 
@@ -106,8 +107,8 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
 #pragma unroll
                         for (int a = 0; a < 3; a++) {
                             // subcell box (cuda_gvdb_particles.cuh:237-242) as a voxel range
-                            int lo = ((int)(pw[a] - R) / sc) * sc;
-                            int hi = ((int)(pw[a] + 1 + R) / sc) * sc + sc - 1;
+                            int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
+                            int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
                             lo = max(lo, (int)ceilf(pw[a] - cut));
                             hi = min(hi, (int)floorf(pw[a] + cut));
                             lo_r[a] = max(lo - T0[a], 0);
@@ -148,16 +149,174 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
 }
 
 // ---------------------------------------------------------------------------
+// Same semantics as k_gather without LDS: each 64-point chunk of a
+// neighbouring bin is filtered lane-parallel, then the surviving candidates
+// are walked one at a time with ballot + readlane (their coordinates land in
+// scalar registers), and the next chunk's load is in flight meanwhile.
+// ---------------------------------------------------------------------------
+struct ChunkIter {
+    int d = -1;       // neighbour index 0..26
+    int j0 = 0, end = 0;
+};
+
+__device__ __forceinline__ bool next_chunk(ChunkIter& it, const GatherParams& gp, int cx, int cy, int cz) {
+    if (it.j0 + 64 < it.end) {
+        it.j0 += 64;
+        return true;
+    }
+    while (++it.d < 27) {
+        int bx = cx + it.d / 9 - 1, by = cy + (it.d / 3) % 3 - 1, bz = cz + it.d % 3 - 1;
+        if (!gp.gb.contains(bx, by, bz)) continue;
+        int64_t ci = gp.gb.index(bx, by, bz);
+        int beg = gp.off[ci], end = gp.off[ci + 1];
+        if (beg < end) {
+            it.j0 = beg;
+            it.end = end;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <bool HALVES>
+__global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
+    const int lane = threadIdx.x;
+    const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
+    if (tile < 0) return;
+    const int64_t k = tile / tiles_per_brick;
+    if (k >= gp.nleaf) return;
+    const int td = tile_dim(res0);
+    int tox, toy, toz;
+    tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
+    const GNode nd = gp.leaves[k];
+    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    const int lx = lane % td, ly = lane / td;
+    const bool lane_ok = ly < td;
+    const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
+    float vz[8];
+    uint32_t m[8];   // bits of min |p-v|^2: non-negative floats order like their bit patterns
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        vz[z] = (float)(T0z + z);
+        m[z] = 0x7F800000u;
+    }
+    const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
+    const float R = gp.box_radius;
+    const int sc = gp.sc, scs = gp.sc_shift;
+    const float cut = gp.cutoff;
+    const float cut2 = cut * cut;
+
+    ChunkIter it;
+    bool have = next_chunk(it, gp, cx, cy, cz);
+    float4 cur = make_float4(0, 0, 0, 0);
+    if (have && it.j0 + lane < it.end) cur = gp.sorted[it.j0 + lane];
+    while (have) {
+        const int cend = it.end, cj = it.j0 + lane;
+        have = next_chunk(it, gp, cx, cy, cz);
+        float4 nxt = make_float4(0, 0, 0, 0);
+        if (have && it.j0 + lane < it.end) nxt = gp.sorted[it.j0 + lane];
+        // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
+        // as a voxel range; a candidate must also come within the cutoff of the tile.
+        // The packed range is the membership range clipped to the tile (z) and to
+        // membership + cutoff (x, y: lanes outside it are masked).
+        bool ok = cj < cend;
+        uint32_t pk = 0;
+        {
+            float pw[3] = {cur.x, cur.y, cur.z};
+            int T0[3] = {T0x, T0y, T0z};
+            float db2 = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
+                int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                int lo_c = max(lo, (int)ceilf(pw[a] - cut));
+                int hi_c = min(hi, (int)floorf(pw[a] + cut));
+                int lr = max(lo_c - T0[a], 0), hr = min(hi_c - T0[a], td - 1);
+                ok = ok && lr <= hr;
+                float e = fmaxf(fmaxf((float)(T0[a] + lr) - pw[a], pw[a] - (float)(T0[a] + hr)), 0.0f);
+                db2 = __builtin_fmaf(e, e, db2);
+                if (HALVES && a == 2) {
+                    // z: membership only (voxels beyond the cutoff cannot change the result)
+                    lr = max(lo - T0[a], 0);
+                    hr = min(hi - T0[a], td - 1);
+                }
+                pk |= ((uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3)) << (6 * a);
+            }
+            ok = ok && db2 <= cut2;
+        }
+        uint64_t mask = __ballot(ok);
+        while (mask) {
+            const int b = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.x), b));
+            const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), b));
+            const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), b));
+            const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pk, b);
+            const int xlo = q & 7, xhi = (q >> 3) & 7, ylo = (q >> 6) & 7, yhi = (q >> 9) & 7;
+            const int zlo = (q >> 12) & 7, zhi = (q >> 15) & 7;
+            if (lane_ok && lx >= xlo && lx <= xhi && ly >= ylo && ly <= yhi) {
+                const float dx = vx - px, dy = vy - py;
+                const float dxy = __builtin_fmaf(dy, dy, dx * dx);
+                if (dxy <= cut2) {
+                    if (HALVES) {
+                        // with sc >= 4 the z membership inside an 8-voxel tile is
+                        // one or both 4-voxel halves
+                        if (zlo < 4) {
+#pragma unroll
+                            for (int z = 0; z < 4; z++) {
+                                const float dz = vz[z] - pz;
+                                m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                            }
+                        }
+                        if (zhi >= 4) {
+#pragma unroll
+                            for (int z = 4; z < 8; z++) {
+                                const float dz = vz[z] - pz;
+                                m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int z = 0; z < 8; z++) {
+                            if (z >= zlo && z <= zhi) {
+                                const float dz = vz[z] - pz;
+                                m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        cur = nxt;
+    }
+
+    if (!lane_ok) return;
+    const int br = gp.brick_res, a = gp.apron;
+    float* brick = gp.chan + (int64_t)k * br * br * br;
+    const float r = gp.radius;
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        if (z < td) {
+            int64_t vi = ((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a);
+            float init = gp.accum ? brick[vi] : (gp.mode == 1 ? 3.0f : 0.0f);
+            float c = sqrtf(__uint_as_float(m[z]));
+            brick[vi] = gp.mode == 1 ? fminf(init, c - r) : fmaxf(init, r - c);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weighted cubic splat: rho(v) = sum_p w_p * max(0, 1 - |v-p|^2/r^2)^3 at the
 // integer voxel coordinates of every leaf (extension of distFunc,
 // cuda_gvdb_particles.cuh:516-523).  Sums are taken in a fixed order
 // (neighbour bins in a fixed order, then ascending point index within a
 // bin), so results are deterministic.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, int tiles_per_brick) {
+__global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
     __shared__ float4 cand[CAND];
     const int lane = threadIdx.x;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles);
+    if (tile < 0) return;
     const int64_t k = tile / tiles_per_brick;
     if (k >= sp.nleaf) return;
     const int td = tile_dim(res0);
@@ -260,8 +419,8 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
 // mPos + (x,y,z) - 1 in whichever leaf contains it, else boundval.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
-    const int64_t k = blockIdx.x;
-    if (k >= ap.nleaf) return;
+    const int64_t k = gvdbx_xcd_item(blockIdx.x, ap.nleaf);
+    if (k < 0) return;
     const int br = ap.brick_res;
     const int r0 = ap.range0;
     const int a = (br - r0) / 2;
@@ -275,7 +434,7 @@ __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
         int ux = nd.mPos[0] + x - a, uy = nd.mPos[1] + y - a, uz = nd.mPos[2] + z - a;
         float v = ap.boundval;
         if (ux >= 0 && uy >= 0 && uz >= 0) {
-            int cx = ux / r0, cy = uy / r0, cz = uz / r0;
+            int cx = ux >> ap.logr0, cy = uy >> ap.logr0, cz = uz >> ap.logr0;
             if (ap.g0.contains(cx, cy, cz)) {
                 int32_t leaf = ap.idx0[ap.g0.index(cx, cy, cz)];
                 if (leaf >= 0 && leaf < ap.nleaf) {
@@ -298,8 +457,7 @@ __global__ __launch_bounds__(256) void k_insert_points(InsertParams ip) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
         float4 p = ip.sorted[j];
         int i = __float_as_int(p.w);
-        int c[3] = {gvdbx_cover(p.x, ip.range0) / ip.range0, gvdbx_cover(p.y, ip.range0) / ip.range0,
-                    gvdbx_cover(p.z, ip.range0) / ip.range0};
+        int c[3] = {gvdbx_cell(p.x, ip.logr0), gvdbx_cell(p.y, ip.logr0), gvdbx_cell(p.z, ip.logr0)};
         int32_t leaf = -1;
         if (ip.g0.contains(c[0], c[1], c[2])) leaf = ip.idx0[ip.g0.index(c[0], c[1], c[2])];
         if (leaf >= 0) {
@@ -387,7 +545,15 @@ void launch_gather(GatherParams gp, hipStream_t s) {
     int per = res0 / td;
     int tiles = per * per * per;
     int64_t nblk = gp.nleaf * tiles;
-    if (nblk > 0) hipLaunchKernelGGL(k_gather, dim3((unsigned)nblk), dim3(64), 0, s, gp, res0, tiles);
+    if (nblk <= 0) return;
+    dim3 grid((unsigned)gvdbx_xcd_blocks(nblk));
+    const bool halves = td == 8 && gp.sc >= 4;
+    if (gp.variant == 0)
+        hipLaunchKernelGGL(k_gather, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
+    else if (halves)
+        hipLaunchKernelGGL(k_gather_rl<true>, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
+    else
+        hipLaunchKernelGGL(k_gather_rl<false>, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
 }
 
 void launch_splat_cubic(SplatParams sp, hipStream_t s) {
@@ -396,11 +562,12 @@ void launch_splat_cubic(SplatParams sp, hipStream_t s) {
     int per = res0 / td;
     int tiles = per * per * per;
     int64_t nblk = sp.nleaf * tiles;
-    if (nblk > 0) hipLaunchKernelGGL(k_splat_cubic, dim3((unsigned)nblk), dim3(64), 0, s, sp, res0, tiles);
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_splat_cubic, dim3((unsigned)gvdbx_xcd_blocks(nblk)), dim3(64), 0, s, sp, res0, tiles, nblk);
 }
 
 void launch_apron(ApronParams ap, hipStream_t s) {
-    if (ap.nleaf > 0) hipLaunchKernelGGL(k_apron, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
+    if (ap.nleaf > 0) hipLaunchKernelGGL(k_apron, dim3((unsigned)gvdbx_xcd_blocks(ap.nleaf)), dim3(256), 0, s, ap);
 }
 
 void launch_insert_points(InsertParams ip, hipStream_t s) {

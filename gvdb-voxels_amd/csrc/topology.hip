@@ -221,27 +221,44 @@ __global__ __launch_bounds__(256) void k_bbox(PointSrc src, int64_t n, float tx,
         mx2 = fmaxf(mx2, __shfl_xor(mx2, d, 64));
         cnt += __shfl_xor(cnt, d, 64);
     }
-    if (lane_id() == 0 && cnt > 0) {
-        atomicMin(&acc->mn[0], __float_as_int(mn0));
-        atomicMin(&acc->mn[1], __float_as_int(mn1));
-        atomicMin(&acc->mn[2], __float_as_int(mn2));
-        atomicMax(&acc->mx[0], __float_as_int(mx0));
-        atomicMax(&acc->mx[1], __float_as_int(mx1));
-        atomicMax(&acc->mx[2], __float_as_int(mx2));
-        atomicAdd(&acc->nvalid, cnt);
+    // block reduction, then one set of atomics per block: same-address
+    // atomics from every wave serialise in L2
+    __shared__ float red[4][6];
+    __shared__ unsigned long long redc[4];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        red[w][0] = mn0; red[w][1] = mn1; red[w][2] = mn2;
+        red[w][3] = mx0; red[w][4] = mx1; red[w][5] = mx2;
+        redc[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+            mn0 = fminf(mn0, red[i][0]); mn1 = fminf(mn1, red[i][1]); mn2 = fminf(mn2, red[i][2]);
+            mx0 = fmaxf(mx0, red[i][3]); mx1 = fmaxf(mx1, red[i][4]); mx2 = fmaxf(mx2, red[i][5]);
+            cnt += redc[i];
+        }
+        if (cnt > 0) {
+            atomicMin(&acc->mn[0], __float_as_int(mn0));
+            atomicMin(&acc->mn[1], __float_as_int(mn1));
+            atomicMin(&acc->mn[2], __float_as_int(mn2));
+            atomicMax(&acc->mx[0], __float_as_int(mx0));
+            atomicMax(&acc->mx[1], __float_as_int(mx1));
+            atomicMax(&acc->mx[2], __float_as_int(mx2));
+            atomicAdd(&acc->nvalid, cnt);
+        }
     }
 }
 
 // ---- occupancy + reach ----
 __global__ __launch_bounds__(256) void k_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz,
                                                     ReachParams rp, uint32_t* reach0) {
-    const int r0 = rp.range0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float w[3];
         if (!gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) continue;
         int c[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) c[a] = gvdbx_cover(w[a], r0) / r0;
+        for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
         if (!rp.g0.contains(c[0], c[1], c[2])) continue;
         uint32_t m = 1u << 13;
         if (rp.use_mask) {
@@ -252,8 +269,8 @@ __global__ __launch_bounds__(256) void k_mark_reach(PointSrc src, int64_t n, flo
                 int ix = (int)w[a];
                 int bmn = (int)((float)ix - rp.radius);
                 int bmx = (int)((float)ix + 2.0f * rp.radius);
-                lo[a] = max(bmn / r0 - c[a], -1);
-                hi[a] = min(bmx / r0 - c[a], 1);
+                lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
+                hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
             }
             m = 0;
             for (int dz = lo[2]; dz <= hi[2]; dz++)
@@ -385,34 +402,43 @@ __global__ __launch_bounds__(256) void k_atlas_assign(GNode* leaves, int64_t nle
 }
 
 __global__ __launch_bounds__(256) void k_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6) {
-    int mn[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    int v[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nleaf; k += (int64_t)gridDim.x * blockDim.x) {
         if (!leaves[k].mFlags) continue;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            mn[a] = min(mn[a], leaves[k].mPos[a]);
-            mx[a] = max(mx[a], leaves[k].mPos[a] + range0);
+            v[a] = min(v[a], leaves[k].mPos[a]);
+            v[3 + a] = max(v[3 + a], leaves[k].mPos[a] + range0);
         }
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1)
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
-            mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
+            v[a] = min(v[a], __shfl_xor(v[a], d, 64));
+            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], d, 64));
         }
+    __shared__ int red[4][6];
+    const int w = threadIdx.x >> 6;
     if (lane_id() == 0)
-#pragma unroll
+        for (int a = 0; a < 6; a++) red[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++)
+            for (int a = 0; a < 3; a++) {
+                v[a] = min(v[a], red[i][a]);
+                v[3 + a] = max(v[3 + a], red[i][3 + a]);
+            }
         for (int a = 0; a < 3; a++) {
-            atomicMin(&out6[a], mn[a]);
-            atomicMax(&out6[3 + a], mx[a]);
+            atomicMin(&out6[a], v[a]);
+            atomicMax(&out6[3 + a], v[3 + a]);
         }
+    }
 }
 
 // ---- point binning ----
 __global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp,
                                                    int32_t* cnt, int32_t* pslot, unsigned long long* len) {
-    const int r0 = bp.range0;
     unsigned long long lsum = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float w[3];
@@ -420,25 +446,25 @@ __global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, floa
         if (gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) {
             int c[3];
 #pragma unroll
-            for (int a = 0; a < 3; a++) c[a] = gvdbx_cover(w[a], r0) / r0;
+            for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], bp.logr0);
             if (bp.gb.contains(c[0], c[1], c[2])) slot = atomicAdd(&cnt[bp.gb.index(c[0], c[1], c[2])], 1);
             if (bp.count_len) {
                 // subcell box of gvdbCountSubcell (cuda_gvdb_particles.cuh:288-293)
                 int lo[3], hi[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    lo[a] = ((int)(w[a] - bp.radius) / bp.sc) * bp.sc;
-                    hi[a] = ((int)(w[a] + 1 + bp.radius) / bp.sc) * bp.sc;
+                    lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * bp.sc;
+                    hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * bp.sc;
                 }
                 for (int x = lo[0]; x <= hi[0]; x += bp.sc) {
                     if (x < 0) continue;
-                    int cx = x / r0;
+                    int cx = x >> bp.logr0;
                     for (int y = lo[1]; y <= hi[1]; y += bp.sc) {
                         if (y < 0) continue;
-                        int cy = y / r0;
+                        int cy = y >> bp.logr0;
                         for (int z = lo[2]; z <= hi[2]; z += bp.sc) {
                             if (z < 0) continue;
-                            int cz = z / r0;
+                            int cz = z >> bp.logr0;
                             if (!bp.g0.contains(cx, cy, cz)) continue;
                             int64_t ci = bp.g0.index(cx, cy, cz);
                             if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum++;
@@ -452,12 +478,18 @@ __global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, floa
     if (bp.count_len) {
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) lsum += __shfl_xor(lsum, d, 64);
-        if (lane_id() == 0 && lsum) atomicAdd(len, lsum);
+        __shared__ unsigned long long red[4];
+        if (lane_id() == 0) red[threadIdx.x >> 6] = lsum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++) lsum += red[i];
+            if (lsum) atomicAdd(len, lsum);
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz,
-                                                     CellGrid gb, int r0, const int32_t* off, const int32_t* pslot,
+                                                     CellGrid gb, int logr0, const int32_t* off, const int32_t* pslot,
                                                      float4* sorted) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int32_t slot = pslot[i];
@@ -466,7 +498,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(PointSrc src, int64_t n, fl
         gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2]);
         int c[3];
 #pragma unroll
-        for (int a = 0; a < 3; a++) c[a] = gvdbx_cover(w[a], r0) / r0;
+        for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], logr0);
         int64_t dst = (int64_t)off[gb.index(c[0], c[1], c[2])] + slot;
         sorted[dst] = make_float4(w[0], w[1], w[2], __int_as_float((int)i));
     }
@@ -546,7 +578,7 @@ int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 void launch_bbox_init(BBoxAccum* acc, hipStream_t s) { hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, acc); }
 
 void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_bbox, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, n, tx, ty, tz, acc);
+    hipLaunchKernelGGL(k_bbox, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, src, n, tx, ty, tz, acc);
 }
 
 void launch_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz, ReachParams rp, uint32_t* reach0,
@@ -625,17 +657,17 @@ void launch_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int brick
 }
 
 void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6, hipStream_t s) {
-    hipLaunchKernelGGL(k_leaf_bounds, dim3(grid_for(nleaf, 256, 1024)), dim3(256), 0, s, leaves, nleaf, range0, out6);
+    hipLaunchKernelGGL(k_leaf_bounds, dim3(grid_for(nleaf, 256, 256)), dim3(256), 0, s, leaves, nleaf, range0, out6);
 }
 
 void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
                       int32_t* pslot, unsigned long long* len, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, bp, cnt, pslot, len);
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, src, n, tx, ty, tz, bp, cnt, pslot, len);
 }
 
-void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int range0,
+void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0,
                         const int32_t* off, const int32_t* pslot, float4* sorted, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, range0, off,
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, logr0, off,
                        pslot, sorted);
 }
 

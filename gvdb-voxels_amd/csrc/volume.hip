@@ -81,6 +81,8 @@ struct gvdbx_volume {
         flag_all[GVDBX_MAXLEV];
     DevBuf reach0, leafbits, partials, totals, bbox, misc;
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
+    int gather_variant = 1;
+    int xcd_remap = 0;
     // atlas map
     bool has_atlas = false;
     DevBuf amap;
@@ -258,8 +260,10 @@ int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool c
         BinParams bp{};
         bp.gb = gb;
         bp.range0 = v->res[0];
+        bp.logr0 = v->logdim[0];
         bp.count_len = count_len ? 1 : 0;
         bp.sc = sc;
+        bp.sc_shift = sc > 0 ? __builtin_ctz((unsigned)sc) : 0;
         bp.radius = radius;
         bp.g0 = g0;
         if (count_len) {
@@ -274,7 +278,7 @@ int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool c
         launch_count_scan_reduce(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(),
                                  (uint32_t*)((char*)v->misc.p + 96), s);
         launch_count_scan_apply(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(), v->bin_off.as<int32_t>(), s);
-        launch_bin_scatter(v->pos, n, t[0], t[1], t[2], gb, v->res[0], v->bin_off.as<int32_t>(),
+        launch_bin_scatter(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], v->bin_off.as<int32_t>(),
                            v->pslot.as<int32_t>(), v->sorted.as<float4>(), s);
         TRY(check_launch("bin points"));
         v->gbin = gb;
@@ -379,7 +383,7 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
     launch_fill_i32((int32_t*)v->reach0.p, G0, 0, s);
     launch_fill_u8(v->flag_all[0].as<uint8_t>(), G0, 0, s);
-    ReachParams rp{grid[0], r0, (extra && mask_ok) ? 1 : 0, radius};
+    ReachParams rp{grid[0], r0, v->logdim[0], (extra && mask_ok) ? 1 : 0, radius};
     launch_mark_reach(v->pos, n, orig[0], orig[1], orig[2], rp, v->reach0.as<uint32_t>(), s);
     if (extra && !mask_ok)
         launch_mark_boxes(v->pos, n, orig[0], orig[1], orig[2], grid[0], r0, radius, v->flag_all[0].as<uint8_t>(), s);
@@ -546,9 +550,16 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.mode = mode;
     gp.accum = accum ? 1 : 0;
     gp.sc = sc;
+    gp.sc_shift = __builtin_ctz((unsigned)sc);
     gp.box_radius = v->sc_radius;
     gp.radius = radius;
-    gp.cutoff = accum ? 1.0e7f : (mode == 1 ? radius + 4.0f : radius + 1.0f);
+    // Points farther than the cutoff cannot change a voxel: the level set
+    // starts at 3.0 and a member at distance c contributes c - r, the density
+    // starts at 0 and a member contributes r - c.  The margin covers FP32
+    // rounding of the distance (relative error < 1e-6).
+    gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
+    gp.variant = v->gather_variant;
+    gp.xcd_remap = v->xcd_remap;
     launch_gather(gp, v->stream);
     return check_launch(what);
 }
@@ -696,6 +707,36 @@ int gvdbx_set_topology_mode(gvdbx_volume* v, int mode) {
     return GVDBX_OK;
 }
 
+int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
+    TRY(check_vol(v));
+    if (!key) FAIL(GVDBX_ERR_ARG, "null tuning key");
+    if (!strcmp(key, "gather_variant")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "gather_variant must be 0 or 1");
+        v->gather_variant = value;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "xcd_remap")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "xcd_remap must be 0 or 1");
+        v->xcd_remap = value;
+        return GVDBX_OK;
+    }
+    FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
+}
+
+int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
+    TRY(check_vol(v));
+    if (!key || !value) FAIL(GVDBX_ERR_ARG, "null tuning key or output");
+    if (!strcmp(key, "gather_variant")) {
+        *value = v->gather_variant;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "xcd_remap")) {
+        *value = v->xcd_remap;
+        return GVDBX_OK;
+    }
+    FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
+}
+
 int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_stride, const void* vel_dev,
                      int vel_off, int vel_stride, const void* clr_dev, int clr_off, int clr_stride) {
     (void)vel_dev; (void)vel_off; (void)vel_stride; (void)clr_dev; (void)clr_off; (void)clr_stride;
@@ -783,7 +824,7 @@ int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float
     launch_fill_i32(v->pnode.as<int32_t>(), num_pnts, (int32_t)GVDBX_ID_UNDEFL, v->stream);
     launch_fill_i32(v->poff.as<int32_t>(), num_pnts, 0, v->stream);
     InsertParams ip{v->gbin, v->grid[0], v->idx[0].as<int32_t>(), v->bin_off.as<int32_t>(), v->sorted.as<float4>(),
-                    v->res[0], v->pnode.as<int32_t>(), v->poff.as<int32_t>(), num_pnts};
+                    v->logdim[0], v->pnode.as<int32_t>(), v->poff.as<int32_t>(), num_pnts};
     launch_insert_points(ip, v->stream);
     v->npnode = num_pnts;
     return check_launch("InsertPoints");
@@ -836,7 +877,7 @@ int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
     if (v->apron == 0) return GVDBX_OK;   // VG.cpp:4420
     TRY(need_atlas(v, "UpdateApron"));
     ApronParams ap{v->nodes[0].as<GNode>(), v->cnt[0], v->grid[0], v->idx[0].as<int32_t>(), v->res[0],
-                   v->chan[chan].as<float>(), brick_res(v), boundval};
+                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval};
     launch_apron(ap, v->stream);
     return check_launch("UpdateApron");
 }

@@ -81,6 +81,14 @@ class VolumeGVDB:
     def SetTopologyMode(self, mode):
         call("gvdbx_set_topology_mode", self._h, int(mode))
 
+    def SetTuning(self, key, value):
+        call("gvdbx_set_tuning", self._h, key.encode(), int(value))
+
+    def GetTuning(self, key):
+        out = C.c_int(0)
+        call("gvdbx_get_tuning", self._h, key.encode(), C.byref(out))
+        return out.value
+
     # ------------------------------------------------------------ points
     def SetPoints(self, pos, vel=None, clr=None, pos_off=0, pos_stride=12):
         """pos: device float32 tensor (N,3) (or an address with offset/stride)."""

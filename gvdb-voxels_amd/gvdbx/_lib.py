@@ -38,6 +38,8 @@ _SIGS = {
     "gvdbx_add_channel": [_vp, _i, _i, _i, _i, _i],
     "gvdbx_fill_channel": [_vp, _i, _f],
     "gvdbx_set_topology_mode": [_vp, _i],
+    "gvdbx_set_tuning": [_vp, C.c_char_p, _i],
+    "gvdbx_get_tuning": [_vp, C.c_char_p, C.POINTER(C.c_int)],
     "gvdbx_set_points": [_vp, _vp, _i, _i, _vp, _i, _i, _vp, _i, _i],
     "gvdbx_set_point_weights": [_vp, _vp, _i],
     "gvdbx_rebuild_topology": [_vp, _i, _f, _f, _f, _f],

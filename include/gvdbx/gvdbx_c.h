@@ -84,6 +84,11 @@ int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filte
 int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value);
 /* choose the topology builder (no reference counterpart; default AUTO) */
 int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
+/* kernel tuning knobs (no reference counterpart): key "gather_variant"
+ * (0 = LDS-staged candidates, 1 = ballot/readlane walk; default 1),
+ * "xcd_remap" (1 = contiguous brick runs per XCD; default 0) */
+int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value);
+int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value);
 
 /* ---- points ---- */
 /* SetPoints(DataPtr pos, vel, clr) VG.h:541 / VG.cpp:4900-4908 with

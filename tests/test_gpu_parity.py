@@ -272,3 +272,27 @@ def test_full_size_cfg2_properties():
     assert 130000 < nb < 170000
     # every leaf is the cell of some point or within the extra-brick box of one
     assert inf.node_cnt[3] == 1
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_gather_variants_bit_identical(mode):
+    """All gather kernel variants produce the oracle's bits."""
+    _log(f"[gpu] gather variants mode={mode}")
+    rng = np.random.default_rng(101 + mode)
+    pts = (rng.random((40000, 3)) * [100, 60, 90] + [17, 25, 9]).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 3.0)
+    gl = gv.InsertPointsSubcell(4, len(pts), 3.0)
+    ov.insert_points_subcell(4, pts, 3.0)
+    if mode == 1:
+        ov.gather_level_set(1.5)
+    else:
+        ov.gather_density(1.5)
+    ref = ov.bricks()[:, 1:9, 1:9, 1:9]
+    for var in (0, 1):
+        gv.SetTuning("gather_variant", var)
+        gv.FillChannel(0, -7.0)
+        if mode == 1:
+            gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), gl, 0)
+        else:
+            gv.GatherDensity(4, len(pts), 1.5, (0, 0, 0), gl, 0)
+        np.testing.assert_array_equal(gv.bricks()[:, 1:9, 1:9, 1:9], ref, err_msg=f"variant {var}")
