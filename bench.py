@@ -58,6 +58,19 @@ def frame_bytes_alg(n, nleaf, node_cnt, res, depth, br=10, r0=8, passes=3):
     return b
 
 
+def measured_traffic(kernel_profile):
+    """HBM bytes per launch of the dominant kernel from its committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", kernel_profile)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(pts):
     """The CPU oracle (single thread) on a full-density slab of frame 0 (x < 96: 1/16 of the points)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -173,6 +186,7 @@ def main():
     achieved = gather_bytes / gather_s
     fb = frame_bytes_alg(n, nleaf, list(inf.node_cnt), res, inf.depth)
     value = n / (ms / 1e3) / 1e6
+    traffic = measured_traffic("gather_levelset_config2_counters.md") if n == 1 << 24 else None
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -182,10 +196,11 @@ def main():
                                "UpdateApron(3.0), 10 ABC-advected frames, full rebuild per frame",
                    "n_points": n, "frames": args.frames, "leaf_bricks": int(nleaf), "depth": inf.depth,
                    "sc_len": int(inf.sc_len)},
-        "roofline": {"bound": "hbm", "kernel": "k_gather (GatherLevelSet)", "achieved": round(achieved / 1e9, 2),
+        "roofline": {"bound": "hbm", "kernel": "k_gather_rl (GatherLevelSet)", "achieved": round(achieved / 1e9, 2),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
-                     "traffic": None, "alg_bytes_per_launch": gather_bytes,
-                     "avg_launch_ms": round(stage["gather_levelset_ms"], 4)},
+                     "traffic": traffic, "alg_bytes_per_launch": gather_bytes,
+                     "avg_launch_ms": round(stage["gather_levelset_ms"], 4),
+                     "traffic_source": "profiles/gather_levelset_config2_counters.md" if traffic else None},
         "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
                            "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},

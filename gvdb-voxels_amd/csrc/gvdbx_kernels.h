@@ -21,8 +21,6 @@ struct ReachParams {
     int use_mask;      // 1: 3x3x3 reach mask per cell; 0: no extra bricks (radius < 1)
     float radius;
 };
-void launch_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz, ReachParams rp, uint32_t* reach0,
-                       hipStream_t s);
 // generic path for radii beyond half a brick: mark every box cell directly
 void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid g0, int range0,
                        float radius, uint8_t* all0, hipStream_t s);
@@ -81,14 +79,19 @@ struct BinParams {
     int logr0;
     int sc_shift;
     // subcell list length (InsertPointsSubcell out-parameter)
-    int count_len;
     int sc;
     float radius;
     CellGrid g0;
     const uint32_t* leafbits;   // one bit per g0 cell
 };
 void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
-                      int32_t* pslot, unsigned long long* len, hipStream_t s);
+                      int32_t* pslot, hipStream_t s);
+// reach masks of level-0 cells from cell-sorted points (total = off[gb.count()])
+void launch_reach_sorted(const float4* sorted, const int32_t* total, int64_t nmax, ReachParams rp, uint32_t* reach0,
+                         hipStream_t s);
+// InsertPointsSubcell list length over cell-sorted points
+void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax, BinParams bp,
+                        unsigned long long* len, hipStream_t s);
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s);
 void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
                              hipStream_t s);

@@ -418,32 +418,37 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
 // apron voxel (x,y,z) of brick k holds the value of world voxel
 // mPos + (x,y,z) - 1 in whichever leaf contains it, else boundval.
 // ---------------------------------------------------------------------------
+// BR > 0: padded brick size known at compile time; BR == 0: runtime size.
+template <int BR>
 __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
-    const int64_t k = gvdbx_xcd_item(blockIdx.x, ap.nleaf);
-    if (k < 0) return;
-    const int br = ap.brick_res;
+    const int64_t k = blockIdx.x;
+    if (k >= ap.nleaf) return;
+    const int br = BR > 0 ? BR : ap.brick_res;
     const int r0 = ap.range0;
     const int a = (br - r0) / 2;
+    __shared__ int32_t nbl[27];
     const GNode nd = ap.leaves[k];
-    float* brick = ap.chan + k * (int64_t)br * br * br;
+    const int cx = nd.mPos[0] >> ap.logr0, cy = nd.mPos[1] >> ap.logr0, cz = nd.mPos[2] >> ap.logr0;
+    if (threadIdx.x < 27) {
+        int d = threadIdx.x;
+        int nx = cx + d % 3 - 1, ny = cy + (d / 3) % 3 - 1, nz = cz + d / 9 - 1;
+        int32_t leaf = -1;
+        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        nbl[d] = (leaf >= 0 && leaf < ap.nleaf) ? leaf : -1;
+    }
+    __syncthreads();
     const int nv = br * br * br;
+    float* brick = ap.chan + k * (int64_t)nv;
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
-        int x = i % br, y = (i / br) % br, z = i / (br * br);
-        bool interior = x >= a && y >= a && z >= a && x < a + r0 && y < a + r0 && z < a + r0;
-        if (interior) continue;
-        int ux = nd.mPos[0] + x - a, uy = nd.mPos[1] + y - a, uz = nd.mPos[2] + z - a;
+        const int x = i % br, y = (i / br) % br, z = i / (br * br);
+        const int dx = x < a ? -1 : (x >= a + r0 ? 1 : 0);
+        const int dy = y < a ? -1 : (y >= a + r0 ? 1 : 0);
+        const int dz = z < a ? -1 : (z >= a + r0 ? 1 : 0);
+        if ((dx | dy | dz) == 0) continue;   // interior voxel
+        const int32_t leaf = nbl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
         float v = ap.boundval;
-        if (ux >= 0 && uy >= 0 && uz >= 0) {
-            int cx = ux >> ap.logr0, cy = uy >> ap.logr0, cz = uz >> ap.logr0;
-            if (ap.g0.contains(cx, cy, cz)) {
-                int32_t leaf = ap.idx0[ap.g0.index(cx, cy, cz)];
-                if (leaf >= 0 && leaf < ap.nleaf) {
-                    const int32_t* lp = ap.leaves[leaf].mPos;
-                    v = ap.chan[(int64_t)leaf * nv + ((int64_t)(uz - lp[2] + a) * br + (uy - lp[1] + a)) * br +
-                                (ux - lp[0] + a)];
-                }
-            }
-        }
+        if (leaf >= 0)   // interior voxel of the neighbour holding world voxel mPos + (x,y,z) - a
+            v = ap.chan[(int64_t)leaf * nv + ((int64_t)(z - dz * r0) * br + (y - dy * r0)) * br + (x - dx * r0)];
         brick[i] = v;
     }
 }
@@ -567,7 +572,11 @@ void launch_splat_cubic(SplatParams sp, hipStream_t s) {
 }
 
 void launch_apron(ApronParams ap, hipStream_t s) {
-    if (ap.nleaf > 0) hipLaunchKernelGGL(k_apron, dim3((unsigned)gvdbx_xcd_blocks(ap.nleaf)), dim3(256), 0, s, ap);
+    if (ap.nleaf <= 0) return;
+    if (ap.brick_res == 10)
+        hipLaunchKernelGGL(k_apron<10>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
+    else
+        hipLaunchKernelGGL(k_apron<0>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
 }
 
 void launch_insert_points(InsertParams ip, hipStream_t s) {

@@ -251,36 +251,6 @@ __global__ __launch_bounds__(256) void k_bbox(PointSrc src, int64_t n, float tx,
 }
 
 // ---- occupancy + reach ----
-__global__ __launch_bounds__(256) void k_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz,
-                                                    ReachParams rp, uint32_t* reach0) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float w[3];
-        if (!gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) continue;
-        int c[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
-        if (!rp.g0.contains(c[0], c[1], c[2])) continue;
-        uint32_t m = 1u << 13;
-        if (rp.use_mask) {
-            int lo[3], hi[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                // extra-brick box of gvdbCalcExtraBrickId (cuda_gvdb_particles.cuh:493-494)
-                int ix = (int)w[a];
-                int bmn = (int)((float)ix - rp.radius);
-                int bmx = (int)((float)ix + 2.0f * rp.radius);
-                lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
-                hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
-            }
-            m = 0;
-            for (int dz = lo[2]; dz <= hi[2]; dz++)
-                for (int dy = lo[1]; dy <= hi[1]; dy++)
-                    for (int dx = lo[0]; dx <= hi[0]; dx++) m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
-        }
-        atomicOr(&reach0[rp.g0.index(c[0], c[1], c[2])], m);
-    }
-}
-
 __global__ __launch_bounds__(256) void k_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz,
                                                     CellGrid g0, int r0, float radius, uint8_t* all0) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -438,8 +408,7 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(const GNode* leaves, int64_
 
 // ---- point binning ----
 __global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp,
-                                                   int32_t* cnt, int32_t* pslot, unsigned long long* len) {
-    unsigned long long lsum = 0;
+                                                   int32_t* cnt, int32_t* pslot) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float w[3];
         int32_t slot = -1;
@@ -448,43 +417,100 @@ __global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, floa
 #pragma unroll
             for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], bp.logr0);
             if (bp.gb.contains(c[0], c[1], c[2])) slot = atomicAdd(&cnt[bp.gb.index(c[0], c[1], c[2])], 1);
-            if (bp.count_len) {
-                // subcell box of gvdbCountSubcell (cuda_gvdb_particles.cuh:288-293)
-                int lo[3], hi[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * bp.sc;
-                    hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * bp.sc;
-                }
-                for (int x = lo[0]; x <= hi[0]; x += bp.sc) {
-                    if (x < 0) continue;
-                    int cx = x >> bp.logr0;
-                    for (int y = lo[1]; y <= hi[1]; y += bp.sc) {
-                        if (y < 0) continue;
-                        int cy = y >> bp.logr0;
-                        for (int z = lo[2]; z <= hi[2]; z += bp.sc) {
-                            if (z < 0) continue;
-                            int cz = z >> bp.logr0;
-                            if (!bp.g0.contains(cx, cy, cz)) continue;
-                            int64_t ci = bp.g0.index(cx, cy, cz);
-                            if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum++;
-                        }
-                    }
-                }
-            }
         }
         pslot[i] = slot;
     }
-    if (bp.count_len) {
+}
+
+// Extra-brick reach of every level-0 cell from the cell-sorted points: each
+// point's box of gvdbCalcExtraBrickId (cuda_gvdb_particles.cuh:493-494) as a
+// 3x3x3 mask; the masks of one cell are OR-ed within the wave (points of a
+// cell are contiguous) and one atomicOr per cell run lands in reach0.
+__global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
+                                                      ReachParams rp, uint32_t* reach0) {
+    const int64_t total = min((int64_t)*total_ptr, nmax);
+    const int lane = lane_id();
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = base + threadIdx.x;
+        int key = -1;
+        uint32_t m = 0;
+        if (j < total) {
+            float4 p = sorted[j];
+            float w[3] = {p.x, p.y, p.z};
+            int c[3];
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) lsum += __shfl_xor(lsum, d, 64);
-        __shared__ unsigned long long red[4];
-        if (lane_id() == 0) red[threadIdx.x >> 6] = lsum;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int i = 1; i < (int)(blockDim.x >> 6); i++) lsum += red[i];
-            if (lsum) atomicAdd(len, lsum);
+            for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
+            if (rp.g0.contains(c[0], c[1], c[2])) {
+                key = (int)rp.g0.index(c[0], c[1], c[2]);
+                m = 1u << 13;
+                if (rp.use_mask) {
+                    int lo[3], hi[3];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        int ix = (int)w[a];
+                        int bmn = (int)((float)ix - rp.radius);
+                        int bmx = (int)((float)ix + 2.0f * rp.radius);
+                        lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
+                        hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
+                    }
+                    m = 0;
+                    for (int dz = lo[2]; dz <= hi[2]; dz++)
+                        for (int dy = lo[1]; dy <= hi[1]; dy++)
+                            for (int dx = lo[0]; dx <= hi[0]; dx++) m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+                }
+            }
         }
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t om = __shfl_down(m, d, 64);
+            int ok = __shfl_down(key, d, 64);
+            if (lane + d < 64 && ok == key) m |= om;
+        }
+        int prev = __shfl_up(key, 1, 64);
+        if (key >= 0 && (lane == 0 || prev != key)) atomicOr(&reach0[key], m);
+    }
+}
+
+// Subcell list length of InsertPointsSubcell (gvdbCountSubcell,
+// cuda_gvdb_particles.cuh:275-315): per point, the subcells of its box that
+// lie in a leaf; summed per block, one atomic per block.
+__global__ __launch_bounds__(256) void k_subcell_len(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
+                                                     BinParams bp, unsigned long long* len) {
+    const int64_t total = min((int64_t)*total_ptr, nmax);
+    unsigned long long lsum = 0;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
+        float4 p = sorted[j];
+        float w[3] = {p.x, p.y, p.z};
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * bp.sc;
+            hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * bp.sc;
+        }
+        for (int x = lo[0]; x <= hi[0]; x += bp.sc) {
+            if (x < 0) continue;
+            int cx = x >> bp.logr0;
+            for (int y = lo[1]; y <= hi[1]; y += bp.sc) {
+                if (y < 0) continue;
+                int cy = y >> bp.logr0;
+                for (int z = lo[2]; z <= hi[2]; z += bp.sc) {
+                    if (z < 0) continue;
+                    int cz = z >> bp.logr0;
+                    if (!bp.g0.contains(cx, cy, cz)) continue;
+                    int64_t ci = bp.g0.index(cx, cy, cz);
+                    if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum++;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) lsum += __shfl_xor(lsum, d, 64);
+    __shared__ unsigned long long red[4];
+    if (lane_id() == 0) red[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) lsum += red[i];
+        if (lsum) atomicAdd(len, lsum);
     }
 }
 
@@ -581,11 +607,6 @@ void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccu
     hipLaunchKernelGGL(k_bbox, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, src, n, tx, ty, tz, acc);
 }
 
-void launch_mark_reach(PointSrc src, int64_t n, float tx, float ty, float tz, ReachParams rp, uint32_t* reach0,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_mark_reach, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, rp, reach0);
-}
-
 void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid g0, int range0,
                        float radius, uint8_t* all0, hipStream_t s) {
     hipLaunchKernelGGL(k_mark_boxes, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, g0, range0, radius,
@@ -661,8 +682,20 @@ void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t*
 }
 
 void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
-                      int32_t* pslot, unsigned long long* len, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, src, n, tx, ty, tz, bp, cnt, pslot, len);
+                      int32_t* pslot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, bp, cnt, pslot);
+}
+
+void launch_reach_sorted(const float4* sorted, const int32_t* total, int64_t nmax, ReachParams rp, uint32_t* reach0,
+                         hipStream_t s) {
+    if (nmax > 0)
+        hipLaunchKernelGGL(k_reach_sorted, dim3(grid_for(nmax, 256)), dim3(256), 0, s, sorted, total, nmax, rp, reach0);
+}
+
+void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax, BinParams bp,
+                        unsigned long long* len, hipStream_t s) {
+    if (nmax > 0)
+        hipLaunchKernelGGL(k_subcell_len, dim3(grid_for(nmax, 256, 2048)), dim3(256), 0, s, sorted, total, nmax, bp, len);
 }
 
 void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0,

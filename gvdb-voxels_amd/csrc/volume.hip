@@ -66,6 +66,7 @@ struct gvdbx_volume {
     int64_t atlas_cap = 0;   // bricks per channel buffer
     // points (SetPoints VG.cpp:4900-4908)
     bool has_pos = false;
+    uint64_t pos_gen = 0;   // bumped by SetPoints; bins are reused only within one generation
     PointSrc pos{nullptr, 12};
     const char* wts = nullptr;
     int64_t wstride = 4;
@@ -96,6 +97,7 @@ struct gvdbx_volume {
     float bins_trans[3] = {};
     const char* bins_base = nullptr;
     int64_t bins_stride = 0;
+    uint64_t bins_gen = 0;
     // subcell state (InsertPointsSubcell)
     int sc_size = 0;
     float sc_radius = 0.0f;
@@ -239,18 +241,20 @@ int determine_depth(const gvdbx_volume* v, const float pmin[3], const float pmax
     return -1;
 }
 
-// Bin the current points by home brick cell (one cell margin around the
-// leaf grid).  stable=true additionally orders each bin by point index.
-int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool count_len, int sc, float radius,
-               int64_t* len_out) {
+// Bin the current points by home brick cell (counting sort over the leaf grid
+// grown by one cell).  Bins made by RebuildTopology or an earlier call are
+// reused while SetPoints has not been called again and the point count and
+// translation match.  stable=true additionally orders each bin by point index.
+int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable) {
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
-    bool same = v->bins_valid && v->bins_n == n && v->bins_base == v->pos.base && v->bins_stride == v->pos.stride &&
-                v->bins_trans[0] == t[0] && v->bins_trans[1] == t[1] && v->bins_trans[2] == t[2];
     const CellGrid& g0 = v->grid[0];
     CellGrid gb = {g0.ox - 1, g0.oy - 1, g0.oz - 1, g0.nx + 2, g0.ny + 2, g0.nz + 2};
     int64_t nb = gb.count();
+    bool same = v->bins_valid && v->bins_gen == v->pos_gen && v->bins_n == n && v->bins_base == v->pos.base &&
+                v->bins_stride == v->pos.stride && v->bins_trans[0] == t[0] && v->bins_trans[1] == t[1] &&
+                v->bins_trans[2] == t[2] && !memcmp(&v->gbin, &gb, sizeof(gb));
     hipStream_t s = v->stream;
-    if (!same || count_len) {
+    if (!same) {
         TRY(ensure(v, v->bin_cnt, (size_t)(nb + 1) * 4));
         TRY(ensure(v, v->bin_off, (size_t)(nb + 1) * 4));
         TRY(ensure(v, v->pslot, (size_t)std::max<int64_t>(n, 1) * 4));
@@ -261,20 +265,9 @@ int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool c
         bp.gb = gb;
         bp.range0 = v->res[0];
         bp.logr0 = v->logdim[0];
-        bp.count_len = count_len ? 1 : 0;
-        bp.sc = sc;
-        bp.sc_shift = sc > 0 ? __builtin_ctz((unsigned)sc) : 0;
-        bp.radius = radius;
         bp.g0 = g0;
-        if (count_len) {
-            TRY(ensure(v, v->leafbits, (size_t)((g0.count() + 31) / 32) * 4));
-            launch_leaf_bits(v->idx[0].as<int32_t>(), g0.count(), v->leafbits.as<uint32_t>(), s);
-            bp.leafbits = v->leafbits.as<uint32_t>();
-        }
-        unsigned long long* len = (unsigned long long*)((char*)v->misc.p + 64);
         launch_fill_i32(v->bin_cnt.as<int32_t>(), nb + 1, 0, s);
-        launch_fill_u64((uint64_t*)len, 1, 0ull, s);
-        launch_bin_count(v->pos, n, t[0], t[1], t[2], bp, v->bin_cnt.as<int32_t>(), v->pslot.as<int32_t>(), len, s);
+        launch_bin_count(v->pos, n, t[0], t[1], t[2], bp, v->bin_cnt.as<int32_t>(), v->pslot.as<int32_t>(), s);
         launch_count_scan_reduce(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(),
                                  (uint32_t*)((char*)v->misc.p + 96), s);
         launch_count_scan_apply(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(), v->bin_off.as<int32_t>(), s);
@@ -285,15 +278,10 @@ int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool c
         v->bins_valid = true;
         v->bins_stable = false;
         v->bins_n = n;
+        v->bins_gen = v->pos_gen;
         v->bins_base = v->pos.base;
         v->bins_stride = v->pos.stride;
         memcpy(v->bins_trans, t, sizeof(v->bins_trans));
-        if (count_len) {
-            TRY(ensure_host(v, 64));
-            GVDBX_CHECK(hipMemcpyAsync(v->host, len, 8, hipMemcpyDeviceToHost, s));
-            TRY(sync(v));
-            if (len_out) *len_out = (int64_t)(*(unsigned long long*)v->host);
-        }
     }
     if (stable && !v->bins_stable) {
         TRY(ensure(v, v->sorted_tmp, (size_t)std::max<int64_t>(n, 1) * 16));
@@ -301,6 +289,33 @@ int bin_points(gvdbx_volume* v, int64_t n, const float t[3], bool stable, bool c
         TRY(check_launch("bin sort"));
         v->bins_stable = true;
     }
+    return GVDBX_OK;
+}
+
+// InsertPointsSubcell list length over the current bins.
+int subcell_len(gvdbx_volume* v, int sc, float radius, int64_t* len_out) {
+    hipStream_t s = v->stream;
+    const CellGrid& g0 = v->grid[0];
+    TRY(ensure(v, v->leafbits, (size_t)((g0.count() + 31) / 32) * 4));
+    TRY(ensure(v, v->misc, 256));
+    TRY(ensure_host(v, 4096));
+    launch_leaf_bits(v->idx[0].as<int32_t>(), g0.count(), v->leafbits.as<uint32_t>(), s);
+    BinParams bp{};
+    bp.gb = v->gbin;
+    bp.range0 = v->res[0];
+    bp.logr0 = v->logdim[0];
+    bp.sc = sc;
+    bp.sc_shift = __builtin_ctz((unsigned)sc);
+    bp.radius = radius;
+    bp.g0 = g0;
+    bp.leafbits = v->leafbits.as<uint32_t>();
+    unsigned long long* len = (unsigned long long*)((char*)v->misc.p + 64);
+    launch_fill_u64((uint64_t*)len, 1, 0ull, s);
+    launch_subcell_len(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), v->bins_n, bp, len, s);
+    TRY(check_launch("subcell length"));
+    GVDBX_CHECK(hipMemcpyAsync(v->host, len, 8, hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    *len_out = (int64_t)(*(unsigned long long*)v->host);
     return GVDBX_OK;
 }
 
@@ -337,6 +352,9 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     for (int a = 0; a < 3; a++)
         if ((int64_t)pmax[a] / v->range[0] > 65535) FAIL(GVDBX_ERR_RANGE, "RebuildTopology: extents exceed 16-bit brick keys");
 
+    // validation is done: from here on the previous topology is being replaced
+    v->has_topo = false;
+    v->has_atlas = false;
     const bool extra = radius >= 1.0f;   // VG.cpp:1207-1208
     const int r0 = v->res[0];
     const bool mask_ok = 2.0f * radius <= (float)r0 && radius + 1.0f <= (float)r0;
@@ -383,8 +401,14 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
     launch_fill_i32((int32_t*)v->reach0.p, G0, 0, s);
     launch_fill_u8(v->flag_all[0].as<uint8_t>(), G0, 0, s);
+    // bin the points by level-0 cell (kept for InsertPointsSubcell / InsertPoints),
+    // then derive occupancy and extra-brick reach from the cell-sorted points
+    v->grid[0] = grid[0];
+    v->bins_valid = false;
+    TRY(ensure_bins(v, n, orig, false));
     ReachParams rp{grid[0], r0, v->logdim[0], (extra && mask_ok) ? 1 : 0, radius};
-    launch_mark_reach(v->pos, n, orig[0], orig[1], orig[2], rp, v->reach0.as<uint32_t>(), s);
+    launch_reach_sorted(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), n, rp,
+                        v->reach0.as<uint32_t>(), s);
     if (extra && !mask_ok)
         launch_mark_boxes(v->pos, n, orig[0], orig[1], orig[2], grid[0], r0, radius, v->flag_all[0].as<uint8_t>(), s);
     launch_expand_reach(grid[0], v->reach0.as<uint32_t>(), v->flag_base[0].as<uint8_t>(), v->flag_all[0].as<uint8_t>(),
@@ -476,7 +500,6 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     }
     v->has_topo = true;
     v->has_atlas = false;
-    v->bins_valid = false;
     v->sc_valid = false;
     v->topo_mode_used = GVDBX_TOPO_DENSE;
     return GVDBX_OK;
@@ -747,6 +770,7 @@ int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_
     v->pos.base = (const char*)pos_dev + pos_off;
     v->pos.stride = pos_stride;
     v->has_pos = true;
+    v->pos_gen++;
     v->bins_valid = false;
     return GVDBX_OK;
 }
@@ -788,8 +812,8 @@ int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float rad
         FAIL(GVDBX_ERR_RANGE, "InsertPointsSubcell: radius %g + subcell %d exceeds the brick (%d)", radius, sc, v->res[0]);
     float t[3] = {tx, ty, tz};
     int64_t len = 0;
-    v->bins_valid = false;
-    TRY(bin_points(v, num_pnts, t, false, true, sc, radius, &len));
+    TRY(ensure_bins(v, num_pnts, t, false));
+    TRY(subcell_len(v, sc, radius, &len));
     v->sc_size = sc;
     v->sc_radius = radius;
     v->sc_len = len;
@@ -818,7 +842,7 @@ int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float
     TRY(need_atlas(v, "InsertPoints"));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "InsertPoints: negative point count");
     float t[3] = {tx, ty, tz};
-    TRY(bin_points(v, num_pnts, t, true, false, 0, 0.0f, nullptr));
+    TRY(ensure_bins(v, num_pnts, t, true));
     TRY(ensure(v, v->pnode, (size_t)std::max(num_pnts, 1) * 4));
     TRY(ensure(v, v->poff, (size_t)std::max(num_pnts, 1) * 4));
     launch_fill_i32(v->pnode.as<int32_t>(), num_pnts, (int32_t)GVDBX_ID_UNDEFL, v->stream);
@@ -838,7 +862,7 @@ int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp
     TRY(check_chan(v, 0));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "ScatterDensity: negative point count");
     float t[3] = {tx, ty, tz};
-    TRY(bin_points(v, num_pnts, t, true, false, 0, 0.0f, nullptr));
+    TRY(ensure_bins(v, num_pnts, t, true));
     ScatterParams sp{v->nodes[0].as<GNode>(), v->cnt[0], v->gbin, v->bin_off.as<int32_t>(), v->sorted.as<float4>(),
                      v->chan[0].as<float>(), brick_res(v), std::max(v->apron, 0), v->res[0], radius, expand ? 1 : 0};
     if (sp.apron == 0 && expand) FAIL(GVDBX_ERR_ARG, "ScatterDensity: expand needs an apron");
@@ -853,7 +877,7 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "SplatCubic: negative point count");
     if (!(radius > 0.0f) || radius > (float)v->res[0]) FAIL(GVDBX_ERR_RANGE, "SplatCubic: radius must be in (0, brick]");
     float t[3] = {tx, ty, tz};
-    TRY(bin_points(v, num_pnts, t, true, false, 0, 0.0f, nullptr));
+    TRY(ensure_bins(v, num_pnts, t, true));
     SplatParams sp{};
     sp.leaves = v->nodes[0].as<GNode>();
     sp.nleaf = v->cnt[0];

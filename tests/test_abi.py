@@ -62,3 +62,44 @@ def test_generators_deterministic_and_in_range():
     assert r.min() >= 180 - 1e-3 and r.max() <= 200 + 1e-3
     # uniform by volume: median radius = cbrt((180^3+200^3)/2)
     assert abs(np.median(r) - ((180 ** 3 + 200 ** 3) / 2) ** (1 / 3)) < 0.5
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """The header-only VolumeGVDB facade compiles against the C-ABI and links libgvdbx.so."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "facade.cpp"
+    src.write_text(r'''
+#include "gvdbx/VolumeGVDB.h"
+#include <cstdio>
+int main(int argc, char**) {
+    if (argc > 5) {   // never executed here: only the call sequence is compiled and linked
+        gvdbx::VolumeGVDB gvdb;
+        gvdb.Configure(3, 3, 3, 3, 5);
+        gvdb.SetChannelDefault(16, 16, 1);
+        gvdb.AddChannel(0, gvdbx::T_FLOAT, 1);
+        gvdbx::DataPtr pos, vel, clr;
+        gvdb.SetDataGPU(pos, 10, nullptr, 0, 12);
+        gvdb.SetPoints(pos, vel, clr);
+        gvdb.RebuildTopology(10, 3.0f, gvdbx::Vector3DF(0, 0, 0));
+        gvdb.FinishTopology(false, true);
+        gvdb.UpdateAtlas();
+        int len = 0;
+        gvdb.InsertPointsSubcell(4, 10, 3.0f, gvdbx::Vector3DF(0, 0, 0), len);
+        gvdb.GatherLevelSet(4, 10, 1.5f, gvdbx::Vector3DF(0, 0, 0), len, 0, 1);
+        gvdb.UpdateApron(0, 3.0f);
+    }
+    std::printf("%s\n", gvdbx_version());
+    return 0;
+}
+''')
+    import gvdbx
+    libdir = os.path.dirname(gvdbx._lib.LIB_PATH)
+    exe = tmp_path / "facade"
+    cmd = ["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir, "-lgvdbx",
+           f"-Wl,-rpath,{libdir}", "-o", str(exe)]
+    subprocess.run(cmd, check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert "gvdbx" in out
