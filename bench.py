@@ -9,6 +9,11 @@ hot path on resident positions:
     -> InsertPointsSubcell(4, N, 3) -> GatherLevelSet(4, N, r=1.5) -> UpdateApron(0, 3.0)
 The ten frames are generated and advected on the GPU before timing.
 
+A second, differentiable leg (2^21 points in 64 anisotropic Gaussian
+clusters, per-point weights, r = 2 cubic splat, 512x512 orthographic
+emission-absorption raycast, step 0.5, extinction 0.05, loss 1/2|I - I*|^2)
+times the forward and the forward+backward and reports their ratio.
+
 Prints one JSON line (contract: metric/value/unit/.../roofline/cpu_baseline).
 """
 import argparse
@@ -71,6 +76,115 @@ def measured_traffic(kernel_profile):
     return None
 
 
+def make_clusters(seed, n_clusters=64, per=32768):
+    """64 anisotropic Gaussian clusters: means U[64,320)^3, per-axis sigma
+    U(4,20), truncated at 3 sigma; weights U(0.1,1); and the target cloud
+    with every cluster mean shifted by N(0, 2^2)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(64, 320, size=(n_clusters, 3))
+    sig = rng.uniform(4, 20, size=(n_clusters, 3))
+    z = rng.standard_normal(size=(n_clusters * per, 3))
+    bad = np.abs(z) > 3
+    while bad.any():
+        z[bad] = rng.standard_normal(size=int(bad.sum()))
+        bad = np.abs(z) > 3
+    lab = np.repeat(np.arange(n_clusters), per)
+    pts = (means[lab] + z * sig[lab]).astype(np.float32)
+    w = rng.uniform(0.1, 1.0, size=n_clusters * per).astype(np.float32)
+    shift = rng.normal(0.0, 2.0, size=(n_clusters, 3))
+    tgt = (means[lab] + shift[lab] + z * sig[lab]).astype(np.float32)
+    return pts, w, tgt
+
+
+def bench_differentiable(torch, gvdbx, dev, steps, warmup):
+    """Forward and forward+backward of splat -> apron -> EA raycast."""
+    pts, w, tgt = make_clusters(20261203 + 2)
+    n = len(pts)
+    P = torch.from_numpy(pts).to(dev)
+    W = torch.from_numpy(w).to(dev)
+    PT = torch.from_numpy(tgt).to(dev)
+    stream = torch.cuda.current_stream()
+    gd = gvdbx.VolumeGVDB(0, stream=stream)
+    gd.configure([3, 3, 3, 3, 5])
+    gd.SetChannelDefault(16, 16, 1)
+    gd.AddChannel(0, gvdbx.T_FLOAT, 1)
+    cam = gvdbx.camera_ortho(512, 512, (64.0, 64.0, 0.0), (256.0, 0, 0), (0, 256.0, 0), (0, 0, 1))
+    rp = gvdbx.ea_params(step=0.5, kappa=0.05)
+    radius, topo_r = 2.0, 4.0
+    ev = {}
+
+    def mark(name, on):
+        if on:
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(stream)
+            ev.setdefault(name, []).append(e)
+
+    def forward(pos, record, timed=False):
+        gd.SetPoints(pos)
+        gd.SetPointWeights(W)
+        mark("f0", timed)
+        gd.RebuildTopology(n, topo_r)
+        gd.FinishTopology(False, True)
+        gd.UpdateAtlas()
+        mark("f1", timed)
+        gd.SplatCubic(n, radius)
+        mark("f2", timed)
+        gd.UpdateApron(0, 0.0)
+        mark("f3", timed)
+        I, T = gd.RaycastEA(0, cam, rp, record=record)
+        mark("f4", timed)
+        return I, T
+
+    Itgt, _ = forward(PT, False)
+    Itgt = Itgt.clone()
+
+    def backward(I, T, timed=False):
+        gI = I - Itgt
+        grad = gd.RaycastEABackward(0, cam, rp, I, T, gI)
+        mark("b1", timed)
+        gd.ApronAdjoint(grad)
+        mark("b2", timed)
+        dpos, dw = gd.SplatCubicBackward(n, radius, (0.0, 0.0, 0.0), grad)
+        mark("b3", timed)
+        return dpos, dw
+
+    for _ in range(warmup):
+        forward(P, False)
+        I, T = forward(P, True)
+        backward(I, T)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        forward(P, False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        I, T = forward(P, True, timed=True)
+        backward(I, T, timed=True)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd_ms = (t1 - t0) * 1e3 / steps
+    train_ms = (t2 - t1) * 1e3 / steps
+
+    def avg(a, b):
+        return sum(x.elapsed_time(y) for x, y in zip(ev[a], ev[b])) / len(ev[a])
+
+    st = {"topology_ms": avg("f0", "f1"), "splat_cubic_ms": avg("f1", "f2"), "apron_ms": avg("f2", "f3"),
+          "raycast_ea_record_ms": avg("f3", "f4"), "raycast_ea_backward_ms": avg("f4", "b1"),
+          "apron_adjoint_ms": avg("b1", "b2"), "splat_backward_ms": avg("b2", "b3")}
+    inf = gd.info()
+    return {
+        "workload": "2^21 points in 64 Gaussian clusters (means U[64,320)^3, sigma U(4,20) per axis, 3-sigma "
+                    "truncation), weights U(0.1,1), cubic splat r=2 (topology radius 4), UpdateApron(0), "
+                    "512x512 orthographic +z EA raycast over [64,320)^2, step 0.5, kappa 0.05; "
+                    "loss 1/2|I-I*|^2, I* = image of the clusters shifted by N(0,2^2)",
+        "n_points": n, "leaf_bricks": int(inf.node_cnt[0]), "fwd_ms": round(fwd_ms, 4),
+        "fwd_plus_bwd_ms": round(train_ms, 4), "fwd_bwd_over_fwd": round(train_ms / fwd_ms, 3),
+        "stages_ms": {k: round(v, 4) for k, v in st.items()},
+    }
+
+
 def cpu_baseline(pts):
     """The CPU oracle (single thread) on a full-density slab of frame 0 (x < 96: 1/16 of the points)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -98,6 +212,7 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 24)
     ap.add_argument("--frames", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable splat+render leg")
     ap.add_argument("--gather-variant", type=int, default=None,
                     help="0: LDS-staged candidates, 1: ballot/readlane walk (default)")
     ap.add_argument("--xcd-remap", type=int, default=None, help="1: contiguous brick runs per XCD (default 0)")
@@ -207,6 +322,10 @@ def main():
         "gather_variant": gv.GetTuning("gather_variant"), "xcd_remap": gv.GetTuning("xcd_remap"),
         "fwd_bwd_over_fwd": None,
     }
+    if not args.no_diff:
+        diff = bench_differentiable(torch, gvdbx, dev, max(3, min(args.steps, 10)), max(1, min(args.warmup, 2)))
+        out["differentiable"] = diff
+        out["fwd_bwd_over_fwd"] = diff["fwd_bwd_over_fwd"]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pts0)
     print(json.dumps(out), flush=True)

@@ -179,3 +179,95 @@ struct ScatterParams {
     int expand;
 };
 void launch_scatter_density(ScatterParams sp, hipStream_t s);
+
+// ---------------- radix sort (LSD, 8-bit digits) ----------------
+// Sorts n (key, value) pairs on key bits [begin_bit, end_bit), stable.
+// Buffers: keys/vals hold the input; tkeys/tvals are scratch of n entries;
+// hist needs radix_hist_entries(n) int32, off the same + 1, partials
+// scan_tiles(radix_hist_entries(n) + 1) uint32.  Returns 1 when the result
+// ended in tkeys/tvals (odd pass count), else 0.
+int64_t radix_hist_entries(int64_t n);
+int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
+                     int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s);
+void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
+
+// ---------------- rendering ----------------
+struct RenderTree {
+    const GNode* nodes[GVDBX_MAXLEV];
+    const uint64_t* clist[GVDBX_MAXLEV];
+    int64_t ncnt[GVDBX_MAXLEV];
+    int logdim[GVDBX_MAXLEV];
+    int res[GVDBX_MAXLEV];
+    float vdel[GVDBX_MAXLEV];   // child range of level l (range of level l-1)
+    int levs, top_lev;
+    float bmin[3], bmax[3];
+    const float* chan;          // brick-linear padded atlas channel
+    int brick_res;
+};
+struct RenderCam {
+    int width, height;
+    float org[3], org_u[3], org_v[3];
+    float dir[3], dir_u[3], dir_v[3];
+};
+struct RenderScene {
+    float step, extinct, albedo, minval, alphacut, thresh, vmin, vmax, epsilon;
+    int max_iter;
+    float backclr[4];
+    float light[3];
+    float shadow_amt, shadow_bias;
+    const float4* lut;
+};
+// per (ray, brick) segment entry state of a recorded EA raycast
+struct VisitRec {
+    uint64_t* key;       // leaf << ray_bits | ray
+    float* t;            // t handed to the brick function
+    float* T;            // transmittance at entry
+    float* I;            // accumulated intensity at entry
+    uint32_t* count;     // number of segments (may exceed cap: then rerun)
+    int64_t cap;
+    int ray_bits;
+};
+void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba, uchar4* rgba8,
+                   hipStream_t s);
+void launch_raycast_ea(RenderTree tv, RenderCam cam, RenderScene sc, float* I, float* T, int record, VisitRec vr,
+                       hipStream_t s);
+
+struct EABackParams {
+    RenderTree tv;
+    RenderCam cam;
+    RenderScene sc;
+    const uint64_t* key;    // sorted segment keys
+    const uint32_t* idx;    // segment record index of each sorted key
+    const float* vt;
+    const float* vT;
+    const float* vI;
+    const int32_t* seg;     // per leaf [begin, end) into the sorted keys, 2 per leaf
+    const float* I;
+    const float* T;
+    const float* gI;
+    const float* gT;        // may be null
+    float* grad;            // padded bricks
+    int ray_bits;
+};
+void launch_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf, hipStream_t s);
+void launch_ea_backward(EABackParams p, hipStream_t s);
+
+struct SplatBackParams {
+    CellGrid g0;
+    const int32_t* idx0;
+    const GNode* leaves;
+    int64_t nleaf;
+    int logr0;
+    int brick_res;
+    const float4* sorted;      // bin-sorted points (x, y, z, original index bits)
+    const int32_t* total;      // number of sorted points
+    int64_t nmax;
+    const char* weights;
+    int64_t wstride;
+    float radius;
+    const float* grad;
+    float* dpos;
+    float* dw;
+};
+void launch_splat_backward(SplatBackParams p, hipStream_t s);
+void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s);

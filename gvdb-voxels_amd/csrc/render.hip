@@ -1,0 +1,658 @@
+// render.hip -- HDDA raycasting over the sparse tree (Render) and the
+// emission-absorption adjoint, for gfx950.
+//
+// One thread per ray, 8x8-pixel tiles per wave (neighbouring rays walk the
+// same bricks), 16x16 pixels per 256-thread workgroup.  The traversal is the
+// reference's hierarchical DDA (rayCast RC.cuh:543-611, HDDAState
+// DDA.cuh:38-91): empty-space skipping through the child lists down to
+// level 1, then a brick function marches the 8^3 brick at t = step*k.
+// Every float expression keeps the reference's operand order with no
+// contraction, so the set of samples (positions, bricks, transfer-function
+// indices, surface hits) is a fixed function of the inputs:
+//   dot(a,b) = a.x*b.x + a.y*b.y + a.z*b.z; normalize(v) = v * (1/sqrtf(dot))
+//   trilinear: c = p + 0.5, i = floor(c), a = c - i, lerp u*(1-a) + w*a in
+//   x, then y, then z, over the leaf's padded brick (voxel i centred at
+//   i + 0.5, the texel convention of RC.cuh:104-108; indices clamped to the
+//   padded brick).  Hardware tex3D's 8-bit weights are not reproduced.
+// An HDDA axis with a zero direction component never steps (the reference's
+// mask*tDel is 0*inf = NaN there and stalls the ray).
+//
+// The adjoint of the emission-absorption march records, per (ray, brick)
+// segment, the entry t, transmittance T and intensity I (the transmittance
+// checkpoint), sorts the segments by (leaf, ray) with the radix sort, and
+// lets one wave per leaf replay its segments from the checkpoints against
+// the brick staged in LDS, accumulating d loss / d voxel in LDS: no stored
+// samples, no global float atomics, and a fixed accumulation order.
+#include "gvdbx/gvdbx_c.h"
+#include "gvdbx_kernels.h"
+
+namespace {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 scale(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 normalize(V3 a) { return scale(a, 1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// pixel ray (gvdbRayDeep MOD.cu:66-71 with getViewRay GEO.cuh:54-63)
+__device__ __forceinline__ void camera_ray(const RenderCam& c, int x, int y, V3& pos, V3& dir) {
+    const float u = ((float)x + 0.5f) / (float)c.width;
+    const float w = ((float)y + 0.5f) / (float)c.height;
+    pos = v3((u * c.org_u[0] + w * c.org_v[0]) + c.org[0], (u * c.org_u[1] + w * c.org_v[1]) + c.org[1],
+             (u * c.org_u[2] + w * c.org_v[2]) + c.org[2]);
+    V3 d = v3((u * c.dir_u[0] + w * c.dir_v[0]) + c.dir[0], (u * c.dir_u[1] + w * c.dir_v[1]) + c.dir[1],
+              (u * c.dir_u[2] + w * c.dir_v[2]) + c.dir[2]);
+    dir = normalize(d);
+}
+
+// rayBoxIntersect GEO.cuh:85-98
+__device__ __forceinline__ V3 ray_box(V3 pos, V3 dir, const float* mn, const float* mx) {
+    float h0 = (mn[0] - pos.x) / dir.x, h1 = (mx[0] - pos.x) / dir.x;
+    float h2 = (mn[1] - pos.y) / dir.y, h3 = (mx[1] - pos.y) / dir.y;
+    float h4 = (mn[2] - pos.z) / dir.z, h5 = (mx[2] - pos.z) / dir.z;
+    float h6 = fmaxf(fmaxf(fminf(h0, h1), fminf(h2, h3)), fminf(h4, h5));
+    float h7 = fminf(fminf(fmaxf(h0, h1), fmaxf(h2, h3)), fmaxf(h4, h5));
+    h6 = (h6 < 0) ? 0.0f : h6;
+    return v3(h6, h7, (h7 < h6 || h7 < 0) ? GVDBX_NOHIT : 0.0f);
+}
+
+// child of a node (getChild ND.cuh:115-124); reads outside the allocated
+// lists are "no child"; returns the child's pool index or -1
+__device__ __forceinline__ int64_t get_child(const RenderTree& tv, uint64_t listid, int64_t b) {
+    if (listid == GVDBX_ID_UNDEFL) return -1;
+    const int clev = (int)((listid >> 8) & 0xFF);
+    if (clev <= 0 || clev >= tv.levs) return -1;
+    const int64_t len = (int64_t)1 << (3 * tv.logdim[clev]);
+    const int64_t flat = (int64_t)(listid >> 16) * len + b;
+    if (flat < 0 || flat >= tv.ncnt[clev] * len) return -1;
+    const uint64_t e = tv.clist[clev][flat];
+    if (e == GVDBX_ID_UNDEF64) return -1;
+    return (int64_t)(int32_t)(uint32_t)(e >> 16);
+}
+
+struct Hdda {
+    V3 pos, dir;
+    int ps[3];
+    float tdel[3];
+    V3 t;
+    int p[3];
+    float ts[3];
+    int mask[3];
+
+    __device__ __forceinline__ void set(V3 po, V3 di, V3 t0) {
+        pos = po;
+        dir = di;
+        ps[0] = di.x > 0 ? 1 : -1;
+        ps[1] = di.y > 0 ? 1 : -1;
+        ps[2] = di.z > 0 ? 1 : -1;
+        t = t0;
+    }
+    __device__ __forceinline__ void prepare(V3 vmin, float vdel) {
+        const V3 q = sub(add(pos, scale(dir, t.x)), vmin);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float pf = comp(q, a) / vdel;
+            tdel[a] = fabsf(vdel / comp(dir, a));
+            const float fl = floorf(pf);
+            const float br = ((fl - pf) + 0.5f) * (float)ps[a] + 0.5f;
+            ts[a] = __builtin_isinf(tdel[a]) ? __builtin_inff() : br * tdel[a] + t.x;
+            p[a] = (int)fl;
+        }
+    }
+    __device__ __forceinline__ void next() {
+        mask[0] = (ts[0] < ts[1]) & (ts[0] <= ts[2]);
+        mask[1] = (ts[1] < ts[2]) & (ts[1] <= ts[0]);
+        mask[2] = (ts[2] < ts[0]) & (ts[2] <= ts[1]);
+        t.y = mask[0] ? ts[0] : (mask[1] ? ts[1] : ts[2]);
+    }
+    __device__ __forceinline__ void step() {
+        t.x = t.y;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (mask[a]) ts[a] = ts[a] + tdel[a];
+            p[a] += mask[a] * ps[a];
+        }
+    }
+};
+
+// trilinear sample of a padded brick (global memory or LDS)
+template <int BR>
+__device__ __forceinline__ float trilinear(const float* B, int br, V3 p) {
+    const int n = BR > 0 ? BR : br;
+    const float cx = p.x + 0.5f, cy = p.y + 0.5f, cz = p.z + 0.5f;
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+    const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    const int x0 = min(max(ix, 0), n - 1), x1 = min(max(ix + 1, 0), n - 1);
+    const int y0 = min(max(iy, 0), n - 1), y1 = min(max(iy + 1, 0), n - 1);
+    const int z0 = min(max(iz, 0), n - 1), z1 = min(max(iz + 1, 0), n - 1);
+    const float* r00 = B + (z0 * n + y0) * n;
+    const float* r10 = B + (z0 * n + y1) * n;
+    const float* r01 = B + (z1 * n + y0) * n;
+    const float* r11 = B + (z1 * n + y1) * n;
+    const float c00 = r00[x0] * bx + r00[x1] * ax;
+    const float c10 = r10[x0] * bx + r10[x1] * ax;
+    const float c01 = r01[x0] * bx + r01[x1] * ax;
+    const float c11 = r11[x0] * bx + r11[x1] * ax;
+    const float c0 = c00 * by + c10 * ay;
+    const float c1 = c01 * by + c11 * ay;
+    return c0 * bz + c1 * az;
+}
+
+// transfer DDA.cuh:20-23 (float quotient, double clamp and scale, truncation)
+__device__ __forceinline__ int transfer_index(const RenderScene& sc, float s) {
+    const float q = (s - sc.thresh) / (sc.vmax - sc.vmin);
+    const double qc = fmin(1.0, fmax(0.0, (double)q));
+    return (int)(qc * 16300.0);
+}
+
+enum { MODE_DEEP_LUT = 0, MODE_DEEP_LINEAR = 1, MODE_SURF_TRI = 2, MODE_SURF_LS = 3 };
+
+struct RayState {
+    float clr[4];
+    V3 hit, norm;
+};
+
+__device__ __forceinline__ V3 leaf_pos(const GNode* nodes, int64_t leaf) {
+    const int4 q = *(const int4*)(&nodes[leaf]);   // mLev.. | mPos[0..2]
+    return v3((float)q.y, (float)q.z, (float)q.w);
+}
+
+// getGradient RC.cuh:132-142 (sign -1) / getGradientLevelSet RC.cuh:146-157 (+1)
+template <int BR>
+__device__ __forceinline__ V3 gradient(const float* B, int br, V3 p, bool ls) {
+    float lo, hi;
+    V3 g;
+    lo = trilinear<BR>(B, br, v3(p.x - 0.5f, p.y, p.z));
+    hi = trilinear<BR>(B, br, v3(p.x + 0.5f, p.y, p.z));
+    g.x = ls ? hi - lo : lo - hi;
+    lo = trilinear<BR>(B, br, v3(p.x, p.y - 0.5f, p.z));
+    hi = trilinear<BR>(B, br, v3(p.x, p.y + 0.5f, p.z));
+    g.y = ls ? hi - lo : lo - hi;
+    lo = trilinear<BR>(B, br, v3(p.x, p.y, p.z - 0.5f));
+    hi = trilinear<BR>(B, br, v3(p.x, p.y, p.z + 0.5f));
+    g.z = ls ? hi - lo : lo - hi;
+    return normalize(g);
+}
+
+// Brick functions: rayDeepBrick RC.cuh:485-533 (DEEP_*),
+// raySurfaceTrilinearBrick RC.cuh:281-300, rayLevelSetBrick RC.cuh:389-410
+// (its rayLevelSet repeats the caller's test at the same point and returns
+// p + vmin at once).
+template <int MODE, int BR>
+__device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene& sc, int64_t leaf, V3 t, V3 pos,
+                                         V3 dir, RayState& st) {
+    const int br = BR > 0 ? BR : tv.brick_res;
+    const float* B = tv.chan + leaf * (int64_t)(br * br * br);
+    const V3 vmin = leaf_pos(tv.nodes[0], leaf);
+    const float r0 = (float)tv.res[0];
+    if (MODE == MODE_DEEP_LUT || MODE == MODE_DEEP_LINEAR) {
+        const float step = sc.step;
+        t.x = step * ceilf(t.x / step);
+        V3 p = sub(add(pos, scale(dir, t.x)), vmin);
+        const V3 wpt = scale(dir, step);
+        const float dt = sqrtf(dot(wpt, wpt));
+        if (st.hit.x == 0) st.hit.x = t.x;
+        for (int iter = 0; st.clr[3] > sc.alphacut && iter < sc.max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 &&
+                           p.x < r0 && p.y < r0 && p.z < r0;
+             iter++) {
+            const float s = trilinear<BR>(B, br, p);
+            if (s >= sc.minval) {
+                float4 val;
+                if (MODE == MODE_DEEP_LUT)
+                    val = sc.lut[transfer_index(sc, s)];
+                else
+                    val = make_float4(s, s, s, s);
+                val.w = expf(sc.extinct * val.w * step);
+                st.clr[0] += val.x * st.clr[3] * (1 - val.w) * sc.albedo;
+                st.clr[1] += val.y * st.clr[3] * (1 - val.w) * sc.albedo;
+                st.clr[2] += val.z * st.clr[3] * (1 - val.w) * sc.albedo;
+                st.clr[3] *= val.w;
+            }
+            p = add(p, wpt);
+            t.x += dt;
+        }
+        st.hit.y = t.x;
+        if (MODE == MODE_DEEP_LUT) {
+            st.clr[0] = fminf(st.clr[0], 1.f);
+            st.clr[1] = fminf(st.clr[1], 1.f);
+            st.clr[2] = fminf(st.clr[2], 1.f);
+            st.clr[3] = fmaxf(st.clr[3], 0.f);
+        }
+    } else if (MODE == MODE_SURF_TRI) {
+        t.x = sc.step * ceilf(t.x / sc.step);
+        V3 p = sub(add(pos, scale(dir, t.x)), vmin);
+        for (int iter = 0; iter < sc.max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0;
+             iter++) {
+            if (trilinear<BR>(B, br, p) >= sc.thresh) {
+                st.hit = add(p, vmin);
+                st.norm = gradient<BR>(B, br, p, false);
+                return;
+            }
+            p = add(p, scale(dir, sc.step));
+        }
+    } else {
+        V3 p = sub(add(pos, scale(dir, t.x)), vmin);
+        for (int iter = 0;
+             iter < sc.max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x <= r0 && p.y <= r0 && p.z <= r0; iter++) {
+            if (trilinear<BR>(B, br, p) < sc.thresh) {
+                st.hit = add(p, vmin);
+                st.norm = gradient<BR>(B, br, p, true);
+                return;
+            }
+            p = add(p, scale(dir, sc.step));
+        }
+    }
+}
+
+// one segment checkpoint, allocated wave-aggregated (one atomic per wave)
+__device__ __forceinline__ void record_visit(const VisitRec& vr, int64_t leaf, uint32_t ray, float t, float T,
+                                             float I) {
+    const uint64_t act = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(vr.count, (uint32_t)__popcll(act));
+    base = __builtin_amdgcn_readlane(base, leader);
+    const uint64_t slot = (uint64_t)base + __popcll(act & ((1ull << lane) - 1));
+    if ((int64_t)slot < vr.cap) {
+        vr.key[slot] = ((uint64_t)leaf << vr.ray_bits) | ray;
+        vr.t[slot] = t;
+        vr.T[slot] = T;
+        vr.I[slot] = I;
+    }
+}
+
+template <int MODE, int BR, bool RECORD>
+__device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, V3 pos, V3 dir, RayState& st, uint32_t ray,
+                         const VisitRec& vr) {
+    const int top = tv.top_lev;
+    if (top < 1 || top >= tv.levs || tv.ncnt[top] < 1) return;
+    int64_t nodeid[GVDBX_MAXLEV];
+    float tmax[GVDBX_MAXLEV];
+    int lev = top;
+    nodeid[lev] = 0;
+    V3 ts = ray_box(pos, dir, tv.bmin, tv.bmax);
+    uint64_t list = tv.nodes[lev][0].mChildList;
+    V3 vmin = leaf_pos(tv.nodes[lev], 0);
+    if (ts.z == GVDBX_NOHIT) return;
+    ts.x += sc.epsilon;
+    tmax[lev] = ts.y - sc.epsilon;
+    Hdda d;
+    d.set(pos, dir, ts);
+    d.prepare(vmin, tv.vdel[lev]);
+    for (int iter = 0; iter < sc.max_iter && lev > 0 && lev <= top && d.p[0] >= 0 && d.p[1] >= 0 && d.p[2] >= 0 &&
+                       d.p[0] <= tv.res[lev] && d.p[1] <= tv.res[lev] && d.p[2] <= tv.res[lev];
+         iter++) {
+        d.next();
+        const int dim = tv.logdim[lev];
+        const int64_t b = ((((int64_t)d.p[2] << dim) + d.p[1]) << dim) + d.p[0];
+        const int64_t child = get_child(tv, list, b);
+        if (child >= 0 && child < tv.ncnt[lev - 1]) {
+            if (lev == 1) {
+                d.t.x += sc.epsilon;
+                if (RECORD) record_visit(vr, child, ray, d.t.x, st.clr[3], st.clr[0]);
+                brick_fn<MODE, BR>(tv, sc, child, d.t, pos, dir, st);
+                if (st.clr[3] <= 0) {
+                    st.clr[3] = 0;
+                    return;
+                }
+                if (st.hit.z != GVDBX_NOHIT) return;
+                d.step();
+            } else {
+                lev--;
+                nodeid[lev] = child;
+                list = tv.nodes[lev][child].mChildList;
+                vmin = leaf_pos(tv.nodes[lev], child);
+                d.t.x += sc.epsilon;
+                tmax[lev] = d.t.y - sc.epsilon;
+                d.prepare(vmin, tv.vdel[lev]);
+            }
+        } else {
+            d.step();
+        }
+        while (lev <= top && d.t.x > tmax[lev]) {
+            lev++;
+            if (lev <= top) {
+                list = tv.nodes[lev][nodeid[lev]].mChildList;
+                vmin = leaf_pos(tv.nodes[lev], nodeid[lev]);
+                d.prepare(vmin, tv.vdel[lev]);
+            }
+        }
+    }
+}
+
+// 16x16 pixels per workgroup, 8x8 per wave
+__device__ __forceinline__ bool pixel_of(const RenderCam& cam, int& x, int& y) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    return x < cam.width && y < cam.height;
+}
+
+__device__ __forceinline__ uchar4 to_u8(float4 c) {
+    auto q = [](float v) -> unsigned char {
+        float s = v * 255.0f;
+        s = fminf(fmaxf(s, 0.0f), 255.0f);
+        return (unsigned char)(int)s;
+    };
+    return make_uchar4(q(c.x), q(c.y), q(c.z), q(c.w));
+}
+
+// gvdbRayDeep MOD.cu:60-78
+template <int BR>
+__global__ __launch_bounds__(256) void k_render_volume(RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba,
+                                                       uchar4* rgba8) {
+    int x, y;
+    if (!pixel_of(cam, x, y)) return;
+    V3 pos, dir;
+    camera_ray(cam, x, y, pos, dir);
+    RayState st;
+    st.clr[0] = st.clr[1] = st.clr[2] = 0.0f;
+    st.clr[3] = 1.0f;
+    st.hit = v3(0, 0, GVDBX_NOHIT);
+    VisitRec none{};
+    ray_cast<MODE_DEEP_LUT, BR, false>(tv, sc, pos, dir, st, 0, none);
+    const float a = 1.0f - st.clr[3];
+    float4 o;
+    o.x = sc.backclr[0] + a * (st.clr[0] - sc.backclr[0]);
+    o.y = sc.backclr[1] + a * (st.clr[1] - sc.backclr[1]);
+    o.z = sc.backclr[2] + a * (st.clr[2] - sc.backclr[2]);
+    o.w = a;
+    const int64_t px = (int64_t)y * cam.width + x;
+    if (rgba) rgba[px] = o;
+    if (rgba8) rgba8[px] = to_u8(o);
+}
+
+// gvdbRaySurfaceTrilinear / gvdbRayLevelSet MOD.cu:101-181 with
+// performPhongShading MOD.cu:38-57 (no colour channel)
+template <int MODE, int BR>
+__global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba,
+                                                        uchar4* rgba8) {
+    int x, y;
+    if (!pixel_of(cam, x, y)) return;
+    V3 pos, dir;
+    camera_ray(cam, x, y, pos, dir);
+    RayState st;
+    st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
+    st.hit = MODE == MODE_SURF_TRI ? v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT) : v3(0, 0, GVDBX_NOHIT);
+    st.norm = v3(0, 0, 0);
+    VisitRec none{};
+    ray_cast<MODE, BR, false>(tv, sc, pos, dir, st, 0, none);
+    float4 o;
+    if (st.hit.z == GVDBX_NOHIT) {
+        o = make_float4(sc.backclr[0], sc.backclr[1], sc.backclr[2], sc.backclr[3]);
+    } else {
+        const V3 light = v3(sc.light[0], sc.light[1], sc.light[2]);
+        const V3 ldir = normalize(sub(light, st.hit));
+        float diff = (float)(0.9 * (double)fmaxf(0.0f, dot(st.norm, ldir)));
+        const float amb = 0.1f;
+        if (sc.shadow_amt > 0) {
+            RayState s2;
+            s2.clr[0] = s2.clr[1] = s2.clr[2] = 0.0f;
+            s2.clr[3] = 1.0f;
+            s2.hit = v3(0, 0, GVDBX_NOHIT);
+            s2.norm = v3(0, 0, 0);
+            ray_cast<MODE, BR, false>(tv, sc, add(st.hit, scale(st.norm, sc.shadow_bias)), ldir, s2, 0, none);
+            if (s2.hit.z != GVDBX_NOHIT) diff = (float)((double)diff * (1.0 - (double)sc.shadow_amt));
+        }
+        const float sh = diff + amb;
+        o = make_float4(sh, sh, sh, 1.0f);
+    }
+    const int64_t px = (int64_t)y * cam.width + x;
+    if (rgba) rgba[px] = o;
+    if (rgba8) rgba8[px] = to_u8(o);
+}
+
+template <int BR, bool RECORD>
+__global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam, RenderScene sc, float* I, float* T,
+                                                    VisitRec vr) {
+    int x, y;
+    const bool ok = pixel_of(cam, x, y);
+    if (!ok) return;
+    V3 pos, dir;
+    camera_ray(cam, x, y, pos, dir);
+    RayState st;
+    st.clr[0] = st.clr[1] = st.clr[2] = 0.0f;
+    st.clr[3] = 1.0f;
+    st.hit = v3(0, 0, GVDBX_NOHIT);
+    const uint32_t ray = (uint32_t)((int64_t)y * cam.width + x);
+    ray_cast<MODE_DEEP_LINEAR, BR, RECORD>(tv, sc, pos, dir, st, ray, vr);
+    I[ray] = st.clr[0];
+    T[ray] = st.clr[3];
+}
+
+// [begin, end) of each leaf's run in the sorted segment keys
+__global__ void k_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t l = key[i] >> ray_bits;
+        if (i == 0 || (key[i - 1] >> ray_bits) != l) seg[2 * l] = (int32_t)i;
+        if (i == n - 1 || (key[i + 1] >> ray_bits) != l) seg[2 * l + 1] = (int32_t)(i + 1);
+    }
+}
+
+// One wave per leaf: replay every segment of the leaf from its checkpoint
+// and accumulate dL/dvoxel = sum_k q_k * trilinear weight in LDS, with
+//   e = extinct*step, a_k = exp(extinct*s_k*step),
+//   q_k = gI * (albedo T_k ((1-a_k) - e s_k a_k) + e (I_final - I_k)) + gT e T_final
+// (I_k: intensity after sample k).  Lanes take segments in key order, so the
+// LDS accumulation order is fixed.
+template <int BR>
+__global__ __launch_bounds__(64) void k_ea_backward(EABackParams p) {
+    constexpr int NV = BR * BR * BR;
+    __shared__ float dens[NV];
+    __shared__ float grad[NV];
+    const int64_t leaf = blockIdx.x;
+    const int lane = threadIdx.x;
+    const float* src = p.tv.chan + leaf * (int64_t)NV;
+    for (int i = lane; i < NV; i += 64) {
+        dens[i] = src[i];
+        grad[i] = 0.0f;
+    }
+    __syncthreads();
+    const int32_t b0 = p.seg[2 * leaf], b1 = p.seg[2 * leaf + 1];
+    const V3 vmin = leaf_pos(p.tv.nodes[0], leaf);
+    const RenderScene& sc = p.sc;
+    const float r0 = (float)p.tv.res[0];
+    const float e = sc.extinct * sc.step;
+    const uint64_t rmask = (1ull << p.ray_bits) - 1;
+    for (int32_t c = b0 + lane; c < b1; c += 64) {
+        const uint32_t id = p.idx[c];
+        const uint32_t ray = (uint32_t)(p.key[c] & rmask);
+        V3 pos, dir;
+        camera_ray(p.cam, (int)(ray % (uint32_t)p.cam.width), (int)(ray / (uint32_t)p.cam.width), pos, dir);
+        float T = p.vT[id], I = p.vI[id];
+        const float Ifin = p.I[ray], Tfin = p.T[ray], gi = p.gI[ray];
+        const float gt = p.gT ? p.gT[ray] : 0.0f;
+        const float step = sc.step;
+        const float tt = step * ceilf(p.vt[id] / step);
+        V3 q = sub(add(pos, scale(dir, tt)), vmin);
+        const V3 wpt = scale(dir, step);
+        for (int iter = 0; T > sc.alphacut && iter < sc.max_iter && q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < r0 &&
+                           q.y < r0 && q.z < r0;
+             iter++) {
+            const float s = trilinear<BR>(dens, BR, q);
+            if (s >= sc.minval) {
+                const float a = expf(sc.extinct * s * step);
+                I += s * T * (1 - a) * sc.albedo;
+                const float g = gi * (sc.albedo * T * ((1.0f - a) - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
+                T *= a;
+                const float cx = q.x + 0.5f, cy = q.y + 0.5f, cz = q.z + 0.5f;
+                const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+                const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+                const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
+                    const int xi = min(max(ix + dx, 0), BR - 1), yi = min(max(iy + dy, 0), BR - 1),
+                              zi = min(max(iz + dz, 0), BR - 1);
+                    const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dz ? az : 1.0f - az);
+                    atomicAdd(&grad[(zi * BR + yi) * BR + xi], g * w);
+                }
+            }
+            q = add(q, wpt);
+        }
+    }
+    __syncthreads();
+    float* dst = p.grad + leaf * (int64_t)NV;
+    for (int i = lane; i < NV; i += 64) dst[i] = grad[i];
+}
+
+// Adjoint of UpdateApron: interior voxel (x,y,z) of leaf k is copied into the
+// apron of the neighbour in direction (dx,dy,dz) at (x,y,z) - r0*(dx,dy,dz)
+// whenever that lands in the neighbour's apron; gather those gradients in a
+// fixed neighbour order.  Reads only apron entries and writes only interior
+// entries, so it runs in place.
+template <int BR>
+__global__ __launch_bounds__(256) void k_apron_adjoint(ApronParams ap, float* grad) {
+    const int64_t k = blockIdx.x;
+    if (k >= ap.nleaf) return;
+    const int br = BR > 0 ? BR : ap.brick_res;
+    const int r0 = ap.range0;
+    const int a = (br - r0) / 2;
+    __shared__ int32_t nbl[27];
+    const GNode nd = ap.leaves[k];
+    const int cx = nd.mPos[0] >> ap.logr0, cy = nd.mPos[1] >> ap.logr0, cz = nd.mPos[2] >> ap.logr0;
+    if (threadIdx.x < 27) {
+        int d = threadIdx.x;
+        int nx = cx + d % 3 - 1, ny = cy + (d / 3) % 3 - 1, nz = cz + d / 9 - 1;
+        int32_t leaf = -1;
+        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        nbl[d] = (leaf >= 0 && leaf < ap.nleaf) ? leaf : -1;
+    }
+    __syncthreads();
+    const int nv = br * br * br;
+    float* brick = grad + k * (int64_t)nv;
+    for (int i = threadIdx.x; i < r0 * r0 * r0; i += blockDim.x) {
+        const int x = a + i % r0, y = a + (i / r0) % r0, z = a + i / (r0 * r0);
+        const int lx = x == a ? -1 : 0, hx = x == a + r0 - 1 ? 1 : 0;
+        const int ly = y == a ? -1 : 0, hy = y == a + r0 - 1 ? 1 : 0;
+        const int lz = z == a ? -1 : 0, hz = z == a + r0 - 1 ? 1 : 0;
+        if ((lx | hx | ly | hy | lz | hz) == 0) continue;
+        float acc = 0.0f;
+        for (int dz = lz; dz <= hz; dz++)
+            for (int dy = ly; dy <= hy; dy++)
+                for (int dx = lx; dx <= hx; dx++) {
+                    if ((dx | dy | dz) == 0) continue;
+                    const int32_t n = nbl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
+                    if (n < 0) continue;
+                    acc += grad[(int64_t)n * nv + ((int64_t)(z - dz * r0) * br + (y - dy * r0)) * br + (x - dx * r0)];
+                }
+        brick[(z * br + y) * br + x] += acc;
+    }
+}
+
+// Adjoint of the cubic splat, one thread per bin-sorted point:
+//   dL/dw = sum_v g(v) t^3,  dL/dp = sum_v g(v) w 6 t^2 (v - p) / r^2,
+//   t = 1 - |v-p|^2/r^2, over integer voxels v of active leaves, |v-p| < r.
+__global__ __launch_bounds__(256) void k_splat_backward(SplatBackParams p) {
+    const int64_t total = *p.total;
+    const int br = p.brick_res;
+    const int64_t nv = (int64_t)br * br * br;
+    const float r2 = p.radius * p.radius;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total && j < p.nmax;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const float4 q = p.sorted[j];
+        const int i = __float_as_int(q.w);
+        const float wt = p.weights ? *(const float*)(p.weights + (int64_t)i * p.wstride) : 1.0f;
+        const int x0 = max((int)floorf(q.x - p.radius), 0), x1 = (int)ceilf(q.x + p.radius);
+        const int y0 = max((int)floorf(q.y - p.radius), 0), y1 = (int)ceilf(q.y + p.radius);
+        const int z0 = max((int)floorf(q.z - p.radius), 0), z1 = (int)ceilf(q.z + p.radius);
+        float gw = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        for (int z = z0; z <= z1; z++) {
+            const float dz = (float)z - q.z;
+            for (int y = y0; y <= y1; y++) {
+                const float dy = (float)y - q.y;
+                for (int x = x0; x <= x1; x++) {
+                    const float dx = (float)x - q.x;
+                    const float d2 = gvdbx_d2(dx, dy, dz);
+                    if (!(d2 < r2)) continue;
+                    const int cx = x >> p.logr0, cy = y >> p.logr0, cz = z >> p.logr0;
+                    if (!p.g0.contains(cx, cy, cz)) continue;
+                    const int32_t leaf = p.idx0[p.g0.index(cx, cy, cz)];
+                    if (leaf < 0 || leaf >= p.nleaf) continue;
+                    const int lx = x - (cx << p.logr0) + 1, ly = y - (cy << p.logr0) + 1, lz = z - (cz << p.logr0) + 1;
+                    const float g = p.grad[(int64_t)leaf * nv + (lz * br + ly) * br + lx];
+                    const float t = 1.0f - d2 / r2;
+                    gw += g * (t * t * t);
+                    const float f = g * wt * 6.0f * (t * t) / r2;
+                    gx += f * dx;
+                    gy += f * dy;
+                    gz += f * dz;
+                }
+            }
+        }
+        if (p.dw) p.dw[i] = gw;
+        p.dpos[3 * (int64_t)i] = gx;
+        p.dpos[3 * (int64_t)i + 1] = gy;
+        p.dpos[3 * (int64_t)i + 2] = gz;
+    }
+}
+
+inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba, uchar4* rgba8,
+                   hipStream_t s) {
+    dim3 grid((unsigned)((cam.width + 15) / 16), (unsigned)((cam.height + 15) / 16));
+    const bool b10 = tv.brick_res == 10;
+    if (shade == GVDBX_SHADE_VOLUME) {
+        if (b10) hipLaunchKernelGGL(k_render_volume<10>, grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+        else hipLaunchKernelGGL(k_render_volume<0>, grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+    } else if (shade == GVDBX_SHADE_TRILINEAR) {
+        if (b10) hipLaunchKernelGGL((k_render_surface<MODE_SURF_TRI, 10>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+        else hipLaunchKernelGGL((k_render_surface<MODE_SURF_TRI, 0>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+    } else {
+        if (b10) hipLaunchKernelGGL((k_render_surface<MODE_SURF_LS, 10>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+        else hipLaunchKernelGGL((k_render_surface<MODE_SURF_LS, 0>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+    }
+}
+
+void launch_raycast_ea(RenderTree tv, RenderCam cam, RenderScene sc, float* I, float* T, int record, VisitRec vr,
+                       hipStream_t s) {
+    dim3 grid((unsigned)((cam.width + 15) / 16), (unsigned)((cam.height + 15) / 16));
+    const bool b10 = tv.brick_res == 10;
+    if (record) {
+        if (b10) hipLaunchKernelGGL((k_raycast_ea<10, true>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
+        else hipLaunchKernelGGL((k_raycast_ea<0, true>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
+    } else {
+        if (b10) hipLaunchKernelGGL((k_raycast_ea<10, false>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
+        else hipLaunchKernelGGL((k_raycast_ea<0, false>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
+    }
+}
+
+void launch_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf, hipStream_t s) {
+    (void)nleaf;
+    if (n > 0) hipLaunchKernelGGL(k_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, ray_bits, seg);
+}
+
+void launch_ea_backward(EABackParams p, hipStream_t s) {
+    // the padded brick is staged in LDS; only the 10^3 brick (8^3 + apron 1)
+    // has a kernel, checked by the caller
+    if (p.tv.ncnt[0] > 0)
+        hipLaunchKernelGGL(k_ea_backward<10>, dim3((unsigned)p.tv.ncnt[0]), dim3(64), 0, s, p);
+}
+
+void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s) {
+    if (ap.nleaf <= 0) return;
+    if (ap.brick_res == 10)
+        hipLaunchKernelGGL(k_apron_adjoint<10>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap, grad);
+    else
+        hipLaunchKernelGGL(k_apron_adjoint<0>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap, grad);
+}
+
+void launch_splat_backward(SplatBackParams p, hipStream_t s) {
+    if (p.nmax > 0) hipLaunchKernelGGL(k_splat_backward, dim3(grid_for(p.nmax, 256)), dim3(256), 0, s, p);
+}

@@ -1,0 +1,137 @@
+// sort.hip -- stable LSD radix sort of (uint64 key, uint32 value) pairs for
+// gfx950.  Replaces the reference's Thrust sort (cuda_gvdb_copydata.cu:11-16,
+// called from RadixSortByByte VG.cpp:904-928) wherever this library needs a
+// sorted key list.
+//
+// One pass per 8-bit digit, three launches per pass:
+//   1. k_rs_hist: per 4096-key tile, a 256-bin digit histogram in LDS,
+//      written digit-major (hist[d * ntiles + tile]);
+//   2. an exclusive scan over the digit-major histogram gives every
+//      (digit, tile) its output base (the count scan of the topology code);
+//   3. k_rs_scatter: each tile re-reads its keys in 16 rounds of 256 (index
+//      order), ranks equal digits inside each wave with eight ballots, adds
+//      the counts of lower waves and earlier rounds from LDS, and writes the
+//      pair to base + rank.  Equal digits keep their input order: stable.
+// Reads 12 B and writes 12 B per pair per pass plus the histogram traffic.
+#include "gvdbx_kernels.h"
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_BINS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* keys, int64_t n, int shift, int64_t ntiles,
+                                                         int32_t* hist) {
+    __shared__ int32_t cnt[RS_BINS];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t i = t0 + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* keys, const uint32_t* vals, int64_t n,
+                                                            int shift, int64_t ntiles, const int32_t* off,
+                                                            uint64_t* okeys, uint32_t* ovals) {
+    __shared__ int32_t base[RS_BINS];
+    __shared__ int32_t run[RS_BINS];
+    __shared__ int32_t wcnt[RS_WAVES][RS_BINS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    base[tid] = off[(int64_t)tid * ntiles + blockIdx.x];
+    run[tid] = 0;
+    for (int w = 0; w < RS_WAVES; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int64_t i = t0 + r * RS_THREADS + tid;
+        const bool ok = i < n;
+        uint64_t k = 0;
+        uint32_t val = 0;
+        int d = 0;
+        if (ok) {
+            k = keys[i];
+            val = vals[i];
+            d = (int)((k >> shift) & 0xFF);
+        }
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t m = __ballot(ok && ((d >> b) & 1));
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        if (ok && rank == 0) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (ok) {
+            int pos = base[d] + run[d] + rank;
+            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+            okeys[pos] = k;
+            ovals[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int w = 0; w < RS_WAVES; w++) {
+            add += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+__global__ void k_iota_u32(uint32_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+}  // namespace
+
+int64_t radix_hist_entries(int64_t n) {
+    int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    if (ntiles < 1) ntiles = 1;
+    return ntiles * RS_BINS;
+}
+
+void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+}
+
+int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
+                     int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
+    if (n <= 1 || end_bit <= begin_bit) return 0;
+    const int64_t ntiles = radix_hist_entries(n) / RS_BINS;
+    const int64_t nh = ntiles * RS_BINS;
+    const int64_t stiles = scan_tiles(nh + 1);
+    uint64_t* kin = keys;
+    uint32_t* vin = vals;
+    uint64_t* kout = tkeys;
+    uint32_t* vout = tvals;
+    int flips = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, n, shift, ntiles, hist);
+        launch_count_scan_reduce(hist, nh, partials, partials + stiles, s);
+        launch_count_scan_apply(hist, nh, partials, off, s);
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, vin, n, shift, ntiles,
+                           off, kout, vout);
+        uint64_t* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+        flips ^= 1;
+    }
+    return flips;
+}

@@ -106,6 +106,14 @@ struct gvdbx_volume {
     // InsertPoints output
     DevBuf pnode, poff;
     int64_t npnode = 0;
+    // recorded emission-absorption segments (gvdbx_raycast_ea with record)
+    DevBuf vis_key, vis_t, vis_T, vis_I, vis_idx, vis_tkey, vis_tidx, vis_hist, vis_off, vis_part, vis_seg, vis_cnt;
+    int64_t vis_n = 0, vis_cap = 0;
+    int vis_ray_bits = 0, vis_leaf_bits = 0;
+    bool vis_valid = false, vis_sorted = false, vis_in_tmp = false;
+    int vis_w = 0, vis_h = 0, vis_chan = -1;
+    uint64_t vis_topo_gen = 0;
+    uint64_t topo_gen = 0;   // bumped by every RebuildTopology
     // pinned host staging
     void* host = nullptr;
     size_t host_cap = 0;
@@ -355,6 +363,8 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     // validation is done: from here on the previous topology is being replaced
     v->has_topo = false;
     v->has_atlas = false;
+    v->topo_gen++;
+    v->vis_valid = false;
     const bool extra = radius >= 1.0f;   // VG.cpp:1207-1208
     const int r0 = v->res[0];
     const bool mask_ok = 2.0f * radius <= (float)r0 && radius + 1.0f <= (float)r0;
@@ -587,6 +597,63 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     return check_launch(what);
 }
 
+int bits_for(int64_t n) {   // bits to hold values 0..n-1 (at least 1)
+    int b = 1;
+    while (b < 62 && ((int64_t)1 << b) < n) b++;
+    return b;
+}
+
+int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp, const char* what,
+                RenderTree& tv, RenderCam& rc, RenderScene& sc) {
+    TRY(need_atlas(v, what));
+    TRY(check_chan(v, chan));
+    if (!cam || !rp) FAIL(GVDBX_ERR_ARG, "%s: null camera or parameters", what);
+    if (v->apron != 1) FAIL(GVDBX_ERR_ARG, "%s: rendering needs apron 1 (AddChannel)", what);
+    if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height >= ((int64_t)1 << 31))
+        FAIL(GVDBX_ERR_ARG, "%s: bad image size %dx%d", what, cam->width, cam->height);
+    if (!(rp->step > 0.0f) || !isfinite(rp->step)) FAIL(GVDBX_ERR_ARG, "%s: step must be > 0", what);
+    if (rp->max_iter < 0) FAIL(GVDBX_ERR_ARG, "%s: negative max_iter", what);
+    memset(&tv, 0, sizeof(tv));
+    for (int l = 0; l < v->levs && l < GVDBX_MAXLEV; l++) {
+        tv.nodes[l] = v->nodes[l].as<GNode>();
+        tv.clist[l] = v->clist[l].as<uint64_t>();
+        tv.ncnt[l] = v->cnt[l];
+        tv.logdim[l] = v->logdim[l];
+        tv.res[l] = v->res[l];
+        tv.vdel[l] = l > 0 ? (float)v->range[l - 1] : 1.0f;
+    }
+    tv.levs = v->levs;
+    tv.top_lev = v->top_lev;
+    memcpy(tv.bmin, v->obj_min, sizeof(tv.bmin));
+    memcpy(tv.bmax, v->obj_max, sizeof(tv.bmax));
+    tv.chan = v->chan[chan].as<float>();
+    tv.brick_res = brick_res(v);
+    rc.width = cam->width;
+    rc.height = cam->height;
+    memcpy(rc.org, cam->org, sizeof(rc.org));
+    memcpy(rc.org_u, cam->org_u, sizeof(rc.org_u));
+    memcpy(rc.org_v, cam->org_v, sizeof(rc.org_v));
+    memcpy(rc.dir, cam->dir, sizeof(rc.dir));
+    memcpy(rc.dir_u, cam->dir_u, sizeof(rc.dir_u));
+    memcpy(rc.dir_v, cam->dir_v, sizeof(rc.dir_v));
+    sc.step = rp->step;
+    sc.extinct = rp->extinct;
+    sc.albedo = rp->albedo;
+    sc.minval = rp->minval;
+    sc.alphacut = rp->alphacut;
+    sc.thresh = rp->thresh;
+    sc.vmin = rp->vmin;
+    sc.vmax = rp->vmax;
+    sc.epsilon = rp->epsilon;
+    sc.max_iter = rp->max_iter;
+    memcpy(sc.backclr, rp->backclr, sizeof(sc.backclr));
+    memcpy(sc.light, rp->light, sizeof(sc.light));
+    sc.shadow_amt = rp->shadow_amt;
+    sc.shadow_bias = rp->shadow_bias;
+    sc.lut = (const float4*)rp->lut_dev;
+    return GVDBX_OK;
+}
+
 }  // namespace
 
 // =====================================================================
@@ -630,7 +697,9 @@ int gvdbx_destroy(gvdbx_volume* v) {
         free_buf(v->flag_all[l]);
     }
     DevBuf* bufs[] = {&v->reach0, &v->leafbits, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
-                      &v->bin_cnt, &v->bin_off, &v->pslot, &v->sorted, &v->sorted_tmp, &v->pnode, &v->poff};
+                      &v->bin_cnt, &v->bin_off, &v->pslot, &v->sorted, &v->sorted_tmp, &v->pnode, &v->poff,
+                      &v->vis_key, &v->vis_t, &v->vis_T, &v->vis_I, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
+                      &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt};
     for (DevBuf* b : bufs) free_buf(*b);
     if (v->host) (void)hipHostFree(v->host);
     delete v;
@@ -904,6 +973,181 @@ int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
                    v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval};
     launch_apron(ap, v->stream);
     return check_launch("UpdateApron");
+}
+
+int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                 float* rgba_dev, uint8_t* rgba8_dev) {
+    TRY(check_vol(v));
+    RenderTree tv;
+    RenderCam rc;
+    RenderScene sc;
+    TRY(make_render(v, chan, cam, rp, "Render", tv, rc, sc));
+    if (shade != GVDBX_SHADE_VOLUME && shade != GVDBX_SHADE_TRILINEAR && shade != GVDBX_SHADE_LEVELSET)
+        FAIL(GVDBX_ERR_ARG, "Render: shading %d is not supported (VOLUME, TRILINEAR, LEVELSET)", shade);
+    if (shade == GVDBX_SHADE_VOLUME && !rp->lut_dev) FAIL(GVDBX_ERR_ARG, "Render: SHADE_VOLUME needs a transfer function");
+    if (((uintptr_t)rp->lut_dev & 15) != 0) FAIL(GVDBX_ERR_ARG, "Render: transfer function must be 16-byte aligned");
+    if (!rgba_dev && !rgba8_dev) return GVDBX_OK;
+    launch_render(shade, tv, rc, sc, (float4*)rgba_dev, (uchar4*)rgba8_dev, v->stream);
+    return check_launch("Render");
+}
+
+int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                     float* I_dev, float* T_dev, int record) {
+    TRY(check_vol(v));
+    RenderTree tv;
+    RenderCam rc;
+    RenderScene sc;
+    TRY(make_render(v, chan, cam, rp, "RaycastEA", tv, rc, sc));
+    if (!I_dev || !T_dev) FAIL(GVDBX_ERR_ARG, "RaycastEA: null output");
+    hipStream_t s = v->stream;
+    VisitRec vr{};
+    if (!record) {
+        launch_raycast_ea(tv, rc, sc, I_dev, T_dev, 0, vr, s);
+        return check_launch("RaycastEA");
+    }
+    const int64_t rays = (int64_t)cam->width * cam->height;
+    const int rb = bits_for(rays), lb = bits_for(v->cnt[0]);
+    if (rb + lb > 63) FAIL(GVDBX_ERR_RANGE, "RaycastEA: too many rays x leaves for 64-bit segment keys");
+    v->vis_valid = false;
+    int64_t cap = std::max<int64_t>(v->vis_cap, std::max<int64_t>(rays * 8, 1 << 20));
+    TRY(ensure(v, v->vis_cnt, 16));
+    TRY(ensure_host(v, 4096));
+    for (int attempt = 0; attempt < 2; attempt++) {
+        TRY(ensure(v, v->vis_key, (size_t)cap * 8));
+        TRY(ensure(v, v->vis_t, (size_t)cap * 4));
+        TRY(ensure(v, v->vis_T, (size_t)cap * 4));
+        TRY(ensure(v, v->vis_I, (size_t)cap * 4));
+        launch_fill_i32(v->vis_cnt.as<int32_t>(), 1, 0, s);
+        vr.key = v->vis_key.as<uint64_t>();
+        vr.t = v->vis_t.as<float>();
+        vr.T = v->vis_T.as<float>();
+        vr.I = v->vis_I.as<float>();
+        vr.count = v->vis_cnt.as<uint32_t>();
+        vr.cap = cap;
+        vr.ray_bits = rb;
+        launch_raycast_ea(tv, rc, sc, I_dev, T_dev, 1, vr, s);
+        TRY(check_launch("RaycastEA"));
+        GVDBX_CHECK(hipMemcpyAsync(v->host, v->vis_cnt.p, 4, hipMemcpyDeviceToHost, s));
+        TRY(sync(v));
+        uint32_t n = *(uint32_t*)v->host;
+        if ((int64_t)n <= cap) {
+            v->vis_n = n;
+            v->vis_cap = cap;
+            v->vis_ray_bits = rb;
+            v->vis_leaf_bits = lb;
+            v->vis_w = cam->width;
+            v->vis_h = cam->height;
+            v->vis_chan = chan;
+            v->vis_topo_gen = v->topo_gen;
+            v->vis_valid = true;
+            v->vis_sorted = false;
+            return GVDBX_OK;
+        }
+        cap = (int64_t)n + (int64_t)n / 8 + 1024;   // the segment count of this view; rerun once
+    }
+    FAIL(GVDBX_ERR_STATE, "RaycastEA: segment count did not settle");
+}
+
+int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                              const float* I_dev, const float* T_dev, const float* gI_dev, const float* gT_dev,
+                              float* grad_dev) {
+    TRY(check_vol(v));
+    RenderTree tv;
+    RenderCam rc;
+    RenderScene sc;
+    TRY(make_render(v, chan, cam, rp, "RaycastEABackward", tv, rc, sc));
+    if (!I_dev || !T_dev || !gI_dev || !grad_dev) FAIL(GVDBX_ERR_ARG, "RaycastEABackward: null buffer");
+    if (!v->vis_valid || v->vis_topo_gen != v->topo_gen || v->vis_w != cam->width || v->vis_h != cam->height ||
+        v->vis_chan != chan)
+        FAIL(GVDBX_ERR_STATE, "RaycastEABackward: no recorded RaycastEA for this view and topology");
+    if (brick_res(v) != 10) FAIL(GVDBX_ERR_ARG, "RaycastEABackward: needs 8^3 bricks with apron 1");
+    hipStream_t s = v->stream;
+    const int64_t n = v->vis_n, nleaf = v->cnt[0];
+    if (!v->vis_sorted) {
+        const int64_t nh = radix_hist_entries(n);
+        TRY(ensure(v, v->vis_idx, (size_t)std::max<int64_t>(n, 1) * 4));
+        TRY(ensure(v, v->vis_tkey, (size_t)std::max<int64_t>(n, 1) * 8));
+        TRY(ensure(v, v->vis_tidx, (size_t)std::max<int64_t>(n, 1) * 4));
+        TRY(ensure(v, v->vis_hist, (size_t)nh * 4));
+        TRY(ensure(v, v->vis_off, (size_t)(nh + 1) * 4));
+        TRY(ensure(v, v->vis_part, (size_t)(scan_tiles(nh + 1) + 1) * 4));
+        launch_iota_u32(v->vis_idx.as<uint32_t>(), n, s);
+        int flip = radix_sort_pairs(v->vis_key.as<uint64_t>(), v->vis_idx.as<uint32_t>(), v->vis_tkey.as<uint64_t>(),
+                                    v->vis_tidx.as<uint32_t>(), n, 0, v->vis_ray_bits + v->vis_leaf_bits,
+                                    v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(), v->vis_part.as<uint32_t>(), s);
+        TRY(check_launch("segment sort"));
+        v->vis_in_tmp = flip != 0;
+        TRY(ensure(v, v->vis_seg, (size_t)std::max<int64_t>(2 * nleaf, 2) * 4));
+        launch_fill_i32(v->vis_seg.as<int32_t>(), 2 * nleaf, 0, s);
+        const uint64_t* skey = v->vis_in_tmp ? v->vis_tkey.as<uint64_t>() : v->vis_key.as<uint64_t>();
+        launch_segments(skey, n, v->vis_ray_bits, v->vis_seg.as<int32_t>(), nleaf, s);
+        TRY(check_launch("segment ranges"));
+        v->vis_sorted = true;
+    }
+    EABackParams p{};
+    p.tv = tv;
+    p.cam = rc;
+    p.sc = sc;
+    p.key = v->vis_in_tmp ? v->vis_tkey.as<uint64_t>() : v->vis_key.as<uint64_t>();
+    p.idx = v->vis_in_tmp ? v->vis_tidx.as<uint32_t>() : v->vis_idx.as<uint32_t>();
+    p.vt = v->vis_t.as<float>();
+    p.vT = v->vis_T.as<float>();
+    p.vI = v->vis_I.as<float>();
+    p.seg = v->vis_seg.as<int32_t>();
+    p.I = I_dev;
+    p.T = T_dev;
+    p.gI = gI_dev;
+    p.gT = gT_dev;
+    p.grad = grad_dev;
+    p.ray_bits = v->vis_ray_bits;
+    launch_ea_backward(p, s);
+    return check_launch("RaycastEABackward");
+}
+
+int gvdbx_apron_adjoint(gvdbx_volume* v, float* grad_dev) {
+    TRY(check_vol(v));
+    TRY(need_atlas(v, "ApronAdjoint"));
+    if (!grad_dev) FAIL(GVDBX_ERR_ARG, "ApronAdjoint: null gradient");
+    if (v->apron != 1) FAIL(GVDBX_ERR_ARG, "ApronAdjoint: needs apron 1");
+    ApronParams ap{v->nodes[0].as<GNode>(), v->cnt[0], v->grid[0], v->idx[0].as<int32_t>(), v->res[0],
+                   v->logdim[0], nullptr, brick_res(v), 0.0f};
+    launch_apron_adjoint(ap, grad_dev, v->stream);
+    return check_launch("ApronAdjoint");
+}
+
+int gvdbx_splat_cubic_backward(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz,
+                               const float* grad_dev, float* dpos_dev, float* dw_dev) {
+    TRY(check_vol(v));
+    TRY(need_atlas(v, "SplatCubicBackward"));
+    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "SplatCubicBackward: negative point count");
+    if (!grad_dev || (!dpos_dev && num_pnts > 0)) FAIL(GVDBX_ERR_ARG, "SplatCubicBackward: null buffer");
+    if (!(radius > 0.0f) || radius > (float)v->res[0])
+        FAIL(GVDBX_ERR_RANGE, "SplatCubicBackward: radius must be in (0, brick]");
+    if (v->apron < 1) FAIL(GVDBX_ERR_ARG, "SplatCubicBackward: needs an apron");
+    if (num_pnts == 0) return GVDBX_OK;
+    float t[3] = {tx, ty, tz};
+    TRY(ensure_bins(v, num_pnts, t, false));
+    hipStream_t s = v->stream;
+    launch_fill_f32(dpos_dev, 3 * (int64_t)num_pnts, 0.0f, s);
+    if (dw_dev) launch_fill_f32(dw_dev, num_pnts, 0.0f, s);
+    SplatBackParams p{};
+    p.g0 = v->grid[0];
+    p.idx0 = v->idx[0].as<int32_t>();
+    p.leaves = v->nodes[0].as<GNode>();
+    p.nleaf = v->cnt[0];
+    p.logr0 = v->logdim[0];
+    p.brick_res = brick_res(v);
+    p.sorted = v->sorted.as<float4>();
+    p.total = v->bin_off.as<int32_t>() + v->gbin.count();
+    p.nmax = num_pnts;
+    p.weights = v->wts;
+    p.wstride = v->wstride;
+    p.radius = radius;
+    p.grad = grad_dev;
+    p.dpos = dpos_dev;
+    p.dw = dw_dev;
+    launch_splat_backward(p, s);
+    return check_launch("SplatCubicBackward");
 }
 
 int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out) {

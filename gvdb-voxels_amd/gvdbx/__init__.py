@@ -4,14 +4,16 @@
 (source/gvdb_library/src/gvdb_volume_gvdb.h:319-901) on the points path:
 Configure, AddChannel, SetPoints, RebuildTopology, FinishTopology,
 UpdateAtlas, InsertPointsSubcell, GatherDensity, GatherLevelSet,
-InsertPoints, ScatterDensity, UpdateApron.  Every call goes through the
-C-ABI of libgvdbx.so; there is no CPU fallback.
+InsertPoints, ScatterDensity, UpdateApron, Render, plus the extensions
+(weighted cubic splat, emission-absorption raycast and their adjoints).
+Every call goes through the C-ABI of libgvdbx.so; there is no CPU fallback.
 """
 import ctypes as C
 
 import numpy as np
 
-from ._lib import GvdbxError, Info, TOPO_AUTO, TOPO_DENSE, TOPO_RADIX, call, exported_names, lib, last_error  # noqa: F401
+from ._lib import (Camera, GvdbxError, Info, RenderParams, TOPO_AUTO, TOPO_DENSE, TOPO_RADIX,  # noqa: F401
+                   call, exported_names, lib, last_error)
 
 NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("pad", "u1"),
                        ("mPos", "<i4", 3), ("mValue", "<i4", 3), ("mVRange", "<f4", 3),
@@ -19,6 +21,63 @@ NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("
 AMAP_DTYPE = np.dtype([("mPos", "<i4", 3), ("mLeafID", "<i4")])
 T_FLOAT = 0
 F_LINEAR, F_CLAMP = 1, 1
+SHADE_TRILINEAR, SHADE_LEVELSET, SHADE_VOLUME = 4, 6, 7   # gvdb_types.h:109-112
+
+
+def _v3(x):
+    return (C.c_float * 3)(*[float(a) for a in x])
+
+
+def camera(width, height, org=(0, 0, 0), org_u=(0, 0, 0), org_v=(0, 0, 0), dir=(0, 0, 1), dir_u=(0, 0, 0),
+           dir_v=(0, 0, 0)):
+    """gvdbx_camera from its six vectors (index space)."""
+    c = Camera()
+    c.width, c.height = int(width), int(height)
+    c.org, c.org_u, c.org_v = _v3(org), _v3(org_u), _v3(org_v)
+    c.dir, c.dir_u, c.dir_v = _v3(dir), _v3(dir_u), _v3(dir_v)
+    return c
+
+
+def camera_ortho(width, height, corner, extent_u, extent_v, direction):
+    """Orthographic camera: pixel centres span corner + [0,1)*extent_u + [0,1)*extent_v."""
+    return camera(width, height, org=corner, org_u=extent_u, org_v=extent_v, dir=direction)
+
+
+def camera_perspective(width, height, campos, cams, camu, camv):
+    """PrepareRender's camera (VG.cpp:4260-4265): rays from campos through
+    cams + u*camu + w*camv (top-left ray and the two image edges)."""
+    return camera(width, height, org=campos, dir=cams, dir_u=camu, dir_v=camv)
+
+
+def render_params(**kw):
+    """gvdbx_render_params with the Scene defaults (gvdb_scene.cpp:46-53,
+    VG.cpp:71, RC.cuh:24) for every field not given.  lut: a device float32
+    tensor of 16384*4 entries (SHADE_VOLUME)."""
+    p = RenderParams()
+    p.step = float(kw.get("step", 1.0))
+    p.extinct = float(kw.get("extinct", -1.1))
+    p.albedo = float(kw.get("albedo", 1.5))
+    p.minval = float(kw.get("minval", 0.005))
+    p.alphacut = float(kw.get("alphacut", 0.01))
+    p.thresh = float(kw.get("thresh", 0.1))
+    p.vmin = float(kw.get("vmin", 0.0))
+    p.vmax = float(kw.get("vmax", 1.0))
+    p.epsilon = float(kw.get("epsilon", 0.001))
+    p.max_iter = int(kw.get("max_iter", 256))
+    p.backclr = (C.c_float * 4)(*[float(a) for a in kw.get("backclr", (0, 0, 0, 0))])
+    p.light = _v3(kw.get("light", (0, 0, 0)))
+    p.shadow_amt = float(kw.get("shadow_amt", 0.8))
+    p.shadow_bias = float(kw.get("shadow_bias", 1.0))
+    lut = kw.get("lut")
+    p._lut = lut
+    p.lut_dev = _ptr(lut) if lut is not None else None
+    return p
+
+
+def ea_params(step=0.5, kappa=0.05, albedo=1.0, **kw):
+    """Parameters of the differentiable emission-absorption raycast: extinction
+    kappa per unit density and length, no minval test, no alpha cut."""
+    return render_params(step=step, extinct=-kappa, albedo=albedo, minval=float("-inf"), alphacut=0.0, **kw)
 
 
 def _ptr(x):
@@ -140,6 +199,58 @@ class VolumeGVDB:
 
     def UpdateApron(self, chan=0, boundval=0.0):
         call("gvdbx_update_apron", self._h, int(chan), float(boundval))
+
+    # ------------------------------------------------------------ rendering
+    def Render(self, shade, chan, cam, rp, rgba=None, rgba8=None):
+        """Render(shade, chan, buf) (VG.cpp:4336-4381).  Returns (rgba float32
+        (H,W,4), rgba8 uint8 (H,W,4)) device tensors."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if rgba is None:
+            rgba = torch.empty((cam.height, cam.width, 4), dtype=torch.float32, device=dev)
+        if rgba8 is None:
+            rgba8 = torch.empty((cam.height, cam.width, 4), dtype=torch.uint8, device=dev)
+        call("gvdbx_render", self._h, int(shade), int(chan), C.byref(cam), C.byref(rp), C.c_void_p(_ptr(rgba)),
+             C.c_void_p(_ptr(rgba8)))
+        return rgba, rgba8
+
+    def RaycastEA(self, chan, cam, rp, I=None, T=None, record=False):
+        """Linear-transfer emission-absorption raycast; (I, T) float32 (H,W)."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if I is None:
+            I = torch.empty((cam.height, cam.width), dtype=torch.float32, device=dev)
+        if T is None:
+            T = torch.empty((cam.height, cam.width), dtype=torch.float32, device=dev)
+        call("gvdbx_raycast_ea", self._h, int(chan), C.byref(cam), C.byref(rp), C.c_void_p(_ptr(I)),
+             C.c_void_p(_ptr(T)), int(record))
+        return I, T
+
+    def RaycastEABackward(self, chan, cam, rp, I, T, gI, gT=None, grad=None):
+        """d(sum gI*I + gT*T)/d voxel over the padded bricks, (B, br, br, br)."""
+        import torch
+        inf = self.info()
+        br, B = inf.brick_res, inf.node_cnt[0]
+        if grad is None:
+            grad = torch.empty((B, br, br, br), dtype=torch.float32, device=I.device)
+        call("gvdbx_raycast_ea_backward", self._h, int(chan), C.byref(cam), C.byref(rp), C.c_void_p(_ptr(I)),
+             C.c_void_p(_ptr(T)), C.c_void_p(_ptr(gI)), C.c_void_p(_ptr(gT)) if gT is not None else None,
+             C.c_void_p(_ptr(grad)))
+        return grad
+
+    def ApronAdjoint(self, grad):
+        call("gvdbx_apron_adjoint", self._h, C.c_void_p(_ptr(grad)))
+        return grad
+
+    def SplatCubicBackward(self, num_pnts, radius, trans, grad, dpos=None, dw=None, want_dw=True):
+        import torch
+        if dpos is None:
+            dpos = torch.empty((num_pnts, 3), dtype=torch.float32, device=grad.device)
+        if dw is None and want_dw:
+            dw = torch.empty((num_pnts,), dtype=torch.float32, device=grad.device)
+        call("gvdbx_splat_cubic_backward", self._h, int(num_pnts), float(radius), *[float(x) for x in trans],
+             C.c_void_p(_ptr(grad)), C.c_void_p(_ptr(dpos)), C.c_void_p(_ptr(dw)) if dw is not None else None)
+        return dpos, dw
 
     # ------------------------------------------------------------ readback
     def info(self):

@@ -26,6 +26,22 @@ class Info(C.Structure):
                 ("obj_max", C.c_float * 3), ("sc_len", C.c_int64), ("topology_mode_used", C.c_int)]
 
 
+class Camera(C.Structure):
+    """gvdbx_camera: pixel (x,y), u=(x+0.5)/width, w=(y+0.5)/height, ray
+    origin (u*org_u + w*org_v) + org, direction normalize((u*dir_u + w*dir_v) + dir)."""
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("org", C.c_float * 3), ("org_u", C.c_float * 3),
+                ("org_v", C.c_float * 3), ("dir", C.c_float * 3), ("dir_u", C.c_float * 3),
+                ("dir_v", C.c_float * 3)]
+
+
+class RenderParams(C.Structure):
+    """gvdbx_render_params (ScnInfo fields of PrepareRender VG.cpp:4254-4306)."""
+    _fields_ = [("step", C.c_float), ("extinct", C.c_float), ("albedo", C.c_float), ("minval", C.c_float),
+                ("alphacut", C.c_float), ("thresh", C.c_float), ("vmin", C.c_float), ("vmax", C.c_float),
+                ("epsilon", C.c_float), ("max_iter", C.c_int), ("backclr", C.c_float * 4), ("light", C.c_float * 3),
+                ("shadow_amt", C.c_float), ("shadow_bias", C.c_float), ("lut_dev", C.c_void_p)]
+
+
 # name -> argtypes (all return int status unless listed in _RESTYPES)
 _vp, _i, _i64, _f, _u64 = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_uint64
 _SIGS = {
@@ -52,6 +68,11 @@ _SIGS = {
     "gvdbx_scatter_density": [_vp, _i, _f, _f, _f, _f, _f, _i, _i],
     "gvdbx_splat_cubic": [_vp, _i, _f, _f, _f, _f, _i, _i],
     "gvdbx_update_apron": [_vp, _i, _f],
+    "gvdbx_render": [_vp, _i, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp],
+    "gvdbx_raycast_ea": [_vp, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp, _i],
+    "gvdbx_raycast_ea_backward": [_vp, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp, _vp, _vp, _vp],
+    "gvdbx_apron_adjoint": [_vp, _vp],
+    "gvdbx_splat_cubic_backward": [_vp, _i, _f, _f, _f, _f, _vp, _vp, _vp],
     "gvdbx_get_info": [_vp, C.POINTER(Info)],
     "gvdbx_read_nodes": [_vp, _i, _vp, _i64],
     "gvdbx_read_child_lists": [_vp, _i, _vp, _i64],

@@ -130,6 +130,72 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
 /* UpdateApron(uchar, float, bool) VG.h:408 / VG.cpp:4418-4457 */
 int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
 
+/* ---- rendering (RC.cuh, DDA.cuh, GEO.cuh, MOD.cu) ---- */
+enum gvdbx_shade {
+    GVDBX_SHADE_TRILINEAR = 4,   /* gvdb_types.h:109 */
+    GVDBX_SHADE_LEVELSET = 6,    /* gvdb_types.h:111 */
+    GVDBX_SHADE_VOLUME = 7       /* gvdb_types.h:112 */
+};
+
+/* Camera in index space.  Pixel (x,y), u = (x+0.5)/width, w = (y+0.5)/height:
+ *   origin    = (u*org_u + w*org_v) + org
+ *   direction = normalize((u*dir_u + w*dir_v) + dir)
+ * PrepareRender's perspective camera (VG.cpp:4260-4265; GEO.cuh:48-63) is
+ * org = campos, org_u = org_v = 0, dir = cams, dir_u = camu, dir_v = camv;
+ * an orthographic camera has dir_u = dir_v = 0. */
+typedef struct {
+    int width, height;
+    float org[3], org_u[3], org_v[3];
+    float dir[3], dir_u[3], dir_v[3];
+} gvdbx_camera;
+
+/* Scene parameters of ScnInfo (PrepareRender VG.cpp:4254-4306); defaults of
+ * Scene::Scene (gvdb_scene.cpp:46-53) in brackets. */
+typedef struct {
+    float step;          /* SCN_DIRECTSTEP [1.0] */
+    float extinct;       /* SCN_EXTINCT [-1.1]: a = exp(extinct * sigma * step) */
+    float albedo;        /* SCN_ALBEDO [1.5] */
+    float minval;        /* SCN_MINVAL [0.005] */
+    float alphacut;      /* SCN_ALPHACUT [0.01] */
+    float thresh, vmin, vmax;   /* SCN_THRESH, SCN_VMIN, SCN_VMAX [0.1, 0, 1] */
+    float epsilon;       /* VDBInfo.epsilon [0.001] (VG.cpp:71) */
+    int max_iter;        /* MAX_ITER [256] (RC.cuh:24) */
+    float backclr[4];    /* SCN_BACKCLR */
+    float light[3];      /* light position, index space */
+    float shadow_amt, shadow_bias;   /* SCN_SHADOWAMT, SCN_SHADOWBIAS [0.8, 1.0] */
+    const float* lut_dev;   /* transfer function: 16384 float4 in device memory (SHADE_VOLUME) */
+} gvdbx_render_params;
+
+/* Render(char shade, uchar chan, uchar buf) VG.h:341 / VG.cpp:4336-4381 with
+ * ReadRenderBuf: gvdbRayDeep (SHADE_VOLUME), gvdbRaySurfaceTrilinear,
+ * gvdbRayLevelSet (MOD.cu:60-181).  rgba_dev (optional): 4 floats per pixel,
+ * the colour before quantisation; rgba8_dev (optional): the reference's
+ * uchar4 output.  Rows of width pixels, y-major. */
+int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                 float* rgba_dev, uint8_t* rgba8_dev);
+/* Emission-absorption raycast with the linear transfer s -> (s,s,s,s)
+ * (extension for differentiation; same traversal and sample set as
+ * SHADE_VOLUME, rayDeepBrick RC.cuh:485-533, no final clamp):
+ *   a = exp(extinct*s*step); I += s*T*(1-a)*albedo; T *= a.
+ * I_dev, T_dev: one float per pixel.  record != 0 keeps, per (ray, brick)
+ * segment, the entry t, T and I for gvdbx_raycast_ea_backward. */
+int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                     float* I_dev, float* T_dev, int record);
+/* Adjoint of the last recorded gvdbx_raycast_ea (same chan, cam, rp, I, T):
+ * grad_dev (leaf-major padded bricks, nleaf * brick_res^3 floats) is
+ * overwritten with d(sum gI*I + gT*T)/d voxel.  gT_dev may be NULL. */
+int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                              const float* I_dev, const float* T_dev, const float* gI_dev, const float* gT_dev,
+                              float* grad_dev);
+/* Adjoint of UpdateApron: each apron gradient is added to the interior voxel
+ * it copies (in place; apron entries are left as they are). */
+int gvdbx_apron_adjoint(gvdbx_volume* v, float* grad_dev);
+/* Adjoint of gvdbx_splat_cubic: from the interior voxel gradient grad_dev
+ * (padded layout), dpos_dev (3 floats per point) and dw_dev (one per point,
+ * may be NULL) for the points of the last SetPoints/SetPointWeights. */
+int gvdbx_splat_cubic_backward(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz,
+                               const float* grad_dev, float* dpos_dev, float* dw_dev);
+
 /* ---- readback (host copies; synchronising) ---- */
 int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out);
 /* pool group 0 of one level: 64-byte records (VG.h getNode / AL.cpp PoolFetch) */

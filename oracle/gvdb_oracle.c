@@ -1046,8 +1046,767 @@ int or_read_brick(or_volume* v, int chan, int64_t leaf, float* out) {
     return OR_OK;
 }
 
-int or_raycast_ea(or_volume* v, int chan, const or_render_params* rp, int64_t nrays, const float* origins,
-                  const float* dirs, float* out_rgba, double* out_f64) {
-    (void)v; (void)chan; (void)rp; (void)nrays; (void)origins; (void)dirs; (void)out_rgba; (void)out_f64;
-    return OR_ERR_STATE;
+
+/* ------------------------------------------------------------------ */
+/* raycast: HDDA traversal + brick functions                           */
+/* ------------------------------------------------------------------ */
+/*
+ * Every float expression below is written in the reference's operand order
+ * (cuda_math.cuh operators, evaluated left to right, no contraction):
+ *   dot(a,b) = a.x*b.x + a.y*b.y + a.z*b.z;  normalize(v) = v * (1/sqrtf(dot(v,v)))
+ *   (reference: rsqrtf, an approximation; here the correctly rounded form).
+ * Trilinear sampling (reference: hardware tex3D with 8-bit weights) is the
+ * exact FP32 interpolation of the brick's padded voxels, voxel i centred at
+ * i + 0.5 in brick-local coordinates (RC.cuh:104-108 texel convention):
+ *   c = p + 0.5, i = floor(c), a = c - i, x then y then z,
+ *   lerp(u, w, a) = u*(1-a) + w*a; indices clamped to the padded brick.
+ * D7: an HDDA axis whose direction component is zero never steps (tSide =
+ *     +inf).  The reference computes mask*tDel = 0*inf = NaN there
+ *     (DDA.cuh:60-89), which stalls every ray with a zero component; for all
+ *     other rays the two are identical.
+ */
+
+typedef struct { float x, y, z; } f3;
+static f3 f3m(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+static f3 f3add(f3 a, f3 b) { return f3m(a.x + b.x, a.y + b.y, a.z + b.z); }
+static f3 f3sub(f3 a, f3 b) { return f3m(a.x - b.x, a.y - b.y, a.z - b.z); }
+static f3 f3scale(f3 a, float s) { return f3m(a.x * s, a.y * s, a.z * s); }
+static float f3dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static f3 f3norm(f3 a) { return f3scale(a, 1.0f / sqrtf(f3dot(a, a))); }
+
+enum { M_DEEP_LUT = 0, M_DEEP_LINEAR = 1, M_SURF_TRI = 2, M_SURF_LS = 3 };
+
+typedef struct {
+    int64_t leaf;
+    f3 p;
+    int lut;
+} or_sample;
+
+typedef struct {
+    const or_volume* v;
+    const float* A;
+    const double* B64;
+    const or_render_params* rp;
+    int mode;
+    /* per ray */
+    f3 hit, norm;
+    float clr[4];
+    double clr64[4];
+    int64_t hit_leaf;
+    f3 hit_p;
+    /* optional sample recording (EA adjoint) */
+    or_sample* rec;
+    int64_t nrec, caprec;
+} ray_ctx;
+
+static float tri_at32(const ray_ctx* c, const or_node* nd, int i, int j, int k) {
+    int br = brick_res(c->v);
+    i = i < 0 ? 0 : (i >= br ? br - 1 : i);
+    j = j < 0 ? 0 : (j >= br ? br - 1 : j);
+    k = k < 0 ? 0 : (k >= br ? br - 1 : k);
+    int a = c->v->apron;
+    return c->A[atlas_index(c->v, (int64_t)nd->mValue[0] - a + i, (int64_t)nd->mValue[1] - a + j,
+                            (int64_t)nd->mValue[2] - a + k)];
+}
+
+static double tri_at64(const ray_ctx* c, int64_t leaf, int i, int j, int k) {
+    int br = brick_res(c->v);
+    i = i < 0 ? 0 : (i >= br ? br - 1 : i);
+    j = j < 0 ? 0 : (j >= br ? br - 1 : j);
+    k = k < 0 ? 0 : (k >= br ? br - 1 : k);
+    if (c->B64) return c->B64[leaf * ipow3(br) + ((int64_t)k * br + j) * br + i];
+    return (double)tri_at32(c, &c->v->nodes[0][leaf], i, j, k);
+}
+
+typedef struct {
+    int i[3];
+    float a[3];
+} tri_cell;
+
+static tri_cell tri_locate(f3 p) {
+    tri_cell t;
+    float q[3] = {p.x + 0.5f, p.y + 0.5f, p.z + 0.5f};
+    for (int d = 0; d < 3; d++) {
+        float f = floorf(q[d]);
+        t.i[d] = (int)f;
+        t.a[d] = q[d] - f;
+    }
+    return t;
+}
+
+static float tri32(const ray_ctx* c, int64_t leaf, f3 p) {
+    const or_node* nd = &c->v->nodes[0][leaf];
+    tri_cell t = tri_locate(p);
+    int ix = t.i[0], iy = t.i[1], iz = t.i[2];
+    float ax = t.a[0], ay = t.a[1], az = t.a[2];
+    float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    float c00 = tri_at32(c, nd, ix, iy, iz) * bx + tri_at32(c, nd, ix + 1, iy, iz) * ax;
+    float c10 = tri_at32(c, nd, ix, iy + 1, iz) * bx + tri_at32(c, nd, ix + 1, iy + 1, iz) * ax;
+    float c01 = tri_at32(c, nd, ix, iy, iz + 1) * bx + tri_at32(c, nd, ix + 1, iy, iz + 1) * ax;
+    float c11 = tri_at32(c, nd, ix, iy + 1, iz + 1) * bx + tri_at32(c, nd, ix + 1, iy + 1, iz + 1) * ax;
+    float c0 = c00 * by + c10 * ay;
+    float c1 = c01 * by + c11 * ay;
+    return c0 * bz + c1 * az;
+}
+
+/* weights of the 8 corners, corner index (dz*2+dy)*2+dx */
+static void tri_weights64(const tri_cell* t, double w[8]) {
+    double a[3] = {t->a[0], t->a[1], t->a[2]};
+    for (int dz = 0; dz < 2; dz++)
+        for (int dy = 0; dy < 2; dy++)
+            for (int dx = 0; dx < 2; dx++)
+                w[(dz * 2 + dy) * 2 + dx] = (dx ? a[0] : 1.0 - a[0]) * (dy ? a[1] : 1.0 - a[1]) * (dz ? a[2] : 1.0 - a[2]);
+}
+
+static double tri64(const ray_ctx* c, int64_t leaf, f3 p) {
+    tri_cell t = tri_locate(p);
+    double w[8], s = 0.0;
+    tri_weights64(&t, w);
+    for (int k = 0; k < 8; k++)
+        s += w[k] * tri_at64(c, leaf, t.i[0] + (k & 1), t.i[1] + ((k >> 1) & 1), t.i[2] + (k >> 2));
+    return s;
+}
+
+/* rayBoxIntersect GEO.cuh:85-98 */
+static f3 ray_box(f3 pos, f3 dir, f3 vmin, f3 vmax) {
+    float ht[8];
+    ht[0] = (vmin.x - pos.x) / dir.x;
+    ht[1] = (vmax.x - pos.x) / dir.x;
+    ht[2] = (vmin.y - pos.y) / dir.y;
+    ht[3] = (vmax.y - pos.y) / dir.y;
+    ht[4] = (vmin.z - pos.z) / dir.z;
+    ht[5] = (vmax.z - pos.z) / dir.z;
+    ht[6] = fmaxf(fmaxf(fminf(ht[0], ht[1]), fminf(ht[2], ht[3])), fminf(ht[4], ht[5]));
+    ht[7] = fminf(fminf(fmaxf(ht[0], ht[1]), fmaxf(ht[2], ht[3])), fmaxf(ht[4], ht[5]));
+    ht[6] = (ht[6] < 0) ? 0.0f : ht[6];
+    return f3m(ht[6], ht[7], (ht[7] < ht[6] || ht[7] < 0) ? OR_NOHIT : 0.0f);
+}
+
+/* HDDAState DDA.cuh:38-91 (+ D7) */
+typedef struct {
+    f3 pos, dir;
+    int ps[3];
+    float tdel[3];
+    f3 t;
+    int p[3];
+    float ts[3];
+    int mask[3];
+} hdda;
+
+static void hdda_set(hdda* d, f3 pos, f3 dir, f3 t) {
+    d->pos = pos;
+    d->dir = dir;
+    d->ps[0] = dir.x > 0 ? 1 : -1;   /* isign3 cuda_math.cuh:1541-1545 */
+    d->ps[1] = dir.y > 0 ? 1 : -1;
+    d->ps[2] = dir.z > 0 ? 1 : -1;
+    d->t = t;
+}
+
+static void hdda_prepare(hdda* d, f3 vmin, float vdel) {
+    float dir[3] = {d->dir.x, d->dir.y, d->dir.z};
+    f3 q = f3sub(f3add(d->pos, f3scale(d->dir, d->t.x)), vmin);
+    float pf[3] = {q.x / vdel, q.y / vdel, q.z / vdel};
+    for (int a = 0; a < 3; a++) {
+        d->tdel[a] = fabsf(vdel / dir[a]);
+        float fl = floorf(pf[a]);
+        float br = ((fl - pf[a]) + 0.5f) * (float)d->ps[a] + 0.5f;
+        d->ts[a] = isinf(d->tdel[a]) ? INFINITY : br * d->tdel[a] + d->t.x;
+        d->p[a] = (int)fl;
+    }
+}
+
+static void hdda_next(hdda* d) {
+    float* s = d->ts;
+    d->mask[0] = (s[0] < s[1]) & (s[0] <= s[2]);
+    d->mask[1] = (s[1] < s[2]) & (s[1] <= s[0]);
+    d->mask[2] = (s[2] < s[0]) & (s[2] <= s[1]);
+    d->t.y = d->mask[0] ? s[0] : (d->mask[1] ? s[1] : s[2]);
+}
+
+static void hdda_step(hdda* d) {
+    d->t.x = d->t.y;
+    for (int a = 0; a < 3; a++) {
+        if (d->mask[a]) d->ts[a] = d->ts[a] + d->tdel[a];
+        d->p[a] += d->mask[a] * d->ps[a];
+    }
+}
+
+static void record_sample(ray_ctx* c, int64_t leaf, f3 p, int lut) {
+    if (!c->rec && c->caprec == 0) return;
+    if (c->nrec >= c->caprec) {
+        int64_t nc = c->caprec ? c->caprec * 2 : 256;
+        c->rec = (or_sample*)realloc(c->rec, (size_t)nc * sizeof(or_sample));
+        c->caprec = nc;
+    }
+    or_sample s = {leaf, p, lut};
+    c->rec[c->nrec++] = s;
+}
+
+/* transfer DDA.cuh:20-23: float division, double clamp and scale, truncation */
+static int transfer_index(const or_render_params* rp, float s) {
+    float q = (s - rp->thresh) / (rp->vmax - rp->vmin);
+    double qc = fmin(1.0, fmax(0.0, (double)q));
+    return (int)(qc * 16300.0f);
+}
+
+/* rayDeepBrick RC.cuh:485-533 (no depth buffer, no colour channel).
+ * M_DEEP_LUT uses the transfer function; M_DEEP_LINEAR is the transfer
+ * s -> (s,s,s,s) with no final clamp (the differentiable form). */
+static void brick_deep(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
+    const or_volume* v = c->v;
+    const or_render_params* rp = c->rp;
+    const or_node* nd = &v->nodes[0][leaf];
+    f3 vmin = f3m((float)nd->mPos[0], (float)nd->mPos[1], (float)nd->mPos[2]);
+    float step = rp->step;
+    t.x = step * ceilf(t.x / step);
+    f3 wp = f3add(pos, f3scale(dir, t.x));
+    f3 p = f3sub(wp, vmin);
+    f3 wpt = f3scale(dir, step);
+    float dt = sqrtf(f3dot(wpt, wpt));
+    float r0 = (float)v->res[0];
+    if (c->hit.x == 0) c->hit.x = t.x;
+    for (int iter = 0; c->clr[3] > rp->alphacut && iter < rp->max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 &&
+                       p.x < r0 && p.y < r0 && p.z < r0;
+         iter++) {
+        float s = tri32(c, leaf, p);
+        if (s >= rp->minval) {
+            float val[4];
+            double v64[4];
+            int li = -1;
+            if (c->mode == M_DEEP_LUT) {
+                li = transfer_index(rp, s);
+                for (int k = 0; k < 4; k++) val[k] = rp->lut[4 * li + k];
+                for (int k = 0; k < 4; k++) v64[k] = val[k];
+            } else {
+                double s64 = tri64(c, leaf, p);
+                for (int k = 0; k < 4; k++) { val[k] = s; v64[k] = s64; }
+            }
+            record_sample(c, leaf, p, li);
+            val[3] = expf(rp->extinct * val[3] * step);
+            for (int k = 0; k < 3; k++) c->clr[k] += val[k] * c->clr[3] * (1 - val[3]) * rp->albedo;
+            c->clr[3] *= val[3];
+            double a64 = exp((double)rp->extinct * v64[3] * (double)step);
+            for (int k = 0; k < 3; k++) c->clr64[k] += v64[k] * c->clr64[3] * (1.0 - a64) * (double)rp->albedo;
+            c->clr64[3] *= a64;
+        }
+        p = f3add(p, wpt);
+        wp = f3add(wp, wpt);
+        t.x += dt;
+    }
+    c->hit.y = t.x;
+    if (c->mode == M_DEEP_LUT) {
+        for (int k = 0; k < 3; k++) {
+            c->clr[k] = fminf(c->clr[k], 1.f);
+            c->clr64[k] = fmin(c->clr64[k], 1.0);
+        }
+        c->clr[3] = fmaxf(c->clr[3], 0.f);
+        c->clr64[3] = fmax(c->clr64[3], 0.0);
+    }
+}
+
+/* getGradient RC.cuh:132-142 (sign -1) and getGradientLevelSet RC.cuh:146-157
+ * (sign +1), unnormalised, FP32 */
+static f3 gradient32(const ray_ctx* c, int64_t leaf, f3 p, int sign) {
+    f3 g;
+    float lo, hi;
+    lo = tri32(c, leaf, f3m(p.x - 0.5f, p.y, p.z)); hi = tri32(c, leaf, f3m(p.x + 0.5f, p.y, p.z));
+    g.x = sign > 0 ? hi - lo : lo - hi;
+    lo = tri32(c, leaf, f3m(p.x, p.y - 0.5f, p.z)); hi = tri32(c, leaf, f3m(p.x, p.y + 0.5f, p.z));
+    g.y = sign > 0 ? hi - lo : lo - hi;
+    lo = tri32(c, leaf, f3m(p.x, p.y, p.z - 0.5f)); hi = tri32(c, leaf, f3m(p.x, p.y, p.z + 0.5f));
+    g.z = sign > 0 ? hi - lo : lo - hi;
+    return f3norm(g);
+}
+
+static void gradient64(const ray_ctx* c, int64_t leaf, f3 p, int sign, double g[3]) {
+    double lo, hi;
+    lo = tri64(c, leaf, f3m(p.x - 0.5f, p.y, p.z)); hi = tri64(c, leaf, f3m(p.x + 0.5f, p.y, p.z));
+    g[0] = sign > 0 ? hi - lo : lo - hi;
+    lo = tri64(c, leaf, f3m(p.x, p.y - 0.5f, p.z)); hi = tri64(c, leaf, f3m(p.x, p.y + 0.5f, p.z));
+    g[1] = sign > 0 ? hi - lo : lo - hi;
+    lo = tri64(c, leaf, f3m(p.x, p.y, p.z - 0.5f)); hi = tri64(c, leaf, f3m(p.x, p.y, p.z + 0.5f));
+    g[2] = sign > 0 ? hi - lo : lo - hi;
+    double n = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    for (int k = 0; k < 3; k++) g[k] /= n;
+}
+
+/* raySurfaceTrilinearBrick RC.cuh:281-300 */
+static void brick_surf_tri(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
+    const or_volume* v = c->v;
+    const or_render_params* rp = c->rp;
+    const or_node* nd = &v->nodes[0][leaf];
+    f3 vmin = f3m((float)nd->mPos[0], (float)nd->mPos[1], (float)nd->mPos[2]);
+    t.x = rp->step * ceilf(t.x / rp->step);
+    f3 p = f3sub(f3add(pos, f3scale(dir, t.x)), vmin);
+    float r0 = (float)v->res[0];
+    for (int iter = 0; iter < rp->max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0;
+         iter++) {
+        if (tri32(c, leaf, p) >= rp->thresh) {
+            c->hit = f3add(p, vmin);
+            c->norm = gradient32(c, leaf, p, -1);
+            c->hit_leaf = leaf;
+            c->hit_p = p;
+            return;
+        }
+        p = f3add(p, f3scale(dir, rp->step));
+        t.x += rp->step;
+    }
+}
+
+/* rayLevelSetBrick RC.cuh:389-410; its rayLevelSet (RC.cuh:186-197) repeats
+ * the caller's test at the same point and so returns p + vmin at once. */
+static void brick_surf_ls(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
+    const or_volume* v = c->v;
+    const or_render_params* rp = c->rp;
+    const or_node* nd = &v->nodes[0][leaf];
+    f3 vmin = f3m((float)nd->mPos[0], (float)nd->mPos[1], (float)nd->mPos[2]);
+    f3 p = f3sub(f3add(pos, f3scale(dir, t.x)), vmin);
+    float r0 = (float)v->res[0];
+    for (int iter = 0; iter < rp->max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x <= r0 && p.y <= r0 && p.z <= r0;
+         iter++) {
+        if (tri32(c, leaf, p) < rp->thresh) {
+            c->hit = f3add(p, vmin);
+            c->norm = gradient32(c, leaf, p, +1);
+            c->hit_leaf = leaf;
+            c->hit_p = p;
+            return;
+        }
+        p = f3add(p, f3scale(dir, rp->step));
+    }
+}
+
+static void brick_call(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
+    switch (c->mode) {
+        case M_DEEP_LUT:
+        case M_DEEP_LINEAR: brick_deep(c, leaf, t, pos, dir); break;
+        case M_SURF_TRI: brick_surf_tri(c, leaf, t, pos, dir); break;
+        default: brick_surf_ls(c, leaf, t, pos, dir); break;
+    }
+}
+
+/* rayCast RC.cuh:543-611 (no depth buffer) */
+static void ray_cast(ray_ctx* c, f3 pos, f3 dir) {
+    const or_volume* v = c->v;
+    const or_render_params* rp = c->rp;
+    int top = v->top_lev;
+    if (top < 1 || top >= v->levs || v->ncnt[top] < 1) return;
+    int64_t nodeid[OR_MAXLEV];
+    float tmax[OR_MAXLEV + 1];
+    int lev = top;
+    nodeid[lev] = 0;
+    f3 bmin = f3m(v->obj_min[0], v->obj_min[1], v->obj_min[2]);
+    f3 bmax = f3m(v->obj_max[0], v->obj_max[1], v->obj_max[2]);
+    f3 ts = ray_box(pos, dir, bmin, bmax);
+    const or_node* node = &v->nodes[lev][0];
+    f3 vmin = f3m((float)node->mPos[0], (float)node->mPos[1], (float)node->mPos[2]);
+    if (ts.z == OR_NOHIT) return;
+    ts.x += rp->epsilon;
+    tmax[lev] = ts.y - rp->epsilon;
+    hdda d;
+    hdda_set(&d, pos, dir, ts);
+    hdda_prepare(&d, vmin, (float)or_range(v, lev - 1));
+    for (int iter = 0; iter < rp->max_iter && lev > 0 && lev <= top && d.p[0] >= 0 && d.p[1] >= 0 && d.p[2] >= 0 &&
+                       d.p[0] <= v->res[lev] && d.p[1] <= v->res[lev] && d.p[2] <= v->res[lev];
+         iter++) {
+        hdda_next(&d);
+        int dim = v->logdim[lev];
+        int64_t b = ((((int64_t)d.p[2] << dim) + d.p[1]) << dim) + d.p[0];
+        int64_t child = get_child(v, node, b);
+        if (child >= 0 && child < v->ncnt[lev - 1]) {
+            if (lev == 1) {
+                d.t.x += rp->epsilon;
+                brick_call(c, child, d.t, pos, dir);
+                if (c->clr[3] <= 0) {
+                    c->clr[3] = 0;
+                    return;
+                }
+                if (c->hit.z != OR_NOHIT) return;
+                hdda_step(&d);
+            } else {
+                lev--;
+                nodeid[lev] = child;
+                node = &v->nodes[lev][child];
+                vmin = f3m((float)node->mPos[0], (float)node->mPos[1], (float)node->mPos[2]);
+                d.t.x += rp->epsilon;
+                tmax[lev] = d.t.y - rp->epsilon;
+                hdda_prepare(&d, vmin, (float)or_range(v, lev - 1));
+            }
+        } else {
+            hdda_step(&d);
+        }
+        while (lev <= top && d.t.x > tmax[lev]) {
+            lev++;
+            if (lev <= top) {
+                node = &v->nodes[lev][nodeid[lev]];
+                vmin = f3m((float)node->mPos[0], (float)node->mPos[1], (float)node->mPos[2]);
+                hdda_prepare(&d, vmin, (float)or_range(v, lev - 1));
+            }
+        }
+    }
+}
+
+/* pixel ray, gvdbRayDeep MOD.cu:66-71 with GEO.cuh:48-63 (identity transform) */
+static void camera_ray(const or_camera* cam, int x, int y, f3* pos, f3* dir) {
+    float u = ((float)x + 0.5f) / (float)cam->width;
+    float w = ((float)y + 0.5f) / (float)cam->height;
+    *pos = f3m((u * cam->org_u[0] + w * cam->org_v[0]) + cam->org[0], (u * cam->org_u[1] + w * cam->org_v[1]) + cam->org[1],
+               (u * cam->org_u[2] + w * cam->org_v[2]) + cam->org[2]);
+    f3 d = f3m((u * cam->dir_u[0] + w * cam->dir_v[0]) + cam->dir[0], (u * cam->dir_u[1] + w * cam->dir_v[1]) + cam->dir[1],
+               (u * cam->dir_u[2] + w * cam->dir_v[2]) + cam->dir[2]);
+    *dir = f3norm(d);
+}
+
+static int render_check(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp) {
+    if (!v || !cam || !rp || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (cam->width <= 0 || cam->height <= 0 || !(rp->step > 0)) return OR_ERR_ARG;
+    if (!v->amap || v->apron != 1) return OR_ERR_STATE;
+    return OR_OK;
+}
+
+/* performPhongShading MOD.cu:38-57 (colour (1,1,1,1)); FP32 and FP64 */
+static void phong(ray_ctx* c, f3 pos_unused, float out[4], double out64[4]) {
+    (void)pos_unused;
+    const or_render_params* rp = c->rp;
+    if (c->hit.z == OR_NOHIT) {
+        for (int k = 0; k < 4; k++) { out[k] = rp->backclr[k]; out64[k] = rp->backclr[k]; }
+        return;
+    }
+    f3 light = f3m(rp->light[0], rp->light[1], rp->light[2]);
+    f3 shit = c->hit, snorm = c->norm;
+    int64_t hl = c->hit_leaf;
+    f3 hp = c->hit_p;
+    f3 ldir = f3norm(f3sub(light, shit));
+    float diff = (float)(0.9 * (double)fmaxf(0.0f, f3dot(snorm, ldir)));
+    float amb = 0.1f;
+    /* FP64 shading at the same hit point */
+    double g[3];
+    gradient64(c, hl, hp, c->mode == M_SURF_LS ? +1 : -1, g);
+    double L[3] = {(double)rp->light[0] - shit.x, (double)rp->light[1] - shit.y, (double)rp->light[2] - shit.z};
+    double ln = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+    double dd = (g[0] * L[0] + g[1] * L[1] + g[2] * L[2]) / ln;
+    double diff64 = 0.9 * (dd > 0 ? dd : 0.0);
+    if (rp->shadow_amt > 0) {
+        ray_ctx s2 = *c;
+        s2.hit = f3m(0, 0, OR_NOHIT);
+        s2.clr[0] = s2.clr[1] = s2.clr[2] = 0; s2.clr[3] = 1;
+        s2.rec = NULL; s2.caprec = 0; s2.nrec = 0;
+        f3 o = f3add(shit, f3scale(snorm, rp->shadow_bias));
+        ray_cast(&s2, o, ldir);
+        if (s2.hit.z != OR_NOHIT) {
+            diff = (float)((double)diff * (1.0 - (double)rp->shadow_amt));
+            diff64 = diff64 * (1.0 - (double)rp->shadow_amt);
+        }
+    }
+    float sh = diff + amb;
+    out[0] = out[1] = out[2] = sh;
+    out[3] = 1.0f;
+    out64[0] = out64[1] = out64[2] = diff64 + 0.1;
+    out64[3] = 1.0;
+}
+
+int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
+              float* out32, double* out64) {
+    int st = render_check(v, chan, cam, rp);
+    if (st) return st;
+    int mode;
+    switch (rp->shade) {
+        case OR_SHADE_TRILINEAR: mode = M_SURF_TRI; break;
+        case OR_SHADE_LEVELSET: mode = M_SURF_LS; break;
+        case OR_SHADE_VOLUME: mode = M_DEEP_LUT; if (!rp->lut) return OR_ERR_ARG; break;
+        default: return OR_ERR_ARG;
+    }
+    for (int y = 0; y < cam->height; y++)
+        for (int x = 0; x < cam->width; x++) {
+            ray_ctx c;
+            memset(&c, 0, sizeof(c));
+            c.v = v; c.A = v->atlas[chan]; c.B64 = bricks64; c.rp = rp; c.mode = mode;
+            f3 pos, dir;
+            camera_ray(cam, x, y, &pos, &dir);
+            float o32[4];
+            double o64[4];
+            if (mode == M_DEEP_LUT) {
+                c.hit = f3m(0, 0, OR_NOHIT);
+                c.clr[0] = c.clr[1] = c.clr[2] = 0; c.clr[3] = 1;
+                c.clr64[0] = c.clr64[1] = c.clr64[2] = 0; c.clr64[3] = 1;
+                ray_cast(&c, pos, dir);
+                /* MOD.cu:75: (lerp3(back, clr, 1-T), 1-T) */
+                float a = 1.0f - c.clr[3];
+                double a64 = 1.0 - c.clr64[3];
+                for (int k = 0; k < 3; k++) {
+                    o32[k] = rp->backclr[k] + a * (c.clr[k] - rp->backclr[k]);
+                    o64[k] = rp->backclr[k] + a64 * (c.clr64[k] - rp->backclr[k]);
+                }
+                o32[3] = a;
+                o64[3] = a64;
+            } else {
+                c.hit = mode == M_SURF_TRI ? f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT) : f3m(0, 0, OR_NOHIT);
+                c.clr[0] = c.clr[1] = c.clr[2] = c.clr[3] = 1;
+                ray_cast(&c, pos, dir);
+                phong(&c, pos, o32, o64);
+            }
+            int64_t px = (int64_t)y * cam->width + x;
+            if (out32) for (int k = 0; k < 4; k++) out32[4 * px + k] = o32[k];
+            if (out64) for (int k = 0; k < 4; k++) out64[4 * px + k] = o64[k];
+        }
+    return OR_OK;
+}
+
+/* Emission-absorption raycast with the linear transfer (extension): per
+ * sample s, a = exp(extinct*s*step), I += s*T*(1-a)*albedo, T *= a. */
+int or_raycast_ea(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
+                  float* I32, float* T32, double* I64, double* T64) {
+    int st = render_check(v, chan, cam, rp);
+    if (st) return st;
+    for (int y = 0; y < cam->height; y++)
+        for (int x = 0; x < cam->width; x++) {
+            ray_ctx c;
+            memset(&c, 0, sizeof(c));
+            c.v = v; c.A = v->atlas[chan]; c.B64 = bricks64; c.rp = rp; c.mode = M_DEEP_LINEAR;
+            c.hit = f3m(0, 0, OR_NOHIT);
+            c.clr[3] = 1; c.clr64[3] = 1;
+            f3 pos, dir;
+            camera_ray(cam, x, y, &pos, &dir);
+            ray_cast(&c, pos, dir);
+            int64_t px = (int64_t)y * cam->width + x;
+            if (I32) I32[px] = c.clr[0];
+            if (T32) T32[px] = c.clr[3];
+            if (I64) I64[px] = c.clr64[0];
+            if (T64) T64[px] = c.clr64[3];
+        }
+    return OR_OK;
+}
+
+/* Adjoint of or_raycast_ea in FP64 by reverse accumulation over the recorded
+ * samples of each ray: with e = extinct*step, a_k = exp(e s_k),
+ *   dI/ds_k = albedo T_k ((1-a_k) - e s_k a_k) + e * sum_{j>k} albedo s_j T_j (1-a_j)
+ *   dT/ds_k = e T_final
+ * and ds_k/dvoxel = trilinear weight.  grad has nb*br^3 entries (padded
+ * bricks, leaf-major, x fastest) and is accumulated into. */
+int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
+                           const double* bricks64, const double* gI, const double* gT, double* grad) {
+    int st = render_check(v, chan, cam, rp);
+    if (st) return st;
+    if (!grad) return OR_ERR_ARG;
+    int br = brick_res(v);
+    int64_t br3 = ipow3(br);
+    double e = (double)rp->extinct * (double)rp->step;
+    double alb = rp->albedo;
+    ray_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.caprec = 0;
+    for (int y = 0; y < cam->height; y++)
+        for (int x = 0; x < cam->width; x++) {
+            c.v = v; c.A = v->atlas[chan]; c.B64 = bricks64; c.rp = rp; c.mode = M_DEEP_LINEAR;
+            c.hit = f3m(0, 0, OR_NOHIT);
+            c.clr[0] = c.clr[1] = c.clr[2] = 0; c.clr[3] = 1;
+            c.clr64[0] = c.clr64[1] = c.clr64[2] = 0; c.clr64[3] = 1;
+            c.nrec = 0;
+            if (c.caprec == 0) { c.caprec = 256; c.rec = (or_sample*)malloc(256 * sizeof(or_sample)); }
+            f3 pos, dir;
+            camera_ray(cam, x, y, &pos, &dir);
+            ray_cast(&c, pos, dir);
+            int64_t px = (int64_t)y * cam->width + x;
+            double gi = gI ? gI[px] : 0.0, gt = gT ? gT[px] : 0.0;
+            int64_t n = c.nrec;
+            double* Tk = (double*)malloc((size_t)(n + 1) * sizeof(double));
+            double* sk = (double*)malloc((size_t)(n + 1) * sizeof(double));
+            double T = 1.0;
+            for (int64_t k = 0; k < n; k++) {
+                sk[k] = tri64(&c, c.rec[k].leaf, c.rec[k].p);
+                Tk[k] = T;
+                T *= exp(e * sk[k]);
+            }
+            double Tfin = T, R = 0.0;
+            for (int64_t k = n - 1; k >= 0; k--) {
+                double a = exp(e * sk[k]);
+                double q = gi * (alb * Tk[k] * ((1.0 - a) - e * sk[k] * a) + e * R) + gt * e * Tfin;
+                R += alb * sk[k] * Tk[k] * (1.0 - a);
+                tri_cell tc = tri_locate(c.rec[k].p);
+                double w[8];
+                tri_weights64(&tc, w);
+                for (int m = 0; m < 8; m++) {
+                    int i = tc.i[0] + (m & 1), j = tc.i[1] + ((m >> 1) & 1), kk = tc.i[2] + (m >> 2);
+                    i = i < 0 ? 0 : (i >= br ? br - 1 : i);
+                    j = j < 0 ? 0 : (j >= br ? br - 1 : j);
+                    kk = kk < 0 ? 0 : (kk >= br ? br - 1 : kk);
+                    grad[c.rec[k].leaf * br3 + ((int64_t)kk * br + j) * br + i] += q * w[m];
+                }
+            }
+            free(Tk);
+            free(sk);
+        }
+    free(c.rec);
+    return OR_OK;
+}
+
+/* number of recorded samples per ray (for sample-set tests) */
+int or_raycast_ea_samples(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, int64_t px,
+                          int64_t max_out, int64_t* leaf_out, float* p_out, int64_t* count) {
+    int st = render_check(v, chan, cam, rp);
+    if (st) return st;
+    if (px < 0 || px >= (int64_t)cam->width * cam->height) return OR_ERR_ARG;
+    ray_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.v = v; c.A = v->atlas[chan]; c.rp = rp; c.mode = M_DEEP_LINEAR;
+    c.hit = f3m(0, 0, OR_NOHIT);
+    c.clr[3] = 1; c.clr64[3] = 1;
+    c.caprec = 256; c.rec = (or_sample*)malloc(256 * sizeof(or_sample));
+    f3 pos, dir;
+    camera_ray(cam, (int)(px % cam->width), (int)(px / cam->width), &pos, &dir);
+    ray_cast(&c, pos, dir);
+    for (int64_t k = 0; k < c.nrec && k < max_out; k++) {
+        if (leaf_out) leaf_out[k] = c.rec[k].leaf;
+        if (p_out) { p_out[3 * k] = c.rec[k].p.x; p_out[3 * k + 1] = c.rec[k].p.y; p_out[3 * k + 2] = c.rec[k].p.z; }
+    }
+    *count = c.nrec;
+    free(c.rec);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP64 padded-brick helpers for the adjoint chain                     */
+/* ------------------------------------------------------------------ */
+
+/* source leaf and padded index of apron voxel (i,j,k) of leaf b, by the
+ * lookup of UpdateApron (OP.cuh:100-125); returns -1 for boundval */
+static int64_t apron_source(const or_volume* v, int64_t b, int i, int j, int k, int64_t* src) {
+    const or_node* nd = &v->nodes[0][b];
+    int64_t w[3] = {(int64_t)nd->mPos[0] + i - 1, (int64_t)nd->mPos[1] + j - 1, (int64_t)nd->mPos[2] + k - 1};
+    float wp[3] = {(float)w[0] + 0.5f, (float)w[1] + 0.5f, (float)w[2] + 0.5f};
+    int64_t L = point_leaf(v, wp);
+    if (L < 0) return -1;
+    const or_node* ln = &v->nodes[0][L];
+    int br = brick_res(v);
+    int64_t li = w[0] - ln->mPos[0] + 1, lj = w[1] - ln->mPos[1] + 1, lk = w[2] - ln->mPos[2] + 1;
+    *src = L * ipow3(br) + (lk * br + lj) * br + li;
+    return L;
+}
+
+static int is_apron(int i, int j, int k, int br) {
+    return i == 0 || j == 0 || k == 0 || i == br - 1 || j == br - 1 || k == br - 1;
+}
+
+/* UpdateApron on an FP64 padded brick array (interiors given) */
+int or_apron_f64(or_volume* v, double* bricks, double boundval) {
+    if (!v || !bricks || v->apron != 1) return OR_ERR_ARG;
+    int br = brick_res(v);
+    int64_t br3 = ipow3(br);
+    for (int64_t b = 0; b < v->ncnt[0]; b++)
+        for (int k = 0; k < br; k++)
+            for (int j = 0; j < br; j++)
+                for (int i = 0; i < br; i++) {
+                    if (!is_apron(i, j, k, br)) continue;
+                    int64_t src;
+                    double val = apron_source(v, b, i, j, k, &src) < 0 ? boundval : bricks[src];
+                    bricks[b * br3 + ((int64_t)k * br + j) * br + i] = val;
+                }
+    return OR_OK;
+}
+
+/* adjoint of or_apron_f64: each apron gradient is added to its source
+ * interior voxel, then the apron gradients are cleared */
+int or_apron_adjoint_f64(or_volume* v, double* grad) {
+    if (!v || !grad || v->apron != 1) return OR_ERR_ARG;
+    int br = brick_res(v);
+    int64_t br3 = ipow3(br);
+    for (int64_t b = 0; b < v->ncnt[0]; b++)
+        for (int k = 0; k < br; k++)
+            for (int j = 0; j < br; j++)
+                for (int i = 0; i < br; i++) {
+                    if (!is_apron(i, j, k, br)) continue;
+                    int64_t src;
+                    int64_t a = b * br3 + ((int64_t)k * br + j) * br + i;
+                    if (apron_source(v, b, i, j, k, &src) >= 0) grad[src] += grad[a];
+                }
+    for (int64_t b = 0; b < v->ncnt[0]; b++)
+        for (int k = 0; k < br; k++)
+            for (int j = 0; j < br; j++)
+                for (int i = 0; i < br; i++)
+                    if (is_apron(i, j, k, br)) grad[b * br3 + ((int64_t)k * br + j) * br + i] = 0.0;
+    return OR_OK;
+}
+
+/* adjoint of the weighted cubic splat in FP64: for the interior gradient g
+ * (padded layout; apron entries ignored),
+ *   dL/dw_i = sum_v g(v) t^3,  dL/dp_i = sum_v g(v) w_i 6 t^2 (v - p_i) / r^2,
+ * t = 1 - |v-p_i|^2/r^2 over voxels of active leaves with |v-p_i| < r.
+ * Positions are taken as doubles (pos64, 3 per point) so that finite
+ * differences can perturb them below FP32 resolution. */
+int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                                const double* grad, double* dpos, double* dw) {
+    if (!v || (!pos64 && n > 0) || !grad) return OR_ERR_ARG;
+    int br = brick_res(v);
+    int64_t br3 = ipow3(br);
+    double R = radius, R2 = R * R;
+    for (int64_t i = 0; i < n; i++) {
+        const double* w = pos64 + 3 * i;
+        double wt = weights ? weights[i] : 1.0;
+        double gw = 0.0, gp[3] = {0, 0, 0};
+        if (!(w[0] >= 0 && w[1] >= 0 && w[2] >= 0)) { if (dw) dw[i] = 0; if (dpos) dpos[3*i] = dpos[3*i+1] = dpos[3*i+2] = 0; continue; }
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)floor(w[a] - R);
+            hi[a] = (int64_t)ceil(w[a] + R);
+            if (lo[a] < 0) lo[a] = 0;
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    double dx = x - w[0], dy = y - w[1], dz = z - w[2];
+                    double d2 = dx * dx + dy * dy + dz * dz;
+                    if (!(d2 < R2)) continue;
+                    int64_t leaf = voxel_leaf(v, x, y, z);
+                    if (leaf < 0) continue;
+                    const or_node* nd = &v->nodes[0][leaf];
+                    int64_t lx = x - nd->mPos[0] + 1, ly = y - nd->mPos[1] + 1, lz = z - nd->mPos[2] + 1;
+                    double g = grad[leaf * br3 + (lz * br + ly) * br + lx];
+                    double t = 1.0 - d2 / R2;
+                    gw += g * t * t * t;
+                    double f = g * wt * 6.0 * t * t / R2;
+                    gp[0] += f * dx; gp[1] += f * dy; gp[2] += f * dz;
+                }
+        if (dw) dw[i] = gw;
+        if (dpos) { dpos[3 * i] = gp[0]; dpos[3 * i + 1] = gp[1]; dpos[3 * i + 2] = gp[2]; }
+    }
+    return OR_OK;
+}
+
+/* forward cubic splat in FP64 from FP64 positions into padded bricks
+ * (interiors only; aprons left as they are) */
+int or_splat_cubic_f64_padded(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                              double* bricks) {
+    if (!v || (!pos64 && n > 0) || !bricks) return OR_ERR_ARG;
+    int br = brick_res(v);
+    int64_t br3 = ipow3(br);
+    int r0 = v->res[0];
+    for (int64_t b = 0; b < v->ncnt[0]; b++)
+        for (int k = 1; k <= r0; k++)
+            for (int j = 1; j <= r0; j++)
+                for (int i = 1; i <= r0; i++) bricks[b * br3 + ((int64_t)k * br + j) * br + i] = 0.0;
+    double R = radius, R2 = R * R;
+    for (int64_t i = 0; i < n; i++) {
+        const double* w = pos64 + 3 * i;
+        if (!(w[0] >= 0 && w[1] >= 0 && w[2] >= 0)) continue;
+        double wt = weights ? weights[i] : 1.0;
+        int64_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int64_t)floor(w[a] - R);
+            hi[a] = (int64_t)ceil(w[a] + R);
+            if (lo[a] < 0) lo[a] = 0;
+        }
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    double dx = x - w[0], dy = y - w[1], dz = z - w[2];
+                    double d2 = dx * dx + dy * dy + dz * dz;
+                    if (!(d2 < R2)) continue;
+                    int64_t leaf = voxel_leaf(v, x, y, z);
+                    if (leaf < 0) continue;
+                    const or_node* nd = &v->nodes[0][leaf];
+                    int64_t lx = x - nd->mPos[0] + 1, ly = y - nd->mPos[1] + 1, lz = z - nd->mPos[2] + 1;
+                    double t = 1.0 - d2 / R2;
+                    bricks[leaf * br3 + (lz * br + ly) * br + lx] += wt * t * t * t;
+                }
+    }
+    return OR_OK;
 }

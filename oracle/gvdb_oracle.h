@@ -129,26 +129,69 @@ int or_read_brick(or_volume* v, int chan, int64_t leaf, float* out);
 /* pointer to the full 3-D atlas, dims subdim*(res0+2a), x fastest */
 const float* or_atlas(or_volume* v, int chan);
 
-/* ---- raycast (emission-absorption / trilinear) ---- */
+/* ---- raycast (RC.cuh, DDA.cuh, GEO.cuh, MOD.cu) ---- */
+#define OR_SHADE_TRILINEAR 4   /* gvdb_types.h:109 */
+#define OR_SHADE_LEVELSET 6    /* gvdb_types.h:111 */
+#define OR_SHADE_VOLUME 7      /* gvdb_types.h:112 */
+
+/* Camera: the ray of pixel (x,y), u = (x+0.5)/width, w = (y+0.5)/height, has
+ *   origin    = (u*org_u + w*org_v) + org
+ *   direction = normalize((u*dir_u + w*dir_v) + dir)
+ * in index space.  A perspective camera (GEO.cuh:48-63: campos, cams, camu,
+ * camv) is org = campos, org_u = org_v = 0, dir = cams, dir_u = camu,
+ * dir_v = camv; an orthographic one has dir_u = dir_v = 0. */
 typedef struct {
-    float extinct;    /* SCN_EXTINCT (negative in the reference) */
-    float albedo;     /* SCN_ALBEDO */
-    float step;       /* SCN_DIRECTSTEP */
-    float minval;     /* SCN_MINVAL */
-    float alphacut;   /* SCN_ALPHACUT */
-    float thresh, vmin, vmax;  /* SCN_THRESH/VMIN/VMAX */
-    float epsilon;
-    int max_iter;
-    float backclr[4];
+    int width, height;
+    float org[3], org_u[3], org_v[3];
+    float dir[3], dir_u[3], dir_v[3];
+} or_camera;
+
+/* Scene parameters (ScnInfo, PrepareRender VG.cpp:4254-4306) */
+typedef struct {
+    int shade;          /* OR_SHADE_* (or_render only) */
+    float step;         /* SCN_DIRECTSTEP */
+    float extinct;      /* SCN_EXTINCT (negative: a = exp(extinct*sigma*step)) */
+    float albedo;       /* SCN_ALBEDO */
+    float minval;       /* SCN_MINVAL */
+    float alphacut;     /* SCN_ALPHACUT */
+    float thresh, vmin, vmax;   /* SCN_THRESH, SCN_VMIN, SCN_VMAX */
+    float epsilon;      /* VDBInfo.epsilon (VG.cpp:71) */
+    int max_iter;       /* MAX_ITER (RC.cuh:24) */
+    float backclr[4];   /* SCN_BACKCLR */
+    float light[3];     /* light position in index space */
+    float shadow_amt, shadow_bias;   /* SCN_SHADOWAMT, SCN_SHADOWBIAS */
+    const float* lut;   /* transfer function, 16384 x float4 (SHADE_VOLUME) */
 } or_render_params;
 
-/* Orthographic emission-absorption raycast with analytic transfer
- * (sigma = -extinct * density, emission = density). rays given as
- * origin/direction arrays (3 floats each).  Output rgba (4 floats per ray).
- * Follows rayCast RC.cuh:543-611 / rayDeepBrick RC.cuh:485-533 with
- * software trilinear sampling. */
-int or_raycast_ea(or_volume* v, int chan, const or_render_params* rp, int64_t nrays,
-                  const float* origins, const float* dirs, float* out_rgba, double* out_f64);
+/* Render(shade) VG.cpp:4336-4381: gvdbRayDeep / gvdbRaySurfaceTrilinear /
+ * gvdbRayLevelSet (MOD.cu:60-181).  out32: 4 floats per pixel, the colour
+ * before the uchar4 quantisation.  out64 (optional): the same pixel evaluated
+ * in FP64 on the FP32-defined sample set (sample positions, transfer-function
+ * indices, hits and shadow decisions).  bricks64 (optional): FP64 padded
+ * bricks to sample instead of the FP32 atlas (leaf-major, x fastest). */
+int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
+              float* out32, double* out64);
+
+/* Emission-absorption raycast with the linear transfer s -> (s,s,s,s)
+ * (extension; same traversal and samples as SHADE_VOLUME, no final clamp):
+ *   a = exp(extinct*s*step); I += s*T*(1-a)*albedo; T *= a.
+ * One I and one T per pixel, FP32 and FP64 (any output may be NULL). */
+int or_raycast_ea(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
+                  float* I32, float* T32, double* I64, double* T64);
+/* FP64 adjoint: grad (nb*br^3 padded bricks) += d(sum gI*I + gT*T)/d voxel */
+int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
+                           const double* bricks64, const double* gI, const double* gT, double* grad);
+/* the samples of one pixel's ray: leaf ids and brick-local positions */
+int or_raycast_ea_samples(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, int64_t px,
+                          int64_t max_out, int64_t* leaf_out, float* p_out, int64_t* count);
+
+/* FP64 padded-brick chain used by the adjoint tests */
+int or_apron_f64(or_volume* v, double* bricks, double boundval);
+int or_apron_adjoint_f64(or_volume* v, double* grad);
+int or_splat_cubic_f64_padded(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                              double* bricks);
+int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                                const double* grad, double* dpos, double* dw);
 
 #ifdef __cplusplus
 }

@@ -28,10 +28,59 @@ class OrInfo(C.Structure):
                 ("obj_min", C.c_float * 3), ("obj_max", C.c_float * 3), ("sc_len", C.c_int64)]
 
 
+class OrCamera(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("org", C.c_float * 3), ("org_u", C.c_float * 3),
+                ("org_v", C.c_float * 3), ("dir", C.c_float * 3), ("dir_u", C.c_float * 3),
+                ("dir_v", C.c_float * 3)]
+
+
 class OrRenderParams(C.Structure):
-    _fields_ = [("extinct", C.c_float), ("albedo", C.c_float), ("step", C.c_float), ("minval", C.c_float),
-                ("alphacut", C.c_float), ("thresh", C.c_float), ("vmin", C.c_float), ("vmax", C.c_float),
-                ("epsilon", C.c_float), ("max_iter", C.c_int), ("backclr", C.c_float * 4)]
+    _fields_ = [("shade", C.c_int), ("step", C.c_float), ("extinct", C.c_float), ("albedo", C.c_float),
+                ("minval", C.c_float), ("alphacut", C.c_float), ("thresh", C.c_float), ("vmin", C.c_float),
+                ("vmax", C.c_float), ("epsilon", C.c_float), ("max_iter", C.c_int), ("backclr", C.c_float * 4),
+                ("light", C.c_float * 3), ("shadow_amt", C.c_float), ("shadow_bias", C.c_float),
+                ("lut", C.POINTER(C.c_float))]
+
+
+SHADE_TRILINEAR, SHADE_LEVELSET, SHADE_VOLUME = 4, 6, 7
+
+
+def make_camera(cam):
+    """cam: dict with width, height, org, org_u, org_v, dir, dir_u, dir_v."""
+    c = OrCamera()
+    c.width, c.height = int(cam["width"]), int(cam["height"])
+    for k in ("org", "org_u", "org_v", "dir", "dir_u", "dir_v"):
+        setattr(c, k, (C.c_float * 3)(*[float(x) for x in cam.get(k, (0, 0, 0))]))
+    return c
+
+
+def make_params(rp):
+    """rp: dict of scene parameters; missing keys take the Scene defaults
+    (gvdb_scene.cpp:46-53, VG.cpp:71, RC.cuh:24)."""
+    p = OrRenderParams()
+    p.shade = int(rp.get("shade", SHADE_VOLUME))
+    p.step = float(rp.get("step", 1.0))
+    p.extinct = float(rp.get("extinct", -1.1))
+    p.albedo = float(rp.get("albedo", 1.5))
+    p.minval = float(rp.get("minval", 0.005))
+    p.alphacut = float(rp.get("alphacut", 0.01))
+    p.thresh = float(rp.get("thresh", 0.1))
+    p.vmin = float(rp.get("vmin", 0.0))
+    p.vmax = float(rp.get("vmax", 1.0))
+    p.epsilon = float(rp.get("epsilon", 0.001))
+    p.max_iter = int(rp.get("max_iter", 256))
+    p.backclr = (C.c_float * 4)(*[float(x) for x in rp.get("backclr", (0, 0, 0, 0))])
+    p.light = (C.c_float * 3)(*[float(x) for x in rp.get("light", (0, 0, 0))])
+    p.shadow_amt = float(rp.get("shadow_amt", 0.8))
+    p.shadow_bias = float(rp.get("shadow_bias", 1.0))
+    lut = rp.get("lut")
+    p._lut_keep = None
+    if lut is not None:
+        a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1)
+        assert a.size == 16384 * 4
+        p._lut_keep = a
+        p.lut = a.ctypes.data_as(C.POINTER(C.c_float))
+    return p
 
 
 NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("pad", "u1"),
@@ -76,7 +125,16 @@ def lib():
         L.or_atlas.argtypes = [vp, i32]
         L.or_atlas.restype = vp
         L.or_read_brick.argtypes = [vp, i32, i64, fp]
-        L.or_raycast_ea.argtypes = [vp, i32, C.POINTER(OrRenderParams), i64, fp, fp, fp, dp]
+        cp, rpp = C.POINTER(OrCamera), C.POINTER(OrRenderParams)
+        L.or_render.argtypes = [vp, i32, cp, rpp, dp, fp, dp]
+        L.or_raycast_ea.argtypes = [vp, i32, cp, rpp, dp, fp, fp, dp, dp]
+        L.or_raycast_ea_backward.argtypes = [vp, i32, cp, rpp, dp, dp, dp, dp]
+        L.or_raycast_ea_samples.argtypes = [vp, i32, cp, rpp, i64, i64, C.POINTER(C.c_int64), fp,
+                                            C.POINTER(C.c_int64)]
+        L.or_apron_f64.argtypes = [vp, dp, C.c_double]
+        L.or_apron_adjoint_f64.argtypes = [vp, dp]
+        L.or_splat_cubic_f64_padded.argtypes = [vp, dp, i64, dp, f32, dp]
+        L.or_splat_cubic_backward_f64.argtypes = [vp, dp, i64, dp, f32, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -232,3 +290,90 @@ class OracleVolume:
             _chk(self._L.or_read_brick(self._h, chan, k, tmp.ctypes.data_as(C.POINTER(C.c_float))), "read_brick")
             out[k] = tmp.reshape(br, br, br)
         return out
+
+    # ---------------- raycast and adjoints ----------------
+    def _padded_count(self):
+        inf = self.info()
+        return inf.node_cnt[0], inf.brick_res
+
+    @staticmethod
+    def _dp(a):
+        return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def render(self, cam, rp, chan=0, bricks64=None):
+        """Reference Render(shade): returns (rgba32, rgba64), each (H, W, 4)."""
+        c, p = make_camera(cam), make_params(rp)
+        n = c.width * c.height
+        o32 = np.zeros(n * 4, dtype=np.float32)
+        o64 = np.zeros(n * 4, dtype=np.float64)
+        b = None if bricks64 is None else np.ascontiguousarray(bricks64, dtype=np.float64)
+        _chk(self._L.or_render(self._h, chan, C.byref(c), C.byref(p), self._dp(b),
+                               o32.ctypes.data_as(C.POINTER(C.c_float)), self._dp(o64)), "render")
+        return o32.reshape(c.height, c.width, 4), o64.reshape(c.height, c.width, 4)
+
+    def raycast_ea(self, cam, rp, chan=0, bricks64=None):
+        """Linear-transfer emission-absorption raycast: (I32, T32, I64, T64), each (H, W)."""
+        c, p = make_camera(cam), make_params(rp)
+        n = c.width * c.height
+        I32, T32 = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        I64, T64 = np.zeros(n, np.float64), np.zeros(n, np.float64)
+        b = None if bricks64 is None else np.ascontiguousarray(bricks64, dtype=np.float64)
+        fpp = C.POINTER(C.c_float)
+        _chk(self._L.or_raycast_ea(self._h, chan, C.byref(c), C.byref(p), self._dp(b), I32.ctypes.data_as(fpp),
+                                   T32.ctypes.data_as(fpp), self._dp(I64), self._dp(T64)), "raycast_ea")
+        sh = (c.height, c.width)
+        return I32.reshape(sh), T32.reshape(sh), I64.reshape(sh), T64.reshape(sh)
+
+    def raycast_ea_backward(self, cam, rp, gI, gT=None, chan=0, bricks64=None):
+        """FP64 gradient over padded bricks (B, br, br, br) [z, y, x]."""
+        c, p = make_camera(cam), make_params(rp)
+        B, br = self._padded_count()
+        grad = np.zeros(B * br ** 3, dtype=np.float64)
+        gi = np.ascontiguousarray(gI, dtype=np.float64).reshape(-1)
+        gt = None if gT is None else np.ascontiguousarray(gT, dtype=np.float64).reshape(-1)
+        b = None if bricks64 is None else np.ascontiguousarray(bricks64, dtype=np.float64)
+        _chk(self._L.or_raycast_ea_backward(self._h, chan, C.byref(c), C.byref(p), self._dp(b), self._dp(gi),
+                                            self._dp(gt), self._dp(grad)), "raycast_ea_backward")
+        return grad.reshape(B, br, br, br)
+
+    def raycast_ea_samples(self, cam, rp, px, chan=0, max_out=4096):
+        c, p = make_camera(cam), make_params(rp)
+        leaf = np.zeros(max_out, dtype=np.int64)
+        pp = np.zeros(max_out * 3, dtype=np.float32)
+        cnt = C.c_int64(0)
+        _chk(self._L.or_raycast_ea_samples(self._h, chan, C.byref(c), C.byref(p), int(px), max_out,
+                                           leaf.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           pp.ctypes.data_as(C.POINTER(C.c_float)), C.byref(cnt)),
+             "raycast_ea_samples")
+        k = min(cnt.value, max_out)
+        return leaf[:k], pp[:3 * k].reshape(k, 3)
+
+    def apron_f64(self, bricks64, boundval=0.0):
+        b = np.ascontiguousarray(bricks64, dtype=np.float64).copy()
+        _chk(self._L.or_apron_f64(self._h, self._dp(b), float(boundval)), "apron_f64")
+        return b
+
+    def apron_adjoint_f64(self, grad):
+        g = np.ascontiguousarray(grad, dtype=np.float64).copy()
+        _chk(self._L.or_apron_adjoint_f64(self._h, self._dp(g)), "apron_adjoint_f64")
+        return g
+
+    def splat_cubic_f64_padded(self, pos64, radius, weights=None):
+        B, br = self._padded_count()
+        out = np.zeros((B, br, br, br), dtype=np.float64)
+        p = np.ascontiguousarray(pos64, dtype=np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        _chk(self._L.or_splat_cubic_f64_padded(self._h, self._dp(p), p.shape[0], self._dp(w), float(radius),
+                                               self._dp(out)), "splat_cubic_f64_padded")
+        return out
+
+    def splat_cubic_backward_f64(self, pos64, radius, grad, weights=None):
+        p = np.ascontiguousarray(pos64, dtype=np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        g = np.ascontiguousarray(grad, dtype=np.float64)
+        dpos = np.zeros_like(p)
+        dw = np.zeros(p.shape[0], dtype=np.float64)
+        _chk(self._L.or_splat_cubic_backward_f64(self._h, self._dp(p), p.shape[0], self._dp(w), float(radius),
+                                                 self._dp(g), self._dp(dpos), self._dp(dw)),
+             "splat_cubic_backward_f64")
+        return dpos, dw
