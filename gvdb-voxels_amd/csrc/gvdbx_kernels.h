@@ -84,8 +84,6 @@ struct BinParams {
     CellGrid g0;
     const uint32_t* leafbits;   // one bit per g0 cell
 };
-void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
-                      int32_t* pslot, hipStream_t s);
 // reach masks of level-0 cells from cell-sorted points (total = off[gb.count()])
 void launch_reach_sorted(const float4* sorted, const int32_t* total, int64_t nmax, ReachParams rp, uint32_t* reach0,
                          hipStream_t s);
@@ -95,11 +93,7 @@ void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s);
 void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
                              hipStream_t s);
-void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0,
-                        const int32_t* off, const int32_t* pslot, float4* sorted, hipStream_t s);
 void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s);
-// sort each bin's points by original index (stable per-cell order)
-void launch_bin_sort(const int32_t* off, int64_t ncell, float4* sorted, float4* tmp, hipStream_t s);
 
 // ---------------- points -> voxels ----------------
 struct GatherParams {
@@ -189,7 +183,20 @@ void launch_scatter_density(ScatterParams sp, hipStream_t s);
 int64_t radix_hist_entries(int64_t n);
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
                      int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s);
+int radix_sort_pairs32(uint32_t* keys, uint32_t* vals, uint32_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
+                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s);
 void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
+
+// ---------------- binning by radix sort ----------------
+// key[i] = bin cell of point i (gb.count() for invalid / outside points),
+// val[i] = i
+void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, uint32_t* key,
+                     uint32_t* val, hipStream_t s);
+// sorted[j] = (w of point val[j], val[j]) for keys < ncell
+void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
+                       const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s);
+// off[c] = first sorted position with key >= c, c in [0, ncell]
+void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* off, hipStream_t s);
 
 // ---------------- rendering ----------------
 struct RenderTree {
@@ -220,9 +227,7 @@ struct RenderScene {
 // per (ray, brick) segment entry state of a recorded EA raycast
 struct VisitRec {
     uint64_t* key;       // leaf << ray_bits | ray
-    float* t;            // t handed to the brick function
-    float* T;            // transmittance at entry
-    float* I;            // accumulated intensity at entry
+    float4* rec;         // (t handed to the brick function, T and I at entry, 0)
     uint32_t* count;     // number of segments (may exceed cap: then rerun)
     int64_t cap;
     int ray_bits;
@@ -238,9 +243,7 @@ struct EABackParams {
     RenderScene sc;
     const uint64_t* key;    // sorted segment keys
     const uint32_t* idx;    // segment record index of each sorted key
-    const float* vt;
-    const float* vT;
-    const float* vI;
+    const float4* rec;
     const int32_t* seg;     // per leaf [begin, end) into the sorted keys, 2 per leaf
     const float* I;
     const float* T;

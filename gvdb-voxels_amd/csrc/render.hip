@@ -263,9 +263,7 @@ __device__ __forceinline__ void record_visit(const VisitRec& vr, int64_t leaf, u
     const uint64_t slot = (uint64_t)base + __popcll(act & ((1ull << lane) - 1));
     if ((int64_t)slot < vr.cap) {
         vr.key[slot] = ((uint64_t)leaf << vr.ray_bits) | ray;
-        vr.t[slot] = t;
-        vr.T[slot] = T;
-        vr.I[slot] = I;
+        vr.rec[slot] = make_float4(t, T, I, 0.0f);
     }
 }
 
@@ -437,43 +435,87 @@ __global__ void k_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t
     }
 }
 
-// One wave per leaf: replay every segment of the leaf from its checkpoint
-// and accumulate dL/dvoxel = sum_k q_k * trilinear weight in LDS, with
+// Move the four corner sums a cell shares with its neighbour along axis A:
+// up (the ray moved +1) takes the d=1 layer to d=0, down takes d=0 to d=1.
+template <int A>
+__device__ __forceinline__ void shift_corners(float* acc, bool up) {
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        if (m & (1 << A)) continue;
+        const float lo = acc[m], hi = acc[m | (1 << A)];
+        acc[m] = up ? hi : 0.0f;
+        acc[m | (1 << A)] = up ? 0.0f : lo;
+    }
+}
+
+// One workgroup per leaf: replay every segment of the leaf from its
+// checkpoint and accumulate dL/dvoxel = sum_k q_k * trilinear weight, with
 //   e = extinct*step, a_k = exp(extinct*s_k*step),
 //   q_k = gI * (albedo T_k ((1-a_k) - e s_k a_k) + e (I_final - I_k)) + gT e T_final
-// (I_k: intensity after sample k).  Lanes take segments in key order, so the
-// LDS accumulation order is fixed.
+// (I_k: intensity after sample k).  The leaf's segments (sorted by ray) are
+// cut into four contiguous quarters, one per wave, and each lane takes a
+// contiguous run of its quarter, so the lanes of one LDS add are spread over
+// the brick face instead of piling onto the same voxels.  Each wave adds
+// into its own LDS copy in an order fixed by the data; the four copies are
+// summed in a fixed order at the end.
 template <int BR>
-__global__ __launch_bounds__(64) void k_ea_backward(EABackParams p) {
+__global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     constexpr int NV = BR * BR * BR;
+    constexpr int NW = 4;
     __shared__ float dens[NV];
-    __shared__ float grad[NV];
+    __shared__ float grad[NW][NV];
     const int64_t leaf = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float* src = p.tv.chan + leaf * (int64_t)NV;
-    for (int i = lane; i < NV; i += 64) {
+    for (int i = tid; i < NV; i += 256) {
         dens[i] = src[i];
-        grad[i] = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; w++) grad[w][i] = 0.0f;
     }
     __syncthreads();
     const int32_t b0 = p.seg[2 * leaf], b1 = p.seg[2 * leaf + 1];
+    const int32_t nseg = b1 - b0;
+    const int32_t per_wave = (nseg + NW - 1) / NW;
+    const int32_t w0 = b0 + wave * per_wave, w1 = min(b1, w0 + per_wave);
+    const int32_t per_lane = (per_wave + 63) / 64;
+    const int32_t c0 = w0 + lane * per_lane, c1 = min(w1, c0 + per_lane);
+    float* g = grad[wave];
     const V3 vmin = leaf_pos(p.tv.nodes[0], leaf);
     const RenderScene& sc = p.sc;
     const float r0 = (float)p.tv.res[0];
     const float e = sc.extinct * sc.step;
     const uint64_t rmask = (1ull << p.ray_bits) - 1;
-    for (int32_t c = b0 + lane; c < b1; c += 64) {
+    for (int32_t c = c0; c < c1; c++) {
         const uint32_t id = p.idx[c];
         const uint32_t ray = (uint32_t)(p.key[c] & rmask);
+        const float4 cp = p.rec[id];
         V3 pos, dir;
         camera_ray(p.cam, (int)(ray % (uint32_t)p.cam.width), (int)(ray / (uint32_t)p.cam.width), pos, dir);
-        float T = p.vT[id], I = p.vI[id];
+        float T = cp.y, I = cp.z;
         const float Ifin = p.I[ray], Tfin = p.T[ray], gi = p.gI[ray];
         const float gt = p.gT ? p.gT[ray] : 0.0f;
         const float step = sc.step;
-        const float tt = step * ceilf(p.vt[id] / step);
+        const float tt = step * ceilf(cp.x / step);
         V3 q = sub(add(pos, scale(dir, tt)), vmin);
         const V3 wpt = scale(dir, step);
+        // corner sums of the current trilinear cell, kept in registers while
+        // the ray stays in it; a step of one cell along one axis keeps the
+        // four shared corners, anything else flushes all eight
+        float acc[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) acc[m] = 0.0f;
+        int cx = 0, cy = 0, cz = 0;
+        bool have = false;
+        auto flush = [&](int mask_keep, int axis_bit) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                if (((m >> axis_bit) & 1) == mask_keep) continue;
+                const int xi = min(max(cx + (m & 1), 0), BR - 1), yi = min(max(cy + ((m >> 1) & 1), 0), BR - 1),
+                          zi = min(max(cz + (m >> 2), 0), BR - 1);
+                if (acc[m] != 0.0f) atomicAdd(&g[(zi * BR + yi) * BR + xi], acc[m]);
+                acc[m] = 0.0f;
+            }
+        };
         for (int iter = 0; T > sc.alphacut && iter < sc.max_iter && q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < r0 &&
                            q.y < r0 && q.z < r0;
              iter++) {
@@ -481,27 +523,45 @@ __global__ __launch_bounds__(64) void k_ea_backward(EABackParams p) {
             if (s >= sc.minval) {
                 const float a = expf(sc.extinct * s * step);
                 I += s * T * (1 - a) * sc.albedo;
-                const float g = gi * (sc.albedo * T * ((1.0f - a) - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
+                const float gq = gi * (sc.albedo * T * ((1.0f - a) - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
                 T *= a;
-                const float cx = q.x + 0.5f, cy = q.y + 0.5f, cz = q.z + 0.5f;
-                const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-                const float ax = cx - fx, ay = cy - fy, az = cz - fz;
+                const float fx = floorf(q.x + 0.5f), fy = floorf(q.y + 0.5f), fz = floorf(q.z + 0.5f);
+                const float ax = (q.x + 0.5f) - fx, ay = (q.y + 0.5f) - fy, az = (q.z + 0.5f) - fz;
                 const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+                if (have && !(ix == cx && iy == cy && iz == cz)) {
+                    const int ddx = ix - cx, ddy = iy - cy, ddz = iz - cz;
+                    int axis = -1, sgn = 0;
+                    if (ddy == 0 && ddz == 0 && (ddx == 1 || ddx == -1)) axis = 0, sgn = ddx;
+                    else if (ddx == 0 && ddz == 0 && (ddy == 1 || ddy == -1)) axis = 1, sgn = ddy;
+                    else if (ddx == 0 && ddy == 0 && (ddz == 1 || ddz == -1)) axis = 2, sgn = ddz;
+                    if (axis < 0) {
+                        flush(-1, 0);   // keep nothing
+                    } else {
+                        // +1: old corners at d=1 become the new cell's d=0 (flush d=0, shift down)
+                        // -1: old corners at d=0 become the new cell's d=1 (flush d=1, shift up)
+                        const int keep = sgn > 0 ? 1 : 0;
+                        flush(keep, axis);
+                        if (axis == 0) shift_corners<0>(acc, keep != 0);
+                        else if (axis == 1) shift_corners<1>(acc, keep != 0);
+                        else shift_corners<2>(acc, keep != 0);
+                    }
+                }
+                cx = ix, cy = iy, cz = iz;
+                have = true;
 #pragma unroll
                 for (int m = 0; m < 8; m++) {
                     const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
-                    const int xi = min(max(ix + dx, 0), BR - 1), yi = min(max(iy + dy, 0), BR - 1),
-                              zi = min(max(iz + dz, 0), BR - 1);
                     const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dz ? az : 1.0f - az);
-                    atomicAdd(&grad[(zi * BR + yi) * BR + xi], g * w);
+                    acc[m] += gq * w;
                 }
             }
             q = add(q, wpt);
         }
+        if (have) flush(-1, 0);
     }
     __syncthreads();
     float* dst = p.grad + leaf * (int64_t)NV;
-    for (int i = lane; i < NV; i += 64) dst[i] = grad[i];
+    for (int i = tid; i < NV; i += 256) dst[i] = ((grad[0][i] + grad[1][i]) + grad[2][i]) + grad[3][i];
 }
 
 // Adjoint of UpdateApron: interior voxel (x,y,z) of leaf k is copied into the
@@ -642,7 +702,7 @@ void launch_ea_backward(EABackParams p, hipStream_t s) {
     // the padded brick is staged in LDS; only the 10^3 brick (8^3 + apron 1)
     // has a kernel, checked by the caller
     if (p.tv.ncnt[0] > 0)
-        hipLaunchKernelGGL(k_ea_backward<10>, dim3((unsigned)p.tv.ncnt[0]), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(k_ea_backward<10>, dim3((unsigned)p.tv.ncnt[0]), dim3(256), 0, s, p);
 }
 
 void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s) {

@@ -149,6 +149,126 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
 }
 
 // ---------------------------------------------------------------------------
+// Same semantics as k_gather, point-centric: the staged members are spread
+// over the lanes, and each lane walks its member's voxel box (subcell box,
+// cutoff box and tile intersected) row by row, skipping the rows and the x
+// spans outside the cutoff sphere, and folds |p-v|^2 into the tile's LDS copy
+// with an integer min (non-negative floats order as their bit patterns).  The
+// min is exact and order-free, so the result is bit-identical to k_gather.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
+    __shared__ float4 cand[CAND];
+    __shared__ uint32_t acc[512];
+    const int lane = threadIdx.x;
+    const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
+    if (tile < 0) return;
+    const int64_t k = tile / tiles_per_brick;
+    if (k >= gp.nleaf) return;
+    const int td = tile_dim(res0);
+    const int nv = td * td * td;
+    int tox, toy, toz;
+    tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
+    const GNode nd = gp.leaves[k];
+    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    for (int i = lane; i < nv; i += 64) acc[i] = 0x7F800000u;   // +inf
+    const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
+    const float R = gp.box_radius;
+    const int sc = gp.sc, scs = gp.sc_shift;
+    const float cut = gp.cutoff;
+    // row pruning radius: a little beyond the cutoff (the per-voxel values are
+    // exact; pruning only has to keep every voxel the cutoff keeps)
+    const float cutm = (cut + 0.01f) * (cut + 0.01f);
+    int cnt = 0;
+
+    auto process = [&](int count) {
+        __syncthreads();
+        for (int j = lane; j < count; j += 64) {
+            const float4 c = cand[j];
+            const uint32_t pk = __float_as_uint(c.w);
+            const float px = c.x, py = c.y, pz = c.z;
+            const int xlo = pk & 7, xhi = (pk >> 3) & 7, ylo = (pk >> 6) & 7, yhi = (pk >> 9) & 7;
+            const int zlo = (pk >> 12) & 7, zhi = (pk >> 15) & 7;
+            const float xc = px - (float)T0x;
+            for (int z = zlo; z <= zhi; z++) {
+                const float dz = (float)(T0z + z) - pz;
+                const float dz2 = dz * dz;
+                if (dz2 > cutm) continue;
+                for (int y = ylo; y <= yhi; y++) {
+                    const float dy = (float)(T0y + y) - py;
+                    const float row = dz2 + dy * dy;
+                    if (row > cutm) continue;
+                    const float half = sqrtf(cutm - row);
+                    const int xa = max(xlo, (int)ceilf(xc - half)), xb = min(xhi, (int)floorf(xc + half));
+                    uint32_t* arow = acc + (z * td + y) * td;
+                    for (int x = xa; x <= xb; x++) {
+                        const float dx = (float)(T0x + x) - px;
+                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        atomicMin(&arow[x], __float_as_uint(d2));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    };
+
+    for (int dcx = -1; dcx <= 1; dcx++)
+        for (int dcy = -1; dcy <= 1; dcy++)
+            for (int dcz = -1; dcz <= 1; dcz++) {
+                int bx = cx + dcx, by = cy + dcy, bz = cz + dcz;
+                if (!gp.gb.contains(bx, by, bz)) continue;
+                int64_t ci = gp.gb.index(bx, by, bz);
+                int beg = gp.off[ci], end = gp.off[ci + 1];
+                for (int j0 = beg; j0 < end; j0 += 64) {
+                    int j = j0 + lane;
+                    bool ok = j < end;
+                    float4 p = make_float4(0, 0, 0, 0);
+                    uint32_t pk = 0;
+                    if (ok) {
+                        p = gp.sorted[j];
+                        float pw[3] = {p.x, p.y, p.z};
+                        int T0[3] = {T0x, T0y, T0z};
+                        int lo_r[3], hi_r[3];
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            // subcell box (cuda_gvdb_particles.cuh:237-242) as a voxel range
+                            int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
+                            int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                            lo = max(lo, (int)ceilf(pw[a] - cut));
+                            hi = min(hi, (int)floorf(pw[a] + cut));
+                            lo_r[a] = max(lo - T0[a], 0);
+                            hi_r[a] = min(hi - T0[a], td - 1);
+                            ok = ok && lo_r[a] <= hi_r[a];
+                        }
+                        pk = (uint32_t)lo_r[0] | ((uint32_t)hi_r[0] << 3) | ((uint32_t)lo_r[1] << 6) |
+                             ((uint32_t)hi_r[1] << 9) | ((uint32_t)lo_r[2] << 12) | ((uint32_t)hi_r[2] << 15);
+                    }
+                    uint64_t bal = __ballot(ok);
+                    if (ok) {
+                        int pos = cnt + (int)__popcll(bal & ((1ull << lane) - 1ull));
+                        cand[pos] = make_float4(p.x, p.y, p.z, __uint_as_float(pk));
+                    }
+                    cnt += (int)__popcll(bal);
+                    if (cnt > CAND - 64) {
+                        process(cnt);
+                        cnt = 0;
+                    }
+                }
+            }
+    if (cnt > 0) process(cnt);
+    __syncthreads();
+    const int br = gp.brick_res, a = gp.apron;
+    float* brick = gp.chan + (int64_t)k * br * br * br;
+    const float r = gp.radius;
+    for (int i = lane; i < nv; i += 64) {
+        const int x = i % td, y = (i / td) % td, z = i / (td * td);
+        const int64_t vi = ((int64_t)(toz + z + a) * br + (toy + y + a)) * br + (tox + x + a);
+        const float init = gp.accum ? brick[vi] : (gp.mode == 1 ? 3.0f : 0.0f);
+        const float c = sqrtf(__uint_as_float(acc[i]));
+        brick[vi] = gp.mode == 1 ? fminf(init, c - r) : fmaxf(init, r - c);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Same semantics as k_gather without LDS: each 64-point chunk of a
 // neighbouring bin is filtered lane-parallel, then the surviving candidates
 // are walked one at a time with ballot + readlane (their coordinates land in
@@ -308,34 +428,34 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
 // ---------------------------------------------------------------------------
 // Weighted cubic splat: rho(v) = sum_p w_p * max(0, 1 - |v-p|^2/r^2)^3 at the
 // integer voxel coordinates of every leaf (extension of distFunc,
-// cuda_gvdb_particles.cuh:516-523).  Sums are taken in a fixed order
-// (neighbour bins in a fixed order, then ascending point index within a
-// bin), so results are deterministic.
+// cuda_gvdb_particles.cuh:516-523).
+// One wave per 8^3 tile.  The points of the 27 neighbour bins whose support
+// box overlaps the tile are compacted into LDS in a fixed order (neighbour
+// bins in a fixed order, ascending point index within a bin); then each lane
+// takes one staged point and adds its kernel into the tile's LDS copy over
+// the voxels of its support (point-centric: no lane idles on voxels the point
+// cannot reach).  A single wave issues every LDS add, in an order fixed by
+// the data, so results are deterministic.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
     __shared__ float4 cand[CAND];
+    __shared__ float acc[512];
     const int lane = threadIdx.x;
     const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles);
     if (tile < 0) return;
     const int64_t k = tile / tiles_per_brick;
     if (k >= sp.nleaf) return;
     const int td = tile_dim(res0);
+    const int nv = td * td * td;
     int tox, toy, toz;
     tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
     const GNode nd = sp.leaves[k];
     const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
-    const int lx = lane % td, ly = lane / td;
-    const bool lane_ok = ly < td;
-    const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
     const int br = sp.brick_res, a = sp.apron;
     float* brick = sp.chan + (int64_t)k * br * br * br;
-    float vz[8], acc[8];
-#pragma unroll
-    for (int z = 0; z < 8; z++) {
-        vz[z] = (float)(T0z + z);
-        acc[z] = 0.0f;
-        if (sp.accum && lane_ok && z < td)
-            acc[z] = brick[((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a)];
+    for (int i = lane; i < nv; i += 64) {
+        const int x = i % td, y = (i / td) % td, z = i / (td * td);
+        acc[i] = sp.accum ? brick[((int64_t)(toz + z + a) * br + (toy + y + a)) * br + (tox + x + a)] : 0.0f;
     }
     const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
     const float r = sp.radius;
@@ -344,24 +464,22 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
 
     auto process = [&](int count) {
         __syncthreads();
-        for (int j = 0; j < count; j++) {
-            float4 c = cand[j];
-            float px = rflf(c.x), py = rflf(c.y), pz = rflf(c.z), wt = rflf(c.w);
-            int zlo = max((int)ceilf(pz - r) - T0z, 0), zhi = min((int)floorf(pz + r) - T0z, td - 1);
-            if (lane_ok) {
-                float dx = vx - px, dy = vy - py;
-                float dxy = __builtin_fmaf(dy, dy, dx * dx);
-                if (dxy < r2) {
-#pragma unroll
-                    for (int z = 0; z < 8; z++) {
-                        if (z >= zlo && z <= zhi) {
-                            float dz = vz[z] - pz;
-                            float d2 = __builtin_fmaf(dz, dz, dxy);
-                            if (d2 < r2) {
-                                float t = 1.0f - d2 / r2;
-                                float kk = (t * t) * t;
-                                acc[z] = acc[z] + wt * kk;
-                            }
+        for (int j = lane; j < count; j += 64) {
+            const float4 c = cand[j];
+            const float px = c.x, py = c.y, pz = c.z, wt = c.w;
+            const int x0 = max((int)ceilf(px - r) - T0x, 0), x1 = min((int)floorf(px + r) - T0x, td - 1);
+            const int y0 = max((int)ceilf(py - r) - T0y, 0), y1 = min((int)floorf(py + r) - T0y, td - 1);
+            const int z0 = max((int)ceilf(pz - r) - T0z, 0), z1 = min((int)floorf(pz + r) - T0z, td - 1);
+            for (int z = z0; z <= z1; z++) {
+                const float dz = (float)(T0z + z) - pz;
+                for (int y = y0; y <= y1; y++) {
+                    const float dy = (float)(T0y + y) - py;
+                    for (int x = x0; x <= x1; x++) {
+                        const float dx = (float)(T0x + x) - px;
+                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        if (d2 < r2) {
+                            const float t = 1.0f - d2 / r2;
+                            atomicAdd(&acc[(z * td + y) * td + x], wt * ((t * t) * t));
                         }
                     }
                 }
@@ -392,7 +510,7 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
                             int hi = (int)floorf(pw[a2] + r) - T0[a2];
                             ok = ok && hi >= 0 && lo <= td - 1;
                         }
-                        if (sp.weights) wt = *(const float*)(sp.weights + (int64_t)__float_as_int(p.w) * sp.wstride);
+                        if (ok && sp.weights) wt = *(const float*)(sp.weights + (int64_t)__float_as_int(p.w) * sp.wstride);
                     }
                     uint64_t bal = __ballot(ok);
                     if (ok) {
@@ -407,10 +525,11 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
                 }
             }
     if (cnt > 0) process(cnt);
-    if (!lane_ok) return;
-#pragma unroll
-    for (int z = 0; z < 8; z++)
-        if (z < td) brick[((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a)] = acc[z];
+    __syncthreads();
+    for (int i = lane; i < nv; i += 64) {
+        const int x = i % td, y = (i / td) % td, z = i / (td * td);
+        brick[((int64_t)(toz + z + a) * br + (toy + y + a)) * br + (tox + x + a)] = acc[i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -555,6 +674,8 @@ void launch_gather(GatherParams gp, hipStream_t s) {
     const bool halves = td == 8 && gp.sc >= 4;
     if (gp.variant == 0)
         hipLaunchKernelGGL(k_gather, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
+    else if (gp.variant == 2)
+        hipLaunchKernelGGL(k_gather_pc, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
     else if (halves)
         hipLaunchKernelGGL(k_gather_rl<true>, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
     else

@@ -407,21 +407,6 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(const GNode* leaves, int64_
 }
 
 // ---- point binning ----
-__global__ __launch_bounds__(256) void k_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp,
-                                                   int32_t* cnt, int32_t* pslot) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float w[3];
-        int32_t slot = -1;
-        if (gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) {
-            int c[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], bp.logr0);
-            if (bp.gb.contains(c[0], c[1], c[2])) slot = atomicAdd(&cnt[bp.gb.index(c[0], c[1], c[2])], 1);
-        }
-        pslot[i] = slot;
-    }
-}
-
 // Extra-brick reach of every level-0 cell from the cell-sorted points: each
 // point's box of gvdbCalcExtraBrickId (cuda_gvdb_particles.cuh:493-494) as a
 // 3x3x3 mask; the masks of one cell are OR-ed within the wave (points of a
@@ -514,19 +499,49 @@ __global__ __launch_bounds__(256) void k_subcell_len(const float4* sorted, const
     }
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz,
-                                                     CellGrid gb, int logr0, const int32_t* off, const int32_t* pslot,
-                                                     float4* sorted) {
+// Binning by sort: the key of a point is its home cell in the bin grid
+// (ncell for points that are invalid or outside it, so they sort last), the
+// value its index.  A stable sort then leaves every bin in ascending point
+// order.
+__global__ __launch_bounds__(256) void k_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb,
+                                                  int logr0, uint32_t* key, uint32_t* val) {
+    const uint32_t none = (uint32_t)gb.count();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int32_t slot = pslot[i];
-        if (slot < 0) continue;
+        float w[3];
+        uint32_t k = none;
+        if (gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2])) {
+            int c[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], logr0);
+            if (gb.contains(c[0], c[1], c[2])) k = (uint32_t)gb.index(c[0], c[1], c[2]);
+        }
+        key[i] = k;
+        val[i] = (uint32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz,
+                                                    const uint32_t* key, const uint32_t* val, uint32_t ncell,
+                                                    float4* sorted) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        if (key[j] >= ncell) continue;
+        const int64_t i = val[j];
         float w[3];
         gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2]);
-        int c[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], logr0);
-        int64_t dst = (int64_t)off[gb.index(c[0], c[1], c[2])] + slot;
-        sorted[dst] = make_float4(w[0], w[1], w[2], __int_as_float((int)i));
+        sorted[j] = make_float4(w[0], w[1], w[2], __int_as_float((int)i));
+    }
+}
+
+// off[c] = lower bound of c in the sorted keys (binary search per cell)
+__global__ __launch_bounds__(256) void k_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* off) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)key[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+        off[c] = (int32_t)lo;
     }
 }
 
@@ -539,54 +554,6 @@ __global__ __launch_bounds__(256) void k_leaf_bits(const int32_t* idx0, int64_t 
             if (c < ncell && idx0[c] >= 0) b |= 1u << k;
         }
         bits[wi] = b;
-    }
-}
-
-// per-cell stable order: sort each bin by original point index.  Small bins
-// are sorted in LDS (bitonic); large bins use a rank count in global memory.
-constexpr int BINSORT_LDS = 2048;
-__global__ __launch_bounds__(256) void k_bin_sort(const int32_t* off, int64_t ncell, float4* sorted, float4* tmp) {
-    __shared__ float4 s[BINSORT_LDS];
-    for (int64_t c = blockIdx.x; c < ncell; c += gridDim.x) {
-        int64_t b = off[c], e = off[c + 1];
-        int64_t m = e - b;
-        if (m <= 1) continue;
-        if (m <= BINSORT_LDS) {
-            int p2 = 1;
-            while (p2 < m) p2 <<= 1;
-            for (int i = threadIdx.x; i < p2; i += blockDim.x)
-                s[i] = i < m ? sorted[b + i] : make_float4(0, 0, 0, __int_as_float(0x7FFFFFFF));
-            __syncthreads();
-            for (int k = 2; k <= p2; k <<= 1)
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    for (int i = threadIdx.x; i < p2; i += blockDim.x) {
-                        int ixj = i ^ j;
-                        if (ixj > i) {
-                            int ki = __float_as_int(s[i].w), kj = __float_as_int(s[ixj].w);
-                            bool up = (i & k) == 0;
-                            if ((ki > kj) == up) {
-                                float4 t = s[i];
-                                s[i] = s[ixj];
-                                s[ixj] = t;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                }
-            for (int i = threadIdx.x; i < m; i += blockDim.x) sorted[b + i] = s[i];
-            __syncthreads();
-        } else {
-            for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
-                float4 v = sorted[b + i];
-                int key = __float_as_int(v.w);
-                int64_t rank = 0;
-                for (int64_t j = 0; j < m; j++) rank += __float_as_int(sorted[b + j].w) < key;
-                tmp[b + rank] = v;
-            }
-            __syncthreads();
-            for (int64_t i = threadIdx.x; i < m; i += blockDim.x) sorted[b + i] = tmp[b + i];
-            __syncthreads();
-        }
     }
 }
 
@@ -681,11 +648,6 @@ void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t*
     hipLaunchKernelGGL(k_leaf_bounds, dim3(grid_for(nleaf, 256, 256)), dim3(256), 0, s, leaves, nleaf, range0, out6);
 }
 
-void launch_bin_count(PointSrc src, int64_t n, float tx, float ty, float tz, BinParams bp, int32_t* cnt,
-                      int32_t* pslot, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_count, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, bp, cnt, pslot);
-}
-
 void launch_reach_sorted(const float4* sorted, const int32_t* total, int64_t nmax, ReachParams rp, uint32_t* reach0,
                          hipStream_t s) {
     if (nmax > 0)
@@ -698,18 +660,24 @@ void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax
         hipLaunchKernelGGL(k_subcell_len, dim3(grid_for(nmax, 256, 2048)), dim3(256), 0, s, sorted, total, nmax, bp, len);
 }
 
-void launch_bin_scatter(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0,
-                        const int32_t* off, const int32_t* pslot, float4* sorted, hipStream_t s) {
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, logr0, off,
-                       pslot, sorted);
+void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, uint32_t* key,
+                     uint32_t* val, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, logr0, key, val);
+}
+
+void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
+                       const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, key, val, ncell,
+                           sorted);
+}
+
+void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* off, hipStream_t s) {
+    hipLaunchKernelGGL(k_bin_offsets, dim3(grid_for(ncell + 1, 256)), dim3(256), 0, s, key, n, ncell, off);
 }
 
 void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s) {
     hipLaunchKernelGGL(k_leaf_bits, dim3(grid_for((ncell + 31) / 32, 256)), dim3(256), 0, s, idx0, ncell, bits);
 }
 
-void launch_bin_sort(const int32_t* off, int64_t ncell, float4* sorted, float4* tmp, hipStream_t s) {
-    if (ncell > 0)
-        hipLaunchKernelGGL(k_bin_sort, dim3((unsigned)(ncell < 65536 ? ncell : 65536)), dim3(256), 0, s, off, ncell,
-                           sorted, tmp);
-}

@@ -89,10 +89,9 @@ struct gvdbx_volume {
     DevBuf amap;
     int64_t amap_cnt = 0;
     // binning
-    DevBuf bin_cnt, bin_off, pslot, sorted, sorted_tmp;
+    DevBuf bin_off, sorted, bkey, bval, bkey2, bval2, rs_hist, rs_off, rs_part;
     CellGrid gbin = {};
     bool bins_valid = false;
-    bool bins_stable = false;
     int64_t bins_n = -1;
     float bins_trans[3] = {};
     const char* bins_base = nullptr;
@@ -107,7 +106,7 @@ struct gvdbx_volume {
     DevBuf pnode, poff;
     int64_t npnode = 0;
     // recorded emission-absorption segments (gvdbx_raycast_ea with record)
-    DevBuf vis_key, vis_t, vis_T, vis_I, vis_idx, vis_tkey, vis_tidx, vis_hist, vis_off, vis_part, vis_seg, vis_cnt;
+    DevBuf vis_key, vis_rec, vis_idx, vis_tkey, vis_tidx, vis_hist, vis_off, vis_part, vis_seg, vis_cnt;
     int64_t vis_n = 0, vis_cap = 0;
     int vis_ray_bits = 0, vis_leaf_bits = 0;
     bool vis_valid = false, vis_sorted = false, vis_in_tmp = false;
@@ -249,54 +248,52 @@ int determine_depth(const gvdbx_volume* v, const float pmin[3], const float pmax
     return -1;
 }
 
-// Bin the current points by home brick cell (counting sort over the leaf grid
-// grown by one cell).  Bins made by RebuildTopology or an earlier call are
-// reused while SetPoints has not been called again and the point count and
-// translation match.  stable=true additionally orders each bin by point index.
+// Bin the current points by home brick cell: a stable radix sort of
+// (cell, point index) over the leaf grid grown by one cell, so every bin
+// holds its points in ascending index order.  Bins made by RebuildTopology
+// or an earlier call are reused while SetPoints has not been called again and
+// the point count and translation match.
 int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable) {
+    (void)stable;   // sorted bins are always in ascending point order
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
     const CellGrid& g0 = v->grid[0];
     CellGrid gb = {g0.ox - 1, g0.oy - 1, g0.oz - 1, g0.nx + 2, g0.ny + 2, g0.nz + 2};
     int64_t nb = gb.count();
+    if (nb >= (int64_t)UINT32_MAX) FAIL(GVDBX_ERR_RANGE, "bin grid exceeds 32-bit cell keys");
     bool same = v->bins_valid && v->bins_gen == v->pos_gen && v->bins_n == n && v->bins_base == v->pos.base &&
                 v->bins_stride == v->pos.stride && v->bins_trans[0] == t[0] && v->bins_trans[1] == t[1] &&
                 v->bins_trans[2] == t[2] && !memcmp(&v->gbin, &gb, sizeof(gb));
+    if (same) return GVDBX_OK;
     hipStream_t s = v->stream;
-    if (!same) {
-        TRY(ensure(v, v->bin_cnt, (size_t)(nb + 1) * 4));
-        TRY(ensure(v, v->bin_off, (size_t)(nb + 1) * 4));
-        TRY(ensure(v, v->pslot, (size_t)std::max<int64_t>(n, 1) * 4));
-        TRY(ensure(v, v->sorted, (size_t)std::max<int64_t>(n, 1) * 16));
-        TRY(ensure(v, v->partials, (size_t)(scan_tiles(nb + 1) + 2) * 8));
-        TRY(ensure(v, v->misc, 256));
-        BinParams bp{};
-        bp.gb = gb;
-        bp.range0 = v->res[0];
-        bp.logr0 = v->logdim[0];
-        bp.g0 = g0;
-        launch_fill_i32(v->bin_cnt.as<int32_t>(), nb + 1, 0, s);
-        launch_bin_count(v->pos, n, t[0], t[1], t[2], bp, v->bin_cnt.as<int32_t>(), v->pslot.as<int32_t>(), s);
-        launch_count_scan_reduce(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(),
-                                 (uint32_t*)((char*)v->misc.p + 96), s);
-        launch_count_scan_apply(v->bin_cnt.as<int32_t>(), nb, v->partials.as<uint32_t>(), v->bin_off.as<int32_t>(), s);
-        launch_bin_scatter(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], v->bin_off.as<int32_t>(),
-                           v->pslot.as<int32_t>(), v->sorted.as<float4>(), s);
-        TRY(check_launch("bin points"));
-        v->gbin = gb;
-        v->bins_valid = true;
-        v->bins_stable = false;
-        v->bins_n = n;
-        v->bins_gen = v->pos_gen;
-        v->bins_base = v->pos.base;
-        v->bins_stride = v->pos.stride;
-        memcpy(v->bins_trans, t, sizeof(v->bins_trans));
-    }
-    if (stable && !v->bins_stable) {
-        TRY(ensure(v, v->sorted_tmp, (size_t)std::max<int64_t>(n, 1) * 16));
-        launch_bin_sort(v->bin_off.as<int32_t>(), nb, v->sorted.as<float4>(), v->sorted_tmp.as<float4>(), s);
-        TRY(check_launch("bin sort"));
-        v->bins_stable = true;
-    }
+    const int64_t m = std::max<int64_t>(n, 1);
+    const int64_t nh = radix_hist_entries(m);
+    TRY(ensure(v, v->bin_off, (size_t)(nb + 1) * 4));
+    TRY(ensure(v, v->bkey, (size_t)m * 4));
+    TRY(ensure(v, v->bval, (size_t)m * 4));
+    TRY(ensure(v, v->bkey2, (size_t)m * 4));
+    TRY(ensure(v, v->bval2, (size_t)m * 4));
+    TRY(ensure(v, v->rs_hist, (size_t)nh * 4));
+    TRY(ensure(v, v->rs_off, (size_t)(nh + 1) * 4));
+    TRY(ensure(v, v->rs_part, (size_t)(scan_tiles(nh + 1) + 1) * 4));
+    TRY(ensure(v, v->sorted, (size_t)m * 16));
+    int end_bit = 1;
+    while (((int64_t)1 << end_bit) <= nb) end_bit++;
+    launch_bin_keys(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), s);
+    int flip = radix_sort_pairs32(v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), v->bkey2.as<uint32_t>(),
+                                  v->bval2.as<uint32_t>(), n, 0, end_bit, v->rs_hist.as<int32_t>(),
+                                  v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s);
+    const uint32_t* skey = flip ? v->bkey2.as<uint32_t>() : v->bkey.as<uint32_t>();
+    const uint32_t* sval = flip ? v->bval2.as<uint32_t>() : v->bval.as<uint32_t>();
+    launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), s);
+    launch_bin_offsets(skey, n, nb, v->bin_off.as<int32_t>(), s);
+    TRY(check_launch("bin points"));
+    v->gbin = gb;
+    v->bins_valid = true;
+    v->bins_n = n;
+    v->bins_gen = v->pos_gen;
+    v->bins_base = v->pos.base;
+    v->bins_stride = v->pos.stride;
+    memcpy(v->bins_trans, t, sizeof(v->bins_trans));
     return GVDBX_OK;
 }
 
@@ -697,8 +694,9 @@ int gvdbx_destroy(gvdbx_volume* v) {
         free_buf(v->flag_all[l]);
     }
     DevBuf* bufs[] = {&v->reach0, &v->leafbits, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
-                      &v->bin_cnt, &v->bin_off, &v->pslot, &v->sorted, &v->sorted_tmp, &v->pnode, &v->poff,
-                      &v->vis_key, &v->vis_t, &v->vis_T, &v->vis_I, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
+                      &v->bin_off, &v->sorted, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
+                      &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
+                      &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt};
     for (DevBuf* b : bufs) free_buf(*b);
     if (v->host) (void)hipHostFree(v->host);
@@ -803,7 +801,7 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
     TRY(check_vol(v));
     if (!key) FAIL(GVDBX_ERR_ARG, "null tuning key");
     if (!strcmp(key, "gather_variant")) {
-        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "gather_variant must be 0 or 1");
+        if (value < 0 || value > 2) FAIL(GVDBX_ERR_ARG, "gather_variant must be 0, 1 or 2");
         v->gather_variant = value;
         return GVDBX_OK;
     }
@@ -1014,14 +1012,10 @@ int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const g
     TRY(ensure_host(v, 4096));
     for (int attempt = 0; attempt < 2; attempt++) {
         TRY(ensure(v, v->vis_key, (size_t)cap * 8));
-        TRY(ensure(v, v->vis_t, (size_t)cap * 4));
-        TRY(ensure(v, v->vis_T, (size_t)cap * 4));
-        TRY(ensure(v, v->vis_I, (size_t)cap * 4));
+        TRY(ensure(v, v->vis_rec, (size_t)cap * 16));
         launch_fill_i32(v->vis_cnt.as<int32_t>(), 1, 0, s);
         vr.key = v->vis_key.as<uint64_t>();
-        vr.t = v->vis_t.as<float>();
-        vr.T = v->vis_T.as<float>();
-        vr.I = v->vis_I.as<float>();
+        vr.rec = v->vis_rec.as<float4>();
         vr.count = v->vis_cnt.as<uint32_t>();
         vr.cap = cap;
         vr.ray_bits = rb;
@@ -1090,9 +1084,7 @@ int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam
     p.sc = sc;
     p.key = v->vis_in_tmp ? v->vis_tkey.as<uint64_t>() : v->vis_key.as<uint64_t>();
     p.idx = v->vis_in_tmp ? v->vis_tidx.as<uint32_t>() : v->vis_idx.as<uint32_t>();
-    p.vt = v->vis_t.as<float>();
-    p.vT = v->vis_T.as<float>();
-    p.vI = v->vis_I.as<float>();
+    p.rec = v->vis_rec.as<float4>();
     p.seg = v->vis_seg.as<int32_t>();
     p.I = I_dev;
     p.T = T_dev;

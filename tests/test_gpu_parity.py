@@ -288,7 +288,8 @@ def test_gather_variants_bit_identical(mode):
     else:
         ov.gather_density(1.5)
     ref = ov.bricks()[:, 1:9, 1:9, 1:9]
-    for var in (0, 1):
+    acc_ref = None
+    for var in (0, 1, 2):
         gv.SetTuning("gather_variant", var)
         gv.FillChannel(0, -7.0)
         if mode == 1:
@@ -296,3 +297,13 @@ def test_gather_variants_bit_identical(mode):
         else:
             gv.GatherDensity(4, len(pts), 1.5, (0, 0, 0), gl, 0)
         np.testing.assert_array_equal(gv.bricks()[:, 1:9, 1:9, 1:9], ref, err_msg=f"variant {var}")
+        # accumulate mode (no cutoff pruning) on a non-trivial initial field
+        gv.FillChannel(0, 0.25)
+        if mode == 1:
+            gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), gl, 0, accum=True)
+        else:
+            gv.GatherDensity(4, len(pts), 1.5, (0, 0, 0), gl, 0, accum=True)
+        got = gv.bricks()[:, 1:9, 1:9, 1:9]
+        if acc_ref is None:
+            acc_ref = got
+        np.testing.assert_array_equal(got, acc_ref, err_msg=f"accumulate, variant {var}")
