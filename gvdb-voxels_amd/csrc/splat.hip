@@ -275,9 +275,15 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
 // scalar registers), and the next chunk's load is in flight meanwhile.
 // ---------------------------------------------------------------------------
 struct ChunkIter {
-    int d = -1;       // neighbour index 0..26
+    int d = -1;       // position in NB_ORDER
     int j0 = 0, end = 0;
 };
+
+// Neighbour bins nearest first: the tile's own bin, the 6 faces, the 12
+// edges, the 8 corners (index (dx+1)*9 + (dy+1)*3 + (dz+1)).  Near points
+// come first, so the running minima (and the pruning bound) tighten early.
+__constant__ int8_t NB_ORDER[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11, 15,
+                                    17, 19, 21, 23, 25, 0, 2, 6, 8, 18, 20, 24, 26};
 
 __device__ __forceinline__ bool next_chunk(ChunkIter& it, const GatherParams& gp, int cx, int cy, int cz) {
     if (it.j0 + 64 < it.end) {
@@ -285,7 +291,8 @@ __device__ __forceinline__ bool next_chunk(ChunkIter& it, const GatherParams& gp
         return true;
     }
     while (++it.d < 27) {
-        int bx = cx + it.d / 9 - 1, by = cy + (it.d / 3) % 3 - 1, bz = cz + it.d % 3 - 1;
+        const int nb = NB_ORDER[it.d];
+        int bx = cx + nb / 9 - 1, by = cy + (nb / 3) % 3 - 1, bz = cz + nb % 3 - 1;
         if (!gp.gb.contains(bx, by, bz)) continue;
         int64_t ci = gp.gb.index(bx, by, bz);
         int beg = gp.off[ci], end = gp.off[ci + 1];
@@ -335,6 +342,21 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
         have = next_chunk(it, gp, cx, cy, cz);
         float4 nxt = make_float4(0, 0, 0, 0);
         if (have && it.j0 + lane < it.end) nxt = gp.sorted[it.j0 + lane];
+        // Pruning bound: every voxel's running min is <= the tile's max of
+        // them, and a candidate at squared distance db2 from the voxels it
+        // may reach changes none of them if db2 exceeds that max (fma is
+        // monotone, so every reached voxel's |p-v|^2 >= db2).  Same per lane.
+        uint32_t lmax = 0;
+        if (lane_ok) {
+#pragma unroll
+            for (int z = 0; z < 8; z++)
+                if (z < td) lmax = max(lmax, m[z]);
+        }
+        uint32_t wmax = lmax;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, o));
+        const float bound = fminf(cut2, __uint_as_float(wmax));
+        const float lbound = fminf(cut2, __uint_as_float(lmax));
         // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
         // as a voxel range; a candidate must also come within the cutoff of the tile.
         // The packed range is the membership range clipped to the tile (z) and to
@@ -362,7 +384,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
                 }
                 pk |= ((uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3)) << (6 * a);
             }
-            ok = ok && db2 <= cut2;
+            ok = ok && db2 <= bound;
         }
         uint64_t mask = __ballot(ok);
         while (mask) {
@@ -377,7 +399,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
             if (lane_ok && lx >= xlo && lx <= xhi && ly >= ylo && ly <= yhi) {
                 const float dx = vx - px, dy = vy - py;
                 const float dxy = __builtin_fmaf(dy, dy, dx * dx);
-                if (dxy <= cut2) {
+                if (dxy <= lbound) {
                     if (HALVES) {
                         // with sc >= 4 the z membership inside an 8-voxel tile is
                         // one or both 4-voxel halves
