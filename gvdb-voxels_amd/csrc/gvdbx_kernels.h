@@ -274,3 +274,8 @@ struct SplatBackParams {
 };
 void launch_splat_backward(SplatBackParams p, hipStream_t s);
 void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s);
+
+// ---------------- batched inputs ----------------
+void launch_backproject(const float* depth, int nframes, int width, int height, const float* intr4,
+                        const float* poses, float voxel, const float* origin3, float dmin, float dmax, float* out,
+                        hipStream_t s);

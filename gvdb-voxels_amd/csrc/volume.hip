@@ -1142,6 +1142,19 @@ int gvdbx_splat_cubic_backward(gvdbx_volume* v, int num_pnts, float radius, floa
     return check_launch("SplatCubicBackward");
 }
 
+int gvdbx_backproject_depth(const float* depth_dev, int nframes, int width, int height, const float* intr4,
+                            const float* poses_dev, float voxel, const float* origin3, float dmin, float dmax,
+                            float* out_xyz_dev, void* stream) {
+    if (nframes < 0 || width < 0 || height < 0) FAIL(GVDBX_ERR_ARG, "BackprojectDepth: negative size");
+    if ((int64_t)nframes * width * height == 0) return GVDBX_OK;
+    if (!depth_dev || !intr4 || !poses_dev || !origin3 || !out_xyz_dev) FAIL(GVDBX_ERR_ARG, "BackprojectDepth: null pointer");
+    if (!(voxel > 0.0f) || !isfinite(voxel)) FAIL(GVDBX_ERR_ARG, "BackprojectDepth: voxel size must be > 0");
+    if (!(intr4[0] != 0.0f) || !(intr4[1] != 0.0f)) FAIL(GVDBX_ERR_ARG, "BackprojectDepth: zero focal length");
+    launch_backproject(depth_dev, nframes, width, height, intr4, poses_dev, voxel, origin3, dmin, dmax, out_xyz_dev,
+                       (hipStream_t)stream);
+    return check_launch("BackprojectDepth");
+}
+
 int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out) {
     TRY(check_vol(v));
     if (!out) FAIL(GVDBX_ERR_ARG, "null info pointer");

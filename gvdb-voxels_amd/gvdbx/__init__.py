@@ -302,6 +302,23 @@ class VolumeGVDB:
         return pnode, poff
 
 
+def backproject_depth(depth, intr, poses, voxel, origin, dmin=0.5, dmax=8.0, out=None, stream=None):
+    """Index-space points of every pixel of a batch of depth frames (one launch).
+    depth: (F,H,W) float32 device tensor; poses: (F,12) device tensor; intr:
+    (fx, fy, cx, cy).  Returns (F*H*W, 3); invalid pixels are (0, 0, NOHIT)."""
+    import torch
+    F, H, W = depth.shape
+    if out is None:
+        out = torch.empty((F * H * W, 3), dtype=torch.float32, device=depth.device)
+    fi = (C.c_float * 4)(*[float(x) for x in intr])
+    fo = (C.c_float * 3)(*[float(x) for x in origin])
+    handle = getattr(stream, "cuda_stream", stream) if stream is not None else torch.cuda.current_stream().cuda_stream
+    call("gvdbx_backproject_depth", C.c_void_p(_ptr(depth.contiguous())), int(F), int(W), int(H), fi,
+         C.c_void_p(_ptr(poses.contiguous())), float(voxel), fo, float(dmin), float(dmax), C.c_void_p(_ptr(out)),
+         C.c_void_p(handle))
+    return out
+
+
 def gen_shell(seed, n, center=(256.0, 256.0, 256.0), r0=180.0, r1=200.0):
     """Seeded points uniform by volume in a spherical shell (host, float32 (n,3))."""
     out = np.zeros((n, 3), dtype=np.float32)

@@ -73,6 +73,8 @@ _SIGS = {
     "gvdbx_raycast_ea_backward": [_vp, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp, _vp, _vp, _vp],
     "gvdbx_apron_adjoint": [_vp, _vp],
     "gvdbx_splat_cubic_backward": [_vp, _i, _f, _f, _f, _f, _vp, _vp, _vp],
+    "gvdbx_backproject_depth": [_vp, _i, _i, _i, C.POINTER(C.c_float), _vp, _f, C.POINTER(C.c_float), _f, _f, _vp,
+                                _vp],
     "gvdbx_get_info": [_vp, C.POINTER(Info)],
     "gvdbx_read_nodes": [_vp, _i, _vp, _i64],
     "gvdbx_read_child_lists": [_vp, _i, _vp, _i64],

@@ -1,0 +1,80 @@
+"""Seeded synthetic inputs for the batched workloads (host, numpy).
+
+depth_scene(): frames of a room of random planes and spheres seen from
+cameras on an orbit, as a depth sensor would report them:
+  - 20 primitives: a floor (y = -1.5 m), a ceiling (y = 2.5 m), 4 walls with
+    random horizontal normals at distance U(4.5,6) m from the centre (outside
+    the camera orbit), 14 spheres with centres in the ball of radius 2.5 m and
+    radii U(0.15,0.8) m (inside the orbit);
+  - camera f on a 4 m orbit around the centre, looking at it, angle
+    2*pi*f/F plus a seeded jitter, height 0.3 + 0.2 sin(3*angle);
+  - pinhole fx = fy = 525, cx = (W-1)/2, cy = (H-1)/2; depth = distance along
+    the optical axis of the first hit; no hit or depth outside [dmin, dmax]
+    reads as invalid (NaN), and a seeded mask of U(5%,60%) of the pixels per
+    frame is invalid as well.
+poses are camera-to-world [R row-major (3x3) | t] with camera axes x right,
+y down, z forward, the convention gvdbx_backproject_depth expects.
+"""
+import numpy as np
+
+
+def _look_at(pos, target, up=(0.0, 1.0, 0.0)):
+    f = np.asarray(target, np.float64) - pos
+    f /= np.linalg.norm(f)
+    r = np.cross(f, up)
+    r /= np.linalg.norm(r)
+    d = np.cross(f, r)
+    return np.stack([r, d, f], axis=1)   # columns: right, down, forward
+
+
+def depth_scene(seed, nframes=64, width=640, height=480, fx=525.0, fy=525.0, dmin=0.5, dmax=8.0):
+    rng = np.random.default_rng(seed)
+    cx, cy = (width - 1) / 2.0, (height - 1) / 2.0
+    planes = [(np.array([0.0, 1.0, 0.0]), 1.5), (np.array([0.0, -1.0, 0.0]), 2.5)]   # n.x + dist = 0
+    for _ in range(4):
+        a = rng.uniform(0, 2 * np.pi)
+        planes.append((np.array([np.cos(a), 0.0, np.sin(a)]), rng.uniform(4.5, 6.0)))
+    spheres = []
+    for _ in range(14):
+        c = rng.uniform(-2.5, 2.5, size=3)
+        while np.linalg.norm(c) > 2.5:
+            c = rng.uniform(-2.5, 2.5, size=3)
+        spheres.append((c, rng.uniform(0.15, 0.8)))
+    u = np.arange(width, dtype=np.float64)[None, :]
+    v = np.arange(height, dtype=np.float64)[:, None]
+    dirs_c = np.stack(np.broadcast_arrays((u - cx) / fx, (v - cy) / fy, np.ones((height, width))), -1)
+    depth = np.empty((nframes, height, width), np.float32)
+    poses = np.empty((nframes, 12), np.float32)
+    for f in range(nframes):
+        ang = 2 * np.pi * f / nframes + rng.uniform(-0.02, 0.02)
+        pos = np.array([4.0 * np.cos(ang), 0.3 + 0.2 * np.sin(3 * ang), 4.0 * np.sin(ang)])
+        R = _look_at(pos, (0.0, 0.0, 0.0))
+        poses[f, :9] = R.reshape(-1)
+        poses[f, 9:] = pos
+        dw = dirs_c @ R.T                                      # world direction per unit depth
+        best = np.full((height, width), np.inf)
+        for n, dist in planes:
+            den = dw @ n
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = -(pos @ n + dist) / den
+            t[~(t > 0)] = np.inf
+            best = np.minimum(best, t)
+        for c, r in spheres:
+            oc = pos - c
+            a = np.einsum("hwk,hwk->hw", dw, dw)
+            b = 2.0 * (dw @ oc)
+            cc = oc @ oc - r * r
+            disc = b * b - 4 * a * cc
+            with np.errstate(invalid="ignore"):
+                sq = np.sqrt(disc)
+            t1 = (-b - sq) / (2 * a)
+            t2 = (-b + sq) / (2 * a)
+            t = np.where(t1 > 0, t1, t2)
+            t[~(disc >= 0) | ~(t > 0)] = np.inf
+            best = np.minimum(best, t)
+        d = best.astype(np.float32)
+        d[~((d >= dmin) & (d <= dmax))] = np.nan
+        frac = rng.uniform(0.05, 0.60)
+        d[rng.random((height, width)) < frac] = np.nan
+        depth[f] = d
+    return depth, poses, np.array([fx, fy, cx, cy], np.float32)

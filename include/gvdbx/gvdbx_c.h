@@ -196,6 +196,22 @@ int gvdbx_apron_adjoint(gvdbx_volume* v, float* grad_dev);
 int gvdbx_splat_cubic_backward(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz,
                                const float* grad_dev, float* dpos_dev, float* dw_dev);
 
+/* ---- depth frames -> points (extension for batched fusion) ---- */
+/* Back-projects nframes depth images (row-major, width*height floats each,
+ * metres) into index-space points, all frames in one launch.  intr4 = fx,
+ * fy, cx, cy (host); poses_dev: 12 floats per frame, the camera-to-world
+ * rotation R (row-major 3x3) then the translation t.  Pixel (u,v) with depth
+ * d in [dmin, dmax] gives, in FP32 with no contraction,
+ *   c = (d*((u-cx)/fx), d*((v-cy)/fy), d),
+ *   w_i = ((R_i0*c.x + R_i1*c.y) + R_i2*c.z) + t_i,
+ *   p_i = (w_i - origin_i) / voxel
+ * (the world-to-index map of SetTransform, VG.h:672, with a uniform voxel
+ * size); other pixels get the invalid point (0, 0, NOHIT), which every stage
+ * skips.  out_xyz_dev: nframes*height*width float3. */
+int gvdbx_backproject_depth(const float* depth_dev, int nframes, int width, int height, const float* intr4,
+                            const float* poses_dev, float voxel, const float* origin3, float dmin, float dmax,
+                            float* out_xyz_dev, void* stream);
+
 /* ---- readback (host copies; synchronising) ---- */
 int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out);
 /* pool group 0 of one level: 64-byte records (VG.h getNode / AL.cpp PoolFetch) */

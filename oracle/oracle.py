@@ -377,3 +377,33 @@ class OracleVolume:
                                                  self._dp(g), self._dp(dpos), self._dp(dw)),
              "splat_cubic_backward_f64")
         return dpos, dw
+
+
+def backproject_depth(depth, intr, poses, voxel, origin, dmin=0.5, dmax=8.0):
+    """Depth frames -> index-space points, FP32 in the operand order of
+    gvdbx_backproject_depth (numpy float32 arithmetic has no contraction):
+    c = (d*((u-cx)/fx), d*((v-cy)/fy), d); w_i = ((R_i0 c0 + R_i1 c1) + R_i2 c2) + t_i;
+    p = (w - origin) / voxel; pixels with d outside [dmin, dmax] -> (0, 0, 1e10)."""
+    f32 = np.float32
+    depth = np.asarray(depth, f32)
+    poses = np.asarray(poses, f32)
+    F, H, W = depth.shape
+    fx, fy, cx, cy = (f32(x) for x in intr)
+    ox, oy, oz = (f32(x) for x in origin)
+    vox = f32(voxel)
+    xn = (np.arange(W, dtype=f32)[None, :] - cx) / fx
+    yn = (np.arange(H, dtype=f32)[:, None] - cy) / fy
+    out = np.zeros((F, H, W, 3), f32)
+    out[..., 2] = f32(1.0e10)
+    for f in range(F):
+        d = depth[f]
+        with np.errstate(invalid="ignore"):
+            ok = (d >= f32(dmin)) & (d <= f32(dmax))
+        c0, c1, c2 = d * xn, d * yn, d
+        P = poses[f]
+        w0 = ((P[0] * c0 + P[1] * c1) + P[2] * c2) + P[9]
+        w1 = ((P[3] * c0 + P[4] * c1) + P[5] * c2) + P[10]
+        w2 = ((P[6] * c0 + P[7] * c1) + P[8] * c2) + P[11]
+        p = np.stack([(w0 - ox) / vox, (w1 - oy) / vox, (w2 - oz) / vox], -1)
+        out[f][ok] = p[ok]
+    return out.reshape(-1, 3)
