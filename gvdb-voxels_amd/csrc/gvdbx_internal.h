@@ -103,6 +103,26 @@ __device__ __forceinline__ float gvdbx_d2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
+// Batched clouds share one packed integer index space.  Cloud c owns points
+// [pt_off[c], pt_off[c+1]) in its own FP32 coordinates; its level-0 cells
+// are shifted by cell_off[3c..3c+2] in the packed space (an integer shift:
+// float positions and distances stay cloud-local and exact).  ncloud == 0
+// means one cloud and no shift.
+struct CloudMap {
+    int ncloud;
+    const int64_t* pt_off;
+    const int32_t* cell_off;
+    __device__ __forceinline__ int cloud_of_point(int64_t i) const {
+        int lo = 0, hi = ncloud - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pt_off[mid] <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    }
+};
+
 #define GVDBX_CHECK(call)                                                          \
     do {                                                                            \
         hipError_t _e = (call);                                                     \

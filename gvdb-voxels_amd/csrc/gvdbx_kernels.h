@@ -20,13 +20,15 @@ struct ReachParams {
     int logr0;
     int use_mask;      // 1: 3x3x3 reach mask per cell; 0: no extra bricks (radius < 1)
     float radius;
+    CloudMap cm;       // batched clouds (ncloud 0: none)
 };
 // generic path for radii beyond half a brick: mark every box cell directly
 void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid g0, int range0,
                        float radius, uint8_t* all0, hipStream_t s);
 // base0 = reach0 != 0; all0 |= base0 and every reached neighbour
+// allowed (optional): per g0 cell, >= 0 where extra bricks may be created
 void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, uint8_t* all0, int generic,
-                         hipStream_t s);
+                         const int32_t* allowed, hipStream_t s);
 // level l flags from level l-1 flags (children gather)
 void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
                      uint8_t* base_p, uint8_t* all_p, hipStream_t s);
@@ -97,6 +99,8 @@ void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStr
 
 // ---------------- points -> voxels ----------------
 struct GatherParams {
+    const int32_t* leaf_cloud;   // batched clouds: cloud of each leaf (else null)
+    const int32_t* cell_off;     // 3 per cloud, level-0 cells
     const GNode* leaves;
     int64_t nleaf;
     CellGrid gb;
@@ -118,6 +122,8 @@ struct GatherParams {
 void launch_gather(GatherParams gp, hipStream_t s);
 
 struct SplatParams {
+    const int32_t* leaf_cloud;   // batched clouds: cloud of each leaf (else null)
+    const int32_t* cell_off;     // 3 per cloud, level-0 cells
     const GNode* leaves;
     int64_t nleaf;
     CellGrid gb;
@@ -190,8 +196,8 @@ void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 // ---------------- binning by radix sort ----------------
 // key[i] = bin cell of point i (gb.count() for invalid / outside points),
 // val[i] = i
-void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, uint32_t* key,
-                     uint32_t* val, hipStream_t s);
+void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, CloudMap cm,
+                     uint32_t* key, uint32_t* val, hipStream_t s);
 // sorted[j] = (w of point val[j], val[j]) for keys < ncell
 void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
                        const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s);
@@ -215,6 +221,11 @@ struct RenderCam {
     int width, height;
     float org[3], org_u[3], org_v[3];
     float dir[3], dir_u[3], dir_v[3];
+    // batched clouds: the camera's cloud sits shifted by `shift` voxels in the
+    // packed tree, and its rays are bounded by its own leaf bounds
+    int shift[3];
+    int has_bounds;
+    float bmin[3], bmax[3];
 };
 struct RenderScene {
     float step, extinct, albedo, minval, alphacut, thresh, vmin, vmax, epsilon;
@@ -234,12 +245,15 @@ struct VisitRec {
 };
 void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba, uchar4* rgba8,
                    hipStream_t s);
-void launch_raycast_ea(RenderTree tv, RenderCam cam, RenderScene sc, float* I, float* T, int record, VisitRec vr,
-                       hipStream_t s);
+// cams (device, ncam entries, all of cam's size) renders one image per
+// camera, image c at c*width*height; cams == nullptr renders cam alone
+void launch_raycast_ea(RenderTree tv, RenderCam cam, const RenderCam* cams, int ncam, RenderScene sc, float* I,
+                       float* T, int record, VisitRec vr, hipStream_t s);
 
 struct EABackParams {
     RenderTree tv;
     RenderCam cam;
+    const RenderCam* cams;  // batched: camera of ray r is cams[r / (width*height)]
     RenderScene sc;
     const uint64_t* key;    // sorted segment keys
     const uint32_t* idx;    // segment record index of each sorted key
@@ -256,6 +270,7 @@ void launch_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg,
 void launch_ea_backward(EABackParams p, hipStream_t s);
 
 struct SplatBackParams {
+    CloudMap cm;
     CellGrid g0;
     const int32_t* idx0;
     const GNode* leaves;
@@ -279,3 +294,22 @@ void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s);
 void launch_backproject(const float* depth, int nframes, int width, int height, const float* intr4,
                         const float* poses, float voxel, const float* origin3, float dmin, float dmax, float* out,
                         hipStream_t s);
+
+// per-cloud bounding boxes over (cloud, chunk) tiles: tiles[2t] = cloud,
+// tiles[2t+1] = first point of the chunk (chunks of BBOX_CHUNK points)
+constexpr int BBOX_CHUNK = 8192;
+void launch_bbox_clouds(PointSrc src, const int32_t* tiles, int64_t ntiles, const int64_t* pt_off, BBoxAccum* acc,
+                        hipStream_t s);
+void launch_bbox_clouds_init(BBoxAccum* acc, int ncloud, hipStream_t s);
+// cell_cloud[g0 cell] = c over each cloud's box of cells (boxes 6 ints per
+// cloud, packed cells, inclusive lo / exclusive hi); tiles as for bboxes over
+// CELL_CHUNK cells of each box
+constexpr int CELL_CHUNK = 4096;
+void launch_paint_clouds(CellGrid g0, const int32_t* boxes, const int32_t* tiles, int64_t ntiles, int32_t* cell_cloud,
+                         hipStream_t s);
+// leaf_cloud[k] = cell_cloud[cell of leaf k]
+void launch_leaf_clouds(const GNode* leaves, int64_t nleaf, CellGrid g0, int logr0, const int32_t* cell_cloud,
+                        int32_t* leaf_cloud, hipStream_t s);
+// per-cloud leaf bounds (6 ints per cloud, packed voxels): min mPos, max mPos + range0
+void launch_leaf_bounds_clouds(const GNode* leaves, int64_t nleaf, int range0, const int32_t* leaf_cloud,
+                               int32_t* out6, hipStream_t s);

@@ -159,10 +159,6 @@ struct RayState {
     V3 hit, norm;
 };
 
-__device__ __forceinline__ V3 leaf_pos(const GNode* nodes, int64_t leaf) {
-    const int4 q = *(const int4*)(&nodes[leaf]);   // mLev.. | mPos[0..2]
-    return v3((float)q.y, (float)q.z, (float)q.w);
-}
 
 // getGradient RC.cuh:132-142 (sign -1) / getGradientLevelSet RC.cuh:146-157 (+1)
 template <int BR>
@@ -185,12 +181,18 @@ __device__ __forceinline__ V3 gradient(const float* B, int br, V3 p, bool ls) {
 // raySurfaceTrilinearBrick RC.cuh:281-300, rayLevelSetBrick RC.cuh:389-410
 // (its rayLevelSet repeats the caller's test at the same point and returns
 // p + vmin at once).
+// node corner in the camera's (cloud-local) coordinates
+__device__ __forceinline__ V3 node_pos(const GNode* nodes, int64_t id, const int* sh) {
+    const int4 q = *(const int4*)(&nodes[id]);   // mLev.. | mPos[0..2]
+    return v3((float)(q.y - sh[0]), (float)(q.z - sh[1]), (float)(q.w - sh[2]));
+}
+
 template <int MODE, int BR>
 __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene& sc, int64_t leaf, V3 t, V3 pos,
-                                         V3 dir, RayState& st) {
+                                         V3 dir, RayState& st, const int* sh) {
     const int br = BR > 0 ? BR : tv.brick_res;
     const float* B = tv.chan + leaf * (int64_t)(br * br * br);
-    const V3 vmin = leaf_pos(tv.nodes[0], leaf);
+    const V3 vmin = node_pos(tv.nodes[0], leaf, sh);
     const float r0 = (float)tv.res[0];
     if (MODE == MODE_DEEP_LUT || MODE == MODE_DEEP_LINEAR) {
         const float step = sc.step;
@@ -268,17 +270,18 @@ __device__ __forceinline__ void record_visit(const VisitRec& vr, int64_t leaf, u
 }
 
 template <int MODE, int BR, bool RECORD>
-__device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, V3 pos, V3 dir, RayState& st, uint32_t ray,
-                         const VisitRec& vr) {
+__device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, const RenderCam& cam, V3 pos, V3 dir,
+                         RayState& st, uint32_t ray, const VisitRec& vr) {
     const int top = tv.top_lev;
     if (top < 1 || top >= tv.levs || tv.ncnt[top] < 1) return;
+    const int* sh = cam.shift;
     int64_t nodeid[GVDBX_MAXLEV];
     float tmax[GVDBX_MAXLEV];
     int lev = top;
     nodeid[lev] = 0;
-    V3 ts = ray_box(pos, dir, tv.bmin, tv.bmax);
+    V3 ts = cam.has_bounds ? ray_box(pos, dir, cam.bmin, cam.bmax) : ray_box(pos, dir, tv.bmin, tv.bmax);
     uint64_t list = tv.nodes[lev][0].mChildList;
-    V3 vmin = leaf_pos(tv.nodes[lev], 0);
+    V3 vmin = node_pos(tv.nodes[lev], 0, sh);
     if (ts.z == GVDBX_NOHIT) return;
     ts.x += sc.epsilon;
     tmax[lev] = ts.y - sc.epsilon;
@@ -296,7 +299,7 @@ __device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, V3 pos, V3
             if (lev == 1) {
                 d.t.x += sc.epsilon;
                 if (RECORD) record_visit(vr, child, ray, d.t.x, st.clr[3], st.clr[0]);
-                brick_fn<MODE, BR>(tv, sc, child, d.t, pos, dir, st);
+                brick_fn<MODE, BR>(tv, sc, child, d.t, pos, dir, st, sh);
                 if (st.clr[3] <= 0) {
                     st.clr[3] = 0;
                     return;
@@ -307,7 +310,7 @@ __device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, V3 pos, V3
                 lev--;
                 nodeid[lev] = child;
                 list = tv.nodes[lev][child].mChildList;
-                vmin = leaf_pos(tv.nodes[lev], child);
+                vmin = node_pos(tv.nodes[lev], child, sh);
                 d.t.x += sc.epsilon;
                 tmax[lev] = d.t.y - sc.epsilon;
                 d.prepare(vmin, tv.vdel[lev]);
@@ -319,7 +322,7 @@ __device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, V3 pos, V3
             lev++;
             if (lev <= top) {
                 list = tv.nodes[lev][nodeid[lev]].mChildList;
-                vmin = leaf_pos(tv.nodes[lev], nodeid[lev]);
+                vmin = node_pos(tv.nodes[lev], nodeid[lev], sh);
                 d.prepare(vmin, tv.vdel[lev]);
             }
         }
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(256) void k_render_volume(RenderTree tv, RenderCam 
     st.clr[3] = 1.0f;
     st.hit = v3(0, 0, GVDBX_NOHIT);
     VisitRec none{};
-    ray_cast<MODE_DEEP_LUT, BR, false>(tv, sc, pos, dir, st, 0, none);
+    ray_cast<MODE_DEEP_LUT, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
     const float a = 1.0f - st.clr[3];
     float4 o;
     o.x = sc.backclr[0] + a * (st.clr[0] - sc.backclr[0]);
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
     st.hit = MODE == MODE_SURF_TRI ? v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT) : v3(0, 0, GVDBX_NOHIT);
     st.norm = v3(0, 0, 0);
     VisitRec none{};
-    ray_cast<MODE, BR, false>(tv, sc, pos, dir, st, 0, none);
+    ray_cast<MODE, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
     float4 o;
     if (st.hit.z == GVDBX_NOHIT) {
         o = make_float4(sc.backclr[0], sc.backclr[1], sc.backclr[2], sc.backclr[3]);
@@ -397,7 +400,7 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
             s2.clr[3] = 1.0f;
             s2.hit = v3(0, 0, GVDBX_NOHIT);
             s2.norm = v3(0, 0, 0);
-            ray_cast<MODE, BR, false>(tv, sc, add(st.hit, scale(st.norm, sc.shadow_bias)), ldir, s2, 0, none);
+            ray_cast<MODE, BR, false>(tv, sc, cam, add(st.hit, scale(st.norm, sc.shadow_bias)), ldir, s2, 0, none);
             if (s2.hit.z != GVDBX_NOHIT) diff = (float)((double)diff * (1.0 - (double)sc.shadow_amt));
         }
         const float sh = diff + amb;
@@ -409,10 +412,11 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
 }
 
 template <int BR, bool RECORD>
-__global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam, RenderScene sc, float* I, float* T,
-                                                    VisitRec vr) {
+__global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam0, const RenderCam* cams,
+                                                    RenderScene sc, float* I, float* T, VisitRec vr) {
+    const RenderCam& cam = cams ? cams[blockIdx.z] : cam0;
     int x, y;
-    const bool ok = pixel_of(cam, x, y);
+    const bool ok = pixel_of(cam0, x, y);
     if (!ok) return;
     V3 pos, dir;
     camera_ray(cam, x, y, pos, dir);
@@ -420,8 +424,8 @@ __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam
     st.clr[0] = st.clr[1] = st.clr[2] = 0.0f;
     st.clr[3] = 1.0f;
     st.hit = v3(0, 0, GVDBX_NOHIT);
-    const uint32_t ray = (uint32_t)((int64_t)y * cam.width + x);
-    ray_cast<MODE_DEEP_LINEAR, BR, RECORD>(tv, sc, pos, dir, st, ray, vr);
+    const uint32_t ray = (uint32_t)((int64_t)blockIdx.z * cam0.width * cam0.height + (int64_t)y * cam0.width + x);
+    ray_cast<MODE_DEEP_LINEAR, BR, RECORD>(tv, sc, cam, pos, dir, st, ray, vr);
     I[ray] = st.clr[0];
     T[ray] = st.clr[3];
 }
@@ -480,7 +484,6 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     const int32_t per_lane = (per_wave + 63) / 64;
     const int32_t c0 = w0 + lane * per_lane, c1 = min(w1, c0 + per_lane);
     float* g = grad[wave];
-    const V3 vmin = leaf_pos(p.tv.nodes[0], leaf);
     const RenderScene& sc = p.sc;
     const float r0 = (float)p.tv.res[0];
     const float e = sc.extinct * sc.step;
@@ -489,8 +492,12 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
         const uint32_t id = p.idx[c];
         const uint32_t ray = (uint32_t)(p.key[c] & rmask);
         const float4 cp = p.rec[id];
+        const uint32_t npix = (uint32_t)p.cam.width * (uint32_t)p.cam.height;
+        const RenderCam& cam = p.cams ? p.cams[ray / npix] : p.cam;
+        const uint32_t pix = ray % npix;
+        const V3 vmin = node_pos(p.tv.nodes[0], leaf, cam.shift);
         V3 pos, dir;
-        camera_ray(p.cam, (int)(ray % (uint32_t)p.cam.width), (int)(ray / (uint32_t)p.cam.width), pos, dir);
+        camera_ray(cam, (int)(pix % (uint32_t)p.cam.width), (int)(pix / (uint32_t)p.cam.width), pos, dir);
         float T = cp.y, I = cp.z;
         const float Ifin = p.I[ray], Tfin = p.T[ray], gi = p.gI[ray];
         const float gt = p.gT ? p.gT[ray] : 0.0f;
@@ -620,6 +627,12 @@ __global__ __launch_bounds__(256) void k_splat_backward(SplatBackParams p) {
          j += (int64_t)gridDim.x * blockDim.x) {
         const float4 q = p.sorted[j];
         const int i = __float_as_int(q.w);
+        int sh[3] = {0, 0, 0};   // cloud shift in level-0 cells
+        if (p.cm.ncloud > 0) {
+            const int cl = p.cm.cloud_of_point(i);
+#pragma unroll
+            for (int a = 0; a < 3; a++) sh[a] = p.cm.cell_off[3 * cl + a];
+        }
         const float wt = p.weights ? *(const float*)(p.weights + (int64_t)i * p.wstride) : 1.0f;
         const int x0 = max((int)floorf(q.x - p.radius), 0), x1 = (int)ceilf(q.x + p.radius);
         const int y0 = max((int)floorf(q.y - p.radius), 0), y1 = (int)ceilf(q.y + p.radius);
@@ -634,8 +647,8 @@ __global__ __launch_bounds__(256) void k_splat_backward(SplatBackParams p) {
                     const float d2 = gvdbx_d2(dx, dy, dz);
                     if (!(d2 < r2)) continue;
                     const int cx = x >> p.logr0, cy = y >> p.logr0, cz = z >> p.logr0;
-                    if (!p.g0.contains(cx, cy, cz)) continue;
-                    const int32_t leaf = p.idx0[p.g0.index(cx, cy, cz)];
+                    if (!p.g0.contains(cx + sh[0], cy + sh[1], cz + sh[2])) continue;
+                    const int32_t leaf = p.idx0[p.g0.index(cx + sh[0], cy + sh[1], cz + sh[2])];
                     if (leaf < 0 || leaf >= p.nleaf) continue;
                     const int lx = x - (cx << p.logr0) + 1, ly = y - (cy << p.logr0) + 1, lz = z - (cz << p.logr0) + 1;
                     const float g = p.grad[(int64_t)leaf * nv + (lz * br + ly) * br + lx];
@@ -680,16 +693,16 @@ void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, floa
     }
 }
 
-void launch_raycast_ea(RenderTree tv, RenderCam cam, RenderScene sc, float* I, float* T, int record, VisitRec vr,
-                       hipStream_t s) {
-    dim3 grid((unsigned)((cam.width + 15) / 16), (unsigned)((cam.height + 15) / 16));
+void launch_raycast_ea(RenderTree tv, RenderCam cam, const RenderCam* cams, int ncam, RenderScene sc, float* I,
+                       float* T, int record, VisitRec vr, hipStream_t s) {
+    dim3 grid((unsigned)((cam.width + 15) / 16), (unsigned)((cam.height + 15) / 16), (unsigned)(cams ? ncam : 1));
     const bool b10 = tv.brick_res == 10;
     if (record) {
-        if (b10) hipLaunchKernelGGL((k_raycast_ea<10, true>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
-        else hipLaunchKernelGGL((k_raycast_ea<0, true>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
+        if (b10) hipLaunchKernelGGL((k_raycast_ea<10, true>), grid, dim3(256), 0, s, tv, cam, cams, sc, I, T, vr);
+        else hipLaunchKernelGGL((k_raycast_ea<0, true>), grid, dim3(256), 0, s, tv, cam, cams, sc, I, T, vr);
     } else {
-        if (b10) hipLaunchKernelGGL((k_raycast_ea<10, false>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
-        else hipLaunchKernelGGL((k_raycast_ea<0, false>), grid, dim3(256), 0, s, tv, cam, sc, I, T, vr);
+        if (b10) hipLaunchKernelGGL((k_raycast_ea<10, false>), grid, dim3(256), 0, s, tv, cam, cams, sc, I, T, vr);
+        else hipLaunchKernelGGL((k_raycast_ea<0, false>), grid, dim3(256), 0, s, tv, cam, cams, sc, I, T, vr);
     }
 }
 

@@ -47,7 +47,12 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
     int tox, toy, toz;
     tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
     const GNode nd = gp.leaves[k];
-    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    int shx = 0, shy = 0, shz = 0;   // cloud-local voxel coordinates (batched clouds)
+    if (gp.leaf_cloud) {
+        const int cl = gp.leaf_cloud[k];
+        shx = gp.cell_off[3 * cl] * res0, shy = gp.cell_off[3 * cl + 1] * res0, shz = gp.cell_off[3 * cl + 2] * res0;
+    }
+    const int T0x = nd.mPos[0] - shx + tox, T0y = nd.mPos[1] - shy + toy, T0z = nd.mPos[2] - shz + toz;
     const int lx = lane % td, ly = lane / td;
     const bool lane_ok = ly < td;
     const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
@@ -169,7 +174,12 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
     int tox, toy, toz;
     tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
     const GNode nd = gp.leaves[k];
-    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    int shx = 0, shy = 0, shz = 0;   // cloud-local voxel coordinates (batched clouds)
+    if (gp.leaf_cloud) {
+        const int cl = gp.leaf_cloud[k];
+        shx = gp.cell_off[3 * cl] * res0, shy = gp.cell_off[3 * cl + 1] * res0, shz = gp.cell_off[3 * cl + 2] * res0;
+    }
+    const int T0x = nd.mPos[0] - shx + tox, T0y = nd.mPos[1] - shy + toy, T0z = nd.mPos[2] - shz + toz;
     for (int i = lane; i < nv; i += 64) acc[i] = 0x7F800000u;   // +inf
     const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
     const float R = gp.box_radius;
@@ -316,7 +326,12 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
     int tox, toy, toz;
     tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
     const GNode nd = gp.leaves[k];
-    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    int shx = 0, shy = 0, shz = 0;   // cloud-local voxel coordinates (batched clouds)
+    if (gp.leaf_cloud) {
+        const int cl = gp.leaf_cloud[k];
+        shx = gp.cell_off[3 * cl] * res0, shy = gp.cell_off[3 * cl + 1] * res0, shz = gp.cell_off[3 * cl + 2] * res0;
+    }
+    const int T0x = nd.mPos[0] - shx + tox, T0y = nd.mPos[1] - shy + toy, T0z = nd.mPos[2] - shz + toz;
     const int lx = lane % td, ly = lane / td;
     const bool lane_ok = ly < td;
     const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
@@ -472,7 +487,12 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
     int tox, toy, toz;
     tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
     const GNode nd = sp.leaves[k];
-    const int T0x = nd.mPos[0] + tox, T0y = nd.mPos[1] + toy, T0z = nd.mPos[2] + toz;
+    int shx = 0, shy = 0, shz = 0;   // cloud-local voxel coordinates (batched clouds)
+    if (sp.leaf_cloud) {
+        const int cl = sp.leaf_cloud[k];
+        shx = sp.cell_off[3 * cl] * res0, shy = sp.cell_off[3 * cl + 1] * res0, shz = sp.cell_off[3 * cl + 2] * res0;
+    }
+    const int T0x = nd.mPos[0] - shx + tox, T0y = nd.mPos[1] - shy + toy, T0z = nd.mPos[2] - shz + toz;
     const int br = sp.brick_res, a = sp.apron;
     float* brick = sp.chan + (int64_t)k * br * br * br;
     for (int i = lane; i < nv; i += 64) {

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_mark_boxes(PointSrc src, int64_t n, flo
 }
 
 __global__ __launch_bounds__(256) void k_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0,
-                                                      uint8_t* all0, int generic) {
+                                                      uint8_t* all0, int generic, const int32_t* allowed) {
     int64_t n = g0.count();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint32_t m = reach0[i];
@@ -288,7 +288,10 @@ __global__ __launch_bounds__(256) void k_expand_reach(CellGrid g0, const uint32_
             int b = __ffs(m) - 1;
             m &= m - 1;
             int dx = b % 3 - 1, dy = (b / 3) % 3 - 1, dz = b / 9 - 1;
-            if (g0.contains(cx + dx, cy + dy, cz + dz)) all0[g0.index(cx + dx, cy + dy, cz + dz)] = 1;
+            if (g0.contains(cx + dx, cy + dy, cz + dz)) {
+                const int64_t t = g0.index(cx + dx, cy + dy, cz + dz);
+                if (!allowed || allowed[t] >= 0) all0[t] = 1;
+            }
         }
     }
 }
@@ -422,11 +425,16 @@ __global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, cons
         if (j < total) {
             float4 p = sorted[j];
             float w[3] = {p.x, p.y, p.z};
-            int c[3];
+            int c[3], sh[3] = {0, 0, 0};
 #pragma unroll
             for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
-            if (rp.g0.contains(c[0], c[1], c[2])) {
-                key = (int)rp.g0.index(c[0], c[1], c[2]);
+            if (rp.cm.ncloud > 0) {
+                const int cl = rp.cm.cloud_of_point(__float_as_int(p.w));
+#pragma unroll
+                for (int a = 0; a < 3; a++) sh[a] = rp.cm.cell_off[3 * cl + a];
+            }
+            if (rp.g0.contains(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2])) {
+                key = (int)rp.g0.index(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
                 m = 1u << 13;
                 if (rp.use_mask) {
                     int lo[3], hi[3];
@@ -504,7 +512,7 @@ __global__ __launch_bounds__(256) void k_subcell_len(const float4* sorted, const
 // value its index.  A stable sort then leaves every bin in ascending point
 // order.
 __global__ __launch_bounds__(256) void k_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb,
-                                                  int logr0, uint32_t* key, uint32_t* val) {
+                                                  int logr0, CloudMap cm, uint32_t* key, uint32_t* val) {
     const uint32_t none = (uint32_t)gb.count();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float w[3];
@@ -513,6 +521,11 @@ __global__ __launch_bounds__(256) void k_bin_keys(PointSrc src, int64_t n, float
             int c[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], logr0);
+            if (cm.ncloud > 0) {
+                const int cl = cm.cloud_of_point(i);
+#pragma unroll
+                for (int a = 0; a < 3; a++) c[a] += cm.cell_off[3 * cl + a];
+            }
             if (gb.contains(c[0], c[1], c[2])) k = (uint32_t)gb.index(c[0], c[1], c[2]);
         }
         key[i] = k;
@@ -557,6 +570,134 @@ __global__ __launch_bounds__(256) void k_leaf_bits(const int32_t* idx0, int64_t 
     }
 }
 
+// ---- batched clouds ----
+// per-cloud bounding boxes: one block per (cloud, chunk) tile, block
+// reduction, one set of atomics per block into that cloud's accumulator
+__global__ __launch_bounds__(256) void k_bbox_clouds(PointSrc src, const int32_t* tiles, const int64_t* pt_off,
+                                                     BBoxAccum* acc) {
+    const int cl = tiles[2 * blockIdx.x];
+    const int64_t b0 = pt_off[cl] + (int64_t)tiles[2 * blockIdx.x + 1];
+    const int64_t b1 = min(b0 + (int64_t)BBOX_CHUNK, pt_off[cl + 1]);
+    float mn0 = __builtin_inff(), mn1 = __builtin_inff(), mn2 = __builtin_inff();
+    float mx0 = -__builtin_inff(), mx1 = -__builtin_inff(), mx2 = -__builtin_inff();
+    unsigned long long cnt = 0;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        float x, y, z;
+        if (!gvdbx_point(src, i, 0.0f, 0.0f, 0.0f, x, y, z)) continue;
+        cnt++;
+        mn0 = fminf(mn0, x); mn1 = fminf(mn1, y); mn2 = fminf(mn2, z);
+        mx0 = fmaxf(mx0, x); mx1 = fmaxf(mx1, y); mx2 = fmaxf(mx2, z);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        mn0 = fminf(mn0, __shfl_xor(mn0, d, 64));
+        mn1 = fminf(mn1, __shfl_xor(mn1, d, 64));
+        mn2 = fminf(mn2, __shfl_xor(mn2, d, 64));
+        mx0 = fmaxf(mx0, __shfl_xor(mx0, d, 64));
+        mx1 = fmaxf(mx1, __shfl_xor(mx1, d, 64));
+        mx2 = fmaxf(mx2, __shfl_xor(mx2, d, 64));
+        cnt += __shfl_xor(cnt, d, 64);
+    }
+    __shared__ float red[4][6];
+    __shared__ unsigned long long redc[4];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        red[w][0] = mn0; red[w][1] = mn1; red[w][2] = mn2;
+        red[w][3] = mx0; red[w][4] = mx1; red[w][5] = mx2;
+        redc[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+            mn0 = fminf(mn0, red[i][0]); mn1 = fminf(mn1, red[i][1]); mn2 = fminf(mn2, red[i][2]);
+            mx0 = fmaxf(mx0, red[i][3]); mx1 = fmaxf(mx1, red[i][4]); mx2 = fmaxf(mx2, red[i][5]);
+            cnt += redc[i];
+        }
+        if (cnt > 0) {
+            BBoxAccum* a = acc + cl;
+            atomicMin(&a->mn[0], __float_as_int(mn0));
+            atomicMin(&a->mn[1], __float_as_int(mn1));
+            atomicMin(&a->mn[2], __float_as_int(mn2));
+            atomicMax(&a->mx[0], __float_as_int(mx0));
+            atomicMax(&a->mx[1], __float_as_int(mx1));
+            atomicMax(&a->mx[2], __float_as_int(mx2));
+            atomicAdd(&a->nvalid, cnt);
+        }
+    }
+}
+
+__global__ void k_bbox_clouds_init(BBoxAccum* acc, int ncloud) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncloud; c += gridDim.x * blockDim.x) {
+        for (int a = 0; a < 3; a++) {
+            acc[c].mn[a] = 0x7F800000;
+            acc[c].mx[a] = -1;
+        }
+        acc[c].nvalid = 0;
+    }
+}
+
+// cell_cloud over each cloud's cell box, one block per (cloud, chunk) tile
+__global__ __launch_bounds__(256) void k_paint_clouds(CellGrid g0, const int32_t* boxes, const int32_t* tiles,
+                                                      int32_t* cell_cloud) {
+    const int cl = tiles[2 * blockIdx.x];
+    const int64_t c0 = tiles[2 * blockIdx.x + 1];
+    const int32_t* b = boxes + 6 * cl;
+    const int64_t nx = b[3] - b[0], ny = b[4] - b[1], nz = b[5] - b[2];
+    const int64_t nb = nx * ny * nz;
+    for (int64_t t = c0 + threadIdx.x; t < min(c0 + (int64_t)CELL_CHUNK, nb); t += blockDim.x) {
+        const int x = b[0] + (int)(t / (ny * nz)), y = b[1] + (int)((t / nz) % ny), z = b[2] + (int)(t % nz);
+        if (g0.contains(x, y, z)) cell_cloud[g0.index(x, y, z)] = cl;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_clouds(const GNode* leaves, int64_t nleaf, CellGrid g0, int logr0,
+                                                     const int32_t* cell_cloud, int32_t* leaf_cloud) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nleaf; k += (int64_t)gridDim.x * blockDim.x) {
+        const int x = leaves[k].mPos[0] >> logr0, y = leaves[k].mPos[1] >> logr0, z = leaves[k].mPos[2] >> logr0;
+        leaf_cloud[k] = g0.contains(x, y, z) ? cell_cloud[g0.index(x, y, z)] : -1;
+    }
+}
+
+// per-cloud leaf bounds; leaves come in key order, so a wave's leaves mostly
+// share a cloud: the wave reduces when they all do, else lanes go alone
+__global__ __launch_bounds__(256) void k_leaf_bounds_clouds(const GNode* leaves, int64_t nleaf, int range0,
+                                                            const int32_t* leaf_cloud, int32_t* out6) {
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nleaf; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = base + threadIdx.x;
+        int cl = -1;
+        int v[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
+        if (k < nleaf && leaves[k].mFlags) {
+            cl = leaf_cloud[k];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                v[a] = leaves[k].mPos[a];
+                v[3 + a] = leaves[k].mPos[a] + range0;
+            }
+        }
+        const int c0 = __shfl(cl, 0, 64);
+        const bool same = __ballot(cl == c0 || cl < 0) == ~0ull;
+        if (same) {
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    v[a] = min(v[a], __shfl_xor(v[a], d, 64));
+                    v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], d, 64));
+                }
+            if (lane_id() == 0 && c0 >= 0)
+                for (int a = 0; a < 3; a++) {
+                    atomicMin(&out6[6 * c0 + a], v[a]);
+                    atomicMax(&out6[6 * c0 + 3 + a], v[3 + a]);
+                }
+        } else if (cl >= 0) {
+            for (int a = 0; a < 3; a++) {
+                atomicMin(&out6[6 * cl + a], v[a]);
+                atomicMax(&out6[6 * cl + 3 + a], v[3 + a]);
+            }
+        }
+    }
+}
+
 inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
     int64_t g = (n + threads - 1) / threads;
     if (g < 1) g = 1;
@@ -581,9 +722,9 @@ void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, Ce
 }
 
 void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, uint8_t* all0, int generic,
-                         hipStream_t s) {
+                         const int32_t* allowed, hipStream_t s) {
     hipLaunchKernelGGL(k_expand_reach, dim3(grid_for(g0.count(), 256)), dim3(256), 0, s, g0, reach0, base0, all0,
-                       generic);
+                       generic, allowed);
 }
 
 void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
@@ -660,10 +801,11 @@ void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax
         hipLaunchKernelGGL(k_subcell_len, dim3(grid_for(nmax, 256, 2048)), dim3(256), 0, s, sorted, total, nmax, bp, len);
 }
 
-void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, uint32_t* key,
-                     uint32_t* val, hipStream_t s) {
+void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, CloudMap cm,
+                     uint32_t* key, uint32_t* val, hipStream_t s) {
     if (n > 0)
-        hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, logr0, key, val);
+        hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, gb, logr0, cm, key,
+                           val);
 }
 
 void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
@@ -681,3 +823,32 @@ void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStr
     hipLaunchKernelGGL(k_leaf_bits, dim3(grid_for((ncell + 31) / 32, 256)), dim3(256), 0, s, idx0, ncell, bits);
 }
 
+
+void launch_bbox_clouds(PointSrc src, const int32_t* tiles, int64_t ntiles, const int64_t* pt_off, BBoxAccum* acc,
+                        hipStream_t s) {
+    if (ntiles > 0) hipLaunchKernelGGL(k_bbox_clouds, dim3((unsigned)ntiles), dim3(256), 0, s, src, tiles, pt_off, acc);
+}
+
+void launch_bbox_clouds_init(BBoxAccum* acc, int ncloud, hipStream_t s) {
+    if (ncloud > 0) hipLaunchKernelGGL(k_bbox_clouds_init, dim3(grid_for(ncloud, 256)), dim3(256), 0, s, acc, ncloud);
+}
+
+void launch_paint_clouds(CellGrid g0, const int32_t* boxes, const int32_t* tiles, int64_t ntiles, int32_t* cell_cloud,
+                         hipStream_t s) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_paint_clouds, dim3((unsigned)ntiles), dim3(256), 0, s, g0, boxes, tiles, cell_cloud);
+}
+
+void launch_leaf_clouds(const GNode* leaves, int64_t nleaf, CellGrid g0, int logr0, const int32_t* cell_cloud,
+                        int32_t* leaf_cloud, hipStream_t s) {
+    if (nleaf > 0)
+        hipLaunchKernelGGL(k_leaf_clouds, dim3(grid_for(nleaf, 256)), dim3(256), 0, s, leaves, nleaf, g0, logr0,
+                           cell_cloud, leaf_cloud);
+}
+
+void launch_leaf_bounds_clouds(const GNode* leaves, int64_t nleaf, int range0, const int32_t* leaf_cloud,
+                               int32_t* out6, hipStream_t s) {
+    if (nleaf > 0)
+        hipLaunchKernelGGL(k_leaf_bounds_clouds, dim3(grid_for(nleaf, 256)), dim3(256), 0, s, leaves, nleaf, range0,
+                           leaf_cloud, out6);
+}

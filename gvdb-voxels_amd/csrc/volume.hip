@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "gvdbx/gvdbx_c.h"
 #include "gvdbx_kernels.h"
@@ -110,9 +111,16 @@ struct gvdbx_volume {
     int64_t vis_n = 0, vis_cap = 0;
     int vis_ray_bits = 0, vis_leaf_bits = 0;
     bool vis_valid = false, vis_sorted = false, vis_in_tmp = false;
-    int vis_w = 0, vis_h = 0, vis_chan = -1;
+    int vis_w = 0, vis_h = 0, vis_chan = -1, vis_ncam = 0;
     uint64_t vis_topo_gen = 0;
     uint64_t topo_gen = 0;   // bumped by every RebuildTopology
+    // batched clouds (gvdbx_set_points_batched): cloud c owns points
+    // [pt_off[c], pt_off[c+1]), level-0 cells shifted by cell_off[3c..3c+2]
+    int ncloud = 0;
+    std::vector<int64_t> h_ptoff;
+    std::vector<int32_t> h_celloff, h_boxes;
+    std::vector<float> h_cbounds;   // 6 per cloud, cloud-local leaf bounds
+    DevBuf c_ptoff, c_celloff, c_boxes, c_tiles, c_cellcloud, c_leafcloud, c_bbox, c_bounds, c_cams;
     // pinned host staging
     void* host = nullptr;
     size_t host_cap = 0;
@@ -182,6 +190,14 @@ int brick_res(const gvdbx_volume* v) { return v->res[0] + 2 * std::max(v->apron,
 int64_t brick_vox(const gvdbx_volume* v) {
     int64_t br = brick_res(v);
     return br * br * br;
+}
+
+CloudMap cloud_map(const gvdbx_volume* v) {
+    CloudMap cm{};
+    cm.ncloud = v->ncloud;
+    cm.pt_off = v->ncloud ? v->c_ptoff.as<int64_t>() : nullptr;
+    cm.cell_off = v->ncloud ? v->c_celloff.as<int32_t>() : nullptr;
+    return cm;
 }
 
 bool any_chan(const gvdbx_volume* v) {
@@ -278,7 +294,8 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable) {
     TRY(ensure(v, v->sorted, (size_t)m * 16));
     int end_bit = 1;
     while (((int64_t)1 << end_bit) <= nb) end_bit++;
-    launch_bin_keys(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), s);
+    launch_bin_keys(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], cloud_map(v), v->bkey.as<uint32_t>(),
+                    v->bval.as<uint32_t>(), s);
     int flip = radix_sort_pairs32(v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), v->bkey2.as<uint32_t>(),
                                   v->bval2.as<uint32_t>(), n, 0, end_bit, v->rs_hist.as<int32_t>(),
                                   v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s);
@@ -324,59 +341,174 @@ int subcell_len(gvdbx_volume* v, int sc, float radius, int64_t* len_out) {
     return GVDBX_OK;
 }
 
+// GetBoundingBox (VG.cpp:5452-5463) of one accumulator, DetermineDepth and
+// the level-0 cell box the topology may occupy (the root node clipped to the
+// points' box grown by the extra-brick reach).  Shared by one volume and by
+// each cloud of a batch.
+struct CloudFrame {
+    float mn[3], mx[3], pmin[3], pmax[3];
+    int depth;
+    int root[3];
+    int lo_v[3], hi_v[3];   // voxel box, inclusive
+};
+
+int frame_of(const gvdbx_volume* v, const BBoxAccum& bb, float radius, const char* who, CloudFrame& f) {
+    if (bb.nvalid == 0) FAIL(GVDBX_ERR_NOPOINTS, "RebuildTopology: no valid point%s", who);
+    for (int a = 0; a < 3; a++) {
+        memcpy(&f.mn[a], &bb.mn[a], 4);
+        memcpy(&f.mx[a], &bb.mx[a], 4);
+        f.pmin[a] = f.mn[a] - 1;
+        f.pmax[a] = f.mx[a] + 2;
+        f.pmin[a] = 0 > f.pmin[a] ? 0 : f.pmin[a];
+        f.pmax[a] = 0 > f.pmax[a] ? 0 : f.pmax[a];
+    }
+    f.depth = determine_depth(v, f.pmin, f.pmax, f.root);
+    if (f.depth == 0) FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: point extents smaller than one brick%s", who);
+    if (f.depth < 0 || f.depth >= v->levs)
+        FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: point extents exceed the configured levels%s", who);
+    for (int a = 0; a < 3; a++)
+        if ((int64_t)f.pmax[a] / v->range[0] > 65535)
+            FAIL(GVDBX_ERR_RANGE, "RebuildTopology: extents exceed 16-bit brick keys%s", who);
+    const bool extra = radius >= 1.0f;   // VG.cpp:1207-1208
+    const int64_t rtop = v->range[f.depth];
+    for (int a = 0; a < 3; a++) {
+        int fmn = (int)f.mn[a], fmx = (int)f.mx[a];
+        if (extra) {
+            f.lo_v[a] = (int)std::max<int64_t>(f.root[a], (int64_t)fmn - (int64_t)ceilf(radius) - 1);
+            f.hi_v[a] = (int)std::min<int64_t>(f.root[a] + rtop - 1, (int64_t)fmx + (int64_t)ceilf(2.0f * radius) + 1);
+        } else {
+            f.lo_v[a] = std::max(f.root[a], fmn);
+            f.hi_v[a] = (int)std::min<int64_t>(f.root[a] + rtop - 1, fmx);
+        }
+    }
+    return GVDBX_OK;
+}
+
 int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[3]) {
     if (!v->configured) FAIL(GVDBX_ERR_STATE, "Configure has not been called");
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
     if (n < 0) FAIL(GVDBX_ERR_ARG, "negative point count");
     if (!(radius >= 0.0f) || !isfinite(radius)) FAIL(GVDBX_ERR_ARG, "radius must be finite and >= 0");
     hipStream_t s = v->stream;
-    TRY(ensure(v, v->bbox, sizeof(BBoxAccum)));
-    TRY(ensure_host(v, 4096));
-    launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
-    if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), s);
-    TRY(check_launch("bounding box"));
-    GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
-    TRY(sync(v));
-    BBoxAccum bb;
-    memcpy(&bb, v->host, sizeof(bb));
-    if (bb.nvalid == 0) FAIL(GVDBX_ERR_NOPOINTS, "RebuildTopology: no valid point");
-    // GetBoundingBox VG.cpp:5452-5463
-    float mn[3], mx[3], pmin[3], pmax[3];
-    for (int a = 0; a < 3; a++) {
-        memcpy(&mn[a], &bb.mn[a], 4);
-        memcpy(&mx[a], &bb.mx[a], 4);
-        pmin[a] = mn[a] - 1;
-        pmax[a] = mx[a] + 2;
-        pmin[a] = 0 > pmin[a] ? 0 : pmin[a];
-        pmax[a] = 0 > pmax[a] ? 0 : pmax[a];
-    }
+    const int r0 = v->res[0];
+    const bool extra = radius >= 1.0f;   // VG.cpp:1207-1208
+    const bool mask_ok = 2.0f * radius <= (float)r0 && radius + 1.0f <= (float)r0;
+    int depth;
     int root[3];
-    int depth = determine_depth(v, pmin, pmax, root);
-    if (depth == 0) FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: point extents smaller than one brick");
-    if (depth < 0 || depth >= v->levs) FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: point extents exceed the configured levels");
-    for (int a = 0; a < 3; a++)
-        if ((int64_t)pmax[a] / v->range[0] > 65535) FAIL(GVDBX_ERR_RANGE, "RebuildTopology: extents exceed 16-bit brick keys");
+    float pmin[3], pmax[3];
+    int lo_v[3], hi_v[3];
+    if (v->ncloud == 0) {
+        TRY(ensure(v, v->bbox, sizeof(BBoxAccum)));
+        TRY(ensure_host(v, 4096));
+        launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
+        if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), s);
+        TRY(check_launch("bounding box"));
+        GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
+        TRY(sync(v));
+        BBoxAccum bb;
+        memcpy(&bb, v->host, sizeof(bb));
+        CloudFrame f;
+        TRY(frame_of(v, bb, radius, "", f));
+        depth = f.depth;
+        memcpy(root, f.root, sizeof(root));
+        memcpy(pmin, f.pmin, sizeof(pmin));
+        memcpy(pmax, f.pmax, sizeof(pmax));
+        memcpy(lo_v, f.lo_v, sizeof(lo_v));
+        memcpy(hi_v, f.hi_v, sizeof(hi_v));
+    } else {
+        // ---- batched clouds: per-cloud frames, then one packed index space ----
+        if (orig[0] != 0.0f || orig[1] != 0.0f || orig[2] != 0.0f)
+            FAIL(GVDBX_ERR_ARG, "RebuildTopology: batched clouds take no origin shift");
+        if (extra && !mask_ok) FAIL(GVDBX_ERR_ARG, "RebuildTopology: batched clouds need radius <= half a brick");
+        if (n != v->h_ptoff[v->ncloud]) FAIL(GVDBX_ERR_ARG, "RebuildTopology: point count differs from the batch");
+        const int nc = v->ncloud;
+        std::vector<int32_t> tiles;
+        for (int c = 0; c < nc; c++)
+            for (int64_t o = 0; o < v->h_ptoff[c + 1] - v->h_ptoff[c]; o += BBOX_CHUNK) {
+                tiles.push_back(c);
+                tiles.push_back((int32_t)o);
+            }
+        const int64_t ntiles = (int64_t)tiles.size() / 2;
+        TRY(ensure(v, v->c_bbox, (size_t)nc * sizeof(BBoxAccum)));
+        TRY(ensure(v, v->c_tiles, std::max<size_t>(tiles.size(), 2) * 4));
+        TRY(ensure_host(v, std::max<size_t>(tiles.size() * 4, (size_t)nc * sizeof(BBoxAccum)) + 4096));
+        memcpy(v->host, tiles.data(), tiles.size() * 4);
+        GVDBX_CHECK(hipMemcpyAsync(v->c_tiles.p, v->host, tiles.size() * 4, hipMemcpyHostToDevice, s));
+        launch_bbox_clouds_init(v->c_bbox.as<BBoxAccum>(), nc, s);
+        launch_bbox_clouds(v->pos, v->c_tiles.as<int32_t>(), ntiles, v->c_ptoff.as<int64_t>(), v->c_bbox.as<BBoxAccum>(), s);
+        TRY(check_launch("cloud bounding boxes"));
+        GVDBX_CHECK(hipMemcpyAsync(v->host, v->c_bbox.p, (size_t)nc * sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
+        TRY(sync(v));
+        std::vector<BBoxAccum> bbs((size_t)nc);
+        memcpy(bbs.data(), v->host, (size_t)nc * sizeof(BBoxAccum));
+        // each cloud's level-0 cell box, then shelf packing with 2-cell gaps
+        std::vector<int32_t> box((size_t)nc * 6);
+        int64_t area = 0;
+        int maxx = 1;
+        for (int c = 0; c < nc; c++) {
+            CloudFrame f;
+            char who[48];
+            snprintf(who, sizeof(who), " in cloud %d", c);
+            TRY(frame_of(v, bbs[c], radius, who, f));
+            for (int a = 0; a < 3; a++) {
+                box[6 * c + a] = (int32_t)(f.lo_v[a] / r0);
+                box[6 * c + 3 + a] = (int32_t)(f.hi_v[a] / r0) + 1;   // exclusive
+            }
+            const int sx = box[6 * c + 3] - box[6 * c], sy = box[6 * c + 4] - box[6 * c + 1];
+            area += (int64_t)(sx + 2) * (sy + 2);
+            maxx = std::max(maxx, sx + 2);
+        }
+        const int row = std::max<int>(maxx, (int)ceil(sqrt((double)area)));
+        v->h_celloff.assign((size_t)nc * 3, 0);
+        v->h_boxes.assign((size_t)nc * 6, 0);
+        int cx = 2, cy = 2, rowh = 0, ext[3] = {0, 0, 0};
+        for (int c = 0; c < nc; c++) {
+            const int sx = box[6 * c + 3] - box[6 * c], sy = box[6 * c + 4] - box[6 * c + 1],
+                      sz = box[6 * c + 5] - box[6 * c + 2];
+            if (cx + sx + 2 > row + 2 && cx > 2) {
+                cx = 2;
+                cy += rowh + 2;
+                rowh = 0;
+            }
+            const int at[3] = {cx, cy, 2};
+            for (int a = 0; a < 3; a++) {
+                v->h_celloff[3 * c + a] = at[a] - box[6 * c + a];
+                v->h_boxes[6 * c + a] = at[a];
+                v->h_boxes[6 * c + 3 + a] = at[a] + (a == 0 ? sx : (a == 1 ? sy : sz));
+                ext[a] = std::max(ext[a], v->h_boxes[6 * c + 3 + a] + 2);
+            }
+            cx += sx + 2;
+            rowh = std::max(rowh, sy);
+        }
+        // the packed space is one volume with its own root
+        for (int a = 0; a < 3; a++) {
+            pmin[a] = 0.0f;
+            pmax[a] = (float)((int64_t)ext[a] * r0);
+            lo_v[a] = 0;
+            hi_v[a] = ext[a] * r0 - 1;
+        }
+        depth = determine_depth(v, pmin, pmax, root);
+        if (depth <= 0 || depth >= v->levs)
+            FAIL(GVDBX_ERR_DEPTH, "RebuildTopology: the packed clouds exceed the configured levels");
+        if ((int64_t)ext[0] > 65535 || (int64_t)ext[1] > 65535 || (int64_t)ext[2] > 65535)
+            FAIL(GVDBX_ERR_RANGE, "RebuildTopology: the packed clouds exceed 16-bit brick keys");
+        TRY(ensure(v, v->c_celloff, (size_t)nc * 12));
+        TRY(ensure(v, v->c_boxes, (size_t)nc * 24));
+        memcpy(v->host, v->h_celloff.data(), (size_t)nc * 12);
+        memcpy((char*)v->host + (size_t)nc * 12, v->h_boxes.data(), (size_t)nc * 24);
+        GVDBX_CHECK(hipMemcpyAsync(v->c_celloff.p, v->host, (size_t)nc * 12, hipMemcpyHostToDevice, s));
+        GVDBX_CHECK(hipMemcpyAsync(v->c_boxes.p, (char*)v->host + (size_t)nc * 12, (size_t)nc * 24,
+                                   hipMemcpyHostToDevice, s));
+        TRY(sync(v));
+    }
 
     // validation is done: from here on the previous topology is being replaced
     v->has_topo = false;
     v->has_atlas = false;
     v->topo_gen++;
     v->vis_valid = false;
-    const bool extra = radius >= 1.0f;   // VG.cpp:1207-1208
-    const int r0 = v->res[0];
-    const bool mask_ok = 2.0f * radius <= (float)r0 && radius + 1.0f <= (float)r0;
     int64_t rtop = v->range[depth];
-    int lo_v[3], hi_v[3];
-    for (int a = 0; a < 3; a++) {
-        int fmn = (int)mn[a], fmx = (int)mx[a];
-        if (extra) {
-            lo_v[a] = std::max<int64_t>(root[a], (int64_t)fmn - (int64_t)ceilf(radius) - 1);
-            hi_v[a] = (int)std::min<int64_t>(root[a] + rtop - 1, (int64_t)fmx + (int64_t)ceilf(2.0f * radius) + 1);
-        } else {
-            lo_v[a] = std::max(root[a], fmn);
-            hi_v[a] = (int)std::min<int64_t>(root[a] + rtop - 1, fmx);
-        }
-    }
+    (void)rtop;
     CellGrid grid[GVDBX_MAXLEV] = {};
     for (int l = 0; l < depth; l++) {
         int64_t rg = v->range[l];
@@ -388,6 +520,31 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     }
     if (grid[0].count() > MAX_DENSE_CELLS)
         FAIL(GVDBX_ERR_RANGE, "RebuildTopology: %lld level-0 cells exceed the dense-grid limit", (long long)grid[0].count());
+    const int32_t* allowed = nullptr;
+    if (v->ncloud > 0) {
+        // cell -> cloud over every cloud's cell box; extras only land inside their cloud's box
+        const int nc = v->ncloud;
+        TRY(ensure(v, v->c_cellcloud, (size_t)grid[0].count() * 4));
+        launch_fill_i32(v->c_cellcloud.as<int32_t>(), grid[0].count(), -1, s);
+        std::vector<int32_t> tiles;
+        for (int c = 0; c < nc; c++) {
+            const int32_t* b = &v->h_boxes[6 * c];
+            const int64_t cells = (int64_t)(b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]);
+            for (int64_t o = 0; o < cells; o += CELL_CHUNK) {
+                tiles.push_back(c);
+                tiles.push_back((int32_t)o);
+            }
+        }
+        TRY(ensure(v, v->c_tiles, std::max<size_t>(tiles.size(), 2) * 4));
+        TRY(ensure_host(v, tiles.size() * 4 + 4096));
+        memcpy(v->host, tiles.data(), tiles.size() * 4);
+        GVDBX_CHECK(hipMemcpyAsync(v->c_tiles.p, v->host, tiles.size() * 4, hipMemcpyHostToDevice, s));
+        launch_paint_clouds(grid[0], v->c_boxes.as<int32_t>(), v->c_tiles.as<int32_t>(), (int64_t)tiles.size() / 2,
+                            v->c_cellcloud.as<int32_t>(), s);
+        TRY(check_launch("cloud cells"));
+        TRY(sync(v));   // the staging buffer is reused below
+        allowed = v->c_cellcloud.as<int32_t>();
+    }
 
     // ---- occupancy ----
     int64_t G0 = grid[0].count();
@@ -413,13 +570,13 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     v->grid[0] = grid[0];
     v->bins_valid = false;
     TRY(ensure_bins(v, n, orig, false));
-    ReachParams rp{grid[0], r0, v->logdim[0], (extra && mask_ok) ? 1 : 0, radius};
+    ReachParams rp{grid[0], r0, v->logdim[0], (extra && mask_ok) ? 1 : 0, radius, cloud_map(v)};
     launch_reach_sorted(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), n, rp,
                         v->reach0.as<uint32_t>(), s);
     if (extra && !mask_ok)
         launch_mark_boxes(v->pos, n, orig[0], orig[1], orig[2], grid[0], r0, radius, v->flag_all[0].as<uint8_t>(), s);
     launch_expand_reach(grid[0], v->reach0.as<uint32_t>(), v->flag_base[0].as<uint8_t>(), v->flag_all[0].as<uint8_t>(),
-                        (extra && !mask_ok) ? 1 : 0, s);
+                        (extra && !mask_ok) ? 1 : 0, allowed, s);
     for (int l = 1; l < depth; l++)
         launch_level_up(grid[l - 1], v->flag_base[l - 1].as<uint8_t>(), v->flag_all[l - 1].as<uint8_t>(), grid[l],
                         v->logdim[l], v->flag_base[l].as<uint8_t>(), v->flag_all[l].as<uint8_t>(), s);
@@ -492,6 +649,11 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
         lp.nparent = cnt[l + 1];
         launch_link(lp, s);
     }
+    if (v->ncloud > 0) {
+        TRY(ensure(v, v->c_leafcloud, (size_t)std::max<int64_t>(cnt[0], 1) * 4));
+        launch_leaf_clouds(v->nodes[0].as<GNode>(), cnt[0], grid[0], v->logdim[0], v->c_cellcloud.as<int32_t>(),
+                           v->c_leafcloud.as<int32_t>(), s);
+    }
     TRY(check_launch("node pools"));
     TRY(sync(v));
 
@@ -529,6 +691,33 @@ int compute_bounds(gvdbx_volume* v) {
     for (int a = 0; a < 3; a++) {
         v->obj_min[a] = (float)out[a];
         v->obj_max[a] = (float)out[3 + a];
+    }
+    if (v->ncloud > 0) {
+        // ComputeBounds of every cloud, in its own coordinates
+        const int nc = v->ncloud;
+        TRY(ensure(v, v->c_bounds, (size_t)nc * 24));
+        TRY(ensure_host(v, (size_t)nc * 24 + 4096));
+        std::vector<int32_t> init6((size_t)nc * 6);
+        for (int c = 0; c < nc; c++)
+            for (int a = 0; a < 3; a++) {
+                init6[6 * c + a] = 0x7FFFFFFF;
+                init6[6 * c + 3 + a] = INT32_MIN;
+            }
+        memcpy(v->host, init6.data(), (size_t)nc * 24);
+        GVDBX_CHECK(hipMemcpyAsync(v->c_bounds.p, v->host, (size_t)nc * 24, hipMemcpyHostToDevice, s));
+        launch_leaf_bounds_clouds(v->nodes[0].as<GNode>(), v->cnt[0], v->res[0], v->c_leafcloud.as<int32_t>(),
+                                  v->c_bounds.as<int32_t>(), s);
+        TRY(check_launch("cloud bounds"));
+        GVDBX_CHECK(hipMemcpyAsync(v->host, v->c_bounds.p, (size_t)nc * 24, hipMemcpyDeviceToHost, s));
+        TRY(sync(v));
+        memcpy(init6.data(), v->host, (size_t)nc * 24);
+        v->h_cbounds.assign((size_t)nc * 6, 0.0f);
+        for (int c = 0; c < nc; c++)
+            for (int a = 0; a < 3; a++) {
+                const int32_t sh = v->h_celloff[3 * c + a] * v->res[0];
+                v->h_cbounds[6 * c + a] = (float)(init6[6 * c + a] - sh);
+                v->h_cbounds[6 * c + 3 + a] = (float)(init6[6 * c + 3 + a] - sh);
+            }
     }
     return GVDBX_OK;
 }
@@ -600,6 +789,18 @@ int bits_for(int64_t n) {   // bits to hold values 0..n-1 (at least 1)
     return b;
 }
 
+void fill_cam(const gvdbx_camera* cam, RenderCam& rc) {
+    memset(&rc, 0, sizeof(rc));
+    rc.width = cam->width;
+    rc.height = cam->height;
+    memcpy(rc.org, cam->org, sizeof(rc.org));
+    memcpy(rc.org_u, cam->org_u, sizeof(rc.org_u));
+    memcpy(rc.org_v, cam->org_v, sizeof(rc.org_v));
+    memcpy(rc.dir, cam->dir, sizeof(rc.dir));
+    memcpy(rc.dir_u, cam->dir_u, sizeof(rc.dir_u));
+    memcpy(rc.dir_v, cam->dir_v, sizeof(rc.dir_v));
+}
+
 int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp, const char* what,
                 RenderTree& tv, RenderCam& rc, RenderScene& sc) {
     TRY(need_atlas(v, what));
@@ -625,6 +826,7 @@ int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_
     memcpy(tv.bmax, v->obj_max, sizeof(tv.bmax));
     tv.chan = v->chan[chan].as<float>();
     tv.brick_res = brick_res(v);
+    memset(&rc, 0, sizeof(rc));
     rc.width = cam->width;
     rc.height = cam->height;
     memcpy(rc.org, cam->org, sizeof(rc.org));
@@ -839,7 +1041,55 @@ int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_
     v->has_pos = true;
     v->pos_gen++;
     v->bins_valid = false;
+    v->ncloud = 0;
     return GVDBX_OK;
+}
+
+int gvdbx_set_points_batched(gvdbx_volume* v, const void* pos_dev, int pos_stride, const int64_t* cloud_off,
+                             int ncloud) {
+    TRY(check_vol(v));
+    if (!pos_dev || !cloud_off || ncloud < 1) FAIL(GVDBX_ERR_ARG, "SetPointsBatched: null pointer or no cloud");
+    if (pos_stride < 12 || pos_stride % 4 || (uintptr_t)pos_dev % 4)
+        FAIL(GVDBX_ERR_ARG, "SetPointsBatched: positions must be 4-byte aligned with stride >= 12");
+    if (cloud_off[0] != 0) FAIL(GVDBX_ERR_ARG, "SetPointsBatched: cloud offsets must start at 0");
+    for (int c = 0; c < ncloud; c++)
+        if (cloud_off[c + 1] < cloud_off[c]) FAIL(GVDBX_ERR_ARG, "SetPointsBatched: cloud offsets must not decrease");
+    if (cloud_off[ncloud] >= ((int64_t)1 << 31)) FAIL(GVDBX_ERR_RANGE, "SetPointsBatched: too many points");
+    v->h_ptoff.assign(cloud_off, cloud_off + ncloud + 1);
+    TRY(ensure(v, v->c_ptoff, (size_t)(ncloud + 1) * 8));
+    TRY(ensure_host(v, (size_t)(ncloud + 1) * 8 + 4096));
+    TRY(sync(v));
+    memcpy(v->host, cloud_off, (size_t)(ncloud + 1) * 8);
+    GVDBX_CHECK(hipMemcpyAsync(v->c_ptoff.p, v->host, (size_t)(ncloud + 1) * 8, hipMemcpyHostToDevice, v->stream));
+    TRY(sync(v));
+    v->pos.base = (const char*)pos_dev;
+    v->pos.stride = pos_stride;
+    v->has_pos = true;
+    v->pos_gen++;
+    v->bins_valid = false;
+    v->ncloud = ncloud;
+    v->h_cbounds.clear();
+    return GVDBX_OK;
+}
+
+int gvdbx_read_cloud_layout(gvdbx_volume* v, int32_t* cell_off_host, float* bounds_host, int ncloud) {
+    TRY(check_vol(v));
+    if (v->ncloud == 0 || ncloud != v->ncloud) FAIL(GVDBX_ERR_ARG, "ReadCloudLayout: the volume holds %d clouds", v->ncloud);
+    if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "ReadCloudLayout: no topology");
+    if (cell_off_host) memcpy(cell_off_host, v->h_celloff.data(), (size_t)ncloud * 12);
+    if (bounds_host) {
+        if ((int)v->h_cbounds.size() != 6 * ncloud) FAIL(GVDBX_ERR_STATE, "ReadCloudLayout: bounds not computed");
+        memcpy(bounds_host, v->h_cbounds.data(), (size_t)ncloud * 24);
+    }
+    return GVDBX_OK;
+}
+
+int gvdbx_read_leaf_clouds(gvdbx_volume* v, int32_t* out_host, int64_t max_entries) {
+    TRY(check_vol(v));
+    if (v->ncloud == 0 || !v->has_topo) FAIL(GVDBX_ERR_STATE, "ReadLeafClouds: no batched topology");
+    if (!out_host || max_entries < v->cnt[0]) FAIL(GVDBX_ERR_ARG, "ReadLeafClouds: output too small");
+    GVDBX_CHECK(hipMemcpyAsync(out_host, v->c_leafcloud.p, (size_t)v->cnt[0] * 4, hipMemcpyDeviceToHost, v->stream));
+    return sync(v);
 }
 
 int gvdbx_set_point_weights(gvdbx_volume* v, const float* w_dev, int stride) {
@@ -872,6 +1122,7 @@ int gvdbx_update_atlas(gvdbx_volume* v) {
 int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
                                 int64_t* out_len) {
     TRY(check_vol(v));
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "InsertPointsSubcell: not available for batched clouds");
     TRY(need_atlas(v, "InsertPointsSubcell"));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "InsertPointsSubcell: negative point count");
     if (sc <= 0 || v->res[0] % sc) FAIL(GVDBX_ERR_ARG, "InsertPointsSubcell: subcell size must divide the brick");
@@ -906,6 +1157,7 @@ int gvdbx_gather_level_set(gvdbx_volume* v, int sc, int num_pnts, float radius, 
 int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix) {
     (void)prefix;
     TRY(check_vol(v));
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "InsertPoints: not available for batched clouds");
     TRY(need_atlas(v, "InsertPoints"));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "InsertPoints: negative point count");
     float t[3] = {tx, ty, tz};
@@ -925,6 +1177,7 @@ int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp
                           int expand, int avg_color) {
     (void)amp; (void)avg_color;   // amp is unused by the reference kernel
     TRY(check_vol(v));
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "ScatterDensity: not available for batched clouds");
     TRY(need_atlas(v, "ScatterDensity"));
     TRY(check_chan(v, 0));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "ScatterDensity: negative point count");
@@ -946,6 +1199,10 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
     float t[3] = {tx, ty, tz};
     TRY(ensure_bins(v, num_pnts, t, true));
     SplatParams sp{};
+    if (v->ncloud > 0) {
+        sp.leaf_cloud = v->c_leafcloud.as<int32_t>();
+        sp.cell_off = v->c_celloff.as<int32_t>();
+    }
     sp.leaves = v->nodes[0].as<GNode>();
     sp.nleaf = v->cnt[0];
     sp.gb = v->gbin;
@@ -976,6 +1233,7 @@ int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
 int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
                  float* rgba_dev, uint8_t* rgba8_dev) {
     TRY(check_vol(v));
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "Render: not available for batched clouds");
     RenderTree tv;
     RenderCam rc;
     RenderScene sc;
@@ -989,23 +1247,61 @@ int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, 
     return check_launch("Render");
 }
 
-int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
-                     float* I_dev, float* T_dev, int record) {
+}  // extern "C"
+
+namespace {
+
+// Upload one camera per cloud, each with its cloud's shift and bounds.
+int upload_cams(gvdbx_volume* v, const gvdbx_camera* cams, int ncam, const char* what) {
+    if (v->ncloud == 0) FAIL(GVDBX_ERR_STATE, "%s: the volume holds no batched clouds", what);
+    if (ncam != v->ncloud) FAIL(GVDBX_ERR_ARG, "%s: %d cameras for %d clouds", what, ncam, v->ncloud);
+    if ((int)v->h_cbounds.size() != 6 * ncam) FAIL(GVDBX_ERR_STATE, "%s: FinishTopology(compute_bounds) has not run", what);
+    std::vector<RenderCam> rc((size_t)ncam);
+    for (int c = 0; c < ncam; c++) {
+        if (cams[c].width != cams[0].width || cams[c].height != cams[0].height)
+            FAIL(GVDBX_ERR_ARG, "%s: every camera must have the same image size", what);
+        fill_cam(&cams[c], rc[c]);
+        for (int a = 0; a < 3; a++) {
+            rc[c].shift[a] = v->h_celloff[3 * c + a] * v->res[0];
+            rc[c].bmin[a] = v->h_cbounds[6 * c + a];
+            rc[c].bmax[a] = v->h_cbounds[6 * c + 3 + a];
+        }
+        rc[c].has_bounds = 1;
+    }
+    TRY(ensure(v, v->c_cams, (size_t)ncam * sizeof(RenderCam)));
+    TRY(ensure_host(v, (size_t)ncam * sizeof(RenderCam) + 4096));
+    TRY(sync(v));   // the staging buffer may still feed an earlier copy
+    memcpy(v->host, rc.data(), (size_t)ncam * sizeof(RenderCam));
+    GVDBX_CHECK(hipMemcpyAsync(v->c_cams.p, v->host, (size_t)ncam * sizeof(RenderCam), hipMemcpyHostToDevice, v->stream));
+    return GVDBX_OK;
+}
+
+int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bool batched,
+               const gvdbx_render_params* rp, float* I_dev, float* T_dev, int record) {
     TRY(check_vol(v));
+    const char* what = batched ? "RaycastEABatched" : "RaycastEA";
+    if (!cams || ncam < 1) FAIL(GVDBX_ERR_ARG, "%s: no camera", what);
     RenderTree tv;
     RenderCam rc;
     RenderScene sc;
-    TRY(make_render(v, chan, cam, rp, "RaycastEA", tv, rc, sc));
-    if (!I_dev || !T_dev) FAIL(GVDBX_ERR_ARG, "RaycastEA: null output");
+    TRY(make_render(v, chan, &cams[0], rp, what, tv, rc, sc));
+    if (!I_dev || !T_dev) FAIL(GVDBX_ERR_ARG, "%s: null output", what);
+    if (!batched && v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "%s: the volume holds batched clouds (one camera each)", what);
+    const RenderCam* dcams = nullptr;
+    if (batched) {
+        TRY(upload_cams(v, cams, ncam, what));
+        dcams = v->c_cams.as<RenderCam>();
+    }
     hipStream_t s = v->stream;
     VisitRec vr{};
     if (!record) {
-        launch_raycast_ea(tv, rc, sc, I_dev, T_dev, 0, vr, s);
-        return check_launch("RaycastEA");
+        launch_raycast_ea(tv, rc, dcams, ncam, sc, I_dev, T_dev, 0, vr, s);
+        return check_launch(what);
     }
-    const int64_t rays = (int64_t)cam->width * cam->height;
+    const int64_t rays = (int64_t)rc.width * rc.height * (batched ? ncam : 1);
+    if (rays >= ((int64_t)1 << 32)) FAIL(GVDBX_ERR_RANGE, "%s: too many rays", what);
     const int rb = bits_for(rays), lb = bits_for(v->cnt[0]);
-    if (rb + lb > 63) FAIL(GVDBX_ERR_RANGE, "RaycastEA: too many rays x leaves for 64-bit segment keys");
+    if (rb + lb > 63) FAIL(GVDBX_ERR_RANGE, "%s: too many rays x leaves for 64-bit segment keys", what);
     v->vis_valid = false;
     int64_t cap = std::max<int64_t>(v->vis_cap, std::max<int64_t>(rays * 8, 1 << 20));
     TRY(ensure(v, v->vis_cnt, 16));
@@ -1019,8 +1315,8 @@ int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const g
         vr.count = v->vis_cnt.as<uint32_t>();
         vr.cap = cap;
         vr.ray_bits = rb;
-        launch_raycast_ea(tv, rc, sc, I_dev, T_dev, 1, vr, s);
-        TRY(check_launch("RaycastEA"));
+        launch_raycast_ea(tv, rc, dcams, ncam, sc, I_dev, T_dev, 1, vr, s);
+        TRY(check_launch(what));
         GVDBX_CHECK(hipMemcpyAsync(v->host, v->vis_cnt.p, 4, hipMemcpyDeviceToHost, s));
         TRY(sync(v));
         uint32_t n = *(uint32_t*)v->host;
@@ -1029,8 +1325,9 @@ int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const g
             v->vis_cap = cap;
             v->vis_ray_bits = rb;
             v->vis_leaf_bits = lb;
-            v->vis_w = cam->width;
-            v->vis_h = cam->height;
+            v->vis_w = rc.width;
+            v->vis_h = rc.height;
+            v->vis_ncam = batched ? ncam : 0;
             v->vis_chan = chan;
             v->vis_topo_gen = v->topo_gen;
             v->vis_valid = true;
@@ -1039,22 +1336,29 @@ int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const g
         }
         cap = (int64_t)n + (int64_t)n / 8 + 1024;   // the segment count of this view; rerun once
     }
-    FAIL(GVDBX_ERR_STATE, "RaycastEA: segment count did not settle");
+    FAIL(GVDBX_ERR_STATE, "%s: segment count did not settle", what);
 }
 
-int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
-                              const float* I_dev, const float* T_dev, const float* gI_dev, const float* gT_dev,
-                              float* grad_dev) {
+int raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bool batched,
+                        const gvdbx_render_params* rp, const float* I_dev, const float* T_dev, const float* gI_dev,
+                        const float* gT_dev, float* grad_dev) {
     TRY(check_vol(v));
+    const char* what = batched ? "RaycastEABackwardBatched" : "RaycastEABackward";
+    if (!cams || ncam < 1) FAIL(GVDBX_ERR_ARG, "%s: no camera", what);
     RenderTree tv;
     RenderCam rc;
     RenderScene sc;
-    TRY(make_render(v, chan, cam, rp, "RaycastEABackward", tv, rc, sc));
-    if (!I_dev || !T_dev || !gI_dev || !grad_dev) FAIL(GVDBX_ERR_ARG, "RaycastEABackward: null buffer");
-    if (!v->vis_valid || v->vis_topo_gen != v->topo_gen || v->vis_w != cam->width || v->vis_h != cam->height ||
-        v->vis_chan != chan)
-        FAIL(GVDBX_ERR_STATE, "RaycastEABackward: no recorded RaycastEA for this view and topology");
-    if (brick_res(v) != 10) FAIL(GVDBX_ERR_ARG, "RaycastEABackward: needs 8^3 bricks with apron 1");
+    TRY(make_render(v, chan, &cams[0], rp, what, tv, rc, sc));
+    if (!I_dev || !T_dev || !gI_dev || !grad_dev) FAIL(GVDBX_ERR_ARG, "%s: null buffer", what);
+    if (!v->vis_valid || v->vis_topo_gen != v->topo_gen || v->vis_w != rc.width || v->vis_h != rc.height ||
+        v->vis_chan != chan || v->vis_ncam != (batched ? ncam : 0))
+        FAIL(GVDBX_ERR_STATE, "%s: no recorded raycast for this view and topology", what);
+    if (brick_res(v) != 10) FAIL(GVDBX_ERR_ARG, "%s: needs 8^3 bricks with apron 1", what);
+    const RenderCam* dcams = nullptr;
+    if (batched) {
+        TRY(upload_cams(v, cams, ncam, what));
+        dcams = v->c_cams.as<RenderCam>();
+    }
     hipStream_t s = v->stream;
     const int64_t n = v->vis_n, nleaf = v->cnt[0];
     if (!v->vis_sorted) {
@@ -1081,6 +1385,7 @@ int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam
     EABackParams p{};
     p.tv = tv;
     p.cam = rc;
+    p.cams = dcams;
     p.sc = sc;
     p.key = v->vis_in_tmp ? v->vis_tkey.as<uint64_t>() : v->vis_key.as<uint64_t>();
     p.idx = v->vis_in_tmp ? v->vis_tidx.as<uint32_t>() : v->vis_idx.as<uint32_t>();
@@ -1093,7 +1398,33 @@ int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam
     p.grad = grad_dev;
     p.ray_bits = v->vis_ray_bits;
     launch_ea_backward(p, s);
-    return check_launch("RaycastEABackward");
+    return check_launch(what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gvdbx_raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                     float* I_dev, float* T_dev, int record) {
+    return raycast_ea(v, chan, cam, 1, false, rp, I_dev, T_dev, record);
+}
+
+int gvdbx_raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
+                              const float* I_dev, const float* T_dev, const float* gI_dev, const float* gT_dev,
+                              float* grad_dev) {
+    return raycast_ea_backward(v, chan, cam, 1, false, rp, I_dev, T_dev, gI_dev, gT_dev, grad_dev);
+}
+
+int gvdbx_raycast_ea_batched(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam,
+                             const gvdbx_render_params* rp, float* I_dev, float* T_dev, int record) {
+    return raycast_ea(v, chan, cams, ncam, true, rp, I_dev, T_dev, record);
+}
+
+int gvdbx_raycast_ea_backward_batched(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam,
+                                      const gvdbx_render_params* rp, const float* I_dev, const float* T_dev,
+                                      const float* gI_dev, const float* gT_dev, float* grad_dev) {
+    return raycast_ea_backward(v, chan, cams, ncam, true, rp, I_dev, T_dev, gI_dev, gT_dev, grad_dev);
 }
 
 int gvdbx_apron_adjoint(gvdbx_volume* v, float* grad_dev) {
@@ -1123,6 +1454,7 @@ int gvdbx_splat_cubic_backward(gvdbx_volume* v, int num_pnts, float radius, floa
     launch_fill_f32(dpos_dev, 3 * (int64_t)num_pnts, 0.0f, s);
     if (dw_dev) launch_fill_f32(dw_dev, num_pnts, 0.0f, s);
     SplatBackParams p{};
+    p.cm = cloud_map(v);
     p.g0 = v->grid[0];
     p.idx0 = v->idx[0].as<int32_t>();
     p.leaves = v->nodes[0].as<GNode>();

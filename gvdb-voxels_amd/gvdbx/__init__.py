@@ -156,6 +156,28 @@ class VolumeGVDB:
              C.c_void_p(_ptr(vel)) if vel is not None else None, 0, 12,
              C.c_void_p(_ptr(clr)) if clr is not None else None, 0, 4)
 
+    def SetPointsBatched(self, pos, cloud_off):
+        """Many clouds, each in its own index space: pos (N,3) device tensor,
+        cloud_off: ncloud+1 point offsets (host sequence)."""
+        self._keep = [pos]
+        off = (C.c_int64 * len(cloud_off))(*[int(x) for x in cloud_off])
+        call("gvdbx_set_points_batched", self._h, C.c_void_p(_ptr(pos)), 12, off, len(cloud_off) - 1)
+        self.ncloud = len(cloud_off) - 1
+
+    def cloud_layout(self):
+        n = self.ncloud
+        off = np.zeros((n, 3), np.int32)
+        bnd = np.zeros((n, 6), np.float32)
+        call("gvdbx_read_cloud_layout", self._h, off.ctypes.data, bnd.ctypes.data, n)
+        return off, bnd
+
+    def leaf_clouds(self):
+        n = self.info().node_cnt[0]
+        out = np.zeros(n, np.int32)
+        if n:
+            call("gvdbx_read_leaf_clouds", self._h, out.ctypes.data, n)
+        return out
+
     def SetPointWeights(self, w, stride=4):
         self._keep_w = w
         call("gvdbx_set_point_weights", self._h, C.c_void_p(_ptr(w)) if w is not None else None, stride)
@@ -225,6 +247,34 @@ class VolumeGVDB:
         call("gvdbx_raycast_ea", self._h, int(chan), C.byref(cam), C.byref(rp), C.c_void_p(_ptr(I)),
              C.c_void_p(_ptr(T)), int(record))
         return I, T
+
+    def RaycastEABatched(self, chan, cams, rp, I=None, T=None, record=False):
+        """One camera per batched cloud; (I, T) of shape (ncloud, H, W)."""
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device())
+        n = len(cams)
+        arr = (Camera * n)(*cams)
+        self._cams = arr
+        if I is None:
+            I = torch.empty((n, cams[0].height, cams[0].width), dtype=torch.float32, device=dev)
+        if T is None:
+            T = torch.empty((n, cams[0].height, cams[0].width), dtype=torch.float32, device=dev)
+        call("gvdbx_raycast_ea_batched", self._h, int(chan), arr, n, C.byref(rp), C.c_void_p(_ptr(I)),
+             C.c_void_p(_ptr(T)), int(record))
+        return I, T
+
+    def RaycastEABackwardBatched(self, chan, cams, rp, I, T, gI, gT=None, grad=None):
+        import torch
+        inf = self.info()
+        br, B = inf.brick_res, inf.node_cnt[0]
+        if grad is None:
+            grad = torch.empty((B, br, br, br), dtype=torch.float32, device=I.device)
+        n = len(cams)
+        arr = (Camera * n)(*cams)
+        call("gvdbx_raycast_ea_backward_batched", self._h, int(chan), arr, n, C.byref(rp), C.c_void_p(_ptr(I)),
+             C.c_void_p(_ptr(T)), C.c_void_p(_ptr(gI)), C.c_void_p(_ptr(gT)) if gT is not None else None,
+             C.c_void_p(_ptr(grad)))
+        return grad
 
     def RaycastEABackward(self, chan, cam, rp, I, T, gI, gT=None, grad=None):
         """d(sum gI*I + gT*T)/d voxel over the padded bricks, (B, br, br, br)."""

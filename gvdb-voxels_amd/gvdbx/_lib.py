@@ -75,6 +75,12 @@ _SIGS = {
     "gvdbx_splat_cubic_backward": [_vp, _i, _f, _f, _f, _f, _vp, _vp, _vp],
     "gvdbx_backproject_depth": [_vp, _i, _i, _i, C.POINTER(C.c_float), _vp, _f, C.POINTER(C.c_float), _f, _f, _vp,
                                 _vp],
+    "gvdbx_set_points_batched": [_vp, _vp, _i, C.POINTER(C.c_int64), _i],
+    "gvdbx_read_cloud_layout": [_vp, _vp, _vp, _i],
+    "gvdbx_read_leaf_clouds": [_vp, _vp, _i64],
+    "gvdbx_raycast_ea_batched": [_vp, _i, C.POINTER(Camera), _i, C.POINTER(RenderParams), _vp, _vp, _i],
+    "gvdbx_raycast_ea_backward_batched": [_vp, _i, C.POINTER(Camera), _i, C.POINTER(RenderParams), _vp, _vp, _vp,
+                                          _vp, _vp],
     "gvdbx_get_info": [_vp, C.POINTER(Info)],
     "gvdbx_read_nodes": [_vp, _i, _vp, _i64],
     "gvdbx_read_child_lists": [_vp, _i, _vp, _i64],

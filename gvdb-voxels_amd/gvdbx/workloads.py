@@ -78,3 +78,50 @@ def depth_scene(seed, nframes=64, width=640, height=480, fx=525.0, fy=525.0, dmi
         d[rng.random((height, width)) < frac] = np.nan
         depth[f] = d
     return depth, poses, np.array([fx, fy, cx, cy], np.float32)
+
+
+def cloud_batch(seed, nclouds=256, nmin=10_000, nmax=2_000_000):
+    """Independent point clouds, each in its own index space (coordinates
+    >= 8): size floor(exp(U(ln nmin, ln nmax))), shape a seeded choice of
+      - shell: radius R ~ U(40,120), thickness 8, centred at R + 16;
+      - box: sides U(40,200), corner (16,16,16);
+      - clusters: 4..16 Gaussians, means U(40,200)^3, sigma U(3,10) per
+        axis, truncated at 3 sigma;
+    per-point weights U(0.1,1).  Returns a list of (points (n,3) float32,
+    weights (n,) float32, shape name)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nclouds):
+        n = int(np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        kind = ("shell", "box", "clusters")[int(rng.integers(0, 3))]
+        if kind == "shell":
+            R = rng.uniform(40.0, 120.0)
+            d = rng.normal(size=(n, 3))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            r = np.cbrt(rng.uniform((R - 8.0) ** 3, R ** 3, size=n))
+            p = d * r[:, None] + (R + 16.0)
+        elif kind == "box":
+            side = rng.uniform(40.0, 200.0, size=3)
+            p = rng.random((n, 3)) * side + 16.0
+        else:
+            k = int(rng.integers(4, 17))
+            means = rng.uniform(40.0, 200.0, size=(k, 3))
+            sig = rng.uniform(3.0, 10.0, size=(k, 3))
+            lab = rng.integers(0, k, size=n)
+            z = rng.standard_normal((n, 3))
+            bad = np.abs(z) > 3
+            while bad.any():
+                z[bad] = rng.standard_normal(int(bad.sum()))
+                bad = np.abs(z) > 3
+            p = means[lab] + z * sig[lab]
+        w = rng.uniform(0.1, 1.0, size=n).astype(np.float32)
+        out.append((p.astype(np.float32), w, kind))
+    return out
+
+
+def cloud_camera_box(points):
+    """xy rectangle (lo, hi) of a cloud, padded by 4 voxels, for an
+    orthographic +z view."""
+    lo = np.floor(points[:, :2].min(0)) - 4.0
+    hi = np.ceil(points[:, :2].max(0)) + 4.0
+    return lo, hi

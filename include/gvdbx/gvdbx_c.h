@@ -212,6 +212,33 @@ int gvdbx_backproject_depth(const float* depth_dev, int nframes, int width, int 
                             const float* poses_dev, float voxel, const float* origin3, float dmin, float dmax,
                             float* out_xyz_dev, void* stream);
 
+/* ---- batched clouds (extension) ---- */
+/* Many independent clouds in one volume: cloud c is the points
+ * [cloud_off[c], cloud_off[c+1]) of pos_dev (host array of ncloud+1 offsets),
+ * each in its own index space.  RebuildTopology then builds every cloud's
+ * topology as the reference would for that cloud alone (its own bounding
+ * box, depth, root and extra bricks) and packs the clouds into disjoint
+ * slots of one index space, so every later stage runs once for all clouds.
+ * The packing shifts cells by whole bricks (integers); positions, distances
+ * and voxel values stay those of the cloud's own space.  SplatCubic,
+ * UpdateApron, the adjoints and the batched raycasts work on batched clouds;
+ * the subcell gathers, InsertPoints, ScatterDensity and Render do not. */
+int gvdbx_set_points_batched(gvdbx_volume* v, const void* pos_dev, int pos_stride, const int64_t* cloud_off,
+                             int ncloud);
+/* per-cloud layout: cell_off (3 ints per cloud, level-0 cells added to the
+ * cloud's cells in the packed space) and bounds (6 floats per cloud: its
+ * ComputeBounds result in its own space); either may be NULL */
+int gvdbx_read_cloud_layout(gvdbx_volume* v, int32_t* cell_off_host, float* bounds_host, int ncloud);
+/* the cloud of every leaf (node_cnt[0] entries) */
+int gvdbx_read_leaf_clouds(gvdbx_volume* v, int32_t* out_host, int64_t max_entries);
+/* one camera per cloud (ncam == ncloud, equal image sizes), camera c in
+ * cloud c's own space; image c at c*width*height of I_dev / T_dev */
+int gvdbx_raycast_ea_batched(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam,
+                             const gvdbx_render_params* rp, float* I_dev, float* T_dev, int record);
+int gvdbx_raycast_ea_backward_batched(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam,
+                                      const gvdbx_render_params* rp, const float* I_dev, const float* T_dev,
+                                      const float* gI_dev, const float* gT_dev, float* grad_dev);
+
 /* ---- readback (host copies; synchronising) ---- */
 int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out);
 /* pool group 0 of one level: 64-byte records (VG.h getNode / AL.cpp PoolFetch) */

@@ -87,3 +87,89 @@ def test_depth_fusion_batched_equals_sequential_and_oracle():
     ov.gather_level_set(1.0)
     ov.update_apron(0, 0.0)
     np.testing.assert_array_equal(bb, ov.bricks(0))
+
+
+def _cloud_cam(G, pts, W=64, H=64):
+    from gvdbx import workloads
+    lo, hi = workloads.cloud_camera_box(pts)
+    # pixel centres a quarter voxel off the integer grid; rays start 0.3 below z = 0
+    return G.camera_ortho(W, H, (lo[0] + 0.25, lo[1] + 0.25, -0.3), (hi[0] - lo[0], 0, 0), (0, hi[1] - lo[1], 0),
+                          (0, 0, 1))
+
+
+def _leaf_key(pos):
+    return [tuple(int(x) for x in p) for p in pos]
+
+
+def test_batched_clouds_equal_per_cloud():
+    """Topology, splat, apron, EA raycast and every adjoint of a batch of
+    clouds equal the same calls made for each cloud alone, bit for bit."""
+    import gvdbx as G
+    from gvdbx import workloads
+    dev = _dev()
+    clouds = workloads.cloud_batch(9, nclouds=6, nmin=3000, nmax=40000)
+    _log("batched clouds: " + ", ".join(f"{len(p)} {k}" for p, w, k in clouds))
+    rp = G.ea_params(step=0.5, kappa=0.05)
+    rng = np.random.default_rng(4)
+    gIs = [rng.normal(size=(64, 64)).astype(np.float32) for _ in clouds]
+    singles = []
+    for (p, w, k), gI in zip(clouds, gIs):
+        gv = _vol(G)
+        tp = torch.from_numpy(p).to(dev)
+        tw = torch.from_numpy(w).to(dev)
+        gv.SetPoints(tp)
+        gv.SetPointWeights(tw)
+        gv.RebuildTopology(len(p), 4.0)
+        gv.FinishTopology(False, True)
+        gv.UpdateAtlas()
+        gv.SplatCubic(len(p), 2.0)
+        gv.UpdateApron(0, 0.0)
+        cam = _cloud_cam(G, p)
+        I, T = gv.RaycastEA(0, cam, rp, record=True)
+        grad = gv.RaycastEABackward(0, cam, rp, I, T, torch.from_numpy(gI).to(dev))
+        gv.ApronAdjoint(grad)
+        dpos, dw = gv.SplatCubicBackward(len(p), 2.0, (0, 0, 0), grad)
+        torch.cuda.synchronize()
+        inf = gv.info()
+        singles.append(dict(nodes=gv.nodes(0), bricks=gv.bricks(0), I=I.cpu().numpy(), T=T.cpu().numpy(),
+                            grad=grad.cpu().numpy(), dpos=dpos.cpu().numpy(), dw=dw.cpu().numpy(),
+                            bounds=np.array(list(inf.obj_min) + list(inf.obj_max), np.float32), keep=(tp, tw)))
+        gv.close()
+    off = np.concatenate([[0], np.cumsum([len(p) for p, w, k in clouds])])
+    P = torch.from_numpy(np.concatenate([p for p, w, k in clouds])).to(dev)
+    Wt = torch.from_numpy(np.concatenate([w for p, w, k in clouds])).to(dev)
+    gb = _vol(G)
+    gb.SetPointsBatched(P, off)
+    gb.SetPointWeights(Wt)
+    gb.RebuildTopology(int(off[-1]), 4.0)
+    gb.FinishTopology(False, True)
+    gb.UpdateAtlas()
+    gb.SplatCubic(int(off[-1]), 2.0)
+    gb.UpdateApron(0, 0.0)
+    cams = [_cloud_cam(G, p) for p, w, k in clouds]
+    I, T = gb.RaycastEABatched(0, cams, rp, record=True)
+    grad = gb.RaycastEABackwardBatched(0, cams, rp, I, T, torch.from_numpy(np.stack(gIs)).to(dev))
+    gb.ApronAdjoint(grad)
+    dpos, dw = gb.SplatCubicBackward(int(off[-1]), 2.0, (0, 0, 0), grad)
+    torch.cuda.synchronize()
+    cell_off, bounds = gb.cloud_layout()
+    lc = gb.leaf_clouds()
+    nodes, bricks, gradb = gb.nodes(0), gb.bricks(0), grad.cpu().numpy()
+    Ib, Tb = I.cpu().numpy(), T.cpu().numpy()
+    dpb, dwb = dpos.cpu().numpy(), dw.cpu().numpy()
+    for c, s in enumerate(singles):
+        sel = np.nonzero(lc == c)[0]
+        local = nodes["mPos"][sel] - cell_off[c] * 8
+        ref = {k: i for i, k in enumerate(_leaf_key(s["nodes"]["mPos"]))}
+        keys = _leaf_key(local)
+        assert sorted(keys) == sorted(ref), f"cloud {c}: leaf sets differ"
+        order = np.array([ref[k] for k in keys])
+        np.testing.assert_array_equal(bricks[sel], s["bricks"][order], err_msg=f"cloud {c} bricks")
+        np.testing.assert_array_equal(bounds[c], s["bounds"], err_msg=f"cloud {c} bounds")
+        np.testing.assert_array_equal(Ib[c], s["I"], err_msg=f"cloud {c} I")
+        np.testing.assert_array_equal(Tb[c], s["T"], err_msg=f"cloud {c} T")
+        inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        np.testing.assert_array_equal(gradb[sel][inner], s["grad"][order][inner], err_msg=f"cloud {c} grad")
+        np.testing.assert_array_equal(dpb[off[c]:off[c + 1]], s["dpos"], err_msg=f"cloud {c} dpos")
+        np.testing.assert_array_equal(dwb[off[c]:off[c + 1]], s["dw"], err_msg=f"cloud {c} dw")
+    assert (lc >= 0).all()
