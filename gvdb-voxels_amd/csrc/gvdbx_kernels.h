@@ -128,7 +128,9 @@ struct SplatParams {
     int64_t nleaf;
     CellGrid gb;
     const int32_t* off;
-    const float4* sorted;
+    const float4* sorted;     // (x, y, z, weight) in bin order (launch_splat_src)
+    const float4* bins;       // the bins themselves (x, y, z, point index)
+    CloudMap cm;              // batched clouds (ncloud 0 otherwise)
     const char* weights;      // may be null
     int64_t wstride;
     float* chan;
@@ -137,7 +139,23 @@ struct SplatParams {
     int accum;
     float radius;
 };
-void launch_splat_cubic(SplatParams sp, hipStream_t s);
+// Cubic splat.  Cell push: a bin cell's points in runs of `chunk` (runs[c]
+// per cell, run_off = exclusive scan, run_cell maps run -> cell), each run's
+// (res0 + 2h)^3 grid in `part` (h = ceil(r)), collected per leaf.  Tile pull:
+// one kernel, for grids that do not fit LDS.
+constexpr int SPLAT_CHUNK_DEFAULT = 4096;
+int splat_halo(float radius);
+int64_t splat_cell_lds(int res0, float radius);   // bytes of LDS per run
+void launch_splat_runs(SplatParams sp, int chunk, int32_t* runs, hipStream_t s);
+void launch_splat_run_map(SplatParams sp, const int32_t* run_off, int32_t* run_cell, hipStream_t s);
+void launch_splat_cell(SplatParams sp, const int32_t* run_cell, const int32_t* run_off, int nruns, int chunk,
+                       float* part, hipStream_t s);
+void launch_splat_collect(SplatParams sp, const int32_t* run_off, const float* part, hipStream_t s);
+void launch_splat_tile(SplatParams sp, hipStream_t s);
+// (x, y, z, weight) of the first *count bin-sorted points (weight 1 without
+// weights); count is the device-side number of binned points.
+void launch_splat_src(const float4* sorted, int64_t n, const int32_t* count, const char* weights, int64_t wstride,
+                      float4* out, hipStream_t s);
 
 struct ApronParams {
     const GNode* leaves;

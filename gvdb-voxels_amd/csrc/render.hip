@@ -638,11 +638,18 @@ __global__ __launch_bounds__(256) void k_splat_backward(SplatBackParams p) {
         const int y0 = max((int)floorf(q.y - p.radius), 0), y1 = (int)ceilf(q.y + p.radius);
         const int z0 = max((int)floorf(q.z - p.radius), 0), z1 = (int)ceilf(q.z + p.radius);
         float gw = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        const float r2m = (p.radius + 0.01f) * (p.radius + 0.01f);   // row pruning (d2 < r2 decides)
         for (int z = z0; z <= z1; z++) {
             const float dz = (float)z - q.z;
+            const float dz2 = dz * dz;
+            if (dz2 > r2m) continue;
             for (int y = y0; y <= y1; y++) {
                 const float dy = (float)y - q.y;
-                for (int x = x0; x <= x1; x++) {
+                const float row = dz2 + dy * dy;
+                if (row > r2m) continue;
+                const float half = sqrtf(r2m - row);
+                const int xa = max(x0, (int)ceilf(q.x - half)), xb = min(x1, (int)floorf(q.x + half));
+                for (int x = xa; x <= xb; x++) {
                     const float dx = (float)x - q.x;
                     const float d2 = gvdbx_d2(dx, dy, dz);
                     if (!(d2 < r2)) continue;

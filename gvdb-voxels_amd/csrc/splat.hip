@@ -5,8 +5,12 @@
 // registers.  Candidate points come from the 27 neighbouring bins of the
 // brick (points sorted by home brick), are filtered to those whose support
 // reaches the tile, and are staged in LDS.  Results are written once to the
-// brick-linear atlas (leaf k at k*brick_res^3 floats).  No float atomics are
-// used anywhere, so every result is deterministic.
+// brick-linear atlas (leaf k at k*brick_res^3 floats).  No global float
+// atomics are used; the cubic splat's LDS adds come from the tile's single
+// wave in an order fixed by the data, so every result is deterministic.
+#include <climits>
+#include <cmath>
+
 #include "gvdbx_kernels.h"
 
 namespace {
@@ -465,17 +469,274 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
 // ---------------------------------------------------------------------------
 // Weighted cubic splat: rho(v) = sum_p w_p * max(0, 1 - |v-p|^2/r^2)^3 at the
 // integer voxel coordinates of every leaf (extension of distFunc,
-// cuda_gvdb_particles.cuh:516-523).
-// One wave per 8^3 tile.  The points of the 27 neighbour bins whose support
-// box overlaps the tile are compacted into LDS in a fixed order (neighbour
-// bins in a fixed order, ascending point index within a bin); then each lane
-// takes one staged point and adds its kernel into the tile's LDS copy over
-// the voxels of its support (point-centric: no lane idles on voxels the point
-// cannot reach).  A single wave issues every LDS add, in an order fixed by
-// the data, so results are deterministic.
+// cuda_gvdb_particles.cuh:516-523).  Two forms, both deterministic:
+//
+// Cell push (default).  Pass 1: the points of each bin cell, in runs of at
+// most `chunk` points (a tuning knob) per wavefront, are splat into an LDS
+// grid covering the cell's voxels plus a halo of h = ceil(r) voxels
+// (G = res0 + 2h per axis), which holds every voxel any of them reaches.
+// Points are taken one at a time in point order and the lanes cover the
+// point's W^3 support window (W = ceil(2r)), so every grid voxel sums its
+// contributions in point order with plain LDS adds.  Each point is read once.
+// The grid goes to the run's slot of a scratch buffer.  Pass 2, one block per
+// leaf: every interior voxel adds, to its initial value, the entries covering
+// it of the runs of its own cell and of its 26 neighbour cells, in a fixed
+// order (cells nearest first, NB_ORDER; runs in point order).
+//
+// Tile pull (fallback for grids that do not fit LDS).  One wave per 8^3
+// tile scans the 27 neighbour bins nearest first, 64 points per step with the
+// next step's load in flight, compacts the points whose support box overlaps
+// the tile into LDS in that order, and splats every full round of 64 staged
+// points point-centric into the tile's LDS copy.
+//
+// Tile pull reads the points from an (x, y, z, weight) copy of the bins
+// made once per call.  POW2: r^2 is a power of two, so d2 * (1/r^2) equals
+// d2 / r^2 bit for bit.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
-    __shared__ float4 cand[CAND];
+constexpr int SPLAT_STAGE = 128;   // tile pull: staged points per wave (one round of 64 plus one step)
+
+__global__ __launch_bounds__(256) void k_splat_src(const float4* sorted, int64_t n, const int32_t* count,
+                                                   const char* weights, int64_t wstride, float4* out) {
+    // Only the first *count entries (the binned points; invalid ones sort
+    // last and are never written) are read.
+    const int64_t m = min(n, (int64_t)*count);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+        const float4 p = sorted[j];
+        const float w = weights ? *(const float*)(weights + (int64_t)__float_as_int(p.w) * wstride) : 1.0f;
+        out[j] = make_float4(p.x, p.y, p.z, w);
+    }
+}
+
+// Bin of neighbour d (NB_ORDER position) of cell (cx,cy,cz), or -1.
+__device__ __forceinline__ int64_t nb_bin(const CellGrid& gb, int d, int cx, int cy, int cz) {
+    const int nb = NB_ORDER[d];
+    const int bx = cx + nb / 9 - 1, by = cy + (nb / 3) % 3 - 1, bz = cz + nb % 3 - 1;
+    return gb.contains(bx, by, bz) ? gb.index(bx, by, bz) : -1;
+}
+
+// ---- cell push ----
+__global__ __launch_bounds__(256) void k_splat_runs(const int32_t* off, int64_t ncell, int chunk, int32_t* runs) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+        const int n = off[c + 1] - off[c];
+        runs[c] = (int)(((int64_t)n + chunk - 1) / chunk);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_splat_run_map(const int32_t* run_off, int64_t ncell, int32_t* run_cell) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t b = run_off[c]; b < run_off[c + 1]; b++) run_cell[b] = (int32_t)c;
+}
+
+// First voxel of a point's support along one axis, as a grid index: the
+// smallest integer x with |x - p| < r (ceil(p - r), one more when that voxel
+// is at distance >= r), clamped so that the W-voxel window stays in the grid
+// (the voxels the clamp drops are at distance >= r).
+__device__ __forceinline__ int support_start(float p, float r, float r2, int g0, int G, int W) {
+    int x = (int)ceilf(p - r);
+    const float d = (float)x - p;
+    if (!(d * d < r2)) x++;
+    return min(max(x - g0, 0), G - W);
+}
+
+// Pass 1.  The run's points are walked one at a time (ballot + readlane);
+// the 64 lanes cover a 4x4x4 block of the point's W^3 support window (several
+// blocks when W > 4), so the lanes of one LDS read-add-write never share a
+// voxel and need no atomics, and every grid voxel sums its contributions in
+// point order.  Weights are fetched two steps ahead.
+// SMALL: W <= 4 (r <= 2), a single block.
+template <bool POW2, bool SMALL>
+__global__ __launch_bounds__(64) void k_splat_cell(SplatParams sp, const int32_t* run_cell, const int32_t* run_off,
+                                                   int nruns, int res0, int h, int W, int chunk, float* part) {
+    extern __shared__ float grid[];
+    const int G = res0 + 2 * h;
+    const int G3 = G * G * G;
+    const int lane = threadIdx.x;
+    const int b = (int)blockIdx.x;
+    if (b >= nruns) return;
+    const int64_t c = run_cell[b];
+    const int sub = b - run_off[c];
+    const int beg = sp.off[c] + sub * chunk;
+    const int end = (int)min((int64_t)sp.off[c + 1], (int64_t)beg + chunk);
+    for (int i = lane; i < G3; i += 64) grid[i] = 0.0f;
+    // cell coordinates; batched clouds: shift to the cloud's own voxel space
+    const int64_t nyz = (int64_t)sp.gb.ny * sp.gb.nz;
+    int cx = (int)(c / nyz) + sp.gb.ox, cy = (int)((c / sp.gb.nz) % sp.gb.ny) + sp.gb.oy, cz = (int)(c % sp.gb.nz) + sp.gb.oz;
+    if (sp.cm.ncloud > 0) {
+        const int cl = sp.cm.cloud_of_point(__float_as_int(sp.bins[beg].w));
+        cx -= sp.cm.cell_off[3 * cl];
+        cy -= sp.cm.cell_off[3 * cl + 1];
+        cz -= sp.cm.cell_off[3 * cl + 2];
+    }
+    const int g0x = cx * res0 - h, g0y = cy * res0 - h, g0z = cz * res0 - h;
+    const float r = sp.radius;
+    const float r2 = r * r;
+    const float inv_r2 = 1.0f / r2;
+    const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+    const float lxf = (float)lx, lyf = (float)ly, lzf = (float)lz;
+    const int loff = (lz * G + ly) * G + lx;
+    const bool lane_in = lx < W && ly < W && lz < W;   // SMALL: lane inside the window
+    auto weight_of = [&](const float4& q, bool valid) -> float {
+        if (!valid || !sp.weights) return 1.0f;
+        return *(const float*)(sp.weights + (int64_t)__float_as_int(q.w) * sp.wstride);
+    };
+    auto add = [&](int gi, float d2, float w) {
+        // POW2: d2 * inv_r2 is exact, so the fma rounds 1 - d2/r^2 once, as the division form does
+        const float t = POW2 ? __builtin_fmaf(-d2, inv_r2, 1.0f) : 1.0f - d2 / r2;
+        // a plain read-add-write: the wave waits for its own read, which in
+        // dense cells is cheaper than back-to-back LDS atomics on one address
+        grid[gi] = grid[gi] + w * ((t * t) * t);
+    };
+    __syncthreads();
+    // pipeline: cur (+ its weight) is splat while the weight of nxt and the
+    // points of the step after are in flight
+    float4 cur = make_float4(0, 0, 0, 0), nxt = make_float4(0, 0, 0, 0);
+    if (beg + lane < end) cur = sp.bins[beg + lane];
+    float cw = weight_of(cur, beg + lane < end);
+    if (beg + 64 + lane < end) nxt = sp.bins[beg + 64 + lane];
+    for (int j0 = beg; j0 < end; j0 += 64) {
+        const bool ok = j0 + lane < end;
+        const float nw = weight_of(nxt, j0 + 64 + lane < end);
+        float4 nn = make_float4(0, 0, 0, 0);
+        if (j0 + 128 + lane < end) nn = sp.bins[j0 + 128 + lane];
+        // per-lane point: grid index of its support window's first voxel and
+        // that voxel's offset from the point
+        int pk = 0;
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (ok) {
+            const int sx = support_start(cur.x, r, r2, g0x, G, W);
+            const int sy = support_start(cur.y, r, r2, g0y, G, W);
+            const int sz = support_start(cur.z, r, r2, g0z, G, W);
+            ax = (float)(g0x + sx) - cur.x;
+            ay = (float)(g0y + sy) - cur.y;
+            az = (float)(g0z + sz) - cur.z;
+            pk = (sz * G + sy) * G + sx;
+        }
+        uint64_t m = __ballot(ok);
+        while (m) {
+            const int q = __builtin_ctzll(m);
+            m &= m - 1;
+            const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ax), q));
+            const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ay), q));
+            const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(az), q));
+            const float qw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cw), q));
+            const int qp = __builtin_amdgcn_readlane(pk, q);
+            if (SMALL) {
+                const float dx = qx + lxf, dy = qy + lyf, dz = qz + lzf;
+                const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                if (lane_in && d2 < r2) add(qp + loff, d2, qw);
+            } else {
+                for (int bz = 0; bz < W; bz += 4)
+                    for (int by = 0; by < W; by += 4)
+                        for (int bx = 0; bx < W; bx += 4) {
+                            if (bx + lx >= W || by + ly >= W || bz + lz >= W) continue;
+                            const float dx = qx + (float)bx + lxf, dy = qy + (float)by + lyf,
+                                        dz = qz + (float)bz + lzf;
+                            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                            if (d2 < r2) add(qp + loff + (bz * G + by) * G + bx, d2, qw);
+                        }
+            }
+        }
+        cur = nxt;
+        cw = nw;
+        nxt = nn;
+    }
+    __syncthreads();
+    float* out = part + (int64_t)b * G3;
+    for (int i = lane; i < G3; i += 64) out[i] = grid[i];
+}
+
+// Pass 2: interior voxel (x,y,z) of leaf k = initial value + the entries at
+// that voxel of every run of the 27 cells around the leaf's cell, cells in
+// NB_ORDER, runs in point order.  A voxel is covered by neighbour (ox,oy,oz)
+// only within h of the matching faces; per-axis masks over NB_ORDER
+// positions select those neighbours.  R0: res0 at compile time (0: runtime).
+template <int R0>
+__global__ __launch_bounds__(256) void k_splat_collect(SplatParams sp, const int32_t* run_off, const float* part,
+                                                       int res0_rt, int h) {
+    const int64_t k = blockIdx.x;
+    if (k >= sp.nleaf) return;
+    const int res0 = R0 > 0 ? R0 : res0_rt;
+    __shared__ int32_t nrun[27];
+    __shared__ int32_t goff[27];       // grid index offset of the neighbour's grid
+    __shared__ const float* pbase[27];  // first run's grid of the neighbour
+    __shared__ uint32_t axm[3][4];      // [axis][low | high << 1]: NB positions whose offset on the axis is allowed
+    __shared__ uint32_t nonempty;
+    const int G = res0 + 2 * h;
+    const int64_t G3 = (int64_t)G * G * G;
+    const GNode nd = sp.leaves[k];
+    const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
+    if (threadIdx.x < 27) {
+        const int d = threadIdx.x;
+        const int64_t ci = nb_bin(sp.gb, d, cx, cy, cz);
+        const int rb = ci >= 0 ? run_off[ci] : 0, re = ci >= 0 ? run_off[ci + 1] : 0;
+        nrun[d] = re - rb;
+        pbase[d] = part + (int64_t)rb * G3;
+        const int nb = NB_ORDER[d];
+        const int ox = nb / 9 - 1, oy = (nb / 3) % 3 - 1, oz = nb % 3 - 1;
+        goff[d] = ((h - oz * res0) * G + (h - oy * res0)) * G + (h - ox * res0);
+    } else if (threadIdx.x >= 32 && threadIdx.x < 44) {
+        const int ax = (threadIdx.x - 32) >> 2, cs = (threadIdx.x - 32) & 3;
+        uint32_t mm = 0;
+        for (int d = 0; d < 27; d++) {
+            const int nb = NB_ORDER[d];
+            const int o = (ax == 0 ? nb / 9 : ax == 1 ? (nb / 3) % 3 : nb % 3) - 1;
+            if (o == 0 || (o < 0 && (cs & 1)) || (o > 0 && (cs & 2))) mm |= 1u << d;
+        }
+        axm[ax][cs] = mm;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const bool ne = threadIdx.x < 27 && nrun[threadIdx.x] > 0;
+        const uint64_t bal = __ballot(ne);
+        if (threadIdx.x == 0) nonempty = (uint32_t)bal;
+    }
+    __syncthreads();
+    const uint32_t nem = nonempty;
+    const int br = sp.brick_res, a = sp.apron;
+    float* brick = sp.chan + k * (int64_t)br * br * br;
+    const int nv = res0 * res0 * res0;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        const int x = i % res0, y = (i / res0) % res0, z = i / (res0 * res0);
+        const int64_t vi = ((int64_t)(z + a) * br + (y + a)) * br + (x + a);
+        float v = sp.accum ? brick[vi] : 0.0f;
+        const int vg = (z * G + y) * G + x;
+        uint32_t mm = nem & axm[0][(x < h) | ((x >= res0 - h) << 1)] & axm[1][(y < h) | ((y >= res0 - h) << 1)] &
+                      axm[2][(z < h) | ((z >= res0 - h) << 1)];
+        while (mm) {
+            const int d = __builtin_ctz(mm);
+            mm &= mm - 1;
+            const float* src = pbase[d] + (vg + goff[d]);
+            const int n = nrun[d];
+            v += src[0];
+            for (int q = 1; q < n; q++) v += src[(int64_t)q * G3];
+        }
+        brick[vi] = v;
+    }
+}
+
+// ---- tile pull ----
+template <class P>
+__device__ __forceinline__ bool next_bin_chunk(ChunkIter& it, const P& gp, int cx, int cy, int cz) {
+    if (it.j0 + 64 < it.end) {
+        it.j0 += 64;
+        return true;
+    }
+    while (++it.d < 27) {
+        const int64_t ci = nb_bin(gp.gb, it.d, cx, cy, cz);
+        if (ci < 0) continue;
+        const int beg = gp.off[ci], end = gp.off[ci + 1];
+        if (beg < end) {
+            it.j0 = beg;
+            it.end = end;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <bool POW2>
+__global__ __launch_bounds__(64) void k_splat_tile(SplatParams sp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
+    __shared__ float4 cand[SPLAT_STAGE];
     __shared__ float acc[512];
     const int lane = threadIdx.x;
     const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles);
@@ -502,71 +763,70 @@ __global__ __launch_bounds__(64) void k_splat_cubic(SplatParams sp, int res0, in
     const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
     const float r = sp.radius;
     const float r2 = r * r;
-    int cnt = 0;
+    const float inv_r2 = 1.0f / r2;
 
-    auto process = [&](int count) {
-        __syncthreads();
-        for (int j = lane; j < count; j += 64) {
-            const float4 c = cand[j];
-            const float px = c.x, py = c.y, pz = c.z, wt = c.w;
-            const int x0 = max((int)ceilf(px - r) - T0x, 0), x1 = min((int)floorf(px + r) - T0x, td - 1);
-            const int y0 = max((int)ceilf(py - r) - T0y, 0), y1 = min((int)floorf(py + r) - T0y, td - 1);
-            const int z0 = max((int)ceilf(pz - r) - T0z, 0), z1 = min((int)floorf(pz + r) - T0z, td - 1);
-            for (int z = z0; z <= z1; z++) {
-                const float dz = (float)(T0z + z) - pz;
-                for (int y = y0; y <= y1; y++) {
-                    const float dy = (float)(T0y + y) - py;
-                    for (int x = x0; x <= x1; x++) {
-                        const float dx = (float)(T0x + x) - px;
-                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        if (d2 < r2) {
-                            const float t = 1.0f - d2 / r2;
-                            atomicAdd(&acc[(z * td + y) * td + x], wt * ((t * t) * t));
-                        }
+    // lane `lane` splats staged point `slot` (slot < count)
+    auto splat_one = [&](int slot, int count) {
+        if (slot >= count) return;
+        const float4 c = cand[slot];
+        const float px = c.x, py = c.y, pz = c.z, wt = c.w;
+        const int x0 = max((int)ceilf(px - r) - T0x, 0), x1 = min((int)floorf(px + r) - T0x, td - 1);
+        const int y0 = max((int)ceilf(py - r) - T0y, 0), y1 = min((int)floorf(py + r) - T0y, td - 1);
+        const int z0 = max((int)ceilf(pz - r) - T0z, 0), z1 = min((int)floorf(pz + r) - T0z, td - 1);
+        for (int z = z0; z <= z1; z++) {
+            const float dz = (float)(T0z + z) - pz;
+            for (int y = y0; y <= y1; y++) {
+                const float dy = (float)(T0y + y) - py;
+                for (int x = x0; x <= x1; x++) {
+                    const float dx = (float)(T0x + x) - px;
+                    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    if (d2 < r2) {
+                        const float t = 1.0f - (POW2 ? d2 * inv_r2 : d2 / r2);
+                        atomicAdd(&acc[(z * td + y) * td + x], wt * ((t * t) * t));
                     }
                 }
             }
         }
-        __syncthreads();
     };
 
-    for (int dcx = -1; dcx <= 1; dcx++)
-        for (int dcy = -1; dcy <= 1; dcy++)
-            for (int dcz = -1; dcz <= 1; dcz++) {
-                int bx = cx + dcx, by = cy + dcy, bz = cz + dcz;
-                if (!sp.gb.contains(bx, by, bz)) continue;
-                int64_t ci = sp.gb.index(bx, by, bz);
-                int beg = sp.off[ci], end = sp.off[ci + 1];
-                for (int j0 = beg; j0 < end; j0 += 64) {
-                    int j = j0 + lane;
-                    bool ok = j < end;
-                    float4 p = make_float4(0, 0, 0, 0);
-                    float wt = 1.0f;
-                    if (ok) {
-                        p = sp.sorted[j];
-                        float pw[3] = {p.x, p.y, p.z};
-                        int T0[3] = {T0x, T0y, T0z};
+    int cnt = 0;
+    ChunkIter it;
+    bool have = next_bin_chunk(it, sp, cx, cy, cz);
+    float4 cur = make_float4(0, 0, 0, 0);
+    if (have && it.j0 + lane < it.end) cur = sp.sorted[it.j0 + lane];
+    while (have) {
+        const int cend = it.end, cj = it.j0 + lane;
+        have = next_bin_chunk(it, sp, cx, cy, cz);
+        float4 nxt = make_float4(0, 0, 0, 0);
+        if (have && it.j0 + lane < it.end) nxt = sp.sorted[it.j0 + lane];
+        bool ok = cj < cend;
+        {
+            const float pw[3] = {cur.x, cur.y, cur.z};
+            const int T0[3] = {T0x, T0y, T0z};
 #pragma unroll
-                        for (int a2 = 0; a2 < 3; a2++) {
-                            int lo = (int)ceilf(pw[a2] - r) - T0[a2];
-                            int hi = (int)floorf(pw[a2] + r) - T0[a2];
-                            ok = ok && hi >= 0 && lo <= td - 1;
-                        }
-                        if (ok && sp.weights) wt = *(const float*)(sp.weights + (int64_t)__float_as_int(p.w) * sp.wstride);
-                    }
-                    uint64_t bal = __ballot(ok);
-                    if (ok) {
-                        int pos = cnt + (int)__popcll(bal & ((1ull << lane) - 1ull));
-                        cand[pos] = make_float4(p.x, p.y, p.z, wt);
-                    }
-                    cnt += (int)__popcll(bal);
-                    if (cnt > CAND - 64) {
-                        process(cnt);
-                        cnt = 0;
-                    }
-                }
+            for (int a2 = 0; a2 < 3; a2++) {
+                const int lo = (int)ceilf(pw[a2] - r) - T0[a2];
+                const int hi = (int)floorf(pw[a2] + r) - T0[a2];
+                ok = ok && hi >= 0 && lo <= td - 1;
             }
-    if (cnt > 0) process(cnt);
+        }
+        const uint64_t bal = __ballot(ok);
+        if (ok) cand[cnt + (int)__popcll(bal & ((1ull << lane) - 1ull))] = cur;
+        cnt += (int)__popcll(bal);
+        if (cnt >= 64) {
+            __syncthreads();
+            splat_one(lane, 64);
+            // move the remainder (< 64) to the front
+            float4 rest = make_float4(0, 0, 0, 0);
+            if (lane < cnt - 64) rest = cand[64 + lane];
+            __syncthreads();
+            if (lane < cnt - 64) cand[lane] = rest;
+            cnt -= 64;
+        }
+        cur = nxt;
+    }
+    __syncthreads();
+    splat_one(lane, cnt);
     __syncthreads();
     for (int i = lane; i < nv; i += 64) {
         const int x = i % td, y = (i / td) % td, z = i / (td * td);
@@ -724,14 +984,80 @@ void launch_gather(GatherParams gp, hipStream_t s) {
         hipLaunchKernelGGL(k_gather_rl<false>, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
 }
 
-void launch_splat_cubic(SplatParams sp, hipStream_t s) {
-    int res0 = sp.brick_res - 2 * sp.apron;
-    int td = res0 < 8 ? res0 : 8;
-    int per = res0 / td;
-    int tiles = per * per * per;
-    int64_t nblk = sp.nleaf * tiles;
-    if (nblk > 0)
-        hipLaunchKernelGGL(k_splat_cubic, dim3((unsigned)gvdbx_xcd_blocks(nblk)), dim3(64), 0, s, sp, res0, tiles, nblk);
+int splat_halo(float radius) { return (int)std::ceil(radius); }
+int splat_width(float radius) { return (int)std::ceil(2.0f * radius); }
+
+int64_t splat_cell_lds(int res0, float radius) {
+    const int64_t G = res0 + 2 * splat_halo(radius);
+    return G * G * G * 4;
+}
+
+void launch_splat_runs(SplatParams sp, int chunk, int32_t* runs, hipStream_t s) {
+    const int64_t nc = sp.gb.count();
+    if (nc > 0) hipLaunchKernelGGL(k_splat_runs, dim3(grid_for(nc, 256)), dim3(256), 0, s, sp.off, nc, chunk, runs);
+}
+
+void launch_splat_run_map(SplatParams sp, const int32_t* run_off, int32_t* run_cell, hipStream_t s) {
+    const int64_t nc = sp.gb.count();
+    if (nc > 0) hipLaunchKernelGGL(k_splat_run_map, dim3(grid_for(nc, 256)), dim3(256), 0, s, run_off, nc, run_cell);
+}
+
+static bool radius_pow2(float radius) {
+    int e = 0;
+    return std::frexp(radius * radius, &e) == 0.5f;
+}
+
+void launch_splat_cell(SplatParams sp, const int32_t* run_cell, const int32_t* run_off, int nruns, int chunk,
+                       float* part, hipStream_t s) {
+    if (nruns <= 0) return;
+    const int res0 = sp.brick_res - 2 * sp.apron;
+    const int h = splat_halo(sp.radius), W = splat_width(sp.radius);
+    const size_t lds = (size_t)splat_cell_lds(res0, sp.radius);
+    const bool pow2 = radius_pow2(sp.radius), small = W <= 4;
+    const dim3 grid((unsigned)nruns);
+    if (pow2 && small)
+        hipLaunchKernelGGL((k_splat_cell<true, true>), grid, dim3(64), lds, s, sp, run_cell, run_off, nruns, res0, h, W,
+                           chunk, part);
+    else if (pow2)
+        hipLaunchKernelGGL((k_splat_cell<true, false>), grid, dim3(64), lds, s, sp, run_cell, run_off, nruns, res0, h,
+                           W, chunk, part);
+    else if (small)
+        hipLaunchKernelGGL((k_splat_cell<false, true>), grid, dim3(64), lds, s, sp, run_cell, run_off, nruns, res0, h,
+                           W, chunk, part);
+    else
+        hipLaunchKernelGGL((k_splat_cell<false, false>), grid, dim3(64), lds, s, sp, run_cell, run_off, nruns, res0, h,
+                           W, chunk, part);
+}
+
+void launch_splat_collect(SplatParams sp, const int32_t* run_off, const float* part, hipStream_t s) {
+    const int res0 = sp.brick_res - 2 * sp.apron;
+    if (sp.nleaf <= 0) return;
+    if (res0 == 8)
+        hipLaunchKernelGGL(k_splat_collect<8>, dim3((unsigned)sp.nleaf), dim3(256), 0, s, sp, run_off, part, res0,
+                           splat_halo(sp.radius));
+    else
+        hipLaunchKernelGGL(k_splat_collect<0>, dim3((unsigned)sp.nleaf), dim3(256), 0, s, sp, run_off, part, res0,
+                           splat_halo(sp.radius));
+}
+
+void launch_splat_tile(SplatParams sp, hipStream_t s) {
+    const int res0 = sp.brick_res - 2 * sp.apron;
+    const int td = res0 < 8 ? res0 : 8;
+    const int per = res0 / td;
+    const int tiles = per * per * per;
+    const int64_t nblk = sp.nleaf * tiles;
+    if (nblk <= 0) return;
+    const dim3 grid((unsigned)gvdbx_xcd_blocks(nblk));
+    if (radius_pow2(sp.radius))
+        hipLaunchKernelGGL(k_splat_tile<true>, grid, dim3(64), 0, s, sp, res0, tiles, nblk);
+    else
+        hipLaunchKernelGGL(k_splat_tile<false>, grid, dim3(64), 0, s, sp, res0, tiles, nblk);
+}
+
+void launch_splat_src(const float4* sorted, int64_t n, const int32_t* count, const char* weights, int64_t wstride,
+                      float4* out, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_splat_src, dim3(grid_for(n, 256)), dim3(256), 0, s, sorted, n, count, weights, wstride, out);
 }
 
 void launch_apron(ApronParams ap, hipStream_t s) {

@@ -85,12 +85,16 @@ struct gvdbx_volume {
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int xcd_remap = 0;
+    int splat_chunk = SPLAT_CHUNK_DEFAULT;
+    int splat_variant = 1;
     // atlas map
     bool has_atlas = false;
     DevBuf amap;
     int64_t amap_cnt = 0;
     // binning
     DevBuf bin_off, sorted, bkey, bval, bkey2, bval2, rs_hist, rs_off, rs_part;
+    DevBuf splat_src;   // (x, y, z, weight) in bin order, made per SplatCubic call
+    DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
     bool bins_valid = false;
     int64_t bins_n = -1;
@@ -1012,6 +1016,16 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->xcd_remap = value;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "splat_chunk")) {
+        if (value < 64 || value > (1 << 24)) FAIL(GVDBX_ERR_ARG, "splat_chunk must be in [64, 2^24]");
+        v->splat_chunk = value;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "splat_variant")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "splat_variant must be 0 or 1");
+        v->splat_variant = value;
+        return GVDBX_OK;
+    }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1024,6 +1038,14 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "xcd_remap")) {
         *value = v->xcd_remap;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "splat_chunk")) {
+        *value = v->splat_chunk;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "splat_variant")) {
+        *value = v->splat_variant;
         return GVDBX_OK;
     }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
@@ -1207,7 +1229,6 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
     sp.nleaf = v->cnt[0];
     sp.gb = v->gbin;
     sp.off = v->bin_off.as<int32_t>();
-    sp.sorted = v->sorted.as<float4>();
     sp.weights = v->wts;
     sp.wstride = v->wstride;
     sp.chan = v->chan[chan].as<float>();
@@ -1215,7 +1236,39 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
     sp.apron = std::max(v->apron, 0);
     sp.accum = accum ? 1 : 0;
     sp.radius = radius;
-    launch_splat_cubic(sp, v->stream);
+    sp.bins = v->sorted.as<float4>();
+    sp.cm = cloud_map(v);
+    hipStream_t s = v->stream;
+    const int res0 = v->res[0];
+    if (v->splat_variant == 0 || splat_cell_lds(res0, radius) > 65536) {
+        TRY(ensure(v, v->splat_src, (size_t)std::max(num_pnts, 1) * 16));
+        launch_splat_src(v->sorted.as<float4>(), num_pnts, v->bin_off.as<int32_t>() + v->gbin.count(), v->wts,
+                         v->wstride, v->splat_src.as<float4>(), s);
+        sp.sorted = v->splat_src.as<float4>();
+        launch_splat_tile(sp, s);
+        return check_launch("SplatCubic");
+    }
+    const int64_t nc = v->gbin.count();
+    const int64_t stiles = scan_tiles(nc + 1);
+    TRY(ensure(v, v->sp_runs, (size_t)nc * 4));
+    TRY(ensure(v, v->sp_runoff, (size_t)(nc + 1) * 4));
+    TRY(ensure(v, v->sp_scan, (size_t)(stiles + 1) * 4));
+    TRY(ensure_host(v, 4096));
+    int32_t* runs = v->sp_runs.as<int32_t>();
+    int32_t* run_off = v->sp_runoff.as<int32_t>();
+    uint32_t* spart = v->sp_scan.as<uint32_t>();
+    launch_splat_runs(sp, v->splat_chunk, runs, s);
+    launch_count_scan_reduce(runs, nc, spart, spart + stiles, s);
+    launch_count_scan_apply(runs, nc, spart, run_off, s);
+    GVDBX_CHECK(hipMemcpyAsync(v->host, run_off + nc, 4, hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    const int nruns = *(const int32_t*)v->host;
+    const int64_t G = res0 + 2 * splat_halo(radius);
+    TRY(ensure(v, v->sp_runcell, (size_t)std::max(nruns, 1) * 4));
+    TRY(ensure(v, v->sp_part, (size_t)std::max(nruns, 1) * G * G * G * 4));
+    launch_splat_run_map(sp, run_off, v->sp_runcell.as<int32_t>(), s);
+    launch_splat_cell(sp, v->sp_runcell.as<int32_t>(), run_off, nruns, v->splat_chunk, v->sp_part.as<float>(), s);
+    launch_splat_collect(sp, run_off, v->sp_part.as<float>(), s);
     return check_launch("SplatCubic");
 }
 

@@ -85,8 +85,14 @@ int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value);
 /* choose the topology builder (no reference counterpart; default AUTO) */
 int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
 /* kernel tuning knobs (no reference counterpart): key "gather_variant"
- * (0 = LDS-staged candidates, 1 = ballot/readlane walk; default 1),
- * "xcd_remap" (1 = contiguous brick runs per XCD; default 0) */
+ * (0 = LDS-staged candidates, 1 = ballot/readlane walk, 2 = per-lane
+ * candidates with LDS minima; default 1), "xcd_remap" (1 = contiguous brick
+ * runs per XCD; default 0), "splat_variant" (SplatCubic: 0 = tile pull,
+ * 1 = cell push; default 1; cell push needs a (res0 + 2 ceil(r))^3 float
+ * grid within 64 KiB, else tile pull runs),
+ * "splat_chunk" (cell push: points per wavefront run; the float summation
+ * order, and so the last bits, depend on it; default 4096, range
+ * [64, 2^24]) */
 int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value);
 int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value);
 
