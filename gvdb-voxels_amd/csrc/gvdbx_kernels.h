@@ -143,7 +143,7 @@ struct SplatParams {
 // per cell, run_off = exclusive scan, run_cell maps run -> cell), each run's
 // (res0 + 2h)^3 grid in `part` (h = ceil(r)), collected per leaf.  Tile pull:
 // one kernel, for grids that do not fit LDS.
-constexpr int SPLAT_CHUNK_DEFAULT = 4096;
+constexpr int SPLAT_CHUNK_DEFAULT = 1024;
 int splat_halo(float radius);
 int64_t splat_cell_lds(int res0, float radius);   // bytes of LDS per run
 void launch_splat_runs(SplatParams sp, int chunk, int32_t* runs, hipStream_t s);

@@ -253,25 +253,67 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
     }
 }
 
-// one segment checkpoint, allocated wave-aggregated (one atomic per wave)
-__device__ __forceinline__ void record_visit(const VisitRec& vr, int64_t leaf, uint32_t ray, float t, float T,
-                                             float I) {
-    const uint64_t act = __ballot(1);
+// Segment checkpoints are buffered per wave in LDS and written out VIS_BUF at
+// a time, so a wave takes one slot range (one global atomic) per VIS_BUF
+// records instead of one per march step.  Slot order is irrelevant: the
+// backward sorts checkpoints by their (leaf, ray) key, which is unique.
+constexpr int VIS_BUF = 128;
+struct VisitLds {
+    uint64_t key[VIS_BUF];
+    float4 rec[VIS_BUF];
+    int cnt;
+};
+
+// Write the cnt buffered checkpoints with the lanes in `act` (any subset of
+// the wave).
+__device__ __forceinline__ void flush_visits(VisitLds& vl, const VisitRec& vr, int cnt, uint64_t act) {
     const int lane = __lane_id();
     const int leader = __ffsll((unsigned long long)act) - 1;
+    const int n = __popcll(act);
+    const int rank = __popcll(act & ((1ull << lane) - 1));
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(vr.count, (uint32_t)__popcll(act));
+    if (lane == leader) base = atomicAdd(vr.count, (uint32_t)cnt);
     base = __builtin_amdgcn_readlane(base, leader);
-    const uint64_t slot = (uint64_t)base + __popcll(act & ((1ull << lane) - 1));
-    if ((int64_t)slot < vr.cap) {
-        vr.key[slot] = ((uint64_t)leaf << vr.ray_bits) | ray;
-        vr.rec[slot] = make_float4(t, T, I, 0.0f);
+    __builtin_amdgcn_wave_barrier();
+    for (int i = rank; i < cnt; i += n) {
+        const int64_t slot = (int64_t)base + i;
+        if (slot < vr.cap) {
+            vr.key[slot] = vl.key[i];
+            vr.rec[slot] = vl.rec[i];
+        }
     }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The buffer fill count is shared by the wave's lanes (read by all, written
+// by one): wavefront-scope atomics keep every access in program order.
+__device__ __forceinline__ int visit_count(VisitLds& vl) {
+    return __hip_atomic_load(&vl.cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ __forceinline__ void set_visit_count(VisitLds& vl, int c) {
+    __hip_atomic_store(&vl.cnt, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+__device__ __forceinline__ void record_visit(VisitLds& vl, const VisitRec& vr, int64_t leaf, uint32_t ray, float t,
+                                             float T, float I) {
+    const uint64_t act = __ballot(1);
+    const int lane = __lane_id();
+    const int n = __popcll(act);
+    const int rank = __popcll(act & ((1ull << lane) - 1));
+    int cnt = visit_count(vl);
+    if (cnt + n > VIS_BUF) {
+        flush_visits(vl, vr, cnt, act);
+        cnt = 0;
+    }
+    vl.key[cnt + rank] = ((uint64_t)leaf << vr.ray_bits) | ray;
+    vl.rec[cnt + rank] = make_float4(t, T, I, 0.0f);
+    __builtin_amdgcn_wave_barrier();
+    if (rank == 0) set_visit_count(vl, cnt + n);
 }
 
 template <int MODE, int BR, bool RECORD>
 __device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, const RenderCam& cam, V3 pos, V3 dir,
-                         RayState& st, uint32_t ray, const VisitRec& vr) {
+                         RayState& st, uint32_t ray, const VisitRec& vr, VisitLds* vl = nullptr) {
     const int top = tv.top_lev;
     if (top < 1 || top >= tv.levs || tv.ncnt[top] < 1) return;
     const int* sh = cam.shift;
@@ -298,7 +340,7 @@ __device__ void ray_cast(const RenderTree& tv, const RenderScene& sc, const Rend
         if (child >= 0 && child < tv.ncnt[lev - 1]) {
             if (lev == 1) {
                 d.t.x += sc.epsilon;
-                if (RECORD) record_visit(vr, child, ray, d.t.x, st.clr[3], st.clr[0]);
+                if (RECORD) record_visit(*vl, vr, child, ray, d.t.x, st.clr[3], st.clr[0]);
                 brick_fn<MODE, BR>(tv, sc, child, d.t, pos, dir, st, sh);
                 if (st.clr[3] <= 0) {
                     st.clr[3] = 0;
@@ -414,6 +456,9 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
 template <int BR, bool RECORD>
 __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam0, const RenderCam* cams,
                                                     RenderScene sc, float* I, float* T, VisitRec vr) {
+    __shared__ VisitLds vbuf[RECORD ? 4 : 1];
+    VisitLds& vl = vbuf[RECORD ? threadIdx.x / 64 : 0];
+    if (RECORD && __lane_id() == 0) set_visit_count(vl, 0);
     const RenderCam& cam = cams ? cams[blockIdx.z] : cam0;
     int x, y;
     const bool ok = pixel_of(cam0, x, y);
@@ -425,15 +470,20 @@ __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam
     st.clr[3] = 1.0f;
     st.hit = v3(0, 0, GVDBX_NOHIT);
     const uint32_t ray = (uint32_t)((int64_t)blockIdx.z * cam0.width * cam0.height + (int64_t)y * cam0.width + x);
-    ray_cast<MODE_DEEP_LINEAR, BR, RECORD>(tv, sc, cam, pos, dir, st, ray, vr);
+    ray_cast<MODE_DEEP_LINEAR, BR, RECORD>(tv, sc, cam, pos, dir, st, ray, vr, &vl);
     I[ray] = st.clr[0];
     T[ray] = st.clr[3];
+    if (RECORD) {
+        const int cnt = visit_count(vl);
+        if (cnt > 0) flush_visits(vl, vr, cnt, __ballot(1));
+    }
 }
 
 // [begin, end) of each leaf's run in the sorted segment keys
-__global__ void k_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg) {
+__global__ void k_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t l = key[i] >> ray_bits;
+        if (l >= (uint64_t)nleaf) continue;
         if (i == 0 || (key[i - 1] >> ray_bits) != l) seg[2 * l] = (int32_t)i;
         if (i == n - 1 || (key[i + 1] >> ray_bits) != l) seg[2 * l + 1] = (int32_t)(i + 1);
     }
@@ -714,8 +764,7 @@ void launch_raycast_ea(RenderTree tv, RenderCam cam, const RenderCam* cams, int 
 }
 
 void launch_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf, hipStream_t s) {
-    (void)nleaf;
-    if (n > 0) hipLaunchKernelGGL(k_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, ray_bits, seg);
+    if (n > 0) hipLaunchKernelGGL(k_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, ray_bits, seg, nleaf);
 }
 
 void launch_ea_backward(EABackParams p, hipStream_t s) {

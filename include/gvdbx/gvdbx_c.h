@@ -91,7 +91,7 @@ int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
  * 1 = cell push; default 1; cell push needs a (res0 + 2 ceil(r))^3 float
  * grid within 64 KiB, else tile pull runs),
  * "splat_chunk" (cell push: points per wavefront run; the float summation
- * order, and so the last bits, depend on it; default 4096, range
+ * order, and so the last bits, depend on it; default 1024, range
  * [64, 2^24]) */
 int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value);
 int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value);
