@@ -28,6 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
 
 METRIC = "Mpoints/s topology+splat (HBM % of peak); fwd+bwd/fwd time ratio"
 HBM_PEAK = 8.0e12
+CU_COUNT = 256          # MI355X compute units
+CLOCK_HZ = 2.4e9        # MI355X peak engine clock
 SEED = 20261203 + 1          # configs[1]
 
 
@@ -63,17 +65,22 @@ def frame_bytes_alg(n, nleaf, node_cnt, res, depth, br=10, r0=8, passes=3):
     return b
 
 
-def measured_traffic(kernel_profile):
-    """HBM bytes per launch of the dominant kernel from its committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+def measured_value(kernel_profile, key):
+    """A `key: value` line of a committed rocprofv3 counter summary, or None."""
     path = os.path.join(ROOT, "profiles", kernel_profile)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+            if line.startswith(key + ":"):
+                return int(float(line.split(":")[1]))
     except OSError:
         return None
     return None
+
+
+def measured_traffic(kernel_profile):
+    """HBM bytes per launch of the dominant kernel from its committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    return measured_value(kernel_profile, "traffic_bytes_per_launch")
 
 
 def make_clusters(seed, n_clusters=64, per=32768):
@@ -359,9 +366,11 @@ def main():
     ap.add_argument("--frames", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable splat+render leg")
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched depth-fusion leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched depth-fusion and batched-clouds legs")
+    ap.add_argument("--only", choices=["diff", "depth", "clouds"], default=None,
+                    help="run only this leg and print its JSON (for profiling; not the benchmark line)")
     ap.add_argument("--gather-variant", type=int, default=None,
-                    help="0: LDS-staged candidates, 1: ballot/readlane walk (default)")
+                    help="0: LDS-staged candidates, 1: ballot/readlane walk (default), 2: point-centric LDS minima")
     ap.add_argument("--xcd-remap", type=int, default=None, help="1: contiguous brick runs per XCD (default 0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
@@ -382,6 +391,16 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4}))
+        return
+    if args.only == "diff":
+        print(json.dumps(bench_differentiable(torch, gvdbx, dev, max(3, min(args.steps, 10)),
+                                              max(1, min(args.warmup, 2)))), flush=True)
+        return
+    if args.only == "depth":
+        print(json.dumps(bench_depth_fusion(torch, gvdbx, dev, max(2, min(args.steps, 5)))), flush=True)
+        return
+    if args.only == "clouds":
+        print(json.dumps(bench_batched_clouds(torch, gvdbx, dev, 2)), flush=True)
         return
     n = args.points
     frames, pts0 = make_frames(torch, gvdbx, n, args.frames, dev)
@@ -449,6 +468,7 @@ def main():
     fb = frame_bytes_alg(n, nleaf, list(inf.node_cnt), res, inf.depth)
     value = n / (ms / 1e3) / 1e6
     traffic = measured_traffic("gather_levelset_config2_counters.md") if n == 1 << 24 else None
+    valu = measured_value("gather_levelset_config2_counters.md", "valu_insts_per_launch") if n == 1 << 24 else None
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -462,7 +482,13 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
                      "traffic": traffic, "alg_bytes_per_launch": gather_bytes,
                      "avg_launch_ms": round(stage["gather_levelset_ms"], 4),
-                     "traffic_source": "profiles/gather_levelset_config2_counters.md" if traffic else None},
+                     "traffic_source": "profiles/gather_levelset_config2_counters.md" if traffic else None,
+                     # the kernel is bound by vector-ALU issue, not HBM: wave64 VALU instructions per launch
+                     # (same counter summary) over what 256 CUs issue at one per cycle at 2.4 GHz in the
+                     # event-timed launch duration
+                     "valu_insts_per_launch": valu,
+                     "valu_issue_frac": (round(valu / (stage["gather_levelset_ms"] / 1e3 * CU_COUNT * CLOCK_HZ), 3)
+                                         if valu else None)},
         "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
                            "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
