@@ -508,21 +508,31 @@ __device__ __forceinline__ void shift_corners(float* acc, bool up) {
 //   q_k = gI * (albedo T_k ((1-a_k) - e s_k a_k) + e (I_final - I_k)) + gT e T_final
 // (I_k: intensity after sample k).  The leaf's segments (sorted by ray) are
 // cut into four contiguous quarters, one per wave, and each lane takes a
-// contiguous run of its quarter, so the lanes of one LDS add are spread over
-// the brick face instead of piling onto the same voxels.  Each wave adds
-// into its own LDS copy in an order fixed by the data; the four copies are
-// summed in a fixed order at the end.
+// contiguous run of its quarter.  Each wave adds into its own LDS copy; the
+// four copies are summed in a fixed order at the end.
+// A lane keeps the eight corner sums of its current trilinear cell in
+// registers and flushes a corner when the ray leaves it.  A flush first
+// claims the voxel: the lane writes its tag to claim[voxel] and reads it
+// back.  The one lane that reads its own tag is the only flusher of that
+// voxel that can still see its tag, so it adds with a plain read-add-write;
+// the others (lanes of the same wave on the same voxel, or whose claim
+// another wave overwrote) add afterwards with an LDS atomic.  LDS float
+// atomics are slow (tens of cycles per instruction), so the claim pays for
+// itself whenever most flushes are alone on their voxel, which rays that
+// march side by side make the common case.
 template <int BR>
 __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     constexpr int NV = BR * BR * BR;
     constexpr int NW = 4;
     __shared__ float dens[NV];
     __shared__ float grad[NW][NV];
+    __shared__ int claim[NV];
     const int64_t leaf = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float* src = p.tv.chan + leaf * (int64_t)NV;
     for (int i = tid; i < NV; i += 256) {
         dens[i] = src[i];
+        claim[i] = -1;
 #pragma unroll
         for (int w = 0; w < NW; w++) grad[w][i] = 0.0f;
     }
@@ -569,22 +579,43 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
                 if (((m >> axis_bit) & 1) == mask_keep) continue;
                 const int xi = min(max(cx + (m & 1), 0), BR - 1), yi = min(max(cy + ((m >> 1) & 1), 0), BR - 1),
                           zi = min(max(cz + (m >> 2), 0), BR - 1);
-                if (acc[m] != 0.0f) atomicAdd(&g[(zi * BR + yi) * BR + xi], acc[m]);
+                if (acc[m] != 0.0f) {
+                    const int vi = (zi * BR + yi) * BR + xi;
+                    // relaxed atomics: plain LDS stores and loads the compiler may not fold
+                    __hip_atomic_store(&claim[vi], tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (__hip_atomic_load(&claim[vi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == tid)
+                        g[vi] = g[vi] + acc[m];
+                    else
+                        atomicAdd(&g[vi], acc[m]);
+                }
                 acc[m] = 0.0f;
             }
         };
         for (int iter = 0; T > sc.alphacut && iter < sc.max_iter && q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < r0 &&
                            q.y < r0 && q.z < r0;
              iter++) {
-            const float s = trilinear<BR>(dens, BR, q);
+            // trilinear sample (as trilinear<BR>) sharing its cell and weights with the adjoint
+            const float fx = floorf(q.x + 0.5f), fy = floorf(q.y + 0.5f), fz = floorf(q.z + 0.5f);
+            const float ax = (q.x + 0.5f) - fx, ay = (q.y + 0.5f) - fy, az = (q.z + 0.5f) - fz;
+            const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+            const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+            const int x0 = min(max(ix, 0), BR - 1), x1 = min(max(ix + 1, 0), BR - 1);
+            const int y0 = min(max(iy, 0), BR - 1), y1 = min(max(iy + 1, 0), BR - 1);
+            const int z0 = min(max(iz, 0), BR - 1), z1 = min(max(iz + 1, 0), BR - 1);
+            const float* r00 = dens + (z0 * BR + y0) * BR;
+            const float* r10 = dens + (z0 * BR + y1) * BR;
+            const float* r01 = dens + (z1 * BR + y0) * BR;
+            const float* r11 = dens + (z1 * BR + y1) * BR;
+            const float c00 = r00[x0] * bx + r00[x1] * ax;
+            const float c10 = r10[x0] * bx + r10[x1] * ax;
+            const float c01 = r01[x0] * bx + r01[x1] * ax;
+            const float c11 = r11[x0] * bx + r11[x1] * ax;
+            const float s = (c00 * by + c10 * ay) * bz + (c01 * by + c11 * ay) * az;
             if (s >= sc.minval) {
                 const float a = expf(sc.extinct * s * step);
                 I += s * T * (1 - a) * sc.albedo;
                 const float gq = gi * (sc.albedo * T * ((1.0f - a) - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
                 T *= a;
-                const float fx = floorf(q.x + 0.5f), fy = floorf(q.y + 0.5f), fz = floorf(q.z + 0.5f);
-                const float ax = (q.x + 0.5f) - fx, ay = (q.y + 0.5f) - fy, az = (q.z + 0.5f) - fz;
-                const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
                 if (have && !(ix == cx && iy == cy && iz == cz)) {
                     const int ddx = ix - cx, ddy = iy - cy, ddz = iz - cz;
                     int axis = -1, sgn = 0;
@@ -605,12 +636,9 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
                 }
                 cx = ix, cy = iy, cz = iz;
                 have = true;
+                const float gyz[4] = {gq * by * bz, gq * ay * bz, gq * by * az, gq * ay * az};
 #pragma unroll
-                for (int m = 0; m < 8; m++) {
-                    const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
-                    const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dz ? az : 1.0f - az);
-                    acc[m] += gq * w;
-                }
+                for (int m = 0; m < 8; m++) acc[m] += gyz[m >> 1] * ((m & 1) ? ax : bx);
             }
             q = add(q, wpt);
         }
