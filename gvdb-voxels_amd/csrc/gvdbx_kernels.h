@@ -306,6 +306,9 @@ struct SplatBackParams {
     float* dw;
 };
 void launch_splat_backward(SplatBackParams p, hipStream_t s);
+// cell-push adjoint over the runs of launch_splat_runs (bins gb / off)
+void launch_splat_cell_backward(SplatBackParams p, CellGrid gb, const int32_t* off, const int32_t* run_cell,
+                                const int32_t* run_off, int nruns, int res0, int chunk, hipStream_t s);
 void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s);
 
 // ---------------- batched inputs ----------------

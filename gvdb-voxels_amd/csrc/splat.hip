@@ -714,6 +714,109 @@ __global__ __launch_bounds__(256) void k_splat_collect(SplatParams sp, const int
     }
 }
 
+// Adjoint of the cubic splat over the same runs:
+//   dL/dw = sum_v g(v) t^3,  dL/dp = sum_v g(v) w 6 t^2 (v - p) / r^2,
+//   t = 1 - |v-p|^2/r^2, over integer voxels v of active leaves, |v-p| < r.
+// One wave per run: the upstream gradients of the run's cell plus its halo
+// are staged in an LDS grid (0 where no leaf), one leaf lookup per
+// neighbour cell; then each lane takes one point and sums over its W^3
+// support window, reading LDS only.
+template <bool POW2, int WT>
+__global__ __launch_bounds__(64) void k_splat_cell_backward(SplatBackParams p, CellGrid gb, const int32_t* off,
+                                                            const int32_t* run_cell, const int32_t* run_off, int nruns,
+                                                            int res0, int h, int Wrt, int chunk) {
+    extern __shared__ float grid[];
+    __shared__ int32_t nbleaf[27];   // leaf of neighbour (dx,dy,dz) at dx*9 + dy*3 + dz (offsets + 1)
+    const int W = WT > 0 ? WT : Wrt;
+    const int G = res0 + 2 * h;
+    const int G3 = G * G * G;
+    const int lane = threadIdx.x;
+    const int b = (int)blockIdx.x;
+    if (b >= nruns) return;
+    const int64_t c = run_cell[b];
+    const int sub = b - run_off[c];
+    const int beg = off[c] + sub * chunk;
+    const int end = (int)min((int64_t)off[c + 1], (int64_t)beg + chunk);
+    const int64_t nyz = (int64_t)gb.ny * gb.nz;
+    const int gcx = (int)(c / nyz) + gb.ox, gcy = (int)((c / gb.nz) % gb.ny) + gb.oy, gcz = (int)(c % gb.nz) + gb.oz;
+    if (lane < 27) {
+        const int nx = gcx + lane / 9 - 1, ny = gcy + (lane / 3) % 3 - 1, nz = gcz + lane % 3 - 1;
+        int32_t leaf = -1;
+        if (p.g0.contains(nx, ny, nz)) leaf = p.idx0[p.g0.index(nx, ny, nz)];
+        nbleaf[lane] = (leaf >= 0 && leaf < p.nleaf) ? leaf : -1;
+    }
+    __syncthreads();
+    const int br = p.brick_res, a = (br - res0) / 2;
+    const int64_t nv = (int64_t)br * br * br;
+    for (int i = lane; i < G3; i += 64) {
+        const int ox = i % G - h, oy = (i / G) % G - h, oz = i / (G * G) - h;   // voxel offset from the cell origin
+        const int dx = ox < 0 ? 0 : (ox >= res0 ? 2 : 1), dy = oy < 0 ? 0 : (oy >= res0 ? 2 : 1),
+                  dz = oz < 0 ? 0 : (oz >= res0 ? 2 : 1);
+        const int32_t leaf = nbleaf[dx * 9 + dy * 3 + dz];
+        float v = 0.0f;
+        if (leaf >= 0) {
+            const int lx = ox - (dx - 1) * res0 + a, ly = oy - (dy - 1) * res0 + a, lz = oz - (dz - 1) * res0 + a;
+            v = p.grad[leaf * nv + ((int64_t)lz * br + ly) * br + lx];
+        }
+        grid[i] = v;
+    }
+    // positions are in the cloud's own voxel space
+    int cx = gcx, cy = gcy, cz = gcz;
+    if (p.cm.ncloud > 0) {
+        const int cl = p.cm.cloud_of_point(__float_as_int(p.sorted[beg].w));
+        cx -= p.cm.cell_off[3 * cl];
+        cy -= p.cm.cell_off[3 * cl + 1];
+        cz -= p.cm.cell_off[3 * cl + 2];
+    }
+    const int g0x = cx * res0 - h, g0y = cy * res0 - h, g0z = cz * res0 - h;
+    const float r = p.radius;
+    const float r2 = r * r;
+    const float inv_r2 = 1.0f / r2;
+    __syncthreads();
+    for (int j = beg + lane; j < end; j += 64) {
+        const float4 q = p.sorted[j];
+        const int i = __float_as_int(q.w);
+        const float wt = p.weights ? *(const float*)(p.weights + (int64_t)i * p.wstride) : 1.0f;
+        const int sx = support_start(q.x, r, r2, g0x, G, W);
+        const int sy = support_start(q.y, r, r2, g0y, G, W);
+        const int sz = support_start(q.z, r, r2, g0z, G, W);
+        float dzs[WT > 0 ? WT : 16];
+#pragma unroll
+        for (int k = 0; k < (WT > 0 ? WT : 16); k++)
+            if (k < W) dzs[k] = (float)(g0z + sz + k) - q.z;
+        float gw = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        for (int ix = 0; ix < W; ix++) {
+            const float dx = (float)(g0x + sx + ix) - q.x;
+            const float dxx = dx * dx;
+            for (int iy = 0; iy < W; iy++) {
+                const float dy = (float)(g0y + sy + iy) - q.y;
+                const float s2 = __builtin_fmaf(dy, dy, dxx);
+                const float* row = grid + ((sz * G) + (sy + iy)) * G + (sx + ix);
+#pragma unroll
+                for (int k = 0; k < (WT > 0 ? WT : 16); k++) {
+                    if (k < W) {
+                        const float dz = dzs[k];
+                        const float d2 = __builtin_fmaf(dz, dz, s2);
+                        if (d2 < r2) {
+                            const float g = row[k * G * G];
+                            const float t = POW2 ? __builtin_fmaf(-d2, inv_r2, 1.0f) : 1.0f - d2 / r2;
+                            gw += g * ((t * t) * t);
+                            const float f = POW2 ? g * wt * 6.0f * (t * t) * inv_r2 : g * wt * 6.0f * (t * t) / r2;
+                            gx += f * dx;
+                            gy += f * dy;
+                            gz += f * dz;
+                        }
+                    }
+                }
+            }
+        }
+        if (p.dw) p.dw[i] = gw;
+        p.dpos[3 * (int64_t)i] = gx;
+        p.dpos[3 * (int64_t)i + 1] = gy;
+        p.dpos[3 * (int64_t)i + 2] = gz;
+    }
+}
+
 // ---- tile pull ----
 template <class P>
 __device__ __forceinline__ bool next_bin_chunk(ChunkIter& it, const P& gp, int cx, int cy, int cz) {
@@ -1027,6 +1130,27 @@ void launch_splat_cell(SplatParams sp, const int32_t* run_cell, const int32_t* r
     else
         hipLaunchKernelGGL((k_splat_cell<false, false>), grid, dim3(64), lds, s, sp, run_cell, run_off, nruns, res0, h,
                            W, chunk, part);
+}
+
+void launch_splat_cell_backward(SplatBackParams p, CellGrid gb, const int32_t* off, const int32_t* run_cell,
+                                const int32_t* run_off, int nruns, int res0, int chunk, hipStream_t s) {
+    if (nruns <= 0) return;
+    const int h = splat_halo(p.radius), W = splat_width(p.radius);
+    const size_t lds = (size_t)splat_cell_lds(res0, p.radius);
+    const bool pow2 = radius_pow2(p.radius);
+    const dim3 grid((unsigned)nruns);
+    if (pow2 && W == 4)
+        hipLaunchKernelGGL((k_splat_cell_backward<true, 4>), grid, dim3(64), lds, s, p, gb, off, run_cell, run_off, nruns,
+                           res0, h, W, chunk);
+    else if (pow2)
+        hipLaunchKernelGGL((k_splat_cell_backward<true, 0>), grid, dim3(64), lds, s, p, gb, off, run_cell, run_off, nruns,
+                           res0, h, W, chunk);
+    else if (W == 4)
+        hipLaunchKernelGGL((k_splat_cell_backward<false, 4>), grid, dim3(64), lds, s, p, gb, off, run_cell, run_off,
+                           nruns, res0, h, W, chunk);
+    else
+        hipLaunchKernelGGL((k_splat_cell_backward<false, 0>), grid, dim3(64), lds, s, p, gb, off, run_cell, run_off,
+                           nruns, res0, h, W, chunk);
 }
 
 void launch_splat_collect(SplatParams sp, const int32_t* run_off, const float* part, hipStream_t s) {

@@ -1212,6 +1212,35 @@ int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp
     return check_launch("ScatterDensity");
 }
 
+// Cubic splat work runs over the current bins: each bin cell's points in
+// runs of at most splat_chunk (sp_runoff = exclusive scan of the per-cell run
+// counts, sp_runcell = run -> cell).  One host read of the run count.
+static int splat_run_plan(gvdbx_volume* v, int* nruns_out) {
+    hipStream_t s = v->stream;
+    SplatParams sp{};
+    sp.gb = v->gbin;
+    sp.off = v->bin_off.as<int32_t>();
+    const int64_t nc = v->gbin.count();
+    const int64_t stiles = scan_tiles(nc + 1);
+    TRY(ensure(v, v->sp_runs, (size_t)nc * 4));
+    TRY(ensure(v, v->sp_runoff, (size_t)(nc + 1) * 4));
+    TRY(ensure(v, v->sp_scan, (size_t)(stiles + 1) * 4));
+    TRY(ensure_host(v, 4096));
+    int32_t* runs = v->sp_runs.as<int32_t>();
+    int32_t* run_off = v->sp_runoff.as<int32_t>();
+    uint32_t* spart = v->sp_scan.as<uint32_t>();
+    launch_splat_runs(sp, v->splat_chunk, runs, s);
+    launch_count_scan_reduce(runs, nc, spart, spart + stiles, s);
+    launch_count_scan_apply(runs, nc, spart, run_off, s);
+    GVDBX_CHECK(hipMemcpyAsync(v->host, run_off + nc, 4, hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    const int nruns = *(const int32_t*)v->host;
+    TRY(ensure(v, v->sp_runcell, (size_t)std::max(nruns, 1) * 4));
+    launch_splat_run_map(sp, run_off, v->sp_runcell.as<int32_t>(), s);
+    *nruns_out = nruns;
+    return GVDBX_OK;
+}
+
 int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz, int chan, int accum) {
     TRY(check_vol(v));
     TRY(need_atlas(v, "SplatCubic"));
@@ -1248,25 +1277,11 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
         launch_splat_tile(sp, s);
         return check_launch("SplatCubic");
     }
-    const int64_t nc = v->gbin.count();
-    const int64_t stiles = scan_tiles(nc + 1);
-    TRY(ensure(v, v->sp_runs, (size_t)nc * 4));
-    TRY(ensure(v, v->sp_runoff, (size_t)(nc + 1) * 4));
-    TRY(ensure(v, v->sp_scan, (size_t)(stiles + 1) * 4));
-    TRY(ensure_host(v, 4096));
-    int32_t* runs = v->sp_runs.as<int32_t>();
-    int32_t* run_off = v->sp_runoff.as<int32_t>();
-    uint32_t* spart = v->sp_scan.as<uint32_t>();
-    launch_splat_runs(sp, v->splat_chunk, runs, s);
-    launch_count_scan_reduce(runs, nc, spart, spart + stiles, s);
-    launch_count_scan_apply(runs, nc, spart, run_off, s);
-    GVDBX_CHECK(hipMemcpyAsync(v->host, run_off + nc, 4, hipMemcpyDeviceToHost, s));
-    TRY(sync(v));
-    const int nruns = *(const int32_t*)v->host;
+    int nruns = 0;
+    TRY(splat_run_plan(v, &nruns));
     const int64_t G = res0 + 2 * splat_halo(radius);
-    TRY(ensure(v, v->sp_runcell, (size_t)std::max(nruns, 1) * 4));
+    const int32_t* run_off = v->sp_runoff.as<int32_t>();
     TRY(ensure(v, v->sp_part, (size_t)std::max(nruns, 1) * G * G * G * 4));
-    launch_splat_run_map(sp, run_off, v->sp_runcell.as<int32_t>(), s);
     launch_splat_cell(sp, v->sp_runcell.as<int32_t>(), run_off, nruns, v->splat_chunk, v->sp_part.as<float>(), s);
     launch_splat_collect(sp, run_off, v->sp_part.as<float>(), s);
     return check_launch("SplatCubic");
@@ -1523,7 +1538,15 @@ int gvdbx_splat_cubic_backward(gvdbx_volume* v, int num_pnts, float radius, floa
     p.grad = grad_dev;
     p.dpos = dpos_dev;
     p.dw = dw_dev;
-    launch_splat_backward(p, s);
+    const int res0 = v->res[0];
+    if (v->splat_variant == 0 || splat_cell_lds(res0, radius) > 65536) {
+        launch_splat_backward(p, s);
+        return check_launch("SplatCubicBackward");
+    }
+    int nruns = 0;
+    TRY(splat_run_plan(v, &nruns));
+    launch_splat_cell_backward(p, v->gbin, v->bin_off.as<int32_t>(), v->sp_runcell.as<int32_t>(),
+                               v->sp_runoff.as<int32_t>(), nruns, res0, v->splat_chunk, s);
     return check_launch("SplatCubicBackward");
 }
 

@@ -251,8 +251,24 @@ def test_splat_cubic_variants_and_runs():
             gv.FillChannel(0, 0.25)
             gv.SplatCubic(len(pts), radius, accum=True)
             check(gv.bricks()[:, 1:9, 1:9, 1:9], ref + 0.25)
+    # the adjoint under both forms and several run lengths, against the FP64 adjoint
+    inf = gv.info()
+    rng = np.random.default_rng(22)
+    gr = rng.normal(size=(inf.node_cnt[0], inf.brick_res, inf.brick_res, inf.brick_res)).astype(np.float32)
+    gt = torch.from_numpy(gr).to(tp.device)
+    rdpos, rdw = ov.splat_cubic_backward_f64(pts.astype(np.float64), 2.0, gr.astype(np.float64), w.astype(np.float64))
+    for variant, chunk in ((1, 1024), (1, 64), (0, 1024)):
+        gv.SetTuning("splat_variant", variant)
+        gv.SetTuning("splat_chunk", chunk)
+        dpos, dw = gv.SplatCubicBackward(len(pts), 2.0, (0, 0, 0), gt)
+        dpos2, dw2 = gv.SplatCubicBackward(len(pts), 2.0, (0, 0, 0), gt)
+        torch.cuda.synchronize()
+        assert torch.equal(dpos, dpos2) and torch.equal(dw, dw2)
+        for got, ref, name in ((dpos.cpu().numpy(), rdpos, "dpos"), (dw.cpu().numpy(), rdw, "dw")):
+            scale = np.abs(ref).max()
+            assert np.abs(got - ref).max() <= 1e-5 * scale, f"{name} variant {variant} chunk {chunk}"
     gv.SetTuning("splat_variant", 1)
-    gv.SetTuning("splat_chunk", 4096)
+    gv.SetTuning("splat_chunk", 1024)
     with pytest.raises(Exception):
         gv.SetTuning("splat_chunk", 10)
     with pytest.raises(Exception):
