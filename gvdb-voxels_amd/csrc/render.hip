@@ -145,6 +145,31 @@ __device__ __forceinline__ float trilinear(const float* B, int br, V3 p) {
     return c0 * bz + c1 * az;
 }
 
+// trilinear<BR> for a sample strictly inside the brick's interior range
+// (0 <= p < res0 on every axis, as the brick marches guarantee): the cell
+// floor(p + 0.5) and its +1 neighbour are then inside the padded brick, so
+// the eight corners sit at one base index plus compile-time offsets and no
+// clamps are needed.  Same arithmetic as trilinear<BR>, so the same bits.
+// Returns the sample and its cell / fractions for the adjoint.
+template <int BR>
+__device__ __forceinline__ float trilinear_in(const float* B, V3 p, int& ix, int& iy, int& iz, float& ax, float& ay,
+                                              float& az) {
+    static_assert(BR > 0, "compile-time brick size");
+    const float cx = p.x + 0.5f, cy = p.y + 0.5f, cz = p.z + 0.5f;
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    ax = cx - fx, ay = cy - fy, az = cz - fz;
+    const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
+    ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    const float* b = B + (__mul24(iz, BR * BR) + __mul24(iy, BR) + ix);   // indices < 2^23
+    const float c00 = b[0] * bx + b[1] * ax;
+    const float c10 = b[BR] * bx + b[BR + 1] * ax;
+    const float c01 = b[BR * BR] * bx + b[BR * BR + 1] * ax;
+    const float c11 = b[BR * BR + BR] * bx + b[BR * BR + BR + 1] * ax;
+    const float c0 = c00 * by + c10 * ay;
+    const float c1 = c01 * by + c11 * ay;
+    return c0 * bz + c1 * az;
+}
+
 // transfer DDA.cuh:20-23 (float quotient, double clamp and scale, truncation)
 __device__ __forceinline__ int transfer_index(const RenderScene& sc, float s) {
     const float q = (s - sc.thresh) / (sc.vmax - sc.vmin);
@@ -204,7 +229,14 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
         for (int iter = 0; st.clr[3] > sc.alphacut && iter < sc.max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 &&
                            p.x < r0 && p.y < r0 && p.z < r0;
              iter++) {
-            const float s = trilinear<BR>(B, br, p);
+            float s;
+            if constexpr (MODE == MODE_DEEP_LINEAR && BR > 0) {
+                int ix, iy, iz;
+                float ax, ay, az;
+                s = trilinear_in<BR>(B, p, ix, iy, iz, ax, ay, az);
+            } else {
+                s = trilinear<BR>(B, br, p);
+            }
             if (s >= sc.minval) {
                 float4 val;
                 if (MODE == MODE_DEEP_LUT)
@@ -577,10 +609,9 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
 #pragma unroll
             for (int m = 0; m < 8; m++) {
                 if (((m >> axis_bit) & 1) == mask_keep) continue;
-                const int xi = min(max(cx + (m & 1), 0), BR - 1), yi = min(max(cy + ((m >> 1) & 1), 0), BR - 1),
-                          zi = min(max(cz + (m >> 2), 0), BR - 1);
                 if (acc[m] != 0.0f) {
-                    const int vi = (zi * BR + yi) * BR + xi;
+                    // the cell is inside the padded brick (samples stay in the interior range)
+                    const int vi = __mul24(cz + (m >> 2), BR * BR) + __mul24(cy + ((m >> 1) & 1), BR) + (cx + (m & 1));
                     // relaxed atomics: plain LDS stores and loads the compiler may not fold
                     __hip_atomic_store(&claim[vi], tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (__hip_atomic_load(&claim[vi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == tid)
@@ -594,23 +625,11 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
         for (int iter = 0; T > sc.alphacut && iter < sc.max_iter && q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < r0 &&
                            q.y < r0 && q.z < r0;
              iter++) {
-            // trilinear sample (as trilinear<BR>) sharing its cell and weights with the adjoint
-            const float fx = floorf(q.x + 0.5f), fy = floorf(q.y + 0.5f), fz = floorf(q.z + 0.5f);
-            const float ax = (q.x + 0.5f) - fx, ay = (q.y + 0.5f) - fy, az = (q.z + 0.5f) - fz;
+            // trilinear sample (the forward's value) sharing its cell and weights with the adjoint
+            int ix, iy, iz;
+            float ax, ay, az;
+            const float s = trilinear_in<BR>(dens, q, ix, iy, iz, ax, ay, az);
             const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
-            const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-            const int x0 = min(max(ix, 0), BR - 1), x1 = min(max(ix + 1, 0), BR - 1);
-            const int y0 = min(max(iy, 0), BR - 1), y1 = min(max(iy + 1, 0), BR - 1);
-            const int z0 = min(max(iz, 0), BR - 1), z1 = min(max(iz + 1, 0), BR - 1);
-            const float* r00 = dens + (z0 * BR + y0) * BR;
-            const float* r10 = dens + (z0 * BR + y1) * BR;
-            const float* r01 = dens + (z1 * BR + y0) * BR;
-            const float* r11 = dens + (z1 * BR + y1) * BR;
-            const float c00 = r00[x0] * bx + r00[x1] * ax;
-            const float c10 = r10[x0] * bx + r10[x1] * ax;
-            const float c01 = r01[x0] * bx + r01[x1] * ax;
-            const float c11 = r11[x0] * bx + r11[x1] * ax;
-            const float s = (c00 * by + c10 * ay) * bz + (c01 * by + c11 * ay) * az;
             if (s >= sc.minval) {
                 const float a = expf(sc.extinct * s * step);
                 I += s * T * (1 - a) * sc.albedo;
