@@ -538,10 +538,10 @@ __device__ __forceinline__ void shift_corners(float* acc, bool up) {
 // checkpoint and accumulate dL/dvoxel = sum_k q_k * trilinear weight, with
 //   e = extinct*step, a_k = exp(extinct*s_k*step),
 //   q_k = gI * (albedo T_k ((1-a_k) - e s_k a_k) + e (I_final - I_k)) + gT e T_final
-// (I_k: intensity after sample k).  The leaf's segments (sorted by ray) are
-// cut into four contiguous quarters, one per wave, and each lane takes a
-// contiguous run of its quarter.  Each wave adds into its own LDS copy; the
-// four copies are summed in a fixed order at the end.
+// (I_k: intensity after sample k).  The leaf's segments (sorted by ray) go
+// to the four waves by the parity class of their pixel, one segment per lane
+// at a time.  Each wave adds into its own LDS copy; the four copies are
+// summed in a fixed order at the end.
 // A lane keeps the eight corner sums of its current trilinear cell in
 // registers and flushes a corner when the ray leaves it.  A flush first
 // claims the voxel: the lane writes its tag to claim[voxel] and reads it
@@ -559,6 +559,7 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     __shared__ float dens[NV];
     __shared__ float grad[NW][NV];
     __shared__ int claim[NV];
+    __shared__ int32_t wlist[NW][128];   // per wave: its segments waiting for a lane
     const int64_t leaf = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float* src = p.tv.chan + leaf * (int64_t)NV;
@@ -570,21 +571,16 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     }
     __syncthreads();
     const int32_t b0 = p.seg[2 * leaf], b1 = p.seg[2 * leaf + 1];
-    const int32_t nseg = b1 - b0;
-    const int32_t per_wave = (nseg + NW - 1) / NW;
-    const int32_t w0 = b0 + wave * per_wave, w1 = min(b1, w0 + per_wave);
-    const int32_t per_lane = (per_wave + 63) / 64;
-    const int32_t c0 = w0 + lane * per_lane, c1 = min(w1, c0 + per_lane);
     float* g = grad[wave];
     const RenderScene& sc = p.sc;
     const float r0 = (float)p.tv.res[0];
     const float e = sc.extinct * sc.step;
     const uint64_t rmask = (1ull << p.ray_bits) - 1;
-    for (int32_t c = c0; c < c1; c++) {
+    const uint32_t npix = (uint32_t)p.cam.width * (uint32_t)p.cam.height;
+    auto replay = [&](int32_t c) {
         const uint32_t id = p.idx[c];
         const uint32_t ray = (uint32_t)(p.key[c] & rmask);
         const float4 cp = p.rec[id];
-        const uint32_t npix = (uint32_t)p.cam.width * (uint32_t)p.cam.height;
         const RenderCam& cam = p.cams ? p.cams[ray / npix] : p.cam;
         const uint32_t pix = ray % npix;
         const V3 vmin = node_pos(p.tv.nodes[0], leaf, cam.shift);
@@ -662,7 +658,34 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
             q = add(q, wpt);
         }
         if (have) flush(-1, 0);
+    };
+    // Wave w replays the leaf's segments whose pixel has parity class
+    // (x & 1) | (y & 1) << 1 == w, in segment order, 64 at a time: rays of
+    // one wave are then at least two pixels apart and rarely flush the same
+    // voxel in one instruction.
+    int cnt = 0;
+    int32_t* wl = wlist[wave];
+    for (int32_t j0 = b0; j0 < b1; j0 += 64) {
+        const int32_t c = j0 + lane;
+        bool mine = false;
+        if (c < b1) {
+            const uint32_t pix = (uint32_t)(p.key[c] & rmask) % npix;
+            mine = (int)(((pix % (uint32_t)p.cam.width) & 1u) | (((pix / (uint32_t)p.cam.width) & 1u) << 1)) == wave;
+        }
+        const uint64_t mb = __ballot(mine);
+        if (mine) wl[cnt + (int)__popcll(mb & ((1ull << lane) - 1ull))] = c;
+        cnt += (int)__popcll(mb);
+        if (cnt >= 64) {
+            __builtin_amdgcn_wave_barrier();
+            replay(wl[lane]);
+            const int32_t rest = lane < cnt - 64 ? wl[64 + lane] : 0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < cnt - 64) wl[lane] = rest;
+            cnt -= 64;
+        }
     }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < cnt) replay(wl[lane]);
     __syncthreads();
     float* dst = p.grad + leaf * (int64_t)NV;
     for (int i = tid; i < NV; i += 256) dst[i] = ((grad[0][i] + grad[1][i]) + grad[2][i]) + grad[3][i];
