@@ -76,6 +76,8 @@ struct gvdbx_volume {
     int depth = 0, top_lev = 1;
     int root_pos[3] = {0, 0, 0};
     float pos_min[3] = {}, pos_max[3] = {}, obj_min[3] = {}, obj_max[3] = {};
+    bool bounds_pending = false;   // obj_min/max are in flight to pin + PIN_BOUNDS
+    void* pin = nullptr;           // small pinned buffer of its own (root node upload, bounds readback)
     int64_t cnt[GVDBX_MAXLEV] = {};
     int64_t nbase[GVDBX_MAXLEV] = {};
     CellGrid grid[GVDBX_MAXLEV] = {};
@@ -165,6 +167,37 @@ int ensure(gvdbx_volume* v, DevBuf& b, size_t bytes) {
         FAIL(GVDBX_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
     }
     b.cap = want;
+    return GVDBX_OK;
+}
+
+// Layout of gvdbx_volume::pin: the root node staged for upload, then the six
+// leaf-bound integers read back by FinishTopology.  Neither is ever written
+// while a copy of it may be in flight (each RebuildTopology and each bounds
+// resolution synchronises first).
+constexpr size_t PIN_ROOT = 0, PIN_BOUNDS = 128, PIN_BYTES = 256;
+
+int ensure_pin(gvdbx_volume* v) {
+    if (v->pin) return GVDBX_OK;
+    hipError_t e = hipHostMalloc(&v->pin, PIN_BYTES, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        v->pin = nullptr;
+        FAIL(GVDBX_ERR_NOMEM, "hipHostMalloc(%zu) failed: %s", PIN_BYTES, hipGetErrorString(e));
+    }
+    return GVDBX_OK;
+}
+
+// Bounds computed by FinishTopology come back asynchronously; the first
+// reader waits for them.
+int resolve_bounds(gvdbx_volume* v) {
+    if (!v->bounds_pending) return GVDBX_OK;
+    TRY(sync(v));
+    int32_t out[6];
+    memcpy(out, (const char*)v->pin + PIN_BOUNDS, sizeof(out));
+    for (int a = 0; a < 3; a++) {
+        v->obj_min[a] = (float)out[a];
+        v->obj_max[a] = (float)out[3 + a];
+    }
+    v->bounds_pending = false;
     return GVDBX_OK;
 }
 
@@ -622,8 +655,9 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     }
     rn.mParent = GVDBX_ID_UNDEFL;
     rn.mChildList = 1ull | ((uint64_t)depth << 8);
-    memcpy((char*)v->host + 1024, &rn, sizeof(rn));
-    GVDBX_CHECK(hipMemcpyAsync(v->nodes[depth].p, (char*)v->host + 1024, sizeof(rn), hipMemcpyHostToDevice, s));
+    TRY(ensure_pin(v));
+    memcpy((char*)v->pin + PIN_ROOT, &rn, sizeof(rn));
+    GVDBX_CHECK(hipMemcpyAsync(v->nodes[depth].p, (char*)v->pin + PIN_ROOT, sizeof(rn), hipMemcpyHostToDevice, s));
     // top level (PrepareVDBPartially VG.cpp:3996-4012)
     int tlev = 1;
     for (int l = v->levs - 1; l >= 0; l--)
@@ -659,7 +693,6 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
                            v->c_leafcloud.as<int32_t>(), s);
     }
     TRY(check_launch("node pools"));
-    TRY(sync(v));
 
     v->depth = depth;
     v->top_lev = tlev;
@@ -682,20 +715,15 @@ int compute_bounds(gvdbx_volume* v) {
     // ComputeBounds VG.cpp:1792-1816
     hipStream_t s = v->stream;
     TRY(ensure(v, v->misc, 256));
-    TRY(ensure_host(v, 4096));
-    int32_t init[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, INT32_MIN, INT32_MIN, INT32_MIN};
-    memcpy((char*)v->host + 2048, init, sizeof(init));
-    GVDBX_CHECK(hipMemcpyAsync(v->misc.p, (char*)v->host + 2048, sizeof(init), hipMemcpyHostToDevice, s));
+    TRY(ensure_pin(v));
+    // an unread earlier readback into the same slot is superseded: the host
+    // never writes the slot, and stream order lands this copy last
+    launch_fill_i32(v->misc.as<int32_t>(), 3, 0x7FFFFFFF, s);
+    launch_fill_i32(v->misc.as<int32_t>() + 3, 3, INT32_MIN, s);
     launch_leaf_bounds(v->nodes[0].as<GNode>(), v->cnt[0], v->res[0], v->misc.as<int32_t>(), s);
     TRY(check_launch("leaf bounds"));
-    GVDBX_CHECK(hipMemcpyAsync((char*)v->host + 2048, v->misc.p, sizeof(init), hipMemcpyDeviceToHost, s));
-    TRY(sync(v));
-    int32_t out[6];
-    memcpy(out, (char*)v->host + 2048, sizeof(out));
-    for (int a = 0; a < 3; a++) {
-        v->obj_min[a] = (float)out[a];
-        v->obj_max[a] = (float)out[3 + a];
-    }
+    GVDBX_CHECK(hipMemcpyAsync((char*)v->pin + PIN_BOUNDS, v->misc.p, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    v->bounds_pending = true;
     if (v->ncloud > 0) {
         // ComputeBounds of every cloud, in its own coordinates
         const int nc = v->ncloud;
@@ -826,6 +854,7 @@ int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_
     }
     tv.levs = v->levs;
     tv.top_lev = v->top_lev;
+    TRY(resolve_bounds(v));
     memcpy(tv.bmin, v->obj_min, sizeof(tv.bmin));
     memcpy(tv.bmax, v->obj_max, sizeof(tv.bmax));
     tv.chan = v->chan[chan].as<float>();
@@ -903,9 +932,13 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->bin_off, &v->sorted, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
-                      &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt};
+                      &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
+                      &v->splat_src, &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
+                      &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
+                      &v->c_bbox, &v->c_bounds, &v->c_cams};
     for (DevBuf* b : bufs) free_buf(*b);
     if (v->host) (void)hipHostFree(v->host);
+    if (v->pin) (void)hipHostFree(v->pin);
     delete v;
     return GVDBX_OK;
 }
@@ -1577,6 +1610,7 @@ int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out) {
     out->apron = std::max(v->apron, 0);
     memcpy(out->pos_min, v->pos_min, sizeof(out->pos_min));
     memcpy(out->pos_max, v->pos_max, sizeof(out->pos_max));
+    TRY(resolve_bounds(v));
     memcpy(out->obj_min, v->obj_min, sizeof(out->obj_min));
     memcpy(out->obj_max, v->obj_max, sizeof(out->obj_max));
     out->sc_len = v->sc_len;
