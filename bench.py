@@ -65,6 +65,17 @@ def frame_bytes_alg(n, nleaf, node_cnt, res, depth, br=10, r0=8, passes=3):
     return b
 
 
+def event_pool(torch, stream, n):
+    """n timing events, each recorded once so that its device event exists
+    (torch creates it on first record): creating events inside the timed
+    loop would add host time at points where the device waits on the host."""
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    for e in evs:
+        e.record(stream)
+    torch.cuda.synchronize()
+    return evs
+
+
 def measured_value(kernel_profile, key):
     """A `key: value` line of a committed rocprofv3 counter summary, or None."""
     path = os.path.join(ROOT, "profiles", kernel_profile)
@@ -120,10 +131,11 @@ def bench_differentiable(torch, gvdbx, dev, steps, warmup):
     rp = gvdbx.ea_params(step=0.5, kappa=0.05)
     radius, topo_r = 2.0, 4.0
     ev = {}
+    pool = event_pool(torch, stream, 8 * (steps + 1))
 
     def mark(name, on):
         if on:
-            e = torch.cuda.Event(enable_timing=True)
+            e = pool.pop()
             e.record(stream)
             ev.setdefault(name, []).append(e)
 
@@ -416,9 +428,10 @@ def main():
     torch.cuda.synchronize()
 
     ev = {}
+    pool = event_pool(torch, stream, 5 * (args.steps + 1))
 
     def mark(name):
-        e = torch.cuda.Event(enable_timing=True)
+        e = pool.pop()
         e.record(stream)
         ev.setdefault(name, []).append(e)
 
