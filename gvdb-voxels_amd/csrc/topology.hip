@@ -466,32 +466,44 @@ __global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, cons
 
 // Subcell list length of InsertPointsSubcell (gvdbCountSubcell,
 // cuda_gvdb_particles.cuh:275-315): per point, the subcells of its box that
-// lie in a leaf; summed per block, one atomic per block.
+// lie in a leaf; summed per block, one atomic per block.  The subcell size
+// divides the brick, so along each axis the box's subcells (those >= 0) fall
+// into at most a few level-0 cells; the count is the sum over the box's
+// cells that are leaves of the product of their per-axis subcell counts (one
+// leaf test per cell instead of per subcell).
+__device__ __forceinline__ int subcells_in_cell(int lo, int hi, int c, int r0, int sc) {
+    // subcells x = lo, lo+sc, ..., hi (multiples of sc) with x >= 0 inside cell c
+    const int a = max(max(lo, 0), c * r0), b = min(hi, c * r0 + r0 - sc);
+    return b >= a ? (b - a) / sc + 1 : 0;
+}
+
 __global__ __launch_bounds__(256) void k_subcell_len(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
                                                      BinParams bp, unsigned long long* len) {
     const int64_t total = min((int64_t)*total_ptr, nmax);
+    const int r0 = 1 << bp.logr0, sc = bp.sc;
     unsigned long long lsum = 0;
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
         float4 p = sorted[j];
         float w[3] = {p.x, p.y, p.z};
-        int lo[3], hi[3];
+        int lo[3], hi[3], c0[3], c1[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * bp.sc;
-            hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * bp.sc;
+            lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * sc;
+            hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * sc;
+            c0[a] = max(lo[a], 0) >> bp.logr0;
+            c1[a] = hi[a] >> bp.logr0;   // hi < 0: empty range (c1 < c0 or no subcell counted)
         }
-        for (int x = lo[0]; x <= hi[0]; x += bp.sc) {
-            if (x < 0) continue;
-            int cx = x >> bp.logr0;
-            for (int y = lo[1]; y <= hi[1]; y += bp.sc) {
-                if (y < 0) continue;
-                int cy = y >> bp.logr0;
-                for (int z = lo[2]; z <= hi[2]; z += bp.sc) {
-                    if (z < 0) continue;
-                    int cz = z >> bp.logr0;
-                    if (!bp.g0.contains(cx, cy, cz)) continue;
-                    int64_t ci = bp.g0.index(cx, cy, cz);
-                    if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum++;
+        for (int cx = c0[0]; cx <= c1[0]; cx++) {
+            const int nx = subcells_in_cell(lo[0], hi[0], cx, r0, sc);
+            if (nx == 0) continue;
+            for (int cy = c0[1]; cy <= c1[1]; cy++) {
+                const int ny = subcells_in_cell(lo[1], hi[1], cy, r0, sc);
+                if (ny == 0) continue;
+                for (int cz = c0[2]; cz <= c1[2]; cz++) {
+                    const int nz = subcells_in_cell(lo[2], hi[2], cz, r0, sc);
+                    if (nz == 0 || !bp.g0.contains(cx, cy, cz)) continue;
+                    const int64_t ci = bp.g0.index(cx, cy, cz);
+                    if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum += (unsigned long long)(nx * ny * nz);
                 }
             }
         }
