@@ -148,6 +148,51 @@ public:
         check(gvdbx_update_apron(v_, chan, boundval), "UpdateApron");
     }
 
+    // ---- rendering (Render VG.cpp / rayCast RC.cuh with an explicit camera) ----
+    void Render(int shade, unsigned char chan, const gvdbx_camera& cam, const gvdbx_render_params& rp,
+                float* rgba_dev, uint8_t* rgba8_dev = nullptr) {
+        check(gvdbx_render(v_, shade, chan, &cam, &rp, rgba_dev, rgba8_dev), "Render");
+    }
+
+    // ---- differentiable forms (no reference counterpart) ----
+    void RaycastEA(unsigned char chan, const gvdbx_camera& cam, const gvdbx_render_params& rp, float* I_dev,
+                   float* T_dev, bool record = false) {
+        check(gvdbx_raycast_ea(v_, chan, &cam, &rp, I_dev, T_dev, record ? 1 : 0), "RaycastEA");
+    }
+    void RaycastEABackward(unsigned char chan, const gvdbx_camera& cam, const gvdbx_render_params& rp,
+                           const float* I_dev, const float* T_dev, const float* gI_dev, const float* gT_dev,
+                           float* grad_dev) {
+        check(gvdbx_raycast_ea_backward(v_, chan, &cam, &rp, I_dev, T_dev, gI_dev, gT_dev, grad_dev),
+              "RaycastEABackward");
+    }
+    void ApronAdjoint(float* grad_dev) { check(gvdbx_apron_adjoint(v_, grad_dev), "ApronAdjoint"); }
+    void SplatCubicBackward(int num_pnts, float radius, Vector3DF trans, const float* grad_dev, float* dpos_dev,
+                            float* dw_dev = nullptr) {
+        check(gvdbx_splat_cubic_backward(v_, num_pnts, radius, trans.x, trans.y, trans.z, grad_dev, dpos_dev, dw_dev),
+              "SplatCubicBackward");
+    }
+
+    // ---- batched clouds (no reference counterpart) ----
+    void SetPointsBatched(const void* pos_dev, int stride, const std::vector<int64_t>& cloud_off) {
+        check(gvdbx_set_points_batched(v_, pos_dev, stride, cloud_off.data(), (int)cloud_off.size() - 1),
+              "SetPointsBatched");
+    }
+    void RaycastEABatched(unsigned char chan, const std::vector<gvdbx_camera>& cams, const gvdbx_render_params& rp,
+                          float* I_dev, float* T_dev, bool record = false) {
+        check(gvdbx_raycast_ea_batched(v_, chan, cams.data(), (int)cams.size(), &rp, I_dev, T_dev, record ? 1 : 0),
+              "RaycastEABatched");
+    }
+    void RaycastEABackwardBatched(unsigned char chan, const std::vector<gvdbx_camera>& cams,
+                                  const gvdbx_render_params& rp, const float* I_dev, const float* T_dev,
+                                  const float* gI_dev, const float* gT_dev, float* grad_dev) {
+        check(gvdbx_raycast_ea_backward_batched(v_, chan, cams.data(), (int)cams.size(), &rp, I_dev, T_dev, gI_dev,
+                                                gT_dev, grad_dev),
+              "RaycastEABackwardBatched");
+    }
+
+    // ---- kernel tuning (no reference counterpart) ----
+    void SetTuning(const char* key, int value) { check(gvdbx_set_tuning(v_, key, value), "SetTuning"); }
+
     gvdbx_info Info() const {
         gvdbx_info inf;
         check(gvdbx_get_info(v_, &inf), "GetInfo");

@@ -90,6 +90,19 @@ int main(int argc, char**) {
         gvdb.InsertPointsSubcell(4, 10, 3.0f, gvdbx::Vector3DF(0, 0, 0), len);
         gvdb.GatherLevelSet(4, 10, 1.5f, gvdbx::Vector3DF(0, 0, 0), len, 0, 1);
         gvdb.UpdateApron(0, 3.0f);
+        gvdbx_camera cam{};
+        gvdbx_render_params rp{};
+        gvdb.Render(GVDBX_SHADE_VOLUME, 0, cam, rp, nullptr, nullptr);
+        gvdb.SetTuning("splat_chunk", 1024);
+        gvdb.SplatCubic(10, 2.0f, gvdbx::Vector3DF(0, 0, 0));
+        gvdb.RaycastEA(0, cam, rp, nullptr, nullptr, true);
+        gvdb.RaycastEABackward(0, cam, rp, nullptr, nullptr, nullptr, nullptr, nullptr);
+        gvdb.ApronAdjoint(nullptr);
+        gvdb.SplatCubicBackward(10, 2.0f, gvdbx::Vector3DF(0, 0, 0), nullptr, nullptr);
+        gvdb.SetPointsBatched(nullptr, 12, std::vector<int64_t>{0, 5, 10});
+        std::vector<gvdbx_camera> cams(2);
+        gvdb.RaycastEABatched(0, cams, rp, nullptr, nullptr, true);
+        gvdb.RaycastEABackwardBatched(0, cams, rp, nullptr, nullptr, nullptr, nullptr, nullptr);
     }
     std::printf("%s\n", gvdbx_version());
     return 0;
