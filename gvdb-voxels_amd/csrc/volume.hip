@@ -246,6 +246,10 @@ bool any_chan(const gvdbx_volume* v) {
 // Grow every channel to hold `bricks` bricks, preserving contents (AtlasResize AL.cpp:622-650).
 int grow_atlas(gvdbx_volume* v, int64_t bricks) {
     if (bricks <= v->atlas_cap) return GVDBX_OK;
+    // physical capacity grows by at least a quarter, so a sequence of
+    // slightly larger topologies reallocates (and copies) rarely; the
+    // logical atlas (subdim, node mValue) is what UpdateAtlas computes
+    if (v->atlas_cap > 0) bricks = std::max(bricks, v->atlas_cap + v->atlas_cap / 4);
     int64_t bv = brick_vox(v);
     TRY(sync(v));
     for (int c = 0; c < GVDBX_MAXCHAN; c++) {
