@@ -254,12 +254,20 @@ struct RenderScene {
     const float4* lut;
 };
 // per (ray, brick) segment entry state of a recorded EA raycast
+// Segment checkpoints of a recording EA raycast.  Each wave buffers its
+// checkpoints and writes them out in chunks; a chunk is described by the tag
+// (global wave index << 32 | the wave's chunk sequence number), its first
+// record and its size, so that the records can later be put in an order
+// that does not depend on which wave wrote first.
 struct VisitRec {
-    uint64_t* key;       // leaf << ray_bits | ray
-    float4* rec;         // (t handed to the brick function, T and I at entry, 0)
-    uint32_t* count;     // number of segments (may exceed cap: then rerun)
+    uint32_t* key;       // leaf of each record
+    float4* rec;         // (t handed to the brick function, T and I at entry, ray id bits)
+    uint32_t* count;     // count[0]: records, count[1]: chunks, one 8-byte-aligned pair (may exceed the caps: then rerun)
     int64_t cap;
-    int ray_bits;
+    uint64_t* chunk_tag;
+    uint32_t* chunk_base;
+    uint32_t* chunk_size;
+    int64_t chunk_cap;
 };
 void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba, uchar4* rgba8,
                    hipStream_t s);
@@ -273,18 +281,24 @@ struct EABackParams {
     RenderCam cam;
     const RenderCam* cams;  // batched: camera of ray r is cams[r / (width*height)]
     RenderScene sc;
-    const uint64_t* key;    // sorted segment keys
-    const uint32_t* idx;    // segment record index of each sorted key
-    const float4* rec;
-    const int32_t* seg;     // per leaf [begin, end) into the sorted keys, 2 per leaf
+    const uint32_t* idx;    // record index of each segment, grouped by leaf in a fixed order
+    const float4* rec;      // (t, T, I, ray id bits)
+    const int32_t* seg;     // per leaf [begin, end) into idx, 2 per leaf
     const float* I;
     const float* T;
     const float* gI;
     const float* gT;        // may be null
     float* grad;            // padded bricks
-    int ray_bits;
 };
-void launch_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf, hipStream_t s);
+// per leaf [begin, end) of the leaf-sorted keys
+void launch_segments(const uint32_t* key, int64_t n, int32_t* seg, int64_t nleaf, hipStream_t s);
+// chunk i of the tag-sorted order (original chunk order[i]) is copied to
+// dst_off[i]: key_out = its records' leaves, val_out = their record indices
+void launch_chunk_order(const uint32_t* order, int64_t nchunk, const uint32_t* chunk_base, const uint32_t* chunk_size,
+                        const int32_t* dst_off, const uint32_t* key, uint32_t* key_out, uint32_t* val_out,
+                        hipStream_t s);
+// out[i] = size[order[i]]
+void launch_gather_sizes(const uint32_t* order, int64_t n, const uint32_t* size, int32_t* out, hipStream_t s);
 void launch_ea_backward(EABackParams p, hipStream_t s);
 
 struct SplatBackParams {

@@ -285,26 +285,43 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
     }
 }
 
-// Segment checkpoints are buffered per wave in LDS and written out VIS_BUF at
-// a time, so a wave takes one slot range (one global atomic) per VIS_BUF
-// records instead of one per march step.  Slot order is irrelevant: the
-// backward sorts checkpoints by their (leaf, ray) key, which is unique.
-constexpr int VIS_BUF = 128;
+// Segment checkpoints are buffered per wave in LDS and written out as chunks
+// of up to VIS_BUF, so a wave takes one slot range (one global atomic) per
+// chunk instead of one per march step.  Where a chunk lands depends on which
+// wave flushes first, so each chunk is also described by a tag (global wave
+// index, the wave's chunk number): the backward orders chunks by tag, which
+// makes the order of the records, and of every sum over them, fixed.
+constexpr int VIS_BUF = 448;   // ~9 KB per wave, 4 waves per block: 4 blocks fit a CU, as many as the registers allow
 struct VisitLds {
-    uint64_t key[VIS_BUF];
+    uint32_t key[VIS_BUF];
     float4 rec[VIS_BUF];
     int cnt;
+    int seq;       // chunks written so far
+    uint32_t wave; // global wave index (chunk tags)
 };
 
-// Write the cnt buffered checkpoints with the lanes in `act` (any subset of
-// the wave).
+// Write the cnt buffered checkpoints as one chunk, with the lanes in `act`
+// (any subset of the wave).
 __device__ __forceinline__ void flush_visits(VisitLds& vl, const VisitRec& vr, int cnt, uint64_t act) {
     const int lane = __lane_id();
     const int leader = __ffsll((unsigned long long)act) - 1;
     const int n = __popcll(act);
     const int rank = __popcll(act & ((1ull << lane) - 1));
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(vr.count, (uint32_t)cnt);
+    if (lane == leader) {
+        // one atomic takes the record range (low word) and the chunk slot (high word)
+        const unsigned long long old =
+            atomicAdd((unsigned long long*)vr.count, (unsigned long long)cnt | (1ull << 32));
+        base = (uint32_t)old;
+        const uint32_t cid = (uint32_t)(old >> 32);
+        const int seq = __hip_atomic_load(&vl.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        if ((int64_t)cid < vr.chunk_cap) {
+            vr.chunk_tag[cid] = ((uint64_t)vl.wave << 32) | (uint32_t)seq;
+            vr.chunk_base[cid] = base;
+            vr.chunk_size[cid] = (uint32_t)cnt;
+        }
+        __hip_atomic_store(&vl.seq, seq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
     base = __builtin_amdgcn_readlane(base, leader);
     __builtin_amdgcn_wave_barrier();
     for (int i = rank; i < cnt; i += n) {
@@ -337,8 +354,8 @@ __device__ __forceinline__ void record_visit(VisitLds& vl, const VisitRec& vr, i
         flush_visits(vl, vr, cnt, act);
         cnt = 0;
     }
-    vl.key[cnt + rank] = ((uint64_t)leaf << vr.ray_bits) | ray;
-    vl.rec[cnt + rank] = make_float4(t, T, I, 0.0f);
+    vl.key[cnt + rank] = (uint32_t)leaf;
+    vl.rec[cnt + rank] = make_float4(t, T, I, __uint_as_float(ray));
     __builtin_amdgcn_wave_barrier();
     if (rank == 0) set_visit_count(vl, cnt + n);
 }
@@ -490,7 +507,12 @@ __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam
                                                     RenderScene sc, float* I, float* T, VisitRec vr) {
     __shared__ VisitLds vbuf[RECORD ? 4 : 1];
     VisitLds& vl = vbuf[RECORD ? threadIdx.x / 64 : 0];
-    if (RECORD && __lane_id() == 0) set_visit_count(vl, 0);
+    if (RECORD && __lane_id() == 0) {
+        set_visit_count(vl, 0);
+        __hip_atomic_store(&vl.seq, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        vl.wave = (uint32_t)((((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 4 +
+                             threadIdx.x / 64);
+    }
     const RenderCam& cam = cams ? cams[blockIdx.z] : cam0;
     int x, y;
     const bool ok = pixel_of(cam0, x, y);
@@ -512,13 +534,34 @@ __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam
 }
 
 // [begin, end) of each leaf's run in the sorted segment keys
-__global__ void k_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf) {
+__global__ void k_segments(const uint32_t* key, int64_t n, int32_t* seg, int64_t nleaf) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t l = key[i] >> ray_bits;
+        const uint32_t l = key[i];
         if (l >= (uint64_t)nleaf) continue;
-        if (i == 0 || (key[i - 1] >> ray_bits) != l) seg[2 * l] = (int32_t)i;
-        if (i == n - 1 || (key[i + 1] >> ray_bits) != l) seg[2 * l + 1] = (int32_t)(i + 1);
+        if (i == 0 || key[i - 1] != l) seg[2 * l] = (int32_t)i;
+        if (i == n - 1 || key[i + 1] != l) seg[2 * l + 1] = (int32_t)(i + 1);
     }
+}
+
+// One wave per chunk, in tag order: its records go to consecutive places.
+__global__ __launch_bounds__(64) void k_chunk_order(const uint32_t* order, int64_t nchunk, const uint32_t* chunk_base,
+                                                    const uint32_t* chunk_size, const int32_t* dst_off,
+                                                    const uint32_t* key, uint32_t* key_out, uint32_t* val_out) {
+    const int64_t i = blockIdx.x;
+    if (i >= nchunk) return;
+    const uint32_t c = order[i];
+    const uint32_t base = chunk_base[c], size = chunk_size[c];
+    const int64_t dst = dst_off[i];
+    for (uint32_t j = threadIdx.x; j < size; j += 64) {
+        key_out[dst + j] = key[base + j];
+        val_out[dst + j] = base + j;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_sizes(const uint32_t* order, int64_t n, const uint32_t* size,
+                                                      int32_t* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (int32_t)size[order[i]];
 }
 
 // Move the four corner sums a cell shares with its neighbour along axis A:
@@ -575,12 +618,11 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     const RenderScene& sc = p.sc;
     const float r0 = (float)p.tv.res[0];
     const float e = sc.extinct * sc.step;
-    const uint64_t rmask = (1ull << p.ray_bits) - 1;
     const uint32_t npix = (uint32_t)p.cam.width * (uint32_t)p.cam.height;
     auto replay = [&](int32_t c) {
         const uint32_t id = p.idx[c];
-        const uint32_t ray = (uint32_t)(p.key[c] & rmask);
         const float4 cp = p.rec[id];
+        const uint32_t ray = __float_as_uint(cp.w);
         const RenderCam& cam = p.cams ? p.cams[ray / npix] : p.cam;
         const uint32_t pix = ray % npix;
         const V3 vmin = node_pos(p.tv.nodes[0], leaf, cam.shift);
@@ -669,7 +711,7 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
         const int32_t c = j0 + lane;
         bool mine = false;
         if (c < b1) {
-            const uint32_t pix = (uint32_t)(p.key[c] & rmask) % npix;
+            const uint32_t pix = __float_as_uint(p.rec[p.idx[c]].w) % npix;
             mine = (int)(((pix % (uint32_t)p.cam.width) & 1u) | (((pix / (uint32_t)p.cam.width) & 1u) << 1)) == wave;
         }
         const uint64_t mb = __ballot(mine);
@@ -833,8 +875,20 @@ void launch_raycast_ea(RenderTree tv, RenderCam cam, const RenderCam* cams, int 
     }
 }
 
-void launch_segments(const uint64_t* key, int64_t n, int ray_bits, int32_t* seg, int64_t nleaf, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, ray_bits, seg, nleaf);
+void launch_segments(const uint32_t* key, int64_t n, int32_t* seg, int64_t nleaf, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, seg, nleaf);
+}
+
+void launch_chunk_order(const uint32_t* order, int64_t nchunk, const uint32_t* chunk_base, const uint32_t* chunk_size,
+                        const int32_t* dst_off, const uint32_t* key, uint32_t* key_out, uint32_t* val_out,
+                        hipStream_t s) {
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_chunk_order, dim3((unsigned)nchunk), dim3(64), 0, s, order, nchunk, chunk_base, chunk_size,
+                           dst_off, key, key_out, val_out);
+}
+
+void launch_gather_sizes(const uint32_t* order, int64_t n, const uint32_t* size, int32_t* out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_sizes, dim3(grid_for(n, 256)), dim3(256), 0, s, order, n, size, out);
 }
 
 void launch_ea_backward(EABackParams p, hipStream_t s) {

@@ -114,8 +114,11 @@ struct gvdbx_volume {
     int64_t npnode = 0;
     // recorded emission-absorption segments (gvdbx_raycast_ea with record)
     DevBuf vis_key, vis_rec, vis_idx, vis_tkey, vis_tidx, vis_hist, vis_off, vis_part, vis_seg, vis_cnt;
+    DevBuf vis_lkey, vis_ctag, vis_ctag2, vis_cbase, vis_csize, vis_cidx, vis_cidx2, vis_coff;   // checkpoint chunks
+    int64_t vis_nchunk = 0, vis_ccap = 0;
+    int vis_wave_bits = 0;
     int64_t vis_n = 0, vis_cap = 0;
-    int vis_ray_bits = 0, vis_leaf_bits = 0;
+    int vis_leaf_bits = 0;
     bool vis_valid = false, vis_sorted = false, vis_in_tmp = false;
     int vis_w = 0, vis_h = 0, vis_chan = -1, vis_ncam = 0;
     uint64_t vis_topo_gen = 0;
@@ -939,7 +942,8 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
                       &v->splat_src, &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
-                      &v->c_bbox, &v->c_bounds, &v->c_cams};
+                      &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
+                      &v->vis_cbase, &v->vis_csize, &v->vis_cidx, &v->vis_cidx2, &v->vis_coff};
     for (DevBuf* b : bufs) free_buf(*b);
     if (v->host) (void)hipHostFree(v->host);
     if (v->pin) (void)hipHostFree(v->pin);
@@ -1405,31 +1409,44 @@ int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bo
     }
     const int64_t rays = (int64_t)rc.width * rc.height * (batched ? ncam : 1);
     if (rays >= ((int64_t)1 << 32)) FAIL(GVDBX_ERR_RANGE, "%s: too many rays", what);
-    const int rb = bits_for(rays), lb = bits_for(v->cnt[0]);
-    if (rb + lb > 63) FAIL(GVDBX_ERR_RANGE, "%s: too many rays x leaves for 64-bit segment keys", what);
+    const int lb = bits_for(v->cnt[0]);
+    // one checkpoint chunk per flush of a wave's buffer: every chunk but a
+    // wave's last holds more than VIS_BUF/2 = 64 records
+    const int64_t nwaves = (int64_t)((rc.width + 15) / 16) * ((rc.height + 15) / 16) * (batched ? ncam : 1) * 4;
+    if (nwaves >= ((int64_t)1 << 32)) FAIL(GVDBX_ERR_RANGE, "%s: too many rays", what);
     v->vis_valid = false;
     int64_t cap = std::max<int64_t>(v->vis_cap, std::max<int64_t>(rays * 8, 1 << 20));
     TRY(ensure(v, v->vis_cnt, 16));
     TRY(ensure_host(v, 4096));
     for (int attempt = 0; attempt < 2; attempt++) {
-        TRY(ensure(v, v->vis_key, (size_t)cap * 8));
+        if (cap >= ((int64_t)1 << 31)) FAIL(GVDBX_ERR_RANGE, "%s: more than 2^31 segment checkpoints", what);
+        const int64_t ccap = cap / 64 + nwaves + 16;
+        TRY(ensure(v, v->vis_key, (size_t)cap * 4));
         TRY(ensure(v, v->vis_rec, (size_t)cap * 16));
-        launch_fill_i32(v->vis_cnt.as<int32_t>(), 1, 0, s);
-        vr.key = v->vis_key.as<uint64_t>();
+        TRY(ensure(v, v->vis_ctag, (size_t)ccap * 8));
+        TRY(ensure(v, v->vis_cbase, (size_t)ccap * 4));
+        TRY(ensure(v, v->vis_csize, (size_t)ccap * 4));
+        launch_fill_i32(v->vis_cnt.as<int32_t>(), 2, 0, s);
+        vr.key = v->vis_key.as<uint32_t>();
         vr.rec = v->vis_rec.as<float4>();
         vr.count = v->vis_cnt.as<uint32_t>();
         vr.cap = cap;
-        vr.ray_bits = rb;
+        vr.chunk_tag = v->vis_ctag.as<uint64_t>();
+        vr.chunk_base = v->vis_cbase.as<uint32_t>();
+        vr.chunk_size = v->vis_csize.as<uint32_t>();
+        vr.chunk_cap = ccap;
         launch_raycast_ea(tv, rc, dcams, ncam, sc, I_dev, T_dev, 1, vr, s);
         TRY(check_launch(what));
-        GVDBX_CHECK(hipMemcpyAsync(v->host, v->vis_cnt.p, 4, hipMemcpyDeviceToHost, s));
+        GVDBX_CHECK(hipMemcpyAsync(v->host, v->vis_cnt.p, 8, hipMemcpyDeviceToHost, s));
         TRY(sync(v));
-        uint32_t n = *(uint32_t*)v->host;
-        if ((int64_t)n <= cap) {
+        const uint32_t n = ((const uint32_t*)v->host)[0], m = ((const uint32_t*)v->host)[1];
+        if ((int64_t)n <= cap && (int64_t)m <= ccap) {
             v->vis_n = n;
+            v->vis_nchunk = m;
             v->vis_cap = cap;
-            v->vis_ray_bits = rb;
+            v->vis_ccap = ccap;
             v->vis_leaf_bits = lb;
+            v->vis_wave_bits = bits_for(nwaves);
             v->vis_w = rc.width;
             v->vis_h = rc.height;
             v->vis_ncam = batched ? ncam : 0;
@@ -1439,7 +1456,7 @@ int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bo
             v->vis_sorted = false;
             return GVDBX_OK;
         }
-        cap = (int64_t)n + (int64_t)n / 8 + 1024;   // the segment count of this view; rerun once
+        cap = std::max<int64_t>(cap, (int64_t)n + (int64_t)n / 8 + 1024);   // the segment count of this view; rerun once
     }
     FAIL(GVDBX_ERR_STATE, "%s: segment count did not settle", what);
 }
@@ -1467,23 +1484,48 @@ int raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int
     hipStream_t s = v->stream;
     const int64_t n = v->vis_n, nleaf = v->cnt[0];
     if (!v->vis_sorted) {
-        const int64_t nh = radix_hist_entries(n);
+        // Checkpoints grouped by leaf in an order fixed by the data: chunks
+        // ordered by their tag (wave, sequence), records in chunk order, then
+        // a stable sort by leaf.  Within a leaf the segments are then in
+        // (wave, chunk, position) order whichever wave wrote first.
+        const int64_t m = v->vis_nchunk;
+        const int64_t nh = radix_hist_entries(std::max(n, m));
+        TRY(ensure(v, v->vis_ctag2, (size_t)std::max<int64_t>(m, 1) * 8));
+        TRY(ensure(v, v->vis_cidx, (size_t)std::max<int64_t>(m, 1) * 4));
+        TRY(ensure(v, v->vis_cidx2, (size_t)std::max<int64_t>(m, 1) * 4));
+        TRY(ensure(v, v->vis_coff, (size_t)(m + 1) * 4 + 4 * m));
+        TRY(ensure(v, v->vis_lkey, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_idx, (size_t)std::max<int64_t>(n, 1) * 4));
-        TRY(ensure(v, v->vis_tkey, (size_t)std::max<int64_t>(n, 1) * 8));
+        TRY(ensure(v, v->vis_tkey, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_tidx, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_hist, (size_t)nh * 4));
         TRY(ensure(v, v->vis_off, (size_t)(nh + 1) * 4));
-        TRY(ensure(v, v->vis_part, (size_t)(scan_tiles(nh + 1) + 1) * 4));
-        launch_iota_u32(v->vis_idx.as<uint32_t>(), n, s);
-        int flip = radix_sort_pairs(v->vis_key.as<uint64_t>(), v->vis_idx.as<uint32_t>(), v->vis_tkey.as<uint64_t>(),
-                                    v->vis_tidx.as<uint32_t>(), n, 0, v->vis_ray_bits + v->vis_leaf_bits,
-                                    v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(), v->vis_part.as<uint32_t>(), s);
-        TRY(check_launch("segment sort"));
+        TRY(ensure(v, v->vis_part, (size_t)(scan_tiles(std::max(nh, m) + 1) + 1) * 4));
+        launch_iota_u32(v->vis_cidx.as<uint32_t>(), m, s);
+        const int cflip = radix_sort_pairs(v->vis_ctag.as<uint64_t>(), v->vis_cidx.as<uint32_t>(),
+                                           v->vis_ctag2.as<uint64_t>(), v->vis_cidx2.as<uint32_t>(), m, 0,
+                                           32 + v->vis_wave_bits, v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(),
+                                           v->vis_part.as<uint32_t>(), s);
+        const uint32_t* corder = cflip ? v->vis_cidx2.as<uint32_t>() : v->vis_cidx.as<uint32_t>();
+        int32_t* csz = v->vis_coff.as<int32_t>() + (m + 1);
+        int32_t* coff = v->vis_coff.as<int32_t>();
+        uint32_t* spart = v->vis_part.as<uint32_t>();
+        const int64_t stiles = scan_tiles(m + 1);
+        launch_gather_sizes(corder, m, v->vis_csize.as<uint32_t>(), csz, s);
+        launch_count_scan_reduce(csz, m, spart, spart + stiles, s);
+        launch_count_scan_apply(csz, m, spart, coff, s);
+        launch_chunk_order(corder, m, v->vis_cbase.as<uint32_t>(), v->vis_csize.as<uint32_t>(), coff,
+                           v->vis_key.as<uint32_t>(), v->vis_lkey.as<uint32_t>(), v->vis_idx.as<uint32_t>(), s);
+        const int flip = radix_sort_pairs32(v->vis_lkey.as<uint32_t>(), v->vis_idx.as<uint32_t>(),
+                                            v->vis_tkey.as<uint32_t>(), v->vis_tidx.as<uint32_t>(), n, 0,
+                                            v->vis_leaf_bits, v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(),
+                                            v->vis_part.as<uint32_t>(), s);
+        TRY(check_launch("segment order"));
         v->vis_in_tmp = flip != 0;
         TRY(ensure(v, v->vis_seg, (size_t)std::max<int64_t>(2 * nleaf, 2) * 4));
         launch_fill_i32(v->vis_seg.as<int32_t>(), 2 * nleaf, 0, s);
-        const uint64_t* skey = v->vis_in_tmp ? v->vis_tkey.as<uint64_t>() : v->vis_key.as<uint64_t>();
-        launch_segments(skey, n, v->vis_ray_bits, v->vis_seg.as<int32_t>(), nleaf, s);
+        const uint32_t* skey = v->vis_in_tmp ? v->vis_tkey.as<uint32_t>() : v->vis_lkey.as<uint32_t>();
+        launch_segments(skey, n, v->vis_seg.as<int32_t>(), nleaf, s);
         TRY(check_launch("segment ranges"));
         v->vis_sorted = true;
     }
@@ -1492,7 +1534,6 @@ int raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int
     p.cam = rc;
     p.cams = dcams;
     p.sc = sc;
-    p.key = v->vis_in_tmp ? v->vis_tkey.as<uint64_t>() : v->vis_key.as<uint64_t>();
     p.idx = v->vis_in_tmp ? v->vis_tidx.as<uint32_t>() : v->vis_idx.as<uint32_t>();
     p.rec = v->vis_rec.as<float4>();
     p.seg = v->vis_seg.as<int32_t>();
@@ -1501,7 +1542,6 @@ int raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int
     p.gI = gI_dev;
     p.gT = gT_dev;
     p.grad = grad_dev;
-    p.ray_bits = v->vis_ray_bits;
     launch_ea_backward(p, s);
     return check_launch(what);
 }
