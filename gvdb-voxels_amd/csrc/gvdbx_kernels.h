@@ -119,7 +119,13 @@ struct GatherParams {
     int variant;              // 0: LDS-staged candidates, 1: ballot/readlane walk
     int xcd_remap;            // 1: give each XCD a contiguous run of tiles
 };
+constexpr int GATHER_SPLIT_DEFAULT = 64;   // chunks per work item of a dense tile (0: one wave per tile)
 void launch_gather(GatherParams gp, hipStream_t s);
+// Variant 1 with tiles of more than 2 split chunks of 64 candidates cut into
+// work items of about `split` chunks: items (item_cap int4), counts (2 int32),
+// slot_tile and acc (slot_cap tiles of 512 uint32) are scratch.
+void launch_gather_split(GatherParams gp, int split, int4* items, int64_t item_cap, int32_t* counts,
+                         int32_t* slot_tile, int64_t slot_cap, uint32_t* acc, hipStream_t s);
 
 struct SplatParams {
     const int32_t* leaf_cloud;   // batched clouds: cloud of each leaf (else null)

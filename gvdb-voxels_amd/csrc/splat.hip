@@ -8,7 +8,9 @@
 // brick-linear atlas (leaf k at k*brick_res^3 floats).  No global float
 // atomics are used; the cubic splat's LDS adds come from the tile's single
 // wave in an order fixed by the data, so every result is deterministic.
+#include <algorithm>
 #include <climits>
+#include <type_traits>
 #include <cmath>
 
 #include "gvdbx_kernels.h"
@@ -287,6 +289,15 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
 // neighbouring bin is filtered lane-parallel, then the surviving candidates
 // are walked one at a time with ballot + readlane (their coordinates land in
 // scalar registers), and the next chunk's load is in flight meanwhile.
+//
+// A tile's candidate sequence is the chunks (64 points, the last one of a bin
+// possibly shorter) of its 27 neighbour bins, bins in NB_ORDER.  Dense tiles
+// (more than 2 split chunks, split a tuning knob) are cut into ns =
+// ceil(chunks / split) work items; item s walks chunks s, s + ns, s + 2 ns, ... so that
+// every item meets the near bins first.  Items of one tile fold their minima
+// into a scratch tile with integer atomicMin (exact and order-free: the
+// minima are bit patterns of non-negative floats), and a last pass writes
+// the bricks from it with the same arithmetic as a single wave.
 // ---------------------------------------------------------------------------
 struct ChunkIter {
     int d = -1;       // position in NB_ORDER
@@ -299,11 +310,18 @@ struct ChunkIter {
 __constant__ int8_t NB_ORDER[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11, 15,
                                     17, 19, 21, 23, 25, 0, 2, 6, 8, 18, 20, 24, 26};
 
-__device__ __forceinline__ bool next_chunk(ChunkIter& it, const GatherParams& gp, int cx, int cy, int cz) {
-    if (it.j0 + 64 < it.end) {
-        it.j0 += 64;
-        return true;
+// Advance k >= 1 chunks through the candidate sequence of the tile in cell
+// (cx,cy,cz).  From the initial state (d = -1), k = s + 1 lands on chunk s.
+__device__ __forceinline__ bool skip_chunks(ChunkIter& it, const GatherParams& gp, int cx, int cy, int cz, int k) {
+    if (it.d >= 0) {
+        const int left = (it.end - it.j0 - 1) >> 6;   // chunks after the current one in its bin
+        if (k <= left) {
+            it.j0 += 64 * k;
+            return true;
+        }
+        k -= left;   // step `left + 1` is the next bin's first chunk
     }
+    k -= 1;
     while (++it.d < 27) {
         const int nb = NB_ORDER[it.d];
         int bx = cx + nb / 9 - 1, by = cy + (nb / 3) % 3 - 1, bz = cz + nb % 3 - 1;
@@ -311,158 +329,299 @@ __device__ __forceinline__ bool next_chunk(ChunkIter& it, const GatherParams& gp
         int64_t ci = gp.gb.index(bx, by, bz);
         int beg = gp.off[ci], end = gp.off[ci + 1];
         if (beg < end) {
-            it.j0 = beg;
-            it.end = end;
-            return true;
+            const int nch = (end - beg + 63) >> 6;
+            if (k < nch) {
+                it.j0 = beg + 64 * k;
+                it.end = end;
+                return true;
+            }
+            k -= nch;
         }
     }
     return false;
 }
 
-template <bool HALVES>
-__global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles) {
-    const int lane = threadIdx.x;
-    const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
-    if (tile < 0) return;
-    const int64_t k = tile / tiles_per_brick;
-    if (k >= gp.nleaf) return;
-    const int td = tile_dim(res0);
+// Where a tile sits: its leaf k, its origin inside the brick, its first
+// voxel in (cloud-local) index space and its level-0 cell.
+struct GTile {
+    int64_t k;
     int tox, toy, toz;
-    tile_origin(res0, tile % tiles_per_brick, tox, toy, toz);
-    const GNode nd = gp.leaves[k];
+    int T0x, T0y, T0z;
+    int cx, cy, cz;
+};
+
+__device__ __forceinline__ GTile gather_tile(const GatherParams& gp, int res0, int tiles_per_brick, int64_t tile) {
+    GTile g;
+    g.k = tile / tiles_per_brick;
+    tile_origin(res0, tile % tiles_per_brick, g.tox, g.toy, g.toz);
+    const GNode nd = gp.leaves[g.k];
     int shx = 0, shy = 0, shz = 0;   // cloud-local voxel coordinates (batched clouds)
     if (gp.leaf_cloud) {
-        const int cl = gp.leaf_cloud[k];
+        const int cl = gp.leaf_cloud[g.k];
         shx = gp.cell_off[3 * cl] * res0, shy = gp.cell_off[3 * cl + 1] * res0, shz = gp.cell_off[3 * cl + 2] * res0;
     }
-    const int T0x = nd.mPos[0] - shx + tox, T0y = nd.mPos[1] - shy + toy, T0z = nd.mPos[2] - shz + toz;
+    g.T0x = nd.mPos[0] - shx + g.tox, g.T0y = nd.mPos[1] - shy + g.toy, g.T0z = nd.mPos[2] - shz + g.toz;
+    g.cx = nd.mPos[0] / res0, g.cy = nd.mPos[1] / res0, g.cz = nd.mPos[2] / res0;
+    return g;
+}
+
+// Folds chunks s, s + ns, ... of the tile's candidates into m (lane = one
+// (x, y) column of the tile, m[z] = bits of min |p-v|^2 over its z voxels;
+// non-negative floats order like their bit patterns).  ns = plan(ns) is
+// taken after the first chunk's load is issued, so work that decides the
+// stride overlaps that load.
+template <bool HALVES, typename Plan>
+__device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile& g, int td, int s, int ns,
+                                            Plan&& plan, uint32_t (&m)[8]) {
+    const int lane = threadIdx.x;
+    const int T0x = g.T0x, T0y = g.T0y, T0z = g.T0z;
+    const int cx = g.cx, cy = g.cy, cz = g.cz;
     const int lx = lane % td, ly = lane / td;
     const bool lane_ok = ly < td;
     const float vx = (float)(T0x + lx), vy = (float)(T0y + ly);
     float vz[8];
-    uint32_t m[8];   // bits of min |p-v|^2: non-negative floats order like their bit patterns
 #pragma unroll
-    for (int z = 0; z < 8; z++) {
-        vz[z] = (float)(T0z + z);
-        m[z] = 0x7F800000u;
-    }
-    const int cx = nd.mPos[0] / res0, cy = nd.mPos[1] / res0, cz = nd.mPos[2] / res0;
+    for (int z = 0; z < 8; z++) vz[z] = (float)(T0z + z);
     const float R = gp.box_radius;
     const int sc = gp.sc, scs = gp.sc_shift;
     const float cut = gp.cutoff;
     const float cut2 = cut * cut;
 
     ChunkIter it;
-    bool have = next_chunk(it, gp, cx, cy, cz);
+    bool have = skip_chunks(it, gp, cx, cy, cz, s + 1);
     float4 cur = make_float4(0, 0, 0, 0);
     if (have && it.j0 + lane < it.end) cur = gp.sorted[it.j0 + lane];
-    while (have) {
-        const int cend = it.end, cj = it.j0 + lane;
-        have = next_chunk(it, gp, cx, cy, cz);
-        float4 nxt = make_float4(0, 0, 0, 0);
-        if (have && it.j0 + lane < it.end) nxt = gp.sorted[it.j0 + lane];
-        // Pruning bound: every voxel's running min is <= the tile's max of
-        // them, and a candidate at squared distance db2 from the voxels it
-        // may reach changes none of them if db2 exceeds that max (fma is
-        // monotone, so every reached voxel's |p-v|^2 >= db2).  Same per lane.
-        uint32_t lmax = 0;
-        if (lane_ok) {
-#pragma unroll
-            for (int z = 0; z < 8; z++)
-                if (z < td) lmax = max(lmax, m[z]);
-        }
-        uint32_t wmax = lmax;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, o));
-        const float bound = fminf(cut2, __uint_as_float(wmax));
-        const float lbound = fminf(cut2, __uint_as_float(lmax));
-        // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
-        // as a voxel range; a candidate must also come within the cutoff of the tile.
-        // The packed range is the membership range clipped to the tile (z) and to
-        // membership + cutoff (x, y: lanes outside it are masked).
-        bool ok = cj < cend;
-        uint32_t pk = 0;
-        {
-            float pw[3] = {cur.x, cur.y, cur.z};
-            int T0[3] = {T0x, T0y, T0z};
-            float db2 = 0.0f;
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
-                int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
-                int lo_c = max(lo, (int)ceilf(pw[a] - cut));
-                int hi_c = min(hi, (int)floorf(pw[a] + cut));
-                int lr = max(lo_c - T0[a], 0), hr = min(hi_c - T0[a], td - 1);
-                ok = ok && lr <= hr;
-                float e = fmaxf(fmaxf((float)(T0[a] + lr) - pw[a], pw[a] - (float)(T0[a] + hr)), 0.0f);
-                db2 = __builtin_fmaf(e, e, db2);
-                if (HALVES && a == 2) {
-                    // z: membership only (voxels beyond the cutoff cannot change the result)
-                    lr = max(lo - T0[a], 0);
-                    hr = min(hi - T0[a], td - 1);
-                }
-                pk |= ((uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3)) << (6 * a);
+    ns = plan(ns);
+    // One loop for ns == 1 (the stride folds away) and one for split tiles.
+    auto walk = [&](auto one) {
+        while (have) {
+            const int cend = it.end, cj = it.j0 + lane;
+            have = skip_chunks(it, gp, cx, cy, cz, decltype(one)::value ? 1 : ns);
+            float4 nxt = make_float4(0, 0, 0, 0);
+            if (have && it.j0 + lane < it.end) nxt = gp.sorted[it.j0 + lane];
+            // Pruning bound: every voxel's running min is <= the tile's max of
+            // them, and a candidate at squared distance db2 from the voxels it
+            // may reach changes none of them if db2 exceeds that max (fma is
+            // monotone, so every reached voxel's |p-v|^2 >= db2).  Same per lane.
+            uint32_t lmax = 0;
+            if (lane_ok) {
+    #pragma unroll
+                for (int z = 0; z < 8; z++)
+                    if (z < td) lmax = max(lmax, m[z]);
             }
-            ok = ok && db2 <= bound;
-        }
-        uint64_t mask = __ballot(ok);
-        while (mask) {
-            const int b = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.x), b));
-            const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), b));
-            const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), b));
-            const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pk, b);
-            const int xlo = q & 7, xhi = (q >> 3) & 7, ylo = (q >> 6) & 7, yhi = (q >> 9) & 7;
-            const int zlo = (q >> 12) & 7, zhi = (q >> 15) & 7;
-            if (lane_ok && lx >= xlo && lx <= xhi && ly >= ylo && ly <= yhi) {
-                const float dx = vx - px, dy = vy - py;
-                const float dxy = __builtin_fmaf(dy, dy, dx * dx);
-                if (dxy <= lbound) {
-                    if (HALVES) {
-                        // with sc >= 4 the z membership inside an 8-voxel tile is
-                        // one or both 4-voxel halves
-                        if (zlo < 4) {
-#pragma unroll
-                            for (int z = 0; z < 4; z++) {
-                                const float dz = vz[z] - pz;
-                                m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+            uint32_t wmax = lmax;
+    #pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, o));
+            const float bound = fminf(cut2, __uint_as_float(wmax));
+            const float lbound = fminf(cut2, __uint_as_float(lmax));
+            // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
+            // as a voxel range; a candidate must also come within the cutoff of the tile.
+            // The packed range is the membership range clipped to the tile (z) and to
+            // membership + cutoff (x, y: lanes outside it are masked).
+            bool ok = cj < cend;
+            uint32_t pk = 0;
+            {
+                float pw[3] = {cur.x, cur.y, cur.z};
+                int T0[3] = {T0x, T0y, T0z};
+                float db2 = 0.0f;
+    #pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
+                    int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                    int lo_c = max(lo, (int)ceilf(pw[a] - cut));
+                    int hi_c = min(hi, (int)floorf(pw[a] + cut));
+                    int lr = max(lo_c - T0[a], 0), hr = min(hi_c - T0[a], td - 1);
+                    ok = ok && lr <= hr;
+                    float e = fmaxf(fmaxf((float)(T0[a] + lr) - pw[a], pw[a] - (float)(T0[a] + hr)), 0.0f);
+                    db2 = __builtin_fmaf(e, e, db2);
+                    if (HALVES && a == 2) {
+                        // z: membership only (voxels beyond the cutoff cannot change the result)
+                        lr = max(lo - T0[a], 0);
+                        hr = min(hi - T0[a], td - 1);
+                    }
+                    pk |= ((uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3)) << (6 * a);
+                }
+                ok = ok && db2 <= bound;
+            }
+            uint64_t mask = __ballot(ok);
+            while (mask) {
+                const int b = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.x), b));
+                const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), b));
+                const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), b));
+                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pk, b);
+                const int xlo = q & 7, xhi = (q >> 3) & 7, ylo = (q >> 6) & 7, yhi = (q >> 9) & 7;
+                const int zlo = (q >> 12) & 7, zhi = (q >> 15) & 7;
+                if (lane_ok && lx >= xlo && lx <= xhi && ly >= ylo && ly <= yhi) {
+                    const float dx = vx - px, dy = vy - py;
+                    const float dxy = __builtin_fmaf(dy, dy, dx * dx);
+                    if (dxy <= lbound) {
+                        if (HALVES) {
+                            // with sc >= 4 the z membership inside an 8-voxel tile is
+                            // one or both 4-voxel halves
+                            if (zlo < 4) {
+    #pragma unroll
+                                for (int z = 0; z < 4; z++) {
+                                    const float dz = vz[z] - pz;
+                                    m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                                }
                             }
-                        }
-                        if (zhi >= 4) {
-#pragma unroll
-                            for (int z = 4; z < 8; z++) {
-                                const float dz = vz[z] - pz;
-                                m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                            if (zhi >= 4) {
+    #pragma unroll
+                                for (int z = 4; z < 8; z++) {
+                                    const float dz = vz[z] - pz;
+                                    m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                                }
                             }
-                        }
-                    } else {
-#pragma unroll
-                        for (int z = 0; z < 8; z++) {
-                            if (z >= zlo && z <= zhi) {
-                                const float dz = vz[z] - pz;
-                                m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                        } else {
+    #pragma unroll
+                            for (int z = 0; z < 8; z++) {
+                                if (z >= zlo && z <= zhi) {
+                                    const float dz = vz[z] - pz;
+                                    m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                                }
                             }
                         }
                     }
                 }
             }
+            cur = nxt;
         }
-        cur = nxt;
-    }
+    };
+    if (ns == 1)
+        walk(std::true_type{});
+    else
+        walk(std::false_type{});
+}
 
-    if (!lane_ok) return;
+// Brick values of a tile from its minima: sqrt is correctly rounded, so the
+// stored value is a function of min |p-v|^2 alone.
+__device__ __forceinline__ void gather_store(const GatherParams& gp, const GTile& g, int td, const uint32_t (&m)[8]) {
+    const int lane = threadIdx.x;
+    const int lx = lane % td, ly = lane / td;
+    if (ly >= td) return;
     const int br = gp.brick_res, a = gp.apron;
-    float* brick = gp.chan + (int64_t)k * br * br * br;
+    float* brick = gp.chan + g.k * br * br * br;
     const float r = gp.radius;
 #pragma unroll
     for (int z = 0; z < 8; z++) {
         if (z < td) {
-            int64_t vi = ((int64_t)(toz + z + a) * br + (toy + ly + a)) * br + (tox + lx + a);
+            int64_t vi = ((int64_t)(g.toz + z + a) * br + (g.toy + ly + a)) * br + (g.tox + lx + a);
             float init = gp.accum ? brick[vi] : (gp.mode == 1 ? 3.0f : 0.0f);
             float c = sqrtf(__uint_as_float(m[z]));
             brick[vi] = gp.mode == 1 ? fminf(init, c - r) : fmaxf(init, r - c);
         }
+    }
+}
+
+// One wave per tile.  SPLIT: a tile with more than 2 split chunks is cut
+// into ns = ceil(chunks / split) items; this wave takes a scratch slot and
+// item 0, appends items 1 .. ns-1 to `items` (counts[0], zero on entry) for
+// k_gather_items, and leaves its minima in scratch tile `slot` (counts[1]
+// slots).  Appends beyond item_cap / slot_cap are dropped: the host sizes
+// both from a bound that holds.
+struct GatherSplit {
+    int split;
+    int4* items;
+    int64_t item_cap;
+    int32_t* counts;
+    int32_t* slot_tile;
+    int64_t slot_cap;
+    uint32_t* acc;
+};
+
+template <bool HALVES, bool SPLIT>
+__global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles,
+                                                  GatherSplit gs) {
+    const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
+    if (tile < 0) return;
+    if (tile / tiles_per_brick >= gp.nleaf) return;
+    const int lane = threadIdx.x;
+    const int td = tile_dim(res0);
+    const GTile g = gather_tile(gp, res0, tiles_per_brick, tile);
+    int ns = 1, slot = -1;
+    auto plan = [&](int) {
+        if (!SPLIT) return 1;
+        int nch = 0;   // lane d < 27: chunks of neighbour bin d
+        if (lane < 27) {
+            const int bx = g.cx + lane / 9 - 1, by = g.cy + (lane / 3) % 3 - 1, bz = g.cz + lane % 3 - 1;
+            if (gp.gb.contains(bx, by, bz)) {
+                const int64_t ci = gp.gb.index(bx, by, bz);
+                nch = (gp.off[ci + 1] - gp.off[ci] + 63) >> 6;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) nch += __shfl_xor(nch, o);
+        nch = __builtin_amdgcn_readfirstlane(nch);
+        if (nch > 2 * gs.split) {
+            ns = (nch + gs.split - 1) / gs.split;
+            int base = 0, sl = 0;
+            if (lane == 0) {
+                base = atomicAdd(&gs.counts[0], ns - 1);
+                sl = atomicAdd(&gs.counts[1], 1);
+            }
+            base = __builtin_amdgcn_readfirstlane(base);
+            sl = __builtin_amdgcn_readfirstlane(sl);
+            slot = sl < gs.slot_cap ? sl : -1;
+            for (int s = 1 + lane; s < ns; s += 64)
+                if ((int64_t)base + s - 1 < gs.item_cap) gs.items[base + s - 1] = make_int4((int)tile, s, ns, slot);
+            if (lane == 0 && slot >= 0) gs.slot_tile[slot] = (int)tile;
+        }
+        return ns;
+    };
+    uint32_t m[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
+    gather_walk<HALVES>(gp, g, td, 0, 1, plan, m);
+    if (!SPLIT || ns == 1) {
+        gather_store(gp, g, td, m);
+    } else if (slot >= 0) {
+        const bool lane_ok = lane / td < td;
+        uint32_t* at = gs.acc + (int64_t)slot * 512 + lane;
+#pragma unroll
+        for (int z = 0; z < 8; z++) at[z * 64] = lane_ok && z < td ? m[z] : 0x7F800000u;
+    }
+}
+
+// Persistent walk over the items beyond the first of each split tile
+// (counts[0] of them; the scratch tiles hold item 0's minima).
+template <bool HALVES>
+__global__ __launch_bounds__(64) void k_gather_items(GatherParams gp, int res0, int tiles_per_brick,
+                                                     const int4* items, int64_t item_cap, const int32_t* counts,
+                                                     uint32_t* acc, int64_t slot_cap) {
+    const int lane = threadIdx.x;
+    const int td = tile_dim(res0);
+    const int nitems = (int)min((int64_t)counts[0], item_cap);
+    for (int idx = blockIdx.x; idx < nitems; idx += gridDim.x) {
+        const int4 w = items[idx];
+        const GTile g = gather_tile(gp, res0, tiles_per_brick, w.x);
+        uint32_t m[8];
+#pragma unroll
+        for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
+        gather_walk<HALVES>(gp, g, td, w.y, w.z, [](int n) { return n; }, m);
+        if (lane / td < td && w.w >= 0 && w.w < slot_cap) {
+            uint32_t* at = acc + (int64_t)w.w * 512 + lane;
+#pragma unroll
+            for (int z = 0; z < 8; z++)
+                if (z < td && m[z] != 0x7F800000u) atomicMin(at + z * 64, m[z]);
+        }
+    }
+}
+
+// Bricks of the split tiles from their scratch minima.
+__global__ __launch_bounds__(64) void k_gather_finish(GatherParams gp, int res0, int tiles_per_brick,
+                                                      const int32_t* slot_tile, const int32_t* counts,
+                                                      const uint32_t* acc, int64_t slot_cap) {
+    const int lane = threadIdx.x;
+    const int td = tile_dim(res0);
+    const int nslot = (int)min((int64_t)counts[1], slot_cap);
+    for (int sl = blockIdx.x; sl < nslot; sl += gridDim.x) {
+        const GTile g = gather_tile(gp, res0, tiles_per_brick, slot_tile[sl]);
+        uint32_t m[8];
+#pragma unroll
+        for (int z = 0; z < 8; z++) m[z] = acc[(int64_t)sl * 512 + z * 64 + lane];
+        gather_store(gp, g, td, m);
     }
 }
 
@@ -1082,9 +1241,49 @@ void launch_gather(GatherParams gp, hipStream_t s) {
     else if (gp.variant == 2)
         hipLaunchKernelGGL(k_gather_pc, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
     else if (halves)
-        hipLaunchKernelGGL(k_gather_rl<true>, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
+        hipLaunchKernelGGL((k_gather_rl<true, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
     else
-        hipLaunchKernelGGL(k_gather_rl<false>, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
+        hipLaunchKernelGGL((k_gather_rl<false, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+}
+
+int gather_waves() {
+    static int waves = 0;
+    if (!waves) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_items<true>, 64, 0) != hipSuccess)
+            cus = 256, per_cu = 8;
+        waves = std::max(1, cus) * std::max(1, per_cu);
+    }
+    return waves;
+}
+
+void launch_gather_split(GatherParams gp, int split, int4* items, int64_t item_cap, int32_t* counts,
+                         int32_t* slot_tile, int64_t slot_cap, uint32_t* acc, hipStream_t s) {
+    const int res0 = gp.brick_res - 2 * gp.apron;
+    const int td = res0 < 8 ? res0 : 8;
+    const int per = res0 / td;
+    const int tiles = per * per * per;
+    const int64_t ntiles = gp.nleaf * tiles;
+    if (ntiles <= 0) return;
+    const bool halves = td == 8 && gp.sc >= 4;
+    launch_fill_i32(counts, 2, 0, s);
+    const GatherSplit gs{split, items, item_cap, counts, slot_tile, slot_cap, acc};
+    const dim3 grid((unsigned)gvdbx_xcd_blocks(ntiles));
+    if (halves)
+        hipLaunchKernelGGL((k_gather_rl<true, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+    else
+        hipLaunchKernelGGL((k_gather_rl<false, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+    const int waves = gather_waves();
+    const dim3 gi((unsigned)std::max<int64_t>(1, std::min<int64_t>(item_cap, waves)));
+    if (halves)
+        hipLaunchKernelGGL(k_gather_items<true>, gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts, acc,
+                           slot_cap);
+    else
+        hipLaunchKernelGGL(k_gather_items<false>, gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts, acc,
+                           slot_cap);
+    const dim3 gf((unsigned)std::max<int64_t>(1, std::min<int64_t>(slot_cap, waves)));
+    hipLaunchKernelGGL(k_gather_finish, gf, dim3(64), 0, s, gp, res0, tiles, slot_tile, counts, acc, slot_cap);
 }
 
 int splat_halo(float radius) { return (int)std::ceil(radius); }

@@ -86,6 +86,7 @@ struct gvdbx_volume {
     DevBuf reach0, leafbits, partials, totals, bbox, misc;
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
+    int gather_split = GATHER_SPLIT_DEFAULT;
     int xcd_remap = 0;
     int splat_chunk = SPLAT_CHUNK_DEFAULT;
     int splat_variant = 1;
@@ -96,6 +97,7 @@ struct gvdbx_volume {
     // binning
     DevBuf bin_off, sorted, bkey, bval, bkey2, bval2, rs_hist, rs_off, rs_part;
     DevBuf splat_src;   // (x, y, z, weight) in bin order, made per SplatCubic call
+    DevBuf gs_items, gs_counts, gs_slots, gs_acc;   // Gather* work items of dense tiles
     DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
     bool bins_valid = false;
@@ -818,6 +820,30 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
     gp.variant = v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
+    if (gp.variant == 1 && v->gather_split > 0) {
+        // Bounds: a bin is a neighbour of at most 27 leaves, so the chunks
+        // summed over all tiles are at most 27 * tiles * (chunks of all bins),
+        // and those are at most ceil(n/64) + non-empty bins (<= min(n, bins)).
+        const int res0 = gp.brick_res - 2 * gp.apron;
+        const int td = std::min(res0, 8);
+        const int64_t tpb = (int64_t)(res0 / td) * (res0 / td) * (res0 / td);
+        const int64_t ntiles = gp.nleaf * tpb;
+        const int64_t nb = std::max<int64_t>(v->bins_n, 0);
+        const double chunks = 27.0 * (double)tpb * ((double)((nb + 63) / 64) + (double)std::min<int64_t>(nb, gp.gb.count()));
+        const double split = (double)v->gather_split;
+        const double item_cap = std::ceil(chunks / split) + 64.0;   // items beyond the first of each tile
+        const int64_t slot_cap = std::min<int64_t>(ntiles, (int64_t)std::ceil(chunks / split) + 1);
+        if (item_cap < (double)INT_MAX && ntiles < INT_MAX) {
+            TRY(ensure(v, v->gs_items, (size_t)item_cap * 16));
+            TRY(ensure(v, v->gs_counts, 8));
+            TRY(ensure(v, v->gs_slots, (size_t)std::max<int64_t>(slot_cap, 1) * 4));
+            TRY(ensure(v, v->gs_acc, (size_t)std::max<int64_t>(slot_cap, 1) * 2048));
+            launch_gather_split(gp, v->gather_split, v->gs_items.as<int4>(), (int64_t)item_cap,
+                                v->gs_counts.as<int32_t>(), v->gs_slots.as<int32_t>(), slot_cap,
+                                v->gs_acc.as<uint32_t>(), v->stream);
+            return check_launch(what);
+        }
+    }
     launch_gather(gp, v->stream);
     return check_launch(what);
 }
@@ -940,7 +966,8 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
-                      &v->splat_src, &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
+                      &v->splat_src, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
+                      &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
                       &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
                       &v->vis_cbase, &v->vis_csize, &v->vis_cidx, &v->vis_cidx2, &v->vis_coff};
@@ -1052,6 +1079,11 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->gather_variant = value;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "gather_split")) {
+        if (value < 0 || value > (1 << 20)) FAIL(GVDBX_ERR_ARG, "gather_split must be in [0, 2^20]");
+        v->gather_split = value;
+        return GVDBX_OK;
+    }
     if (!strcmp(key, "xcd_remap")) {
         if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "xcd_remap must be 0 or 1");
         v->xcd_remap = value;
@@ -1075,6 +1107,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     if (!key || !value) FAIL(GVDBX_ERR_ARG, "null tuning key or output");
     if (!strcmp(key, "gather_variant")) {
         *value = v->gather_variant;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "gather_split")) {
+        *value = v->gather_split;
         return GVDBX_OK;
     }
     if (!strcmp(key, "xcd_remap")) {

@@ -86,7 +86,10 @@ int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value);
 int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
 /* kernel tuning knobs (no reference counterpart): key "gather_variant"
  * (0 = LDS-staged candidates, 1 = ballot/readlane walk, 2 = per-lane
- * candidates with LDS minima; default 1), "xcd_remap" (1 = contiguous brick
+ * candidates with LDS minima; default 1), "gather_split" (variant 1: tiles
+ * with more than twice this many 64-point candidate chunks are cut into
+ * work items of about this many; 0 = one wavefront per tile; the result is
+ * the same bit for bit; default 64, range [0, 2^20]), "xcd_remap" (1 = contiguous brick
  * runs per XCD; default 0), "splat_variant" (SplatCubic: 0 = tile pull,
  * 1 = cell push; default 1; cell push needs a (res0 + 2 ceil(r))^3 float
  * grid within 64 KiB, else tile pull runs),

@@ -333,6 +333,44 @@ def test_full_size_cfg2_properties():
     assert inf.node_cnt[3] == 1
 
 
+@pytest.mark.parametrize("log2dims,sc,R", [((3, 3, 3, 3, 5), 4, 3.0), ((3, 3, 3, 3, 5), 2, 3.0),
+                                           ((4, 3, 3, 3, 5), 4, 3.0), ((2, 2, 2, 2, 3), 2, 1.0)])
+def test_gather_split_bit_identical(log2dims, sc, R):
+    """Dense tiles cut into work items (gather_split) give the oracle's bits:
+    a dense Gaussian cluster (~600 candidate chunks per tile) beside a sparse
+    box, level set and density, fresh and accumulated."""
+    _log(f"[gpu] gather split {log2dims} sc={sc}")
+    rng = np.random.default_rng(4242)
+    a = rng.normal([40, 40, 40], 4.0, (40000, 3))
+    b = rng.random((10000, 3)) * [90, 70, 80] + [10, 8, 12]
+    pts = np.abs(np.concatenate([a, b])).astype(np.float32)
+    gv, ov, tp = build_pair(pts, R, log2dims)
+    gl = gv.InsertPointsSubcell(sc, len(pts), R)
+    assert gl == ov.insert_points_subcell(sc, pts, R)
+    assert gv.GetTuning("gather_split") >= 0
+    for mode in (1, 0):
+        gather = gv.GatherLevelSet if mode == 1 else gv.GatherDensity
+        ogather = ov.gather_level_set if mode == 1 else ov.gather_density
+        ogather(1.5)
+        ref = ov.bricks()[:, 1:-1, 1:-1, 1:-1]
+        ov.fill_channel(0, 0.25)
+        ogather(1.5, accum=True)
+        acc_ref = ov.bricks()[:, 1:-1, 1:-1, 1:-1]
+        for split in (0, 1, 5, 64, 1 << 20):
+            gv.SetTuning("gather_split", split)
+            gv.FillChannel(0, -7.0)
+            gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
+            np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], ref, err_msg=f"mode {mode} split {split}")
+            gv.FillChannel(0, 0.25)
+            gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0, accum=True)
+            np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], acc_ref,
+                                          err_msg=f"accumulate, mode {mode} split {split}")
+    with pytest.raises(Exception):
+        gv.SetTuning("gather_split", -1)
+    with pytest.raises(Exception):
+        gv.SetTuning("gather_split", (1 << 20) + 1)
+
+
 @pytest.mark.parametrize("mode", [1, 0])
 def test_gather_variants_bit_identical(mode):
     """All gather kernel variants produce the oracle's bits."""
