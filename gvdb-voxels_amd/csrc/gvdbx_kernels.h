@@ -12,7 +12,10 @@ int64_t scan_tiles(int64_t n);
 
 // ---------------- topology ----------------
 void launch_bbox_init(BBoxAccum* acc, hipStream_t s);
-void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, hipStream_t s);
+// parts: bbox_parts(n) scratch partials
+int bbox_parts(int64_t n);
+void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, BBoxAccum* parts,
+                 hipStream_t s);
 
 struct ReachParams {
     CellGrid g0;
@@ -85,6 +88,7 @@ struct BinParams {
     float radius;
     CellGrid g0;
     const uint32_t* leafbits;   // one bit per g0 cell
+    const uint32_t* nbmask;     // per g0 cell: 27-bit neighbour-leaf mask (launch_leaf_nbmask)
 };
 // reach masks of level-0 cells from cell-sorted points (total = off[gb.count()])
 void launch_reach_sorted(const float4* sorted, const int32_t* total, int64_t nmax, ReachParams rp, uint32_t* reach0,
@@ -95,6 +99,7 @@ void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s);
 void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
                              hipStream_t s);
+void launch_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask, hipStream_t s);
 void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s);
 
 // ---------------- points -> voxels ----------------
@@ -221,7 +226,7 @@ void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 // key[i] = bin cell of point i (gb.count() for invalid / outside points),
 // val[i] = i
 void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, CloudMap cm,
-                     uint32_t* key, uint32_t* val, hipStream_t s);
+                     uint32_t* key, uint32_t* val, hipStream_t s);   // val may be null
 // sorted[j] = (w of point val[j], val[j]) for keys < ncell
 void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
                        const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s);

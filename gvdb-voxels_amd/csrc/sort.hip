@@ -3,13 +3,15 @@
 // called from RadixSortByByte VG.cpp:904-928) wherever this library needs a
 // sorted key list.
 //
-// One pass per 8-bit digit, three launches per pass:
-//   1. k_rs_hist: per 4096-key tile, a 256-bin digit histogram in LDS,
+// The key bits [begin, end) are split into as few passes as digits of at
+// most 10 bits (9 for 64-bit keys, whose tile takes more LDS) allow, of
+// near-equal width (18 bits: two 9-bit passes).  Three launches per pass:
+//   1. k_rs_hist: per 4096-key tile, a 2^DB-bin digit histogram in LDS,
 //      written digit-major (hist[d * ntiles + tile]);
 //   2. an exclusive scan over the digit-major histogram gives every
 //      (digit, tile) its output base (the count scan of the topology code);
 //   3. k_rs_scatter: each tile re-reads its keys in 16 rounds of 256 (index
-//      order), ranks equal digits inside each wave with eight ballots, adds
+//      order), ranks equal digits inside each wave with DB ballots, adds
 //      the counts of lower waves and earlier rounds from LDS, reorders the
 //      tile by digit in LDS, and writes every digit's run to its base.
 //      Equal digits keep their input order: stable.
@@ -22,7 +24,8 @@ namespace {
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
-constexpr int RS_BINS = 256;
+constexpr int RS_MAX_BITS = 10;
+constexpr int RS_MAX_BINS = 1 << RS_MAX_BITS;
 constexpr int RS_WAVES = RS_THREADS / 64;
 
 inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
@@ -32,46 +35,60 @@ inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
     return (unsigned)g;
 }
 
-template <typename K>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* keys, int64_t n, int shift, int64_t ntiles,
-                                                         int32_t* hist) {
-    __shared__ int32_t cnt[RS_BINS];
-    cnt[threadIdx.x] = 0;
+// DB: digit bits of the kernel (2^DB bins); the pass's digit is
+// (key >> shift) & mask with mask < 2^DB.
+template <typename K, int DB, int ITEMS = RS_ITEMS>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* keys, int64_t n, int shift, uint32_t mask,
+                                                         int64_t ntiles, int32_t* hist) {
+    constexpr int B = 1 << DB;
+    __shared__ int32_t cnt[B];
+    for (int b = threadIdx.x; b < B; b += RS_THREADS) cnt[b] = 0;
     __syncthreads();
-    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t t0 = (int64_t)blockIdx.x * RS_THREADS * ITEMS;
 #pragma unroll 4
-    for (int r = 0; r < RS_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         int64_t i = t0 + r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1);
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & mask], 1);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    for (int b = threadIdx.x; b < B; b += RS_THREADS) hist[(int64_t)b * ntiles + blockIdx.x] = cnt[b];
 }
 
 // The tile is first reordered by digit in LDS (stable), then written out in
 // that order: consecutive threads write consecutive addresses of one digit's
-// run, so the global stores coalesce.
-template <typename K>
+// run, so the global stores coalesce.  Thread t owns bins [t R, t R + R).
+template <typename K, int DB>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const uint32_t* vals, int64_t n,
-                                                            int shift, int64_t ntiles, const int32_t* hist,
-                                                            const int32_t* off, K* okeys, uint32_t* ovals) {
-    __shared__ int32_t gbase[RS_BINS];    // global output start of each digit for this tile
-    __shared__ int32_t tstart[RS_BINS];   // start of each digit inside the tile
-    __shared__ int32_t run[RS_BINS];
-    __shared__ int32_t wcnt[RS_WAVES][RS_BINS];
+                                                            int shift, uint32_t mask, int64_t ntiles,
+                                                            const int32_t* hist, const int32_t* off, K* okeys,
+                                                            uint32_t* ovals) {
+    constexpr int B = 1 << DB;
+    constexpr int R = B / RS_THREADS > 0 ? B / RS_THREADS : 1;
+    static_assert(B >= RS_THREADS, "one bin or more per thread");
+    __shared__ int32_t gbase[B];    // global output start of each digit for this tile
+    __shared__ int32_t tstart[B];   // start of each digit inside the tile
+    __shared__ int32_t run[B];
+    __shared__ int32_t wcnt[RS_WAVES][B];
     __shared__ int32_t wtot[RS_WAVES];
     __shared__ K lkey[RS_TILE];
     __shared__ uint32_t lval[RS_TILE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
     const int tn = (int)min((int64_t)RS_TILE, n - t0);
-    gbase[tid] = off[(int64_t)tid * ntiles + blockIdx.x];
-    run[tid] = 0;
-    for (int w = 0; w < RS_WAVES; w++) wcnt[w][tid] = 0;
+    int c[R];
+    int csum = 0;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const int b = tid * R + j;
+        gbase[b] = off[(int64_t)b * ntiles + blockIdx.x];
+        run[b] = 0;
+        for (int w = 0; w < RS_WAVES; w++) wcnt[w][b] = 0;
+        c[j] = hist[(int64_t)b * ntiles + blockIdx.x];
+        csum += c[j];
+    }
     // exclusive scan of this tile's digit counts (from the histogram pass)
     {
-        const int c = hist[(int64_t)tid * ntiles + blockIdx.x];
-        int x = c;
+        int x = csum;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(x, o);
@@ -81,7 +98,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
         __syncthreads();
         int pre = 0;
         for (int w = 0; w < wave; w++) pre += wtot[w];
-        tstart[tid] = pre + x - c;
+        int st = pre + x - csum;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            tstart[tid * R + j] = st;
+            st += c[j];
+        }
         __syncthreads();
     }
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -94,11 +116,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
         if (ok) {
             k = keys[t0 + li];
             val = vals[t0 + li];
-            d = (int)((k >> shift) & 0xFF);
+            d = (int)((uint32_t)(k >> shift) & mask);
         }
         uint64_t peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < DB; b++) {
             uint64_t m = __ballot(ok && ((d >> b) & 1));
             peers &= ((d >> b) & 1) ? m : ~m;
         }
@@ -112,17 +134,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
             lval[pos] = val;
         }
         __syncthreads();
-        int add = 0;
-        for (int w = 0; w < RS_WAVES; w++) {
-            add += wcnt[w][tid];
-            wcnt[w][tid] = 0;
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const int b = tid * R + j;
+            int add = 0;
+            for (int w = 0; w < RS_WAVES; w++) {
+                add += wcnt[w][b];
+                wcnt[w][b] = 0;
+            }
+            run[b] += add;
         }
-        run[tid] += add;
         __syncthreads();
     }
     for (int j = tid; j < tn; j += RS_THREADS) {
         const K k = lkey[j];
-        const int d = (int)((k >> shift) & 0xFF);
+        const int d = (int)((uint32_t)(k >> shift) & mask);
         const int64_t pos = (int64_t)gbase[d] + (j - tstart[d]);
         okeys[pos] = k;
         ovals[pos] = lval[j];
@@ -139,31 +165,49 @@ __global__ void k_iota_u32(uint32_t* p, int64_t n) {
 int64_t radix_hist_entries(int64_t n) {
     int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
     if (ntiles < 1) ntiles = 1;
-    return ntiles * RS_BINS;
+    return ntiles * RS_MAX_BINS;
 }
 
 void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
 }
 
+template <typename K, int DB>
+void radix_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, int64_t n, int shift, int width,
+                int64_t ntiles, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
+    const int64_t nh = ntiles << DB;
+    const int64_t stiles = scan_tiles(nh + 1);
+    const uint32_t mask = (1u << width) - 1u;
+    hipLaunchKernelGGL((k_rs_hist<K, DB>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, n, shift, mask, ntiles,
+                       hist);
+    launch_count_scan_reduce(hist, nh, partials, partials + stiles, s);
+    launch_count_scan_apply(hist, nh, partials, off, s);
+    hipLaunchKernelGGL((k_rs_scatter<K, DB>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, vin, n, shift, mask,
+                       ntiles, hist, off, kout, vout);
+}
+
 template <typename K>
 int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t n, int begin_bit, int end_bit,
                     int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
     if (n <= 1 || end_bit <= begin_bit) return 0;
-    const int64_t ntiles = radix_hist_entries(n) / RS_BINS;
-    const int64_t nh = ntiles * RS_BINS;
-    const int64_t stiles = scan_tiles(nh + 1);
+    const int64_t ntiles = radix_hist_entries(n) / RS_MAX_BINS;
+    const int maxdb = sizeof(K) == 8 ? 9 : RS_MAX_BITS;
+    const int bits = end_bit - begin_bit;
+    const int np = (bits + maxdb - 1) / maxdb;
     K* kin = keys;
     uint32_t* vin = vals;
     K* kout = tkeys;
     uint32_t* vout = tvals;
     int flips = 0;
-    for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, n, shift, ntiles, hist);
-        launch_count_scan_reduce(hist, nh, partials, partials + stiles, s);
-        launch_count_scan_apply(hist, nh, partials, off, s);
-        hipLaunchKernelGGL(k_rs_scatter<K>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, vin, n, shift,
-                           ntiles, hist, off, kout, vout);
+    for (int p = 0, shift = begin_bit; p < np; p++) {
+        const int width = bits / np + (p < bits % np ? 1 : 0);
+        if (width <= 8)
+            radix_pass<K, 8>(kin, vin, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+        else if (width == 9)
+            radix_pass<K, 9>(kin, vin, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+        else if constexpr (sizeof(K) == 4)
+            radix_pass<K, 10>(kin, vin, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+        shift += width;
         K* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
         flips ^= 1;

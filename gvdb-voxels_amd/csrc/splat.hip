@@ -1101,6 +1101,64 @@ __global__ __launch_bounds__(64) void k_splat_tile(SplatParams sp, int res0, int
 // apron voxel (x,y,z) of brick k holds the value of world voxel
 // mPos + (x,y,z) - 1 in whichever leaf contains it, else boundval.
 // ---------------------------------------------------------------------------
+// 8^3 bricks with apron 1: the 488 apron voxels of a 10^3 brick in
+// ascending brick-linear order, so every lane has one (no interior lanes
+// idle) and consecutive lanes write consecutive apron voxels.
+struct Apron10Tab {
+    uint16_t i[488];
+};
+constexpr Apron10Tab make_apron10() {
+    Apron10Tab t{};
+    int n = 0;
+    for (int i = 0; i < 1000; i++) {
+        const int x = i % 10, y = (i / 10) % 10, z = i / 100;
+        if (x == 0 || x == 9 || y == 0 || y == 9 || z == 0 || z == 9) t.i[n++] = (uint16_t)i;
+    }
+    return t;
+}
+__constant__ Apron10Tab APRON10 = make_apron10();
+
+// One wave per brick (four per block): the 27 neighbour leaves sit one per
+// lane and are fetched with a lane shuffle; each lane's 8 apron voxels are
+// loaded before any is stored, so 8 reads per lane are in flight at once.
+// Each XCD takes a contiguous run of bricks, so the neighbour faces a brick
+// reads were mostly brought into the same L2 by its neighbours.
+__global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = gvdbx_xcd_item(blockIdx.x, nblk);
+    if (blk < 0) return;
+    const int64_t k = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (k >= ap.nleaf) return;
+    int iv[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) iv[u] = lane + 64 * u < 488 ? (int)APRON10.i[lane + 64 * u] : -1;
+    const GNode nd = ap.leaves[k];
+    int32_t leaf = -1;
+    if (lane < 27) {
+        const int cx = nd.mPos[0] >> 3, cy = nd.mPos[1] >> 3, cz = nd.mPos[2] >> 3;
+        const int nx = cx + lane % 3 - 1, ny = cy + (lane / 3) % 3 - 1, nz = cz + lane / 9 - 1;
+        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        if (leaf >= ap.nleaf) leaf = -1;
+    }
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int i = iv[u] >= 0 ? iv[u] : 111;   // 111 = (1,1,1): an interior voxel, neighbour 13 (itself)
+        const int x = i % 10, y = (i / 10) % 10, z = i / 100;
+        const int dx = x == 0 ? -1 : (x == 9 ? 1 : 0);
+        const int dy = y == 0 ? -1 : (y == 9 ? 1 : 0);
+        const int dz = z == 0 ? -1 : (z == 9 ? 1 : 0);
+        const int nl = __shfl(leaf, (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));   // every lane takes part
+        // interior voxel of the neighbour holding world voxel mPos + (x,y,z) - 1
+        v[u] = ap.boundval;
+        if (iv[u] >= 0 && nl >= 0) v[u] = ap.chan[(int64_t)nl * 1000 + i - 8 * (dz * 100 + dy * 10 + dx)];
+    }
+    float* brick = ap.chan + k * 1000;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        if (iv[u] >= 0) brick[iv[u]] = v[u];
+}
+
 // BR > 0: padded brick size known at compile time; BR == 0: runtime size.
 template <int BR>
 __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
@@ -1385,7 +1443,10 @@ void launch_splat_src(const float4* sorted, int64_t n, const int32_t* count, con
 
 void launch_apron(ApronParams ap, hipStream_t s) {
     if (ap.nleaf <= 0) return;
-    if (ap.brick_res == 10)
+    if (ap.brick_res == 10 && ap.range0 == 8)
+        hipLaunchKernelGGL(k_apron10, dim3((unsigned)gvdbx_xcd_blocks((ap.nleaf + 3) / 4)), dim3(256), 0, s, ap,
+                           (ap.nleaf + 3) / 4);
+    else if (ap.brick_res == 10)
         hipLaunchKernelGGL(k_apron<10>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
     else
         hipLaunchKernelGGL(k_apron<0>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);

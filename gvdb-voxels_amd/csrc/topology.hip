@@ -199,54 +199,98 @@ __global__ void k_bbox_init(BBoxAccum* acc) {
     if (threadIdx.x == 0) acc->nvalid = 0;
 }
 
-__global__ __launch_bounds__(256) void k_bbox(PointSrc src, int64_t n, float tx, float ty, float tz,
-                                              BBoxAccum* acc) {
-    float mn0 = __builtin_inff(), mn1 = __builtin_inff(), mn2 = __builtin_inff();
-    float mx0 = -__builtin_inff(), mx1 = -__builtin_inff(), mx2 = -__builtin_inff();
-    unsigned long long cnt = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float x, y, z;
-        if (!gvdbx_point(src, i, tx, ty, tz, x, y, z)) continue;
+// Bounding box of the valid points in two passes: each block reduces a
+// strided share (BBOX_UNROLL independent point loads in flight per thread)
+// to one partial, and one block folds the partials into acc with the
+// integer order of the float bits, as same-address atomics would (no
+// atomics: thousands of them on one line serialise in L2).
+constexpr int BBOX_UNROLL = 4;
+constexpr int BBOX_BLOCKS = 2048;
+
+__device__ __forceinline__ void bbox_add(const PointSrc& src, int64_t i, int64_t n, float tx, float ty, float tz,
+                                         float (&mn)[3], float (&mx)[3], unsigned long long& cnt) {
+    float x, y, z;
+    if (i < n && gvdbx_point(src, i, tx, ty, tz, x, y, z)) {
         cnt++;
-        mn0 = fminf(mn0, x); mn1 = fminf(mn1, y); mn2 = fminf(mn2, z);
-        mx0 = fmaxf(mx0, x); mx1 = fmaxf(mx1, y); mx2 = fmaxf(mx2, z);
+        mn[0] = fminf(mn[0], x); mn[1] = fminf(mn[1], y); mn[2] = fminf(mn[2], z);
+        mx[0] = fmaxf(mx[0], x); mx[1] = fmaxf(mx[1], y); mx[2] = fmaxf(mx[2], z);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bbox(PointSrc src, int64_t n, float tx, float ty, float tz,
+                                              BBoxAccum* part) {
+    float mn[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float mx[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    unsigned long long cnt = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += BBOX_UNROLL * stride) {
+#pragma unroll
+        for (int u = 0; u < BBOX_UNROLL; u++) bbox_add(src, i + u * stride, n, tx, ty, tz, mn, mx, cnt);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
-        mn0 = fminf(mn0, __shfl_xor(mn0, d, 64));
-        mn1 = fminf(mn1, __shfl_xor(mn1, d, 64));
-        mn2 = fminf(mn2, __shfl_xor(mn2, d, 64));
-        mx0 = fmaxf(mx0, __shfl_xor(mx0, d, 64));
-        mx1 = fmaxf(mx1, __shfl_xor(mx1, d, 64));
-        mx2 = fmaxf(mx2, __shfl_xor(mx2, d, 64));
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], d, 64));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], d, 64));
+        }
         cnt += __shfl_xor(cnt, d, 64);
     }
-    // block reduction, then one set of atomics per block: same-address
-    // atomics from every wave serialise in L2
     __shared__ float red[4][6];
     __shared__ unsigned long long redc[4];
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
-        red[w][0] = mn0; red[w][1] = mn1; red[w][2] = mn2;
-        red[w][3] = mx0; red[w][4] = mx1; red[w][5] = mx2;
+        for (int a = 0; a < 3; a++) red[w][a] = mn[a], red[w][3 + a] = mx[a];
         redc[w] = cnt;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
-            mn0 = fminf(mn0, red[i][0]); mn1 = fminf(mn1, red[i][1]); mn2 = fminf(mn2, red[i][2]);
-            mx0 = fmaxf(mx0, red[i][3]); mx1 = fmaxf(mx1, red[i][4]); mx2 = fmaxf(mx2, red[i][5]);
+            for (int a = 0; a < 3; a++) mn[a] = fminf(mn[a], red[i][a]), mx[a] = fmaxf(mx[a], red[i][3 + a]);
             cnt += redc[i];
         }
-        if (cnt > 0) {
-            atomicMin(&acc->mn[0], __float_as_int(mn0));
-            atomicMin(&acc->mn[1], __float_as_int(mn1));
-            atomicMin(&acc->mn[2], __float_as_int(mn2));
-            atomicMax(&acc->mx[0], __float_as_int(mx0));
-            atomicMax(&acc->mx[1], __float_as_int(mx1));
-            atomicMax(&acc->mx[2], __float_as_int(mx2));
-            atomicAdd(&acc->nvalid, cnt);
+        BBoxAccum b;
+        for (int a = 0; a < 3; a++) {
+            b.mn[a] = cnt ? __float_as_int(mn[a]) : 0x7F800000;
+            b.mx[a] = cnt ? __float_as_int(mx[a]) : -1;
         }
+        b.nvalid = cnt;
+        part[blockIdx.x] = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bbox_final(const BBoxAccum* part, int nparts, BBoxAccum* acc) {
+    int mn[3] = {0x7F800000, 0x7F800000, 0x7F800000}, mx[3] = {-1, -1, -1};
+    unsigned long long cnt = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const BBoxAccum b = part[i];
+        for (int a = 0; a < 3; a++) mn[a] = min(mn[a], b.mn[a]), mx[a] = max(mx[a], b.mx[a]);
+        cnt += b.nvalid;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = min(mn[a], __shfl_xor(mn[a], d, 64));
+            mx[a] = max(mx[a], __shfl_xor(mx[a], d, 64));
+        }
+        cnt += __shfl_xor(cnt, d, 64);
+    }
+    __shared__ int red[4][6];
+    __shared__ unsigned long long redc[4];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        for (int a = 0; a < 3; a++) red[w][a] = mn[a], red[w][3 + a] = mx[a];
+        redc[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+            for (int a = 0; a < 3; a++) mn[a] = min(mn[a], red[i][a]), mx[a] = max(mx[a], red[i][3 + a]);
+            cnt += redc[i];
+        }
+        for (int a = 0; a < 3; a++) acc->mn[a] = mn[a], acc->mx[a] = mx[a];
+        acc->nvalid = cnt;
     }
 }
 
@@ -414,53 +458,81 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(const GNode* leaves, int64_
 // point's box of gvdbCalcExtraBrickId (cuda_gvdb_particles.cuh:493-494) as a
 // 3x3x3 mask; the masks of one cell are OR-ed within the wave (points of a
 // cell are contiguous) and one atomicOr per cell run lands in reach0.
+// Bits dx+1 for dx in [lo, hi] (-1 <= lo, hi <= 1); 0 when hi < lo.
+__device__ __forceinline__ uint32_t reach_range3(int lo, int hi) {
+    return hi < lo ? 0u : ((2u << (hi + 1)) - 1u) ^ ((1u << (lo + 1)) - 1u);
+}
+
+// Bit (dz+1)*9 + (dy+1)*3 + (dx+1) for every cell offset in the box: the
+// per-axis bit sets spread to positions {0,1,2} x {0,3,6} x {0,9,18} and
+// multiplied (no two terms share a bit, so no carries).
+__device__ __forceinline__ uint32_t reach_mask(const int (&lo)[3], const int (&hi)[3]) {
+    const uint32_t bx = reach_range3(lo[0], hi[0]), by = reach_range3(lo[1], hi[1]), bz = reach_range3(lo[2], hi[2]);
+    const uint32_t ym = (by & 1u) | ((by & 2u) << 2) | ((by & 4u) << 4);
+    const uint32_t zm = (bz & 1u) | ((bz & 2u) << 8) | ((bz & 4u) << 16);
+    return bx * ym * zm;
+}
+
+constexpr int REACH_UNROLL = 4;
+
+// Each wave takes 64 consecutive sorted points REACH_UNROLL times per step
+// (loads issued together), ORs the masks of equal home cells across the
+// wave and sets each cell's bits with one atomicOr.
 __global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
                                                       ReachParams rp, uint32_t* reach0) {
     const int64_t total = min((int64_t)*total_ptr, nmax);
     const int lane = lane_id();
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = base + threadIdx.x;
-        int key = -1;
-        uint32_t m = 0;
-        if (j < total) {
-            float4 p = sorted[j];
-            float w[3] = {p.x, p.y, p.z};
-            int c[3], sh[3] = {0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += REACH_UNROLL * stride) {
+        float4 pts[REACH_UNROLL];
 #pragma unroll
-            for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
-            if (rp.cm.ncloud > 0) {
-                const int cl = rp.cm.cloud_of_point(__float_as_int(p.w));
+        for (int u = 0; u < REACH_UNROLL; u++) {
+            const int64_t j = base + u * stride + threadIdx.x;
+            if (j < total) pts[u] = sorted[j];
+        }
+#pragma unroll 1
+        for (int u = 0; u < REACH_UNROLL; u++) {
+            if (base + u * stride >= total) break;
+            const int64_t j = base + u * stride + threadIdx.x;
+            int key = -1;
+            uint32_t m = 0;
+            if (j < total) {
+                const float4 p = pts[u];
+                const float w[3] = {p.x, p.y, p.z};
+                int c[3], sh[3] = {0, 0, 0};
 #pragma unroll
-                for (int a = 0; a < 3; a++) sh[a] = rp.cm.cell_off[3 * cl + a];
-            }
-            if (rp.g0.contains(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2])) {
-                key = (int)rp.g0.index(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
-                m = 1u << 13;
-                if (rp.use_mask) {
-                    int lo[3], hi[3];
+                for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
+                if (rp.cm.ncloud > 0) {
+                    const int cl = rp.cm.cloud_of_point(__float_as_int(p.w));
 #pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        int ix = (int)w[a];
-                        int bmn = (int)((float)ix - rp.radius);
-                        int bmx = (int)((float)ix + 2.0f * rp.radius);
-                        lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
-                        hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
+                    for (int a = 0; a < 3; a++) sh[a] = rp.cm.cell_off[3 * cl + a];
+                }
+                if (rp.g0.contains(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2])) {
+                    key = (int)rp.g0.index(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
+                    m = 1u << 13;
+                    if (rp.use_mask) {
+                        int lo[3], hi[3];
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            int ix = (int)w[a];
+                            int bmn = (int)((float)ix - rp.radius);
+                            int bmx = (int)((float)ix + 2.0f * rp.radius);
+                            lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
+                            hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
+                        }
+                        m = reach_mask(lo, hi);
                     }
-                    m = 0;
-                    for (int dz = lo[2]; dz <= hi[2]; dz++)
-                        for (int dy = lo[1]; dy <= hi[1]; dy++)
-                            for (int dx = lo[0]; dx <= hi[0]; dx++) m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
                 }
             }
-        }
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t om = __shfl_down(m, d, 64);
-            int ok = __shfl_down(key, d, 64);
-            if (lane + d < 64 && ok == key) m |= om;
+            for (int d = 1; d < 64; d <<= 1) {
+                uint32_t om = __shfl_down(m, d, 64);
+                int ok = __shfl_down(key, d, 64);
+                if (lane + d < 64 && ok == key) m |= om;
+            }
+            int prev = __shfl_up(key, 1, 64);
+            if (key >= 0 && (lane == 0 || prev != key)) atomicOr(&reach0[key], m);
         }
-        int prev = __shfl_up(key, 1, 64);
-        if (key >= 0 && (lane == 0 || prev != key)) atomicOr(&reach0[key], m);
     }
 }
 
@@ -471,42 +543,85 @@ __global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, cons
 // into at most a few level-0 cells; the count is the sum over the box's
 // cells that are leaves of the product of their per-axis subcell counts (one
 // leaf test per cell instead of per subcell).
-__device__ __forceinline__ int subcells_in_cell(int lo, int hi, int c, int r0, int sc) {
-    // subcells x = lo, lo+sc, ..., hi (multiples of sc) with x >= 0 inside cell c
+__device__ __forceinline__ int subcells_in_cell(int lo, int hi, int c, int r0, int sc, int sc_shift) {
+    // subcells x = lo, lo+sc, ..., hi (multiples of sc = 2^sc_shift) with x >= 0 inside cell c
     const int a = max(max(lo, 0), c * r0), b = min(hi, c * r0 + r0 - sc);
-    return b >= a ? (b - a) / sc + 1 : 0;
+    return b >= a ? ((b - a) >> sc_shift) + 1 : 0;
 }
+
+// Subcell list entries of one point.  When the box's cells lie within one
+// cell of the point's home cell (the usual case: radius + subcell size within
+// a brick), the leaf tests come from the home cell's 27-bit neighbour-leaf
+// mask (one load, shared by the cell's points); otherwise each cell of the
+// box is tested in leafbits.
+__device__ __forceinline__ unsigned long long subcell_count(const float4& p, const BinParams& bp, int r0, int sc) {
+    const float w[3] = {p.x, p.y, p.z};
+    int lo[3], hi[3], c0[3], c1[3], hc[3];
+    bool near = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * sc;
+        hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * sc;
+        c0[a] = max(lo[a], 0) >> bp.logr0;
+        c1[a] = hi[a] >> bp.logr0;   // hi < 0: empty range (c1 < c0 or no subcell counted)
+        hc[a] = gvdbx_cell(w[a], bp.logr0);
+        near = near && c0[a] >= hc[a] - 1 && c1[a] <= hc[a] + 1;
+    }
+    unsigned long long cnt = 0;
+    if (near && bp.g0.contains(hc[0], hc[1], hc[2])) {
+        const uint32_t m = bp.nbmask[bp.g0.index(hc[0], hc[1], hc[2])];
+        int n[3][3];   // at most r0 / sc + 1 each: 24-bit products are exact
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) n[a][d] = subcells_in_cell(lo[a], hi[a], hc[a] + d - 1, r0, sc, bp.sc_shift);
+        uint32_t c32 = 0;
+#pragma unroll
+        for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const uint32_t nyz = __umul24(n[1][dy], n[2][dz]);
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++)
+                    c32 += ((m >> (dz * 9 + dy * 3 + dx)) & 1u) ? __umul24(n[0][dx], nyz) : 0u;
+            }
+        return c32;
+    }
+    for (int cx = c0[0]; cx <= c1[0]; cx++) {
+        const int nx = subcells_in_cell(lo[0], hi[0], cx, r0, sc, bp.sc_shift);
+        if (nx == 0) continue;
+        for (int cy = c0[1]; cy <= c1[1]; cy++) {
+            const int ny = subcells_in_cell(lo[1], hi[1], cy, r0, sc, bp.sc_shift);
+            if (ny == 0) continue;
+            for (int cz = c0[2]; cz <= c1[2]; cz++) {
+                const int nz = subcells_in_cell(lo[2], hi[2], cz, r0, sc, bp.sc_shift);
+                if (nz == 0 || !bp.g0.contains(cx, cy, cz)) continue;
+                const int64_t ci = bp.g0.index(cx, cy, cz);
+                if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) cnt += (unsigned long long)(nx * ny * nz);
+            }
+        }
+    }
+    return cnt;
+}
+
+// SUBCELL_UNROLL points per thread per step, their loads issued together.
+constexpr int SUBCELL_UNROLL = 4;
 
 __global__ __launch_bounds__(256) void k_subcell_len(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
                                                      BinParams bp, unsigned long long* len) {
     const int64_t total = min((int64_t)*total_ptr, nmax);
     const int r0 = 1 << bp.logr0, sc = bp.sc;
     unsigned long long lsum = 0;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
-        float4 p = sorted[j];
-        float w[3] = {p.x, p.y, p.z};
-        int lo[3], hi[3], c0[3], c1[3];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j0 < total; j0 += SUBCELL_UNROLL * stride) {
+        float4 pts[SUBCELL_UNROLL];
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * sc;
-            hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * sc;
-            c0[a] = max(lo[a], 0) >> bp.logr0;
-            c1[a] = hi[a] >> bp.logr0;   // hi < 0: empty range (c1 < c0 or no subcell counted)
+        for (int u = 0; u < SUBCELL_UNROLL; u++) {
+            const int64_t j = j0 + u * stride;
+            if (j < total) pts[u] = sorted[j];
         }
-        for (int cx = c0[0]; cx <= c1[0]; cx++) {
-            const int nx = subcells_in_cell(lo[0], hi[0], cx, r0, sc);
-            if (nx == 0) continue;
-            for (int cy = c0[1]; cy <= c1[1]; cy++) {
-                const int ny = subcells_in_cell(lo[1], hi[1], cy, r0, sc);
-                if (ny == 0) continue;
-                for (int cz = c0[2]; cz <= c1[2]; cz++) {
-                    const int nz = subcells_in_cell(lo[2], hi[2], cz, r0, sc);
-                    if (nz == 0 || !bp.g0.contains(cx, cy, cz)) continue;
-                    const int64_t ci = bp.g0.index(cx, cy, cz);
-                    if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) lsum += (unsigned long long)(nx * ny * nz);
-                }
-            }
-        }
+#pragma unroll 1
+        for (int u = 0; u < SUBCELL_UNROLL && j0 + u * stride < total; u++) lsum += subcell_count(pts[u], bp, r0, sc);
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) lsum += __shfl_xor(lsum, d, 64);
@@ -521,8 +636,8 @@ __global__ __launch_bounds__(256) void k_subcell_len(const float4* sorted, const
 
 // Binning by sort: the key of a point is its home cell in the bin grid
 // (ncell for points that are invalid or outside it, so they sort last), the
-// value its index.  A stable sort then leaves every bin in ascending point
-// order.
+// value (optional) its index.  A stable sort then leaves every bin in
+// ascending point order.
 __global__ __launch_bounds__(256) void k_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb,
                                                   int logr0, CloudMap cm, uint32_t* key, uint32_t* val) {
     const uint32_t none = (uint32_t)gb.count();
@@ -541,19 +656,33 @@ __global__ __launch_bounds__(256) void k_bin_keys(PointSrc src, int64_t n, float
             if (gb.contains(c[0], c[1], c[2])) k = (uint32_t)gb.index(c[0], c[1], c[2]);
         }
         key[i] = k;
-        val[i] = (uint32_t)i;
+        if (val) val[i] = (uint32_t)i;
     }
 }
+
+// BIN_UNROLL entries per thread per step: their keys and indices are loaded
+// together, then all their (random) point reads are in flight at once.
+constexpr int BIN_UNROLL = 4;
 
 __global__ __launch_bounds__(256) void k_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz,
                                                     const uint32_t* key, const uint32_t* val, uint32_t ncell,
                                                     float4* sorted) {
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        if (key[j] >= ncell) continue;
-        const int64_t i = val[j];
-        float w[3];
-        gvdbx_point(src, i, tx, ty, tz, w[0], w[1], w[2]);
-        sorted[j] = make_float4(w[0], w[1], w[2], __int_as_float((int)i));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j0 < n; j0 += BIN_UNROLL * stride) {
+        uint32_t kk[BIN_UNROLL], ii[BIN_UNROLL];
+#pragma unroll
+        for (int u = 0; u < BIN_UNROLL; u++) {
+            const int64_t j = j0 + u * stride;
+            kk[u] = j < n ? key[j] : ncell;
+            ii[u] = j < n ? val[j] : 0u;
+        }
+        float w[BIN_UNROLL][3];
+#pragma unroll
+        for (int u = 0; u < BIN_UNROLL; u++)
+            if (kk[u] < ncell) gvdbx_point(src, ii[u], tx, ty, tz, w[u][0], w[u][1], w[u][2]);
+#pragma unroll
+        for (int u = 0; u < BIN_UNROLL; u++)
+            if (kk[u] < ncell) sorted[j0 + u * stride] = make_float4(w[u][0], w[u][1], w[u][2], __int_as_float((int)ii[u]));
     }
 }
 
@@ -579,6 +708,22 @@ __global__ __launch_bounds__(256) void k_leaf_bits(const int32_t* idx0, int64_t 
             if (c < ncell && idx0[c] >= 0) b |= 1u << k;
         }
         bits[wi] = b;
+    }
+}
+
+// Per level-0 cell, bit (dz+1)*9 + (dy+1)*3 + (dx+1) set when the cell at
+// offset (dx, dy, dz) is a leaf.
+__global__ __launch_bounds__(256) void k_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask) {
+    const int64_t n = g0.count();
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(c % g0.nz) + g0.oz, y = (int)((c / g0.nz) % g0.ny) + g0.oy;
+        const int x = (int)(c / ((int64_t)g0.nz * g0.ny)) + g0.ox;
+        uint32_t m = 0;
+        for (int d = 0; d < 27; d++) {
+            const int nx = x + d % 3 - 1, ny = y + (d / 3) % 3 - 1, nz = z + d / 9 - 1;
+            if (g0.contains(nx, ny, nz) && idx0[g0.index(nx, ny, nz)] >= 0) m |= 1u << d;
+        }
+        mask[c] = m;
     }
 }
 
@@ -723,8 +868,13 @@ int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 void launch_bbox_init(BBoxAccum* acc, hipStream_t s) { hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, s, acc); }
 
-void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_bbox, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, src, n, tx, ty, tz, acc);
+int bbox_parts(int64_t n) { return (int)grid_for(n, 256 * BBOX_UNROLL, BBOX_BLOCKS); }
+
+void launch_bbox(PointSrc src, int64_t n, float tx, float ty, float tz, BBoxAccum* acc, BBoxAccum* parts,
+                 hipStream_t s) {
+    const int np = bbox_parts(n);
+    hipLaunchKernelGGL(k_bbox, dim3(np), dim3(256), 0, s, src, n, tx, ty, tz, parts);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, parts, np, acc);
 }
 
 void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid g0, int range0,
@@ -804,7 +954,8 @@ void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t*
 void launch_reach_sorted(const float4* sorted, const int32_t* total, int64_t nmax, ReachParams rp, uint32_t* reach0,
                          hipStream_t s) {
     if (nmax > 0)
-        hipLaunchKernelGGL(k_reach_sorted, dim3(grid_for(nmax, 256)), dim3(256), 0, s, sorted, total, nmax, rp, reach0);
+        hipLaunchKernelGGL(k_reach_sorted, dim3(grid_for(nmax, 256 * REACH_UNROLL)), dim3(256), 0, s, sorted, total, nmax, rp,
+                           reach0);
 }
 
 void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax, BinParams bp,
@@ -823,8 +974,8 @@ void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, Cell
 void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
                        const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s) {
     if (n > 0)
-        hipLaunchKernelGGL(k_bin_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, tx, ty, tz, key, val, ncell,
-                           sorted);
+        hipLaunchKernelGGL(k_bin_gather, dim3(grid_for(n, 256 * BIN_UNROLL)), dim3(256), 0, s, src, n, tx, ty, tz, key,
+                           val, ncell, sorted);
 }
 
 void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* off, hipStream_t s) {
@@ -833,6 +984,10 @@ void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* 
 
 void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s) {
     hipLaunchKernelGGL(k_leaf_bits, dim3(grid_for((ncell + 31) / 32, 256)), dim3(256), 0, s, idx0, ncell, bits);
+}
+
+void launch_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask, hipStream_t s) {
+    if (g0.count() > 0) hipLaunchKernelGGL(k_leaf_nbmask, dim3(grid_for(g0.count(), 256)), dim3(256), 0, s, idx0, g0, mask);
 }
 
 

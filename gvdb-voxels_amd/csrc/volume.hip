@@ -83,7 +83,7 @@ struct gvdbx_volume {
     CellGrid grid[GVDBX_MAXLEV] = {};
     DevBuf nodes[GVDBX_MAXLEV + 1], clist[GVDBX_MAXLEV + 1], idx[GVDBX_MAXLEV], flag_base[GVDBX_MAXLEV],
         flag_all[GVDBX_MAXLEV];
-    DevBuf reach0, leafbits, partials, totals, bbox, misc;
+    DevBuf reach0, leafbits, nbmask, partials, totals, bbox, misc;
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int gather_split = GATHER_SPLIT_DEFAULT;
@@ -365,9 +365,11 @@ int subcell_len(gvdbx_volume* v, int sc, float radius, int64_t* len_out) {
     hipStream_t s = v->stream;
     const CellGrid& g0 = v->grid[0];
     TRY(ensure(v, v->leafbits, (size_t)((g0.count() + 31) / 32) * 4));
+    TRY(ensure(v, v->nbmask, (size_t)std::max<int64_t>(g0.count(), 1) * 4));
     TRY(ensure(v, v->misc, 256));
     TRY(ensure_host(v, 4096));
     launch_leaf_bits(v->idx[0].as<int32_t>(), g0.count(), v->leafbits.as<uint32_t>(), s);
+    launch_leaf_nbmask(v->idx[0].as<int32_t>(), g0, v->nbmask.as<uint32_t>(), s);
     BinParams bp{};
     bp.gb = v->gbin;
     bp.range0 = v->res[0];
@@ -377,6 +379,7 @@ int subcell_len(gvdbx_volume* v, int sc, float radius, int64_t* len_out) {
     bp.radius = radius;
     bp.g0 = g0;
     bp.leafbits = v->leafbits.as<uint32_t>();
+    bp.nbmask = v->nbmask.as<uint32_t>();
     unsigned long long* len = (unsigned long long*)((char*)v->misc.p + 64);
     launch_fill_u64((uint64_t*)len, 1, 0ull, s);
     launch_subcell_len(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), v->bins_n, bp, len, s);
@@ -444,10 +447,10 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     float pmin[3], pmax[3];
     int lo_v[3], hi_v[3];
     if (v->ncloud == 0) {
-        TRY(ensure(v, v->bbox, sizeof(BBoxAccum)));
+        TRY(ensure(v, v->bbox, sizeof(BBoxAccum) * (1 + (size_t)bbox_parts(std::max<int64_t>(n, 1)))));
         TRY(ensure_host(v, 4096));
         launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
-        if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), s);
+        if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), v->bbox.as<BBoxAccum>() + 1, s);
         TRY(check_launch("bounding box"));
         GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
         TRY(sync(v));
@@ -961,7 +964,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
         free_buf(v->flag_base[l]);
         free_buf(v->flag_all[l]);
     }
-    DevBuf* bufs[] = {&v->reach0, &v->leafbits, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
+    DevBuf* bufs[] = {&v->reach0, &v->leafbits, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
                       &v->bin_off, &v->sorted, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
