@@ -777,6 +777,69 @@ __global__ __launch_bounds__(256) void k_apron_adjoint(ApronParams ap, float* gr
     }
 }
 
+// 8^3 bricks with apron 1: the 296 interior voxels on the brick's boundary
+// layer (the only ones copied into a neighbour's apron), ascending.
+struct AdjTab10 {
+    uint16_t i[296];
+};
+constexpr AdjTab10 make_adj10() {
+    AdjTab10 t{};
+    int n = 0;
+    for (int i = 0; i < 1000; i++) {
+        const int x = i % 10, y = (i / 10) % 10, z = i / 100;
+        const bool in = x >= 1 && x <= 8 && y >= 1 && y <= 8 && z >= 1 && z <= 8;
+        if (in && (x == 1 || x == 8 || y == 1 || y == 8 || z == 1 || z == 8)) t.i[n++] = (uint16_t)i;
+    }
+    return t;
+}
+__constant__ AdjTab10 ADJ10 = make_adj10();
+
+// One wave per brick, XCD-contiguous runs of bricks; each lane's (up to 5)
+// voxels issue all their neighbour reads before summing.  Same terms in the
+// same order (dz, dy, dx ascending) as k_apron_adjoint.
+__global__ __launch_bounds__(256) void k_apron_adjoint10(ApronParams ap, float* grad, int64_t nblk) {
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = gvdbx_xcd_item(blockIdx.x, nblk);
+    if (blk < 0) return;
+    const int64_t k = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (k >= ap.nleaf) return;
+    const GNode nd = ap.leaves[k];
+    int32_t leaf = -1;
+    if (lane < 27) {
+        const int cx = nd.mPos[0] >> 3, cy = nd.mPos[1] >> 3, cz = nd.mPos[2] >> 3;
+        const int nx = cx + lane % 3 - 1, ny = cy + (lane / 3) % 3 - 1, nz = cz + lane / 9 - 1;
+        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        if (leaf >= ap.nleaf) leaf = -1;
+    }
+    int nbl[27];
+#pragma unroll
+    for (int d = 0; d < 27; d++) nbl[d] = __builtin_amdgcn_readlane(leaf, d);
+    float* brick = grad + k * 1000;
+#pragma unroll
+    for (int u = 0; u < 5; u++) {
+        const int j = lane + 64 * u;
+        if (j >= 296) break;
+        const int i = ADJ10.i[j];
+        const int x = i % 10, y = (i / 10) % 10, z = i / 100;
+        const int lx = x == 1 ? -1 : 0, hx = x == 8 ? 1 : 0;
+        const int ly = y == 1 ? -1 : 0, hy = y == 8 ? 1 : 0;
+        const int lz = z == 1 ? -1 : 0, hz = z == 8 ? 1 : 0;
+        float v[27];
+        bool on[27];
+#pragma unroll
+        for (int d = 0; d < 27; d++) {
+            const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+            on[d] = d != 13 && dx >= lx && dx <= hx && dy >= ly && dy <= hy && dz >= lz && dz <= hz && nbl[d] >= 0;
+            v[d] = on[d] ? grad[(int64_t)nbl[d] * 1000 + i - 8 * (dz * 100 + dy * 10 + dx)] : 0.0f;
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 27; d++)
+            if (on[d]) acc += v[d];
+        brick[i] += acc;
+    }
+}
+
 // Adjoint of the cubic splat, one thread per bin-sorted point:
 //   dL/dw = sum_v g(v) t^3,  dL/dp = sum_v g(v) w 6 t^2 (v - p) / r^2,
 //   t = 1 - |v-p|^2/r^2, over integer voxels v of active leaves, |v-p| < r.
@@ -900,7 +963,10 @@ void launch_ea_backward(EABackParams p, hipStream_t s) {
 
 void launch_apron_adjoint(ApronParams ap, float* grad, hipStream_t s) {
     if (ap.nleaf <= 0) return;
-    if (ap.brick_res == 10)
+    if (ap.brick_res == 10 && ap.range0 == 8)
+        hipLaunchKernelGGL(k_apron_adjoint10, dim3((unsigned)gvdbx_xcd_blocks((ap.nleaf + 3) / 4)), dim3(256), 0, s, ap,
+                           grad, (ap.nleaf + 3) / 4);
+    else if (ap.brick_res == 10)
         hipLaunchKernelGGL(k_apron_adjoint<10>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap, grad);
     else
         hipLaunchKernelGGL(k_apron_adjoint<0>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap, grad);
