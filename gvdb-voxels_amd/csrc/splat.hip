@@ -365,6 +365,20 @@ __device__ __forceinline__ GTile gather_tile(const GatherParams& gp, int res0, i
     return g;
 }
 
+// Bit ly * td + lx for every column (lx, ly) of the tile with lx in [xlo, xhi]
+// and ly in [ylo, yhi] (non-empty ranges inside the tile).
+__device__ __forceinline__ uint64_t column_mask(int xlo, int xhi, int ylo, int yhi, int td) {
+    const uint64_t xb = ((2ull << xhi) - 1ull) & ~((1ull << xlo) - 1ull);
+    if (td == 8) {
+        // one byte per row: replicate the row bits, keep rows ylo .. yhi
+        const uint64_t rows = ((2ull << (8 * yhi + 7)) - 1ull) & ~((1ull << (8 * ylo)) - 1ull);
+        return (xb * 0x0101010101010101ull) & rows;
+    }
+    uint64_t m = 0;
+    for (int y = ylo; y <= yhi; y++) m |= xb << (y * td);
+    return m;
+}
+
 // Folds chunks s, s + ns, ... of the tile's candidates into m (lane = one
 // (x, y) column of the tile, m[z] = bits of min |p-v|^2 over its z voxels;
 // non-negative floats order like their bit patterns).  ns = plan(ns) is
@@ -405,12 +419,12 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
             // monotone, so every reached voxel's |p-v|^2 >= db2).  Same per lane.
             uint32_t lmax = 0;
             if (lane_ok) {
-    #pragma unroll
+#pragma unroll
                 for (int z = 0; z < 8; z++)
                     if (z < td) lmax = max(lmax, m[z]);
             }
             uint32_t wmax = lmax;
-    #pragma unroll
+#pragma unroll
             for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, o));
             const float bound = fminf(cut2, __uint_as_float(wmax));
             const float lbound = fminf(cut2, __uint_as_float(lmax));
@@ -419,12 +433,14 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
             // The packed range is the membership range clipped to the tile (z) and to
             // membership + cutoff (x, y: lanes outside it are masked).
             bool ok = cj < cend;
-            uint32_t pk = 0;
+            uint32_t pk = 0;   // z range inside the tile: lo | hi << 3
+            uint64_t cm = 0;   // tile columns in the x/y range: bit ly * td + lx (= their lane)
             {
                 float pw[3] = {cur.x, cur.y, cur.z};
                 int T0[3] = {T0x, T0y, T0z};
+                int rlo[2] = {0, 0}, rhi[2] = {0, 0};
                 float db2 = 0.0f;
-    #pragma unroll
+#pragma unroll
                 for (int a = 0; a < 3; a++) {
                     int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
                     int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
@@ -434,14 +450,19 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                     ok = ok && lr <= hr;
                     float e = fmaxf(fmaxf((float)(T0[a] + lr) - pw[a], pw[a] - (float)(T0[a] + hr)), 0.0f);
                     db2 = __builtin_fmaf(e, e, db2);
-                    if (HALVES && a == 2) {
-                        // z: membership only (voxels beyond the cutoff cannot change the result)
-                        lr = max(lo - T0[a], 0);
-                        hr = min(hi - T0[a], td - 1);
+                    if (a < 2) {
+                        rlo[a] = lr, rhi[a] = hr;
+                    } else {
+                        if (HALVES) {
+                            // z: membership only (voxels beyond the cutoff cannot change the result)
+                            lr = max(lo - T0[a], 0);
+                            hr = min(hi - T0[a], td - 1);
+                        }
+                        pk = (uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3);
                     }
-                    pk |= ((uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3)) << (6 * a);
                 }
                 ok = ok && db2 <= bound;
+                if (ok) cm = column_mask(rlo[0], rhi[0], rlo[1], rhi[1], td);
             }
             uint64_t mask = __ballot(ok);
             while (mask) {
@@ -451,9 +472,11 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), b));
                 const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), b));
                 const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pk, b);
-                const int xlo = q & 7, xhi = (q >> 3) & 7, ylo = (q >> 6) & 7, yhi = (q >> 9) & 7;
-                const int zlo = (q >> 12) & 7, zhi = (q >> 15) & 7;
-                if (lane_ok && lx >= xlo && lx <= xhi && ly >= ylo && ly <= yhi) {
+                const uint64_t cols = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cm, b) |
+                                      ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cm >> 32), b) << 32);
+                const int zlo = q & 7, zhi = (q >> 3) & 7;
+                // the lanes whose column the candidate reaches, straight from the mask
+                if (__builtin_amdgcn_inverse_ballot_w64(cols)) {
                     const float dx = vx - px, dy = vy - py;
                     const float dxy = __builtin_fmaf(dy, dy, dx * dx);
                     if (dxy <= lbound) {
@@ -461,21 +484,21 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                             // with sc >= 4 the z membership inside an 8-voxel tile is
                             // one or both 4-voxel halves
                             if (zlo < 4) {
-    #pragma unroll
+#pragma unroll
                                 for (int z = 0; z < 4; z++) {
                                     const float dz = vz[z] - pz;
                                     m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
                                 }
                             }
                             if (zhi >= 4) {
-    #pragma unroll
+#pragma unroll
                                 for (int z = 4; z < 8; z++) {
                                     const float dz = vz[z] - pz;
                                     m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
                                 }
                             }
                         } else {
-    #pragma unroll
+#pragma unroll
                             for (int z = 0; z < 8; z++) {
                                 if (z >= zlo && z <= zhi) {
                                     const float dz = vz[z] - pz;
