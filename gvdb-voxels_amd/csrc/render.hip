@@ -212,6 +212,15 @@ __device__ __forceinline__ V3 node_pos(const GNode* nodes, int64_t id, const int
     return v3((float)(q.y - sh[0]), (float)(q.z - sh[1]), (float)(q.w - sh[2]));
 }
 
+// Attenuation of one sample, e^(extinct * s * step), as 2^(k s) with
+// k = extinct * step * log2(e), on the hardware exp2 (about 1 ulp; rounding
+// k s adds |k s| 2^-24 relative).  The forward and the replay of the adjoint
+// both use it, so the replay sees the forward's values.
+__device__ __forceinline__ float ea_log2_atten(const RenderScene& sc) {
+    return (sc.extinct * sc.step) * 1.44269504088896341f;
+}
+__device__ __forceinline__ float ea_atten(float k, float s) { return __builtin_amdgcn_exp2f(k * s); }
+
 template <int MODE, int BR>
 __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene& sc, int64_t leaf, V3 t, V3 pos,
                                          V3 dir, RayState& st, const int* sh) {
@@ -221,6 +230,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
     const float r0 = (float)tv.res[0];
     if (MODE == MODE_DEEP_LUT || MODE == MODE_DEEP_LINEAR) {
         const float step = sc.step;
+        const float ek = ea_log2_atten(sc);
         t.x = step * ceilf(t.x / step);
         V3 p = sub(add(pos, scale(dir, t.x)), vmin);
         const V3 wpt = scale(dir, step);
@@ -243,7 +253,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
                     val = sc.lut[transfer_index(sc, s)];
                 else
                     val = make_float4(s, s, s, s);
-                val.w = expf(sc.extinct * val.w * step);
+                val.w = ea_atten(ek, val.w);
                 st.clr[0] += val.x * st.clr[3] * (1 - val.w) * sc.albedo;
                 st.clr[1] += val.y * st.clr[3] * (1 - val.w) * sc.albedo;
                 st.clr[2] += val.z * st.clr[3] * (1 - val.w) * sc.albedo;
@@ -618,6 +628,7 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     const RenderScene& sc = p.sc;
     const float r0 = (float)p.tv.res[0];
     const float e = sc.extinct * sc.step;
+    const float ek = ea_log2_atten(sc);
     const uint32_t npix = (uint32_t)p.cam.width * (uint32_t)p.cam.height;
     auto replay = [&](int32_t c) {
         const uint32_t id = p.idx[c];
@@ -669,7 +680,7 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
             const float s = trilinear_in<BR>(dens, q, ix, iy, iz, ax, ay, az);
             const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
             if (s >= sc.minval) {
-                const float a = expf(sc.extinct * s * step);
+                const float a = ea_atten(ek, s);
                 I += s * T * (1 - a) * sc.albedo;
                 const float gq = gi * (sc.albedo * T * ((1.0f - a) - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
                 T *= a;
