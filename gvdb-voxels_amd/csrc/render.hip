@@ -704,9 +704,11 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
                 }
                 cx = ix, cy = iy, cz = iz;
                 have = true;
-                const float gyz[4] = {gq * by * bz, gq * ay * bz, gq * by * az, gq * ay * az};
+                // gradient only (no decision depends on it): fused products
+                const float gz0 = gq * bz, gz1 = gq * az;
+                const float gyz[4] = {gz0 * by, gz0 * ay, gz1 * by, gz1 * ay};
 #pragma unroll
-                for (int m = 0; m < 8; m++) acc[m] += gyz[m >> 1] * ((m & 1) ? ax : bx);
+                for (int m = 0; m < 8; m++) acc[m] = __builtin_fmaf(gyz[m >> 1], (m & 1) ? ax : bx, acc[m]);
             }
             q = add(q, wpt);
         }
