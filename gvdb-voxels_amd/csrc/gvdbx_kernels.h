@@ -70,7 +70,23 @@ struct LinkParams {
     const int32_t* idx_p;
     int64_t nparent;
 };
-void launch_link(LinkParams lp, hipStream_t s);
+// the links of up to GVDBX_MAXLEV levels: level i links children [start[i], start[i+1])
+struct LinkBatch {
+    LinkParams lv[GVDBX_MAXLEV];
+    int64_t start[GVDBX_MAXLEV + 1];
+    int n;
+};
+void launch_link(const LinkBatch& lb, hipStream_t s);
+// child lists list[i][0 .. start[i+1]-start[i]) set to "no child"; root copied to root_dst (if set)
+struct ClistInit {
+    uint64_t* list[GVDBX_MAXLEV];
+    int64_t start[GVDBX_MAXLEV + 1];
+    int n;
+    GNode root;
+    GNode* root_dst;
+};
+void launch_clist_init(const ClistInit& ci, hipStream_t s);
+void launch_bounds_init(int32_t* out6, hipStream_t s);
 
 // leaf k -> mValue (closed form of getAtlasPos) and atlas map slot k
 void launch_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int brick_res, int apron,
@@ -87,7 +103,6 @@ struct BinParams {
     int sc;
     float radius;
     CellGrid g0;
-    const uint32_t* leafbits;   // one bit per g0 cell
     const uint32_t* nbmask;     // per g0 cell: 27-bit neighbour-leaf mask (launch_leaf_nbmask)
 };
 // reach masks of level-0 cells from cell-sorted points (total = off[gb.count()])
@@ -99,8 +114,8 @@ void launch_subcell_len(const float4* sorted, const int32_t* total, int64_t nmax
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s);
 void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
                              hipStream_t s);
-void launch_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask, hipStream_t s);
-void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s);
+// also sets *reset = 0 when reset is non-null
+void launch_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask, unsigned long long* reset, hipStream_t s);
 
 // ---------------- points -> voxels ----------------
 struct GatherParams {

@@ -370,9 +370,7 @@ __global__ __launch_bounds__(256) void k_fill(T* p, int64_t n, T v) {
 }
 
 // ---- link (gvdbLinkBricks cuda_gvdb_nodes.cuh:470-505 for levels below top_lev) ----
-__global__ __launch_bounds__(256) void k_link(LinkParams lp) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= lp.nchild) return;
+__device__ __forceinline__ void link_one(const LinkParams& lp, int64_t k) {
     const GNode& c = lp.child[k];
     int64_t p = 0;
     if (!lp.parent_is_root) {
@@ -393,6 +391,30 @@ __global__ __launch_bounds__(256) void k_link(LinkParams lp) {
     uint64_t cndx = pn.mChildList >> 16;
     int64_t len = (int64_t)lp.res_p * lp.res_p * lp.res_p;
     lp.parent_clist[(int64_t)cndx * len + bit] = ((uint64_t)k << 16) | ((uint64_t)lp.lev << 8);
+}
+
+// Every level's links in one launch (levels write disjoint lists and parents).
+__global__ __launch_bounds__(256) void k_link(LinkBatch lb) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < lb.start[lb.n]; g += (int64_t)gridDim.x * blockDim.x) {
+        int i = 0;
+        while (g >= lb.start[i + 1]) i++;
+        link_one(lb.lv[i], g - lb.start[i]);
+    }
+}
+
+// Child lists of levels 1.. set to "no child" and the root node written, in one launch.
+__global__ __launch_bounds__(256) void k_clist_init(ClistInit ci) {
+    if (ci.root_dst && blockIdx.x == 0 && threadIdx.x == 0) *ci.root_dst = ci.root;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ci.start[ci.n]; g += (int64_t)gridDim.x * blockDim.x) {
+        int i = 0;
+        while (g >= ci.start[i + 1]) i++;
+        ci.list[i][g - ci.start[i]] = GVDBX_ID_UNDEF64;
+    }
+}
+
+// Leaf bounds accumulator: min (3) and max (3) set to their identities.
+__global__ void k_bounds_init(int32_t* out6) {
+    if (threadIdx.x < 6) out6[threadIdx.x] = threadIdx.x < 3 ? 0x7FFFFFFF : INT32_MIN;
 }
 
 // ---- atlas (UpdateAtlas VG.cpp:2630-2704, getAtlasPos AL.cpp:705-715) ----
@@ -553,7 +575,7 @@ __device__ __forceinline__ int subcells_in_cell(int lo, int hi, int c, int r0, i
 // cell of the point's home cell (the usual case: radius + subcell size within
 // a brick), the leaf tests come from the home cell's 27-bit neighbour-leaf
 // mask (one load, shared by the cell's points); otherwise each cell of the
-// box is tested in leafbits.
+// box is tested (bit 13 of its own mask: the cell itself).
 __device__ __forceinline__ unsigned long long subcell_count(const float4& p, const BinParams& bp, int r0, int sc) {
     const float w[3] = {p.x, p.y, p.z};
     int lo[3], hi[3], c0[3], c1[3], hc[3];
@@ -597,7 +619,7 @@ __device__ __forceinline__ unsigned long long subcell_count(const float4& p, con
                 const int nz = subcells_in_cell(lo[2], hi[2], cz, r0, sc, bp.sc_shift);
                 if (nz == 0 || !bp.g0.contains(cx, cy, cz)) continue;
                 const int64_t ci = bp.g0.index(cx, cy, cz);
-                if ((bp.leafbits[ci >> 5] >> (ci & 31)) & 1u) cnt += (unsigned long long)(nx * ny * nz);
+                if ((bp.nbmask[ci] >> 13) & 1u) cnt += (unsigned long long)(nx * ny * nz);   // the cell itself
             }
         }
     }
@@ -699,22 +721,12 @@ __global__ __launch_bounds__(256) void k_bin_offsets(const uint32_t* key, int64_
     }
 }
 
-__global__ __launch_bounds__(256) void k_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits) {
-    int64_t nw = (ncell + 31) / 32;
-    for (int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < nw; wi += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t b = 0;
-        for (int k = 0; k < 32; k++) {
-            int64_t c = wi * 32 + k;
-            if (c < ncell && idx0[c] >= 0) b |= 1u << k;
-        }
-        bits[wi] = b;
-    }
-}
-
 // Per level-0 cell, bit (dz+1)*9 + (dy+1)*3 + (dx+1) set when the cell at
 // offset (dx, dy, dz) is a leaf.
-__global__ __launch_bounds__(256) void k_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask) {
+__global__ __launch_bounds__(256) void k_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask,
+                                                     unsigned long long* reset) {
     const int64_t n = g0.count();
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0ull;   // a counter the next pass adds into
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         const int z = (int)(c % g0.nz) + g0.oz, y = (int)((c / g0.nz) % g0.ny) + g0.oy;
         const int x = (int)(c / ((int64_t)g0.nz * g0.ny)) + g0.ox;
@@ -936,9 +948,16 @@ void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
 }
 
-void launch_link(LinkParams lp, hipStream_t s) {
-    if (lp.nchild > 0) hipLaunchKernelGGL(k_link, dim3(grid_for(lp.nchild, 256, 1 << 30)), dim3(256), 0, s, lp);
+void launch_link(const LinkBatch& lb, hipStream_t s) {
+    if (lb.n > 0 && lb.start[lb.n] > 0)
+        hipLaunchKernelGGL(k_link, dim3(grid_for(lb.start[lb.n], 256)), dim3(256), 0, s, lb);
 }
+
+void launch_clist_init(const ClistInit& ci, hipStream_t s) {
+    hipLaunchKernelGGL(k_clist_init, dim3(grid_for(std::max<int64_t>(ci.start[ci.n], 1), 256)), dim3(256), 0, s, ci);
+}
+
+void launch_bounds_init(int32_t* out6, hipStream_t s) { hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, out6); }
 
 void launch_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int brick_res, int apron, GAtlasNode* amap,
                          int64_t nmap, hipStream_t s) {
@@ -982,12 +1001,9 @@ void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* 
     hipLaunchKernelGGL(k_bin_offsets, dim3(grid_for(ncell + 1, 256)), dim3(256), 0, s, key, n, ncell, off);
 }
 
-void launch_leaf_bits(const int32_t* idx0, int64_t ncell, uint32_t* bits, hipStream_t s) {
-    hipLaunchKernelGGL(k_leaf_bits, dim3(grid_for((ncell + 31) / 32, 256)), dim3(256), 0, s, idx0, ncell, bits);
-}
-
-void launch_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask, hipStream_t s) {
-    if (g0.count() > 0) hipLaunchKernelGGL(k_leaf_nbmask, dim3(grid_for(g0.count(), 256)), dim3(256), 0, s, idx0, g0, mask);
+void launch_leaf_nbmask(const int32_t* idx0, CellGrid g0, uint32_t* mask, unsigned long long* reset, hipStream_t s) {
+    hipLaunchKernelGGL(k_leaf_nbmask, dim3(grid_for(std::max<int64_t>(g0.count(), 1), 256)), dim3(256), 0, s, idx0, g0,
+                       mask, reset);
 }
 
 

@@ -77,13 +77,13 @@ struct gvdbx_volume {
     int root_pos[3] = {0, 0, 0};
     float pos_min[3] = {}, pos_max[3] = {}, obj_min[3] = {}, obj_max[3] = {};
     bool bounds_pending = false;   // obj_min/max are in flight to pin + PIN_BOUNDS
-    void* pin = nullptr;           // small pinned buffer of its own (root node upload, bounds readback)
+    void* pin = nullptr;           // small pinned buffer of its own (bounds readback)
     int64_t cnt[GVDBX_MAXLEV] = {};
     int64_t nbase[GVDBX_MAXLEV] = {};
     CellGrid grid[GVDBX_MAXLEV] = {};
     DevBuf nodes[GVDBX_MAXLEV + 1], clist[GVDBX_MAXLEV + 1], idx[GVDBX_MAXLEV], flag_base[GVDBX_MAXLEV],
         flag_all[GVDBX_MAXLEV];
-    DevBuf reach0, leafbits, nbmask, partials, totals, bbox, misc;
+    DevBuf reach0, nbmask, partials, totals, bbox, misc;
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int gather_split = GATHER_SPLIT_DEFAULT;
@@ -175,11 +175,10 @@ int ensure(gvdbx_volume* v, DevBuf& b, size_t bytes) {
     return GVDBX_OK;
 }
 
-// Layout of gvdbx_volume::pin: the root node staged for upload, then the six
-// leaf-bound integers read back by FinishTopology.  Neither is ever written
-// while a copy of it may be in flight (each RebuildTopology and each bounds
+// Layout of gvdbx_volume::pin: the six leaf-bound integers read back by
+// FinishTopology (never read while the copy may be in flight: the bounds
 // resolution synchronises first).
-constexpr size_t PIN_ROOT = 0, PIN_BOUNDS = 128, PIN_BYTES = 256;
+constexpr size_t PIN_BOUNDS = 0, PIN_BYTES = 64;
 
 int ensure_pin(gvdbx_volume* v) {
     if (v->pin) return GVDBX_OK;
@@ -364,12 +363,11 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable) {
 int subcell_len(gvdbx_volume* v, int sc, float radius, int64_t* len_out) {
     hipStream_t s = v->stream;
     const CellGrid& g0 = v->grid[0];
-    TRY(ensure(v, v->leafbits, (size_t)((g0.count() + 31) / 32) * 4));
     TRY(ensure(v, v->nbmask, (size_t)std::max<int64_t>(g0.count(), 1) * 4));
     TRY(ensure(v, v->misc, 256));
     TRY(ensure_host(v, 4096));
-    launch_leaf_bits(v->idx[0].as<int32_t>(), g0.count(), v->leafbits.as<uint32_t>(), s);
-    launch_leaf_nbmask(v->idx[0].as<int32_t>(), g0, v->nbmask.as<uint32_t>(), s);
+    unsigned long long* len = (unsigned long long*)((char*)v->misc.p + 64);
+    launch_leaf_nbmask(v->idx[0].as<int32_t>(), g0, v->nbmask.as<uint32_t>(), len, s);   // also len = 0
     BinParams bp{};
     bp.gb = v->gbin;
     bp.range0 = v->res[0];
@@ -378,10 +376,7 @@ int subcell_len(gvdbx_volume* v, int sc, float radius, int64_t* len_out) {
     bp.sc_shift = __builtin_ctz((unsigned)sc);
     bp.radius = radius;
     bp.g0 = g0;
-    bp.leafbits = v->leafbits.as<uint32_t>();
     bp.nbmask = v->nbmask.as<uint32_t>();
-    unsigned long long* len = (unsigned long long*)((char*)v->misc.p + 64);
-    launch_fill_u64((uint64_t*)len, 1, 0ull, s);
     launch_subcell_len(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), v->bins_n, bp, len, s);
     TRY(check_launch("subcell length"));
     GVDBX_CHECK(hipMemcpyAsync(v->host, len, 8, hipMemcpyDeviceToHost, s));
@@ -667,18 +662,25 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     }
     rn.mParent = GVDBX_ID_UNDEFL;
     rn.mChildList = 1ull | ((uint64_t)depth << 8);
-    TRY(ensure_pin(v));
-    memcpy((char*)v->pin + PIN_ROOT, &rn, sizeof(rn));
-    GVDBX_CHECK(hipMemcpyAsync(v->nodes[depth].p, (char*)v->pin + PIN_ROOT, sizeof(rn), hipMemcpyHostToDevice, s));
     // top level (PrepareVDBPartially VG.cpp:3996-4012)
     int tlev = 1;
     for (int l = v->levs - 1; l >= 0; l--)
         if (cnt[l] == 1) tlev = l;
+    // root node and every child list ("no child") in one launch
+    ClistInit ci{};
+    ci.root = rn;
+    ci.root_dst = v->nodes[depth].as<GNode>();
+    ci.start[0] = 0;
     for (int l = 1; l <= depth; l++) {
-        int64_t len = (int64_t)v->res[l] * v->res[l] * v->res[l];
-        launch_fill_u64(v->clist[l].as<uint64_t>(), cnt[l] * len, GVDBX_ID_UNDEF64, s);
+        const int64_t len = (int64_t)v->res[l] * v->res[l] * v->res[l];
+        ci.list[ci.n] = v->clist[l].as<uint64_t>();
+        ci.start[ci.n + 1] = ci.start[ci.n] + cnt[l] * len;
+        ci.n++;
     }
-    // links (gvdbLinkBricks, levels below top_lev, top-down)
+    launch_clist_init(ci, s);
+    // links (gvdbLinkBricks, levels below top_lev), all levels in one launch
+    LinkBatch lb{};
+    lb.start[0] = 0;
     for (int l = depth - 1; l >= 0; l--) {
         if (l >= tlev) continue;
         LinkParams lp{};
@@ -697,8 +699,11 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
             lp.idx_p = v->idx[l + 1].as<int32_t>();
         }
         lp.nparent = cnt[l + 1];
-        launch_link(lp, s);
+        lb.lv[lb.n] = lp;
+        lb.start[lb.n + 1] = lb.start[lb.n] + cnt[l];
+        lb.n++;
     }
+    launch_link(lb, s);
     if (v->ncloud > 0) {
         TRY(ensure(v, v->c_leafcloud, (size_t)std::max<int64_t>(cnt[0], 1) * 4));
         launch_leaf_clouds(v->nodes[0].as<GNode>(), cnt[0], grid[0], v->logdim[0], v->c_cellcloud.as<int32_t>(),
@@ -730,8 +735,7 @@ int compute_bounds(gvdbx_volume* v) {
     TRY(ensure_pin(v));
     // an unread earlier readback into the same slot is superseded: the host
     // never writes the slot, and stream order lands this copy last
-    launch_fill_i32(v->misc.as<int32_t>(), 3, 0x7FFFFFFF, s);
-    launch_fill_i32(v->misc.as<int32_t>() + 3, 3, INT32_MIN, s);
+    launch_bounds_init(v->misc.as<int32_t>(), s);
     launch_leaf_bounds(v->nodes[0].as<GNode>(), v->cnt[0], v->res[0], v->misc.as<int32_t>(), s);
     TRY(check_launch("leaf bounds"));
     GVDBX_CHECK(hipMemcpyAsync((char*)v->pin + PIN_BOUNDS, v->misc.p, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -964,7 +968,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
         free_buf(v->flag_base[l]);
         free_buf(v->flag_all[l]);
     }
-    DevBuf* bufs[] = {&v->reach0, &v->leafbits, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
+    DevBuf* bufs[] = {&v->reach0, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
                       &v->bin_off, &v->sorted, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
