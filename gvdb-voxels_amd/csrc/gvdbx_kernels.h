@@ -48,8 +48,23 @@ struct NodeBuildParams {
     GNode* nodes;
     int32_t* idx;        // cell -> node index or -1
 };
-void launch_flag_scan_apply(const uint8_t* base, const uint8_t* all, int64_t n, const U2* partials,
-                            const U2* totals, NodeBuildParams np, hipStream_t s);
+// flag scan apply of up to GVDBX_MAXLEV levels in one launch (tile_start is filled in)
+struct FlagApplyLevel {
+    const uint8_t* base;
+    const uint8_t* all;
+    int64_t n;
+    const U2* partials;
+    const U2* totals;
+    NodeBuildParams np;
+};
+struct FlagApplyBatch {
+    FlagApplyLevel lv[GVDBX_MAXLEV];
+    int64_t tile_start[GVDBX_MAXLEV + 1];
+    int n;
+};
+void launch_flag_scan_apply(FlagApplyBatch& fb, hipStream_t s);
+// a[0..na) = 0 and b[0..nb) = 0 in one launch
+void launch_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb, hipStream_t s);
 void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 void launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);

@@ -102,11 +102,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_flag_reduce(const uint8_t* bas
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_flag_apply(const uint8_t* base, const uint8_t* all, int64_t n,
-                                                               const U2* partials, const U2* totals,
-                                                               NodeBuildParams np) {
+// Every level in one launch: blocks [tile_start[i], tile_start[i+1]) are the
+// tiles of level i.
+__global__ __launch_bounds__(SCAN_THREADS) void k_flag_apply(FlagApplyBatch fb) {
+    int lv = 0;
+    while ((int64_t)blockIdx.x >= fb.tile_start[lv + 1]) lv++;
+    const uint8_t* base = fb.lv[lv].base;
+    const uint8_t* all = fb.lv[lv].all;
+    const int64_t n = fb.lv[lv].n;
+    const U2* partials = fb.lv[lv].partials;
+    const U2* totals = fb.lv[lv].totals;
+    const NodeBuildParams& np = fb.lv[lv].np;
+    const int64_t tile = (int64_t)blockIdx.x - fb.tile_start[lv];
     __shared__ U2 lds[4];
-    int64_t t0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t t0 = tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
     U2 v[SCAN_ITEMS];
     U2 s{0, 0};
 #pragma unroll
@@ -115,7 +124,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_flag_apply(const uint8_t* base
         s = s + v[k];
     }
     U2 tot;
-    U2 run = block_excl_scan<U2, 4>(s, lds, tot) + partials[blockIdx.x];
+    U2 run = block_excl_scan<U2, 4>(s, lds, tot) + partials[tile];
     uint32_t nbase = totals->a;
     const CellGrid& g = np.g;
 #pragma unroll
@@ -915,12 +924,23 @@ void launch_flag_scan_reduce(const uint8_t* base, const uint8_t* all, int64_t n,
     hipLaunchKernelGGL(k_scan_partials<U2>, dim3(1), dim3(1024), 0, s, partials, tiles, totals);
 }
 
-void launch_flag_scan_apply(const uint8_t* base, const uint8_t* all, int64_t n, const U2* partials,
-                            const U2* totals, NodeBuildParams np, hipStream_t s) {
-    int64_t tiles = scan_tiles(n);
-    if (tiles < 1) tiles = 1;
-    hipLaunchKernelGGL(k_flag_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, base, all, n, partials, totals,
-                       np);
+void launch_flag_scan_apply(FlagApplyBatch& fb, hipStream_t s) {
+    fb.tile_start[0] = 0;
+    for (int i = 0; i < fb.n; i++) fb.tile_start[i + 1] = fb.tile_start[i] + std::max<int64_t>(scan_tiles(fb.lv[i].n), 1);
+    if (fb.n > 0)
+        hipLaunchKernelGGL(k_flag_apply, dim3((unsigned)fb.tile_start[fb.n]), dim3(SCAN_THREADS), 0, s, fb);
+}
+
+__global__ __launch_bounds__(256) void k_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max(na, nb); i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < na) a[i] = 0;
+        if (i < nb) b[i] = 0;
+    }
+}
+
+void launch_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb, hipStream_t s) {
+    const int64_t n = std::max(na, nb);
+    if (n > 0) hipLaunchKernelGGL(k_zero_i32_u8, dim3(grid_for(n, 256)), dim3(256), 0, s, a, na, b, nb);
 }
 
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s) {

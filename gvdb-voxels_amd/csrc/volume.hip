@@ -607,8 +607,7 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     }
     TRY(ensure(v, v->partials, (size_t)(ptot + 2) * sizeof(U2)));
     TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
-    launch_fill_i32((int32_t*)v->reach0.p, G0, 0, s);
-    launch_fill_u8(v->flag_all[0].as<uint8_t>(), G0, 0, s);
+    launch_zero_i32_u8((int32_t*)v->reach0.p, G0, v->flag_all[0].as<uint8_t>(), G0, s);
     // bin the points by level-0 cell (kept for InsertPointsSubcell / InsertPoints),
     // then derive occupancy and extra-brick reach from the cell-sorted points
     v->grid[0] = grid[0];
@@ -646,11 +645,13 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
             TRY(ensure(v, v->clist[l], (size_t)(cnt[l] * len) * 8));
         }
     }
+    FlagApplyBatch fb{};
     for (int l = 0; l < depth; l++) {
         NodeBuildParams np{grid[l], l, v->range[l], l == 0 ? 1 : 0, v->nodes[l].as<GNode>(), v->idx[l].as<int32_t>()};
-        launch_flag_scan_apply(v->flag_base[l].as<uint8_t>(), v->flag_all[l].as<uint8_t>(), grid[l].count(),
-                               partials + poffs[l], totals + l, np, s);
+        fb.lv[fb.n++] = FlagApplyLevel{v->flag_base[l].as<uint8_t>(), v->flag_all[l].as<uint8_t>(), grid[l].count(),
+                                       partials + poffs[l], totals + l, np};
     }
+    launch_flag_scan_apply(fb, s);
     // root node (ActivateBricksGPU VG.cpp:1290-1301)
     GNode rn;
     memset(&rn, 0, sizeof(rn));
