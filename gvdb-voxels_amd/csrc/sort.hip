@@ -10,11 +10,11 @@
 //      written digit-major (hist[d * ntiles + tile]);
 //   2. an exclusive scan over the digit-major histogram gives every
 //      (digit, tile) its output base (the count scan of the topology code);
-//   3. k_rs_scatter: each tile re-reads its keys in 16 rounds of 256 (index
-//      order), ranks equal digits inside each wave with DB ballots, adds
-//      the counts of lower waves and earlier rounds from LDS, reorders the
-//      tile by digit in LDS, and writes every digit's run to its base.
-//      Equal digits keep their input order: stable.
+//   3. k_rs_scatter: each wave ranks a quarter of the tile (contiguous keys,
+//      64 per round, DB ballots per round, its own running digit counts in
+//      LDS: no block barrier between rounds), the per-wave counts become
+//      offsets, the tile is reordered by digit in LDS and every digit's run
+//      is written to its base.  Equal digits keep their input order: stable.
 // Reads and writes 8 B (32-bit keys) or 12 B (64-bit keys) per pair per pass
 // plus the histogram traffic.
 #include "gvdbx_kernels.h"
@@ -45,11 +45,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* keys, int64_t n
     for (int b = threadIdx.x; b < B; b += RS_THREADS) cnt[b] = 0;
     __syncthreads();
     const int64_t t0 = (int64_t)blockIdx.x * RS_THREADS * ITEMS;
-#pragma unroll 4
-    for (int r = 0; r < ITEMS; r++) {
-        int64_t i = t0 + r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & mask], 1);
+    K kr[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {   // all loads first, then the LDS counts
+        const int64_t i = t0 + r * RS_THREADS + threadIdx.x;
+        kr[r] = i < n ? keys[i] : (K)0;
     }
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++)
+        if (t0 + r * RS_THREADS + threadIdx.x < n) atomicAdd(&cnt[(uint32_t)(kr[r] >> shift) & mask], 1);
     __syncthreads();
     for (int b = threadIdx.x; b < B; b += RS_THREADS) hist[(int64_t)b * ntiles + blockIdx.x] = cnt[b];
 }
@@ -67,8 +71,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
     static_assert(B >= RS_THREADS, "one bin or more per thread");
     __shared__ int32_t gbase[B];    // global output start of each digit for this tile
     __shared__ int32_t tstart[B];   // start of each digit inside the tile
-    __shared__ int32_t run[B];
-    __shared__ int32_t wcnt[RS_WAVES][B];
+    __shared__ int32_t wcnt[RS_WAVES][B];   // per wave: running digit counts, then offsets
     __shared__ int32_t wtot[RS_WAVES];
     __shared__ K lkey[RS_TILE];
     __shared__ uint32_t lval[RS_TILE];
@@ -81,7 +84,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
     for (int j = 0; j < R; j++) {
         const int b = tid * R + j;
         gbase[b] = off[(int64_t)b * ntiles + blockIdx.x];
-        run[b] = 0;
         for (int w = 0; w < RS_WAVES; w++) wcnt[w][b] = 0;
         c[j] = hist[(int64_t)b * ntiles + blockIdx.x];
         csum += c[j];
@@ -106,18 +108,33 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
         }
         __syncthreads();
     }
+    // Wave w ranks the tile's keys [w * 64 * RS_ITEMS, (w + 1) * 64 * RS_ITEMS)
+    // on its own: 64 per round, lanes in order, against its running digit
+    // counts wcnt[w] (only the wave touches them, and its LDS accesses stay
+    // in program order), so the rounds need no block barrier.  Then the
+    // counts become per-wave offsets (exclusive over waves) and every key
+    // goes to tstart[d] + offset of its wave + its rank: equal digits keep
+    // their tile order.
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int r = 0; r < RS_ITEMS; r++) {
-        const int li = r * RS_THREADS + tid;
-        const bool ok = li < tn;
-        K k = 0;
-        uint32_t val = 0;
-        int d = 0;
-        if (ok) {
-            k = keys[t0 + li];
-            val = vals[t0 + li];
-            d = (int)((uint32_t)(k >> shift) & mask);
+    const int wbase = wave * 64 * RS_ITEMS;
+    K kr[RS_ITEMS];
+    uint32_t vr[RS_ITEMS];
+    int rk[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {   // all loads in flight at once
+        const int li = wbase + r * 64 + lane;
+        kr[r] = 0;
+        vr[r] = 0;
+        if (li < tn) {
+            kr[r] = keys[t0 + li];
+            vr[r] = vals[t0 + li];
         }
+    }
+    int32_t* wc = wcnt[wave];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const bool ok = wbase + r * 64 + lane < tn;
+        const int d = ok ? (int)((uint32_t)(kr[r] >> shift) & mask) : 0;
         uint64_t peers = __ballot(ok);
 #pragma unroll
         for (int b = 0; b < DB; b++) {
@@ -125,27 +142,32 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
             peers &= ((d >> b) & 1) ? m : ~m;
         }
         const int rank = __popcll(peers & lt);
-        if (ok && rank == 0) wcnt[wave][d] = __popcll(peers);
-        __syncthreads();
-        if (ok) {
-            int pos = tstart[d] + run[d] + rank;
-            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-            lkey[pos] = k;
-            lval[pos] = val;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < R; j++) {
-            const int b = tid * R + j;
-            int add = 0;
-            for (int w = 0; w < RS_WAVES; w++) {
-                add += wcnt[w][b];
-                wcnt[w][b] = 0;
-            }
-            run[b] += add;
-        }
-        __syncthreads();
+        const int before = ok ? wc[d] : 0;
+        rk[r] = before + rank;
+        if (ok && rank == 0) wc[d] = before + __popcll(peers);
     }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+        const int b = tid * R + j;
+        int acc = 0;
+        for (int w = 0; w < RS_WAVES; w++) {
+            const int t = wcnt[w][b];
+            wcnt[w][b] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        if (wbase + r * 64 + lane < tn) {
+            const int d = (int)((uint32_t)(kr[r] >> shift) & mask);
+            const int pos = tstart[d] + wc[d] + rk[r];
+            lkey[pos] = kr[r];
+            lval[pos] = vr[r];
+        }
+    }
+    __syncthreads();
     for (int j = tid; j < tn; j += RS_THREADS) {
         const K k = lkey[j];
         const int d = (int)((uint32_t)(k >> shift) & mask);
