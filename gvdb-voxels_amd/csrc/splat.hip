@@ -365,6 +365,21 @@ __device__ __forceinline__ GTile gather_tile(const GatherParams& gp, int res0, i
     return g;
 }
 
+// Wave-wide max of an unsigned value, returned as a scalar: DPP steps give
+// every lane its row's max (quad swaps, half-row and row mirrors), two row
+// broadcasts carry the row maxima into row 3, and lane 63 holds the result.
+// Needs all 64 lanes active.  Lanes a DPP step does not write get 0, the
+// identity of an unsigned max.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));   // row_mirror
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));   // row_bcast15
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));   // row_bcast31
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Bit ly * td + lx for every column (lx, ly) of the tile with lx in [xlo, xhi]
 // and ly in [ylo, yhi] (non-empty ranges inside the tile).
 __device__ __forceinline__ uint64_t column_mask(int xlo, int xhi, int ylo, int yhi, int td) {
@@ -385,8 +400,9 @@ __device__ __forceinline__ uint64_t column_mask(int xlo, int xhi, int ylo, int y
 // taken after the first chunk's load is issued, so work that decides the
 // stride overlaps that load.
 template <bool HALVES, typename Plan>
-__device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile& g, int td, int s, int ns,
+__device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile& g, int td_rt, int s, int ns,
                                             Plan&& plan, uint32_t (&m)[8]) {
+    const int td = HALVES ? 8 : td_rt;   // HALVES: 8-voxel tiles
     const int lane = threadIdx.x;
     const int T0x = g.T0x, T0y = g.T0y, T0z = g.T0z;
     const int cx = g.cx, cy = g.cy, cz = g.cz;
@@ -423,9 +439,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 for (int z = 0; z < 8; z++)
                     if (z < td) lmax = max(lmax, m[z]);
             }
-            uint32_t wmax = lmax;
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, o));
+            const uint32_t wmax = wave_max_u32(lmax);
             const float bound = fminf(cut2, __uint_as_float(wmax));
             const float lbound = fminf(cut2, __uint_as_float(lmax));
             // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
