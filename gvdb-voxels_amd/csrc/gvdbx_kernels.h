@@ -153,7 +153,14 @@ struct GatherParams {
     float cutoff;             // voxels farther than this cannot change a value
     int variant;              // 0: LDS-staged candidates, 1: ballot/readlane walk
     int xcd_remap;            // 1: give each XCD a contiguous run of tiles
+    const uint32_t* box;      // variant 1, optional: per bin entry, launch_gather_box's packed ranges
 };
+// Per bin entry (the first *total), the membership range clipped to the
+// cutoff along each axis, as offsets from floor(p): bits 10a..10a+4 hold
+// floor(p) + 1 - lo, bits 10a+5..10a+9 hold hi - floor(p); bit 31: empty on
+// some axis.  Needs ceil(box_radius) + sc <= 29 (every offset fits 5 bits).
+void launch_gather_box(const float4* sorted, const int32_t* total, int64_t nmax, float box_radius, int sc, int sc_shift,
+                       float cutoff, uint32_t* box, hipStream_t s);
 constexpr int GATHER_SPLIT_DEFAULT = 64;   // chunks per work item of a dense tile (0: one wave per tile)
 void launch_gather(GatherParams gp, hipStream_t s);
 // Variant 1 with tiles of more than 2 split chunks of 64 candidates cut into

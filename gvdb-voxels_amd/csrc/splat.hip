@@ -299,6 +299,39 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
 // minima are bit patterns of non-negative floats), and a last pass writes
 // the bricks from it with the same arithmetic as a single wave.
 // ---------------------------------------------------------------------------
+// Subcell membership range of a point along one axis clipped to the cutoff
+// (cuda_gvdb_particles.cuh:237-242), the arithmetic of the gather's filter.
+__device__ __forceinline__ void gather_range(float p, float R, int sc, int scs, float cut, int& lo_c, int& hi_c) {
+    const int lo = gvdbx_cdiv((int)(p - R), scs) * sc;
+    const int hi = gvdbx_cdiv((int)(p + 1 + R), scs) * sc + sc - 1;
+    lo_c = max(lo, (int)ceilf(p - cut));
+    hi_c = min(hi, (int)floorf(p + cut));
+}
+
+// Each point's ranges once per gather instead of once per tile that reads
+// it (up to 27).  Offsets: floor(p) - lo_c + 1 in [0, ceil(R) + sc + 2] and
+// hi_c - floor(p) in [0, R + sc + 1] when the range is not empty.
+__global__ __launch_bounds__(256) void k_gather_box(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
+                                                    float R, int sc, int scs, float cut, uint32_t* box) {
+    const int64_t total = min((int64_t)*total_ptr, nmax);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
+        const float4 q = sorted[j];
+        const float pw[3] = {q.x, q.y, q.z};
+        uint32_t bw = 0;
+        bool empty = false;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            int lo_c, hi_c;
+            gather_range(pw[a], R, sc, scs, cut, lo_c, hi_c);
+            const int f = (int)floorf(pw[a]);
+            empty = empty || lo_c > hi_c;
+            bw |= ((uint32_t)(f - lo_c + 1) & 31u) << (10 * a);
+            bw |= ((uint32_t)(hi_c - f) & 31u) << (10 * a + 5);
+        }
+        box[j] = empty ? 0x80000000u : bw;
+    }
+}
+
 struct ChunkIter {
     int d = -1;       // position in NB_ORDER
     int j0 = 0, end = 0;
@@ -420,7 +453,11 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
     ChunkIter it;
     bool have = skip_chunks(it, gp, cx, cy, cz, s + 1);
     float4 cur = make_float4(0, 0, 0, 0);
-    if (have && it.j0 + lane < it.end) cur = gp.sorted[it.j0 + lane];
+    uint32_t curb = 0;   // packed ranges (gp.box)
+    if (have && it.j0 + lane < it.end) {
+        cur = gp.sorted[it.j0 + lane];
+        if (gp.box) curb = gp.box[it.j0 + lane];
+    }
     ns = plan(ns);
     // One loop for ns == 1 (the stride folds away) and one for split tiles.
     auto walk = [&](auto one) {
@@ -428,7 +465,11 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
             const int cend = it.end, cj = it.j0 + lane;
             have = skip_chunks(it, gp, cx, cy, cz, decltype(one)::value ? 1 : ns);
             float4 nxt = make_float4(0, 0, 0, 0);
-            if (have && it.j0 + lane < it.end) nxt = gp.sorted[it.j0 + lane];
+            uint32_t nxtb = 0;
+            if (have && it.j0 + lane < it.end) {
+                nxt = gp.sorted[it.j0 + lane];
+                if (gp.box) nxtb = gp.box[it.j0 + lane];
+            }
             // Pruning bound: every voxel's running min is <= the tile's max of
             // them, and a candidate at squared distance db2 from the voxels it
             // may reach changes none of them if db2 exceeds that max (fma is
@@ -454,12 +495,24 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 int T0[3] = {T0x, T0y, T0z};
                 int rlo[2] = {0, 0}, rhi[2] = {0, 0};
                 float db2 = 0.0f;
+                if (gp.box) ok = ok && !(curb >> 31);
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
-                    int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
-                    int lo_c = max(lo, (int)ceilf(pw[a] - cut));
-                    int hi_c = min(hi, (int)floorf(pw[a] + cut));
+                    int lo = 0, hi = 0, lo_c, hi_c;
+                    if (gp.box) {
+                        // the ranges from the pre-pass; z of the halves walk then
+                        // uses the clipped range too (voxels beyond the cutoff
+                        // cannot change the result)
+                        const int f = (int)floorf(pw[a]);
+                        lo_c = f + 1 - (int)((curb >> (10 * a)) & 31u);
+                        hi_c = f + (int)((curb >> (10 * a + 5)) & 31u);
+                        lo = lo_c, hi = hi_c;
+                    } else {
+                        lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
+                        hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                        lo_c = max(lo, (int)ceilf(pw[a] - cut));
+                        hi_c = min(hi, (int)floorf(pw[a] + cut));
+                    }
                     int lr = max(lo_c - T0[a], 0), hr = min(hi_c - T0[a], td - 1);
                     ok = ok && lr <= hr;
                     float e = fmaxf(fmaxf((float)(T0[a] + lr) - pw[a], pw[a] - (float)(T0[a] + hr)), 0.0f);
@@ -524,6 +577,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 }
             }
             cur = nxt;
+            curb = nxtb;
         }
     };
     if (ns == 1)
@@ -1339,6 +1393,13 @@ void launch_gather(GatherParams gp, hipStream_t s) {
         hipLaunchKernelGGL((k_gather_rl<true, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
     else
         hipLaunchKernelGGL((k_gather_rl<false, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+}
+
+void launch_gather_box(const float4* sorted, const int32_t* total, int64_t nmax, float box_radius, int sc, int sc_shift,
+                       float cutoff, uint32_t* box, hipStream_t s) {
+    if (nmax > 0)
+        hipLaunchKernelGGL(k_gather_box, dim3(grid_for(nmax, 256)), dim3(256), 0, s, sorted, total, nmax, box_radius, sc,
+                           sc_shift, cutoff, box);
 }
 
 int gather_waves() {

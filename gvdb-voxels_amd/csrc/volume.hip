@@ -98,6 +98,7 @@ struct gvdbx_volume {
     DevBuf bin_off, sorted, bkey, bval, bkey2, bval2, rs_hist, rs_off, rs_part;
     DevBuf splat_src;   // (x, y, z, weight) in bin order, made per SplatCubic call
     DevBuf gs_items, gs_counts, gs_slots, gs_acc;   // Gather* work items of dense tiles
+    DevBuf gbox;                                    // Gather*: packed ranges per bin entry
     DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
     bool bins_valid = false;
@@ -828,6 +829,13 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
     gp.variant = v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
+    if (gp.variant == 1 && (int)std::ceil(gp.box_radius) + sc <= 29 && v->bins_n > 0) {
+        // each point's clipped ranges once, not once per tile that reads it
+        TRY(ensure(v, v->gbox, (size_t)v->bins_n * 4));
+        launch_gather_box(gp.sorted, gp.off + gp.gb.count(), v->bins_n, gp.box_radius, sc, gp.sc_shift, gp.cutoff,
+                          v->gbox.as<uint32_t>(), v->stream);
+        gp.box = v->gbox.as<uint32_t>();
+    }
     if (gp.variant == 1 && v->gather_split > 0) {
         // Bounds: a bin is a neighbour of at most 27 leaves, so the chunks
         // summed over all tiles are at most 27 * tiles * (chunks of all bins),
@@ -974,7 +982,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
-                      &v->splat_src, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
+                      &v->splat_src, &v->gbox, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
                       &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
                       &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
