@@ -372,6 +372,30 @@ def test_gather_split_bit_identical(log2dims, sc, R):
 
 
 @pytest.mark.parametrize("mode", [1, 0])
+def test_gather_wide_box_parity(mode):
+    """32^3 bricks, subcell 16, radius 14: ceil(R) + sc > 29, so the gather
+    filters without the packed per-point ranges; oracle bits, fresh and
+    accumulated."""
+    _log(f"[gpu] gather wide box mode={mode}")
+    rng = np.random.default_rng(515 + mode)
+    pts = (rng.random((4000, 3)) * [150, 90, 120] + 3).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 14.0, (5, 3, 3, 3, 3))
+    gl = gv.InsertPointsSubcell(16, len(pts), 14.0)
+    assert gl == ov.insert_points_subcell(16, pts, 14.0)
+    gather = gv.GatherLevelSet if mode == 1 else gv.GatherDensity
+    ogather = ov.gather_level_set if mode == 1 else ov.gather_density
+    ogather(2.0)
+    gv.FillChannel(0, -7.0)
+    gather(16, len(pts), 2.0, (0, 0, 0), gl, 0)
+    np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], ov.bricks()[:, 1:-1, 1:-1, 1:-1])
+    ov.fill_channel(0, 0.25)
+    ogather(2.0, accum=True)
+    gv.FillChannel(0, 0.25)
+    gather(16, len(pts), 2.0, (0, 0, 0), gl, 0, accum=True)
+    np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], ov.bricks()[:, 1:-1, 1:-1, 1:-1])
+
+
+@pytest.mark.parametrize("mode", [1, 0])
 def test_gather_variants_bit_identical(mode):
     """All gather kernel variants produce the oracle's bits."""
     _log(f"[gpu] gather variants mode={mode}")
