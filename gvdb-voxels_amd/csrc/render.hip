@@ -609,10 +609,20 @@ template <int BR>
 __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     constexpr int NV = BR * BR * BR;
     constexpr int NW = 4;
-    __shared__ float dens[NV];
-    __shared__ float grad[NW][NV];
-    __shared__ int claim[NV];
-    __shared__ int32_t wlist[NW][128];   // per wave: its segments waiting for a lane
+    // One LDS block laid out by hand with the brick first: its base address is
+    // then 0, so a trilinear cell's four corner pairs are one address plus
+    // small immediate offsets.
+    struct EALds {
+        float dens[NV];
+        float grad[NW][NV];
+        int claim[NV];
+        int32_t wlist[NW][128];   // per wave: its segments waiting for a lane
+    };
+    __shared__ EALds lds;
+    float* const dens = lds.dens;
+    float (*const grad)[NV] = lds.grad;
+    int* const claim = lds.claim;
+    int32_t (*const wlist)[128] = lds.wlist;
     const int64_t leaf = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float* src = p.tv.chan + leaf * (int64_t)NV;
