@@ -221,6 +221,13 @@ __device__ __forceinline__ float ea_log2_atten(const RenderScene& sc) {
 }
 __device__ __forceinline__ float ea_atten(float k, float s) { return __builtin_amdgcn_exp2f(k * s); }
 
+// 0 <= q < r0 on every axis, as 3-way min / max (positions are never NaN):
+// the same predicate as the six compares, in fewer instructions.
+__device__ __forceinline__ bool in_brick(const V3& q, float r0) {
+    return __builtin_fminf(__builtin_fminf(q.x, q.y), q.z) >= 0.0f &&
+           __builtin_fmaxf(__builtin_fmaxf(q.x, q.y), q.z) < r0;
+}
+
 template <int MODE, int BR>
 __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene& sc, int64_t leaf, V3 t, V3 pos,
                                          V3 dir, RayState& st, const int* sh) {
@@ -236,9 +243,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
         const V3 wpt = scale(dir, step);
         const float dt = sqrtf(dot(wpt, wpt));
         if (st.hit.x == 0) st.hit.x = t.x;
-        for (int iter = 0; st.clr[3] > sc.alphacut && iter < sc.max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 &&
-                           p.x < r0 && p.y < r0 && p.z < r0;
-             iter++) {
+        for (int iter = 0; st.clr[3] > sc.alphacut && iter < sc.max_iter && in_brick(p, r0); iter++) {
             float s;
             if constexpr (MODE == MODE_DEEP_LINEAR && BR > 0) {
                 int ix, iy, iz;
@@ -681,9 +686,7 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
                 acc[m] = 0.0f;
             }
         };
-        for (int iter = 0; T > sc.alphacut && iter < sc.max_iter && q.x >= 0 && q.y >= 0 && q.z >= 0 && q.x < r0 &&
-                           q.y < r0 && q.z < r0;
-             iter++) {
+        for (int iter = 0; T > sc.alphacut && iter < sc.max_iter && in_brick(q, r0); iter++) {
             // trilinear sample (the forward's value) sharing its cell and weights with the adjoint
             int ix, iy, iz;
             float ax, ay, az;
