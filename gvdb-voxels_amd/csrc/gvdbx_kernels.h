@@ -154,6 +154,9 @@ struct GatherParams {
     int variant;              // 0: LDS-staged candidates, 1: ballot/readlane walk
     int xcd_remap;            // 1: give each XCD a contiguous run of tiles
     const uint32_t* box;      // variant 1, optional: per bin entry, launch_gather_box's packed ranges
+    const int32_t* coff;      // variant 1, optional: class starts of launch_gather_classes (sorted is then
+                              // its reordered copy, packed ranges in .w; box unused)
+    const uint8_t* cboth;     // with coff: per bin, axes along which a point reaches past both faces
 };
 // Per bin entry (the first *total), the membership range clipped to the
 // cutoff along each axis, as offsets from floor(p): bits 10a..10a+4 hold
@@ -161,7 +164,15 @@ struct GatherParams {
 // some axis.  Needs ceil(box_radius) + sc <= 29 (every offset fits 5 bits).
 void launch_gather_box(const float4* sorted, const int32_t* total, int64_t nmax, float box_radius, int sc, int sc_shift,
                        float cutoff, uint32_t* box, hipStream_t s);
-constexpr int GATHER_SPLIT_DEFAULT = 64;   // chunks per work item of a dense tile (0: one wave per tile)
+// Per bin (level-0 cell of side res0 = 2^k), its entries reordered stably by
+// reach class (bit a: the clipped range along axis a reaches past the bin's
+// upper face) into csorted as (x, y, z, launch_gather_box's packed ranges),
+// the class starts in coff (8 per bin plus the total) and in both[bin] the
+// axes along which some entry reaches past both faces.  Same range limits
+// as launch_gather_box.
+void launch_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0, float box_radius, int sc,
+                           int sc_shift, float cutoff, float4* csorted, int32_t* coff, uint8_t* both, hipStream_t s);
+constexpr int GATHER_SPLIT_DEFAULT = 16;   // chunks per work item of a dense tile (0: one wave per tile)
 void launch_gather(GatherParams gp, hipStream_t s);
 // Variant 1 with tiles of more than 2 split chunks of 64 candidates cut into
 // work items of about `split` chunks: items (item_cap int4), counts (2 int32),

@@ -311,25 +311,127 @@ __device__ __forceinline__ void gather_range(float p, float R, int sc, int scs, 
 // Each point's ranges once per gather instead of once per tile that reads
 // it (up to 27).  Offsets: floor(p) - lo_c + 1 in [0, ceil(R) + sc + 2] and
 // hi_c - floor(p) in [0, R + sc + 1] when the range is not empty.
+__device__ __forceinline__ uint32_t gather_box_word(const float4& q, float R, int sc, int scs, float cut, int (&lo_c)[3],
+                                                    int (&hi_c)[3]) {
+    const float pw[3] = {q.x, q.y, q.z};
+    uint32_t bw = 0;
+    bool empty = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        gather_range(pw[a], R, sc, scs, cut, lo_c[a], hi_c[a]);
+        const int f = (int)floorf(pw[a]);
+        empty = empty || lo_c[a] > hi_c[a];
+        bw |= ((uint32_t)(f - lo_c[a] + 1) & 31u) << (10 * a);
+        bw |= ((uint32_t)(hi_c[a] - f) & 31u) << (10 * a + 5);
+    }
+    return empty ? 0x80000000u : bw;
+}
+
 __global__ __launch_bounds__(256) void k_gather_box(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
                                                     float R, int sc, int scs, float cut, uint32_t* box) {
     const int64_t total = min((int64_t)*total_ptr, nmax);
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (int64_t)gridDim.x * blockDim.x) {
-        const float4 q = sorted[j];
-        const float pw[3] = {q.x, q.y, q.z};
-        uint32_t bw = 0;
-        bool empty = false;
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            int lo_c, hi_c;
-            gather_range(pw[a], R, sc, scs, cut, lo_c, hi_c);
-            const int f = (int)floorf(pw[a]);
-            empty = empty || lo_c > hi_c;
-            bw |= ((uint32_t)(f - lo_c + 1) & 31u) << (10 * a);
-            bw |= ((uint32_t)(hi_c - f) & 31u) << (10 * a + 5);
-        }
-        box[j] = empty ? 0x80000000u : bw;
+        int lo_c[3], hi_c[3];
+        box[j] = gather_box_word(sorted[j], R, sc, scs, cut, lo_c, hi_c);
     }
+}
+
+// Reach classes.  A point's class has bit a set when its range along axis a
+// (membership clipped to the cutoff, as above) reaches past the upper face
+// of its bin (level-0 cell).  A tile in cell c needs, from the bin at c + o,
+// only the classes that can reach c: bit a set where o_a = -1, bit a clear
+// where o_a = +1 (unless a point of that bin reaches past both faces along
+// a), anything where o_a = 0.  One wave per bin reorders the bin's points by
+// class (stably) into csorted as (x, y, z, packed ranges), writes the class
+// starts to coff[8 ci + k] (coff[8 nbin] = the total) and the axes with
+// points reaching past both faces to both[ci].  CLS_UNROLL chunks of 64
+// entries are loaded at once; a bin of at most that many entries stays in
+// registers between counting and placing.  The order inside a bin does not
+// change a gather: its result is a minimum.
+constexpr int CLS_UNROLL = 4;
+
+__global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
+                                                       float R, int sc, int scs, float cut, float4* csorted,
+                                                       int32_t* coff, uint8_t* both) {
+    const int lane = threadIdx.x;
+    const int64_t nbin = gb.count();
+    const int64_t nyz = (int64_t)gb.ny * gb.nz;
+    for (int64_t ci = blockIdx.x; ci < nbin; ci += gridDim.x) {
+        const int beg = off[ci], end = off[ci + 1];
+        const int O[3] = {((int)(ci / nyz) + gb.ox) * res0, ((int)((ci / gb.nz) % gb.ny) + gb.oy) * res0,
+                          ((int)(ci % gb.nz) + gb.oz) * res0};
+        float4 q[CLS_UNROLL];
+        int cls[CLS_UNROLL];   // -1: no entry
+        uint32_t bb = 0;       // axes with a point reaching past both faces (this lane)
+        // entries j0 + 64 u + lane: point, packed ranges in .w, class
+        auto load = [&](int j0) {
+#pragma unroll
+            for (int u = 0; u < CLS_UNROLL; u++) {
+                const int j = j0 + 64 * u + lane;
+                cls[u] = -1;
+                if (j < end) {
+                    q[u] = sorted[j];
+                    cls[u] = 0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < CLS_UNROLL; u++) {
+                if (cls[u] < 0) continue;
+                int lo_c[3], hi_c[3];
+                const uint32_t bw = gather_box_word(q[u], R, sc, scs, cut, lo_c, hi_c);
+                q[u].w = __uint_as_float(bw);
+                if (!(bw >> 31)) {
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        const bool up = hi_c[a] >= O[a] + res0;
+                        cls[u] |= (int)up << a;
+                        bb |= (uint32_t)(up && lo_c[a] < O[a]) << a;
+                    }
+                }
+            }
+        };
+        int base[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) base[k] = 0;
+        for (int j0 = beg; j0 < end; j0 += 64 * CLS_UNROLL) {
+            load(j0);
+#pragma unroll
+            for (int u = 0; u < CLS_UNROLL; u++)
+#pragma unroll
+                for (int k = 0; k < 8; k++) base[k] += __popcll(__ballot(cls[u] == k));
+        }
+        int acc = beg;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int c = base[k];
+            base[k] = acc;
+            if (lane == k) coff[ci * 8 + k] = acc;
+            acc += c;
+        }
+        uint32_t bb_all = 0;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (__ballot((bb >> a) & 1u)) bb_all |= 1u << a;
+        if (lane == 0) both[ci] = (uint8_t)bb_all;
+        const bool held = end - beg <= 64 * CLS_UNROLL;
+        for (int j0 = beg; j0 < end; j0 += 64 * CLS_UNROLL) {
+            if (!held) load(j0);
+#pragma unroll
+            for (int u = 0; u < CLS_UNROLL; u++) {
+                int pos = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const uint64_t bal = __ballot(cls[u] == k);
+                    if (cls[u] == k)
+                        pos = base[k] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    base[k] += __popcll(bal);
+                }
+                if (cls[u] >= 0) csorted[pos] = q[u];
+            }
+        }
+    }
+    if (blockIdx.x == 0 && lane == 0) coff[nbin * 8] = off[nbin];
 }
 
 struct ChunkIter {
@@ -372,6 +474,97 @@ __device__ __forceinline__ bool skip_chunks(ChunkIter& it, const GatherParams& g
         }
     }
     return false;
+}
+
+// Classes of the bin at offset o from the tile's cell that can reach the cell.
+__device__ __forceinline__ uint32_t class_mask(int ox, int oy, int oz, uint32_t both) {
+    const int o[3] = {ox, oy, oz};
+    const uint32_t up[3] = {0xAAu, 0xCCu, 0xF0u};   // classes with bit a set
+    uint32_t m = 0xFFu;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (o[a] < 0)
+            m &= up[a];
+        else if (o[a] > 0 && !((both >> a) & 1u))
+            m &= ~up[a];
+    }
+    return m & 0xFFu;
+}
+
+// A tile's candidates with reach classes: the runs of consecutive wanted
+// classes of its 27 neighbour bins (at most 4 per bin), nearest bins first.
+// Run r starts at entry start[r] and is preceded by pre[r] candidates;
+// pre[nruns .. 127] = INT_MAX, so a 7-step binary search finds the run of
+// any candidate number.
+struct RunTable {
+    int pre[128];
+    int start[128];
+};
+
+// Builds the table (lane d reads neighbour bin d's class starts) and returns
+// the number of candidates.
+__device__ __forceinline__ int build_runs(const GatherParams& gp, int cx, int cy, int cz, RunTable& rt) {
+    const int lane = threadIdx.x;
+    int c[9];
+    uint32_t want = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = 0;
+    if (lane < 27) {
+        const int nb = NB_ORDER[lane];
+        const int ox = nb / 9 - 1, oy = (nb / 3) % 3 - 1, oz = nb % 3 - 1;
+        if (gp.gb.contains(cx + ox, cy + oy, cz + oz)) {
+            const int64_t ci = gp.gb.index(cx + ox, cy + oy, cz + oz);
+#pragma unroll
+            for (int k = 0; k < 9; k++) c[k] = gp.coff[ci * 8 + k];
+            if (c[8] > c[0]) want = class_mask(ox, oy, oz, gp.cboth[ci]);
+        }
+    }
+    // this lane's non-empty runs: count and entries, then (second pass) stores
+    auto runs = [&](auto&& emit) {
+        bool in = false;
+        int rb = 0;
+#pragma unroll
+        for (int k = 0; k <= 8; k++) {
+            const bool w = k < 8 && ((want >> k) & 1u);
+            if (w && !in) rb = c[k];
+            if (!w && in && c[k] > rb) emit(rb, c[k] - rb);
+            in = w;
+        }
+    };
+    int nr = 0, tot = 0;
+    runs([&](int, int len) {
+        nr++;
+        tot += len;
+    });
+    int inr = nr, itot = tot;   // inclusive scans over the lanes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int a = __shfl_up(inr, o), b = __shfl_up(itot, o);
+        if (lane >= o) {
+            inr += a;
+            itot += b;
+        }
+    }
+    const int nruns = __builtin_amdgcn_readlane(inr, 63), total = __builtin_amdgcn_readlane(itot, 63);
+    int r = inr - nr, before = itot - tot;
+    runs([&](int beg, int len) {
+        rt.start[r] = beg;
+        rt.pre[r] = before;
+        r++;
+        before += len;
+    });
+    for (int q = nruns + lane; q < 128; q += 64) rt.pre[q] = INT_MAX;
+    __syncthreads();
+    return total;
+}
+
+// Entry of candidate v (0 <= v < the table's total).
+__device__ __forceinline__ int run_entry(const RunTable& rt, int v) {
+    int r = 0;
+#pragma unroll
+    for (int st = 64; st >= 1; st >>= 1)
+        if (rt.pre[r + st] <= v) r += st;
+    return rt.start[r] + (v - rt.pre[r]);
 }
 
 // Where a tile sits: its leaf k, its origin inside the brick, its first
@@ -431,8 +624,10 @@ __device__ __forceinline__ uint64_t column_mask(int xlo, int xhi, int ylo, int y
 // (x, y) column of the tile, m[z] = bits of min |p-v|^2 over its z voxels;
 // non-negative floats order like their bit patterns).  ns = plan(ns) is
 // taken after the first chunk's load is issued, so work that decides the
-// stride overlaps that load.
-template <bool HALVES, typename Plan>
+// stride overlaps that load.  CLS: the candidates are the class runs that
+// can reach the tile (k_gather_classes), packed into full chunks; otherwise
+// every point of the 27 bins, bin by bin.
+template <bool HALVES, bool CLS, typename Plan>
 __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile& g, int td_rt, int s, int ns,
                                             Plan&& plan, uint32_t (&m)[8]) {
     const int td = HALVES ? 8 : td_rt;   // HALVES: 8-voxel tiles
@@ -451,24 +646,46 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
     const float cut2 = cut * cut;
 
     ChunkIter it;
-    bool have = skip_chunks(it, gp, cx, cy, cz, s + 1);
+    __shared__ RunTable rt;   // CLS only
+    int total = 0, v = 0;     // CLS: candidates of the tile, this lane's candidate number
+    bool have;
+    if constexpr (CLS) {
+        total = build_runs(gp, cx, cy, cz, rt);
+        v = 64 * s + lane;
+        have = 64 * s < total;
+    } else {
+        have = skip_chunks(it, gp, cx, cy, cz, s + 1);
+    }
     float4 cur = make_float4(0, 0, 0, 0);
     uint32_t curb = 0;   // packed ranges (gp.box)
-    if (have && it.j0 + lane < it.end) {
-        cur = gp.sorted[it.j0 + lane];
-        if (gp.box) curb = gp.box[it.j0 + lane];
+    if (have && (CLS ? v < total : it.j0 + lane < it.end)) {
+        const int j = CLS ? run_entry(rt, v) : it.j0 + lane;
+        cur = gp.sorted[j];
+        if (CLS)
+            curb = __float_as_uint(cur.w);
+        else if (gp.box)
+            curb = gp.box[j];
     }
-    ns = plan(ns);
+    ns = plan(ns, total);
     // One loop for ns == 1 (the stride folds away) and one for split tiles.
     auto walk = [&](auto one) {
         while (have) {
-            const int cend = it.end, cj = it.j0 + lane;
-            have = skip_chunks(it, gp, cx, cy, cz, decltype(one)::value ? 1 : ns);
+            const bool cvalid = CLS ? v < total : it.j0 + lane < it.end;
+            if constexpr (CLS) {
+                v += 64 * (decltype(one)::value ? 1 : ns);
+                have = v - lane < total;
+            } else {
+                have = skip_chunks(it, gp, cx, cy, cz, decltype(one)::value ? 1 : ns);
+            }
             float4 nxt = make_float4(0, 0, 0, 0);
             uint32_t nxtb = 0;
-            if (have && it.j0 + lane < it.end) {
-                nxt = gp.sorted[it.j0 + lane];
-                if (gp.box) nxtb = gp.box[it.j0 + lane];
+            if (have && (CLS ? v < total : it.j0 + lane < it.end)) {
+                const int j = CLS ? run_entry(rt, v) : it.j0 + lane;
+                nxt = gp.sorted[j];
+                if (CLS)
+                    nxtb = __float_as_uint(nxt.w);
+                else if (gp.box)
+                    nxtb = gp.box[j];
             }
             // Pruning bound: every voxel's running min is <= the tile's max of
             // them, and a candidate at squared distance db2 from the voxels it
@@ -487,7 +704,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
             // as a voxel range; a candidate must also come within the cutoff of the tile.
             // The packed range is the membership range clipped to the tile (z) and to
             // membership + cutoff (x, y: lanes outside it are masked).
-            bool ok = cj < cend;
+            bool ok = cvalid;
             uint32_t pk = 0;   // z range inside the tile: lo | hi << 3
             uint64_t cm = 0;   // tile columns in the x/y range: bit ly * td + lx (= their lane)
             {
@@ -495,11 +712,11 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 int T0[3] = {T0x, T0y, T0z};
                 int rlo[2] = {0, 0}, rhi[2] = {0, 0};
                 float db2 = 0.0f;
-                if (gp.box) ok = ok && !(curb >> 31);
+                if (CLS || gp.box) ok = ok && !(curb >> 31);
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     int lo = 0, hi = 0, lo_c, hi_c;
-                    if (gp.box) {
+                    if (CLS || gp.box) {
                         // the ranges from the pre-pass; z of the halves walk then
                         // uses the clipped range too (voxels beyond the cutoff
                         // cannot change the result)
@@ -622,7 +839,7 @@ struct GatherSplit {
     uint32_t* acc;
 };
 
-template <bool HALVES, bool SPLIT>
+template <bool HALVES, bool SPLIT, bool CLS>
 __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles,
                                                   GatherSplit gs) {
     const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
@@ -632,19 +849,23 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
     const int td = tile_dim(res0);
     const GTile g = gather_tile(gp, res0, tiles_per_brick, tile);
     int ns = 1, slot = -1;
-    auto plan = [&](int) {
+    auto plan = [&](int, int total) {
         if (!SPLIT) return 1;
-        int nch = 0;   // lane d < 27: chunks of neighbour bin d
-        if (lane < 27) {
-            const int bx = g.cx + lane / 9 - 1, by = g.cy + (lane / 3) % 3 - 1, bz = g.cz + lane % 3 - 1;
-            if (gp.gb.contains(bx, by, bz)) {
-                const int64_t ci = gp.gb.index(bx, by, bz);
-                nch = (gp.off[ci + 1] - gp.off[ci] + 63) >> 6;
+        int nch = 0;   // lane d < 27: chunks of neighbour bin d (CLS: the run table's total)
+        if (CLS) {
+            nch = (total + 63) >> 6;
+        } else {
+            if (lane < 27) {
+                const int bx = g.cx + lane / 9 - 1, by = g.cy + (lane / 3) % 3 - 1, bz = g.cz + lane % 3 - 1;
+                if (gp.gb.contains(bx, by, bz)) {
+                    const int64_t ci = gp.gb.index(bx, by, bz);
+                    nch = (gp.off[ci + 1] - gp.off[ci] + 63) >> 6;
+                }
             }
-        }
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) nch += __shfl_xor(nch, o);
-        nch = __builtin_amdgcn_readfirstlane(nch);
+            for (int o = 32; o >= 1; o >>= 1) nch += __shfl_xor(nch, o);
+            nch = __builtin_amdgcn_readfirstlane(nch);
+        }
         if (nch > 2 * gs.split) {
             ns = (nch + gs.split - 1) / gs.split;
             int base = 0, sl = 0;
@@ -664,7 +885,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
     uint32_t m[8];
 #pragma unroll
     for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
-    gather_walk<HALVES>(gp, g, td, 0, 1, plan, m);
+    gather_walk<HALVES, CLS>(gp, g, td, 0, 1, plan, m);
     if (!SPLIT || ns == 1) {
         gather_store(gp, g, td, m);
     } else if (slot >= 0) {
@@ -677,7 +898,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
 
 // Persistent walk over the items beyond the first of each split tile
 // (counts[0] of them; the scratch tiles hold item 0's minima).
-template <bool HALVES>
+template <bool HALVES, bool CLS>
 __global__ __launch_bounds__(64) void k_gather_items(GatherParams gp, int res0, int tiles_per_brick,
                                                      const int4* items, int64_t item_cap, const int32_t* counts,
                                                      uint32_t* acc, int64_t slot_cap) {
@@ -690,7 +911,7 @@ __global__ __launch_bounds__(64) void k_gather_items(GatherParams gp, int res0, 
         uint32_t m[8];
 #pragma unroll
         for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
-        gather_walk<HALVES>(gp, g, td, w.y, w.z, [](int n) { return n; }, m);
+        gather_walk<HALVES, CLS>(gp, g, td, w.y, w.z, [](int n, int) { return n; }, m);
         if (lane / td < td && w.w >= 0 && w.w < slot_cap) {
             uint32_t* at = acc + (int64_t)w.w * 512 + lane;
 #pragma unroll
@@ -1389,10 +1610,23 @@ void launch_gather(GatherParams gp, hipStream_t s) {
         hipLaunchKernelGGL(k_gather, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
     else if (gp.variant == 2)
         hipLaunchKernelGGL(k_gather_pc, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
-    else if (halves)
-        hipLaunchKernelGGL((k_gather_rl<true, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+    else if (gp.coff) {
+        if (halves)
+            hipLaunchKernelGGL((k_gather_rl<true, false, true>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+        else
+            hipLaunchKernelGGL((k_gather_rl<false, false, true>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+    } else if (halves)
+        hipLaunchKernelGGL((k_gather_rl<true, false, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
     else
-        hipLaunchKernelGGL((k_gather_rl<false, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+        hipLaunchKernelGGL((k_gather_rl<false, false, false>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+}
+
+void launch_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0, float box_radius, int sc,
+                           int sc_shift, float cutoff, float4* csorted, int32_t* coff, uint8_t* both, hipStream_t s) {
+    const int64_t nbin = gb.count();
+    if (nbin > 0)
+        hipLaunchKernelGGL(k_gather_classes, dim3((unsigned)std::min<int64_t>(nbin, 1 << 20)), dim3(64), 0, s, sorted, off,
+                           gb, res0, box_radius, sc, sc_shift, cutoff, csorted, coff, both);
 }
 
 void launch_gather_box(const float4* sorted, const int32_t* total, int64_t nmax, float box_radius, int sc, int sc_shift,
@@ -1407,7 +1641,7 @@ int gather_waves() {
     if (!waves) {
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_items<true>, 64, 0) != hipSuccess)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_items<true, true>, 64, 0) != hipSuccess)
             cus = 256, per_cu = 8;
         waves = std::max(1, cus) * std::max(1, per_cu);
     }
@@ -1426,18 +1660,29 @@ void launch_gather_split(GatherParams gp, int split, int4* items, int64_t item_c
     launch_fill_i32(counts, 2, 0, s);
     const GatherSplit gs{split, items, item_cap, counts, slot_tile, slot_cap, acc};
     const dim3 grid((unsigned)gvdbx_xcd_blocks(ntiles));
-    if (halves)
-        hipLaunchKernelGGL((k_gather_rl<true, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+    const bool cls = gp.coff != nullptr;
+    if (halves && cls)
+        hipLaunchKernelGGL((k_gather_rl<true, true, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+    else if (halves)
+        hipLaunchKernelGGL((k_gather_rl<true, true, false>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+    else if (cls)
+        hipLaunchKernelGGL((k_gather_rl<false, true, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
     else
-        hipLaunchKernelGGL((k_gather_rl<false, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+        hipLaunchKernelGGL((k_gather_rl<false, true, false>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
     const int waves = gather_waves();
     const dim3 gi((unsigned)std::max<int64_t>(1, std::min<int64_t>(item_cap, waves)));
-    if (halves)
-        hipLaunchKernelGGL(k_gather_items<true>, gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts, acc,
-                           slot_cap);
+    if (halves && cls)
+        hipLaunchKernelGGL((k_gather_items<true, true>), gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts,
+                           acc, slot_cap);
+    else if (halves)
+        hipLaunchKernelGGL((k_gather_items<true, false>), gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts,
+                           acc, slot_cap);
+    else if (cls)
+        hipLaunchKernelGGL((k_gather_items<false, true>), gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts,
+                           acc, slot_cap);
     else
-        hipLaunchKernelGGL(k_gather_items<false>, gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts, acc,
-                           slot_cap);
+        hipLaunchKernelGGL((k_gather_items<false, false>), gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts,
+                           acc, slot_cap);
     const dim3 gf((unsigned)std::max<int64_t>(1, std::min<int64_t>(slot_cap, waves)));
     hipLaunchKernelGGL(k_gather_finish, gf, dim3(64), 0, s, gp, res0, tiles, slot_tile, counts, acc, slot_cap);
 }

@@ -87,6 +87,7 @@ struct gvdbx_volume {
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int gather_split = GATHER_SPLIT_DEFAULT;
+    int gather_classes = 1;
     int xcd_remap = 0;
     int splat_chunk = SPLAT_CHUNK_DEFAULT;
     int splat_variant = 1;
@@ -99,6 +100,7 @@ struct gvdbx_volume {
     DevBuf splat_src;   // (x, y, z, weight) in bin order, made per SplatCubic call
     DevBuf gs_items, gs_counts, gs_slots, gs_acc;   // Gather* work items of dense tiles
     DevBuf gbox;                                    // Gather*: packed ranges per bin entry
+    DevBuf gcls_pts, gcoff, gcboth;                 // Gather*: bin entries by reach class, class starts, both-face axes
     DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
     bool bins_valid = false;
@@ -831,10 +833,25 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.xcd_remap = v->xcd_remap;
     if (gp.variant == 1 && (int)std::ceil(gp.box_radius) + sc <= 29 && v->bins_n > 0) {
         // each point's clipped ranges once, not once per tile that reads it
-        TRY(ensure(v, v->gbox, (size_t)v->bins_n * 4));
-        launch_gather_box(gp.sorted, gp.off + gp.gb.count(), v->bins_n, gp.box_radius, sc, gp.sc_shift, gp.cutoff,
-                          v->gbox.as<uint32_t>(), v->stream);
-        gp.box = v->gbox.as<uint32_t>();
+        if (v->gather_classes) {
+            // and each bin's entries grouped by the neighbour cells they can
+            // reach, so a tile reads only the groups that can reach it
+            const int64_t nbin = gp.gb.count();
+            TRY(ensure(v, v->gcls_pts, (size_t)v->bins_n * 16));
+            TRY(ensure(v, v->gcoff, (size_t)(nbin * 8 + 1) * 4));
+            TRY(ensure(v, v->gcboth, (size_t)std::max<int64_t>(nbin, 1)));
+            launch_gather_classes(gp.sorted, gp.off, gp.gb, gp.brick_res - 2 * gp.apron, gp.box_radius, sc, gp.sc_shift,
+                                  gp.cutoff, v->gcls_pts.as<float4>(), v->gcoff.as<int32_t>(), v->gcboth.as<uint8_t>(),
+                                  v->stream);
+            gp.sorted = v->gcls_pts.as<float4>();
+            gp.coff = v->gcoff.as<int32_t>();
+            gp.cboth = v->gcboth.as<uint8_t>();
+        } else {
+            TRY(ensure(v, v->gbox, (size_t)v->bins_n * 4));
+            launch_gather_box(gp.sorted, gp.off + gp.gb.count(), v->bins_n, gp.box_radius, sc, gp.sc_shift, gp.cutoff,
+                              v->gbox.as<uint32_t>(), v->stream);
+            gp.box = v->gbox.as<uint32_t>();
+        }
     }
     if (gp.variant == 1 && v->gather_split > 0) {
         // Bounds: a bin is a neighbour of at most 27 leaves, so the chunks
@@ -982,7 +999,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
-                      &v->splat_src, &v->gbox, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
+                      &v->splat_src, &v->gbox, &v->gcls_pts, &v->gcoff, &v->gcboth, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
                       &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
                       &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
@@ -1100,6 +1117,11 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->gather_split = value;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "gather_classes")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "gather_classes must be 0 or 1");
+        v->gather_classes = value;
+        return GVDBX_OK;
+    }
     if (!strcmp(key, "xcd_remap")) {
         if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "xcd_remap must be 0 or 1");
         v->xcd_remap = value;
@@ -1127,6 +1149,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "gather_split")) {
         *value = v->gather_split;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "gather_classes")) {
+        *value = v->gather_classes;
         return GVDBX_OK;
     }
     if (!strcmp(key, "xcd_remap")) {

@@ -89,7 +89,11 @@ int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
  * candidates with LDS minima; default 1), "gather_split" (variant 1: tiles
  * with more than twice this many 64-point candidate chunks are cut into
  * work items of about this many; 0 = one wavefront per tile; the result is
- * the same bit for bit; default 64, range [0, 2^20]), "xcd_remap" (1 = contiguous brick
+ * the same bit for bit; default 16, range [0, 2^20]), "gather_classes"
+ * (variant 1: 1 = each bin's points grouped per call by the neighbour cells
+ * they can reach, so a tile reads only the groups that can reach it; 0 =
+ * every point of the 27 neighbour bins; same result bit for bit; default 1),
+ * "xcd_remap" (1 = contiguous brick
  * runs per XCD; default 0), "splat_variant" (SplatCubic: 0 = tile pull,
  * 1 = cell push; default 1; cell push needs a (res0 + 2 ceil(r))^3 float
  * grid within 64 KiB, else tile pull runs),

@@ -336,9 +336,11 @@ def test_full_size_cfg2_properties():
 @pytest.mark.parametrize("log2dims,sc,R", [((3, 3, 3, 3, 5), 4, 3.0), ((3, 3, 3, 3, 5), 2, 3.0),
                                            ((4, 3, 3, 3, 5), 4, 3.0), ((2, 2, 2, 2, 3), 2, 1.0)])
 def test_gather_split_bit_identical(log2dims, sc, R):
-    """Dense tiles cut into work items (gather_split) give the oracle's bits:
+    """Dense tiles cut into work items (gather_split), with and without the
+    reach-class grouping of the bins (gather_classes), give the oracle's bits:
     a dense Gaussian cluster (~600 candidate chunks per tile) beside a sparse
-    box, level set and density, fresh and accumulated."""
+    box, level set and density, fresh and accumulated.  With 4^3 bricks and
+    subcell 2 some points reach past both faces of their bin."""
     _log(f"[gpu] gather split {log2dims} sc={sc}")
     rng = np.random.default_rng(4242)
     a = rng.normal([40, 40, 40], 4.0, (40000, 3))
@@ -356,19 +358,25 @@ def test_gather_split_bit_identical(log2dims, sc, R):
         ov.fill_channel(0, 0.25)
         ogather(1.5, accum=True)
         acc_ref = ov.bricks()[:, 1:-1, 1:-1, 1:-1]
-        for split in (0, 1, 5, 64, 1 << 20):
-            gv.SetTuning("gather_split", split)
-            gv.FillChannel(0, -7.0)
-            gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
-            np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], ref, err_msg=f"mode {mode} split {split}")
-            gv.FillChannel(0, 0.25)
-            gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0, accum=True)
-            np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], acc_ref,
-                                          err_msg=f"accumulate, mode {mode} split {split}")
+        for classes in (1, 0):
+            gv.SetTuning("gather_classes", classes)
+            for split in (0, 1, 5, 64, 1 << 20):
+                gv.SetTuning("gather_split", split)
+                gv.FillChannel(0, -7.0)
+                gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
+                np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], ref,
+                                              err_msg=f"mode {mode} classes {classes} split {split}")
+                gv.FillChannel(0, 0.25)
+                gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0, accum=True)
+                np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], acc_ref,
+                                              err_msg=f"accumulate, mode {mode} classes {classes} split {split}")
+    assert gv.GetTuning("gather_classes") == 0
     with pytest.raises(Exception):
         gv.SetTuning("gather_split", -1)
     with pytest.raises(Exception):
         gv.SetTuning("gather_split", (1 << 20) + 1)
+    with pytest.raises(Exception):
+        gv.SetTuning("gather_classes", 2)
 
 
 @pytest.mark.parametrize("mode", [1, 0])
