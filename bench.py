@@ -491,14 +491,15 @@ def main():
                                "UpdateApron(3.0), 10 ABC-advected frames, full rebuild per frame",
                    "n_points": n, "frames": args.frames, "leaf_bricks": int(nleaf), "depth": inf.depth,
                    "sc_len": int(inf.sc_len)},
-        "roofline": {"bound": "hbm", "kernel": "k_gather_rl (GatherLevelSet)", "achieved": round(achieved / 1e9, 2),
+        "roofline": {"bound": "hbm",
+                     "kernel": "GatherLevelSet call (k_gather_classes + k_gather_rl + k_gather_items + k_gather_finish)", "achieved": round(achieved / 1e9, 2),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
                      "traffic": traffic, "alg_bytes_per_launch": gather_bytes,
                      "avg_launch_ms": round(stage["gather_levelset_ms"], 4),
                      "traffic_source": "profiles/gather_levelset_config2_counters.md" if traffic else None,
-                     # the kernel is bound by vector-ALU issue, not HBM: wave64 VALU instructions per launch
+                     # the call is bound by vector-ALU issue, not HBM: wave64 VALU instructions of its kernels
                      # (same counter summary) over what 256 CUs issue at one per cycle at 2.4 GHz in the
-                     # event-timed launch duration
+                     # event-timed call duration
                      "valu_insts_per_launch": valu,
                      "valu_issue_frac": (round(valu / (stage["gather_levelset_ms"] / 1e3 * CU_COUNT * CLOCK_HZ), 3)
                                          if valu else None)},
