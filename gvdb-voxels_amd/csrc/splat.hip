@@ -341,64 +341,123 @@ __global__ __launch_bounds__(256) void k_gather_box(const float4* sorted, const 
 // of its bin (level-0 cell).  A tile in cell c needs, from the bin at c + o,
 // only the classes that can reach c: bit a set where o_a = -1, bit a clear
 // where o_a = +1 (unless a point of that bin reaches past both faces along
-// a), anything where o_a = 0.  One wave per bin reorders the bin's points by
-// class (stably) into csorted as (x, y, z, packed ranges), writes the class
-// starts to coff[8 ci + k] (coff[8 nbin] = the total) and the axes with
-// points reaching past both faces to both[ci].  CLS_UNROLL chunks of 64
-// entries are loaded at once; a bin of at most that many entries stays in
-// registers between counting and placing.  The order inside a bin does not
+// a), anything where o_a = 0.  Each bin's points are reordered by class
+// (stably) into csorted as (x, y, z, packed ranges), the class starts go to
+// coff[8 ci + k] (coff[8 nbin] = the total) and the axes with points
+// reaching past both faces to both[ci].  The order inside a bin does not
 // change a gather: its result is a minimum.
-constexpr int CLS_UNROLL = 4;
+constexpr int CLS_UNROLL = 4;   // chunks of 64 entries loaded at once
+
+__device__ __forceinline__ void bin_origin(const CellGrid& gb, int64_t ci, int res0, int (&O)[3]) {
+    const int64_t nyz = (int64_t)gb.ny * gb.nz;
+    O[0] = ((int)(ci / nyz) + gb.ox) * res0;
+    O[1] = ((int)((ci / gb.nz) % gb.ny) + gb.oy) * res0;
+    O[2] = ((int)(ci % gb.nz) + gb.oz) * res0;
+}
+
+// Entries j0 + 64 u + lane (u < CLS_UNROLL) of a bin ending at `end`: the
+// point with its packed ranges in .w and its class (-1: no entry); bb
+// collects the axes along which one of them reaches past both faces.
+__device__ __forceinline__ void class_load(const float4* sorted, int j0, int end, const int (&O)[3], int res0, float R,
+                                           int sc, int scs, float cut, float4 (&q)[CLS_UNROLL],
+                                           int (&cls)[CLS_UNROLL], uint32_t& bb) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int u = 0; u < CLS_UNROLL; u++) {
+        const int j = j0 + 64 * u + lane;
+        cls[u] = -1;
+        if (j < end) {
+            q[u] = sorted[j];
+            cls[u] = 0;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < CLS_UNROLL; u++) {
+        if (cls[u] < 0) continue;
+        int lo_c[3], hi_c[3];
+        const uint32_t bw = gather_box_word(q[u], R, sc, scs, cut, lo_c, hi_c);
+        q[u].w = __uint_as_float(bw);
+        if (!(bw >> 31)) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const bool up = hi_c[a] >= O[a] + res0;
+                cls[u] |= (int)up << a;
+                bb |= (uint32_t)(up && lo_c[a] < O[a]) << a;
+            }
+        }
+    }
+}
+
+// Places the loaded entries: base[k] is where the next entry of class k
+// goes (advanced past them); entries go in order u, then lane.
+__device__ __forceinline__ void class_place(const float4 (&q)[CLS_UNROLL], const int (&cls)[CLS_UNROLL], int (&base)[8],
+                                            float4* csorted) {
+#pragma unroll
+    for (int u = 0; u < CLS_UNROLL; u++) {
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint64_t bal = __ballot(cls[u] == k);
+            if (cls[u] == k)
+                pos = base[k] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            base[k] += __popcll(bal);
+        }
+        if (cls[u] >= 0) csorted[pos] = q[u];
+    }
+}
+
+__device__ __forceinline__ void class_count(const int (&cls)[CLS_UNROLL], int (&cnt)[8]) {
+#pragma unroll
+    for (int u = 0; u < CLS_UNROLL; u++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) cnt[k] += __popcll(__ballot(cls[u] == k));
+}
+
+__device__ __forceinline__ uint32_t wave_or3(uint32_t bb) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (__ballot((bb >> a) & 1u)) r |= 1u << a;
+    return r;
+}
+
+// One wave per bin.  A bin of at most CLS_SPAN entries is counted and
+// placed from registers.  A larger bin is cut into units of CLS_SPAN entries
+// (in order), appended to units / bigs (ctr[0] units, ctr[1] bins, zero on
+// entry) for k_class_count, k_class_scan and k_class_place.
+constexpr int CLS_SPAN = 64 * CLS_UNROLL;
 
 __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
                                                        float R, int sc, int scs, float cut, float4* csorted,
-                                                       int32_t* coff, uint8_t* both) {
+                                                       int32_t* coff, uint8_t* both, int32_t* ctr, int2* units,
+                                                       int4* bigs) {
     const int lane = threadIdx.x;
     const int64_t nbin = gb.count();
-    const int64_t nyz = (int64_t)gb.ny * gb.nz;
     for (int64_t ci = blockIdx.x; ci < nbin; ci += gridDim.x) {
         const int beg = off[ci], end = off[ci + 1];
-        const int O[3] = {((int)(ci / nyz) + gb.ox) * res0, ((int)((ci / gb.nz) % gb.ny) + gb.oy) * res0,
-                          ((int)(ci % gb.nz) + gb.oz) * res0};
+        if (end - beg > CLS_SPAN) {
+            const int nu = (end - beg + CLS_SPAN - 1) / CLS_SPAN;
+            int ub = 0;
+            if (lane == 0) {
+                ub = atomicAdd(&ctr[0], nu);
+                bigs[atomicAdd(&ctr[1], 1)] = make_int4((int)ci, ub, nu, 0);
+            }
+            ub = __shfl(ub, 0);
+            for (int u = lane; u < nu; u += 64) units[ub + u] = make_int2((int)ci, u);
+            continue;
+        }
+        int O[3];
+        bin_origin(gb, ci, res0, O);
         float4 q[CLS_UNROLL];
-        int cls[CLS_UNROLL];   // -1: no entry
-        uint32_t bb = 0;       // axes with a point reaching past both faces (this lane)
-        // entries j0 + 64 u + lane: point, packed ranges in .w, class
-        auto load = [&](int j0) {
-#pragma unroll
-            for (int u = 0; u < CLS_UNROLL; u++) {
-                const int j = j0 + 64 * u + lane;
-                cls[u] = -1;
-                if (j < end) {
-                    q[u] = sorted[j];
-                    cls[u] = 0;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < CLS_UNROLL; u++) {
-                if (cls[u] < 0) continue;
-                int lo_c[3], hi_c[3];
-                const uint32_t bw = gather_box_word(q[u], R, sc, scs, cut, lo_c, hi_c);
-                q[u].w = __uint_as_float(bw);
-                if (!(bw >> 31)) {
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        const bool up = hi_c[a] >= O[a] + res0;
-                        cls[u] |= (int)up << a;
-                        bb |= (uint32_t)(up && lo_c[a] < O[a]) << a;
-                    }
-                }
-            }
-        };
+        int cls[CLS_UNROLL];
+        uint32_t bb = 0;
         int base[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = 0;
-        for (int j0 = beg; j0 < end; j0 += 64 * CLS_UNROLL) {
-            load(j0);
-#pragma unroll
-            for (int u = 0; u < CLS_UNROLL; u++)
-#pragma unroll
-                for (int k = 0; k < 8; k++) base[k] += __popcll(__ballot(cls[u] == k));
+        if (end > beg) {
+            class_load(sorted, beg, end, O, res0, R, sc, scs, cut, q, cls, bb);
+            class_count(cls, base);
         }
         int acc = beg;
 #pragma unroll
@@ -408,30 +467,124 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
             if (lane == k) coff[ci * 8 + k] = acc;
             acc += c;
         }
-        uint32_t bb_all = 0;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            if (__ballot((bb >> a) & 1u)) bb_all |= 1u << a;
+        const uint32_t bb_all = wave_or3(bb);
         if (lane == 0) both[ci] = (uint8_t)bb_all;
-        const bool held = end - beg <= 64 * CLS_UNROLL;
-        for (int j0 = beg; j0 < end; j0 += 64 * CLS_UNROLL) {
-            if (!held) load(j0);
+        if (end > beg) class_place(q, cls, base, csorted);
+    }
+    if (blockIdx.x == 0 && lane == 0) coff[nbin * 8] = off[nbin];
+}
+
+// Entries of unit ui: its bin and span.
+__device__ __forceinline__ int64_t unit_span(const int2* units, const int32_t* off, int ui, int& beg, int& end) {
+    const int2 u = units[ui];
+    const int b0 = off[u.x];
+    beg = b0 + u.y * CLS_SPAN;
+    end = min(off[u.x + 1], beg + CLS_SPAN);
+    return u.x;
+}
+
+// Per unit (one wave each): class counts to ucnt[9 ui + k], both-face axes
+// to ucnt[9 ui + 8].
+__global__ __launch_bounds__(64) void k_class_count(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
+                                                    float R, int sc, int scs, float cut, const int32_t* ctr,
+                                                    const int2* units, int32_t* ucnt) {
+    const int lane = threadIdx.x;
+    const int nunit = ctr[0];
+    for (int ui = blockIdx.x; ui < nunit; ui += gridDim.x) {
+        int beg, end;
+        const int64_t ci = unit_span(units, off, ui, beg, end);
+        int O[3];
+        bin_origin(gb, ci, res0, O);
+        float4 q[CLS_UNROLL];
+        int cls[CLS_UNROLL];
+        uint32_t bb = 0;
+        class_load(sorted, beg, end, O, res0, R, sc, scs, cut, q, cls, bb);
+        int cnt[8];
 #pragma unroll
-            for (int u = 0; u < CLS_UNROLL; u++) {
-                int pos = 0;
+        for (int k = 0; k < 8; k++) cnt[k] = 0;
+        class_count(cls, cnt);
+        const uint32_t bbw = wave_or3(bb);
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const uint64_t bal = __ballot(cls[u] == k);
-                    if (cls[u] == k)
-                        pos = base[k] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                    base[k] += __popcll(bal);
+        for (int k = 0; k < 8; k++)
+            if (lane == k) ucnt[(int64_t)ui * 9 + k] = cnt[k];
+        if (lane == 8) ucnt[(int64_t)ui * 9 + 8] = (int32_t)bbw;
+    }
+}
+
+// Per large bin (one wave each): class starts (coff), both-face axes, and
+// each unit's first position per class, written over its counts.
+__global__ __launch_bounds__(64) void k_class_scan(const int32_t* off, const int32_t* ctr, const int4* bigs,
+                                                   int32_t* ucnt, int32_t* coff, uint8_t* both) {
+    const int lane = threadIdx.x;
+    const int nbig = ctr[1];
+    for (int bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
+        const int4 bg = bigs[bi];
+        const int64_t ci = bg.x;
+        const int ub = bg.y, nu = bg.z;
+        int tot[8];
+        uint32_t bbo = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) tot[k] = 0;
+        for (int u0 = 0; u0 < nu; u0 += 64) {
+            const int u = u0 + lane;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int c = u < nu ? ucnt[(int64_t)(ub + u) * 9 + k] : 0;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+                tot[k] += c;
+            }
+            bbo |= u < nu ? (uint32_t)ucnt[(int64_t)(ub + u) * 9 + 8] : 0u;
+        }
+        bbo = wave_or3(bbo);
+        int run[8];
+        int acc = off[ci];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            run[k] = acc;
+            if (lane == k) coff[ci * 8 + k] = acc;
+            acc += tot[k];
+        }
+        if (lane == 0) both[ci] = (uint8_t)bbo;
+        for (int u0 = 0; u0 < nu; u0 += 64) {
+            const int u = u0 + lane;
+            int c[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) c[k] = u < nu ? ucnt[(int64_t)(ub + u) * 9 + k] : 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int incl = c[k];   // inclusive scan over the lanes
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(incl, o);
+                    if (lane >= o) incl += t;
                 }
-                if (cls[u] >= 0) csorted[pos] = q[u];
+                if (u < nu) ucnt[(int64_t)(ub + u) * 9 + k] = run[k] + incl - c[k];
+                run[k] += __shfl(incl, 63);
             }
         }
     }
-    if (blockIdx.x == 0 && lane == 0) coff[nbin * 8] = off[nbin];
+}
+
+// Per unit (one wave each): its entries placed from the positions of k_class_scan.
+__global__ __launch_bounds__(64) void k_class_place(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
+                                                    float R, int sc, int scs, float cut, const int32_t* ctr,
+                                                    const int2* units, const int32_t* ucnt, float4* csorted) {
+    const int nunit = ctr[0];
+    for (int ui = blockIdx.x; ui < nunit; ui += gridDim.x) {
+        int beg, end;
+        const int64_t ci = unit_span(units, off, ui, beg, end);
+        int O[3];
+        bin_origin(gb, ci, res0, O);
+        float4 q[CLS_UNROLL];
+        int cls[CLS_UNROLL];
+        uint32_t bb = 0;
+        class_load(sorted, beg, end, O, res0, R, sc, scs, cut, q, cls, bb);
+        int base[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) base[k] = ucnt[(int64_t)ui * 9 + k];
+        class_place(q, cls, base, csorted);
+    }
 }
 
 struct ChunkIter {
@@ -1622,11 +1775,20 @@ void launch_gather(GatherParams gp, hipStream_t s) {
 }
 
 void launch_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0, float box_radius, int sc,
-                           int sc_shift, float cutoff, float4* csorted, int32_t* coff, uint8_t* both, hipStream_t s) {
+                           int sc_shift, float cutoff, float4* csorted, int32_t* coff, uint8_t* both, int32_t* ctr,
+                           int2* units, int64_t unit_cap, int4* bigs, int64_t big_cap, int32_t* ucnt, hipStream_t s) {
     const int64_t nbin = gb.count();
-    if (nbin > 0)
-        hipLaunchKernelGGL(k_gather_classes, dim3((unsigned)std::min<int64_t>(nbin, 1 << 20)), dim3(64), 0, s, sorted, off,
-                           gb, res0, box_radius, sc, sc_shift, cutoff, csorted, coff, both);
+    if (nbin <= 0) return;
+    launch_fill_i32(ctr, 2, 0, s);
+    hipLaunchKernelGGL(k_gather_classes, dim3((unsigned)std::min<int64_t>(nbin, 1 << 20)), dim3(64), 0, s, sorted, off,
+                       gb, res0, box_radius, sc, sc_shift, cutoff, csorted, coff, both, ctr, units, bigs);
+    const dim3 gu((unsigned)std::max<int64_t>(1, std::min<int64_t>(unit_cap, 8192)));   // persistent: most bins are small
+    hipLaunchKernelGGL(k_class_count, gu, dim3(64), 0, s, sorted, off, gb, res0, box_radius, sc, sc_shift, cutoff, ctr,
+                       units, ucnt);
+    hipLaunchKernelGGL(k_class_scan, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(big_cap, 2048))), dim3(64),
+                       0, s, off, ctr, bigs, ucnt, coff, both);
+    hipLaunchKernelGGL(k_class_place, gu, dim3(64), 0, s, sorted, off, gb, res0, box_radius, sc, sc_shift, cutoff, ctr,
+                       units, ucnt, csorted);
 }
 
 void launch_gather_box(const float4* sorted, const int32_t* total, int64_t nmax, float box_radius, int sc, int sc_shift,

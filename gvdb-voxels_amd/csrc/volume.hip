@@ -101,6 +101,7 @@ struct gvdbx_volume {
     DevBuf gs_items, gs_counts, gs_slots, gs_acc;   // Gather* work items of dense tiles
     DevBuf gbox;                                    // Gather*: packed ranges per bin entry
     DevBuf gcls_pts, gcoff, gcboth;                 // Gather*: bin entries by reach class, class starts, both-face axes
+    DevBuf gcctr, gcunit, gcbig, gcucnt;            // Gather*: units of large bins (counters, units, bins, counts)
     DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
     bool bins_valid = false;
@@ -840,9 +841,17 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
             TRY(ensure(v, v->gcls_pts, (size_t)v->bins_n * 16));
             TRY(ensure(v, v->gcoff, (size_t)(nbin * 8 + 1) * 4));
             TRY(ensure(v, v->gcboth, (size_t)std::max<int64_t>(nbin, 1)));
+            // bins of more than 256 entries: at most entries / 256 of them, cut
+            // into at most entries / 256 + (their number) units of 256
+            const int64_t big_cap = v->bins_n / 256 + 1, unit_cap = v->bins_n / 128 + 2;
+            TRY(ensure(v, v->gcctr, 8));
+            TRY(ensure(v, v->gcunit, (size_t)unit_cap * 8));
+            TRY(ensure(v, v->gcbig, (size_t)big_cap * 16));
+            TRY(ensure(v, v->gcucnt, (size_t)unit_cap * 36));
             launch_gather_classes(gp.sorted, gp.off, gp.gb, gp.brick_res - 2 * gp.apron, gp.box_radius, sc, gp.sc_shift,
                                   gp.cutoff, v->gcls_pts.as<float4>(), v->gcoff.as<int32_t>(), v->gcboth.as<uint8_t>(),
-                                  v->stream);
+                                  v->gcctr.as<int32_t>(), v->gcunit.as<int2>(), unit_cap, v->gcbig.as<int4>(), big_cap,
+                                  v->gcucnt.as<int32_t>(), v->stream);
             gp.sorted = v->gcls_pts.as<float4>();
             gp.coff = v->gcoff.as<int32_t>();
             gp.cboth = v->gcboth.as<uint8_t>();
@@ -999,7 +1008,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
-                      &v->splat_src, &v->gbox, &v->gcls_pts, &v->gcoff, &v->gcboth, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
+                      &v->splat_src, &v->gbox, &v->gcls_pts, &v->gcoff, &v->gcboth, &v->gcctr, &v->gcunit, &v->gcbig, &v->gcucnt, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
                       &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
                       &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
