@@ -348,19 +348,13 @@ __global__ __launch_bounds__(256) void k_gather_box(const float4* sorted, const 
 // change a gather: its result is a minimum.
 constexpr int CLS_UNROLL = 4;   // chunks of 64 entries loaded at once
 
-__device__ __forceinline__ void bin_origin(const CellGrid& gb, int64_t ci, int res0, int (&O)[3]) {
-    const int64_t nyz = (int64_t)gb.ny * gb.nz;
-    O[0] = ((int)(ci / nyz) + gb.ox) * res0;
-    O[1] = ((int)((ci / gb.nz) % gb.ny) + gb.oy) * res0;
-    O[2] = ((int)(ci % gb.nz) + gb.oz) * res0;
-}
-
 // Entries j0 + 64 u + lane (u < CLS_UNROLL) of a bin ending at `end`: the
 // point with its packed ranges in .w and its class (-1: no entry); bb
-// collects the axes along which one of them reaches past both faces.
-__device__ __forceinline__ void class_load(const float4* sorted, int j0, int end, const int (&O)[3], int res0, float R,
-                                           int sc, int scs, float cut, float4 (&q)[CLS_UNROLL],
-                                           int (&cls)[CLS_UNROLL], uint32_t& bb) {
+// collects the axes along which one of them reaches past both faces.  A
+// point's bin is the level-0 cell holding it (cells of side 2^logr0), the
+// same floor the binning takes, so its faces come from the point itself.
+__device__ __forceinline__ void class_load(const float4* sorted, int j0, int end, int logr0, float R, int sc, int scs,
+                                           float cut, float4 (&q)[CLS_UNROLL], int (&cls)[CLS_UNROLL], uint32_t& bb) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int u = 0; u < CLS_UNROLL; u++) {
@@ -376,13 +370,15 @@ __device__ __forceinline__ void class_load(const float4* sorted, int j0, int end
         if (cls[u] < 0) continue;
         int lo_c[3], hi_c[3];
         const uint32_t bw = gather_box_word(q[u], R, sc, scs, cut, lo_c, hi_c);
+        const float pw[3] = {q[u].x, q[u].y, q[u].z};
         q[u].w = __uint_as_float(bw);
         if (!(bw >> 31)) {
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                const bool up = hi_c[a] >= O[a] + res0;
+                const int O = gvdbx_cell(pw[a], logr0) << logr0, res0 = 1 << logr0;
+                const bool up = hi_c[a] >= O + res0;
                 cls[u] |= (int)up << a;
-                bb |= (uint32_t)(up && lo_c[a] < O[a]) << a;
+                bb |= (uint32_t)(up && lo_c[a] < O) << a;
             }
         }
     }
@@ -428,7 +424,7 @@ __device__ __forceinline__ uint32_t wave_or3(uint32_t bb) {
 // entry) for k_class_count, k_class_scan and k_class_place.
 constexpr int CLS_SPAN = 64 * CLS_UNROLL;
 
-__global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
+__global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int logr0,
                                                        float R, int sc, int scs, float cut, float4* csorted,
                                                        int32_t* coff, uint8_t* both, int32_t* ctr, int2* units,
                                                        int4* bigs) {
@@ -447,8 +443,6 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
             for (int u = lane; u < nu; u += 64) units[ub + u] = make_int2((int)ci, u);
             continue;
         }
-        int O[3];
-        bin_origin(gb, ci, res0, O);
         float4 q[CLS_UNROLL];
         int cls[CLS_UNROLL];
         uint32_t bb = 0;
@@ -456,7 +450,7 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = 0;
         if (end > beg) {
-            class_load(sorted, beg, end, O, res0, R, sc, scs, cut, q, cls, bb);
+            class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
             class_count(cls, base);
         }
         int acc = beg;
@@ -485,20 +479,18 @@ __device__ __forceinline__ int64_t unit_span(const int2* units, const int32_t* o
 
 // Per unit (one wave each): class counts to ucnt[9 ui + k], both-face axes
 // to ucnt[9 ui + 8].
-__global__ __launch_bounds__(64) void k_class_count(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
+__global__ __launch_bounds__(64) void k_class_count(const float4* sorted, const int32_t* off, int logr0,
                                                     float R, int sc, int scs, float cut, const int32_t* ctr,
                                                     const int2* units, int32_t* ucnt) {
     const int lane = threadIdx.x;
     const int nunit = ctr[0];
     for (int ui = blockIdx.x; ui < nunit; ui += gridDim.x) {
         int beg, end;
-        const int64_t ci = unit_span(units, off, ui, beg, end);
-        int O[3];
-        bin_origin(gb, ci, res0, O);
+        unit_span(units, off, ui, beg, end);
         float4 q[CLS_UNROLL];
         int cls[CLS_UNROLL];
         uint32_t bb = 0;
-        class_load(sorted, beg, end, O, res0, R, sc, scs, cut, q, cls, bb);
+        class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
         int cnt[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) cnt[k] = 0;
@@ -567,19 +559,17 @@ __global__ __launch_bounds__(64) void k_class_scan(const int32_t* off, const int
 }
 
 // Per unit (one wave each): its entries placed from the positions of k_class_scan.
-__global__ __launch_bounds__(64) void k_class_place(const float4* sorted, const int32_t* off, CellGrid gb, int res0,
+__global__ __launch_bounds__(64) void k_class_place(const float4* sorted, const int32_t* off, int logr0,
                                                     float R, int sc, int scs, float cut, const int32_t* ctr,
                                                     const int2* units, const int32_t* ucnt, float4* csorted) {
     const int nunit = ctr[0];
     for (int ui = blockIdx.x; ui < nunit; ui += gridDim.x) {
         int beg, end;
-        const int64_t ci = unit_span(units, off, ui, beg, end);
-        int O[3];
-        bin_origin(gb, ci, res0, O);
+        unit_span(units, off, ui, beg, end);
         float4 q[CLS_UNROLL];
         int cls[CLS_UNROLL];
         uint32_t bb = 0;
-        class_load(sorted, beg, end, O, res0, R, sc, scs, cut, q, cls, bb);
+        class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
         int base[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = ucnt[(int64_t)ui * 9 + k];
@@ -1780,14 +1770,15 @@ void launch_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb
     const int64_t nbin = gb.count();
     if (nbin <= 0) return;
     launch_fill_i32(ctr, 2, 0, s);
+    const int logr0 = __builtin_ctz((unsigned)res0);
     hipLaunchKernelGGL(k_gather_classes, dim3((unsigned)std::min<int64_t>(nbin, 1 << 20)), dim3(64), 0, s, sorted, off,
-                       gb, res0, box_radius, sc, sc_shift, cutoff, csorted, coff, both, ctr, units, bigs);
+                       gb, logr0, box_radius, sc, sc_shift, cutoff, csorted, coff, both, ctr, units, bigs);
     const dim3 gu((unsigned)std::max<int64_t>(1, std::min<int64_t>(unit_cap, 8192)));   // persistent: most bins are small
-    hipLaunchKernelGGL(k_class_count, gu, dim3(64), 0, s, sorted, off, gb, res0, box_radius, sc, sc_shift, cutoff, ctr,
+    hipLaunchKernelGGL(k_class_count, gu, dim3(64), 0, s, sorted, off, logr0, box_radius, sc, sc_shift, cutoff, ctr,
                        units, ucnt);
     hipLaunchKernelGGL(k_class_scan, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(big_cap, 2048))), dim3(64),
                        0, s, off, ctr, bigs, ucnt, coff, both);
-    hipLaunchKernelGGL(k_class_place, gu, dim3(64), 0, s, sorted, off, gb, res0, box_radius, sc, sc_shift, cutoff, ctr,
+    hipLaunchKernelGGL(k_class_place, gu, dim3(64), 0, s, sorted, off, logr0, box_radius, sc, sc_shift, cutoff, ctr,
                        units, ucnt, csorted);
 }
 
