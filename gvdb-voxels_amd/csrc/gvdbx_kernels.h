@@ -279,9 +279,11 @@ void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 // val[i] = i
 void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, CloudMap cm,
                      uint32_t* key, uint32_t* val, hipStream_t s);   // val may be null
-// sorted[j] = (w of point val[j], val[j]) for keys < ncell
+// sorted[j] = (w of point val[j], val[j]) for keys < ncell; with rp, also
+// the extra-brick reach of those points into reach0 (launch_reach_sorted's work)
 void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
-                       const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s);
+                       const uint32_t* val, uint32_t ncell, float4* sorted, const ReachParams* rp, uint32_t* reach0,
+                       hipStream_t s);
 // off[c] = first sorted position with key >= c, c in [0, ncell]
 void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* off, hipStream_t s);
 

@@ -506,13 +506,56 @@ __device__ __forceinline__ uint32_t reach_mask(const int (&lo)[3], const int (&h
 
 constexpr int REACH_UNROLL = 4;
 
+// Home cell (key, -1 if outside g0) and reach mask of one binned point w
+// (point index idx for the batched clouds' cell shift).
+__device__ __forceinline__ void reach_point(const float (&w)[3], int idx, const ReachParams& rp, int& key, uint32_t& m) {
+    key = -1;
+    m = 0;
+    int c[3], sh[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
+    if (rp.cm.ncloud > 0) {
+        const int cl = rp.cm.cloud_of_point(idx);
+#pragma unroll
+        for (int a = 0; a < 3; a++) sh[a] = rp.cm.cell_off[3 * cl + a];
+    }
+    if (!rp.g0.contains(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2])) return;
+    key = (int)rp.g0.index(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
+    m = 1u << 13;
+    if (rp.use_mask) {
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            int ix = (int)w[a];
+            int bmn = (int)((float)ix - rp.radius);
+            int bmx = (int)((float)ix + 2.0f * rp.radius);
+            lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
+            hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
+        }
+        m = reach_mask(lo, hi);
+    }
+}
+
+// ORs the masks of equal keys over the wave (all 64 lanes take part; lanes
+// with consecutive bin entries) and sets each run's bits with one atomicOr.
+__device__ __forceinline__ void reach_commit(int key, uint32_t m, uint32_t* reach0) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t om = __shfl_down(m, d, 64);
+        int ok = __shfl_down(key, d, 64);
+        if (lane + d < 64 && ok == key) m |= om;
+    }
+    int prev = __shfl_up(key, 1, 64);
+    if (key >= 0 && (lane == 0 || prev != key)) atomicOr(&reach0[key], m);
+}
+
 // Each wave takes 64 consecutive sorted points REACH_UNROLL times per step
 // (loads issued together), ORs the masks of equal home cells across the
 // wave and sets each cell's bits with one atomicOr.
 __global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, const int32_t* total_ptr, int64_t nmax,
                                                       ReachParams rp, uint32_t* reach0) {
     const int64_t total = min((int64_t)*total_ptr, nmax);
-    const int lane = lane_id();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += REACH_UNROLL * stride) {
         float4 pts[REACH_UNROLL];
@@ -528,41 +571,10 @@ __global__ __launch_bounds__(256) void k_reach_sorted(const float4* sorted, cons
             int key = -1;
             uint32_t m = 0;
             if (j < total) {
-                const float4 p = pts[u];
-                const float w[3] = {p.x, p.y, p.z};
-                int c[3], sh[3] = {0, 0, 0};
-#pragma unroll
-                for (int a = 0; a < 3; a++) c[a] = gvdbx_cell(w[a], rp.logr0);
-                if (rp.cm.ncloud > 0) {
-                    const int cl = rp.cm.cloud_of_point(__float_as_int(p.w));
-#pragma unroll
-                    for (int a = 0; a < 3; a++) sh[a] = rp.cm.cell_off[3 * cl + a];
-                }
-                if (rp.g0.contains(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2])) {
-                    key = (int)rp.g0.index(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
-                    m = 1u << 13;
-                    if (rp.use_mask) {
-                        int lo[3], hi[3];
-#pragma unroll
-                        for (int a = 0; a < 3; a++) {
-                            int ix = (int)w[a];
-                            int bmn = (int)((float)ix - rp.radius);
-                            int bmx = (int)((float)ix + 2.0f * rp.radius);
-                            lo[a] = max(gvdbx_cdiv(bmn, rp.logr0) - c[a], -1);
-                            hi[a] = min(gvdbx_cdiv(bmx, rp.logr0) - c[a], 1);
-                        }
-                        m = reach_mask(lo, hi);
-                    }
-                }
+                const float w[3] = {pts[u].x, pts[u].y, pts[u].z};
+                reach_point(w, __float_as_int(pts[u].w), rp, key, m);
             }
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                uint32_t om = __shfl_down(m, d, 64);
-                int ok = __shfl_down(key, d, 64);
-                if (lane + d < 64 && ok == key) m |= om;
-            }
-            int prev = __shfl_up(key, 1, 64);
-            if (key >= 0 && (lane == 0 || prev != key)) atomicOr(&reach0[key], m);
+            reach_commit(key, m, reach0);
         }
     }
 }
@@ -697,9 +709,11 @@ constexpr int BIN_UNROLL = 4;
 
 __global__ __launch_bounds__(256) void k_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz,
                                                     const uint32_t* key, const uint32_t* val, uint32_t ncell,
-                                                    float4* sorted) {
+                                                    float4* sorted, ReachParams rp, uint32_t* reach0) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j0 < n; j0 += BIN_UNROLL * stride) {
+    // wave-uniform steps, so that the reach pass below can shuffle across the wave
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < n; b0 += BIN_UNROLL * stride) {
+        const int64_t j0 = b0 + threadIdx.x;
         uint32_t kk[BIN_UNROLL], ii[BIN_UNROLL];
 #pragma unroll
         for (int u = 0; u < BIN_UNROLL; u++) {
@@ -714,6 +728,21 @@ __global__ __launch_bounds__(256) void k_bin_gather(PointSrc src, int64_t n, flo
 #pragma unroll
         for (int u = 0; u < BIN_UNROLL; u++)
             if (kk[u] < ncell) sorted[j0 + u * stride] = make_float4(w[u][0], w[u][1], w[u][2], __int_as_float((int)ii[u]));
+        if (reach0) {
+            // the extra-brick reach of these points while they are in registers
+            // (the work of k_reach_sorted: the valid entries are the binned ones)
+#pragma unroll 1
+            for (int u = 0; u < BIN_UNROLL; u++) {
+                if (b0 + u * stride >= n) break;
+                int rk = -1;
+                uint32_t m = 0;
+                if (kk[u] < ncell) {
+                    const float wu[3] = {w[u][0], w[u][1], w[u][2]};
+                    reach_point(wu, (int)ii[u], rp, rk, m);
+                }
+                reach_commit(rk, m, reach0);
+            }
+        }
     }
 }
 
@@ -1011,10 +1040,12 @@ void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, Cell
 }
 
 void launch_bin_gather(PointSrc src, int64_t n, float tx, float ty, float tz, const uint32_t* key,
-                       const uint32_t* val, uint32_t ncell, float4* sorted, hipStream_t s) {
+                       const uint32_t* val, uint32_t ncell, float4* sorted, const ReachParams* rp, uint32_t* reach0,
+                       hipStream_t s) {
+    const ReachParams r = rp ? *rp : ReachParams{};
     if (n > 0)
         hipLaunchKernelGGL(k_bin_gather, dim3(grid_for(n, 256 * BIN_UNROLL)), dim3(256), 0, s, src, n, tx, ty, tz, key,
-                           val, ncell, sorted);
+                           val, ncell, sorted, r, rp ? reach0 : nullptr);
 }
 
 void launch_bin_offsets(const uint32_t* key, int64_t n, int64_t ncell, int32_t* off, hipStream_t s) {

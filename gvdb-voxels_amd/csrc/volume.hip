@@ -318,7 +318,11 @@ int determine_depth(const gvdbx_volume* v, const float pmin[3], const float pmax
 // holds its points in ascending index order.  Bins made by RebuildTopology
 // or an earlier call are reused while SetPoints has not been called again and
 // the point count and translation match.
-int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable) {
+// With rp, points binned by this call also mark their extra-brick reach in
+// reach0 (*reached set); bins that are reused do not.
+int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const ReachParams* rp = nullptr,
+                bool* reached = nullptr) {
+    if (reached) *reached = false;
     (void)stable;   // sorted bins are always in ascending point order
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
     const CellGrid& g0 = v->grid[0];
@@ -350,7 +354,9 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable) {
                                   v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s);
     const uint32_t* skey = flip ? v->bkey2.as<uint32_t>() : v->bkey.as<uint32_t>();
     const uint32_t* sval = flip ? v->bval2.as<uint32_t>() : v->bval.as<uint32_t>();
-    launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), s);
+    launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), rp,
+                      v->reach0.as<uint32_t>(), s);
+    if (reached) *reached = rp != nullptr;
     launch_bin_offsets(skey, n, nb, v->bin_off.as<int32_t>(), s);
     TRY(check_launch("bin points"));
     v->gbin = gb;
@@ -616,10 +622,12 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     // then derive occupancy and extra-brick reach from the cell-sorted points
     v->grid[0] = grid[0];
     v->bins_valid = false;
-    TRY(ensure_bins(v, n, orig, false));
     ReachParams rp{grid[0], r0, v->logdim[0], (extra && mask_ok) ? 1 : 0, radius, cloud_map(v)};
-    launch_reach_sorted(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), n, rp,
-                        v->reach0.as<uint32_t>(), s);
+    bool reached = false;
+    TRY(ensure_bins(v, n, orig, false, &rp, &reached));
+    if (!reached)
+        launch_reach_sorted(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), n, rp,
+                            v->reach0.as<uint32_t>(), s);
     if (extra && !mask_ok)
         launch_mark_boxes(v->pos, n, orig[0], orig[1], orig[2], grid[0], r0, radius, v->flag_all[0].as<uint8_t>(), s);
     launch_expand_reach(grid[0], v->reach0.as<uint32_t>(), v->flag_base[0].as<uint8_t>(), v->flag_all[0].as<uint8_t>(),
