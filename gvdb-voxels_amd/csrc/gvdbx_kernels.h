@@ -268,10 +268,12 @@ void launch_scatter_density(ScatterParams sp, hipStream_t s);
 // scan_tiles(radix_hist_entries(n) + 1) uint32.  Returns 1 when the result
 // ended in tkeys/tvals (odd pass count), else 0.
 int64_t radix_hist_entries(int64_t n);
+// iota: vals holds no input; the values are the entries' indices 0..n-1
+// (the first pass makes them), vals still serves as a buffer.
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
-                     int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s);
+                     int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota = false);
 int radix_sort_pairs32(uint32_t* keys, uint32_t* vals, uint32_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
-                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s);
+                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota = false);
 void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 
 // ---------------- binning by radix sort ----------------

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
         vr[r] = 0;
         if (li < tn) {
             kr[r] = keys[t0 + li];
-            vr[r] = vals[t0 + li];
+            vr[r] = vals ? vals[t0 + li] : (uint32_t)(t0 + li);   // no vals: each entry's index
         }
     }
     int32_t* wc = wcnt[wave];
@@ -210,8 +210,11 @@ void radix_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, int6
 
 template <typename K>
 int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t n, int begin_bit, int end_bit,
-                    int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
-    if (n <= 1 || end_bit <= begin_bit) return 0;
+                    int32_t* hist, int32_t* off, uint32_t* partials, bool iota, hipStream_t s) {
+    if (n <= 1 || end_bit <= begin_bit) {
+        if (iota) launch_iota_u32(vals, n, s);
+        return 0;
+    }
     const int64_t ntiles = radix_hist_entries(n) / RS_MAX_BINS;
     const int maxdb = sizeof(K) == 8 ? 9 : RS_MAX_BITS;
     const int bits = end_bit - begin_bit;
@@ -223,12 +226,13 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t 
     int flips = 0;
     for (int p = 0, shift = begin_bit; p < np; p++) {
         const int width = bits / np + (p < bits % np ? 1 : 0);
+        const uint32_t* vsrc = iota && p == 0 ? nullptr : vin;   // the first pass makes the indices itself
         if (width <= 8)
-            radix_pass<K, 8>(kin, vin, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+            radix_pass<K, 8>(kin, vsrc, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
         else if (width == 9)
-            radix_pass<K, 9>(kin, vin, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+            radix_pass<K, 9>(kin, vsrc, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
         else if constexpr (sizeof(K) == 4)
-            radix_pass<K, 10>(kin, vin, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+            radix_pass<K, 10>(kin, vsrc, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
         shift += width;
         K* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
@@ -238,11 +242,11 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t 
 }
 
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
-                     int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
-    return radix_sort_impl<uint64_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, s);
+                     int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota) {
+    return radix_sort_impl<uint64_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, iota, s);
 }
 
 int radix_sort_pairs32(uint32_t* keys, uint32_t* vals, uint32_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
-                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
-    return radix_sort_impl<uint32_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, s);
+                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota) {
+    return radix_sort_impl<uint32_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, iota, s);
 }

@@ -347,11 +347,10 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     TRY(ensure(v, v->sorted, (size_t)m * 16));
     int end_bit = 1;
     while (((int64_t)1 << end_bit) <= nb) end_bit++;
-    launch_bin_keys(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], cloud_map(v), v->bkey.as<uint32_t>(),
-                    v->bval.as<uint32_t>(), s);
+    launch_bin_keys(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], cloud_map(v), v->bkey.as<uint32_t>(), nullptr, s);
     int flip = radix_sort_pairs32(v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), v->bkey2.as<uint32_t>(),
                                   v->bval2.as<uint32_t>(), n, 0, end_bit, v->rs_hist.as<int32_t>(),
-                                  v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s);
+                                  v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s, true);   // values: point indices
     const uint32_t* skey = flip ? v->bkey2.as<uint32_t>() : v->bkey.as<uint32_t>();
     const uint32_t* sval = flip ? v->bval2.as<uint32_t>() : v->bval.as<uint32_t>();
     launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), rp,
@@ -1596,11 +1595,10 @@ int raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int
         TRY(ensure(v, v->vis_hist, (size_t)nh * 4));
         TRY(ensure(v, v->vis_off, (size_t)(nh + 1) * 4));
         TRY(ensure(v, v->vis_part, (size_t)(scan_tiles(std::max(nh, m) + 1) + 1) * 4));
-        launch_iota_u32(v->vis_cidx.as<uint32_t>(), m, s);
         const int cflip = radix_sort_pairs(v->vis_ctag.as<uint64_t>(), v->vis_cidx.as<uint32_t>(),
                                            v->vis_ctag2.as<uint64_t>(), v->vis_cidx2.as<uint32_t>(), m, 0,
                                            32 + v->vis_wave_bits, v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(),
-                                           v->vis_part.as<uint32_t>(), s);
+                                           v->vis_part.as<uint32_t>(), s, true);   // values: chunk indices
         const uint32_t* corder = cflip ? v->vis_cidx2.as<uint32_t>() : v->vis_cidx.as<uint32_t>();
         int32_t* csz = v->vis_coff.as<int32_t>() + (m + 1);
         int32_t* coff = v->vis_coff.as<int32_t>();
