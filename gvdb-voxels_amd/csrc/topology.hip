@@ -613,20 +613,32 @@ __device__ __forceinline__ unsigned long long subcell_count(const float4& p, con
     unsigned long long cnt = 0;
     if (near && bp.g0.contains(hc[0], hc[1], hc[2])) {
         const uint32_t m = bp.nbmask[bp.g0.index(hc[0], hc[1], hc[2])];
-        int n[3][3];   // at most r0 / sc + 1 each: 24-bit products are exact
+        // n[a][d]: the box's subcells along axis a inside cell hc[a] + d - 1
+        // (the subcells_in_cell count: box and cell ends are multiples of
+        // sc, so b - a + sc over sc is the count, <= 0 when empty); at most
+        // r0 / sc <= 32 each (bricks are at most 32^3), so they pack into
+        // bytes and every 24-bit product below stays exact
+        int n[3][3];
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+        for (int a = 0; a < 3; a++) {
+            const int lo0 = max(lo[a], 0), base = hc[a] << bp.logr0;
 #pragma unroll
-            for (int d = 0; d < 3; d++) n[a][d] = subcells_in_cell(lo[a], hi[a], hc[a] + d - 1, r0, sc, bp.sc_shift);
+            for (int d = 0; d < 3; d++) {
+                const int cs = base + (d - 1) * r0;
+                const int first = max(lo0, cs), last = min(hi[a], cs + r0 - sc);
+                n[a][d] = max((last - first + sc) >> bp.sc_shift, 0);
+            }
+        }
+        const uint32_t nx = (uint32_t)n[0][0] | (uint32_t)n[0][1] << 8 | (uint32_t)n[0][2] << 16;
         uint32_t c32 = 0;
 #pragma unroll
         for (int dz = 0; dz < 3; dz++)
 #pragma unroll
             for (int dy = 0; dy < 3; dy++) {
-                const uint32_t nyz = __umul24(n[1][dy], n[2][dz]);
-#pragma unroll
-                for (int dx = 0; dx < 3; dx++)
-                    c32 += ((m >> (dz * 9 + dy * 3 + dx)) & 1u) ? __umul24(n[0][dx], nyz) : 0u;
+                // the row's three leaf bits as bytes, dotted with the x counts
+                const uint32_t bits = (m >> (dz * 9 + dy * 3)) & 7u;
+                const uint32_t row = __builtin_amdgcn_udot4(nx, (bits * 0x4081u) & 0x010101u, 0u, false);
+                c32 += __umul24(row, __umul24(n[1][dy], n[2][dz]));
             }
         return c32;
     }
