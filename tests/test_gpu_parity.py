@@ -338,9 +338,9 @@ def test_full_size_cfg2_properties():
 def test_gather_split_bit_identical(log2dims, sc, R):
     """Dense tiles cut into work items (gather_split), with and without the
     reach-class grouping of the bins (gather_classes), give the oracle's bits:
-    a dense Gaussian cluster (~600 candidate chunks per tile) beside a sparse
-    box, level set and density, fresh and accumulated.  With 4^3 bricks and
-    subcell 2 some points reach past both faces of their bin."""
+    a dense Gaussian cluster (~600 candidate chunks per tile, bins of
+    thousands of points) beside a sparse box, level set and density, fresh
+    and accumulated."""
     _log(f"[gpu] gather split {log2dims} sc={sc}")
     rng = np.random.default_rng(4242)
     a = rng.normal([40, 40, 40], 4.0, (40000, 3))
@@ -377,6 +377,45 @@ def test_gather_split_bit_identical(log2dims, sc, R):
         gv.SetTuning("gather_split", (1 << 20) + 1)
     with pytest.raises(Exception):
         gv.SetTuning("gather_classes", 2)
+
+
+@pytest.mark.parametrize("sc", [1, 2])
+def test_gather_classes_both_faces(sc):
+    """8^3 bricks, radius 4: the membership range of a point 3..4 voxels into
+    a bin reaches past both of its faces along an axis (the accumulating
+    gathers, whose cutoff is unbounded, keep all of them; the level-set
+    cutoff 4.5 a thin slice), so the reach-class gather must also take such
+    points from the bin above.  Class path, bin path and oracle agree bit for
+    bit, level set and density, fresh and accumulated."""
+    _log(f"[gpu] gather classes both faces sc={sc}")
+    R = 4.0
+    rng = np.random.default_rng(77 + sc)
+    pts = (rng.random((30000, 3)) * [70, 50, 60] + 5).astype(np.float32)
+    # membership range of InsertPointsSubcell (trunc toward zero, subcell multiples)
+    lo = np.floor(np.trunc(pts - R) / sc) * sc
+    hi = np.floor(np.trunc(pts + 1 + R) / sc) * sc + sc - 1
+    o = np.floor(pts / 8) * 8
+    assert ((lo < o) & (hi >= o + 8)).any(), "no point reaches both faces: the case is not covered"
+    gv, ov, tp = build_pair(pts, R)
+    gl = gv.InsertPointsSubcell(sc, len(pts), R)
+    assert gl == ov.insert_points_subcell(sc, pts, R)
+    for mode in (1, 0):
+        gather = gv.GatherLevelSet if mode == 1 else gv.GatherDensity
+        ogather = ov.gather_level_set if mode == 1 else ov.gather_density
+        ogather(1.5)
+        ref = ov.bricks()[:, 1:-1, 1:-1, 1:-1]
+        ov.fill_channel(0, 0.25)
+        ogather(1.5, accum=True)
+        acc_ref = ov.bricks()[:, 1:-1, 1:-1, 1:-1]
+        for classes in (1, 0):
+            gv.SetTuning("gather_classes", classes)
+            gv.FillChannel(0, -7.0)
+            gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
+            np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], ref, err_msg=f"mode {mode} classes {classes}")
+            gv.FillChannel(0, 0.25)
+            gather(sc, len(pts), 1.5, (0, 0, 0), gl, 0, accum=True)
+            np.testing.assert_array_equal(gv.bricks()[:, 1:-1, 1:-1, 1:-1], acc_ref,
+                                          err_msg=f"accumulate, mode {mode} classes {classes}")
 
 
 @pytest.mark.parametrize("mode", [1, 0])
