@@ -384,12 +384,14 @@ __device__ __forceinline__ void class_load(const float4* sorted, int j0, int end
     }
 }
 
-// Places the loaded entries: base[k] is where the next entry of class k
-// goes (advanced past them); entries go in order u, then lane.
-__device__ __forceinline__ void class_place(const float4 (&q)[CLS_UNROLL], const int (&cls)[CLS_UNROLL], int (&base)[8],
-                                            float4* csorted) {
+// Places the loaded entries of the first nu chunks (the others hold none):
+// base[k] is where the next entry of class k goes (advanced past them);
+// entries go in order u, then lane.
+__device__ __forceinline__ void class_place(const float4 (&q)[CLS_UNROLL], const int (&cls)[CLS_UNROLL], int nu,
+                                            int (&base)[8], float4* csorted) {
 #pragma unroll
     for (int u = 0; u < CLS_UNROLL; u++) {
+        if (u >= nu) break;
         int pos = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
@@ -403,12 +405,17 @@ __device__ __forceinline__ void class_place(const float4 (&q)[CLS_UNROLL], const
     }
 }
 
-__device__ __forceinline__ void class_count(const int (&cls)[CLS_UNROLL], int (&cnt)[8]) {
+__device__ __forceinline__ void class_count(const int (&cls)[CLS_UNROLL], int nu, int (&cnt)[8]) {
 #pragma unroll
-    for (int u = 0; u < CLS_UNROLL; u++)
+    for (int u = 0; u < CLS_UNROLL; u++) {
+        if (u >= nu) break;
 #pragma unroll
         for (int k = 0; k < 8; k++) cnt[k] += __popcll(__ballot(cls[u] == k));
+    }
 }
+
+// Chunks of 64 entries holding any of [j0, end), at most CLS_UNROLL.
+__device__ __forceinline__ int class_chunks(int j0, int end) { return min((end - j0 + 63) >> 6, CLS_UNROLL); }
 
 __device__ __forceinline__ uint32_t wave_or3(uint32_t bb) {
     uint32_t r = 0;
@@ -451,7 +458,7 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
         for (int k = 0; k < 8; k++) base[k] = 0;
         if (end > beg) {
             class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
-            class_count(cls, base);
+            class_count(cls, class_chunks(beg, end), base);
         }
         int acc = beg;
 #pragma unroll
@@ -463,7 +470,7 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
         }
         const uint32_t bb_all = wave_or3(bb);
         if (lane == 0) both[ci] = (uint8_t)bb_all;
-        if (end > beg) class_place(q, cls, base, csorted);
+        if (end > beg) class_place(q, cls, class_chunks(beg, end), base, csorted);
     }
     if (blockIdx.x == 0 && lane == 0) coff[nbin * 8] = off[nbin];
 }
@@ -494,7 +501,7 @@ __global__ __launch_bounds__(64) void k_class_count(const float4* sorted, const 
         int cnt[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) cnt[k] = 0;
-        class_count(cls, cnt);
+        class_count(cls, class_chunks(beg, end), cnt);
         const uint32_t bbw = wave_or3(bb);
 #pragma unroll
         for (int k = 0; k < 8; k++)
@@ -573,7 +580,7 @@ __global__ __launch_bounds__(64) void k_class_place(const float4* sorted, const 
         int base[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = ucnt[(int64_t)ui * 9 + k];
-        class_place(q, cls, base, csorted);
+        class_place(q, cls, class_chunks(beg, end), base, csorted);
     }
 }
 
