@@ -61,7 +61,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* keys, int64_t n
 // The tile is first reordered by digit in LDS (stable), then written out in
 // that order: consecutive threads write consecutive addresses of one digit's
 // run, so the global stores coalesce.  Thread t owns bins [t R, t R + R).
-template <typename K, int DB>
+template <typename K, int DB, int ITEMS = RS_ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const uint32_t* vals, int64_t n,
                                                             int shift, uint32_t mask, int64_t ntiles,
                                                             const int32_t* hist, const int32_t* off, K* okeys,
@@ -73,11 +73,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
     __shared__ int32_t tstart[B];   // start of each digit inside the tile
     __shared__ int32_t wcnt[RS_WAVES][B];   // per wave: running digit counts, then offsets
     __shared__ int32_t wtot[RS_WAVES];
-    __shared__ K lkey[RS_TILE];
-    __shared__ uint32_t lval[RS_TILE];
+    __shared__ K lkey[(RS_THREADS * ITEMS)];
+    __shared__ uint32_t lval[(RS_THREADS * ITEMS)];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
-    const int tn = (int)min((int64_t)RS_TILE, n - t0);
+    const int64_t t0 = (int64_t)blockIdx.x * (RS_THREADS * ITEMS);
+    const int tn = (int)min((int64_t)(RS_THREADS * ITEMS), n - t0);
     int c[R];
     int csum = 0;
 #pragma unroll
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
         }
         __syncthreads();
     }
-    // Wave w ranks the tile's keys [w * 64 * RS_ITEMS, (w + 1) * 64 * RS_ITEMS)
+    // Wave w ranks the tile's keys [w * 64 * ITEMS, (w + 1) * 64 * ITEMS)
     // on its own: 64 per round, lanes in order, against its running digit
     // counts wcnt[w] (only the wave touches them, and its LDS accesses stay
     // in program order), so the rounds need no block barrier.  Then the
@@ -116,12 +116,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
     // goes to tstart[d] + offset of its wave + its rank: equal digits keep
     // their tile order.
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int wbase = wave * 64 * RS_ITEMS;
-    K kr[RS_ITEMS];
-    uint32_t vr[RS_ITEMS];
-    int rk[RS_ITEMS];
+    const int wbase = wave * 64 * ITEMS;
+    K kr[ITEMS];
+    uint32_t vr[ITEMS];
+    int rk[ITEMS];
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {   // all loads in flight at once
+    for (int r = 0; r < ITEMS; r++) {   // all loads in flight at once
         const int li = wbase + r * 64 + lane;
         kr[r] = 0;
         vr[r] = 0;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
     }
     int32_t* wc = wcnt[wave];
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         const bool ok = wbase + r * 64 + lane < tn;
         const int d = ok ? (int)((uint32_t)(kr[r] >> shift) & mask) : 0;
         uint64_t peers = __ballot(ok);
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const K* keys, const 
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
+    for (int r = 0; r < ITEMS; r++) {
         if (wbase + r * 64 + lane < tn) {
             const int d = (int)((uint32_t)(kr[r] >> shift) & mask);
             const int pos = tstart[d] + wc[d] + rk[r];
@@ -194,28 +194,28 @@ void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
 }
 
-template <typename K, int DB>
+template <typename K, int DB, int ITEMS>
 void radix_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, int64_t n, int shift, int width,
-                int64_t ntiles, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
+                int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s) {
+    const int64_t ntiles = std::max<int64_t>((n + RS_THREADS * ITEMS - 1) / (RS_THREADS * ITEMS), 1);
     const int64_t nh = ntiles << DB;
     const int64_t stiles = scan_tiles(nh + 1);
     const uint32_t mask = (1u << width) - 1u;
-    hipLaunchKernelGGL((k_rs_hist<K, DB>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, n, shift, mask, ntiles,
-                       hist);
+    hipLaunchKernelGGL((k_rs_hist<K, DB, ITEMS>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, n, shift, mask,
+                       ntiles, hist);
     launch_count_scan_reduce(hist, nh, partials, partials + stiles, s);
     launch_count_scan_apply(hist, nh, partials, off, s);
-    hipLaunchKernelGGL((k_rs_scatter<K, DB>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, vin, n, shift, mask,
-                       ntiles, hist, off, kout, vout);
+    hipLaunchKernelGGL((k_rs_scatter<K, DB, ITEMS>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, kin, vin, n, shift,
+                       mask, ntiles, hist, off, kout, vout);
 }
 
 template <typename K>
 int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t n, int begin_bit, int end_bit,
-                    int32_t* hist, int32_t* off, uint32_t* partials, bool iota, hipStream_t s) {
+                    int32_t* hist, int32_t* off, uint32_t* partials, bool iota, bool wide, hipStream_t s) {
     if (n <= 1 || end_bit <= begin_bit) {
         if (iota) launch_iota_u32(vals, n, s);
         return 0;
     }
-    const int64_t ntiles = radix_hist_entries(n) / RS_MAX_BINS;
     const int maxdb = sizeof(K) == 8 ? 9 : RS_MAX_BITS;
     const int bits = end_bit - begin_bit;
     const int np = (bits + maxdb - 1) / maxdb;
@@ -227,12 +227,21 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t 
     for (int p = 0, shift = begin_bit; p < np; p++) {
         const int width = bits / np + (p < bits % np ? 1 : 0);
         const uint32_t* vsrc = iota && p == 0 ? nullptr : vin;   // the first pass makes the indices itself
-        if (width <= 8)
-            radix_pass<K, 8>(kin, vsrc, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
-        else if (width == 9)
-            radix_pass<K, 9>(kin, vsrc, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
-        else if constexpr (sizeof(K) == 4)
-            radix_pass<K, 10>(kin, vsrc, kout, vout, n, shift, width, ntiles, hist, off, partials, s);
+        if (wide && p == 0) {   // longer digit runs where the digits are most spread
+            if (width <= 8)
+                radix_pass<K, 8, 2 * RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
+            else if (width == 9)
+                radix_pass<K, 9, 2 * RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
+            else if constexpr (sizeof(K) == 4)
+                radix_pass<K, 10, 2 * RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
+        } else {
+            if (width <= 8)
+                radix_pass<K, 8, RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
+            else if (width == 9)
+                radix_pass<K, 9, RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
+            else if constexpr (sizeof(K) == 4)
+                radix_pass<K, 10, RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
+        }
         shift += width;
         K* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
@@ -243,10 +252,12 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t 
 
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
                      int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota) {
-    return radix_sort_impl<uint64_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, iota, s);
+    return radix_sort_impl<uint64_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, iota, false,
+                                     s);
 }
 
 int radix_sort_pairs32(uint32_t* keys, uint32_t* vals, uint32_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
-                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota) {
-    return radix_sort_impl<uint32_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, iota, s);
+                       int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota,
+                       bool wide) {
+    return radix_sort_impl<uint32_t>(keys, vals, tkeys, tvals, n, begin_bit, end_bit, hist, off, partials, iota, wide, s);
 }

@@ -350,7 +350,8 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     launch_bin_keys(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], cloud_map(v), v->bkey.as<uint32_t>(), nullptr, s);
     int flip = radix_sort_pairs32(v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), v->bkey2.as<uint32_t>(),
                                   v->bval2.as<uint32_t>(), n, 0, end_bit, v->rs_hist.as<int32_t>(),
-                                  v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s, true);   // values: point indices
+                                  v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s, true,
+                                  n >= ((int64_t)1 << 23));   // values: point indices; wide tiles for large point sets
     const uint32_t* skey = flip ? v->bkey2.as<uint32_t>() : v->bkey.as<uint32_t>();
     const uint32_t* sval = flip ? v->bval2.as<uint32_t>() : v->bval.as<uint32_t>();
     launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), rp,
