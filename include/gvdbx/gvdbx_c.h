@@ -92,7 +92,9 @@ int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
  * the same bit for bit; default 16, range [0, 2^20]), "gather_classes"
  * (variant 1: 1 = each bin's points grouped per call by the neighbour cells
  * they can reach, so a tile reads only the groups that can reach it; 0 =
- * every point of the 27 neighbour bins; same result bit for bit; default 1),
+ * every point of the 27 neighbour bins, whose tiles have about four times
+ * the chunks: set gather_split to about 64 with it; same result bit for
+ * bit; default 1),
  * "xcd_remap" (1 = contiguous brick
  * runs per XCD; default 0), "splat_variant" (SplatCubic: 0 = tile pull,
  * 1 = cell push; default 1; cell push needs a (res0 + 2 ceil(r))^3 float
