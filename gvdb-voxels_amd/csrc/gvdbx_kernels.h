@@ -269,9 +269,9 @@ void launch_scatter_density(ScatterParams sp, hipStream_t s);
 // ended in tkeys/tvals (odd pass count), else 0.
 int64_t radix_hist_entries(int64_t n);
 // iota: vals holds no input; the values are the entries' indices 0..n-1
-// (the first pass makes them), vals still serves as a buffer.  wide: the
-// first pass (lowest digit, spread over all its values) takes tiles of twice
-// the entries: longer runs per digit, fewer blocks per CU.
+// (the first pass makes them), vals still serves as a buffer.  wide: tiles
+// of twice the entries (longer runs per digit, fewer blocks per CU), for
+// large sorts.
 int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,
                      int end_bit, int32_t* hist, int32_t* off, uint32_t* partials, hipStream_t s, bool iota = false);
 int radix_sort_pairs32(uint32_t* keys, uint32_t* vals, uint32_t* tkeys, uint32_t* tvals, int64_t n, int begin_bit,

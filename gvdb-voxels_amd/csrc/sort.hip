@@ -227,7 +227,7 @@ int radix_sort_impl(K* keys, uint32_t* vals, K* tkeys, uint32_t* tvals, int64_t 
     for (int p = 0, shift = begin_bit; p < np; p++) {
         const int width = bits / np + (p < bits % np ? 1 : 0);
         const uint32_t* vsrc = iota && p == 0 ? nullptr : vin;   // the first pass makes the indices itself
-        if (wide && p == 0) {   // longer digit runs where the digits are most spread
+        if (wide) {
             if (width <= 8)
                 radix_pass<K, 8, 2 * RS_ITEMS>(kin, vsrc, kout, vout, n, shift, width, hist, off, partials, s);
             else if (width == 9)

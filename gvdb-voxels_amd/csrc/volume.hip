@@ -106,6 +106,7 @@ struct gvdbx_volume {
     CellGrid gbin = {};
     bool bins_valid = false;
     int64_t bins_n = -1;
+    int64_t pts_valid = -1;   // valid points of the last RebuildTopology (-1: unknown)
     float bins_trans[3] = {};
     const char* bins_base = nullptr;
     int64_t bins_stride = 0;
@@ -351,7 +352,8 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     int flip = radix_sort_pairs32(v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), v->bkey2.as<uint32_t>(),
                                   v->bval2.as<uint32_t>(), n, 0, end_bit, v->rs_hist.as<int32_t>(),
                                   v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s, true,
-                                  n >= ((int64_t)1 << 23));   // values: point indices; wide tiles for large point sets
+                                  (v->pts_valid >= 0 ? v->pts_valid : n) >= ((int64_t)1 << 23));
+    // values: point indices; wide tiles for 2^23 valid points or more
     const uint32_t* skey = flip ? v->bkey2.as<uint32_t>() : v->bkey.as<uint32_t>();
     const uint32_t* sval = flip ? v->bval2.as<uint32_t>() : v->bval.as<uint32_t>();
     launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), rp,
@@ -463,6 +465,7 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
         memcpy(&bb, v->host, sizeof(bb));
         CloudFrame f;
         TRY(frame_of(v, bb, radius, "", f));
+        v->pts_valid = (int64_t)bb.nvalid;
         depth = f.depth;
         memcpy(root, f.root, sizeof(root));
         memcpy(pmin, f.pmin, sizeof(pmin));
@@ -499,11 +502,13 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
         std::vector<int32_t> box((size_t)nc * 6);
         int64_t area = 0;
         int maxx = 1;
+        v->pts_valid = 0;
         for (int c = 0; c < nc; c++) {
             CloudFrame f;
             char who[48];
             snprintf(who, sizeof(who), " in cloud %d", c);
             TRY(frame_of(v, bbs[c], radius, who, f));
+            v->pts_valid += (int64_t)bbs[c].nvalid;
             for (int a = 0; a < 3; a++) {
                 box[6 * c + a] = (int32_t)(f.lo_v[a] / r0);
                 box[6 * c + 3 + a] = (int32_t)(f.hi_v[a] / r0) + 1;   // exclusive
