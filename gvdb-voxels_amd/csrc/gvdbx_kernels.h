@@ -227,11 +227,45 @@ struct ApronParams {
     const int32_t* idx0;
     int range0;
     int logr0;
-    float* chan;
+    float* chan;              // 4-byte voxels of either type (the apron copies words)
     int brick_res;
     float boundval;
+    uint32_t bound_bits;      // the word stored outside every leaf (float bits or packed uchar4)
 };
 void launch_apron(ApronParams ap, hipStream_t s);
+
+// ---------------- colour channels (uchar4 words) ----------------
+struct ColorGatherParams {
+    const GNode* leaves;
+    int64_t nleaf;
+    CellGrid gb;              // bins of the last InsertPointsSubcell
+    const int32_t* off;
+    const float4* sorted;     // (x, y, z, point index bits)
+    const char* clr;          // uchar4 per point at clr + i * clr_stride
+    int64_t clr_stride;
+    uint32_t* chan;           // brick-linear colour channel
+    int brick_res, apron, res0, logr0;
+    int sc, sc_shift;
+    float R;                  // InsertPointsSubcell radius
+    int accum;
+};
+// colour mean of the gathers, one workgroup per leaf
+void launch_gather_color(ColorGatherParams cp, hipStream_t s);
+int color_max_subcells();
+struct ColorScatterParams {
+    const GNode* leaves;
+    int64_t nleaf;
+    CellGrid gb;
+    const int32_t* off;
+    const float4* sorted;
+    const char* clr;
+    int64_t clr_stride;
+    uint32_t* chan;
+    int brick_res, apron, res0, logr0;
+    int avg;
+};
+// colour part of ScatterDensity, one workgroup per leaf
+void launch_scatter_color(ColorScatterParams cp, hipStream_t s);
 
 struct InsertParams {
     CellGrid gb;

@@ -602,14 +602,17 @@ __device__ __forceinline__ void shift_corners(float* acc, bool up) {
 // summed in a fixed order at the end.
 // A lane keeps the eight corner sums of its current trilinear cell in
 // registers and flushes a corner when the ray leaves it.  A flush first
-// claims the voxel: the lane writes its tag to claim[voxel] and reads it
-// back.  The one lane that reads its own tag is the only flusher of that
-// voxel that can still see its tag, so it adds with a plain read-add-write;
-// the others (lanes of the same wave on the same voxel, or whose claim
-// another wave overwrote) add afterwards with an LDS atomic.  LDS float
-// atomics are slow (tens of cycles per instruction), so the claim pays for
-// itself whenever most flushes are alone on their voxel, which rays that
-// march side by side make the common case.
+// claims the voxel in its wave's own claim array: the lane writes its lane
+// number to claim[wave][voxel] and reads it back.  Only lanes of the same
+// wave write that array, and all of them in the same instruction, so which
+// lane reads its own number back (the one the LDS kept for that byte) and
+// therefore which lane adds with a plain read-add-write and which add
+// afterwards with an LDS atomic depends on that wave's lanes alone: the
+// summation order inside each wave's gradient copy, and so every bit of the
+// result, does not depend on how the waves are timed against each other.
+// LDS float atomics are slow (tens of cycles per instruction), so the claim
+// pays for itself whenever most flushes are alone on their voxel, which
+// rays that march side by side make the common case.
 template <int BR>
 __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     constexpr int NV = BR * BR * BR;
@@ -620,22 +623,24 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
     struct EALds {
         float dens[NV];
         float grad[NW][NV];
-        int claim[NV];
         int32_t wlist[NW][128];   // per wave: its segments waiting for a lane
+        uint8_t claim[NW][NV];    // per wave: lane number of the last claim of each voxel
     };
     __shared__ EALds lds;
     float* const dens = lds.dens;
     float (*const grad)[NV] = lds.grad;
-    int* const claim = lds.claim;
+    uint8_t* const claim = lds.claim[threadIdx.x >> 6];
     int32_t (*const wlist)[128] = lds.wlist;
     const int64_t leaf = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float* src = p.tv.chan + leaf * (int64_t)NV;
     for (int i = tid; i < NV; i += 256) {
         dens[i] = src[i];
-        claim[i] = -1;
 #pragma unroll
-        for (int w = 0; w < NW; w++) grad[w][i] = 0.0f;
+        for (int w = 0; w < NW; w++) {
+            grad[w][i] = 0.0f;
+            lds.claim[w][i] = 0xFF;
+        }
     }
     __syncthreads();
     const int32_t b0 = p.seg[2 * leaf], b1 = p.seg[2 * leaf + 1];
@@ -676,9 +681,9 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
                 if (acc[m] != 0.0f) {
                     // the cell is inside the padded brick (samples stay in the interior range)
                     const int vi = __mul24(cz + (m >> 2), BR * BR) + __mul24(cy + ((m >> 1) & 1), BR) + (cx + (m & 1));
-                    // relaxed atomics: plain LDS stores and loads the compiler may not fold
-                    __hip_atomic_store(&claim[vi], tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (__hip_atomic_load(&claim[vi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == tid)
+                    // relaxed atomics: plain LDS byte stores and loads the compiler may not fold
+                    __hip_atomic_store(&claim[vi], (uint8_t)lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    if (__hip_atomic_load(&claim[vi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) == (uint8_t)lane)
                         g[vi] = g[vi] + acc[m];
                     else
                         atomicAdd(&g[vi], acc[m]);

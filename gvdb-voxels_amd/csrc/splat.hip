@@ -1602,7 +1602,9 @@ __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
         if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
         if (leaf >= ap.nleaf) leaf = -1;
     }
-    float v[8];
+    // voxels are moved as 4-byte words: float or packed uchar4 channels alike
+    const uint32_t* chan = (const uint32_t*)ap.chan;
+    uint32_t v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
         const int i = iv[u] >= 0 ? iv[u] : 111;   // 111 = (1,1,1): an interior voxel, neighbour 13 (itself)
@@ -1612,10 +1614,10 @@ __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
         const int dz = z == 0 ? -1 : (z == 9 ? 1 : 0);
         const int nl = __shfl(leaf, (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));   // every lane takes part
         // interior voxel of the neighbour holding world voxel mPos + (x,y,z) - 1
-        v[u] = ap.boundval;
-        if (iv[u] >= 0 && nl >= 0) v[u] = ap.chan[(int64_t)nl * 1000 + i - 8 * (dz * 100 + dy * 10 + dx)];
+        v[u] = ap.bound_bits;
+        if (iv[u] >= 0 && nl >= 0) v[u] = chan[(int64_t)nl * 1000 + i - 8 * (dz * 100 + dy * 10 + dx)];
     }
-    float* brick = ap.chan + k * 1000;
+    uint32_t* brick = (uint32_t*)ap.chan + k * 1000;
 #pragma unroll
     for (int u = 0; u < 8; u++)
         if (iv[u] >= 0) brick[iv[u]] = v[u];
@@ -1641,7 +1643,8 @@ __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
     }
     __syncthreads();
     const int nv = br * br * br;
-    float* brick = ap.chan + k * (int64_t)nv;
+    const uint32_t* chan = (const uint32_t*)ap.chan;   // 4-byte words of either channel type
+    uint32_t* brick = (uint32_t*)ap.chan + k * (int64_t)nv;
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
         const int x = i % br, y = (i / br) % br, z = i / (br * br);
         const int dx = x < a ? -1 : (x >= a + r0 ? 1 : 0);
@@ -1649,9 +1652,9 @@ __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
         const int dz = z < a ? -1 : (z >= a + r0 ? 1 : 0);
         if ((dx | dy | dz) == 0) continue;   // interior voxel
         const int32_t leaf = nbl[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)];
-        float v = ap.boundval;
+        uint32_t v = ap.bound_bits;
         if (leaf >= 0)   // interior voxel of the neighbour holding world voxel mPos + (x,y,z) - a
-            v = ap.chan[(int64_t)leaf * nv + ((int64_t)(z - dz * r0) * br + (y - dy * r0)) * br + (x - dx * r0)];
+            v = chan[(int64_t)leaf * nv + ((int64_t)(z - dz * r0) * br + (y - dy * r0)) * br + (x - dx * r0)];
         brick[i] = v;
     }
 }

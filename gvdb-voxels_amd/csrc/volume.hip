@@ -61,6 +61,7 @@ struct gvdbx_volume {
     int subdim[3] = {0, 0, 0};
     int apron = -1;
     bool has_chan[GVDBX_MAXCHAN] = {};
+    int chan_type[GVDBX_MAXCHAN] = {};   // GVDBX_FLOAT or GVDBX_UCHAR4 (4-byte voxels either way)
     int chan_filter[GVDBX_MAXCHAN] = {};
     int chan_border[GVDBX_MAXCHAN] = {};
     DevBuf chan[GVDBX_MAXCHAN];
@@ -71,6 +72,8 @@ struct gvdbx_volume {
     PointSrc pos{nullptr, 12};
     const char* wts = nullptr;
     int64_t wstride = 4;
+    const char* clr = nullptr;   // uchar4 colour per point (SetPoints clr), or null
+    int64_t clr_stride = 4;
     // topology
     bool has_topo = false;
     int depth = 0, top_lev = 1;
@@ -291,6 +294,21 @@ int check_vol(gvdbx_volume* v) {
 int check_chan(gvdbx_volume* v, int chan) {
     if (chan < 0 || chan >= GVDBX_MAXCHAN || !v->has_chan[chan]) FAIL(GVDBX_ERR_ARG, "channel %d does not exist", chan);
     return GVDBX_OK;
+}
+
+int check_chan_type(gvdbx_volume* v, int chan, int dtype, const char* what) {
+    TRY(check_chan(v, chan));
+    if (v->chan_type[chan] != dtype)
+        FAIL(GVDBX_ERR_ARG, "%s: channel %d is a %s channel, this call needs a %s one", what, chan,
+             v->chan_type[chan] == GVDBX_UCHAR4 ? "uchar4" : "float", dtype == GVDBX_UCHAR4 ? "uchar4" : "float");
+    return GVDBX_OK;
+}
+
+// float -> uchar as CUDA's saturating conversion (truncation, clamp, NaN -> 0)
+uint32_t sat_u8(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 255.0f) return 255u;
+    return (uint32_t)x;
 }
 
 // Host restatement of DetermineDepth (VG.cpp:821-837) over GetCoveringNode (VG.cpp:2915-2931).
@@ -815,14 +833,41 @@ int need_atlas(gvdbx_volume* v, const char* what) {
     return GVDBX_OK;
 }
 
-int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int chan, int accum) {
+int gather_color(gvdbx_volume* v, int chan_clr, int accum) {
+    ColorGatherParams cp{};
+    cp.leaves = v->nodes[0].as<GNode>();
+    cp.nleaf = v->cnt[0];
+    cp.gb = v->gbin;
+    cp.off = v->bin_off.as<int32_t>();
+    cp.sorted = v->sorted.as<float4>();
+    cp.clr = v->clr;
+    cp.clr_stride = v->clr_stride;
+    cp.chan = v->chan[chan_clr].as<uint32_t>();
+    cp.brick_res = brick_res(v);
+    cp.apron = std::max(v->apron, 0);
+    cp.res0 = v->res[0];
+    cp.logr0 = v->logdim[0];
+    cp.sc = v->sc_size;
+    cp.sc_shift = __builtin_ctz((unsigned)v->sc_size);
+    cp.R = v->sc_radius;
+    cp.accum = accum ? 1 : 0;
+    launch_gather_color(cp, v->stream);
+    return check_launch("gather colour");
+}
+
+int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int chan, int chan_clr, int accum) {
     const char* what = mode ? "GatherLevelSet" : "GatherDensity";
     TRY(need_atlas(v, what));
-    TRY(check_chan(v, chan));
+    TRY(check_chan_type(v, chan, GVDBX_FLOAT, what));
+    // point colours make the gather average them into chan_clr
+    // (cuda_gvdb_particles.cuh:718-731,757-771: active when the subcell
+    // colour list exists, i.e. SetPoints was given colours)
+    if (v->clr) TRY(check_chan_type(v, chan_clr, GVDBX_UCHAR4, what));
     if (!v->sc_valid || !v->bins_valid) FAIL(GVDBX_ERR_STATE, "%s: InsertPointsSubcell has not been called", what);
     if (sc != v->sc_size) FAIL(GVDBX_ERR_ARG, "%s: subcell size %d differs from InsertPointsSubcell (%d)", what, sc, v->sc_size);
     if (!(radius >= 0.0f) || !isfinite(radius)) FAIL(GVDBX_ERR_ARG, "%s: bad radius", what);
     if (mode == 0 && sc_len == 0) return GVDBX_OK;   // VG.cpp:5321
+    if (v->clr) TRY(gather_color(v, chan_clr, accum));
     GatherParams gp{};
     gp.leaves = v->nodes[0].as<GNode>();
     gp.nleaf = v->cnt[0];
@@ -924,7 +969,7 @@ void fill_cam(const gvdbx_camera* cam, RenderCam& rc) {
 int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp, const char* what,
                 RenderTree& tv, RenderCam& rc, RenderScene& sc) {
     TRY(need_atlas(v, what));
-    TRY(check_chan(v, chan));
+    TRY(check_chan_type(v, chan, GVDBX_FLOAT, what));
     if (!cam || !rp) FAIL(GVDBX_ERR_ARG, "%s: null camera or parameters", what);
     if (v->apron != 1) FAIL(GVDBX_ERR_ARG, "%s: rendering needs apron 1 (AddChannel)", what);
     if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height >= ((int64_t)1 << 31))
@@ -1083,7 +1128,8 @@ int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filte
     TRY(check_vol(v));
     if (!v->configured) FAIL(GVDBX_ERR_STATE, "AddChannel before Configure");
     if (chan < 0 || chan >= GVDBX_MAXCHAN) FAIL(GVDBX_ERR_ARG, "AddChannel: channel %d out of range", chan);
-    if (dtype != GVDBX_FLOAT) FAIL(GVDBX_ERR_ARG, "AddChannel: only float channels are supported");
+    if (dtype != GVDBX_FLOAT && dtype != GVDBX_UCHAR4)
+        FAIL(GVDBX_ERR_ARG, "AddChannel: type %d is not supported (T_FLOAT = 3, T_UCHAR4 = 2)", dtype);
     if (apron < 0 || apron > 1) FAIL(GVDBX_ERR_ARG, "AddChannel: apron must be 0 or 1");
     if (!any_chan(v)) {
         for (int a = 0; a < 3; a++) v->subdim[a] = v->def_axis[a];
@@ -1105,6 +1151,7 @@ int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filte
     v->chan[chan].cap = bytes;
     launch_fill_f32(v->chan[chan].as<float>(), cap * brick_vox(v), 0.0f, v->stream);
     v->has_chan[chan] = true;
+    v->chan_type[chan] = dtype;
     v->chan_filter[chan] = filter;
     v->chan_border[chan] = border;
     v->atlas_cap = cap;
@@ -1114,6 +1161,17 @@ int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filte
 int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value) {
     TRY(check_vol(v));
     TRY(check_chan(v, chan));
+    if (v->chan_type[chan] == GVDBX_UCHAR4) {
+        // all zero clears (ClearChannel); otherwise gvdbOpFillC4
+        // (cuda_gvdb_operators.cuh:368-373) stores (value, value, value, 1) * 255
+        uint32_t w = 0u;
+        if (value != 0.0f) {
+            const uint32_t c = sat_u8(value * 255.0f);
+            w = c | c << 8 | c << 16 | 255u << 24;
+        }
+        launch_fill_i32(v->chan[chan].as<int32_t>(), v->atlas_cap * brick_vox(v), (int32_t)w, v->stream);
+        return check_launch("FillChannel");
+    }
     launch_fill_f32(v->chan[chan].as<float>(), v->atlas_cap * brick_vox(v), value, v->stream);
     return check_launch("FillChannel");
 }
@@ -1194,11 +1252,18 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
 
 int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_stride, const void* vel_dev,
                      int vel_off, int vel_stride, const void* clr_dev, int clr_off, int clr_stride) {
-    (void)vel_dev; (void)vel_off; (void)vel_stride; (void)clr_dev; (void)clr_off; (void)clr_stride;
+    // velocities are accepted and not read: no stage of this path uses them
+    // (the gathers copy them into the subcell lists but never read them,
+    // cuda_gvdb_particles.cuh:696-773)
+    (void)vel_dev; (void)vel_off; (void)vel_stride;
     TRY(check_vol(v));
     if (!pos_dev) FAIL(GVDBX_ERR_ARG, "SetPoints: null position pointer");
     if (pos_stride < 12 || pos_off < 0) FAIL(GVDBX_ERR_ARG, "SetPoints: position stride must be >= 12");
     if (((uintptr_t)pos_dev + pos_off) % 4 || pos_stride % 4) FAIL(GVDBX_ERR_ARG, "SetPoints: positions must be 4-byte aligned");
+    if (clr_dev && (clr_off < 0 || clr_stride < 4 || clr_stride % 4 || ((uintptr_t)clr_dev + clr_off) % 4))
+        FAIL(GVDBX_ERR_ARG, "SetPoints: colours must be 4-byte aligned uchar4 with a stride that is a multiple of 4");
+    v->clr = clr_dev ? (const char*)clr_dev + clr_off : nullptr;
+    v->clr_stride = clr_stride;
     v->pos.base = (const char*)pos_dev + pos_off;
     v->pos.stride = pos_stride;
     v->has_pos = true;
@@ -1227,6 +1292,7 @@ int gvdbx_set_points_batched(gvdbx_volume* v, const void* pos_dev, int pos_strid
     TRY(sync(v));
     v->pos.base = (const char*)pos_dev;
     v->pos.stride = pos_stride;
+    v->clr = nullptr;
     v->has_pos = true;
     v->pos_gen++;
     v->bins_valid = false;
@@ -1305,16 +1371,16 @@ int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float rad
 
 int gvdbx_gather_density(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
                          int64_t sc_len, int chan_density, int chan_clr, int accum) {
-    (void)num_pnts; (void)tx; (void)ty; (void)tz; (void)chan_clr;
+    (void)num_pnts; (void)tx; (void)ty; (void)tz;   // the lists of InsertPointsSubcell fix the points
     TRY(check_vol(v));
-    return gather(v, 0, sc, radius, sc_len, chan_density, accum);
+    return gather(v, 0, sc, radius, sc_len, chan_density, chan_clr, accum);
 }
 
 int gvdbx_gather_level_set(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
                            int64_t sc_len, int chan_levelset, int chan_clr, int accum) {
-    (void)num_pnts; (void)tx; (void)ty; (void)tz; (void)chan_clr;
+    (void)num_pnts; (void)tx; (void)ty; (void)tz;
     TRY(check_vol(v));
-    return gather(v, 1, sc, radius, sc_len, chan_levelset, accum);
+    return gather(v, 1, sc, radius, sc_len, chan_levelset, chan_clr, accum);
 }
 
 int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix) {
@@ -1338,11 +1404,16 @@ int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float
 
 int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp, float tx, float ty, float tz,
                           int expand, int avg_color) {
-    (void)amp; (void)avg_color;   // amp is unused by the reference kernel
+    (void)amp;   // amp is unused by the reference kernel
     TRY(check_vol(v));
     if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "ScatterDensity: not available for batched clouds");
     TRY(need_atlas(v, "ScatterDensity"));
-    TRY(check_chan(v, 0));
+    TRY(check_chan_type(v, 0, GVDBX_FLOAT, "ScatterDensity"));
+    // point colours go to channel 1 (averaged) or 3 (last point), the
+    // channels gvdbScatterPointAvgCol / gvdbScatterPointDensity write
+    // (cuda_gvdb_particles.cuh:570,641)
+    const int cchan = avg_color ? 1 : 3;
+    if (v->clr) TRY(check_chan_type(v, cchan, GVDBX_UCHAR4, "ScatterDensity"));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "ScatterDensity: negative point count");
     float t[3] = {tx, ty, tz};
     TRY(ensure_bins(v, num_pnts, t, true));
@@ -1350,6 +1421,12 @@ int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp
                      v->chan[0].as<float>(), brick_res(v), std::max(v->apron, 0), v->res[0], radius, expand ? 1 : 0};
     if (sp.apron == 0 && expand) FAIL(GVDBX_ERR_ARG, "ScatterDensity: expand needs an apron");
     launch_scatter_density(sp, v->stream);
+    if (v->clr) {
+        ColorScatterParams cp{v->nodes[0].as<GNode>(), v->cnt[0], v->gbin, v->bin_off.as<int32_t>(),
+                              v->sorted.as<float4>(), v->clr, v->clr_stride, v->chan[cchan].as<uint32_t>(),
+                              brick_res(v), std::max(v->apron, 0), v->res[0], v->logdim[0], avg_color ? 1 : 0};
+        launch_scatter_color(cp, v->stream);
+    }
     return check_launch("ScatterDensity");
 }
 
@@ -1385,7 +1462,7 @@ static int splat_run_plan(gvdbx_volume* v, int* nruns_out) {
 int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz, int chan, int accum) {
     TRY(check_vol(v));
     TRY(need_atlas(v, "SplatCubic"));
-    TRY(check_chan(v, chan));
+    TRY(check_chan_type(v, chan, GVDBX_FLOAT, "SplatCubic"));
     if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "SplatCubic: negative point count");
     if (!(radius > 0.0f) || radius > (float)v->res[0]) FAIL(GVDBX_ERR_RANGE, "SplatCubic: radius must be in (0, brick]");
     float t[3] = {tx, ty, tz};
@@ -1433,8 +1510,17 @@ int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
     TRY(check_chan(v, chan));
     if (v->apron == 0) return GVDBX_OK;   // VG.cpp:4420
     TRY(need_atlas(v, "UpdateApron"));
+    // the word stored outside every leaf: the float itself, or
+    // make_uchar4(b, b, b, b) for colour channels (cuda_gvdb_operators.cuh:139-142)
+    uint32_t bits;
+    if (v->chan_type[chan] == GVDBX_UCHAR4) {
+        const uint32_t c = sat_u8(boundval);
+        bits = c | c << 8 | c << 16 | c << 24;
+    } else {
+        memcpy(&bits, &boundval, 4);
+    }
     ApronParams ap{v->nodes[0].as<GNode>(), v->cnt[0], v->grid[0], v->idx[0].as<int32_t>(), v->res[0],
-                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval};
+                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval, bits};
     launch_apron(ap, v->stream);
     return check_launch("UpdateApron");
 }

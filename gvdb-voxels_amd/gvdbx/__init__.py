@@ -19,7 +19,7 @@ NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("
                        ("mPos", "<i4", 3), ("mValue", "<i4", 3), ("mVRange", "<f4", 3),
                        ("mParent", "<u8"), ("mChildList", "<u8"), ("mMask", "<u8")])
 AMAP_DTYPE = np.dtype([("mPos", "<i4", 3), ("mLeafID", "<i4")])
-T_FLOAT = 0
+T_UCHAR4, T_FLOAT = 2, 3   # gvdb_types.h:75-76
 F_LINEAR, F_CLAMP = 1, 1
 SHADE_TRILINEAR, SHADE_LEVELSET, SHADE_VOLUME = 4, 6, 7   # gvdb_types.h:109-112
 
@@ -149,12 +149,13 @@ class VolumeGVDB:
         return out.value
 
     # ------------------------------------------------------------ points
-    def SetPoints(self, pos, vel=None, clr=None, pos_off=0, pos_stride=12):
-        """pos: device float32 tensor (N,3) (or an address with offset/stride)."""
+    def SetPoints(self, pos, vel=None, clr=None, pos_off=0, pos_stride=12, clr_off=0, clr_stride=4):
+        """pos: device float32 tensor (N,3) (or an address with offset/stride);
+        clr: optional device uint8 tensor (N,4) of uchar4 colours."""
         self._keep = [pos, vel, clr]
         call("gvdbx_set_points", self._h, C.c_void_p(_ptr(pos)), pos_off, pos_stride,
              C.c_void_p(_ptr(vel)) if vel is not None else None, 0, 12,
-             C.c_void_p(_ptr(clr)) if clr is not None else None, 0, 4)
+             C.c_void_p(_ptr(clr)) if clr is not None else None, clr_off, clr_stride)
 
     def SetPointsBatched(self, pos, cloud_off):
         """Many clouds, each in its own index space: pos (N,3) device tensor,
@@ -330,6 +331,11 @@ class VolumeGVDB:
         out = np.zeros(n, dtype=AMAP_DTYPE)
         call("gvdbx_read_atlas_map", self._h, out.ctypes.data, n)
         return out
+
+    def bricks_u8(self, chan=1):
+        """A uchar4 channel's bricks incl. apron, (B, br, br, br, 4) uint8."""
+        b = self.bricks(chan)
+        return b.view(np.uint8).reshape(b.shape + (4,))
 
     def bricks(self, chan=0):
         """All leaf bricks incl. apron, (B, br, br, br) indexed [k, z, y, x]."""

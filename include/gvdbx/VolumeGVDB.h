@@ -22,6 +22,7 @@
 // Differences from the reference: errors throw gvdbx::Error (the reference
 // prints and exits); the DataPtr handles are non-owning device views.
 #pragma once
+#include <climits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,7 +45,7 @@ struct DataPtr {
     int stride = 0;
 };
 
-enum { T_FLOAT = GVDBX_FLOAT };
+enum { T_UCHAR4 = GVDBX_UCHAR4, T_FLOAT = GVDBX_FLOAT };
 enum { F_LINEAR = 1, F_CLAMP = 1 };
 
 class Error : public std::runtime_error {
@@ -115,11 +116,20 @@ public:
     }
     void UpdateAtlas() { check(gvdbx_update_atlas(v_), "UpdateAtlas"); }
 
+    // the reference's out-parameter is an int: a list longer than INT_MAX
+    // throws GVDBX_ERR_RANGE here (the int64_t overload returns any length)
     void InsertPointsSubcell(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength) {
         int64_t len = 0;
-        check(gvdbx_insert_points_subcell(v_, subcell_size, num_pnts, radius, trans.x, trans.y, trans.z, &len),
-              "InsertPointsSubcell");
+        InsertPointsSubcell(subcell_size, num_pnts, radius, trans, len);
+        if (len > INT_MAX)
+            throw Error(GVDBX_ERR_RANGE, "InsertPointsSubcell: list length " + std::to_string(len) +
+                                             " does not fit the int out-parameter (use the int64_t overload)");
         pSCPntsLength = (int)len;
+    }
+    void InsertPointsSubcell(int subcell_size, int num_pnts, float radius, Vector3DF trans, int64_t& pSCPntsLength) {
+        check(gvdbx_insert_points_subcell(v_, subcell_size, num_pnts, radius, trans.x, trans.y, trans.z,
+                                          &pSCPntsLength),
+              "InsertPointsSubcell");
     }
     void GatherDensity(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
                        int chanDensity, int chanClr, bool bAccum = false) {

@@ -37,10 +37,14 @@ enum gvdbx_status {
     GVDBX_ERR_NOMEM = 7      /* device allocation failed */
 };
 
-enum gvdbx_dtype { GVDBX_FLOAT = 0 };
+/* channel types, numbered as the reference's T_UCHAR4 / T_FLOAT
+ * (source/gvdb_library/src/gvdb_types.h:75-76); both store 4 bytes per
+ * voxel in the same brick layout (uchar4: x in the lowest byte) */
+enum gvdbx_dtype { GVDBX_UCHAR4 = 2, GVDBX_FLOAT = 3 };
 
 enum gvdbx_topology_mode {
-    GVDBX_TOPO_AUTO = 0,       /* dense cell grids when they fit, else radix sort */
+    GVDBX_TOPO_AUTO = 0,       /* dense cell grids (RADIX is not available: AUTO
+                                  fails with GVDBX_ERR_RANGE beyond 2^28 level-0 cells) */
     GVDBX_TOPO_DENSE = 1,
     GVDBX_TOPO_RADIX = 2
 };
@@ -77,10 +81,12 @@ int gvdbx_synchronize(gvdbx_volume* v);
 int gvdbx_configure(gvdbx_volume* v, int levs, const int* log2dims);
 /* SetChannelDefault VG.h:390 */
 int gvdbx_set_channel_default(gvdbx_volume* v, int sx, int sy, int sz);
-/* AddChannel VG.h:392 / VG.cpp:2437-2453 (float channels; filter/border
- * are recorded for Render) */
+/* AddChannel VG.h:392 / VG.cpp:2437-2453 (GVDBX_FLOAT or GVDBX_UCHAR4;
+ * filter/border are recorded for Render) */
 int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filter, int border);
-/* FillChannel VG.cpp:2456-2470 (float) */
+/* FillChannel VG.cpp:2456-2470: float channels take value; uchar4 channels
+ * take (value, value, value, 1) * 255 (gvdbOpFillC4,
+ * cuda_gvdb_operators.cuh:368-373), or all zero when value == 0 */
 int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value);
 /* choose the topology builder (no reference counterpart; default AUTO) */
 int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
@@ -107,8 +113,11 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value);
 
 /* ---- points ---- */
 /* SetPoints(DataPtr pos, vel, clr) VG.h:541 / VG.cpp:4900-4908 with
- * SetDataGPU VG.cpp:4872-4880: float3 positions at pos_dev + i*stride + off.
- * vel/clr may be NULL. */
+ * SetDataGPU VG.cpp:4872-4880: float3 positions at pos_dev + i*stride + off;
+ * uchar4 colours at clr_dev + i*clr_stride + clr_off (4-byte aligned).
+ * vel/clr may be NULL; velocities are not read by any stage of this path.
+ * With colours, the gathers average them into their colour channel and
+ * ScatterDensity writes them into channel 1 or 3, as the reference does. */
 int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_stride,
                      const void* vel_dev, int vel_off, int vel_stride,
                      const void* clr_dev, int clr_off, int clr_stride);
@@ -127,22 +136,40 @@ int gvdbx_update_atlas(gvdbx_volume* v);
 /* InsertPointsSubcell(int, int, float, Vector3DF, int&) VG.h:551 / VG.cpp:4971-5092 */
 int gvdbx_insert_points_subcell(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
                                 float tx, float ty, float tz, int64_t* out_len);
-/* GatherDensity VG.h:556 / VG.cpp:5318-5344 */
+/* GatherDensity VG.h:556 / VG.cpp:5318-5344.  With point colours (SetPoints
+ * clr), chan_clr must be a GVDBX_UCHAR4 channel and every interior voxel of
+ * a subcell gets the mean colour of the subcell's list
+ * (cuda_gvdb_particles.cuh:718-731): per component
+ *   f = init + (float)S / 255,  stored: sat(trunc((f / (cnt + accum)) * 255))
+ * where S is the exact integer sum over the list, init = (0,0,0,1) or the
+ * stored colour / 255 when accumulating, and sat clamps to [0,255] (NaN -> 0;
+ * an empty list without accumulate gives (0,0,0,255)).  Without point
+ * colours chan_clr is not used. */
 int gvdbx_gather_density(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
                          float tx, float ty, float tz, int64_t sc_len, int chan_density, int chan_clr, int accum);
-/* GatherLevelSet VG.h:557 / VG.cpp:5348-5373 */
+/* GatherLevelSet VG.h:557 / VG.cpp:5348-5373 (colour as GatherDensity,
+ * cuda_gvdb_particles.cuh:757-771) */
 int gvdbx_gather_level_set(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
                            float tx, float ty, float tz, int64_t sc_len, int chan_levelset, int chan_clr, int accum);
 /* InsertPoints(int, Vector3DF, bool) VG.h:542 / VG.cpp:5219-5267 */
 int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix);
-/* ScatterDensity(int, float, float, Vector3DF, bool, bool) VG.h:555 / VG.cpp:5624-5660 */
+/* ScatterDensity(int, float, float, Vector3DF, bool, bool) VG.h:555 / VG.cpp:5624-5660.
+ * Density goes to channel 0 (float).  With point colours, each point's home
+ * voxel takes colour too: avg_color != 0 writes (Sx/n, Sy/n, Sz/n, 255)
+ * (integer division of the voxel's uint32 sums) into channel 1
+ * (gvdbScatterPointAvgCol, cuda_gvdb_particles.cuh:610-642); avg_color == 0
+ * writes the colour of the voxel's highest-numbered point into channel 3
+ * (cuda_gvdb_particles.cuh:570, whose race this resolves in point order).
+ * That channel must be GVDBX_UCHAR4. */
 int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp,
                           float tx, float ty, float tz, int expand, int avg_color);
 /* weighted cubic splat rho(v) = sum w_p max(0,1-|v-p|^2/r^2)^3 (extension of
  * the ScatterDensity kernel cuda_gvdb_particles.cuh:516-523) */
 int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, float ty, float tz,
                       int chan, int accum);
-/* UpdateApron(uchar, float, bool) VG.h:408 / VG.cpp:4418-4457 */
+/* UpdateApron(uchar, float, bool) VG.h:408 / VG.cpp:4418-4457; on a uchar4
+ * channel the value outside every leaf is make_uchar4(b, b, b, b) with b
+ * converted by saturating truncation (cuda_gvdb_operators.cuh:139-142) */
 int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
 
 /* ---- rendering (RC.cuh, DDA.cuh, GEO.cuh, MOD.cu) ---- */

@@ -33,6 +33,16 @@
  *      order, PT.cuh:35,266);
  *   D6 ScatterDensity accumulates serially in point order (reference: racy
  *      surface read-modify-write, PT.cuh:543-551).
+ *   D8 colour means (GatherDensity / GatherLevelSet with point colours,
+ *      PT.cuh:718-731,757-771) sum the listed points' uchar4 components as
+ *      exact integers and convert once: f = init + (float)S / 255, mean =
+ *      f / (float)(count + accumulate), stored as a saturating truncation of
+ *      mean * 255 (NaN -> 0); an empty list without accumulate gives
+ *      (0,0,0,255), which is what 0/0 and 1/0 saturate to (reference: an
+ *      FP32 sum of c/255 in atomic list order, then the same division);
+ *   D9 ScatterDensity without colour averaging writes the colour of the
+ *      highest-numbered point of each voxel (reference: racy surface write,
+ *      PT.cuh:570).
  */
 #include "gvdb_oracle.h"
 #include <math.h>
@@ -59,7 +69,8 @@ struct or_volume {
     int subdim[3];
     int apron;
     int has_chan[OR_MAXCHAN];
-    float* atlas[OR_MAXCHAN];
+    float* atlas[OR_MAXCHAN];       /* 4-byte words: float, or packed uchar4 (x lowest byte) */
+    int chan_type[OR_MAXCHAN];      /* OR_T_FLOAT or OR_T_UCHAR4 */
     int64_t atlas_vox;
     or_atlas_node* amap;
     int64_t amap_cnt;
@@ -71,6 +82,7 @@ struct or_volume {
     int32_t* sc_cnt;
     int64_t* sc_off;
     float* sc_pnt;
+    int64_t* sc_pidx;   /* point index of each list entry (colour lookup) */
     int32_t* sc_nid;
     int32_t* sc_pos;
 };
@@ -80,6 +92,27 @@ struct or_volume {
 /* ------------------------------------------------------------------ */
 
 static int64_t ipow3(int64_t r) { return r * r * r; }
+
+/* float -> uchar as CUDA's saturating conversion: truncation toward zero,
+ * clamped to [0, 255], NaN -> 0 */
+static uint32_t sat_u8(float x) {
+    if (!(x > 0.0f)) return 0;
+    if (x >= 255.0f) return 255;
+    return (uint32_t)x;
+}
+
+/* the word UpdateApron stores outside every leaf: the float itself, or
+ * make_uchar4(b, b, b, b) (OP.cuh:139-142) */
+static uint32_t bound_word(int dtype, float boundval) {
+    uint32_t w;
+    if (dtype == OR_T_UCHAR4) {
+        uint32_t c = sat_u8(boundval);
+        w = c | c << 8 | c << 16 | c << 24;
+    } else {
+        memcpy(&w, &boundval, 4);
+    }
+    return w;
+}
 
 /* getRange VG.h:614-619 */
 static int64_t or_range(const or_volume* v, int lev) {
@@ -303,8 +336,8 @@ or_volume* or_create(void) {
 }
 
 static void free_sc(or_volume* v) {
-    free(v->sc_cnt); free(v->sc_off); free(v->sc_pnt); free(v->sc_nid); free(v->sc_pos);
-    v->sc_cnt = NULL; v->sc_off = NULL; v->sc_pnt = NULL; v->sc_nid = NULL; v->sc_pos = NULL;
+    free(v->sc_cnt); free(v->sc_off); free(v->sc_pnt); free(v->sc_pidx); free(v->sc_nid); free(v->sc_pos);
+    v->sc_cnt = NULL; v->sc_off = NULL; v->sc_pnt = NULL; v->sc_pidx = NULL; v->sc_nid = NULL; v->sc_pos = NULL;
     v->num_sc = 0; v->sc_len = 0;
 }
 
@@ -361,8 +394,11 @@ static void atlas_resize(or_volume* v, int sz) {
     v->atlas_vox = nvox;
 }
 
-int or_add_channel(or_volume* v, int chan, int apron) {
+int or_add_channel(or_volume* v, int chan, int apron) { return or_add_channel_typed(v, chan, apron, OR_T_FLOAT); }
+
+int or_add_channel_typed(or_volume* v, int chan, int apron, int dtype) {
     if (!v || !v->configured || chan < 0 || chan >= OR_MAXCHAN || apron < 0) return OR_ERR_ARG;
+    if (dtype != OR_T_FLOAT && dtype != OR_T_UCHAR4) return OR_ERR_ARG;
     int any = 0;
     for (int c = 0; c < OR_MAXCHAN; c++) any |= v->has_chan[c];
     if (!any) {
@@ -373,6 +409,7 @@ int or_add_channel(or_volume* v, int chan, int apron) {
         return OR_ERR_ARG;
     }
     v->has_chan[chan] = 1;
+    v->chan_type[chan] = dtype;
     int br = brick_res(v);
     int64_t nvox = (int64_t)v->subdim[0] * br * (int64_t)v->subdim[1] * br * (int64_t)v->subdim[2] * br;
     free(v->atlas[chan]);
@@ -676,6 +713,7 @@ int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, 
             v->sc_off[nsc] = run;
             v->sc_len = run;
             v->sc_pnt = (float*)malloc((size_t)(run * 3 + 3) * sizeof(float));
+            v->sc_pidx = (int64_t*)malloc((size_t)(run + 1) * sizeof(int64_t));
             memset(v->sc_cnt, 0, (size_t)nsc * sizeof(int32_t));
         }
         for (int64_t i = 0; i < n; i++) {
@@ -703,6 +741,7 @@ int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, 
                             v->sc_pnt[e * 3 + 0] = w[0];
                             v->sc_pnt[e * 3 + 1] = w[1];
                             v->sc_pnt[e * 3 + 2] = w[2];
+                            v->sc_pidx[e] = i;
                         }
                         v->sc_cnt[sidx]++;
                     }
@@ -803,7 +842,8 @@ int or_update_apron(or_volume* v, int chan, float boundval) {
     if (v->apron == 0) return OR_OK;        /* VG.cpp:4420 */
     if (v->apron != 1) return OR_ERR_ARG;   /* OP.cuh kernels assume apron 1 (VG.cpp:4450) */
     if (!v->amap) return OR_ERR_STATE;
-    float* A = v->atlas[chan];
+    uint32_t* A = (uint32_t*)v->atlas[chan];   /* the apron copies 4-byte voxels of either type */
+    const uint32_t bound = bound_word(v->chan_type[chan], boundval);
     int br = brick_res(v);
     int sx = v->subdim[0], sy = v->subdim[1];
     for (int64_t b = 0; b < v->ncnt[0]; b++) {   /* UpdateApron<T> OP.cuh:73-126 */
@@ -831,9 +871,9 @@ int or_update_apron(or_volume* v, int chan, float boundval) {
                     for (int a = 0; a < 3; a++)
                         wpos[a] = (float)v->amap[bid].mPos[a] + (float)(int)(av[a] % br - v->apron) + 0.5f;
                     int64_t leaf = point_leaf(v, wpos);
-                    float value;
+                    uint32_t value;
                     if (leaf < 0) {
-                        value = boundval;
+                        value = bound;
                     } else {
                         const or_node* ln = &v->nodes[0][leaf];
                         float ox = (float)ln->mValue[0] + (wpos[0] - (float)ln->mPos[0]);
@@ -999,7 +1039,120 @@ int or_splat_cubic_f64(or_volume* v, const float* pos, int64_t n, int64_t stride
 
 int or_fill_channel(or_volume* v, int chan, float val) {
     if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (v->chan_type[chan] == OR_T_UCHAR4) {
+        /* FillChannel VG.cpp:2456-2470: all-zero clears (ClearChannel), else
+         * gvdbOpFillC4 OP.cuh:368-373 stores CLR2INT(val, val, val, 1) */
+        uint32_t w = 0;
+        if (val != 0.0f) {
+            uint32_t c = sat_u8(val * 255.0f);
+            w = c | c << 8 | c << 16 | 255u << 24;
+        }
+        uint32_t* A = (uint32_t*)v->atlas[chan];
+        for (int64_t i = 0; i < v->atlas_vox; i++) A[i] = w;
+        return OR_OK;
+    }
     for (int64_t i = 0; i < v->atlas_vox; i++) v->atlas[chan][i] = val;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* colour channels (uchar4, packed x | y<<8 | z<<16 | w<<24)           */
+/* ------------------------------------------------------------------ */
+
+/* Colour mean of GatherDensity / GatherLevelSet (PT.cuh:718-731,757-771)
+ * under D8: every interior voxel of a subcell gets
+ *   f_a  = init_a + (float)S_a / 255     S_a = sum of component a over the
+ *                                        subcell's list (exact integer)
+ *   out_a = sat_u8((f_a / (float)(cnt + accum)) * 255)
+ * with init = (0,0,0,1), or the stored colour / 255 when accumulating. */
+int or_gather_color(or_volume* v, const uint8_t* clr, int64_t clr_stride, int chan, int accum) {
+    if (!v || !clr || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan] || v->chan_type[chan] != OR_T_UCHAR4)
+        return OR_ERR_ARG;
+    if (!v->sc_cnt || !v->sc_nid || !v->sc_pidx) return OR_ERR_STATE;
+    if (clr_stride == 0) clr_stride = 4;
+    uint32_t* A = (uint32_t*)v->atlas[chan];
+    int sc = v->sc_size;
+    for (int64_t s = 0; s < v->num_sc; s++) {
+        int32_t nid = v->sc_nid[s];
+        if (nid < 0) continue;
+        const or_node* nd = &v->nodes[0][nid];
+        uint64_t S[4] = {0, 0, 0, 0};
+        for (int64_t j = v->sc_off[s]; j < v->sc_off[s] + v->sc_cnt[s]; j++) {
+            const uint8_t* c = clr + v->sc_pidx[j] * clr_stride;
+            for (int a = 0; a < 4; a++) S[a] += c[a];
+        }
+        const float n = (float)(v->sc_cnt[s] + (accum ? 1 : 0));
+        for (int tz = 0; tz < sc; tz++)
+            for (int ty = 0; ty < sc; ty++)
+                for (int tx = 0; tx < sc; tx++) {
+                    int64_t vx = nd->mValue[0] + (v->sc_pos[s * 3] + tx - nd->mPos[0]);
+                    int64_t vy = nd->mValue[1] + (v->sc_pos[s * 3 + 1] + ty - nd->mPos[1]);
+                    int64_t vz = nd->mValue[2] + (v->sc_pos[s * 3 + 2] + tz - nd->mPos[2]);
+                    int64_t ai = atlas_index(v, vx, vy, vz);
+                    uint32_t out = 0;
+                    for (int a = 0; a < 4; a++) {
+                        float init = accum ? (float)((A[ai] >> (8 * a)) & 255u) / 255.0f : (a == 3 ? 1.0f : 0.0f);
+                        float f = init + (float)S[a] / 255.0f;
+                        out |= sat_u8((f / n) * 255.0f) << (8 * a);
+                    }
+                    A[ai] = out;
+                }
+    }
+    return OR_OK;
+}
+
+/* Colour part of gvdbScatterPointDensity (PT.cuh:554-572) and
+ * gvdbScatterPointAvgCol (PT.cuh:610-642): each point colours its home voxel
+ * (the voxel the density pass adds to, same validity tests).  avg: per voxel
+ * the uint32 sums of x, y, z and the count, then (Sx/cnt, Sy/cnt, Sz/cnt,
+ * 255) in integer division where cnt > 0; otherwise the colour of the
+ * highest-numbered point (D9). */
+int or_scatter_color(or_volume* v, const float* pos, int64_t n, int64_t stride, const uint8_t* clr,
+                     int64_t clr_stride, const float trans[3], int avg, int chan) {
+    if (!v || (!pos && n > 0) || !clr || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan] ||
+        v->chan_type[chan] != OR_T_UCHAR4)
+        return OR_ERR_ARG;
+    if (stride == 0) stride = 12;
+    if (clr_stride == 0) clr_stride = 4;
+    uint32_t* A = (uint32_t*)v->atlas[chan];
+    int r0 = v->res[0];
+    int64_t nvox = v->ncnt[0] * (int64_t)r0 * r0 * r0;
+    uint32_t* acc = avg ? (uint32_t*)calloc((size_t)nvox * 4, sizeof(uint32_t)) : NULL;
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), trans, w)) continue;
+        int64_t leaf = point_leaf(v, w);
+        if (leaf < 0) continue;
+        const or_node* nd = &v->nodes[0][leaf];
+        float p[3] = {w[0] - (float)nd->mPos[0], w[1] - (float)nd->mPos[1], w[2] - (float)nd->mPos[2]};
+        float pi[3] = {(float)(int)p[0], (float)(int)p[1], (float)(int)p[2]};
+        if (pi[0] < 0 || pi[1] < 0 || pi[2] < 0 || pi[0] >= r0 || pi[1] >= r0 || pi[2] >= r0) continue;
+        const uint8_t* c = clr + i * clr_stride;
+        if (avg) {
+            int64_t vid = leaf * r0 * r0 * r0 + ((int64_t)pi[2] * r0 + (int64_t)pi[1]) * r0 + (int64_t)pi[0];
+            acc[vid * 4 + 0] += 1u;
+            acc[vid * 4 + 1] += c[0];
+            acc[vid * 4 + 2] += c[1];
+            acc[vid * 4 + 3] += c[2];
+        } else {
+            int64_t ai = atlas_index(v, (int64_t)pi[0] + nd->mValue[0], (int64_t)pi[1] + nd->mValue[1],
+                                     (int64_t)pi[2] + nd->mValue[2]);
+            A[ai] = (uint32_t)c[0] | (uint32_t)c[1] << 8 | (uint32_t)c[2] << 16 | (uint32_t)c[3] << 24;
+        }
+    }
+    if (avg) {
+        for (int64_t vid = 0; vid < nvox; vid++) {
+            uint32_t cnt = acc[vid * 4];
+            if (cnt == 0) continue;
+            int64_t leaf = vid / ((int64_t)r0 * r0 * r0), l = vid % ((int64_t)r0 * r0 * r0);
+            const or_node* nd = &v->nodes[0][leaf];
+            int64_t ai = atlas_index(v, nd->mValue[0] + l % r0, nd->mValue[1] + (l / r0) % r0,
+                                     nd->mValue[2] + l / ((int64_t)r0 * r0));
+            A[ai] = (acc[vid * 4 + 1] / cnt) | (acc[vid * 4 + 2] / cnt) << 8 | (acc[vid * 4 + 3] / cnt) << 16 |
+                    255u << 24;
+        }
+        free(acc);
+    }
     return OR_OK;
 }
 

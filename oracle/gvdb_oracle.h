@@ -68,8 +68,12 @@ void or_destroy(or_volume* v);
 int or_configure(or_volume* v, int levs, const int* logdim);
 /* SetChannelDefault VG.h:390 */
 void or_set_channel_default(or_volume* v, int sx, int sy, int sz);
-/* AddChannel VG.cpp:2437-2453 (float channels only) */
+/* channel types, gvdb_types.h:75-76 */
+#define OR_T_UCHAR4 2
+#define OR_T_FLOAT 3
+/* AddChannel VG.cpp:2437-2453: or_add_channel makes a float channel */
 int or_add_channel(or_volume* v, int chan, int apron);
+int or_add_channel_typed(or_volume* v, int chan, int apron, int dtype);
 
 /* RebuildTopology VG.cpp:1181-1218 (+ ActivateBricksGPU 1229-1413,
  * ActivateExtraBricksGPU 931-1016) followed by FinishTopology(false,true). */
@@ -102,8 +106,15 @@ int or_splat_cubic(or_volume* v, const float* pos, int64_t n, int64_t stride_byt
                    const float* weights, float radius, const float trans[3], int chan, int accum);
 int or_splat_cubic_f64(or_volume* v, const float* pos, int64_t n, int64_t stride_bytes,
                        const float* weights, float radius, const float trans[3], double* out);
-/* FillChannel (float) */
+/* FillChannel VG.cpp:2456-2470 (float; uchar4: zero, or (val,val,val,1)*255) */
 int or_fill_channel(or_volume* v, int chan, float val);
+/* colour mean of the gathers (PT.cuh:718-731,757-771, defined per D8 in
+ * gvdb_oracle.c) over the lists of the last or_insert_points_subcell;
+ * clr: uchar4 per point at clr + i*clr_stride */
+int or_gather_color(or_volume* v, const uint8_t* clr, int64_t clr_stride, int chan, int accum);
+/* colour part of ScatterDensity (PT.cuh:554-572,610-642; D9) */
+int or_scatter_color(or_volume* v, const float* pos, int64_t n, int64_t stride_bytes, const uint8_t* clr,
+                     int64_t clr_stride, const float trans[3], int avg, int chan);
 
 /* ---- readback ---- */
 typedef struct {

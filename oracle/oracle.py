@@ -43,6 +43,7 @@ class OrRenderParams(C.Structure):
 
 
 SHADE_TRILINEAR, SHADE_LEVELSET, SHADE_VOLUME = 4, 6, 7
+T_UCHAR4, T_FLOAT = 2, 3   # gvdb_types.h:75-76
 
 
 def make_camera(cam):
@@ -104,6 +105,9 @@ def lib():
         L.or_configure.argtypes = [vp, i32, C.POINTER(C.c_int)]
         L.or_set_channel_default.argtypes = [vp, i32, i32, i32]
         L.or_add_channel.argtypes = [vp, i32, i32]
+        L.or_add_channel_typed.argtypes = [vp, i32, i32, i32]
+        L.or_gather_color.argtypes = [vp, vp, i64, i32, i32]
+        L.or_scatter_color.argtypes = [vp, vp, i64, i64, vp, i64, fp, i32, i32]
         L.or_rebuild_topology.argtypes = [vp, vp, i64, i64, f32, fp]
         L.or_update_atlas.argtypes = [vp]
         L.or_insert_points_subcell.argtypes = [vp, i32, vp, i64, i64, f32, fp, C.POINTER(C.c_int64)]
@@ -181,8 +185,8 @@ class OracleVolume:
     def res0(self):
         return 1 << self.log2dims[0]
 
-    def add_channel(self, chan=0, apron=1):
-        _chk(self._L.or_add_channel(self._h, chan, apron), "add_channel")
+    def add_channel(self, chan=0, apron=1, dtype=T_FLOAT):
+        _chk(self._L.or_add_channel_typed(self._h, chan, apron, int(dtype)), "add_channel")
 
     def rebuild_topology(self, points, radius, orig=(0, 0, 0)):
         p = _pts(points)
@@ -248,6 +252,21 @@ class OracleVolume:
     def fill_channel(self, chan, val):
         _chk(self._L.or_fill_channel(self._h, chan, float(val)), "fill_channel")
 
+    def gather_color(self, clr, chan=1, accum=False):
+        """Colour mean of the gathers over the current subcell lists; clr:
+        (N, 4) uint8 per point, in the order of the points given to
+        insert_points_subcell."""
+        c = np.ascontiguousarray(clr, dtype=np.uint8)
+        assert c.ndim == 2 and c.shape[1] == 4
+        _chk(self._L.or_gather_color(self._h, c.ctypes.data, 4, chan, int(accum)), "gather_color")
+
+    def scatter_color(self, points, clr, avg=True, chan=1, trans=(0, 0, 0)):
+        p = _pts(points)
+        c = np.ascontiguousarray(clr, dtype=np.uint8)
+        assert c.shape == (p.shape[0], 4)
+        _chk(self._L.or_scatter_color(self._h, p.ctypes.data, p.shape[0], 12, c.ctypes.data, 4, _f3(trans),
+                                      int(avg), chan), "scatter_color")
+
     # ---------------- readback ----------------
     def info(self):
         inf = OrInfo()
@@ -278,6 +297,11 @@ class OracleVolume:
             return np.zeros(0, dtype=AMAP_DTYPE)
         buf = (C.c_char * (cnt.value * 16)).from_address(ptr)
         return np.frombuffer(bytes(buf), dtype=AMAP_DTYPE).copy()
+
+    def bricks_u8(self, chan=1):
+        """A uchar4 channel's bricks as (B, br, br, br, 4) uint8 indexed [z, y, x, component]."""
+        b = self.bricks(chan)
+        return b.view(np.uint8).reshape(b.shape + (4,))
 
     def bricks(self, chan=0):
         """All leaf bricks incl. apron as (B, br, br, br) array indexed [z, y, x]."""

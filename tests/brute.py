@@ -200,7 +200,9 @@ def cubic_f64(points, orig, leaf_pos, res0, r, weights=None):
 
 
 def apron_expected(bricks, leaf_pos, res0, boundval):
-    """Recompute every apron voxel from the interiors (bricks: (B,br,br,br) [z,y,x])."""
+    """Recompute every apron voxel from the interiors (bricks: (B,br,br,br) [z,y,x]).
+    bricks may be any 4-byte dtype (uint32 words of a colour channel: then
+    boundval is the word stored outside every leaf)."""
     br = bricks.shape[1]
     a = (br - res0) // 2
     lut = leaf_lookup(leaf_pos, res0)
@@ -212,7 +214,7 @@ def apron_expected(bricks, leaf_pos, res0, boundval):
     for k in range(len(leaf_pos)):
         p0 = leaf_pos[k]
         u = np.stack([p0[0] + sx - a, p0[1] + sy - a, p0[2] + sz - a], 1)
-        vals = np.full(len(u), boundval, dtype=np.float32)
+        vals = np.full(len(u), boundval, dtype=bricks.dtype)
         for i, uu in enumerate(u):
             if (uu < 0).any():
                 continue
@@ -221,4 +223,56 @@ def apron_expected(bricks, leaf_pos, res0, boundval):
                 l = uu - leaf_pos[j] + a
                 vals[i] = bricks[j, l[2], l[1], l[0]]
         out[k][shell] = vals
+    return out
+
+
+def subcell_members(points, orig, leaf_pos, res0, sc, R):
+    """Brute-force subcell lists: for leaf k and subcell s (x fastest), the
+    indices of the valid points whose insertion box holds the subcell origin."""
+    ok, w = valid_points(points, orig)
+    idx = np.nonzero(ok)[0]
+    w = w[ok].astype(np.float32)
+    lo, hi = subcell_box(w, R, sc)
+    nsd = res0 // sc
+    out = []
+    g = np.arange(nsd) * sc
+    oz, oy, ox = np.meshgrid(g, g, g, indexing="ij")
+    offs = np.stack([ox.ravel(), oy.ravel(), oz.ravel()], 1)
+    for k in range(len(leaf_pos)):
+        o = leaf_pos[k][None, :] + offs
+        mem = ((lo[None, :, :] <= o[:, None, :]) & (o[:, None, :] <= hi[None, :, :])).all(2)
+        out.append([idx[np.nonzero(mem[s])[0]] for s in range(len(offs))])
+    return out
+
+
+def sat_u8(x):
+    """float32 -> uint8 as a saturating truncation (NaN -> 0)."""
+    x = np.asarray(x, dtype=np.float32)
+    with np.errstate(invalid="ignore"):
+        y = np.where(np.isnan(x) | (x <= 0), 0, np.where(x >= 255, 255, np.trunc(np.nan_to_num(x))))
+    return y.astype(np.uint8)
+
+
+def gather_color_expected(members, clr, res0, sc, init=None):
+    """Colour mean per interior voxel (B, res0, res0, res0, 4) [z,y,x,c] from
+    brute-force lists: f = init + float32(S)/255, sat(trunc(f/n * 255)) with n
+    = count (+1 when init, the accumulated colour, is given)."""
+    nsd = res0 // sc
+    B = len(members)
+    out = np.zeros((B, res0, res0, res0, 4), np.uint8)
+    c64 = clr.astype(np.int64)
+    for k in range(B):
+        for s, m in enumerate(members[k]):
+            sx, sy, sz = s % nsd, (s // nsd) % nsd, s // (nsd * nsd)
+            S = c64[m].sum(0) if len(m) else np.zeros(4, np.int64)
+            sl = (k, slice(sz * sc, sz * sc + sc), slice(sy * sc, sy * sc + sc), slice(sx * sc, sx * sc + sc))
+            if init is None:
+                base = np.broadcast_to(np.array([0, 0, 0, 1], np.float32), (sc, sc, sc, 4))
+                n = np.float32(len(m))
+            else:
+                base = init[sl].astype(np.float32) / np.float32(255)
+                n = np.float32(len(m) + 1)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                f = base + S.astype(np.float32) / np.float32(255)
+                out[sl] = sat_u8((f / n) * np.float32(255))
     return out
