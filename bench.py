@@ -33,26 +33,9 @@ CLOCK_HZ = 2.4e9        # MI355X peak engine clock
 SEED = 20261203 + 1          # configs[1]
 
 
-def abc_velocity(torch, p, phases, amp=(1.0, 1.0, 1.0)):
-    k = 2.0 * math.pi / 128.0
-    A, B, Cc = amp
-    x, y, z = p[:, 0], p[:, 1], p[:, 2]
-    u = A * torch.sin(k * z + phases[0]) + Cc * torch.cos(k * y + phases[1])
-    v = B * torch.sin(k * x + phases[2]) + A * torch.cos(k * z + phases[3])
-    w = Cc * torch.sin(k * y + phases[4]) + B * torch.cos(k * x + phases[5])
-    return torch.stack([u, v, w], 1)
-
-
 def make_frames(torch, gvdbx, n, nframes, dev):
-    import numpy as np
-    pts = gvdbx.gen_box(SEED, n, (64, 64, 64), (576, 320, 576), 0.5)
-    rng = np.random.default_rng(SEED)
-    phases = [float(x) for x in rng.uniform(0, 2 * math.pi, 6)]
-    frames = [torch.from_numpy(pts).to(dev)]
-    for _ in range(nframes - 1):
-        p = frames[-1]
-        frames.append((p + abc_velocity(torch, p, phases)).contiguous())
-    return frames, pts
+    from gvdbx import workloads
+    return workloads.dambreak_frames(torch, gvdbx.gen_box, SEED, n, nframes, dev)
 
 
 def frame_bytes_alg(n, nleaf, node_cnt, res, depth, br=10, r0=8, passes=3):

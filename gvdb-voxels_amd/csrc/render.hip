@@ -221,6 +221,23 @@ __device__ __forceinline__ float ea_log2_atten(const RenderScene& sc) {
 }
 __device__ __forceinline__ float ea_atten(float k, float s) { return __builtin_amdgcn_exp2f(k * s); }
 
+// 1 - a for a = ea_atten(k, s), to a few ulps.  Subtracting the rounded a
+// from 1 keeps only log2(1 / (1 - a)) good bits when a is close to 1 (a
+// sparse density: 1 - a ~ 1e-4 loses ten); for |x| < 1/2, x = k s ln 2,
+// the series of -expm1(x) = -x (1 + x/2 + x^2/6 + ... + x^7/8!) is used
+// instead (truncation below 2^-26 relative there).
+__device__ __forceinline__ float ea_one_minus(float k, float s, float a) {
+    const float x = (k * s) * 0.693147180559945309f;
+    float p = __builtin_fmaf(x, 1.0f / 40320.0f, 1.0f / 5040.0f);
+    p = __builtin_fmaf(x, p, 1.0f / 720.0f);
+    p = __builtin_fmaf(x, p, 1.0f / 120.0f);
+    p = __builtin_fmaf(x, p, 1.0f / 24.0f);
+    p = __builtin_fmaf(x, p, 1.0f / 6.0f);
+    p = __builtin_fmaf(x, p, 0.5f);
+    p = __builtin_fmaf(x, p, 1.0f);
+    return __builtin_fabsf(x) < 0.5f ? -x * p : 1.0f - a;
+}
+
 // 0 <= q < r0 on every axis, as 3-way min / max (positions are never NaN):
 // the same predicate as the six compares, in fewer instructions.
 __device__ __forceinline__ bool in_brick(const V3& q, float r0) {
@@ -258,10 +275,11 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
                     val = sc.lut[transfer_index(sc, s)];
                 else
                     val = make_float4(s, s, s, s);
+                const float om = ea_one_minus(ek, val.w, ea_atten(ek, val.w));
                 val.w = ea_atten(ek, val.w);
-                st.clr[0] += val.x * st.clr[3] * (1 - val.w) * sc.albedo;
-                st.clr[1] += val.y * st.clr[3] * (1 - val.w) * sc.albedo;
-                st.clr[2] += val.z * st.clr[3] * (1 - val.w) * sc.albedo;
+                st.clr[0] += val.x * st.clr[3] * om * sc.albedo;
+                st.clr[1] += val.y * st.clr[3] * om * sc.albedo;
+                st.clr[2] += val.z * st.clr[3] * om * sc.albedo;
                 st.clr[3] *= val.w;
             }
             p = add(p, wpt);
@@ -699,8 +717,9 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
             const float bx = 1.0f - ax, by = 1.0f - ay, bz = 1.0f - az;
             if (s >= sc.minval) {
                 const float a = ea_atten(ek, s);
-                I += s * T * (1 - a) * sc.albedo;
-                const float gq = gi * (sc.albedo * T * ((1.0f - a) - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
+                const float om = ea_one_minus(ek, s, a);
+                I += s * T * om * sc.albedo;
+                const float gq = gi * (sc.albedo * T * (om - e * s * a) + e * (Ifin - I)) + gt * e * Tfin;
                 T *= a;
                 if (have && !(ix == cx && iy == cy && iz == cz)) {
                     const int ddx = ix - cx, ddy = iy - cy, ddz = iz - cz;

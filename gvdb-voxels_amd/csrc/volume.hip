@@ -94,6 +94,7 @@ struct gvdbx_volume {
     int xcd_remap = 0;
     int splat_chunk = SPLAT_CHUNK_DEFAULT;
     int splat_variant = 1;
+    int sort_tiles = 0;   // point binning sort: 0 auto (wide from 2^23 valid points), 1 narrow, 2 wide
     // atlas map
     bool has_atlas = false;
     DevBuf amap;
@@ -370,8 +371,9 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     int flip = radix_sort_pairs32(v->bkey.as<uint32_t>(), v->bval.as<uint32_t>(), v->bkey2.as<uint32_t>(),
                                   v->bval2.as<uint32_t>(), n, 0, end_bit, v->rs_hist.as<int32_t>(),
                                   v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s, true,
-                                  (v->pts_valid >= 0 ? v->pts_valid : n) >= ((int64_t)1 << 23));
-    // values: point indices; wide tiles for 2^23 valid points or more
+                                  v->sort_tiles == 2 ||
+                                      (v->sort_tiles == 0 && (v->pts_valid >= 0 ? v->pts_valid : n) >= ((int64_t)1 << 23)));
+    // values: point indices; wide tiles for 2^23 valid points or more (or as sort_tiles says)
     const uint32_t* skey = flip ? v->bkey2.as<uint32_t>() : v->bkey.as<uint32_t>();
     const uint32_t* sval = flip ? v->bval2.as<uint32_t>() : v->bval.as<uint32_t>();
     launch_bin_gather(v->pos, n, t[0], t[1], t[2], skey, sval, (uint32_t)nb, v->sorted.as<float4>(), rp,
@@ -1217,6 +1219,12 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->splat_variant = value;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "sort_tiles")) {
+        if (value < 0 || value > 2) FAIL(GVDBX_ERR_ARG, "sort_tiles must be 0, 1 or 2");
+        v->sort_tiles = value;
+        v->bins_valid = false;   // the next binning sorts again with the chosen tiles
+        return GVDBX_OK;
+    }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1245,6 +1253,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "splat_variant")) {
         *value = v->splat_variant;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "sort_tiles")) {
+        *value = v->sort_tiles;
         return GVDBX_OK;
     }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);

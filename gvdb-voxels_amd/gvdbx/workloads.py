@@ -14,8 +14,41 @@ cameras on an orbit, as a depth sensor would report them:
     frame is invalid as well.
 poses are camera-to-world [R row-major (3x3) | t] with camera axes x right,
 y down, z forward, the convention gvdbx_backproject_depth expects.
+
+dambreak_frames(): the plain GPU workload, a dam-break block of points
+advected by a divergence-free ABC flow, frame by frame on the device.
 """
+import math
+
 import numpy as np
+
+
+def abc_velocity(torch, p, phases, amp=(1.0, 1.0, 1.0)):
+    """ABC flow with k = 2 pi / 128: u = A sin(kz+f0) + C cos(ky+f1),
+    v = B sin(kx+f2) + A cos(kz+f3), w = C sin(ky+f4) + B cos(kx+f5); with unit
+    amplitudes every component is at most 2 voxels per frame."""
+    k = 2.0 * math.pi / 128.0
+    A, B, Cc = amp
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    u = A * torch.sin(k * z + phases[0]) + Cc * torch.cos(k * y + phases[1])
+    v = B * torch.sin(k * x + phases[2]) + A * torch.cos(k * z + phases[3])
+    w = Cc * torch.sin(k * y + phases[4]) + B * torch.cos(k * x + phases[5])
+    return torch.stack([u, v, w], 1)
+
+
+def dambreak_frames(torch, gen_box, seed, n, nframes, dev):
+    """n points uniform in [64,576)x[64,320)x[64,576) with +-0.5 jitter
+    (gen_box(seed, ...)), then nframes - 1 advection steps by the ABC flow
+    with seeded phases, on dev.  Returns the frames (device tensors) and the
+    frame-0 points on the host."""
+    pts = gen_box(seed, n, (64, 64, 64), (576, 320, 576), 0.5)
+    rng = np.random.default_rng(seed)
+    phases = [float(x) for x in rng.uniform(0, 2 * math.pi, 6)]
+    frames = [torch.from_numpy(pts).to(dev)]
+    for _ in range(nframes - 1):
+        p = frames[-1]
+        frames.append((p + abc_velocity(torch, p, phases)).contiguous())
+    return frames, pts
 
 
 def _look_at(pos, target, up=(0.0, 1.0, 0.0)):

@@ -107,7 +107,10 @@ int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
  * grid within 64 KiB, else tile pull runs),
  * "splat_chunk" (cell push: points per wavefront run; the float summation
  * order, and so the last bits, depend on it; default 1024, range
- * [64, 2^24]) */
+ * [64, 2^24]), "sort_tiles" (the stable radix sort that bins points by
+ * brick: 0 = tiles of 8192 entries from 2^23 valid points on, else 4096;
+ * 1 = always 4096; 2 = always 8192; the bins are the same either way;
+ * default 0) */
 int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value);
 int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value);
 

@@ -713,7 +713,10 @@ int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, 
             v->sc_off[nsc] = run;
             v->sc_len = run;
             v->sc_pnt = (float*)malloc((size_t)(run * 3 + 3) * sizeof(float));
-            v->sc_pidx = (int64_t*)malloc((size_t)(run + 1) * sizeof(int64_t));
+            /* point indices only feed the colour mean: kept while a uchar4 channel exists */
+            int keep = 0;
+            for (int c = 0; c < OR_MAXCHAN; c++) keep |= v->has_chan[c] && v->chan_type[c] == OR_T_UCHAR4;
+            v->sc_pidx = keep ? (int64_t*)malloc((size_t)(run + 1) * sizeof(int64_t)) : NULL;
             memset(v->sc_cnt, 0, (size_t)nsc * sizeof(int32_t));
         }
         for (int64_t i = 0; i < n; i++) {
@@ -741,7 +744,7 @@ int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, 
                             v->sc_pnt[e * 3 + 0] = w[0];
                             v->sc_pnt[e * 3 + 1] = w[1];
                             v->sc_pnt[e * 3 + 2] = w[2];
-                            v->sc_pidx[e] = i;
+                            if (v->sc_pidx) v->sc_pidx[e] = i;
                         }
                         v->sc_cnt[sidx]++;
                     }
@@ -1735,8 +1738,15 @@ int or_raycast_ea(or_volume* v, int chan, const or_camera* cam, const or_render_
  *   dT/ds_k = e T_final
  * and ds_k/dvoxel = trilinear weight.  grad has nb*br^3 entries (padded
  * bricks, leaf-major, x fastest) and is accumulated into. */
-int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
-                           const double* bricks64, const double* gI, const double* gT, double* grad) {
+/* mag != 0: the same sums with every term replaced by the magnitude of its
+ * operands (|gI|, |gT|, |1 - a| + |e s a|, |e| times the magnitude of the
+ * whole I sum instead of I_final - I_k); the result bounds the rounding
+ * error of an FP32 evaluation of the adjoint, entry by entry, as (roundings
+ * on the longest dependent chain) * 2^-24 * result, provided that
+ * evaluation forms 1 - a to a few ulps of itself (from the series of
+ * -expm1 near a = 1, not by subtracting a rounded a from 1). */
+static int ea_backward_impl(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
+                            const double* bricks64, const double* gI, const double* gT, double* grad, int mag) {
     int st = render_check(v, chan, cam, rp);
     if (st) return st;
     if (!grad) return OR_ERR_ARG;
@@ -1769,10 +1779,15 @@ int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const o
                 Tk[k] = T;
                 T *= exp(e * sk[k]);
             }
-            double Tfin = T, R = 0.0;
+            double Tfin = T, R = 0.0, Iabs = 0.0;
+            if (mag)
+                for (int64_t k = 0; k < n; k++) Iabs += fabs(alb * sk[k] * Tk[k] * (1.0 - exp(e * sk[k])));
             for (int64_t k = n - 1; k >= 0; k--) {
                 double a = exp(e * sk[k]);
                 double q = gi * (alb * Tk[k] * ((1.0 - a) - e * sk[k] * a) + e * R) + gt * e * Tfin;
+                if (mag)
+                    q = fabs(gi) * (fabs(alb) * Tk[k] * (fabs(1.0 - a) + fabs(e * sk[k] * a)) + fabs(e) * Iabs) +
+                        fabs(gt * e) * Tfin;
                 R += alb * sk[k] * Tk[k] * (1.0 - a);
                 tri_cell tc = tri_locate(c.rec[k].p);
                 double w[8];
@@ -1790,6 +1805,16 @@ int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const o
         }
     free(c.rec);
     return OR_OK;
+}
+
+int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
+                           const double* bricks64, const double* gI, const double* gT, double* grad) {
+    return ea_backward_impl(v, chan, cam, rp, bricks64, gI, gT, grad, 0);
+}
+
+int or_raycast_ea_backward_mag(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
+                               const double* bricks64, const double* gI, const double* gT, double* grad) {
+    return ea_backward_impl(v, chan, cam, rp, bricks64, gI, gT, grad, 1);
 }
 
 /* number of recorded samples per ray (for sample-set tests) */
@@ -1885,8 +1910,8 @@ int or_apron_adjoint_f64(or_volume* v, double* grad) {
  * t = 1 - |v-p_i|^2/r^2 over voxels of active leaves with |v-p_i| < r.
  * Positions are taken as doubles (pos64, 3 per point) so that finite
  * differences can perturb them below FP32 resolution. */
-int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
-                                const double* grad, double* dpos, double* dw) {
+static int splat_backward_impl(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                               const double* grad, double* dpos, double* dw, int mag) {
     if (!v || (!pos64 && n > 0) || !grad) return OR_ERR_ARG;
     int br = brick_res(v);
     int64_t br3 = ipow3(br);
@@ -1894,6 +1919,7 @@ int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, co
     for (int64_t i = 0; i < n; i++) {
         const double* w = pos64 + 3 * i;
         double wt = weights ? weights[i] : 1.0;
+        if (mag) wt = fabs(wt);
         double gw = 0.0, gp[3] = {0, 0, 0};
         if (!(w[0] >= 0 && w[1] >= 0 && w[2] >= 0)) { if (dw) dw[i] = 0; if (dpos) dpos[3*i] = dpos[3*i+1] = dpos[3*i+2] = 0; continue; }
         int64_t lo[3], hi[3];
@@ -1913,6 +1939,7 @@ int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, co
                     const or_node* nd = &v->nodes[0][leaf];
                     int64_t lx = x - nd->mPos[0] + 1, ly = y - nd->mPos[1] + 1, lz = z - nd->mPos[2] + 1;
                     double g = grad[leaf * br3 + (lz * br + ly) * br + lx];
+                    if (mag) { g = fabs(g); dx = fabs(dx); dy = fabs(dy); dz = fabs(dz); }
                     double t = 1.0 - d2 / R2;
                     gw += g * t * t * t;
                     double f = g * wt * 6.0 * t * t / R2;
@@ -1922,6 +1949,18 @@ int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, co
         if (dpos) { dpos[3 * i] = gp[0]; dpos[3 * i + 1] = gp[1]; dpos[3 * i + 2] = gp[2]; }
     }
     return OR_OK;
+}
+
+int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                                const double* grad, double* dpos, double* dw) {
+    return splat_backward_impl(v, pos64, n, weights, radius, grad, dpos, dw, 0);
+}
+
+/* the same sums over |g|, |w| and |v - p| per axis: a bound on the
+ * magnitude of every term, for rounding-error floors */
+int or_splat_cubic_backward_mag_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                                    const double* grad, double* dpos, double* dw) {
+    return splat_backward_impl(v, pos64, n, weights, radius, grad, dpos, dw, 1);
 }
 
 /* forward cubic splat in FP64 from FP64 positions into padded bricks

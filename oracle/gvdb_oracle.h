@@ -192,6 +192,13 @@ int or_raycast_ea(or_volume* v, int chan, const or_camera* cam, const or_render_
 /* FP64 adjoint: grad (nb*br^3 padded bricks) += d(sum gI*I + gT*T)/d voxel */
 int or_raycast_ea_backward(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
                            const double* bricks64, const double* gI, const double* gT, double* grad);
+/* magnitude form of the adjoint: every term replaced by the magnitude of its
+ * operands (|gI|, |gT|, |1 - a| + |e s a|, |e| times the magnitude of the
+ * I sum), so that
+ * n * 2^-24 * result bounds the rounding error of an FP32 evaluation whose
+ * longest chain of dependent operations has n steps */
+int or_raycast_ea_backward_mag(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp,
+                               const double* bricks64, const double* gI, const double* gT, double* grad);
 /* the samples of one pixel's ray: leaf ids and brick-local positions */
 int or_raycast_ea_samples(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, int64_t px,
                           int64_t max_out, int64_t* leaf_out, float* p_out, int64_t* count);
@@ -203,6 +210,9 @@ int or_splat_cubic_f64_padded(or_volume* v, const double* pos64, int64_t n, cons
                               double* bricks);
 int or_splat_cubic_backward_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
                                 const double* grad, double* dpos, double* dw);
+/* magnitude form: sums of |g| |w| |dk/dp| and |g| k */
+int or_splat_cubic_backward_mag_f64(or_volume* v, const double* pos64, int64_t n, const double* weights, float radius,
+                                    const double* grad, double* dpos, double* dw);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,8 @@ def lib():
         L.or_render.argtypes = [vp, i32, cp, rpp, dp, fp, dp]
         L.or_raycast_ea.argtypes = [vp, i32, cp, rpp, dp, fp, fp, dp, dp]
         L.or_raycast_ea_backward.argtypes = [vp, i32, cp, rpp, dp, dp, dp, dp]
+        L.or_raycast_ea_backward_mag.argtypes = [vp, i32, cp, rpp, dp, dp, dp, dp]
+        L.or_splat_cubic_backward_mag_f64.argtypes = [vp, dp, i64, dp, f32, dp, dp, dp]
         L.or_raycast_ea_samples.argtypes = [vp, i32, cp, rpp, i64, i64, C.POINTER(C.c_int64), fp,
                                             C.POINTER(C.c_int64)]
         L.or_apron_f64.argtypes = [vp, dp, C.c_double]
@@ -348,16 +350,19 @@ class OracleVolume:
         sh = (c.height, c.width)
         return I32.reshape(sh), T32.reshape(sh), I64.reshape(sh), T64.reshape(sh)
 
-    def raycast_ea_backward(self, cam, rp, gI, gT=None, chan=0, bricks64=None):
-        """FP64 gradient over padded bricks (B, br, br, br) [z, y, x]."""
+    def raycast_ea_backward(self, cam, rp, gI, gT=None, chan=0, bricks64=None, mag=False):
+        """FP64 gradient over padded bricks (B, br, br, br) [z, y, x].  mag:
+        the magnitude form (every term's absolute value), whose entries scaled
+        by n * 2^-24 bound an FP32 evaluation's rounding error."""
         c, p = make_camera(cam), make_params(rp)
         B, br = self._padded_count()
         grad = np.zeros(B * br ** 3, dtype=np.float64)
         gi = np.ascontiguousarray(gI, dtype=np.float64).reshape(-1)
         gt = None if gT is None else np.ascontiguousarray(gT, dtype=np.float64).reshape(-1)
         b = None if bricks64 is None else np.ascontiguousarray(bricks64, dtype=np.float64)
-        _chk(self._L.or_raycast_ea_backward(self._h, chan, C.byref(c), C.byref(p), self._dp(b), self._dp(gi),
-                                            self._dp(gt), self._dp(grad)), "raycast_ea_backward")
+        fn = self._L.or_raycast_ea_backward_mag if mag else self._L.or_raycast_ea_backward
+        _chk(fn(self._h, chan, C.byref(c), C.byref(p), self._dp(b), self._dp(gi), self._dp(gt), self._dp(grad)),
+             "raycast_ea_backward")
         return grad.reshape(B, br, br, br)
 
     def raycast_ea_samples(self, cam, rp, px, chan=0, max_out=4096):
@@ -391,15 +396,15 @@ class OracleVolume:
                                                self._dp(out)), "splat_cubic_f64_padded")
         return out
 
-    def splat_cubic_backward_f64(self, pos64, radius, grad, weights=None):
+    def splat_cubic_backward_f64(self, pos64, radius, grad, weights=None, mag=False):
         p = np.ascontiguousarray(pos64, dtype=np.float64)
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         g = np.ascontiguousarray(grad, dtype=np.float64)
         dpos = np.zeros_like(p)
         dw = np.zeros(p.shape[0], dtype=np.float64)
-        _chk(self._L.or_splat_cubic_backward_f64(self._h, self._dp(p), p.shape[0], self._dp(w), float(radius),
-                                                 self._dp(g), self._dp(dpos), self._dp(dw)),
-             "splat_cubic_backward_f64")
+        fn = self._L.or_splat_cubic_backward_mag_f64 if mag else self._L.or_splat_cubic_backward_f64
+        _chk(fn(self._h, self._dp(p), p.shape[0], self._dp(w), float(radius), self._dp(g), self._dp(dpos),
+                self._dp(dw)), "splat_cubic_backward_f64")
         return dpos, dw
 
 

@@ -173,3 +173,83 @@ def test_batched_clouds_equal_per_cloud():
         np.testing.assert_array_equal(dpb[off[c]:off[c + 1]], s["dpos"], err_msg=f"cloud {c} dpos")
         np.testing.assert_array_equal(dwb[off[c]:off[c + 1]], s["dw"], err_msg=f"cloud {c} dw")
     assert (lc >= 0).all()
+
+
+def test_batched_clouds_vs_oracle():
+    """Every cloud of a batch against the oracle run on that cloud alone:
+    leaf set bit-exact; padded bricks of the splat + apron within 1e-5 of the
+    FP64 chain; EA images within 1e-5 (+1e-6 of the image maximum) of FP64;
+    voxel gradients (after the apron adjoint) and point gradients entry by
+    entry within 1e-5 |ref| + n 2^-24 M of the FP64 adjoints (M: the
+    magnitude adjoint, tests/tolerance.py)."""
+    import gvdbx as G
+    from gvdbx import workloads
+    from tolerance import APRON_OPS, check_entries, ea_chain_ops, splat_ops
+    dev = _dev()
+    clouds = workloads.cloud_batch(19, nclouds=5, nmin=3000, nmax=30000)
+    _log("batched clouds vs oracle: " + ", ".join(f"{len(p)} {k}" for p, w, k in clouds))
+    rp = G.ea_params(step=0.5, kappa=0.05)
+    rng = np.random.default_rng(5)
+    W = H = 48
+    gIs = [rng.normal(size=(H, W)).astype(np.float32) for _ in clouds]
+    off = np.concatenate([[0], np.cumsum([len(p) for p, w, k in clouds])])
+    P = torch.from_numpy(np.concatenate([p for p, w, k in clouds])).to(dev)
+    Wt = torch.from_numpy(np.concatenate([w for p, w, k in clouds])).to(dev)
+    gb = _vol(G)
+    gb.SetPointsBatched(P, off)
+    gb.SetPointWeights(Wt)
+    gb.RebuildTopology(int(off[-1]), 4.0)
+    gb.FinishTopology(False, True)
+    gb.UpdateAtlas()
+    gb.SplatCubic(int(off[-1]), 2.0)
+    gb.UpdateApron(0, 0.0)
+    cams = [_cloud_cam(G, p, W, H) for p, w, k in clouds]
+    I, T = gb.RaycastEABatched(0, cams, rp, record=True)
+    grad = gb.RaycastEABackwardBatched(0, cams, rp, I, T, torch.from_numpy(np.stack(gIs)).to(dev))
+    gb.ApronAdjoint(grad)
+    dpos, dw = gb.SplatCubicBackward(int(off[-1]), 2.0, (0, 0, 0), grad)
+    torch.cuda.synchronize()
+    cell_off, bounds = gb.cloud_layout()
+    lc = gb.leaf_clouds()
+    nodes, bricks, gradb = gb.nodes(0), gb.bricks(0), grad.cpu().numpy()
+    Ib, Tb = I.cpu().numpy(), T.cpu().numpy()
+    dpb, dwb = dpos.cpu().numpy(), dw.cpu().numpy()
+    inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    from test_gpu_render import _cam_dict, _img_close, _rp_dict
+    for c, (p, w, k) in enumerate(clouds):
+        ov = O.OracleVolume()
+        ov.add_channel(0, 1)
+        ov.rebuild_topology(p, 4.0)
+        ov.update_atlas()
+        onodes = ov.nodes(0)
+        sel = np.nonzero(lc == c)[0]
+        local = nodes["mPos"][sel] - cell_off[c] * 8
+        ref = {kk: i for i, kk in enumerate(_leaf_key(onodes["mPos"]))}
+        keys = _leaf_key(local)
+        assert sorted(keys) == sorted(ref), f"cloud {c}: leaf set differs from the oracle"
+        order = np.array([ref[kk] for kk in keys])
+        oinf = ov.info()
+        np.testing.assert_array_equal(bounds[c], np.array(list(oinf.obj_min) + list(oinf.obj_max), np.float32))
+        p64, w64 = p.astype(np.float64), w.astype(np.float64)
+        b64 = ov.apron_f64(ov.splat_cubic_f64_padded(p64, 2.0, w64), 0.0)
+        got = bricks[sel].astype(np.float64)
+        want = b64[order]
+        floor = 1e-6 * np.abs(want).max()
+        err = np.abs(got - want)
+        assert (err <= 1e-5 * np.abs(want) + floor).all(), f"cloud {c}: splat max err {err.max():.3e}"
+        cd, rd = _cam_dict(cams[c]), _rp_dict(rp)
+        _, _, I64, T64 = ov.raycast_ea(cd, rd, bricks64=b64)
+        assert (I64 > 0).sum() > 100, f"cloud {c}: the view misses the cloud"
+        _img_close(Ib[c], I64, f"cloud {c} I")
+        _img_close(Tb[c], T64, f"cloud {c} T")
+        gI64 = gIs[c].astype(np.float64)
+        g64 = ov.apron_adjoint_f64(ov.raycast_ea_backward(cd, rd, gI64, bricks64=b64))
+        m64 = ov.apron_adjoint_f64(ov.raycast_ea_backward(cd, rd, gI64, bricks64=b64, mag=True))
+        n_ea = ea_chain_ops(oinf.obj_min, oinf.obj_max, rp.step) + APRON_OPS
+        check_entries(gradb[sel][inner], g64[order][inner], m64[order][inner], n_ea, f"cloud {c} voxel gradient")
+        rdp, rdw = ov.splat_cubic_backward_f64(p64, 2.0, g64, w64)
+        mdp, mdw = ov.splat_cubic_backward_f64(p64, 2.0, m64, w64, mag=True)
+        n_pt = n_ea + splat_ops(2.0)
+        check_entries(dpb[off[c]:off[c + 1]], rdp, mdp, n_pt, f"cloud {c} dL/dp")
+        check_entries(dwb[off[c]:off[c + 1]], rdw, mdw, n_pt, f"cloud {c} dL/dw")
+        _log(f"  cloud {c} ({k}, {len(p)} points, {len(sel)} leaves): oracle parity ok")

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from tolerance import check_entries, splat_ops
 
 pytestmark = pytest.mark.gpu
 
@@ -257,6 +258,8 @@ def test_splat_cubic_variants_and_runs():
     gr = rng.normal(size=(inf.node_cnt[0], inf.brick_res, inf.brick_res, inf.brick_res)).astype(np.float32)
     gt = torch.from_numpy(gr).to(tp.device)
     rdpos, rdw = ov.splat_cubic_backward_f64(pts.astype(np.float64), 2.0, gr.astype(np.float64), w.astype(np.float64))
+    mdpos, mdw = ov.splat_cubic_backward_f64(pts.astype(np.float64), 2.0, gr.astype(np.float64), w.astype(np.float64),
+                                             mag=True)
     for variant, chunk in ((1, 1024), (1, 64), (0, 1024)):
         gv.SetTuning("splat_variant", variant)
         gv.SetTuning("splat_chunk", chunk)
@@ -264,9 +267,8 @@ def test_splat_cubic_variants_and_runs():
         dpos2, dw2 = gv.SplatCubicBackward(len(pts), 2.0, (0, 0, 0), gt)
         torch.cuda.synchronize()
         assert torch.equal(dpos, dpos2) and torch.equal(dw, dw2)
-        for got, ref, name in ((dpos.cpu().numpy(), rdpos, "dpos"), (dw.cpu().numpy(), rdw, "dw")):
-            scale = np.abs(ref).max()
-            assert np.abs(got - ref).max() <= 1e-5 * scale, f"{name} variant {variant} chunk {chunk}"
+        for got, ref, mag, name in ((dpos.cpu().numpy(), rdpos, mdpos, "dpos"), (dw.cpu().numpy(), rdw, mdw, "dw")):
+            check_entries(got, ref, mag, splat_ops(2.0), f"{name} variant {variant} chunk {chunk}")
     gv.SetTuning("splat_variant", 1)
     gv.SetTuning("splat_chunk", 1024)
     with pytest.raises(Exception):

@@ -335,3 +335,34 @@ def test_render_errors():
     v.add_channel(0, 1)
     with pytest.raises(O.OracleError):
         v.raycast_ea(ortho(4, 4, 0, 0, 8), EA)   # no topology / atlas
+
+
+def test_magnitude_adjoints_bound_the_adjoints(cluster):
+    """The magnitude forms (every term's absolute value) bound the adjoints
+    entry by entry, equal them where every term is non-negative (gT >= 0
+    alone, weights and gradients >= 0), and are linear in |gI|, |gT|."""
+    v, pos64, w, cam, _, rng = cluster
+    r, rp = 2.0, EA
+    b64 = v.apron_f64(v.splat_cubic_f64_padded(pos64, r, w), 0.0)
+    gI = rng.normal(size=(cam["height"], cam["width"]))
+    gT = rng.normal(size=(cam["height"], cam["width"]))
+    g = v.raycast_ea_backward(cam, rp, gI, gT, bricks64=b64)
+    m = v.raycast_ea_backward(cam, rp, gI, gT, bricks64=b64, mag=True)
+    assert (m >= np.abs(g) - 1e-12 * np.abs(m).max()).all()
+    assert (m > 0).any()
+    m2 = v.raycast_ea_backward(cam, rp, 2 * np.abs(gI), -np.abs(gT), bricks64=b64, mag=True)
+    m1 = v.raycast_ea_backward(cam, rp, np.abs(gI), np.abs(gT), bricks64=b64, mag=True)
+    mI = v.raycast_ea_backward(cam, rp, np.abs(gI), None, bricks64=b64, mag=True)
+    np.testing.assert_allclose(m2, m1 + mI, rtol=1e-12, atol=1e-300)
+    # dT/drho has every term of one sign: with gT >= 0 alone, |adjoint| == magnitude
+    gpos = np.abs(gT)
+    np.testing.assert_allclose(np.abs(v.raycast_ea_backward(cam, rp, np.zeros_like(gI), gpos, bricks64=b64)),
+                               v.raycast_ea_backward(cam, rp, np.zeros_like(gI), gpos, bricks64=b64, mag=True),
+                               rtol=1e-12, atol=1e-300)
+    gv = rng.normal(size=b64.shape)
+    dp, dw = v.splat_cubic_backward_f64(pos64, r, gv, w)
+    mp, mw = v.splat_cubic_backward_f64(pos64, r, gv, w, mag=True)
+    assert (mp >= np.abs(dp) - 1e-12).all() and (mw >= np.abs(dw) - 1e-12).all()
+    _, mw_pos = v.splat_cubic_backward_f64(pos64, r, np.abs(gv), w, mag=True)
+    _, dw_pos = v.splat_cubic_backward_f64(pos64, r, np.abs(gv), w)
+    np.testing.assert_allclose(mw_pos, dw_pos, rtol=1e-12, atol=1e-300)
