@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "gvdbx/gvdbx_c.h"
@@ -44,6 +45,43 @@ struct DevBuf {
 };
 
 constexpr int64_t MAX_DENSE_CELLS = int64_t(1) << 28;
+
+}  // namespace
+
+// Scratch of gvdbx_gather_level_set_noalloc: the packed membership range of
+// every binned point (one 32-bit word each).  Device memory that the loader
+// reserves on every GPU the library runs on, held until the process ends.
+// In host code the name is only a stand-in: the device address comes from
+// hipGetSymbolAddress (scratch_on) and is the one that reaches kernels.
+__device__ __align__(256) uint32_t scratch[GVDBX_NOALLOC_SCRATCH_BYTES / 4];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (which must be `dev`),
+// looked up once per device.
+int scratch_on(int dev, uint32_t** out) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        gvdbx_set_error("device %d beyond the %d devices the scratch table holds", dev, SCRATCH_MAX_DEVICES);
+        return GVDBX_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            gvdbx_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return GVDBX_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GVDBX_OK;
+}
 
 }  // namespace
 
@@ -857,8 +895,13 @@ int gather_color(gvdbx_volume* v, int chan_clr, int accum) {
     return check_launch("gather colour");
 }
 
-int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int chan, int chan_clr, int accum) {
+// noalloc: the packed ranges go to the library's static scratch, and no
+// other scratch is used (no reach classes, no split tiles), so the call
+// allocates nothing; the result is the same bit for bit.
+int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int chan, int chan_clr, int accum,
+           bool noalloc = false) {
     const char* what = mode ? "GatherLevelSet" : "GatherDensity";
+    if (noalloc) what = "GatherLevelSetNoAlloc";
     TRY(need_atlas(v, what));
     TRY(check_chan_type(v, chan, GVDBX_FLOAT, what));
     // point colours make the gather average them into chan_clr
@@ -890,11 +933,20 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     // starts at 0 and a member contributes r - c.  The margin covers FP32
     // rounding of the distance (relative error < 1e-6).
     gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
-    gp.variant = v->gather_variant;
+    gp.variant = noalloc ? 1 : v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
     if (gp.variant == 1 && (int)std::ceil(gp.box_radius) + sc <= 29 && v->bins_n > 0) {
         // each point's clipped ranges once, not once per tile that reads it
-        if (v->gather_classes) {
+        if (noalloc) {
+            if ((uint64_t)v->bins_n * 4 > GVDBX_NOALLOC_SCRATCH_BYTES)
+                FAIL(GVDBX_ERR_RANGE, "%s: %lld points need %lld bytes of scratch, more than its %llu", what,
+                     (long long)v->bins_n, (long long)v->bins_n * 4, (unsigned long long)GVDBX_NOALLOC_SCRATCH_BYTES);
+            uint32_t* box = nullptr;
+            TRY(scratch_on(v->device, &box));
+            launch_gather_box(gp.sorted, gp.off + gp.gb.count(), v->bins_n, gp.box_radius, sc, gp.sc_shift, gp.cutoff,
+                              box, v->stream);
+            gp.box = box;
+        } else if (v->gather_classes) {
             // and each bin's entries grouped by the neighbour cells they can
             // reach, so a tile reads only the groups that can reach it
             const int64_t nbin = gp.gb.count();
@@ -922,7 +974,7 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
             gp.box = v->gbox.as<uint32_t>();
         }
     }
-    if (gp.variant == 1 && v->gather_split > 0) {
+    if (gp.variant == 1 && v->gather_split > 0 && !noalloc) {
         // Bounds: a bin is a neighbour of at most 27 leaves, so the chunks
         // summed over all tiles are at most 27 * tiles * (chunks of all bins),
         // and those are at most ceil(n/64) + non-empty bins (<= min(n, bins)).
@@ -1041,6 +1093,8 @@ int gvdbx_create(gvdbx_volume** out, int device) {
     gvdbx_volume* v = new gvdbx_volume();
     v->device = device;
     int st = set_dev(v);
+    uint32_t* sp = nullptr;
+    if (st == GVDBX_OK) st = scratch_on(device, &sp);   // set-up: the no-alloc scratch's address on this device
     if (st != GVDBX_OK) {
         delete v;
         return st;
@@ -1393,6 +1447,13 @@ int gvdbx_gather_level_set(gvdbx_volume* v, int sc, int num_pnts, float radius, 
     (void)num_pnts; (void)tx; (void)ty; (void)tz;
     TRY(check_vol(v));
     return gather(v, 1, sc, radius, sc_len, chan_levelset, chan_clr, accum);
+}
+
+int gvdbx_gather_level_set_noalloc(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                                   int64_t sc_len, int chan_levelset, int chan_clr, int accum) {
+    (void)num_pnts; (void)tx; (void)ty; (void)tz;
+    TRY(check_vol(v));
+    return gather(v, 1, sc, radius, sc_len, chan_levelset, chan_clr, accum, true);
 }
 
 int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix) {

@@ -209,6 +209,12 @@ class VolumeGVDB:
         call("gvdbx_gather_level_set", self._h, int(subcell_size), int(num_pnts), float(radius),
              *[float(x) for x in trans], int(sc_len), int(chan_levelset), int(chan_clr), int(accum))
 
+    def GatherLevelSetNoAlloc(self, subcell_size, num_pnts, radius, trans, sc_len, chan_levelset=0, chan_clr=-1,
+                              accum=False):
+        """GatherLevelSet that allocates no device memory (gvdbx_gather_level_set_noalloc)."""
+        call("gvdbx_gather_level_set_noalloc", self._h, int(subcell_size), int(num_pnts), float(radius),
+             *[float(x) for x in trans], int(sc_len), int(chan_levelset), int(chan_clr), int(accum))
+
     def InsertPoints(self, num_pnts, trans=(0.0, 0.0, 0.0), prefix=True):
         call("gvdbx_insert_points", self._h, int(num_pnts), *[float(x) for x in trans], int(prefix))
 

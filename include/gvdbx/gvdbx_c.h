@@ -154,6 +154,21 @@ int gvdbx_gather_density(gvdbx_volume* v, int subcell_size, int num_pnts, float 
  * cuda_gvdb_particles.cuh:757-771) */
 int gvdbx_gather_level_set(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
                            float tx, float ty, float tz, int64_t sc_len, int chan_levelset, int chan_clr, int accum);
+/* GatherLevelSet that allocates nothing (no reference counterpart): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * the same result bit for bit as gvdbx_gather_level_set; its only scratch,
+ * one 32-bit word per binned point, is a static device array of
+ * GVDBX_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until
+ * the process ends: more than GVDBX_NOALLOC_SCRATCH_BYTES / 4 points (the
+ * num_pnts of the last InsertPointsSubcell) fail with GVDBX_ERR_RANGE.  Calls on one GPU share that array, so calls that overlap
+ * on one GPU (other threads or streams) race; calls ordered on one stream do
+ * not.  Allocation happens only in the earlier calls that set the volume up
+ * (topology, atlas, InsertPointsSubcell). */
+#define GVDBX_NOALLOC_SCRATCH_BYTES ((uint64_t)64 << 20)
+int gvdbx_gather_level_set_noalloc(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
+                                   float tx, float ty, float tz, int64_t sc_len, int chan_levelset, int chan_clr,
+                                   int accum);
 /* InsertPoints(int, Vector3DF, bool) VG.h:542 / VG.cpp:5219-5267 */
 int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix);
 /* ScatterDensity(int, float, float, Vector3DF, bool, bool) VG.h:555 / VG.cpp:5624-5660.
