@@ -28,8 +28,6 @@ sys.path.insert(0, os.path.join(ROOT, "gvdb-voxels_amd"))
 
 METRIC = "Mpoints/s topology+splat (HBM % of peak); fwd+bwd/fwd time ratio"
 HBM_PEAK = 8.0e12
-CU_COUNT = 256          # MI355X compute units
-CLOCK_HZ = 2.4e9        # MI355X peak engine clock
 SEED = 20261203 + 1          # configs[1]
 
 
@@ -75,6 +73,26 @@ def measured_traffic(kernel_profile):
     """HBM bytes per launch of the dominant kernel from its committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
     return measured_value(kernel_profile, "traffic_bytes_per_launch")
+
+
+def valu_ceiling(stream):
+    """Vector-issue ceilings measured on this GPU (libgvdbx_calib.so): wave64
+    VALU instructions per second of eight independent FMA chains, and of the
+    level-set walk's own instruction mix (readlanes, subtractions, FMAs,
+    unsigned min), each on 2048 workgroups of 256 threads."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(ROOT, "gvdb-voxels_amd", "libgvdbx_calib.so"))
+    L.gvdbx_calib_valu.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    out = {}
+    for kind, name in ((0, "fma_chains"), (1, "walk_mix")):
+        ms, valu = C.c_double(0), C.c_double(0)
+        code = L.gvdbx_calib_valu(kind, 2048, 20000 if kind == 0 else 8000, C.c_void_p(stream.cuda_stream),
+                                  C.byref(ms), C.byref(valu))
+        if code != 0:
+            raise RuntimeError(f"VALU calibration kernel {kind} failed with status {code}")
+        out[name] = {"valu_per_s": valu.value / (ms.value / 1e3), "ms": round(ms.value, 4)}
+    return out
 
 
 def make_clusters(seed, n_clusters=64, per=32768):
@@ -369,6 +387,8 @@ def main():
     ap.add_argument("--xcd-remap", type=int, default=None, help="1: contiguous brick runs per XCD (default 0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
+    ap.add_argument("--calibrate-valu", action="store_true",
+                    help="only measure the vector-issue ceilings (libgvdbx_calib.so) and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path has no multi-GPU form here); got --gpus {args.gpus}",
@@ -386,6 +406,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": a.numel() * 4}))
+        return
+    if args.calibrate_valu:
+        print(json.dumps(valu_ceiling(torch.cuda.current_stream())), flush=True)
         return
     if args.only == "diff":
         print(json.dumps(bench_differentiable(torch, gvdbx, dev, max(3, min(args.steps, 10)),
@@ -465,6 +488,7 @@ def main():
     value = n / (ms / 1e3) / 1e6
     traffic = measured_traffic("gather_levelset_config2_counters.md") if n == 1 << 24 else None
     valu = measured_value("gather_levelset_config2_counters.md", "valu_insts_per_launch") if n == 1 << 24 else None
+    ceil = valu_ceiling(stream)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -480,12 +504,15 @@ def main():
                      "traffic": traffic, "alg_bytes_per_launch": gather_bytes,
                      "avg_launch_ms": round(stage["gather_levelset_ms"], 4),
                      "traffic_source": "profiles/gather_levelset_config2_counters.md" if traffic else None,
-                     # the call is bound by vector-ALU issue, not HBM: wave64 VALU instructions of its kernels
-                     # (same counter summary) over what 256 CUs issue at one per cycle at 2.4 GHz in the
-                     # event-timed call duration
-                     "valu_insts_per_launch": valu,
-                     "valu_issue_frac": (round(valu / (stage["gather_levelset_ms"] / 1e3 * CU_COUNT * CLOCK_HZ), 3)
-                                         if valu else None)},
+                     # vector issue: wave64 VALU instructions of the call's kernels (same counter
+                     # summary) per second of the event-timed call, against the issue rate of the
+                     # walk's instruction mix measured on this GPU by a calibration kernel
+                     "valu_issue": {
+                         "insts_per_launch": valu,
+                         "achieved_per_s": round(valu / gather_s, -6) if valu else None,
+                         "ceiling_walk_mix_per_s": round(ceil["walk_mix"]["valu_per_s"], -6),
+                         "ceiling_fma_chains_per_s": round(ceil["fma_chains"]["valu_per_s"], -6),
+                         "frac": round(valu / gather_s / ceil["walk_mix"]["valu_per_s"], 3) if valu else None}},
         "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
                            "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},

@@ -314,6 +314,15 @@ int radix_sort_pairs32(uint32_t* keys, uint32_t* vals, uint32_t* tkeys, uint32_t
 void launch_iota_u32(uint32_t* p, int64_t n, hipStream_t s);
 
 // ---------------- binning by radix sort ----------------
+// Points binned by cell of gb (leaf grid grown by one cell) with a stable LSD
+// radix sort that moves (x, y, z, index) records (translated positions;
+// invalid points as (-1, -1, -1, index), sorting last with the points outside
+// gb).  buf0 / buf1: n float4 each; returns which holds the result.  keys
+// (n uint32) gets the sorted keys; with rp, each leaf cell's extra-brick
+// reach goes to reach0.  hist / off / partials as for radix_sort_pairs32.
+int bin_sort_records(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, float4* buf0,
+                     float4* buf1, uint32_t* keys, int32_t* hist, int32_t* off, uint32_t* partials,
+                     const ReachParams* rp, uint32_t* reach0, hipStream_t s);
 // key[i] = bin cell of point i (gb.count() for invalid / outside points),
 // val[i] = i
 void launch_bin_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid gb, int logr0, CloudMap cm,

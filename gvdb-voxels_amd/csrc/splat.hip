@@ -861,7 +861,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 float pw[3] = {cur.x, cur.y, cur.z};
                 int T0[3] = {T0x, T0y, T0z};
                 int rlo[2] = {0, 0}, rhi[2] = {0, 0};
-                float db2 = 0.0f;
+                float db2 = 0.0f, db2xy = 0.0f;
                 if (CLS || gp.box) ok = ok && !(curb >> 31);
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
@@ -883,6 +883,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                     int lr = max(lo_c - T0[a], 0), hr = min(hi_c - T0[a], td - 1);
                     ok = ok && lr <= hr;
                     float e = fmaxf(fmaxf((float)(T0[a] + lr) - pw[a], pw[a] - (float)(T0[a] + hr)), 0.0f);
+                    if (a == 2) db2xy = db2;
                     db2 = __builtin_fmaf(e, e, db2);
                     if (a < 2) {
                         rlo[a] = lr, rhi[a] = hr;
@@ -891,6 +892,16 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                             // z: membership only (voxels beyond the cutoff cannot change the result)
                             lr = max(lo - T0[a], 0);
                             hr = min(hi - T0[a], td - 1);
+                            // A z-half whose nearest voxel is farther than the bound
+                            // (x and y at their nearest columns) holds no voxel the
+                            // candidate can lower: fma is monotone, so every voxel
+                            // there has |p-v|^2 >= that distance > bound >= its
+                            // running minimum (or > cutoff^2).  Same arithmetic as db2.
+                            const float e0 = fmaxf(fmaxf((float)T0[a] - pw[a], pw[a] - (float)(T0[a] + 3)), 0.0f);
+                            const float e1 = fmaxf(fmaxf((float)(T0[a] + 4) - pw[a], pw[a] - (float)(T0[a] + 7)), 0.0f);
+                            if (__builtin_fmaf(e0, e0, db2xy) > bound) lr = max(lr, 4);
+                            if (__builtin_fmaf(e1, e1, db2xy) > bound) hr = min(hr, 3);
+                            ok = ok && lr <= hr;
                         }
                         pk = (uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3);
                     }

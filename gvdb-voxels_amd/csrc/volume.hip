@@ -133,12 +133,13 @@ struct gvdbx_volume {
     int splat_chunk = SPLAT_CHUNK_DEFAULT;
     int splat_variant = 1;
     int sort_tiles = 0;   // point binning sort: 0 auto (wide from 2^23 valid points), 1 narrow, 2 wide
+    int bin_records = 1;  // 1: binning sort moves point records; 0: sorts (cell, index) pairs, then gathers
     // atlas map
     bool has_atlas = false;
     DevBuf amap;
     int64_t amap_cnt = 0;
     // binning
-    DevBuf bin_off, sorted, bkey, bval, bkey2, bval2, rs_hist, rs_off, rs_part;
+    DevBuf bin_off, sorted, sorted2, bkey, bval, bkey2, bval2, rs_hist, rs_off, rs_part;
     DevBuf splat_src;   // (x, y, z, weight) in bin order, made per SplatCubic call
     DevBuf gs_items, gs_counts, gs_slots, gs_acc;   // Gather* work items of dense tiles
     DevBuf gbox;                                    // Gather*: packed ranges per bin entry
@@ -395,6 +396,31 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     const int64_t m = std::max<int64_t>(n, 1);
     const int64_t nh = radix_hist_entries(m);
     TRY(ensure(v, v->bin_off, (size_t)(nb + 1) * 4));
+    if (v->ncloud == 0 && v->bin_records) {
+        // one cloud: the sort moves the point records (no random point reads)
+        TRY(ensure(v, v->bkey, (size_t)m * 4));
+        TRY(ensure(v, v->rs_hist, (size_t)nh * 4));
+        TRY(ensure(v, v->rs_off, (size_t)(nh + 1) * 4));
+        TRY(ensure(v, v->rs_part, (size_t)(scan_tiles(nh + 1) + 1) * 4));
+        TRY(ensure(v, v->sorted, (size_t)m * 16));
+        TRY(ensure(v, v->sorted2, (size_t)m * 16));
+        const int which = bin_sort_records(v->pos, n, t[0], t[1], t[2], gb, v->logdim[0], v->sorted.as<float4>(),
+                                           v->sorted2.as<float4>(), v->bkey.as<uint32_t>(), v->rs_hist.as<int32_t>(),
+                                           v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), rp,
+                                           v->reach0.as<uint32_t>(), s);
+        if (which == 1) std::swap(v->sorted, v->sorted2);
+        if (reached) *reached = rp != nullptr && n > 0;
+        launch_bin_offsets(v->bkey.as<uint32_t>(), n, nb, v->bin_off.as<int32_t>(), s);
+        TRY(check_launch("bin points"));
+        v->gbin = gb;
+        v->bins_valid = true;
+        v->bins_n = n;
+        v->bins_gen = v->pos_gen;
+        v->bins_base = v->pos.base;
+        v->bins_stride = v->pos.stride;
+        memcpy(v->bins_trans, t, sizeof(v->bins_trans));
+        return GVDBX_OK;
+    }
     TRY(ensure(v, v->bkey, (size_t)m * 4));
     TRY(ensure(v, v->bval, (size_t)m * 4));
     TRY(ensure(v, v->bkey2, (size_t)m * 4));
@@ -1118,7 +1144,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
         free_buf(v->flag_all[l]);
     }
     DevBuf* bufs[] = {&v->reach0, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
-                      &v->bin_off, &v->sorted, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
+                      &v->bin_off, &v->sorted, &v->sorted2, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
@@ -1279,6 +1305,12 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->bins_valid = false;   // the next binning sorts again with the chosen tiles
         return GVDBX_OK;
     }
+    if (!strcmp(key, "bin_records")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "bin_records must be 0 or 1");
+        v->bin_records = value;
+        v->bins_valid = false;
+        return GVDBX_OK;
+    }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1311,6 +1343,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "sort_tiles")) {
         *value = v->sort_tiles;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "bin_records")) {
+        *value = v->bin_records;
         return GVDBX_OK;
     }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);

@@ -110,7 +110,10 @@ int gvdbx_set_topology_mode(gvdbx_volume* v, int mode);
  * [64, 2^24]), "sort_tiles" (the stable radix sort that bins points by
  * brick: 0 = tiles of 8192 entries from 2^23 valid points on, else 4096;
  * 1 = always 4096; 2 = always 8192; the bins are the same either way;
- * default 0) */
+ * default 0), "bin_records" (1 = the binning sort of a single cloud moves
+ * the 16-byte point records through its passes; 0 = it sorts (cell, index)
+ * pairs and then gathers each point by index; the bins are the same either
+ * way; default 1) */
 int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value);
 int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value);
 

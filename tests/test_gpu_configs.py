@@ -136,13 +136,15 @@ def test_dambreak_full_frame_vs_oracle_slab(frame):
     assert vals.min() < 0.0 and vals.max() == 3.0
 
 
-@pytest.mark.parametrize("tiles", [1, 2])
-def test_binning_sort_tiles_bit_exact(tiles):
-    """The binning sort with 4096- and 8192-entry tiles forced at 200k
-    points: topology, list length, level set and apron equal the oracle."""
+@pytest.mark.parametrize("tiles,records", [(1, 0), (2, 0), (0, 1)])
+def test_binning_sort_tiles_bit_exact(tiles, records):
+    """Both binning sorts (records moved through the passes; (cell, index)
+    pairs then a gather, with 4096- and 8192-entry tiles forced) at 200k
+    points with invalid points: topology, list length, level set, apron and
+    InsertPoints equal the oracle."""
     G = _G()
     dev = _dev()
-    _log(f"[gpu] sort tiles {tiles}")
+    _log(f"[gpu] sort tiles {tiles} records {records}")
     rng = np.random.default_rng(600 + tiles)
     n = 200000
     pts = (rng.random((n, 3)) * [200, 120, 180] + [10, 20, 30]).astype(np.float32)
@@ -151,7 +153,8 @@ def test_binning_sort_tiles_bit_exact(tiles):
     gv.configure([3, 3, 3, 3, 5])
     gv.AddChannel(0, G.T_FLOAT, 1)
     gv.SetTuning("sort_tiles", tiles)
-    assert gv.GetTuning("sort_tiles") == tiles
+    gv.SetTuning("bin_records", records)
+    assert gv.GetTuning("sort_tiles") == tiles and gv.GetTuning("bin_records") == records
     tp = torch.from_numpy(pts).to(dev)
     gv.SetPoints(tp)
     gv.RebuildTopology(n, 3.0)
@@ -176,3 +179,36 @@ def test_binning_sort_tiles_bit_exact(tiles):
     np.testing.assert_array_equal(poff, opo)   # rank in ascending point order: the bins' stable order
     with pytest.raises(G.GvdbxError):
         gv.SetTuning("sort_tiles", 3)
+    with pytest.raises(G.GvdbxError):
+        gv.SetTuning("bin_records", 2)
+
+
+def test_binning_records_translated_bins():
+    """Bins made for a translation other than the topology's (InsertPoints
+    with trans: some points leave the bin grid) are the same with both
+    binning sorts."""
+    G = _G()
+    dev = _dev()
+    _log("[gpu] binning with a translation")
+    rng = np.random.default_rng(611)
+    n = 150000
+    pts = (rng.random((n, 3)) * [120, 90, 100] + [20, 30, 25]).astype(np.float32)
+    tp = torch.from_numpy(pts).to(dev)
+    outs = []
+    for records in (0, 1):
+        gv = G.VolumeGVDB(0)
+        gv.configure([3, 3, 3, 3, 5])
+        gv.AddChannel(0, G.T_FLOAT, 1)
+        gv.SetTuning("bin_records", records)
+        gv.SetPoints(tp)
+        gv.RebuildTopology(n, 1.0)
+        gv.FinishTopology(False, True)
+        gv.UpdateAtlas()
+        gv.InsertPoints(n, (17.5, -9.25, 3.0))
+        outs.append(gv.point_nodes(n))
+        gv.ScatterDensity(n, 2.0, 1.0, (17.5, -9.25, 3.0), True)
+        outs.append(gv.bricks(0))
+    np.testing.assert_array_equal(outs[0][0], outs[2][0])
+    np.testing.assert_array_equal(outs[0][1], outs[2][1])
+    np.testing.assert_array_equal(outs[1], outs[3])
+    assert (outs[0][0] >= 0).sum() > n // 3 and (outs[0][0] < 0).sum() > 1000
