@@ -351,10 +351,35 @@ def bench_batched_clouds(torch, gvdbx, dev, steps):
     }
 
 
+def host_cpu():
+    """nproc and the CPU model name of this host (lscpu's 'Model name')."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
 def cpu_baseline(pts):
-    """The CPU oracle (single thread) on a full-density slab of frame 0 (x < 96: 1/16 of the points)."""
+    """The CPU oracle (single-threaded C restatement of the reference
+    algorithm) on one host core:
+      - the shell workload in full (2^20 points, topology and subcell
+        radius 4, GatherDensity r = 2, UpdateApron(0));
+      - the dam-break frame 0 on a full-density slab x < 96 (1/16 of the
+        points, every slab point of the full frame), topology + subcell lists
+        + GatherLevelSet + UpdateApron, extrapolated to the whole frame by its
+        point count (the work per point is the same throughout the block).
+    value: the dam-break frame's extrapolated Mpoints/s (the unit of the GPU
+    line)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    import gvdbx
+    nproc, model = host_cpu()
+    # dam-break frame 0, slab x < 96
     sub = pts[pts[:, 0] < 96.0]
     ov = oracle.OracleVolume((3, 3, 3, 3, 5))
     ov.add_channel(0, 1)
@@ -365,9 +390,29 @@ def cpu_baseline(pts):
     ov.gather_level_set(1.5)
     ov.update_apron(0, 3.0)
     dt = time.perf_counter() - t0
-    return {"value": round(len(sub) / dt / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
-            "sample": f"frame 0 slab x<96 ({len(sub)} points, full density), topology+subcell insert+level set+apron,"
-                      f" {dt:.1f} s on 1 host core"}
+    del ov
+    frame_s = dt * len(pts) / len(sub)
+    # shell workload in full
+    shell = gvdbx.gen_shell(20261203, 1 << 20, (256.0, 256.0, 256.0), 180.0, 200.0)
+    ov = oracle.OracleVolume((3, 3, 3, 3, 5))
+    ov.add_channel(0, 1)
+    t0 = time.perf_counter()
+    ov.rebuild_topology(shell, 4.0)
+    ov.update_atlas()
+    ov.insert_points_subcell(4, shell, 4.0)
+    ov.gather_density(2.0)
+    ov.update_apron(0, 0.0)
+    ds = time.perf_counter() - t0
+    leaves = ov.info().node_cnt[0]
+    del ov
+    return {"value": round(len(pts) / frame_s / 1e6, 4), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+            "nproc": nproc, "cpu_model": model, "cores_used": 1,
+            "sample": f"dam-break frame 0: slab x<96 ({len(sub)} of {len(pts)} points, full density), topology + "
+                      f"subcell lists + level set + apron in {dt:.2f} s on 1 host core, extrapolated by point count "
+                      f"to {frame_s:.1f} s per frame",
+            "shell_workload": {"points": len(shell), "leaf_bricks": int(leaves), "seconds": round(ds, 2),
+                               "Mpoints_per_s": round(len(shell) / ds / 1e6, 4),
+                               "steps": "topology r4 + subcell lists + GatherDensity r2 + apron, run in full"}}
 
 
 def main():

@@ -123,6 +123,22 @@ struct CloudMap {
     }
 };
 
+// FP16 subcell lists (InsertPointsSubcell_FP16, cuda_gvdb_particles.cuh:
+// 154-216, read back by gvdbGatherLevelSet_fp16 775-811): a position w is
+// stored as the ushort (w - mn) / rg * 65535 (truncated; clamped to
+// [0, 65535], NaN -> 0) and read back as q / 65535 * rg + mn, every step an
+// FP32 operation in that order.  mn, rg: the bounding box GetBoundingBox
+// left (pos_min and pos_max - pos_min).
+struct Quant16 {
+    int on;
+    float mn[3], rg[3];
+};
+__host__ __device__ __forceinline__ float quant16_round_trip(float w, float mn, float rg) {
+    const float t = ((w - mn) / rg) * 65535.0f;
+    const uint32_t q = !(t > 0.0f) ? 0u : (t >= 65535.0f ? 65535u : (uint32_t)t);
+    return ((float)q / 65535.0f) * rg + mn;
+}
+
 #define GVDBX_CHECK(call)                                                          \
     do {                                                                            \
         hipError_t _e = (call);                                                     \

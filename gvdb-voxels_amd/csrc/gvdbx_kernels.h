@@ -172,10 +172,13 @@ void launch_gather_box(const float4* sorted, const int32_t* total, int64_t nmax,
 // as launch_gather_box.  Scratch for bins of more than 256 entries, which
 // are cut into units of 256: ctr (2 int32), units (unit_cap int2, at least
 // entries / 128 + 2), bigs (big_cap int4, at least entries / 256 + 1),
-// ucnt (9 int32 per unit).
+// ucnt (9 int32 per unit).  qz (optional): FP16 lists; the entries' xyz
+// become their FP16 round trips (membership still from the inserted point),
+// and the offsets need ceil(box_radius) + sc <= 28.
 void launch_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0, float box_radius, int sc,
                            int sc_shift, float cutoff, float4* csorted, int32_t* coff, uint8_t* both, int32_t* ctr,
-                           int2* units, int64_t unit_cap, int4* bigs, int64_t big_cap, int32_t* ucnt, hipStream_t s);
+                           int2* units, int64_t unit_cap, int4* bigs, int64_t big_cap, int32_t* ucnt, hipStream_t s,
+                           const Quant16* qz = nullptr);
 constexpr int GATHER_SPLIT_DEFAULT = 16;   // chunks per work item of a dense tile (0: one wave per tile)
 void launch_gather(GatherParams gp, hipStream_t s);
 // Variant 1 with tiles of more than 2 split chunks of 64 candidates cut into

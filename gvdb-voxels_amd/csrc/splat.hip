@@ -311,6 +311,28 @@ __device__ __forceinline__ void gather_range(float p, float R, int sc, int scs, 
 // Each point's ranges once per gather instead of once per tile that reads
 // it (up to 27).  Offsets: floor(p) - lo_c + 1 in [0, ceil(R) + sc + 2] and
 // hi_c - floor(p) in [0, R + sc + 1] when the range is not empty.
+// pm: the position membership is computed from (the point inserted into
+// the lists); pc: the position distances are measured from (the same, or
+// its FP16 round trip), whose floor the offsets are taken from.
+__device__ __forceinline__ uint32_t gather_box_word2(const float (&pm)[3], const float (&pc)[3], float R, int sc,
+                                                     int scs, float cut, int (&lo_c)[3], int (&hi_c)[3]) {
+    uint32_t bw = 0;
+    bool empty = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        // membership alone (gather_range's box), then the cutoff from pc
+        const int lo = gvdbx_cdiv((int)(pm[a] - R), scs) * sc;
+        const int hi = gvdbx_cdiv((int)(pm[a] + 1 + R), scs) * sc + sc - 1;
+        lo_c[a] = max(lo, (int)ceilf(pc[a] - cut));
+        hi_c[a] = min(hi, (int)floorf(pc[a] + cut));
+        const int f = (int)floorf(pc[a]);
+        empty = empty || lo_c[a] > hi_c[a];
+        bw |= ((uint32_t)(f - lo_c[a] + 1) & 31u) << (10 * a);
+        bw |= ((uint32_t)(hi_c[a] - f) & 31u) << (10 * a + 5);
+    }
+    return empty ? 0x80000000u : bw;
+}
+
 __device__ __forceinline__ uint32_t gather_box_word(const float4& q, float R, int sc, int scs, float cut, int (&lo_c)[3],
                                                     int (&hi_c)[3]) {
     const float pw[3] = {q.x, q.y, q.z};
@@ -354,7 +376,8 @@ constexpr int CLS_UNROLL = 4;   // chunks of 64 entries loaded at once
 // point's bin is the level-0 cell holding it (cells of side 2^logr0), the
 // same floor the binning takes, so its faces come from the point itself.
 __device__ __forceinline__ void class_load(const float4* sorted, int j0, int end, int logr0, float R, int sc, int scs,
-                                           float cut, float4 (&q)[CLS_UNROLL], int (&cls)[CLS_UNROLL], uint32_t& bb) {
+                                           float cut, const Quant16& qz, float4 (&q)[CLS_UNROLL],
+                                           int (&cls)[CLS_UNROLL], uint32_t& bb) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int u = 0; u < CLS_UNROLL; u++) {
@@ -369,8 +392,18 @@ __device__ __forceinline__ void class_load(const float4* sorted, int j0, int end
     for (int u = 0; u < CLS_UNROLL; u++) {
         if (cls[u] < 0) continue;
         int lo_c[3], hi_c[3];
-        const uint32_t bw = gather_box_word(q[u], R, sc, scs, cut, lo_c, hi_c);
         const float pw[3] = {q[u].x, q[u].y, q[u].z};
+        uint32_t bw;
+        if (qz.on) {
+            // FP16 lists: membership from the inserted point, distances from its round trip
+            const float pc[3] = {quant16_round_trip(pw[0], qz.mn[0], qz.rg[0]),
+                                 quant16_round_trip(pw[1], qz.mn[1], qz.rg[1]),
+                                 quant16_round_trip(pw[2], qz.mn[2], qz.rg[2])};
+            bw = gather_box_word2(pw, pc, R, sc, scs, cut, lo_c, hi_c);
+            q[u].x = pc[0], q[u].y = pc[1], q[u].z = pc[2];
+        } else {
+            bw = gather_box_word(q[u], R, sc, scs, cut, lo_c, hi_c);
+        }
         q[u].w = __uint_as_float(bw);
         if (!(bw >> 31)) {
 #pragma unroll
@@ -432,9 +465,9 @@ __device__ __forceinline__ uint32_t wave_or3(uint32_t bb) {
 constexpr int CLS_SPAN = 64 * CLS_UNROLL;
 
 __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int logr0,
-                                                       float R, int sc, int scs, float cut, float4* csorted,
-                                                       int32_t* coff, uint8_t* both, int32_t* ctr, int2* units,
-                                                       int4* bigs) {
+                                                       float R, int sc, int scs, float cut, Quant16 qz,
+                                                       float4* csorted, int32_t* coff, uint8_t* both, int32_t* ctr,
+                                                       int2* units, int4* bigs) {
     const int lane = threadIdx.x;
     const int64_t nbin = gb.count();
     for (int64_t ci = blockIdx.x; ci < nbin; ci += gridDim.x) {
@@ -457,7 +490,7 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = 0;
         if (end > beg) {
-            class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
+            class_load(sorted, beg, end, logr0, R, sc, scs, cut, qz, q, cls, bb);
             class_count(cls, class_chunks(beg, end), base);
         }
         int acc = beg;
@@ -487,7 +520,7 @@ __device__ __forceinline__ int64_t unit_span(const int2* units, const int32_t* o
 // Per unit (one wave each): class counts to ucnt[9 ui + k], both-face axes
 // to ucnt[9 ui + 8].
 __global__ __launch_bounds__(64) void k_class_count(const float4* sorted, const int32_t* off, int logr0,
-                                                    float R, int sc, int scs, float cut, const int32_t* ctr,
+                                                    float R, int sc, int scs, float cut, Quant16 qz, const int32_t* ctr,
                                                     const int2* units, int32_t* ucnt) {
     const int lane = threadIdx.x;
     const int nunit = ctr[0];
@@ -497,7 +530,7 @@ __global__ __launch_bounds__(64) void k_class_count(const float4* sorted, const 
         float4 q[CLS_UNROLL];
         int cls[CLS_UNROLL];
         uint32_t bb = 0;
-        class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
+        class_load(sorted, beg, end, logr0, R, sc, scs, cut, qz, q, cls, bb);
         int cnt[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) cnt[k] = 0;
@@ -567,7 +600,7 @@ __global__ __launch_bounds__(64) void k_class_scan(const int32_t* off, const int
 
 // Per unit (one wave each): its entries placed from the positions of k_class_scan.
 __global__ __launch_bounds__(64) void k_class_place(const float4* sorted, const int32_t* off, int logr0,
-                                                    float R, int sc, int scs, float cut, const int32_t* ctr,
+                                                    float R, int sc, int scs, float cut, Quant16 qz, const int32_t* ctr,
                                                     const int2* units, const int32_t* ucnt, float4* csorted) {
     const int nunit = ctr[0];
     for (int ui = blockIdx.x; ui < nunit; ui += gridDim.x) {
@@ -576,7 +609,7 @@ __global__ __launch_bounds__(64) void k_class_place(const float4* sorted, const 
         float4 q[CLS_UNROLL];
         int cls[CLS_UNROLL];
         uint32_t bb = 0;
-        class_load(sorted, beg, end, logr0, R, sc, scs, cut, q, cls, bb);
+        class_load(sorted, beg, end, logr0, R, sc, scs, cut, qz, q, cls, bb);
         int base[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = ucnt[(int64_t)ui * 9 + k];
@@ -1787,19 +1820,21 @@ void launch_gather(GatherParams gp, hipStream_t s) {
 
 void launch_gather_classes(const float4* sorted, const int32_t* off, CellGrid gb, int res0, float box_radius, int sc,
                            int sc_shift, float cutoff, float4* csorted, int32_t* coff, uint8_t* both, int32_t* ctr,
-                           int2* units, int64_t unit_cap, int4* bigs, int64_t big_cap, int32_t* ucnt, hipStream_t s) {
+                           int2* units, int64_t unit_cap, int4* bigs, int64_t big_cap, int32_t* ucnt, hipStream_t s,
+                           const Quant16* qz) {
+    const Quant16 q = qz ? *qz : Quant16{};
     const int64_t nbin = gb.count();
     if (nbin <= 0) return;
     launch_fill_i32(ctr, 2, 0, s);
     const int logr0 = __builtin_ctz((unsigned)res0);
     hipLaunchKernelGGL(k_gather_classes, dim3((unsigned)std::min<int64_t>(nbin, 1 << 20)), dim3(64), 0, s, sorted, off,
-                       gb, logr0, box_radius, sc, sc_shift, cutoff, csorted, coff, both, ctr, units, bigs);
+                       gb, logr0, box_radius, sc, sc_shift, cutoff, q, csorted, coff, both, ctr, units, bigs);
     const dim3 gu((unsigned)std::max<int64_t>(1, std::min<int64_t>(unit_cap, 8192)));   // persistent: most bins are small
-    hipLaunchKernelGGL(k_class_count, gu, dim3(64), 0, s, sorted, off, logr0, box_radius, sc, sc_shift, cutoff, ctr,
+    hipLaunchKernelGGL(k_class_count, gu, dim3(64), 0, s, sorted, off, logr0, box_radius, sc, sc_shift, cutoff, q, ctr,
                        units, ucnt);
     hipLaunchKernelGGL(k_class_scan, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(big_cap, 2048))), dim3(64),
                        0, s, off, ctr, bigs, ucnt, coff, both);
-    hipLaunchKernelGGL(k_class_place, gu, dim3(64), 0, s, sorted, off, logr0, box_radius, sc, sc_shift, cutoff, ctr,
+    hipLaunchKernelGGL(k_class_place, gu, dim3(64), 0, s, sorted, off, logr0, box_radius, sc, sc_shift, cutoff, q, ctr,
                        units, ucnt, csorted);
 }
 

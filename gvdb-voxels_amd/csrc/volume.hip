@@ -159,6 +159,8 @@ struct gvdbx_volume {
     float sc_radius = 0.0f;
     int64_t sc_len = 0;
     bool sc_valid = false;
+    bool sc_fp16 = false;   // lists of InsertPointsSubcell_FP16 (positions as 16-bit fractions of sc_q's box)
+    Quant16 sc_q = {};
     // InsertPoints output
     DevBuf pnode, poff;
     int64_t npnode = 0;
@@ -925,9 +927,15 @@ int gather_color(gvdbx_volume* v, int chan_clr, int accum) {
 // other scratch is used (no reach classes, no split tiles), so the call
 // allocates nothing; the result is the same bit for bit.
 int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int chan, int chan_clr, int accum,
-           bool noalloc = false) {
+           bool noalloc = false, bool fp16 = false) {
     const char* what = mode ? "GatherLevelSet" : "GatherDensity";
     if (noalloc) what = "GatherLevelSetNoAlloc";
+    if (fp16) what = "GatherLevelSet_FP16";
+    if (v->sc_valid && v->sc_fp16 != fp16)
+        FAIL(GVDBX_ERR_STATE, "%s: the subcell lists were made by InsertPointsSubcell%s", what, v->sc_fp16 ? "_FP16" : "");
+    if (fp16 && v->clr)
+        FAIL(GVDBX_ERR_ARG, "%s: colours are not defined for FP16 lists (the reference sums an uninitialised colour)",
+             what);
     TRY(need_atlas(v, what));
     TRY(check_chan_type(v, chan, GVDBX_FLOAT, what));
     // point colours make the gather average them into chan_clr
@@ -959,8 +967,10 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     // starts at 0 and a member contributes r - c.  The margin covers FP32
     // rounding of the distance (relative error < 1e-6).
     gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
-    gp.variant = noalloc ? 1 : v->gather_variant;
+    gp.variant = (noalloc || fp16) ? 1 : v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
+    if (fp16 && (int)std::ceil(gp.box_radius) + sc > 28)
+        FAIL(GVDBX_ERR_RANGE, "%s: subcell radius + subcell size beyond 28 voxels", what);
     if (gp.variant == 1 && (int)std::ceil(gp.box_radius) + sc <= 29 && v->bins_n > 0) {
         // each point's clipped ranges once, not once per tile that reads it
         if (noalloc) {
@@ -972,7 +982,7 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
             launch_gather_box(gp.sorted, gp.off + gp.gb.count(), v->bins_n, gp.box_radius, sc, gp.sc_shift, gp.cutoff,
                               box, v->stream);
             gp.box = box;
-        } else if (v->gather_classes) {
+        } else if (v->gather_classes || fp16) {
             // and each bin's entries grouped by the neighbour cells they can
             // reach, so a tile reads only the groups that can reach it
             const int64_t nbin = gp.gb.count();
@@ -989,7 +999,7 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
             launch_gather_classes(gp.sorted, gp.off, gp.gb, gp.brick_res - 2 * gp.apron, gp.box_radius, sc, gp.sc_shift,
                                   gp.cutoff, v->gcls_pts.as<float4>(), v->gcoff.as<int32_t>(), v->gcboth.as<uint8_t>(),
                                   v->gcctr.as<int32_t>(), v->gcunit.as<int2>(), unit_cap, v->gcbig.as<int4>(), big_cap,
-                                  v->gcucnt.as<int32_t>(), v->stream);
+                                  v->gcucnt.as<int32_t>(), v->stream, fp16 ? &v->sc_q : nullptr);
             gp.sorted = v->gcls_pts.as<float4>();
             gp.coff = v->gcoff.as<int32_t>();
             gp.cboth = v->gcboth.as<uint8_t>();
@@ -1450,15 +1460,23 @@ int gvdbx_update_atlas(gvdbx_volume* v) {
     return update_atlas(v);
 }
 
-int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
-                                int64_t* out_len) {
-    TRY(check_vol(v));
-    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "InsertPointsSubcell: not available for batched clouds");
-    TRY(need_atlas(v, "InsertPointsSubcell"));
-    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "InsertPointsSubcell: negative point count");
-    if (sc <= 0 || v->res[0] % sc) FAIL(GVDBX_ERR_ARG, "InsertPointsSubcell: subcell size must divide the brick");
+}  // extern "C"
+
+namespace {
+
+// InsertPointsSubcell (VG.cpp:4971-5092) and InsertPointsSubcell_FP16
+// (VG.cpp:5094-5189): the lists are the bins plus each point's subcell box,
+// so only their length is computed here; fp16 records the box the FP16
+// positions are fractions of (GetBoundingBox's pos_min and its range).
+int insert_subcell(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                   int64_t* out_len, bool fp16) {
+    const char* what = fp16 ? "InsertPointsSubcell_FP16" : "InsertPointsSubcell";
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "%s: not available for batched clouds", what);
+    TRY(need_atlas(v, what));
+    if (num_pnts < 0) FAIL(GVDBX_ERR_ARG, "%s: negative point count", what);
+    if (sc <= 0 || v->res[0] % sc) FAIL(GVDBX_ERR_ARG, "%s: subcell size must divide the brick", what);
     if (!(radius >= 0.0f) || radius + (float)sc > (float)v->res[0])
-        FAIL(GVDBX_ERR_RANGE, "InsertPointsSubcell: radius %g + subcell %d exceeds the brick (%d)", radius, sc, v->res[0]);
+        FAIL(GVDBX_ERR_RANGE, "%s: radius %g + subcell %d exceeds the brick (%d)", what, radius, sc, v->res[0]);
     float t[3] = {tx, ty, tz};
     int64_t len = 0;
     TRY(ensure_bins(v, num_pnts, t, false));
@@ -1467,8 +1485,40 @@ int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float rad
     v->sc_radius = radius;
     v->sc_len = len;
     v->sc_valid = true;
+    v->sc_fp16 = fp16;
+    v->sc_q = Quant16{};
+    if (fp16) {
+        v->sc_q.on = 1;
+        for (int a = 0; a < 3; a++) {
+            v->sc_q.mn[a] = v->pos_min[a];
+            v->sc_q.rg[a] = v->pos_max[a] - v->pos_min[a];
+        }
+    }
     if (out_len) *out_len = len;
     return GVDBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gvdbx_insert_points_subcell(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                                int64_t* out_len) {
+    TRY(check_vol(v));
+    return insert_subcell(v, sc, num_pnts, radius, tx, ty, tz, out_len, false);
+}
+
+int gvdbx_insert_points_subcell_fp16(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty,
+                                     float tz, int64_t* out_len) {
+    TRY(check_vol(v));
+    return insert_subcell(v, sc, num_pnts, radius, tx, ty, tz, out_len, true);
+}
+
+int gvdbx_gather_level_set_fp16(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,
+                                int64_t sc_len, int chan_levelset, int chan_clr) {
+    (void)num_pnts; (void)tx; (void)ty; (void)tz;
+    TRY(check_vol(v));
+    return gather(v, 1, sc, radius, sc_len, chan_levelset, chan_clr, 0, false, true);
 }
 
 int gvdbx_gather_density(gvdbx_volume* v, int sc, int num_pnts, float radius, float tx, float ty, float tz,

@@ -200,6 +200,16 @@ class VolumeGVDB:
              *[float(x) for x in trans], C.byref(out))
         return out.value
 
+    def InsertPointsSubcell_FP16(self, subcell_size, num_pnts, radius, trans=(0.0, 0.0, 0.0)):
+        out = C.c_int64(0)
+        call("gvdbx_insert_points_subcell_fp16", self._h, int(subcell_size), int(num_pnts), float(radius),
+             *[float(x) for x in trans], C.byref(out))
+        return out.value
+
+    def GatherLevelSet_FP16(self, subcell_size, num_pnts, radius, trans, sc_len, chan_levelset=0, chan_clr=-1):
+        call("gvdbx_gather_level_set_fp16", self._h, int(subcell_size), int(num_pnts), float(radius),
+             *[float(x) for x in trans], int(sc_len), int(chan_levelset), int(chan_clr))
+
     def GatherDensity(self, subcell_size, num_pnts, radius, trans, sc_len, chan_density=0, chan_clr=-1, accum=False):
         call("gvdbx_gather_density", self._h, int(subcell_size), int(num_pnts), float(radius),
              *[float(x) for x in trans], int(sc_len), int(chan_density), int(chan_clr), int(accum))

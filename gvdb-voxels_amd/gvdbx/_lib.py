@@ -65,6 +65,8 @@ _SIGS = {
     "gvdbx_gather_density": [_vp, _i, _i, _f, _f, _f, _f, _i64, _i, _i, _i],
     "gvdbx_gather_level_set": [_vp, _i, _i, _f, _f, _f, _f, _i64, _i, _i, _i],
     "gvdbx_gather_level_set_noalloc": [_vp, _i, _i, _f, _f, _f, _f, _i64, _i, _i, _i],
+    "gvdbx_insert_points_subcell_fp16": [_vp, _i, _i, _f, _f, _f, _f, C.POINTER(C.c_int64)],
+    "gvdbx_gather_level_set_fp16": [_vp, _i, _i, _f, _f, _f, _f, _i64, _i, _i],
     "gvdbx_insert_points": [_vp, _i, _f, _f, _f, _i],
     "gvdbx_scatter_density": [_vp, _i, _f, _f, _f, _f, _f, _i, _i],
     "gvdbx_splat_cubic": [_vp, _i, _f, _f, _f, _f, _i, _i],

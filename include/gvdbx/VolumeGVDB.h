@@ -131,6 +131,19 @@ public:
                                           &pSCPntsLength),
               "InsertPointsSubcell");
     }
+    void InsertPointsSubcell_FP16(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength) {
+        int64_t len = 0;
+        check(gvdbx_insert_points_subcell_fp16(v_, subcell_size, num_pnts, radius, trans.x, trans.y, trans.z, &len),
+              "InsertPointsSubcell_FP16");
+        if (len > INT_MAX) throw Error(GVDBX_ERR_RANGE, "InsertPointsSubcell_FP16: list length beyond int");
+        pSCPntsLength = (int)len;
+    }
+    void GatherLevelSet_FP16(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
+                             int chanLevelset, int chanClr) {
+        check(gvdbx_gather_level_set_fp16(v_, subcell_size, num_pnts, radius, trans.x, trans.y, trans.z,
+                                          pSCPntsLength, chanLevelset, chanClr),
+              "GatherLevelSet_FP16");
+    }
     void GatherDensity(int subcell_size, int num_pnts, float radius, Vector3DF trans, int& pSCPntsLength,
                        int chanDensity, int chanClr, bool bAccum = false) {
         check(gvdbx_gather_density(v_, subcell_size, num_pnts, radius, trans.x, trans.y, trans.z, pSCPntsLength,

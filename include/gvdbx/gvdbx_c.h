@@ -172,6 +172,24 @@ int gvdbx_gather_level_set(gvdbx_volume* v, int subcell_size, int num_pnts, floa
 int gvdbx_gather_level_set_noalloc(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
                                    float tx, float ty, float tz, int64_t sc_len, int chan_levelset, int chan_clr,
                                    int accum);
+/* InsertPointsSubcell_FP16 VG.h:552 / VG.cpp:5094-5189: the same lists
+ * (and list length) as gvdbx_insert_points_subcell, with every listed
+ * position kept as three 16-bit fractions of the bounding box of the last
+ * RebuildTopology (GetBoundingBox's pos_min and pos_max - pos_min):
+ *   q = ushort(((w - pos_min) / range) * 65535)   (truncated, clamped)
+ * (gvdbInsertSubcell_fp16, cuda_gvdb_particles.cuh:154-216).  Membership is
+ * decided from the full-precision position, as the reference inserts. */
+int gvdbx_insert_points_subcell_fp16(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
+                                     float tx, float ty, float tz, int64_t* out_len);
+/* GatherLevelSet_FP16 VG.h:558 / VG.cpp:5375-5401
+ * (gvdbGatherLevelSet_fp16, cuda_gvdb_particles.cuh:775-811): the level
+ * set over FP16 lists, each position read back as
+ * (q / 65535) * range + pos_min (FP32, in that order), starting from 3.0
+ * (no accumulate).  Needs gvdbx_insert_points_subcell_fp16 and subcell
+ * radius + subcell size <= 28; point colours are rejected (the reference
+ * sums them into an uninitialised colour). */
+int gvdbx_gather_level_set_fp16(gvdbx_volume* v, int subcell_size, int num_pnts, float radius,
+                                float tx, float ty, float tz, int64_t sc_len, int chan_levelset, int chan_clr);
 /* InsertPoints(int, Vector3DF, bool) VG.h:542 / VG.cpp:5219-5267 */
 int gvdbx_insert_points(gvdbx_volume* v, int num_pnts, float tx, float ty, float tz, int prefix);
 /* ScatterDensity(int, float, float, Vector3DF, bool, bool) VG.h:555 / VG.cpp:5624-5660.

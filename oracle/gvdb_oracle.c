@@ -683,8 +683,35 @@ static int64_t atlas_index(const or_volume* v, int64_t x, int64_t y, int64_t z) 
 /* points -> subcells -> gather                                        */
 /* ------------------------------------------------------------------ */
 
+/* FP16 list entry of gvdbInsertSubcell_fp16 (PT.cuh:206-209) read back as
+ * gvdbGatherLevelSet_fp16 does (PT.cuh:794-796): ushort((w - mn) / rg *
+ * 65535) (truncated, clamped to [0, 65535], NaN -> 0), then q / 65535 * rg +
+ * mn; FP32 throughout, in that order. */
+static float fp16_round_trip(float w, float mn, float rg) {
+    float t = ((w - mn) / rg) * 65535.0f;
+    uint32_t q = !(t > 0.0f) ? 0u : (t >= 65535.0f ? 65535u : (uint32_t)t);
+    return ((float)q / 65535.0f) * rg + mn;
+}
+
+static int insert_sc_impl(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride, float radius,
+                          const float trans[3], int64_t* out_len, int fp16);
+
 int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride,
                              float radius, const float trans[3], int64_t* out_len) {
+    return insert_sc_impl(v, sc, pos, n, stride, radius, trans, out_len, 0);
+}
+
+/* InsertPointsSubcell_FP16 VG.cpp:5094-5189: the same lists, positions kept
+ * as 16-bit fractions of GetBoundingBox's box (mPosMin, mPosRange =
+ * mPosMax - mPosMin, VG.cpp:5452-5462); or_gather then reads them back, which
+ * with mode 1 and no accumulate is GatherLevelSet_FP16 (PT.cuh:775-811). */
+int or_insert_points_subcell_fp16(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride,
+                                  float radius, const float trans[3], int64_t* out_len) {
+    return insert_sc_impl(v, sc, pos, n, stride, radius, trans, out_len, 1);
+}
+
+static int insert_sc_impl(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride, float radius,
+                          const float trans[3], int64_t* out_len, int fp16) {
     if (!v || !v->configured || sc <= 0 || (!pos && n > 0)) return OR_ERR_ARG;
     if (stride == 0) stride = 12;
     int64_t nb = v->ncnt[0];
@@ -741,9 +768,9 @@ int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, 
                         if (sidx < 0 || sidx >= nsc) continue;
                         if (pass == 1) {
                             int64_t e = v->sc_off[sidx] + v->sc_cnt[sidx];
-                            v->sc_pnt[e * 3 + 0] = w[0];
-                            v->sc_pnt[e * 3 + 1] = w[1];
-                            v->sc_pnt[e * 3 + 2] = w[2];
+                            for (int a = 0; a < 3; a++)
+                                v->sc_pnt[e * 3 + a] =
+                                    fp16 ? fp16_round_trip(w[a], v->pos_min[a], v->pos_max[a] - v->pos_min[a]) : w[a];
                             if (v->sc_pidx) v->sc_pidx[e] = i;
                         }
                         v->sc_cnt[sidx]++;

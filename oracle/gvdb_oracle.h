@@ -85,6 +85,11 @@ int or_update_atlas(or_volume* v);
 /* InsertPointsSubcell VG.cpp:4971-5092; returns the list length in *out_len */
 int or_insert_points_subcell(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride_bytes,
                              float radius, const float trans[3], int64_t* out_len);
+/* InsertPointsSubcell_FP16 VG.cpp:5094-5189: positions stored as 16-bit
+ * fractions of the bounding box (read back by or_gather: GatherLevelSet_FP16
+ * is or_gather mode 1 without accumulate) */
+int or_insert_points_subcell_fp16(or_volume* v, int sc, const float* pos, int64_t n, int64_t stride_bytes,
+                                  float radius, const float trans[3], int64_t* out_len);
 /* GatherDensity VG.cpp:5318-5344 / PT.cuh:696-733 (mode 0)
  * GatherLevelSet VG.cpp:5348-5373 / PT.cuh:736-773 (mode 1) */
 int or_gather(or_volume* v, int mode, float radius, int chan, int accum);

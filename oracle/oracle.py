@@ -111,6 +111,7 @@ def lib():
         L.or_rebuild_topology.argtypes = [vp, vp, i64, i64, f32, fp]
         L.or_update_atlas.argtypes = [vp]
         L.or_insert_points_subcell.argtypes = [vp, i32, vp, i64, i64, f32, fp, C.POINTER(C.c_int64)]
+        L.or_insert_points_subcell_fp16.argtypes = [vp, i32, vp, i64, i64, f32, fp, C.POINTER(C.c_int64)]
         L.or_gather.argtypes = [vp, i32, f32, i32, i32]
         L.or_gather_f64.argtypes = [vp, i32, f32, i32, dp]
         L.or_update_apron.argtypes = [vp, i32, f32]
@@ -203,6 +204,13 @@ class OracleVolume:
         out = C.c_int64(0)
         _chk(self._L.or_insert_points_subcell(self._h, sc, p.ctypes.data, p.shape[0], 12, float(radius),
                                               _f3(trans), C.byref(out)), "insert_points_subcell")
+        return out.value
+
+    def insert_points_subcell_fp16(self, sc, points, radius, trans=(0, 0, 0)):
+        p = _pts(points)
+        out = C.c_int64(0)
+        _chk(self._L.or_insert_points_subcell_fp16(self._h, sc, p.ctypes.data, p.shape[0], 12, float(radius),
+                                                   _f3(trans), C.byref(out)), "insert_points_subcell_fp16")
         return out.value
 
     def gather_density(self, radius, chan=0, accum=False):

@@ -477,3 +477,46 @@ def test_gather_variants_bit_identical(mode):
         if acc_ref is None:
             acc_ref = got
         np.testing.assert_array_equal(got, acc_ref, err_msg=f"accumulate, variant {var}")
+
+
+@pytest.mark.parametrize("log2dims,sc,R,split", [((3, 3, 3, 3, 5), 4, 3.0, 16), ((3, 3, 3, 3, 5), 2, 2.0, 1),
+                                                 ((4, 3, 3, 3, 5), 4, 3.0, 0)])
+def test_fp16_subcell_parity(log2dims, sc, R, split):
+    """InsertPointsSubcell_FP16 + GatherLevelSet_FP16 (positions read back
+    from 16-bit fractions of the bounding box, membership from the inserted
+    points) equal the oracle bit for bit, with and without split tiles; the
+    FP16 and FP32 paths refuse each other's lists, and colours."""
+    G = _gvdbx()
+    _log(f"[gpu] fp16 subcells {log2dims} sc={sc} split={split}")
+    rng = np.random.default_rng(808 + sc)
+    a = rng.normal([40, 40, 40], 4.0, (30000, 3))
+    b = rng.random((20000, 3)) * [110, 70, 90] + [10, 8, 12]
+    pts = np.abs(np.concatenate([a, b])).astype(np.float32)
+    gv, ov, tp = build_pair(pts, R, log2dims)
+    gv.SetTuning("gather_split", split)
+    gl = gv.InsertPointsSubcell_FP16(sc, len(pts), R)
+    assert gl == ov.insert_points_subcell_fp16(sc, pts, R)
+    with pytest.raises(G.GvdbxError) as e:
+        gv.GatherLevelSet(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
+    assert e.value.code == 2
+    gv.FillChannel(0, -7.0)
+    gv.GatherLevelSet_FP16(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
+    ov.gather_level_set(1.5)
+    r0 = 1 << log2dims[0]
+    inner = (slice(None), slice(1, r0 + 1), slice(1, r0 + 1), slice(1, r0 + 1))
+    np.testing.assert_array_equal(gv.bricks()[inner], ov.bricks()[inner])
+    gv.UpdateApron(0, 3.0)
+    ov.update_apron(0, 3.0)
+    np.testing.assert_array_equal(gv.bricks(), ov.bricks())
+    gl32 = gv.InsertPointsSubcell(sc, len(pts), R)
+    assert gl32 == gl
+    with pytest.raises(G.GvdbxError) as e:
+        gv.GatherLevelSet_FP16(sc, len(pts), 1.5, (0, 0, 0), gl, 0)
+    assert e.value.code == 2
+    gv.AddChannel(1, G.T_UCHAR4, 1)
+    tc = torch.zeros((len(pts), 4), dtype=torch.uint8, device=tp.device)
+    gv.SetPoints(tp, None, tc)
+    gv.InsertPointsSubcell_FP16(sc, len(pts), R)
+    with pytest.raises(G.GvdbxError) as e:
+        gv.GatherLevelSet_FP16(sc, len(pts), 1.5, (0, 0, 0), gl, 0, 1)
+    assert e.value.code == 1
