@@ -47,6 +47,8 @@ struct NodeBuildParams {
     int is_leaf;
     GNode* nodes;
     int32_t* idx;        // cell -> node index or -1
+    int64_t node_off;    // pool index of the first node made (appending to a pool)
+    int keep_idx;        // 1: idx is written only where a node is made
 };
 // flag scan apply of up to GVDBX_MAXLEV levels in one launch (tile_start is filled in)
 struct FlagApplyLevel {
@@ -84,6 +86,7 @@ struct LinkParams {
     CellGrid gp;          // grid of level lev+1 (unused for the root)
     const int32_t* idx_p;
     int64_t nparent;
+    int need_flag;        // 1: only nodes with mFlags set are linked (gvdbLinkBricks)
 };
 // the links of up to GVDBX_MAXLEV levels: level i links children [start[i], start[i+1])
 struct LinkBatch {
@@ -464,3 +467,47 @@ void launch_leaf_clouds(const GNode* leaves, int64_t nleaf, CellGrid g0, int log
 // per-cloud leaf bounds (6 ints per cloud, packed voxels): min mPos, max mPos + range0
 void launch_leaf_bounds_clouds(const GNode* leaves, int64_t nleaf, int range0, const int32_t* leaf_cloud,
                                int32_t* out6, hipStream_t s);
+
+// ---------------- incremental topology (ActivateIncreBricksGPU) ----------------
+// old -> new grid: dst[c] = src[c] where the old grid holds cell c, else -1
+void launch_remap_idx(CellGrid from, const int32_t* src, CellGrid to, int32_t* dst, hipStream_t s);
+struct IncreMark {
+    int depth;
+    int64_t range[GVDBX_MAXLEV];
+    int root_cell[GVDBX_MAXLEV][3];   // the root's first cell of each level
+    int root_span[GVDBX_MAXLEV];      // cells of each level across the root
+    CellGrid g[GVDBX_MAXLEV];         // grids that hold every cell the update can touch
+    const int32_t* idx[GVDBX_MAXLEV]; // cell -> reachable node before the update, or -1
+    uint8_t* newf[GVDBX_MAXLEV];      // out: 1 where a touched cell has no node
+    uint8_t* marker0;                 // out: 1 for every reached leaf
+    int64_t nleaf;
+};
+void launch_incre_mark(PointSrc src, int64_t n, float tx, float ty, float tz, float radius, const IncreMark& im,
+                       hipStream_t s);
+// leaf k < n: mFlags = (accum ? mFlags : 0) | marker[k]
+void launch_incre_leaf_flags(GNode* leaves, int64_t n, const uint8_t* marker, int accum, hipStream_t s);
+struct DelinkParams {
+    int lev;
+    GNode* nodes;
+    int64_t n;
+    const uint64_t* own_clist;   // lists of level lev (lev >= 1)
+    int64_t own_len;             // res(lev)^3
+    int parent_is_root;
+    CellGrid gp;                 // grid of level lev+1
+    const int32_t* idx_p;        // its reachable map before the update
+    GNode* parent_nodes;
+    uint64_t* parent_clist;
+    int res_p;
+    int64_t range_p;
+    int64_t nparent;
+};
+void launch_incre_delink(const DelinkParams& dp, hipStream_t s);
+// idx[c] = -1 where its node was delinked (mParent undefined)
+void launch_incre_final_idx(CellGrid g, int32_t* idx, const GNode* nodes, hipStream_t s);
+// flags[k] = leaves[k].mFlags != 0
+void launch_leaf_flags32(const GNode* leaves, int64_t n, int32_t* flags, hipStream_t s);
+// UpdateAtlas over active leaves: active leaf k takes atlas slot rank[k]
+// (mValue, atlas map entry); inactive leaves keep their mValue; map slots
+// no leaf takes are undefined
+void launch_atlas_assign_rank(GNode* leaves, int64_t nleaf, const int32_t* rank, int sx, int sy, int brick_res,
+                              int apron, GAtlasNode* amap, int64_t nmap, hipStream_t s);

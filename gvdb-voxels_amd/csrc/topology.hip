@@ -133,9 +133,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_flag_apply(FlagApplyBatch fb) 
         int64_t i = t0 + k;
         if (i < n) {
             int32_t node = -1;
-            if (v[k].a) node = (int32_t)run.a;
-            else if (v[k].b) node = (int32_t)(nbase + run.b);
-            np.idx[i] = node;
+            if (v[k].a) node = (int32_t)(np.node_off + run.a);
+            else if (v[k].b) node = (int32_t)(np.node_off + nbase + run.b);
+            if (!np.keep_idx || node >= 0) np.idx[i] = node;
             if (node >= 0) {
                 int64_t yz = (int64_t)g.ny * g.nz;
                 int cx = (int)(i / yz) + g.ox;
@@ -382,6 +382,7 @@ __global__ __launch_bounds__(256) void k_fill(T* p, int64_t n, T v) {
 // ---- link (gvdbLinkBricks cuda_gvdb_nodes.cuh:470-505 for levels below top_lev) ----
 __device__ __forceinline__ void link_one(const LinkParams& lp, int64_t k) {
     const GNode& c = lp.child[k];
+    if (lp.need_flag && !c.mFlags) return;
     int64_t p = 0;
     if (!lp.parent_is_root) {
         int pc[3];
@@ -453,6 +454,14 @@ __global__ __launch_bounds__(256) void k_atlas_assign(GNode* leaves, int64_t nle
 __global__ __launch_bounds__(256) void k_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6) {
     int v[6] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nleaf; k += (int64_t)gridDim.x * blockDim.x) {
+        if (k == 0) {
+            // ComputeBounds seeds both corners with leaf 0's position, active or not
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                v[a] = min(v[a], leaves[0].mPos[a]);
+                v[3 + a] = max(v[3 + a], leaves[0].mPos[a]);
+            }
+        }
         if (!leaves[k].mFlags) continue;
 #pragma unroll
         for (int a = 0; a < 3; a++) {

@@ -125,6 +125,11 @@ struct gvdbx_volume {
     DevBuf nodes[GVDBX_MAXLEV + 1], clist[GVDBX_MAXLEV + 1], idx[GVDBX_MAXLEV], flag_base[GVDBX_MAXLEV],
         flag_all[GVDBX_MAXLEV];
     DevBuf reach0, nbmask, partials, totals, bbox, misc;
+    // incremental updates (ActivateIncreBricksGPU): the next maps, leaf markers, atlas ranks
+    DevBuf idx2[GVDBX_MAXLEV], inc_mark, atl_flag, atl_rank;
+    bool leaf_inactive = false;   // an incremental update may have deactivated leaves
+    bool incre_rebuild = false;   // RequestFullRebuild(false): RebuildTopology updates incrementally
+    bool force_full = false;      // RequestFullRebuild(true): the next topology call rebuilds fully
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int gather_split = GATHER_SPLIT_DEFAULT;
@@ -221,6 +226,30 @@ int ensure(gvdbx_volume* v, DevBuf& b, size_t bytes) {
         (void)hipGetLastError();
         FAIL(GVDBX_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
     }
+    b.cap = want;
+    return GVDBX_OK;
+}
+
+// ensure() that keeps the first `keep` bytes when the buffer grows
+int ensure_keep(gvdbx_volume* v, DevBuf& b, size_t bytes, size_t keep) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return GVDBX_OK;
+    TRY(sync(v));
+    size_t want = bytes + bytes / 4 + 256;
+    void* np = nullptr;
+    hipError_t e = hipMalloc(&np, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        FAIL(GVDBX_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+    }
+    if (b.p && keep > 0) e = hipMemcpyAsync(np, b.p, std::min(keep, b.cap), hipMemcpyDeviceToDevice, v->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(np);
+        FAIL(GVDBX_ERR_HIP, "pool growth copy failed: %s", hipGetErrorString(e));
+    }
+    if (b.p) (void)hipFree(b.p);
+    b.p = np;
     b.cap = want;
     return GVDBX_OK;
 }
@@ -829,6 +858,214 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     v->has_atlas = false;
     v->sc_valid = false;
     v->topo_mode_used = GVDBX_TOPO_DENSE;
+    v->leaf_inactive = false;
+    v->force_full = false;
+    return GVDBX_OK;
+}
+
+// ActivateIncreBricksGPU (VG.cpp:1020-1155) on the current tree.  accum:
+// AccumulateTopology (VG.cpp:1158-1178), existing leaves keep their flags;
+// else the incremental RebuildTopology (VG.cpp:1195-1216), leaves no point
+// reaches are deactivated and delinked.  The tree keeps its depth and root.
+// With `full` (RebuildTopology), *full is set instead of updating when the
+// reference would rebuild (the depth changes) or when a point lies outside
+// the root; AccumulateTopology fails with GVDBX_ERR_RANGE there.
+int incremental_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[3], bool accum, bool* full) {
+    const char* what = accum ? "AccumulateTopology" : "RebuildTopology";
+    if (full) *full = false;
+    if (n < 0) FAIL(GVDBX_ERR_ARG, "%s: negative point count", what);
+    if (!(radius >= 0.0f) || !isfinite(radius)) FAIL(GVDBX_ERR_ARG, "%s: radius must be finite and >= 0", what);
+    if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "%s: SetPoints has not been called", what);
+    if (!v->has_topo || v->cnt[0] == 0) FAIL(GVDBX_ERR_STATE, "%s: no topology to update", what);
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "%s: no incremental update for batched clouds", what);
+    if (v->top_lev != v->depth) FAIL(GVDBX_ERR_STATE, "%s: the tree's top level is below its root", what);
+    hipStream_t s = v->stream;
+    const int depth = v->depth;
+    const int64_t rtop = v->range[depth];
+    // GetBoundingBox (VG.cpp:5404-5467)
+    TRY(ensure(v, v->bbox, sizeof(BBoxAccum) * (1 + (size_t)bbox_parts(std::max<int64_t>(n, 1)))));
+    TRY(ensure_host(v, 4096));
+    launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
+    if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), v->bbox.as<BBoxAccum>() + 1, s);
+    TRY(check_launch("bounding box"));
+    GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    BBoxAccum bb;
+    memcpy(&bb, v->host, sizeof(bb));
+    if (bb.nvalid == 0) FAIL(GVDBX_ERR_NOPOINTS, "%s: no valid point", what);
+    float mn[3], mx[3], pmin[3], pmax[3];
+    bool inside = true;
+    for (int a = 0; a < 3; a++) {
+        memcpy(&mn[a], &bb.mn[a], 4);
+        memcpy(&mx[a], &bb.mx[a], 4);
+        pmin[a] = std::max(0.0f, mn[a] - 1);
+        pmax[a] = std::max(0.0f, mx[a] + 2);
+        inside = inside && mn[a] >= (float)v->root_pos[a] && mx[a] < (float)((int64_t)v->root_pos[a] + rtop);
+    }
+    if (full) {
+        int root2[3];
+        if (!inside || determine_depth(v, pmin, pmax, root2) != depth) {
+            *full = true;
+            return GVDBX_OK;
+        }
+    } else if (!inside) {
+        FAIL(GVDBX_ERR_RANGE, "%s: points lie outside the root node at (%d,%d,%d), %lld voxels wide", what,
+             v->root_pos[0], v->root_pos[1], v->root_pos[2], (long long)rtop);
+    }
+    // grids that hold the old ones and every cell a box can touch (the box
+    // corners are monotone in the point coordinate)
+    CellGrid ug[GVDBX_MAXLEV] = {};
+    IncreMark im{};
+    im.depth = depth;
+    for (int l = 0; l < depth; l++) {
+        const int64_t rg = v->range[l];
+        const CellGrid& og = v->grid[l];
+        int lo[3], hi[3];
+        im.range[l] = rg;
+        im.root_span[l] = (int)(rtop / rg);
+        for (int a = 0; a < 3; a++) {
+            im.root_cell[l][a] = (int)(v->root_pos[a] / rg);
+            const int bmin = (int)((float)(int)mn[a] - radius), bmax = (int)((float)(int)mx[a] + radius);
+            lo[a] = std::max((int)(bmin / rg), im.root_cell[l][a]);
+            hi[a] = std::min((int)(bmax / rg), im.root_cell[l][a] + im.root_span[l] - 1);
+        }
+        const int olo[3] = {og.ox, og.oy, og.oz}, ohi[3] = {og.ox + og.nx - 1, og.oy + og.ny - 1, og.oz + og.nz - 1};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], olo[a]);
+            hi[a] = std::max(hi[a], ohi[a]);
+        }
+        ug[l] = CellGrid{lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    }
+    if (ug[0].count() > MAX_DENSE_CELLS) {
+        if (full) {
+            *full = true;
+            return GVDBX_OK;
+        }
+        FAIL(GVDBX_ERR_RANGE, "%s: %lld level-0 cells exceed the dense-grid limit", what, (long long)ug[0].count());
+    }
+    const int64_t nleaf = v->cnt[0];
+    int64_t ptot = 0, poffs[GVDBX_MAXLEV];
+    for (int l = 0; l < depth; l++) {
+        TRY(ensure(v, v->idx2[l], (size_t)ug[l].count() * 4));
+        TRY(ensure(v, v->flag_base[l], (size_t)ug[l].count()));
+        poffs[l] = ptot;
+        ptot += scan_tiles(ug[l].count()) + 1;
+    }
+    TRY(ensure(v, v->inc_mark, (size_t)nleaf));
+    TRY(ensure(v, v->partials, (size_t)(ptot + 2) * sizeof(U2)));
+    TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
+    for (int l = 0; l < depth; l++) {
+        launch_remap_idx(v->grid[l], v->idx[l].as<int32_t>(), ug[l], v->idx2[l].as<int32_t>(), s);
+        launch_fill_u8(v->flag_base[l].as<uint8_t>(), ug[l].count(), 0, s);
+        im.g[l] = ug[l];
+        im.idx[l] = v->idx2[l].as<int32_t>();
+        im.newf[l] = v->flag_base[l].as<uint8_t>();
+    }
+    launch_fill_u8(v->inc_mark.as<uint8_t>(), nleaf, 0, s);
+    im.marker0 = v->inc_mark.as<uint8_t>();
+    im.nleaf = nleaf;
+    launch_incre_mark(v->pos, n, orig[0], orig[1], orig[2], radius, im, s);
+    U2* partials = v->partials.as<U2>();
+    U2* totals = v->totals.as<U2>();
+    for (int l = 0; l < depth; l++)
+        launch_flag_scan_reduce(v->flag_base[l].as<uint8_t>(), v->flag_base[l].as<uint8_t>(), ug[l].count(),
+                                partials + poffs[l], totals + l, s);
+    TRY(check_launch("incremental marks"));
+    GVDBX_CHECK(hipMemcpyAsync(v->host, totals, (size_t)depth * sizeof(U2), hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    U2 tot[GVDBX_MAXLEV];
+    memcpy(tot, v->host, (size_t)depth * sizeof(U2));
+    memcpy(v->pos_min, pmin, sizeof(pmin));
+    memcpy(v->pos_max, pmax, sizeof(pmax));
+    v->pts_valid = (int64_t)bb.nvalid;
+    v->bins_valid = false;   // the bins hold earlier points
+    v->sc_valid = false;
+    int64_t added = 0;
+    for (int l = 0; l < depth; l++) added += tot[l].a;
+    if (added == 0) return GVDBX_OK;   // no missing node: nothing changes (VG.cpp:1066)
+
+    // pools grow by the new nodes, keeping the old records and lists
+    int64_t cnt[GVDBX_MAXLEV];
+    for (int l = 0; l < depth; l++) {
+        cnt[l] = v->cnt[l] + (int64_t)tot[l].a;
+        TRY(ensure_keep(v, v->nodes[l], (size_t)cnt[l] * sizeof(GNode), (size_t)v->cnt[l] * sizeof(GNode)));
+        if (l >= 1) {
+            const int64_t len = (int64_t)v->res[l] * v->res[l] * v->res[l];
+            TRY(ensure_keep(v, v->clist[l], (size_t)(cnt[l] * len) * 8, (size_t)(v->cnt[l] * len) * 8));
+            launch_fill_u64(v->clist[l].as<uint64_t>() + v->cnt[l] * len, (cnt[l] - v->cnt[l]) * len,
+                            GVDBX_ID_UNDEF64, s);
+        }
+    }
+    // leaf flags from the markers (VG.cpp:1078-1085)
+    launch_incre_leaf_flags(v->nodes[0].as<GNode>(), nleaf, v->inc_mark.as<uint8_t>(), accum ? 1 : 0, s);
+    // new nodes after the old ones, in key order, entered in the maps (VG.cpp:1087-1111)
+    FlagApplyBatch fb{};
+    for (int l = 0; l < depth; l++) {
+        NodeBuildParams np{ug[l], l, v->range[l], l == 0 ? 1 : 0, v->nodes[l].as<GNode>(), v->idx2[l].as<int32_t>(),
+                           v->cnt[l], 1};
+        fb.lv[fb.n++] = FlagApplyLevel{v->flag_base[l].as<uint8_t>(), v->flag_base[l].as<uint8_t>(), ug[l].count(),
+                                       partials + poffs[l], totals + l, np};
+    }
+    launch_flag_scan_apply(fb, s);
+    // links of the flagged nodes (gvdbLinkBricks; the maps give what a descent finds)
+    LinkBatch lb{};
+    lb.start[0] = 0;
+    for (int l = depth - 1; l >= 0; l--) {
+        LinkParams lp{};
+        lp.lev = l;
+        lp.nchild = cnt[l];
+        lp.child = v->nodes[l].as<GNode>();
+        lp.childw = v->nodes[l].as<GNode>();
+        lp.parent_nodes = v->nodes[l + 1].as<GNode>();
+        lp.parent_clist = v->clist[l + 1].as<uint64_t>();
+        lp.res_p = v->res[l + 1];
+        lp.range_p = v->range[l + 1];
+        lp.logres_p = v->logdim[l + 1];
+        lp.parent_is_root = (l + 1 == depth) ? 1 : 0;
+        if (l + 1 < depth) {
+            lp.gp = ug[l + 1];
+            lp.idx_p = v->idx2[l + 1].as<int32_t>();
+        }
+        lp.nparent = l + 1 < depth ? cnt[l + 1] : 1;
+        lp.need_flag = 1;
+        lb.lv[lb.n] = lp;
+        lb.start[lb.n + 1] = lb.start[lb.n] + cnt[l];
+        lb.n++;
+    }
+    launch_link(lb, s);
+    // delinks, leaves first, then one inner level after the other (ND.cuh:401-467)
+    for (int l = 0; l < depth; l++) {
+        DelinkParams dp{};
+        dp.lev = l;
+        dp.nodes = v->nodes[l].as<GNode>();
+        dp.n = cnt[l];
+        if (l >= 1) {
+            dp.own_clist = v->clist[l].as<uint64_t>();
+            dp.own_len = (int64_t)v->res[l] * v->res[l] * v->res[l];
+        }
+        dp.parent_is_root = (l + 1 == depth) ? 1 : 0;
+        if (l + 1 < depth) {
+            dp.gp = ug[l + 1];
+            dp.idx_p = v->idx2[l + 1].as<int32_t>();
+        }
+        dp.parent_nodes = v->nodes[l + 1].as<GNode>();
+        dp.parent_clist = v->clist[l + 1].as<uint64_t>();
+        dp.res_p = v->res[l + 1];
+        dp.range_p = v->range[l + 1];
+        dp.nparent = l + 1 < depth ? cnt[l + 1] : 1;
+        launch_incre_delink(dp, s);
+    }
+    for (int l = 0; l < depth; l++) launch_incre_final_idx(ug[l], v->idx2[l].as<int32_t>(), v->nodes[l].as<GNode>(), s);
+    TRY(check_launch("incremental topology"));
+    for (int l = 0; l < depth; l++) {
+        std::swap(v->idx[l], v->idx2[l]);
+        v->grid[l] = ug[l];
+        v->cnt[l] = cnt[l];
+    }
+    v->topo_gen++;
+    v->has_atlas = false;
+    v->vis_valid = false;
+    v->leaf_inactive = true;
     return GVDBX_OK;
 }
 
@@ -877,6 +1114,40 @@ int compute_bounds(gvdbx_volume* v) {
 int update_atlas(gvdbx_volume* v) {
     if (!v->has_topo) FAIL(GVDBX_ERR_STATE, "UpdateAtlas: no topology");
     if (!any_chan(v)) FAIL(GVDBX_ERR_STATE, "UpdateAtlas: no channel (AddChannel)");
+    if (v->leaf_inactive) {
+        // after an incremental update only the active leaves take atlas
+        // slots, in leaf order (VG.cpp:2654-2661); each leaf's voxels stay at
+        // its own index in the device buffers, so those hold every leaf
+        hipStream_t s = v->stream;
+        const int64_t nl = v->cnt[0];
+        TRY(ensure(v, v->atl_flag, (size_t)nl * 4));
+        TRY(ensure(v, v->atl_rank, (size_t)(nl + 1) * 4));
+        TRY(ensure(v, v->partials, (size_t)(scan_tiles(nl + 1) + 2) * sizeof(U2)));
+        TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
+        TRY(ensure_host(v, 4096));
+        launch_leaf_flags32(v->nodes[0].as<GNode>(), nl, v->atl_flag.as<int32_t>(), s);
+        launch_count_scan_reduce(v->atl_flag.as<int32_t>(), nl, v->partials.as<uint32_t>(), v->totals.as<uint32_t>(), s);
+        launch_count_scan_apply(v->atl_flag.as<int32_t>(), nl, v->partials.as<uint32_t>(), v->atl_rank.as<int32_t>(), s);
+        TRY(check_launch("atlas ranks"));
+        GVDBX_CHECK(hipMemcpyAsync(v->host, v->atl_rank.as<int32_t>() + nl, 4, hipMemcpyDeviceToHost, s));
+        TRY(sync(v));
+        int32_t used = 0;
+        memcpy(&used, v->host, 4);
+        int64_t amax = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+        if (used > amax) {   // VG.cpp:2645-2651 + AtlasResize AL.cpp:622-650
+            int sz = (int)ceil(used / (float)(v->subdim[0] * v->subdim[1]));
+            if (sz > v->subdim[2]) v->subdim[2] = sz;
+        }
+        int64_t slots = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+        TRY(grow_atlas(v, std::max(slots, nl)));
+        TRY(ensure(v, v->amap, (size_t)slots * sizeof(GAtlasNode)));
+        launch_atlas_assign_rank(v->nodes[0].as<GNode>(), nl, v->atl_rank.as<int32_t>(), v->subdim[0], v->subdim[1],
+                                 brick_res(v), std::max(v->apron, 0), v->amap.as<GAtlasNode>(), slots, s);
+        TRY(check_launch("atlas assign"));
+        v->amap_cnt = slots;
+        v->has_atlas = true;
+        return GVDBX_OK;
+    }
     int64_t used = v->cnt[0];
     int64_t amax = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
     if (used > amax) {   // VG.cpp:2645-2651 + AtlasResize AL.cpp:622-650
@@ -1150,10 +1421,11 @@ int gvdbx_destroy(gvdbx_volume* v) {
     }
     for (int l = 0; l < GVDBX_MAXLEV; l++) {
         free_buf(v->idx[l]);
+        free_buf(v->idx2[l]);
         free_buf(v->flag_base[l]);
         free_buf(v->flag_all[l]);
     }
-    DevBuf* bufs[] = {&v->reach0, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
+    DevBuf* bufs[] = {&v->inc_mark, &v->atl_flag, &v->atl_rank, &v->reach0, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
                       &v->bin_off, &v->sorted, &v->sorted2, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
@@ -1444,7 +1716,32 @@ int gvdbx_set_point_weights(gvdbx_volume* v, const float* w_dev, int stride) {
 int gvdbx_rebuild_topology(gvdbx_volume* v, int num_pnts, float radius, float ox, float oy, float oz) {
     TRY(check_vol(v));
     float o[3] = {ox, oy, oz};
+    // RequestFullRebuild(false): the incremental branch (VG.cpp:1195-1216)
+    // while the depth holds and the points stay inside the root
+    if (v->incre_rebuild && !v->force_full && v->has_topo && v->cnt[0] > 0 && v->ncloud == 0 &&
+        v->top_lev == v->depth) {
+        bool full = false;
+        TRY(incremental_topology(v, num_pnts, radius, o, false, &full));
+        if (!full) return GVDBX_OK;
+    }
     return rebuild_topology(v, num_pnts, radius, o);
+}
+
+int gvdbx_accumulate_topology(gvdbx_volume* v, int num_pnts, float radius, float ox, float oy, float oz,
+                              int min_depth) {
+    TRY(check_vol(v));
+    if (min_depth < 0) FAIL(GVDBX_ERR_ARG, "AccumulateTopology: negative minimum depth");
+    float o[3] = {ox, oy, oz};
+    // the first call, or one after RequestFullRebuild(true), rebuilds (VG.cpp:1173-1176)
+    if (!v->has_topo || v->cnt[0] == 0 || v->force_full) return rebuild_topology(v, num_pnts, radius, o);
+    return incremental_topology(v, num_pnts, radius, o, true, nullptr);
+}
+
+int gvdbx_request_full_rebuild(gvdbx_volume* v, int tf) {
+    TRY(check_vol(v));
+    v->incre_rebuild = !tf;
+    v->force_full = tf != 0;
+    return GVDBX_OK;
 }
 
 int gvdbx_finish_topology(gvdbx_volume* v, int commit_pool, int compute_bnds) {

@@ -187,6 +187,12 @@ class VolumeGVDB:
     def RebuildTopology(self, num_pnts, radius, orig=(0.0, 0.0, 0.0)):
         call("gvdbx_rebuild_topology", self._h, int(num_pnts), float(radius), *[float(x) for x in orig])
 
+    def AccumulateTopology(self, num_pnts, radius, orig=(0.0, 0.0, 0.0), iDepth=1):
+        call("gvdbx_accumulate_topology", self._h, int(num_pnts), float(radius), *[float(x) for x in orig], int(iDepth))
+
+    def RequestFullRebuild(self, tf):
+        call("gvdbx_request_full_rebuild", self._h, 1 if tf else 0)
+
     def FinishTopology(self, commit_pool=False, compute_bounds=True):
         call("gvdbx_finish_topology", self._h, int(commit_pool), int(compute_bounds))
 

@@ -59,6 +59,8 @@ _SIGS = {
     "gvdbx_set_points": [_vp, _vp, _i, _i, _vp, _i, _i, _vp, _i, _i],
     "gvdbx_set_point_weights": [_vp, _vp, _i],
     "gvdbx_rebuild_topology": [_vp, _i, _f, _f, _f, _f],
+    "gvdbx_accumulate_topology": [_vp, _i, _f, _f, _f, _f, _i],
+    "gvdbx_request_full_rebuild": [_vp, _i],
     "gvdbx_finish_topology": [_vp, _i, _i],
     "gvdbx_update_atlas": [_vp],
     "gvdbx_insert_points_subcell": [_vp, _i, _i, _f, _f, _f, _f, C.POINTER(C.c_int64)],

@@ -111,6 +111,10 @@ public:
     void RebuildTopology(int num_pnts, float radius, Vector3DF orig) {
         check(gvdbx_rebuild_topology(v_, num_pnts, radius, orig.x, orig.y, orig.z), "RebuildTopology");
     }
+    void AccumulateTopology(int num_pnts, float radius, Vector3DF orig, int iDepth = 1) {
+        check(gvdbx_accumulate_topology(v_, num_pnts, radius, orig.x, orig.y, orig.z, iDepth), "AccumulateTopology");
+    }
+    void RequestFullRebuild(bool tf) { check(gvdbx_request_full_rebuild(v_, tf ? 1 : 0), "RequestFullRebuild"); }
     void FinishTopology(bool commit = false, bool compute_bounds = true) {
         check(gvdbx_finish_topology(v_, commit, compute_bounds), "FinishTopology");
     }

@@ -131,11 +131,30 @@ int gvdbx_set_points(gvdbx_volume* v, const void* pos_dev, int pos_off, int pos_
 int gvdbx_set_point_weights(gvdbx_volume* v, const float* w_dev, int stride);
 
 /* ---- topology ---- */
-/* RebuildTopology(int, float, Vector3DF) VG.h:429 / VG.cpp:1181-1218 */
+/* RebuildTopology(int, float, Vector3DF) VG.h:429 / VG.cpp:1181-1218.  A full
+ * rebuild on every call, unless gvdbx_request_full_rebuild(v, 0) switched on
+ * the reference's incremental branch (ActivateIncreBricksGPU VG.cpp:1020-1155
+ * with bAccum = false): then, while the depth holds and every point lies
+ * inside the root, leaves no point's box [int(p) - r, int(p) + r] reaches are
+ * deactivated (mFlags 0, delinked, kept in the pool), missing nodes are
+ * appended in key order; otherwise it rebuilds fully. */
 int gvdbx_rebuild_topology(gvdbx_volume* v, int num_pnts, float radius, float ox, float oy, float oz);
+/* AccumulateTopology(int, float, Vector3DF, int) VG.h:431 / VG.cpp:1158-1178:
+ * the first call (or the first after gvdbx_request_full_rebuild(v, 1))
+ * rebuilds; later calls add the nodes the points' boxes reach and keep every
+ * existing one (ActivateIncreBricksGPU with bAccum = true).  The tree keeps
+ * its depth and root: points outside the root give GVDBX_ERR_RANGE and no
+ * change.  min_depth is accepted and has no further effect. */
+int gvdbx_accumulate_topology(gvdbx_volume* v, int num_pnts, float radius, float ox, float oy, float oz,
+                              int min_depth);
+/* RequestFullRebuild(bool) VG.h:432: 1 (the default) makes every
+ * RebuildTopology a full rebuild, 0 lets it update incrementally. */
+int gvdbx_request_full_rebuild(gvdbx_volume* v, int tf);
 /* FinishTopology(bool, bool) VG.h:405 / VG.cpp:1579-1593 */
 int gvdbx_finish_topology(gvdbx_volume* v, int commit_pool, int compute_bounds);
-/* UpdateAtlas() VG.h:406 / VG.cpp:2630-2704 */
+/* UpdateAtlas() VG.h:406 / VG.cpp:2630-2704.  Active leaves take atlas slots
+ * in leaf order (mValue, atlas map); leaves an incremental update
+ * deactivated keep their earlier mValue and own no slot. */
 int gvdbx_update_atlas(gvdbx_volume* v);
 
 /* ---- points -> voxels ---- */

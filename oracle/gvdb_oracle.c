@@ -422,6 +422,25 @@ int or_add_channel_typed(or_volume* v, int chan, int apron, int dtype) {
 /* topology                                                            */
 /* ------------------------------------------------------------------ */
 
+/* FinishTopology(false,true) -> ComputeBounds VG.cpp:1792-1816: the first
+ * leaf seeds the box, then every active leaf widens it */
+static void compute_bounds(or_volume* v) {
+    int64_t rg0 = v->range[0];
+    or_node* c0 = &v->nodes[0][0];
+    float vmin[3] = {(float)c0->mPos[0], (float)c0->mPos[1], (float)c0->mPos[2]};
+    float vmax[3] = {vmin[0], vmin[1], vmin[2]};
+    for (int64_t i = 0; i < v->ncnt[0]; i++) {
+        or_node* c = &v->nodes[0][i];
+        if (!c->mFlags) continue;
+        for (int a = 0; a < 3; a++) {
+            if (c->mPos[a] < vmin[a]) vmin[a] = (float)c->mPos[a];
+            if (c->mPos[a] + rg0 > vmax[a]) vmax[a] = (float)(c->mPos[a] + rg0);
+        }
+    }
+    memcpy(v->obj_min, vmin, sizeof(vmin));
+    memcpy(v->obj_max, vmax, sizeof(vmax));
+}
+
 int or_rebuild_topology(or_volume* v, const float* pos, int64_t n, int64_t stride, float radius,
                         const float orig[3]) {
     if (!v || !v->configured || (!pos && n > 0) || n < 0) return OR_ERR_ARG;
@@ -545,21 +564,174 @@ int or_rebuild_topology(or_volume* v, const float* pos, int64_t n, int64_t strid
     }
     free(keys);
 
-    /* FinishTopology(false,true) -> ComputeBounds VG.cpp:1792-1816 */
-    int64_t rg0 = v->range[0];
-    or_node* c0 = &v->nodes[0][0];
-    float vmin[3] = {(float)c0->mPos[0], (float)c0->mPos[1], (float)c0->mPos[2]};
-    float vmax[3] = {vmin[0], vmin[1], vmin[2]};
-    for (int64_t i = 0; i < v->ncnt[0]; i++) {
-        or_node* c = &v->nodes[0][i];
-        if (!c->mFlags) continue;
+    compute_bounds(v);
+    v->atlas_used = 0;
+    return OR_OK;
+}
+
+/* getPosParent ND.cuh:303-348 (no bitmasks): the node of level stop_lev
+ * holding pos, found by descending the child lists from the top node;
+ * -1 when a list on the way has no entry */
+static int64_t pos_parent(const or_volume* v, const int pos[3], int stop_lev) {
+    if (v->ncnt[v->top_lev] < 1) return -1;
+    const or_node* nd = &v->nodes[v->top_lev][0];
+    int l = nd->mLev;
+    if (l == 0) return 0;
+    while (l > 0) {
+        int res = v->res[l];
+        int64_t pin[3];
+        pos_in_node(pos, nd->mPos, res, v->range[l], pin);
+        int64_t bit = (pin[2] * res + pin[1]) * res + pin[0];
+        int64_t c = get_child(v, nd, bit);
+        l--;
+        if (c < 0) return -1;
+        if (l == stop_lev) return c;
+        if (c >= v->ncnt[l]) return -1;
+        nd = &v->nodes[l][c];
+    }
+    return -1;
+}
+
+/* the parent's child-list slot of a node at level lev, or -1 (link/delink
+ * position arithmetic ND.cuh:414-428) */
+static int64_t parent_slot(const or_volume* v, int lev, const or_node* node, int64_t p) {
+    const or_node* pn = &v->nodes[lev + 1][p];
+    int res = v->res[lev + 1];
+    int pos[3] = {node->mPos[0], node->mPos[1], node->mPos[2]};
+    int64_t pin[3];
+    pos_in_node(pos, pn->mPos, res, v->range[lev + 1], pin);
+    for (int a = 0; a < 3; a++)
+        if (pin[a] < 0 || pin[a] >= res) return -1;
+    int64_t len = ipow3(res);
+    int64_t flat = (int64_t)(pn->mChildList >> 16) * len + (pin[2] * res + pin[1]) * res + pin[0];
+    if (flat < 0 || flat >= v->ccnt[lev + 1] * len) return -1;   /* D2 */
+    return flat;
+}
+
+/* ActivateIncreBricksGPU VG.cpp:1020-1155.  accum = 1 is AccumulateTopology
+ * (VG.cpp:1158-1178): existing leaves keep their flags; accum = 0 is the
+ * incremental RebuildTopology (VG.cpp:1215-1216): leaves no point reaches
+ * are deactivated and delinked.  Every valid point's box
+ * [int(w) - r, int(w) + r] (gvdbCalcIncreExtraBrickId PT.cuh:420-465) is
+ * walked at every level below the root: reachable nodes are marked (leaves),
+ * missing ones become keys; the sorted unique keys are appended to the
+ * pools; the flagged nodes are linked top-down (ND.cuh:470-505), then the
+ * unflagged leaves and the childless inner nodes are delinked bottom-up
+ * (gvdbDelinkLeafBricks / gvdbDelinkBricks ND.cuh:401-467).  With no
+ * missing node nothing changes (VG.cpp:1066).  Deviations: the tree's own
+ * depth and root are kept, so every valid point must lie inside the root
+ * (else OR_ERR_RANGE and no change); box cells outside the root are not
+ * emitted (D3). */
+int or_incremental_topology(or_volume* v, const float* pos, int64_t n, int64_t stride, float radius,
+                            const float orig[3], int accum) {
+    if (!v || !v->configured || (!pos && n > 0) || n < 0 || !(radius >= 0.0f) || !isfinite(radius)) return OR_ERR_ARG;
+    if (v->ncnt[0] == 0 || v->depth <= 0) return OR_ERR_STATE;   /* VG.cpp:1022 */
+    if (v->top_lev != v->depth) return OR_ERR_STATE;
+    if (stride == 0) stride = 12;
+    const int depth = v->depth;
+    const int64_t rtop = v->range[depth];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int64_t nvalid = 0;
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), orig, w)) continue;
+        nvalid++;
         for (int a = 0; a < 3; a++) {
-            if (c->mPos[a] < vmin[a]) vmin[a] = (float)c->mPos[a];
-            if (c->mPos[a] + rg0 > vmax[a]) vmax[a] = (float)(c->mPos[a] + rg0);
+            if (w[a] < mn[a]) mn[a] = w[a];
+            if (w[a] > mx[a]) mx[a] = w[a];
         }
     }
-    memcpy(v->obj_min, vmin, sizeof(vmin));
-    memcpy(v->obj_max, vmax, sizeof(vmax));
+    if (nvalid == 0) return OR_ERR_NOPOINTS;
+    for (int a = 0; a < 3; a++)
+        if (mn[a] < (float)v->root_pos[a] || mx[a] >= (float)((int64_t)v->root_pos[a] + rtop)) return OR_ERR_RANGE;
+    /* GetBoundingBox VG.cpp:5404-5467 */
+    for (int a = 0; a < 3; a++) {
+        float lo = mn[a] - 1, hi = mx[a] + 2;
+        v->pos_min[a] = 0 > lo ? 0 : lo;
+        v->pos_max[a] = 0 > hi ? 0 : hi;
+    }
+    /* node markers (VG.cpp:1026-1041) */
+    int64_t nleaf = v->ncnt[0];
+    uint8_t* marker = (uint8_t*)malloc((size_t)nleaf);
+    for (int64_t k = 0; k < nleaf; k++) marker[k] = accum ? v->nodes[0][k].mFlags : 0;
+    int64_t cap = 1024, ne = 0;
+    uint64_t* keys = (uint64_t*)malloc((size_t)cap * sizeof(uint64_t));
+    for (int64_t i = 0; i < n; i++) {
+        float w[3];
+        if (!point_valid(pt(pos, i, stride), orig, w)) continue;
+        int bmin[3], bmax[3];
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = (int)((float)(int)w[a] - radius);
+            bmax[a] = (int)((float)(int)w[a] + radius);
+        }
+        for (int lev = 0; lev < depth; lev++) {
+            int rres = (int)v->range[lev];
+            int nd[3];
+            for (nd[0] = bmin[0] / rres * rres; nd[0] <= bmax[0] / rres * rres; nd[0] += rres)
+                for (nd[1] = bmin[1] / rres * rres; nd[1] <= bmax[1] / rres * rres; nd[1] += rres)
+                    for (nd[2] = bmin[2] / rres * rres; nd[2] <= bmax[2] / rres * rres; nd[2] += rres) {
+                        int inside = 1;   /* D3 */
+                        for (int a = 0; a < 3; a++)
+                            if (nd[a] < v->root_pos[a] || (int64_t)nd[a] >= v->root_pos[a] + rtop) inside = 0;
+                        if (!inside) continue;
+                        int64_t id = pos_parent(v, nd, lev);
+                        if (id < 0) {
+                            if (ne >= cap) {
+                                cap *= 2;
+                                keys = (uint64_t*)realloc(keys, (size_t)cap * sizeof(uint64_t));
+                            }
+                            keys[ne++] = make_key(lev, nd[0] / rres, nd[1] / rres, nd[2] / rres);
+                        } else if (lev == 0 && id < nleaf) {
+                            marker[id] = 1;
+                        }
+                    }
+        }
+    }
+    if (ne == 0) {   /* no new node: nothing changes (VG.cpp:1066) */
+        free(keys);
+        free(marker);
+        return OR_OK;
+    }
+    qsort(keys, (size_t)ne, sizeof(uint64_t), cmp_u64);
+    int64_t nu = unique_u64(keys, ne);
+    for (int64_t k = 0; k < nleaf; k++) v->nodes[0][k].mFlags = marker[k];   /* VG.cpp:1078-1085 */
+    free(marker);
+    alloc_from_keys(v, keys, nu);   /* VG.cpp:1087-1111 */
+    free(keys);
+    compute_top_lev(v);             /* PrepareVDBPartially */
+    for (int lev = depth - 1; lev >= 0; lev--) link_level(v, lev);
+    /* gvdbDelinkLeafBricks ND.cuh:401-431 */
+    for (int64_t k = 0; k < v->ncnt[0]; k++) {
+        or_node* node = &v->nodes[0][k];
+        if (node->mFlags || node->mParent == OR_ID_UNDEF64) continue;
+        int p3[3] = {node->mPos[0], node->mPos[1], node->mPos[2]};
+        int64_t p = get_parent(v, 1, p3);
+        if (p < 0 || p >= v->ncnt[1]) continue;
+        int64_t slot = parent_slot(v, 0, node, p);
+        node->mParent = OR_ID_UNDEF64;
+        if (slot >= 0) v->clist[1][slot] = OR_ID_UNDEF64;
+    }
+    /* gvdbDelinkBricks ND.cuh:433-467, one level after the other */
+    for (int lev = 1; lev < depth; lev++) {
+        int64_t len = ipow3(v->res[lev]);
+        for (int64_t k = 0; k < v->ncnt[lev]; k++) {
+            or_node* node = &v->nodes[lev][k];
+            if (node->mParent == OR_ID_UNDEF64) { node->mFlags = 0; continue; }
+            const uint64_t* own = v->clist[lev] + k * len;
+            int any = 0;
+            for (int64_t c = 0; c < len && !any; c++) any = own[c] != OR_ID_UNDEF64;
+            if (any) { node->mFlags = 1; continue; }
+            node->mFlags = 0;
+            int p3[3] = {node->mPos[0], node->mPos[1], node->mPos[2]};
+            int64_t p = get_parent(v, lev + 1, p3);
+            if (p < 0 || p >= v->ncnt[lev + 1]) continue;
+            int64_t slot = parent_slot(v, lev, node, p);
+            node->mParent = OR_ID_UNDEF64;
+            if (slot >= 0) v->clist[lev + 1][slot] = OR_ID_UNDEF64;
+        }
+    }
+    free_sc(v);
+    compute_bounds(v);
     v->atlas_used = 0;
     return OR_OK;
 }
@@ -569,7 +741,10 @@ int or_update_atlas(or_volume* v) {
     int any = 0;
     for (int c = 0; c < OR_MAXCHAN; c++) any |= v->has_chan[c];
     if (!any) return OR_ERR_STATE;
-    int64_t used = v->ncnt[0];
+    /* the pool's used count: every leaf after a rebuild, the active ones
+     * after an incremental update (UpdateUsed VG.cpp:1145-1149) */
+    int64_t used = 0;
+    for (int64_t k = 0; k < v->ncnt[0]; k++) used += v->nodes[0][k].mFlags ? 1 : 0;
     int64_t amax = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
     if (used > amax) {   /* VG.cpp:2645-2651, AL.cpp:622-650 */
         int sz = (int)ceil(used / (float)(v->subdim[0] * v->subdim[1]));
@@ -876,7 +1051,11 @@ int or_update_apron(or_volume* v, int chan, float boundval) {
     const uint32_t bound = bound_word(v->chan_type[chan], boundval);
     int br = brick_res(v);
     int sx = v->subdim[0], sy = v->subdim[1];
-    for (int64_t b = 0; b < v->ncnt[0]; b++) {   /* UpdateApron<T> OP.cuh:73-126 */
+    /* UpdateApron<T> OP.cuh:73-126 runs over the atlas bricks: every brick
+     * the atlas map assigns to a leaf (deactivated leaves own none) */
+    for (int64_t slot = 0; slot < v->amap_cnt; slot++) {
+        const int32_t b = v->amap[slot].mLeafID;
+        if (b == (int32_t)OR_ID_UNDEFL || b < 0 || b >= v->ncnt[0]) continue;
         const or_node* node = &v->nodes[0][b];
         for (int side = 0; side < 6; side++)
             for (int su = 0; su < br; su++)

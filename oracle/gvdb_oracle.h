@@ -79,6 +79,11 @@ int or_add_channel_typed(or_volume* v, int chan, int apron, int dtype);
  * ActivateExtraBricksGPU 931-1016) followed by FinishTopology(false,true). */
 int or_rebuild_topology(or_volume* v, const float* pos, int64_t n, int64_t stride_bytes,
                         float radius, const float orig[3]);
+/* ActivateIncreBricksGPU VG.cpp:1020-1155 on the current tree: accum = 1 is
+ * AccumulateTopology (VG.cpp:1158-1178), accum = 0 the incremental
+ * RebuildTopology (VG.cpp:1215-1216).  Keeps the tree's depth and root. */
+int or_incremental_topology(or_volume* v, const float* pos, int64_t n, int64_t stride_bytes, float radius,
+                            const float orig[3], int accum);
 /* UpdateAtlas VG.cpp:2630-2704 */
 int or_update_atlas(or_volume* v);
 

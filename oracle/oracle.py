@@ -110,6 +110,7 @@ def lib():
         L.or_scatter_color.argtypes = [vp, vp, i64, i64, vp, i64, fp, i32, i32]
         L.or_rebuild_topology.argtypes = [vp, vp, i64, i64, f32, fp]
         L.or_update_atlas.argtypes = [vp]
+        L.or_incremental_topology.argtypes = [vp, vp, i64, i64, f32, fp, i32]
         L.or_insert_points_subcell.argtypes = [vp, i32, vp, i64, i64, f32, fp, C.POINTER(C.c_int64)]
         L.or_insert_points_subcell_fp16.argtypes = [vp, i32, vp, i64, i64, f32, fp, C.POINTER(C.c_int64)]
         L.or_gather.argtypes = [vp, i32, f32, i32, i32]
@@ -195,6 +196,13 @@ class OracleVolume:
         p = _pts(points)
         code = self._L.or_rebuild_topology(self._h, p.ctypes.data, p.shape[0], 12, float(radius), _f3(orig))
         _chk(code, "rebuild_topology")
+
+    def incremental_topology(self, points, radius, orig=(0, 0, 0), accum=True):
+        """ActivateIncreBricksGPU on the current tree (accum: AccumulateTopology)."""
+        p = _pts(points)
+        code = self._L.or_incremental_topology(self._h, p.ctypes.data, p.shape[0], 12, float(radius), _f3(orig),
+                                               1 if accum else 0)
+        _chk(code, "incremental_topology")
 
     def update_atlas(self):
         _chk(self._L.or_update_atlas(self._h), "update_atlas")

@@ -276,3 +276,52 @@ def gather_color_expected(members, clr, res0, sc, init=None):
                 f = base + S.astype(np.float32) / np.float32(255)
                 out[sl] = sat_u8((f / n) * np.float32(255))
     return out
+
+
+def incre_touched(points, radius, orig, rng, depth, root):
+    """Cells of every level below the root that the incremental update's box
+    [int(w) - r, int(w) + r] reaches (gvdbCalcIncreExtraBrickId,
+    cuda_gvdb_particles.cuh:420-465), C truncation of the corner division,
+    cells outside the root dropped; per level, sorted in key order."""
+    ok, w = valid_points(points, orig)
+    w = w[ok]
+    ix = w.astype(np.int32).astype(np.float32)
+    bmin = (ix - np.float32(radius)).astype(np.int32).astype(np.int64)
+    bmax = (ix + np.float32(radius)).astype(np.int32).astype(np.int64)
+    root = np.asarray(root, dtype=np.int64)
+    out = []
+    for l in range(depth):
+        rg = rng[l]
+        lo, hi = ctrunc_div(bmin, rg), ctrunc_div(bmax, rg)
+        span = int((hi - lo).max()) + 1 if len(lo) else 0
+        cells = [np.zeros((0, 3), dtype=np.int64)]
+        for dx in range(span):
+            for dy in range(span):
+                for dz in range(span):
+                    c = lo + np.array([dx, dy, dz])
+                    cells.append(c[(c <= hi).all(1)])
+        allc = np.unique(np.concatenate(cells), axis=0)
+        inroot = ((allc * rg >= root) & (allc * rg < root + rng[depth])).all(1)
+        out.append(lexsort_cells(allc[inroot]))
+    return out
+
+
+def reachable(vol, depth):
+    """Per level below the root, the node indices reached from the root
+    through the child lists (entries (index << 16) | (level << 8))."""
+    res_lists = {l: vol.child_lists(l) for l in range(1, depth + 1)}
+    nodes = {l: vol.nodes(l) for l in range(depth + 1)}
+    reach = {depth: np.array([0])}
+    for l in range(depth, 0, -1):
+        cl = nodes[l]["mChildList"][reach[l]] >> np.uint64(16)
+        ent = res_lists[l][cl.astype(np.int64)].ravel()
+        ent = ent[ent != np.uint64(0xFFFFFFFFFFFFFFFF)]
+        assert ((ent >> np.uint64(8)) & np.uint64(0xFF) == np.uint64(l - 1)).all()
+        reach[l - 1] = np.sort((ent >> np.uint64(16)).astype(np.int64))
+    return reach
+
+
+def reachable_cells(vol, depth, rng):
+    """reachable() as sorted cell rows per level."""
+    reach = reachable(vol, depth)
+    return [lexsort_cells(vol.nodes(l)["mPos"][reach[l]].astype(np.int64) // rng[l]) for l in range(depth)]
