@@ -371,6 +371,8 @@ struct RenderScene {
     float light[3];
     float shadow_amt, shadow_bias;
     const float4* lut;
+    float finestep;                      // SCN_FINESTEP (SHADE_TRICUBIC)
+    float slice_pnt[3], slice_norm[3];   // SCN_SLICE_PNT / SCN_SLICE_NORM (sections)
 };
 // per (ray, brick) segment entry state of a recorded EA raycast
 // Segment checkpoints of a recording EA raycast.  Each wave buffers its
@@ -390,6 +392,13 @@ struct VisitRec {
 };
 void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba, uchar4* rgba8,
                    hipStream_t s);
+// ScnRay (cuda_gvdb_scene.cuh:66-74): 64-byte ray record of Raytrace
+struct GScnRay {
+    float hit[3], normal[3], orig[3], dir[3];
+    uint32_t clr, pnode, pndx, pad;
+};
+static_assert(sizeof(GScnRay) == 64, "ray record must be 64 bytes");
+void launch_raytrace(RenderTree tv, RenderScene sc, GScnRay* rays, int64_t n, float bias, hipStream_t s);
 // cams (device, ncam entries, all of cam's size) renders one image per
 // camera, image c at c*width*height; cams == nullptr renders cam alone
 void launch_raycast_ea(RenderTree tv, RenderCam cam, const RenderCam* cams, int ncam, RenderScene sc, float* I,

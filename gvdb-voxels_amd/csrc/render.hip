@@ -104,6 +104,20 @@ struct Hdda {
             p[a] = (int)fl;
         }
     }
+    // PrepareLeaf DDA.cuh:67-73: vdel 1 and no "+ t.x" in tSide (only the
+    // order of the steps matters inside a brick)
+    __device__ __forceinline__ void prepare_leaf(V3 vmin) {
+        const V3 q = sub(add(pos, scale(dir, t.x)), vmin);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float pf = comp(q, a);
+            tdel[a] = fabsf(1.0f / comp(dir, a));
+            const float fl = floorf(pf);
+            const float br = ((fl - pf) + 0.5f) * (float)ps[a] + 0.5f;
+            ts[a] = __builtin_isinf(tdel[a]) ? __builtin_inff() : br * tdel[a];
+            p[a] = (int)fl;
+        }
+    }
     __device__ __forceinline__ void next() {
         mask[0] = (ts[0] < ts[1]) & (ts[0] <= ts[2]);
         mask[1] = (ts[1] < ts[2]) & (ts[1] <= ts[0]);
@@ -177,7 +191,8 @@ __device__ __forceinline__ int transfer_index(const RenderScene& sc, float s) {
     return (int)(qc * 16300.0);
 }
 
-enum { MODE_DEEP_LUT = 0, MODE_DEEP_LINEAR = 1, MODE_SURF_TRI = 2, MODE_SURF_LS = 3 };
+enum { MODE_DEEP_LUT = 0, MODE_DEEP_LINEAR = 1, MODE_SURF_TRI = 2, MODE_SURF_LS = 3, MODE_SURF_VOXEL = 4,
+       MODE_SURF_CUBIC = 5, MODE_EMPTY = 6 };
 
 struct RayState {
     float clr[4];
@@ -199,6 +214,52 @@ __device__ __forceinline__ V3 gradient(const float* B, int br, V3 p, bool ls) {
     lo = trilinear<BR>(B, br, v3(p.x, p.y, p.z - 0.5f));
     hi = trilinear<BR>(B, br, v3(p.x, p.y, p.z + 0.5f));
     g.z = ls ? hi - lo : lo - hi;
+    return normalize(g);
+}
+
+// getTricubic RC.cuh:32-98: 27 filtered taps at the integer points
+// floor(p) - 1 + {0, 1, 2} per axis (each the trilinear sample there),
+// weights ta^2, 2 ta tb, tb^2 with tb = frac(p)/2 + 1/4, x then y then z.
+// Brick-local coordinates; taps past the padded brick clamp to it (the
+// reference reads on into the neighbouring atlas brick).
+template <int BR>
+__device__ __forceinline__ float tricubic(const float* B, int br, V3 p) {
+    const float flx = floorf(p.x), fly = floorf(p.y), flz = floorf(p.z);
+    float ta2[3], tab[3], tb2[3];
+    const float fr[3] = {p.x - flx, p.y - fly, p.z - flz};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float tb = fr[k] * 0.5f + 0.25f;
+        const float ta = 1.0f - tb;
+        ta2[k] = ta * ta;
+        tb2[k] = tb * tb;
+        tab[k] = ta * tb * 2.0f;
+    }
+    const float qx = flx - 1.0f, qy = fly - 1.0f, qz = flz - 1.0f;
+    float jkl[3];
+#pragma unroll
+    for (int zz = 0; zz < 3; zz++) {
+        float row[3];
+#pragma unroll
+        for (int yy = 0; yy < 3; yy++) {
+            const float t0 = trilinear<BR>(B, br, v3(qx, qy + (float)yy, qz + (float)zz));
+            const float t1 = trilinear<BR>(B, br, v3(qx + 1.0f, qy + (float)yy, qz + (float)zz));
+            const float t2 = trilinear<BR>(B, br, v3(qx + 2.0f, qy + (float)yy, qz + (float)zz));
+            row[yy] = t0 * ta2[0] + t1 * tab[0] + t2 * tb2[0];
+        }
+        jkl[zz] = row[0] * ta2[1] + row[1] * tab[1] + row[2] * tb2[1];
+    }
+    return jkl[0] * ta2[2] + jkl[1] * tab[2] + jkl[2] * tb2[2];
+}
+
+// getGradientTricubic RC.cuh:161-171 (negated, half-voxel central differences)
+template <int BR>
+__device__ __forceinline__ V3 gradient_cubic(const float* B, int br, V3 p) {
+    const float vs = 0.5f;
+    V3 g;
+    g.x = (tricubic<BR>(B, br, add(p, v3(-vs, 0, 0))) - tricubic<BR>(B, br, add(p, v3(vs, 0, 0)))) / (2 * vs);
+    g.y = (tricubic<BR>(B, br, add(p, v3(0, -vs, 0))) - tricubic<BR>(B, br, add(p, v3(0, vs, 0)))) / (2 * vs);
+    g.z = (tricubic<BR>(B, br, add(p, v3(0, 0, -vs))) - tricubic<BR>(B, br, add(p, v3(0, 0, vs)))) / (2 * vs);
     return normalize(g);
 }
 
@@ -291,6 +352,59 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
             st.clr[1] = fminf(st.clr[1], 1.f);
             st.clr[2] = fminf(st.clr[2], 1.f);
             st.clr[3] = fmaxf(st.clr[3], 0.f);
+        }
+    } else if (MODE == MODE_EMPTY) {
+        st.hit = add(pos, scale(dir, t.x));   // rayEmptySkipBrick RC.cuh:425-428
+    } else if (MODE == MODE_SURF_VOXEL) {
+        // raySurfaceVoxelBrick RC.cuh:227-263: the leaf's voxels in DDA
+        // order; the first above the threshold (its exact value: the texel
+        // centre) is hit as a unit cube; a missed cube repeats the voxel
+        // without a step until the iteration limit, as the reference does
+        const int ap = (br - tv.res[0]) >> 1;
+        const int r0i = tv.res[0];
+        V3 vm = vmin;
+        Hdda d;
+        d.set(pos, dir, t);
+        d.prepare_leaf(vm);
+        for (int iter = 0; iter < sc.max_iter && d.p[0] >= 0 && d.p[1] >= 0 && d.p[2] >= 0 && d.p[0] < r0i &&
+                           d.p[1] < r0i && d.p[2] < r0i;
+             iter++) {
+            if (B[((d.p[2] + ap) * br + (d.p[1] + ap)) * br + (d.p[0] + ap)] > sc.thresh) {
+                vm = add(vm, v3((float)d.p[0], (float)d.p[1], (float)d.p[2]));
+                const float lo[3] = {vm.x, vm.y, vm.z}, hi[3] = {vm.x + 1.0f, vm.y + 1.0f, vm.z + 1.0f};
+                const V3 tb = ray_box(pos, dir, lo, hi);
+                d.t = tb;
+                if (tb.z == GVDBX_NOHIT) {
+                    st.hit.z = GVDBX_NOHIT;
+                    continue;
+                }
+                const V3 hit = add(pos, scale(dir, tb.x));
+                V3 fc = sub(sub(hit, vm), v3(0.5f, 0.5f, 0.5f));
+                fc = sub(fc, scale(dir, 0.01f));
+                const float m = fmaxf(fmaxf(fabsf(fc.x), fabsf(fc.y)), fabsf(fc.z));
+                st.norm = v3(fabsf(fc.x) == m ? copysignf(1.0f, fc.x) : 0.0f, fabsf(fc.y) == m ? copysignf(1.0f, fc.y) : 0.0f,
+                             fabsf(fc.z) == m ? copysignf(1.0f, fc.z) : 0.0f);
+                st.hit = hit;
+                return;
+            }
+            d.next();
+            d.step();
+        }
+    } else if (MODE == MODE_SURF_CUBIC) {
+        // raySurfaceTricubicBrick RC.cuh:316-340
+        V3 p = sub(add(pos, scale(dir, t.x)), vmin);
+        for (int iter = 0; iter < sc.max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0;
+             iter++) {
+            const float vz = tricubic<BR>(B, br, p);
+            if (vz >= sc.thresh) {
+                const float vx = tricubic<BR>(B, br, sub(p, scale(dir, sc.finestep)));
+                const float vy = (vz - sc.thresh) / (vz - vx);
+                p = add(p, scale(dir, -vy * sc.finestep));
+                st.hit = add(p, vmin);
+                st.norm = gradient_cubic<BR>(B, br, p);
+                return;
+            }
+            p = add(p, scale(dir, sc.step));
         }
     } else if (MODE == MODE_SURF_TRI) {
         t.x = sc.step * ceilf(t.x / sc.step);
@@ -506,7 +620,7 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
     camera_ray(cam, x, y, pos, dir);
     RayState st;
     st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
-    st.hit = MODE == MODE_SURF_TRI ? v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT) : v3(0, 0, GVDBX_NOHIT);
+    st.hit = MODE == MODE_SURF_LS ? v3(0, 0, GVDBX_NOHIT) : v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT);
     st.norm = v3(0, 0, 0);
     VisitRec none{};
     ray_cast<MODE, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
@@ -524,7 +638,9 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
             s2.clr[3] = 1.0f;
             s2.hit = v3(0, 0, GVDBX_NOHIT);
             s2.norm = v3(0, 0, 0);
-            ray_cast<MODE, BR, false>(tv, sc, cam, add(st.hit, scale(st.norm, sc.shadow_bias)), ldir, s2, 0, none);
+            // gvdbRaySurfaceTricubic shades with the trilinear brick (MOD.cu:137)
+            constexpr int SMODE = MODE == MODE_SURF_CUBIC ? MODE_SURF_TRI : MODE;
+            ray_cast<SMODE, BR, false>(tv, sc, cam, add(st.hit, scale(st.norm, sc.shadow_bias)), ldir, s2, 0, none);
             if (s2.hit.z != GVDBX_NOHIT) diff = (float)((double)diff * (1.0 - (double)sc.shadow_amt));
         }
         const float sh = diff + amb;
@@ -533,6 +649,178 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
     const int64_t px = (int64_t)y * cam.width + x;
     if (rgba) rgba[px] = o;
     if (rgba8) rgba8[px] = to_u8(o);
+}
+
+// getNodeAtPoint ND.cuh:199-226,243-253: float descent from the top node to
+// the leaf holding w, -1 when a list on the way has no entry or the leaf
+// found does not contain w
+__device__ __forceinline__ int64_t point_leaf(const RenderTree& tv, const float w[3]) {
+    int lev = tv.top_lev;
+    if (lev < 1 || lev >= tv.levs || tv.ncnt[lev] < 1) return -1;
+    const int zero[3] = {0, 0, 0};
+    V3 vmin = node_pos(tv.nodes[lev], 0, zero);
+    uint64_t list = tv.nodes[lev][0].mChildList;
+    int64_t id = 0;
+    while (lev > 0) {
+        const float rg = tv.vdel[lev] * (float)tv.res[lev];
+        if (w[0] < vmin.x || w[1] < vmin.y || w[2] < vmin.z || w[0] >= vmin.x + rg || w[1] >= vmin.y + rg ||
+            w[2] >= vmin.z + rg)
+            return -1;
+        const float vdel = tv.vdel[lev];
+        const int p0 = (int)((w[0] - vmin.x) / vdel), p1 = (int)((w[1] - vmin.y) / vdel), p2 = (int)((w[2] - vmin.z) / vdel);
+        const int dim = tv.logdim[lev];
+        const int64_t b = ((((int64_t)p2 << dim) + p1) << dim) + p0;
+        lev--;
+        const int64_t c = get_child(tv, list, b);
+        if (c < 0 || c >= tv.ncnt[lev]) return -1;
+        id = c;
+        vmin = node_pos(tv.nodes[lev], c, zero);
+        list = tv.nodes[lev][c].mChildList;
+    }
+    const float r0 = (float)tv.res[0];
+    if (!(w[0] >= vmin.x && w[0] < vmin.x + r0 && w[1] >= vmin.y && w[1] < vmin.y + r0 && w[2] >= vmin.z &&
+          w[2] < vmin.z + r0))
+        return -1;
+    return id;
+}
+
+// transfer(gvdb, v): the colour of the transfer function at the sample
+__device__ __forceinline__ float4 transfer_lut(const RenderScene& sc, float s) { return sc.lut[transfer_index(sc, s)]; }
+
+// uchar4 of the colour with alpha 255 (the empty-skip and section kernels)
+__device__ __forceinline__ uchar4 to_u8_opaque(float4 c) {
+    uchar4 q = to_u8(c);
+    q.w = 255;
+    return q;
+}
+
+// gvdbRayEmptySkip MOD.cu:184-204: the first brick entry point / 100
+template <int BR>
+__global__ __launch_bounds__(256) void k_render_empty(RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba,
+                                                      uchar4* rgba8) {
+    int x, y;
+    if (!pixel_of(cam, x, y)) return;
+    V3 pos, dir;
+    camera_ray(cam, x, y, pos, dir);
+    RayState st;
+    st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
+    st.hit = v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT);
+    st.norm = v3(0, 0, 0);
+    VisitRec none{};
+    ray_cast<MODE_EMPTY, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
+    const float4 o = st.hit.z != GVDBX_NOHIT ? make_float4(st.hit.x * 0.01f, st.hit.y * 0.01f, st.hit.z * 0.01f, 1.0f)
+                                            : make_float4(sc.backclr[0], sc.backclr[1], sc.backclr[2], sc.backclr[3]);
+    const int64_t px = (int64_t)y * cam.width + x;
+    if (rgba) rgba[px] = o;
+    if (rgba8) rgba8[px] = to_u8_opaque(o);
+}
+
+// gvdbSection2D MOD.cu:268-293: pixel (x, y) samples the slice point
+// pnt + ((2x/w - 1, 0, 2y/h - 1) * norm), the offsets formed in double
+template <int BR>
+__global__ __launch_bounds__(256) void k_render_section2d(RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba,
+                                                          uchar4* rgba8) {
+    int x, y;
+    if (!pixel_of(cam, x, y)) return;
+    const float sx = (float)((double)(float)x * 2.0 / (double)cam.width - 1.0);
+    const float sz = (float)((double)(float)y * 2.0 / (double)cam.height - 1.0);
+    const float w[3] = {sc.slice_pnt[0] + sx * sc.slice_norm[0], sc.slice_pnt[1] + 0.0f * sc.slice_norm[1],
+                        sc.slice_pnt[2] + sz * sc.slice_norm[2]};
+    const int64_t leaf = point_leaf(tv, w);
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    if (leaf >= 0) {
+        const int br = BR > 0 ? BR : tv.brick_res;
+        const int zero[3] = {0, 0, 0};
+        const V3 vmin = node_pos(tv.nodes[0], leaf, zero);
+        const float s = trilinear<BR>(tv.chan + leaf * (int64_t)(br * br * br), br, v3(w[0] - vmin.x, w[1] - vmin.y, w[2] - vmin.z));
+        const float4 c = transfer_lut(sc, s);
+        o = make_float4(0.0f + c.w * (c.x - 0.0f), 0.0f + c.w * (c.y - 0.0f), 0.0f + c.w * (c.z - 0.0f), 1.0f);
+    }
+    const int64_t px = (int64_t)y * cam.width + x;
+    if (rgba) rgba[px] = o;
+    if (rgba8) rgba8[px] = to_u8_opaque(o);
+}
+
+// gvdbSection3D MOD.cu:222-265: the transfer colour where the ray meets the
+// slice plane, blended over the trilinear surface found beyond it
+template <int BR>
+__global__ __launch_bounds__(256) void k_render_section3d(RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba,
+                                                          uchar4* rgba8) {
+    int x, y;
+    if (!pixel_of(cam, x, y)) return;
+    V3 pos, dir;
+    camera_ray(cam, x, y, pos, dir);
+    float clr[4] = {1.0f, 1.0f, 1.0f, 0.0f};
+    const float* n = sc.slice_norm;
+    const float* q = sc.slice_pnt;
+    // rayPlaneIntersect GEO.cuh:67-71
+    float t = ((q[0] - pos.x) * n[0] + (q[1] - pos.y) * n[1] + (q[2] - pos.z) * n[2]) /
+              (dir.x * n[0] + dir.y * n[1] + dir.z * n[2]);
+    t = t > 0 ? t : GVDBX_NOHIT;
+    if (t > 0) {
+        pos = add(pos, scale(dir, t));
+        const float w[3] = {pos.x, pos.y, pos.z};
+        const int64_t leaf = point_leaf(tv, w);
+        if (leaf >= 0) {
+            const int br = BR > 0 ? BR : tv.brick_res;
+            const int zero[3] = {0, 0, 0};
+            const V3 vmin = node_pos(tv.nodes[0], leaf, zero);
+            t = trilinear<BR>(tv.chan + leaf * (int64_t)(br * br * br), br, sub(pos, vmin));
+            const float4 c = transfer_lut(sc, t);
+            clr[0] = c.x; clr[1] = c.y; clr[2] = c.z; clr[3] = c.w;
+        } else {
+            t = 0;
+        }
+    }
+    RayState st;
+    st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
+    st.hit = v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT);
+    st.norm = v3(0, 0, 0);
+    VisitRec none{};
+    ray_cast<MODE_SURF_TRI, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
+    float o[4];
+    if (st.hit.z != GVDBX_NOHIT) {
+        const V3 ldir = normalize(sub(v3(sc.light[0], sc.light[1], sc.light[2]), st.hit));
+        const float ds = (t > sc.thresh) ? 1.0f : (float)(0.8 * (double)fmaxf(0.0f, dot(st.norm, ldir)));
+        const float a = 1.0f * ds;
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = a + clr[3] * (clr[k] - a);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = sc.backclr[k] + clr[3] * (clr[k] - sc.backclr[k]);
+    }
+    const float4 of = make_float4(o[0], o[1], o[2], o[3]);
+    const int64_t px = (int64_t)y * cam.width + x;
+    if (rgba) rgba[px] = of;
+    if (rgba8) rgba8[px] = to_u8_opaque(of);
+}
+
+// Render(SHADE_OFF) VG.cpp:4340-4345: the image is cleared
+__global__ __launch_bounds__(256) void k_render_off(int64_t n, float4* rgba, uchar4* rgba8) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (rgba) rgba[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (rgba8) rgba8[i] = make_uchar4(0, 0, 0, 0);
+    }
+}
+
+// gvdbRaytrace MOD.cu:207-219: each ray's trilinear surface hit, pulled back
+// by bias along the ray; the normal is written only on a hit
+template <int BR>
+__global__ __launch_bounds__(256) void k_raytrace(RenderTree tv, RenderScene sc, GScnRay* rays, int64_t n, float bias) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    GScnRay& r = rays[i];
+    const V3 pos = v3(r.orig[0], r.orig[1], r.orig[2]), dir = v3(r.dir[0], r.dir[1], r.dir[2]);
+    RenderCam cam{};
+    RayState st;
+    st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
+    st.hit = v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT);
+    st.norm = v3(r.normal[0], r.normal[1], r.normal[2]);
+    VisitRec none{};
+    ray_cast<MODE_SURF_TRI, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
+    if (st.hit.z != GVDBX_NOHIT) st.hit = sub(st.hit, scale(dir, bias));
+    r.hit[0] = st.hit.x; r.hit[1] = st.hit.y; r.hit[2] = st.hit.z;
+    r.normal[0] = st.norm.x; r.normal[1] = st.norm.y; r.normal[2] = st.norm.z;
 }
 
 template <int BR, bool RECORD>
@@ -963,16 +1251,39 @@ void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, floa
                    hipStream_t s) {
     dim3 grid((unsigned)((cam.width + 15) / 16), (unsigned)((cam.height + 15) / 16));
     const bool b10 = tv.brick_res == 10;
-    if (shade == GVDBX_SHADE_VOLUME) {
-        if (b10) hipLaunchKernelGGL(k_render_volume<10>, grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
-        else hipLaunchKernelGGL(k_render_volume<0>, grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
-    } else if (shade == GVDBX_SHADE_TRILINEAR) {
-        if (b10) hipLaunchKernelGGL((k_render_surface<MODE_SURF_TRI, 10>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
-        else hipLaunchKernelGGL((k_render_surface<MODE_SURF_TRI, 0>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
-    } else {
-        if (b10) hipLaunchKernelGGL((k_render_surface<MODE_SURF_LS, 10>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
-        else hipLaunchKernelGGL((k_render_surface<MODE_SURF_LS, 0>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);
+#define GVDBX_RENDER(K)                                                                          \
+    do {                                                                                         \
+        if (b10) hipLaunchKernelGGL(K<10>, grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);    \
+        else hipLaunchKernelGGL(K<0>, grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);          \
+    } while (0)
+#define GVDBX_RENDER_SURF(M)                                                                                    \
+    do {                                                                                                        \
+        if (b10) hipLaunchKernelGGL((k_render_surface<M, 10>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8); \
+        else hipLaunchKernelGGL((k_render_surface<M, 0>), grid, dim3(256), 0, s, tv, cam, sc, rgba, rgba8);      \
+    } while (0)
+    switch (shade) {
+        case GVDBX_SHADE_VOLUME: GVDBX_RENDER(k_render_volume); break;
+        case GVDBX_SHADE_TRILINEAR: GVDBX_RENDER_SURF(MODE_SURF_TRI); break;
+        case GVDBX_SHADE_LEVELSET: GVDBX_RENDER_SURF(MODE_SURF_LS); break;
+        case GVDBX_SHADE_VOXEL: GVDBX_RENDER_SURF(MODE_SURF_VOXEL); break;
+        case GVDBX_SHADE_TRICUBIC: GVDBX_RENDER_SURF(MODE_SURF_CUBIC); break;
+        case GVDBX_SHADE_EMPTYSKIP: GVDBX_RENDER(k_render_empty); break;
+        case GVDBX_SHADE_SECTION2D: GVDBX_RENDER(k_render_section2d); break;
+        case GVDBX_SHADE_SECTION3D: GVDBX_RENDER(k_render_section3d); break;
+        default: {
+            const int64_t n = (int64_t)cam.width * cam.height;
+            hipLaunchKernelGGL(k_render_off, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rgba, rgba8);
+        }
     }
+#undef GVDBX_RENDER
+#undef GVDBX_RENDER_SURF
+}
+
+void launch_raytrace(RenderTree tv, RenderScene sc, GScnRay* rays, int64_t n, float bias, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (tv.brick_res == 10) hipLaunchKernelGGL(k_raytrace<10>, dim3(g), dim3(256), 0, s, tv, sc, rays, n, bias);
+    else hipLaunchKernelGGL(k_raytrace<0>, dim3(g), dim3(256), 0, s, tv, sc, rays, n, bias);
 }
 
 void launch_raycast_ea(RenderTree tv, RenderCam cam, const RenderCam* cams, int ncam, RenderScene sc, float* I,

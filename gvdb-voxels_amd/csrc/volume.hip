@@ -1377,6 +1377,9 @@ int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_
     sc.shadow_amt = rp->shadow_amt;
     sc.shadow_bias = rp->shadow_bias;
     sc.lut = (const float4*)rp->lut_dev;
+    sc.finestep = rp->finestep;
+    memcpy(sc.slice_pnt, rp->slice_pnt, sizeof(sc.slice_pnt));
+    memcpy(sc.slice_norm, rp->slice_norm, sizeof(sc.slice_norm));
     return GVDBX_OK;
 }
 
@@ -1985,17 +1988,48 @@ int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, 
                  float* rgba_dev, uint8_t* rgba8_dev) {
     TRY(check_vol(v));
     if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "Render: not available for batched clouds");
+    if (!(shade >= GVDBX_SHADE_VOXEL && shade <= GVDBX_SHADE_VOLUME) && shade != GVDBX_SHADE_OFF)
+        FAIL(GVDBX_ERR_ARG, "Render: unknown shading %d", shade);
     RenderTree tv;
     RenderCam rc;
     RenderScene sc;
-    TRY(make_render(v, chan, cam, rp, "Render", tv, rc, sc));
-    if (shade != GVDBX_SHADE_VOLUME && shade != GVDBX_SHADE_TRILINEAR && shade != GVDBX_SHADE_LEVELSET)
-        FAIL(GVDBX_ERR_ARG, "Render: shading %d is not supported (VOLUME, TRILINEAR, LEVELSET)", shade);
-    if (shade == GVDBX_SHADE_VOLUME && !rp->lut_dev) FAIL(GVDBX_ERR_ARG, "Render: SHADE_VOLUME needs a transfer function");
-    if (((uintptr_t)rp->lut_dev & 15) != 0) FAIL(GVDBX_ERR_ARG, "Render: transfer function must be 16-byte aligned");
+    if (shade == GVDBX_SHADE_OFF) {   // VG.cpp:4340-4345: cleared before anything is prepared
+        if (!cam) FAIL(GVDBX_ERR_ARG, "Render: null camera");
+        if (cam->width <= 0 || cam->height <= 0 || (int64_t)cam->width * cam->height >= ((int64_t)1 << 31))
+            FAIL(GVDBX_ERR_ARG, "Render: bad image size %dx%d", cam->width, cam->height);
+        memset(&tv, 0, sizeof(tv));
+        memset(&rc, 0, sizeof(rc));
+        memset(&sc, 0, sizeof(sc));
+        rc.width = cam->width;
+        rc.height = cam->height;
+    } else {
+        TRY(make_render(v, chan, cam, rp, "Render", tv, rc, sc));
+        const bool lut = shade == GVDBX_SHADE_VOLUME || shade == GVDBX_SHADE_SECTION2D || shade == GVDBX_SHADE_SECTION3D;
+        if (lut && !rp->lut_dev) FAIL(GVDBX_ERR_ARG, "Render: shading %d needs a transfer function", shade);
+        if (((uintptr_t)rp->lut_dev & 15) != 0) FAIL(GVDBX_ERR_ARG, "Render: transfer function must be 16-byte aligned");
+        if (shade == GVDBX_SHADE_TRICUBIC && !(rp->finestep > 0.0f && isfinite(rp->finestep)))
+            FAIL(GVDBX_ERR_ARG, "Render: SHADE_TRICUBIC needs finestep > 0");
+    }
     if (!rgba_dev && !rgba8_dev) return GVDBX_OK;
     launch_render(shade, tv, rc, sc, (float4*)rgba_dev, (uchar4*)rgba8_dev, v->stream);
     return check_launch("Render");
+}
+
+int gvdbx_raytrace(gvdbx_volume* v, int chan, const gvdbx_render_params* rp, gvdbx_scn_ray* rays_dev, int64_t num_rays,
+                   float bias) {
+    TRY(check_vol(v));
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "Raytrace: not available for batched clouds");
+    if (num_rays < 0 || (num_rays > 0 && !rays_dev)) FAIL(GVDBX_ERR_ARG, "Raytrace: bad ray buffer");
+    if ((uintptr_t)rays_dev & 15) FAIL(GVDBX_ERR_ARG, "Raytrace: ray records must be 16-byte aligned");
+    gvdbx_camera one;
+    memset(&one, 0, sizeof(one));
+    one.width = one.height = 1;   // PrepareRender(1, 1, 0) VG.cpp:4390
+    RenderTree tv;
+    RenderCam rc;
+    RenderScene sc;
+    TRY(make_render(v, chan, &one, rp, "Raytrace", tv, rc, sc));
+    launch_raytrace(tv, sc, (GScnRay*)rays_dev, num_rays, bias, v->stream);
+    return check_launch("Raytrace");
 }
 
 }  // extern "C"

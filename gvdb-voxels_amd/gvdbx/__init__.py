@@ -21,7 +21,8 @@ NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("
 AMAP_DTYPE = np.dtype([("mPos", "<i4", 3), ("mLeafID", "<i4")])
 T_UCHAR4, T_FLOAT = 2, 3   # gvdb_types.h:75-76
 F_LINEAR, F_CLAMP = 1, 1
-SHADE_TRILINEAR, SHADE_LEVELSET, SHADE_VOLUME = 4, 6, 7   # gvdb_types.h:109-112
+SHADE_VOXEL, SHADE_SECTION2D, SHADE_SECTION3D, SHADE_EMPTYSKIP = 0, 1, 2, 3   # gvdb_types.h:105-108
+SHADE_TRILINEAR, SHADE_TRICUBIC, SHADE_LEVELSET, SHADE_VOLUME, SHADE_OFF = 4, 5, 6, 7, 100   # gvdb_types.h:109-115
 
 
 def _v3(x):
@@ -68,6 +69,9 @@ def render_params(**kw):
     p.light = _v3(kw.get("light", (0, 0, 0)))
     p.shadow_amt = float(kw.get("shadow_amt", 0.8))
     p.shadow_bias = float(kw.get("shadow_bias", 1.0))
+    p.finestep = float(kw.get("finestep", 0.1))
+    p.slice_pnt = _v3(kw.get("slice_pnt", (0, 0, 0)))
+    p.slice_norm = _v3(kw.get("slice_norm", (0, 0, 1)))
     lut = kw.get("lut")
     p._lut = lut
     p.lut_dev = _ptr(lut) if lut is not None else None
@@ -258,6 +262,13 @@ class VolumeGVDB:
         call("gvdbx_render", self._h, int(shade), int(chan), C.byref(cam), C.byref(rp), C.c_void_p(_ptr(rgba)),
              C.c_void_p(_ptr(rgba8)))
         return rgba, rgba8
+
+    def Raytrace(self, rays, chan, rp, bias=0.0):
+        """Raytrace (VG.cpp:4384-4408) over a device uint8 tensor of 64-byte
+        ray records (gvdbx_scn_ray), updated in place."""
+        assert rays.is_cuda and rays.numel() % 64 == 0
+        call("gvdbx_raytrace", self._h, int(chan), C.byref(rp), C.c_void_p(_ptr(rays)), rays.numel() // 64, float(bias))
+        return rays
 
     def RaycastEA(self, chan, cam, rp, I=None, T=None, record=False):
         """Linear-transfer emission-absorption raycast; (I, T) float32 (H,W)."""

@@ -39,7 +39,8 @@ class RenderParams(C.Structure):
     _fields_ = [("step", C.c_float), ("extinct", C.c_float), ("albedo", C.c_float), ("minval", C.c_float),
                 ("alphacut", C.c_float), ("thresh", C.c_float), ("vmin", C.c_float), ("vmax", C.c_float),
                 ("epsilon", C.c_float), ("max_iter", C.c_int), ("backclr", C.c_float * 4), ("light", C.c_float * 3),
-                ("shadow_amt", C.c_float), ("shadow_bias", C.c_float), ("lut_dev", C.c_void_p)]
+                ("shadow_amt", C.c_float), ("shadow_bias", C.c_float), ("lut_dev", C.c_void_p),
+                ("finestep", C.c_float), ("slice_pnt", C.c_float * 3), ("slice_norm", C.c_float * 3)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)
@@ -59,6 +60,7 @@ _SIGS = {
     "gvdbx_set_points": [_vp, _vp, _i, _i, _vp, _i, _i, _vp, _i, _i],
     "gvdbx_set_point_weights": [_vp, _vp, _i],
     "gvdbx_rebuild_topology": [_vp, _i, _f, _f, _f, _f],
+    "gvdbx_raytrace": [_vp, _i, _vp, _vp, _i64, _f],
     "gvdbx_accumulate_topology": [_vp, _i, _f, _f, _f, _f, _i],
     "gvdbx_request_full_rebuild": [_vp, _i],
     "gvdbx_finish_topology": [_vp, _i, _i],

@@ -180,6 +180,10 @@ public:
                 float* rgba_dev, uint8_t* rgba8_dev = nullptr) {
         check(gvdbx_render(v_, shade, chan, &cam, &rp, rgba_dev, rgba8_dev), "Render");
     }
+    void Raytrace(gvdbx_scn_ray* rays_dev, int64_t num_rays, unsigned char chan, const gvdbx_render_params& rp,
+                  float bias) {
+        check(gvdbx_raytrace(v_, chan, &rp, rays_dev, num_rays, bias), "Raytrace");
+    }
 
     // ---- differentiable forms (no reference counterpart) ----
     void RaycastEA(unsigned char chan, const gvdbx_camera& cam, const gvdbx_render_params& rp, float* I_dev,

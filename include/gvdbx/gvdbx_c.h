@@ -232,9 +232,15 @@ int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
 
 /* ---- rendering (RC.cuh, DDA.cuh, GEO.cuh, MOD.cu) ---- */
 enum gvdbx_shade {
+    GVDBX_SHADE_VOXEL = 0,       /* gvdb_types.h:105 */
+    GVDBX_SHADE_SECTION2D = 1,   /* gvdb_types.h:106 */
+    GVDBX_SHADE_SECTION3D = 2,   /* gvdb_types.h:107 */
+    GVDBX_SHADE_EMPTYSKIP = 3,   /* gvdb_types.h:108 */
     GVDBX_SHADE_TRILINEAR = 4,   /* gvdb_types.h:109 */
+    GVDBX_SHADE_TRICUBIC = 5,    /* gvdb_types.h:110 */
     GVDBX_SHADE_LEVELSET = 6,    /* gvdb_types.h:111 */
-    GVDBX_SHADE_VOLUME = 7       /* gvdb_types.h:112 */
+    GVDBX_SHADE_VOLUME = 7,      /* gvdb_types.h:112 */
+    GVDBX_SHADE_OFF = 100        /* gvdb_types.h:115 */
 };
 
 /* Camera in index space.  Pixel (x,y), u = (x+0.5)/width, w = (y+0.5)/height:
@@ -263,16 +269,35 @@ typedef struct {
     float backclr[4];    /* SCN_BACKCLR */
     float light[3];      /* light position, index space */
     float shadow_amt, shadow_bias;   /* SCN_SHADOWAMT, SCN_SHADOWBIAS [0.8, 1.0] */
-    const float* lut_dev;   /* transfer function: 16384 float4 in device memory (SHADE_VOLUME) */
+    const float* lut_dev;   /* transfer function: 16384 float4 in device memory (SHADE_VOLUME, sections) */
+    float finestep;         /* SCN_FINESTEP [0.1] (SHADE_TRICUBIC) */
+    float slice_pnt[3], slice_norm[3];   /* SCN_SLICE_PNT / SCN_SLICE_NORM (SetCrossSection, sections) */
 } gvdbx_render_params;
 
+/* ScnRay (cuda_gvdb_scene.cuh:66-74): the 64-byte ray record of Raytrace */
+typedef struct {
+    float hit[3], normal[3], orig[3], dir[3];
+    uint32_t clr, pnode, pndx, pad;
+} gvdbx_scn_ray;
+
 /* Render(char shade, uchar chan, uchar buf) VG.h:341 / VG.cpp:4336-4381 with
- * ReadRenderBuf: gvdbRayDeep (SHADE_VOLUME), gvdbRaySurfaceTrilinear,
- * gvdbRayLevelSet (MOD.cu:60-181).  rgba_dev (optional): 4 floats per pixel,
- * the colour before quantisation; rgba8_dev (optional): the reference's
- * uchar4 output.  Rows of width pixels, y-major. */
+ * ReadRenderBuf, every shading of the reference's switch: gvdbRayDeep
+ * (SHADE_VOLUME), gvdbRaySurfaceVoxel / Trilinear / Tricubic, gvdbRayLevelSet,
+ * gvdbRayEmptySkip, gvdbSection2D / 3D (MOD.cu:60-293) and SHADE_OFF (the
+ * image cleared).  No colour channel is applied.  rgba_dev (optional): 4
+ * floats per pixel, the colour before quantisation; rgba8_dev (optional):
+ * the reference's uchar4 output (alpha 255 for EMPTYSKIP and the sections,
+ * as there).  Rows of width pixels, y-major.  Tricubic taps past a padded
+ * brick clamp to it (the reference reads into the neighbouring atlas brick). */
 int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
                  float* rgba_dev, uint8_t* rgba8_dev);
+/* Raytrace(DataPtr rays, uchar chan, char, int, float bias) VG.h:342 /
+ * VG.cpp:4384-4408 (gvdbRaytrace MOD.cu:207-219): for every ray record in
+ * device memory the trilinear surface hit (orig + t dir, first sample >=
+ * thresh), pulled back by bias along dir, or (NOHIT, NOHIT, NOHIT); the
+ * normal is written only on a hit. */
+int gvdbx_raytrace(gvdbx_volume* v, int chan, const gvdbx_render_params* rp, gvdbx_scn_ray* rays_dev, int64_t num_rays,
+                   float bias);
 /* Emission-absorption raycast with the linear transfer s -> (s,s,s,s)
  * (extension for differentiation; same traversal and sample set as
  * SHADE_VOLUME, rayDeepBrick RC.cuh:485-533, no final clamp):

@@ -1396,6 +1396,19 @@ const or_atlas_node* or_atlas_map(or_volume* v, int64_t* count) {
 }
 const float* or_atlas(or_volume* v, int chan) { return (v && chan >= 0 && chan < OR_MAXCHAN) ? v->atlas[chan] : NULL; }
 
+/* test input: overwrite one leaf's padded brick (the inverse of or_read_brick) */
+int or_write_brick(or_volume* v, int chan, int64_t leaf, const float* in) {
+    if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan] || leaf < 0 || leaf >= v->ncnt[0]) return OR_ERR_ARG;
+    int br = brick_res(v);
+    const or_node* nd = &v->nodes[0][leaf];
+    int64_t c0 = nd->mValue[0] - v->apron, c1 = nd->mValue[1] - v->apron, c2 = nd->mValue[2] - v->apron;
+    for (int z = 0; z < br; z++)
+        for (int y = 0; y < br; y++)
+            for (int x = 0; x < br; x++)
+                v->atlas[chan][atlas_index(v, c0 + x, c1 + y, c2 + z)] = in[((int64_t)z * br + y) * br + x];
+    return OR_OK;
+}
+
 int or_read_brick(or_volume* v, int chan, int64_t leaf, float* out) {
     if (!v || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan] || leaf < 0 || leaf >= v->ncnt[0]) return OR_ERR_ARG;
     int br = brick_res(v);
@@ -1436,7 +1449,8 @@ static f3 f3scale(f3 a, float s) { return f3m(a.x * s, a.y * s, a.z * s); }
 static float f3dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static f3 f3norm(f3 a) { return f3scale(a, 1.0f / sqrtf(f3dot(a, a))); }
 
-enum { M_DEEP_LUT = 0, M_DEEP_LINEAR = 1, M_SURF_TRI = 2, M_SURF_LS = 3 };
+enum { M_DEEP_LUT = 0, M_DEEP_LINEAR = 1, M_SURF_TRI = 2, M_SURF_LS = 3, M_SURF_VOXEL = 4, M_SURF_CUBIC = 5,
+       M_EMPTY = 6 };
 
 typedef struct {
     int64_t leaf;
@@ -1573,6 +1587,21 @@ static void hdda_prepare(hdda* d, f3 vmin, float vdel) {
         float fl = floorf(pf[a]);
         float br = ((fl - pf[a]) + 0.5f) * (float)d->ps[a] + 0.5f;
         d->ts[a] = isinf(d->tdel[a]) ? INFINITY : br * d->tdel[a] + d->t.x;
+        d->p[a] = (int)fl;
+    }
+}
+
+/* PrepareLeaf DDA.cuh:67-73: vdel 1, and tSide without the "+ t.x" of
+ * Prepare (only the order of the steps matters inside a brick) */
+static void hdda_prepare_leaf(hdda* d, f3 vmin) {
+    float dir[3] = {d->dir.x, d->dir.y, d->dir.z};
+    f3 q = f3sub(f3add(d->pos, f3scale(d->dir, d->t.x)), vmin);
+    float pf[3] = {q.x, q.y, q.z};
+    for (int a = 0; a < 3; a++) {
+        d->tdel[a] = fabsf(1.0f / dir[a]);
+        float fl = floorf(pf[a]);
+        float br = ((fl - pf[a]) + 0.5f) * (float)d->ps[a] + 0.5f;
+        d->ts[a] = isinf(d->tdel[a]) ? INFINITY : br * d->tdel[a];
         d->p[a] = (int)fl;
     }
 }
@@ -1737,11 +1766,125 @@ static void brick_surf_ls(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
     }
 }
 
+/* raySurfaceVoxelBrick RC.cuh:227-263: the leaf's voxels in DDA order, the
+ * first one above the threshold is hit as a unit cube (its exact value: the
+ * reference samples the texel centre, p + o + 0.5); a missed cube repeats
+ * the same voxel (no step) until the iteration limit, as the reference's
+ * `continue` does. */
+static void brick_surf_voxel(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
+    const or_volume* v = c->v;
+    const or_render_params* rp = c->rp;
+    const or_node* nd = &v->nodes[0][leaf];
+    f3 vmin = f3m((float)nd->mPos[0], (float)nd->mPos[1], (float)nd->mPos[2]);
+    int r0 = v->res[0], a = v->apron;
+    hdda d;
+    hdda_set(&d, pos, dir, t);
+    hdda_prepare_leaf(&d, vmin);
+    for (int iter = 0; iter < rp->max_iter && d.p[0] >= 0 && d.p[1] >= 0 && d.p[2] >= 0 && d.p[0] < r0 &&
+                       d.p[1] < r0 && d.p[2] < r0;
+         iter++) {
+        if (tri_at32(c, nd, d.p[0] + a, d.p[1] + a, d.p[2] + a) > rp->thresh) {
+            vmin = f3add(vmin, f3m((float)d.p[0], (float)d.p[1], (float)d.p[2]));
+            f3 tb = ray_box(pos, dir, vmin, f3add(vmin, f3m(1.0f, 1.0f, 1.0f)));
+            d.t = tb;
+            if (tb.z == OR_NOHIT) {
+                c->hit.z = OR_NOHIT;
+                continue;
+            }
+            f3 hit = f3add(pos, f3scale(dir, tb.x));
+            f3 fc = f3sub(f3sub(hit, vmin), f3m(0.5f, 0.5f, 0.5f));
+            fc = f3sub(fc, f3scale(dir, 0.01f));
+            float m = fmaxf(fmaxf(fabsf(fc.x), fabsf(fc.y)), fabsf(fc.z));
+            c->norm = f3m(fabsf(fc.x) == m ? copysignf(1.0f, fc.x) : 0.0f, fabsf(fc.y) == m ? copysignf(1.0f, fc.y) : 0.0f,
+                          fabsf(fc.z) == m ? copysignf(1.0f, fc.z) : 0.0f);
+            c->hit = hit;
+            c->hit_leaf = leaf;
+            c->hit_p = f3sub(hit, vmin);
+            return;
+        }
+        hdda_next(&d);
+        hdda_step(&d);
+    }
+}
+
+/* getTricubic RC.cuh:32-98: 27 filtered taps at the integer points
+ * floor(p) - 1 + {0, 1, 2} per axis (each the trilinear sample there, i.e.
+ * the mean of 8 voxels), combined with the weights ta^2, 2 ta tb, tb^2,
+ * tb = frac(p)/2 + 1/4, x then y then z.  Brick-local coordinates: the
+ * reference forms floor(p + atlas offset) in FP32 and reads past the padded
+ * brick into its atlas neighbour, where the taps here clamp to the brick. */
+static float tricubic32(const ray_ctx* c, int64_t leaf, f3 p) {
+    float fl[3] = {floorf(p.x), floorf(p.y), floorf(p.z)};
+    float pv[3] = {p.x, p.y, p.z};
+    float ta2[3], tab[3], tb2[3];
+    for (int k = 0; k < 3; k++) {
+        float tb = (pv[k] - fl[k]) * 0.5f + 0.25f;
+        float ta = 1.0f - tb;
+        ta2[k] = ta * ta;
+        tb2[k] = tb * tb;
+        tab[k] = ta * tb * 2.0f;
+    }
+    float q[3] = {fl[0] - 1.0f, fl[1] - 1.0f, fl[2] - 1.0f};
+    float row[3][3];   /* [z][y]: the x-combined value of each (y, z) row */
+    for (int zz = 0; zz < 3; zz++)
+        for (int yy = 0; yy < 3; yy++) {
+            float t0 = tri32(c, leaf, f3m(q[0], q[1] + (float)yy, q[2] + (float)zz));
+            float t1 = tri32(c, leaf, f3m(q[0] + 1.0f, q[1] + (float)yy, q[2] + (float)zz));
+            float t2 = tri32(c, leaf, f3m(q[0] + 2.0f, q[1] + (float)yy, q[2] + (float)zz));
+            row[zz][yy] = t0 * ta2[0] + t1 * tab[0] + t2 * tb2[0];
+        }
+    float jkl[3];
+    for (int zz = 0; zz < 3; zz++) jkl[zz] = row[zz][0] * ta2[1] + row[zz][1] * tab[1] + row[zz][2] * tb2[1];
+    return jkl[0] * ta2[2] + jkl[1] * tab[2] + jkl[2] * tb2[2];
+}
+
+/* getGradientTricubic RC.cuh:161-171 (the negated gradient, half-voxel
+ * central differences) */
+static f3 gradient_cubic32(const ray_ctx* c, int64_t leaf, f3 p) {
+    const float vs = 0.5f;
+    f3 g;
+    g.x = (tricubic32(c, leaf, f3add(p, f3m(-vs, 0, 0))) - tricubic32(c, leaf, f3add(p, f3m(vs, 0, 0)))) / (2 * vs);
+    g.y = (tricubic32(c, leaf, f3add(p, f3m(0, -vs, 0))) - tricubic32(c, leaf, f3add(p, f3m(0, vs, 0)))) / (2 * vs);
+    g.z = (tricubic32(c, leaf, f3add(p, f3m(0, 0, -vs))) - tricubic32(c, leaf, f3add(p, f3m(0, 0, vs)))) / (2 * vs);
+    return f3norm(g);
+}
+
+/* raySurfaceTricubicBrick RC.cuh:316-340: march from the entry point (no
+ * snap to the step grid), at the first sample >= thresh step back by the
+ * linear estimate of the crossing over one fine step */
+static void brick_surf_cubic(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
+    const or_volume* v = c->v;
+    const or_render_params* rp = c->rp;
+    const or_node* nd = &v->nodes[0][leaf];
+    f3 vmin = f3m((float)nd->mPos[0], (float)nd->mPos[1], (float)nd->mPos[2]);
+    f3 p = f3sub(f3add(pos, f3scale(dir, t.x)), vmin);
+    float r0 = (float)v->res[0];
+    for (int iter = 0; iter < rp->max_iter && p.x >= 0 && p.y >= 0 && p.z >= 0 && p.x < r0 && p.y < r0 && p.z < r0;
+         iter++) {
+        float vz = tricubic32(c, leaf, p);
+        if (vz >= rp->thresh) {
+            float vx = tricubic32(c, leaf, f3sub(p, f3scale(dir, rp->finestep)));
+            float vy = (vz - rp->thresh) / (vz - vx);
+            p = f3add(p, f3scale(dir, -vy * rp->finestep));
+            c->hit = f3add(p, vmin);
+            c->norm = gradient_cubic32(c, leaf, p);
+            c->hit_leaf = leaf;
+            c->hit_p = p;
+            return;
+        }
+        p = f3add(p, f3scale(dir, rp->step));
+        t.x += rp->step;
+    }
+}
+
 static void brick_call(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
     switch (c->mode) {
         case M_DEEP_LUT:
         case M_DEEP_LINEAR: brick_deep(c, leaf, t, pos, dir); break;
         case M_SURF_TRI: brick_surf_tri(c, leaf, t, pos, dir); break;
+        case M_SURF_VOXEL: brick_surf_voxel(c, leaf, t, pos, dir); break;
+        case M_SURF_CUBIC: brick_surf_cubic(c, leaf, t, pos, dir); break;
+        case M_EMPTY: c->hit = f3add(pos, f3scale(dir, t.x)); break;   /* rayEmptySkipBrick RC.cuh:425-428 */
         default: brick_surf_ls(c, leaf, t, pos, dir); break;
     }
 }
@@ -1840,15 +1983,22 @@ static void phong(ray_ctx* c, f3 pos_unused, float out[4], double out64[4]) {
     f3 ldir = f3norm(f3sub(light, shit));
     float diff = (float)(0.9 * (double)fmaxf(0.0f, f3dot(snorm, ldir)));
     float amb = 0.1f;
-    /* FP64 shading at the same hit point */
-    double g[3];
-    gradient64(c, hl, hp, c->mode == M_SURF_LS ? +1 : -1, g);
-    double L[3] = {(double)rp->light[0] - shit.x, (double)rp->light[1] - shit.y, (double)rp->light[2] - shit.z};
-    double ln = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
-    double dd = (g[0] * L[0] + g[1] * L[1] + g[2] * L[2]) / ln;
-    double diff64 = 0.9 * (dd > 0 ? dd : 0.0);
+    /* FP64 shading at the same hit point (trilinear / level-set normals; the
+     * other surfaces repeat the FP32 value) */
+    const int fp64 = c->mode == M_SURF_TRI || c->mode == M_SURF_LS;
+    double diff64 = diff;
+    if (fp64) {
+        double g[3];
+        gradient64(c, hl, hp, c->mode == M_SURF_LS ? +1 : -1, g);
+        double L[3] = {(double)rp->light[0] - shit.x, (double)rp->light[1] - shit.y, (double)rp->light[2] - shit.z};
+        double ln = sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+        double dd = (g[0] * L[0] + g[1] * L[1] + g[2] * L[2]) / ln;
+        diff64 = 0.9 * (dd > 0 ? dd : 0.0);
+    }
     if (rp->shadow_amt > 0) {
         ray_ctx s2 = *c;
+        /* gvdbRaySurfaceTricubic shades with the trilinear brick (MOD.cu:137) */
+        if (c->mode == M_SURF_CUBIC) s2.mode = M_SURF_TRI;
         s2.hit = f3m(0, 0, OR_NOHIT);
         s2.clr[0] = s2.clr[1] = s2.clr[2] = 0; s2.clr[3] = 1;
         s2.rec = NULL; s2.caprec = 0; s2.nrec = 0;
@@ -1862,8 +2012,76 @@ static void phong(ray_ctx* c, f3 pos_unused, float out[4], double out64[4]) {
     float sh = diff + amb;
     out[0] = out[1] = out[2] = sh;
     out[3] = 1.0f;
-    out64[0] = out64[1] = out64[2] = diff64 + 0.1;
+    out64[0] = out64[1] = out64[2] = fp64 ? diff64 + 0.1 : (double)sh;
     out64[3] = 1.0;
+}
+
+/* transfer(gvdb, v) with the LUT of the sections: entry transfer_index(v) */
+static void lut_at(const or_render_params* rp, float s, float out[4]) {
+    int i = transfer_index(rp, s);
+    for (int k = 0; k < 4; k++) out[k] = rp->lut[4 * i + k];
+}
+
+/* gvdbSection2D MOD.cu:268-293: pixel (x, y) samples the slice point
+ * pnt + ((2x/w - 1, 0, 2y/h - 1) * norm) (componentwise, the offset formed
+ * in double), black outside every leaf; colour = transfer colour * alpha */
+static void section2d(ray_ctx* c, const or_camera* cam, int x, int y, float o[4]) {
+    const or_render_params* rp = c->rp;
+    float sx = (float)((double)(float)x * 2.0 / (double)cam->width - 1.0);
+    float sz = (float)((double)(float)y * 2.0 / (double)cam->height - 1.0);
+    float w[3] = {rp->slice_pnt[0] + sx * rp->slice_norm[0], rp->slice_pnt[1] + 0.0f * rp->slice_norm[1],
+                  rp->slice_pnt[2] + sz * rp->slice_norm[2]};
+    int64_t leaf = point_leaf(c->v, w);
+    o[0] = o[1] = o[2] = 0.0f;
+    o[3] = 1.0f;
+    if (leaf < 0) return;
+    const or_node* nd = &c->v->nodes[0][leaf];
+    float s = tri32(c, leaf, f3m(w[0] - (float)nd->mPos[0], w[1] - (float)nd->mPos[1], w[2] - (float)nd->mPos[2]));
+    float clr[4];
+    lut_at(rp, s, clr);
+    for (int k = 0; k < 3; k++) o[k] = 0.0f + clr[3] * (clr[k] - 0.0f);
+}
+
+/* gvdbSection3D MOD.cu:222-265: the transfer colour where the ray meets the
+ * slice plane, blended over the trilinear surface found beyond it */
+static void section3d(ray_ctx* c, f3 pos, f3 dir, float o[4]) {
+    const or_render_params* rp = c->rp;
+    float clr[4] = {1, 1, 1, 0};
+    f3 wpos = pos;
+    const float* n = rp->slice_norm;
+    const float* q = rp->slice_pnt;
+    /* rayPlaneIntersect GEO.cuh:67-71 */
+    float t = ((q[0] - wpos.x) * n[0] + (q[1] - wpos.y) * n[1] + (q[2] - wpos.z) * n[2]) /
+              (dir.x * n[0] + dir.y * n[1] + dir.z * n[2]);
+    t = t > 0 ? t : OR_NOHIT;
+    if (t > 0) {
+        wpos = f3add(wpos, f3scale(dir, t));
+        float w[3] = {wpos.x, wpos.y, wpos.z};
+        int64_t leaf = point_leaf(c->v, w);
+        if (leaf >= 0) {
+            const or_node* nd = &c->v->nodes[0][leaf];
+            t = tri32(c, leaf, f3m(wpos.x - (float)nd->mPos[0], wpos.y - (float)nd->mPos[1], wpos.z - (float)nd->mPos[2]));
+            lut_at(rp, t, clr);
+        } else {
+            t = 0;
+        }
+    }
+    ray_ctx s = *c;
+    s.mode = M_SURF_TRI;
+    s.hit = f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT);
+    s.norm = f3m(0, 0, 0);
+    s.clr[0] = s.clr[1] = s.clr[2] = s.clr[3] = 1;
+    ray_cast(&s, wpos, dir);
+    if (s.hit.z != OR_NOHIT) {
+        f3 ldir = f3norm(f3sub(f3m(rp->light[0], rp->light[1], rp->light[2]), s.hit));
+        float ds = (t > rp->thresh) ? 1.0f : (float)(0.8 * (double)fmaxf(0.0f, f3dot(s.norm, ldir)));
+        for (int k = 0; k < 4; k++) {
+            float a = 1.0f * ds;
+            o[k] = a + clr[3] * (clr[k] - a);
+        }
+    } else {
+        for (int k = 0; k < 4; k++) o[k] = rp->backclr[k] + clr[3] * (clr[k] - rp->backclr[k]);
+    }
 }
 
 int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
@@ -1872,9 +2090,20 @@ int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_para
     if (st) return st;
     int mode;
     switch (rp->shade) {
+        case OR_SHADE_VOXEL: mode = M_SURF_VOXEL; break;
         case OR_SHADE_TRILINEAR: mode = M_SURF_TRI; break;
+        case OR_SHADE_TRICUBIC: mode = M_SURF_CUBIC; if (!(rp->finestep > 0)) return OR_ERR_ARG; break;
         case OR_SHADE_LEVELSET: mode = M_SURF_LS; break;
+        case OR_SHADE_EMPTYSKIP: mode = M_EMPTY; break;
+        case OR_SHADE_SECTION2D:
+        case OR_SHADE_SECTION3D: mode = M_SURF_TRI; if (!rp->lut) return OR_ERR_ARG; break;
         case OR_SHADE_VOLUME: mode = M_DEEP_LUT; if (!rp->lut) return OR_ERR_ARG; break;
+        case OR_SHADE_OFF:   /* Render VG.cpp:4340-4345: the buffer is cleared */
+            for (int64_t i = 0; i < (int64_t)cam->width * cam->height * 4; i++) {
+                if (out32) out32[i] = 0.0f;
+                if (out64) out64[i] = 0.0;
+            }
+            return OR_OK;
         default: return OR_ERR_ARG;
     }
     for (int y = 0; y < cam->height; y++)
@@ -1886,7 +2115,24 @@ int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_para
             camera_ray(cam, x, y, &pos, &dir);
             float o32[4];
             double o64[4];
-            if (mode == M_DEEP_LUT) {
+            if (rp->shade == OR_SHADE_SECTION2D) {
+                section2d(&c, cam, x, y, o32);
+                for (int k = 0; k < 4; k++) o64[k] = o32[k];
+            } else if (rp->shade == OR_SHADE_SECTION3D) {
+                section3d(&c, pos, dir, o32);
+                for (int k = 0; k < 4; k++) o64[k] = o32[k];
+            } else if (mode == M_EMPTY) {
+                /* gvdbRayEmptySkip MOD.cu:184-204: the first brick entry point / 100 */
+                c.hit = f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT);
+                c.clr[0] = c.clr[1] = c.clr[2] = c.clr[3] = 1;
+                ray_cast(&c, pos, dir);
+                if (c.hit.z != OR_NOHIT) {
+                    o32[0] = c.hit.x * 0.01f; o32[1] = c.hit.y * 0.01f; o32[2] = c.hit.z * 0.01f; o32[3] = 1.0f;
+                } else {
+                    for (int k = 0; k < 4; k++) o32[k] = rp->backclr[k];
+                }
+                for (int k = 0; k < 4; k++) o64[k] = o32[k];
+            } else if (mode == M_DEEP_LUT) {
                 c.hit = f3m(0, 0, OR_NOHIT);
                 c.clr[0] = c.clr[1] = c.clr[2] = 0; c.clr[3] = 1;
                 c.clr64[0] = c.clr64[1] = c.clr64[2] = 0; c.clr64[3] = 1;
@@ -1901,7 +2147,7 @@ int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_para
                 o32[3] = a;
                 o64[3] = a64;
             } else {
-                c.hit = mode == M_SURF_TRI ? f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT) : f3m(0, 0, OR_NOHIT);
+                c.hit = mode == M_SURF_LS ? f3m(0, 0, OR_NOHIT) : f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT);
                 c.clr[0] = c.clr[1] = c.clr[2] = c.clr[3] = 1;
                 ray_cast(&c, pos, dir);
                 phong(&c, pos, o32, o64);
@@ -1915,6 +2161,31 @@ int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_para
 
 /* Emission-absorption raycast with the linear transfer (extension): per
  * sample s, a = exp(extinct*s*step), I += s*T*(1-a)*albedo, T *= a. */
+/* gvdbRaytrace MOD.cu:207-219 */
+int or_raytrace(or_volume* v, int chan, const or_render_params* rp, or_scn_ray* rays, int64_t n, float bias) {
+    or_camera one;
+    memset(&one, 0, sizeof(one));
+    one.width = one.height = 1;
+    int st = render_check(v, chan, &one, rp);
+    if (st) return st;
+    if (n < 0 || (n > 0 && !rays)) return OR_ERR_ARG;
+    for (int64_t i = 0; i < n; i++) {
+        or_scn_ray* r = &rays[i];
+        ray_ctx c;
+        memset(&c, 0, sizeof(c));
+        c.v = v; c.A = v->atlas[chan]; c.rp = rp; c.mode = M_SURF_TRI;
+        c.hit = f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT);
+        c.norm = f3m(r->normal[0], r->normal[1], r->normal[2]);
+        c.clr[0] = c.clr[1] = c.clr[2] = c.clr[3] = 1;
+        f3 o = f3m(r->orig[0], r->orig[1], r->orig[2]), d = f3m(r->dir[0], r->dir[1], r->dir[2]);
+        ray_cast(&c, o, d);
+        if (c.hit.z != OR_NOHIT) c.hit = f3sub(c.hit, f3scale(d, bias));
+        r->hit[0] = c.hit.x; r->hit[1] = c.hit.y; r->hit[2] = c.hit.z;
+        r->normal[0] = c.norm.x; r->normal[1] = c.norm.y; r->normal[2] = c.norm.z;
+    }
+    return OR_OK;
+}
+
 int or_raycast_ea(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
                   float* I32, float* T32, double* I64, double* T64) {
     int st = render_check(v, chan, cam, rp);

@@ -146,14 +146,22 @@ const or_node* or_nodes(or_volume* v, int lev);
 const uint64_t* or_child_lists(or_volume* v, int lev);
 const or_atlas_node* or_atlas_map(or_volume* v, int64_t* count);
 /* copy leaf k's full brick (res0+2a)^3, x fastest, into out */
+/* test input: overwrite one leaf's padded brick (inverse of or_read_brick) */
+int or_write_brick(or_volume* v, int chan, int64_t leaf, const float* in);
 int or_read_brick(or_volume* v, int chan, int64_t leaf, float* out);
 /* pointer to the full 3-D atlas, dims subdim*(res0+2a), x fastest */
 const float* or_atlas(or_volume* v, int chan);
 
 /* ---- raycast (RC.cuh, DDA.cuh, GEO.cuh, MOD.cu) ---- */
+#define OR_SHADE_VOXEL 0       /* gvdb_types.h:105 */
+#define OR_SHADE_SECTION2D 1   /* gvdb_types.h:106 */
+#define OR_SHADE_SECTION3D 2   /* gvdb_types.h:107 */
+#define OR_SHADE_EMPTYSKIP 3   /* gvdb_types.h:108 */
 #define OR_SHADE_TRILINEAR 4   /* gvdb_types.h:109 */
+#define OR_SHADE_TRICUBIC 5    /* gvdb_types.h:110 */
 #define OR_SHADE_LEVELSET 6    /* gvdb_types.h:111 */
 #define OR_SHADE_VOLUME 7      /* gvdb_types.h:112 */
+#define OR_SHADE_OFF 100       /* gvdb_types.h:115 */
 
 /* Camera: the ray of pixel (x,y), u = (x+0.5)/width, w = (y+0.5)/height, has
  *   origin    = (u*org_u + w*org_v) + org
@@ -181,8 +189,16 @@ typedef struct {
     float backclr[4];   /* SCN_BACKCLR */
     float light[3];     /* light position in index space */
     float shadow_amt, shadow_bias;   /* SCN_SHADOWAMT, SCN_SHADOWBIAS */
-    const float* lut;   /* transfer function, 16384 x float4 (SHADE_VOLUME) */
+    const float* lut;   /* transfer function, 16384 x float4 (SHADE_VOLUME, sections) */
+    float finestep;     /* SCN_FINESTEP [0.1] (SHADE_TRICUBIC) */
+    float slice_pnt[3], slice_norm[3];   /* SCN_SLICE_PNT / SCN_SLICE_NORM (sections) */
 } or_render_params;
+
+/* ScnRay (cuda_gvdb_scene.cuh:66-74): 64-byte ray record of Raytrace */
+typedef struct {
+    float hit[3], normal[3], orig[3], dir[3];
+    uint32_t clr, pnode, pndx, pad;
+} or_scn_ray;
 
 /* Render(shade) VG.cpp:4336-4381: gvdbRayDeep / gvdbRaySurfaceTrilinear /
  * gvdbRayLevelSet (MOD.cu:60-181).  out32: 4 floats per pixel, the colour
@@ -192,6 +208,10 @@ typedef struct {
  * bricks to sample instead of the FP32 atlas (leaf-major, x fastest). */
 int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp, const double* bricks64,
               float* out32, double* out64);
+
+/* Raytrace VG.cpp:4384-4408 / gvdbRaytrace MOD.cu:207-219: every ray's hit
+ * with the trilinear surface brick, pulled back by bias along the ray */
+int or_raytrace(or_volume* v, int chan, const or_render_params* rp, or_scn_ray* rays, int64_t n, float bias);
 
 /* Emission-absorption raycast with the linear transfer s -> (s,s,s,s)
  * (extension; same traversal and samples as SHADE_VOLUME, no final clamp):

@@ -39,10 +39,18 @@ class OrRenderParams(C.Structure):
                 ("minval", C.c_float), ("alphacut", C.c_float), ("thresh", C.c_float), ("vmin", C.c_float),
                 ("vmax", C.c_float), ("epsilon", C.c_float), ("max_iter", C.c_int), ("backclr", C.c_float * 4),
                 ("light", C.c_float * 3), ("shadow_amt", C.c_float), ("shadow_bias", C.c_float),
-                ("lut", C.POINTER(C.c_float))]
+                ("lut", C.POINTER(C.c_float)), ("finestep", C.c_float), ("slice_pnt", C.c_float * 3),
+                ("slice_norm", C.c_float * 3)]
 
 
-SHADE_TRILINEAR, SHADE_LEVELSET, SHADE_VOLUME = 4, 6, 7
+class OrScnRay(C.Structure):
+    _fields_ = [("hit", C.c_float * 3), ("normal", C.c_float * 3), ("orig", C.c_float * 3), ("dir", C.c_float * 3),
+                ("clr", C.c_uint32), ("pnode", C.c_uint32), ("pndx", C.c_uint32), ("pad", C.c_uint32)]
+
+
+assert C.sizeof(OrScnRay) == 64
+SHADE_VOXEL, SHADE_SECTION2D, SHADE_SECTION3D, SHADE_EMPTYSKIP = 0, 1, 2, 3   # gvdb_types.h:105-108
+SHADE_TRILINEAR, SHADE_TRICUBIC, SHADE_LEVELSET, SHADE_VOLUME, SHADE_OFF = 4, 5, 6, 7, 100
 T_UCHAR4, T_FLOAT = 2, 3   # gvdb_types.h:75-76
 
 
@@ -74,6 +82,9 @@ def make_params(rp):
     p.light = (C.c_float * 3)(*[float(x) for x in rp.get("light", (0, 0, 0))])
     p.shadow_amt = float(rp.get("shadow_amt", 0.8))
     p.shadow_bias = float(rp.get("shadow_bias", 1.0))
+    p.finestep = float(rp.get("finestep", 0.1))
+    p.slice_pnt = (C.c_float * 3)(*[float(x) for x in rp.get("slice_pnt", (0, 0, 0))])
+    p.slice_norm = (C.c_float * 3)(*[float(x) for x in rp.get("slice_norm", (0, 0, 1))])
     lut = rp.get("lut")
     p._lut_keep = None
     if lut is not None:
@@ -89,6 +100,9 @@ NODE_DTYPE = np.dtype([("mLev", "u1"), ("mFlags", "u1"), ("mPriority", "u1"), ("
                        ("mParent", "<u8"), ("mChildList", "<u8"), ("mMask", "<u8")])
 assert NODE_DTYPE.itemsize == 64
 AMAP_DTYPE = np.dtype([("mPos", "<i4", 3), ("mLeafID", "<i4")])
+RAY_DTYPE = np.dtype([("hit", "<f4", 3), ("normal", "<f4", 3), ("orig", "<f4", 3), ("dir", "<f4", 3),
+                      ("clr", "<u4"), ("pnode", "<u4"), ("pndx", "<u4"), ("pad", "<u4")])
+assert RAY_DTYPE.itemsize == 64
 
 _lib = None
 
@@ -131,6 +145,8 @@ def lib():
         L.or_atlas.argtypes = [vp, i32]
         L.or_atlas.restype = vp
         L.or_read_brick.argtypes = [vp, i32, i64, fp]
+        L.or_write_brick.argtypes = [vp, i32, i64, fp]
+        L.or_raytrace.argtypes = [vp, i32, C.POINTER(OrRenderParams), C.POINTER(OrScnRay), i64, f32]
         cp, rpp = C.POINTER(OrCamera), C.POINTER(OrRenderParams)
         L.or_render.argtypes = [vp, i32, cp, rpp, dp, fp, dp]
         L.or_raycast_ea.argtypes = [vp, i32, cp, rpp, dp, fp, fp, dp, dp]
@@ -332,6 +348,21 @@ class OracleVolume:
             _chk(self._L.or_read_brick(self._h, chan, k, tmp.ctypes.data_as(C.POINTER(C.c_float))), "read_brick")
             out[k] = tmp.reshape(br, br, br)
         return out
+
+    def set_bricks(self, bricks, chan=0):
+        """Test input: overwrite every leaf's padded brick ((B, br, br, br), [z, y, x])."""
+        b = np.ascontiguousarray(bricks, dtype=np.float32)
+        for k in range(b.shape[0]):
+            _chk(self._L.or_write_brick(self._h, chan, k, b[k].ctypes.data_as(C.POINTER(C.c_float))), "write_brick")
+
+    def raytrace(self, rays, rp, bias=0.0, chan=0):
+        """Raytrace (VG.cpp:4384-4408): rays is a structured array of RAY_DTYPE,
+        updated in place (hit, normal)."""
+        assert rays.dtype == RAY_DTYPE and rays.flags.c_contiguous
+        p = make_params(rp)
+        _chk(self._L.or_raytrace(self._h, chan, C.byref(p), rays.ctypes.data_as(C.POINTER(OrScnRay)), len(rays),
+                                 float(bias)), "raytrace")
+        return rays
 
     # ---------------- raycast and adjoints ----------------
     def _padded_count(self):

@@ -285,7 +285,13 @@ def test_render_error_paths():
         gv.RaycastEABackward(0, cam, rp, I, T, torch.ones_like(I))
     assert e.value.code == 2
     with pytest.raises(G.GvdbxError) as e:
-        gv.Render(5, 0, cam, rp)           # SHADE_TRICUBIC is not provided
+        gv.Render(9, 0, cam, rp)           # no such shading
+    assert e.value.code == 1
+    with pytest.raises(G.GvdbxError) as e:
+        gv.Render(G.SHADE_TRICUBIC, 0, cam, G.render_params(finestep=0.0))
+    assert e.value.code == 1
+    with pytest.raises(G.GvdbxError) as e:
+        gv.Render(G.SHADE_SECTION2D, 0, cam, rp)   # no transfer function
     assert e.value.code == 1
     with pytest.raises(G.GvdbxError) as e:
         gv.Render(G.SHADE_VOLUME, 0, cam, rp)   # no transfer function
