@@ -130,6 +130,9 @@ struct gvdbx_volume {
     bool leaf_inactive = false;   // an incremental update may have deactivated leaves
     bool incre_rebuild = false;   // RequestFullRebuild(false): RebuildTopology updates incrementally
     bool force_full = false;      // RequestFullRebuild(true): the next topology call rebuilds fully
+    // grid transform of SetTransform (VG.cpp:5770-5795), in VBX file order:
+    // pretranslation, Euler angles, scale, translation
+    float xform[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int gather_split = GATHER_SPLIT_DEFAULT;
@@ -2334,6 +2337,7 @@ int gvdbx_get_info(gvdbx_volume* v, gvdbx_info* out) {
     memcpy(out->obj_max, v->obj_max, sizeof(out->obj_max));
     out->sc_len = v->sc_len;
     out->topology_mode_used = v->topo_mode_used;
+    for (int l = 0; l < GVDBX_MAXLEV && l < 10; l++) out->log2dim[l] = v->logdim[l];
     return GVDBX_OK;
 }
 
@@ -2399,3 +2403,374 @@ int gvdbx_read_point_nodes(gvdbx_volume* v, int32_t* pnode_host, int32_t* poff_h
 }
 
 }  // extern "C"
+
+// ---------------- VBX files (SaveVBX VG.cpp:1626-1767, LoadVBX 507-685) ----------------
+namespace {
+
+struct VbxIO {
+    FILE* fp = nullptr;
+    bool ok = true;
+    void w(const void* p, size_t n) {
+        if (ok && n && fwrite(p, 1, n, fp) != n) ok = false;
+    }
+    bool r(void* p, size_t n) {
+        if (ok && n && fread(p, 1, n, fp) != n) ok = false;
+        return ok;
+    }
+    ~VbxIO() {
+        if (fp) fclose(fp);
+    }
+};
+
+// channels in use must be 0 .. n-1 (the file numbers them by position)
+int vbx_channels(gvdbx_volume* v, const char* what, int* out) {
+    int nch = 0;
+    while (nch < GVDBX_MAXCHAN && v->has_chan[nch]) nch++;
+    for (int c = nch; c < GVDBX_MAXCHAN; c++)
+        if (v->has_chan[c]) FAIL(GVDBX_ERR_ARG, "%s: channels must be numbered 0..n-1", what);
+    *out = nch;
+    return GVDBX_OK;
+}
+
+int dev_to_host(gvdbx_volume* v, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return GVDBX_OK;
+    GVDBX_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, v->stream));
+    return sync(v);
+}
+
+int host_to_dev(gvdbx_volume* v, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return GVDBX_OK;
+    GVDBX_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, v->stream));
+    return sync(v);
+}
+
+// One grid of LoadVBX: Configure, the pools, FinishTopology, the channels
+// and UpdateAtlas (VG.cpp:596-679).  The atlas is taken as it lies in the
+// file: each active leaf gets the brick at the slot UpdateAtlas gives it.
+int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
+    char name[256];
+    unsigned char dtype = 0, comps = 0, compress = 0, topotype = 0, layout = 0;
+    float vsize[3];
+    int leafcnt = 0, leafdim[3], apron = 0, nch = 0, reuse = 0, axiscnt[3], axisres[3], levels = 0;
+    uint64_t atlas_sz = 0, root = 0;
+    f.r(name, 256), f.r(&dtype, 1), f.r(&comps, 1), f.r(&compress, 1), f.r(vsize, 12), f.r(&leafcnt, 4);
+    f.r(leafdim, 12), f.r(&apron, 4), f.r(&nch, 4), f.r(&atlas_sz, 8), f.r(&topotype, 1), f.r(&reuse, 4);
+    f.r(&layout, 1), f.r(axiscnt, 12), f.r(axisres, 12), f.r(&levels, 4), f.r(&root, 8);
+    if (!f.ok) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated grid header", path);
+    if (dtype != 'f' || comps != 1 || compress != 0 || topotype != 2 || layout != 0)
+        FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: only uncompressed float grids with GVDB topology in atlas layout", path);
+    if (levels < 2 || levels > GVDBX_MAXLEV) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: %d levels", path, levels);
+    if (nch < 0 || nch > GVDBX_MAXCHAN || apron < 0 || apron > 1)
+        FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: %d channels with apron %d", path, nch, apron);
+    int ld[GVDBX_MAXLEV], res[GVDBX_MAXLEV], range[GVDBX_MAXLEV][3], cnt0[GVDBX_MAXLEV], w0[GVDBX_MAXLEV];
+    int cnt1[GVDBX_MAXLEV], w1[GVDBX_MAXLEV];
+    for (int l = 0; l < levels; l++) {
+        f.r(&ld[l], 4), f.r(&res[l], 4), f.r(range[l], 12), f.r(&cnt0[l], 4), f.r(&w0[l], 4), f.r(&cnt1[l], 4);
+        f.r(&w1[l], 4);
+    }
+    if (!f.ok) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated level table", path);
+    for (int l = 0; l < levels; l++) {
+        if (ld[l] < 1 || ld[l] > 10 || res[l] != (1 << ld[l]) || cnt0[l] < 0 || cnt1[l] < 0)
+            FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: bad level %d", path, l);
+        if (w0[l] != (int)sizeof(GNode))   // VG.cpp:614-618
+            FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: node records of %d bytes (this library: %d)", path, w0[l], (int)sizeof(GNode));
+        if (l == 0 ? cnt1[0] != 0 : (w1[l] != 8 * res[l] * res[l] * res[l]))
+            FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: child lists of level %d", path, l);
+    }
+    const int depth = (int)((root >> 8) & 0xFF);
+    if ((root & 0xFF) != 0 || (root >> 16) != 0 || depth < 1 || depth >= levels || cnt0[depth] < 1 || cnt1[depth] < 1)
+        FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: root id 0x%llx", path, (unsigned long long)root);
+    TRY(gvdbx_configure(v, levels, ld));
+    if (leafdim[0] != res[0] || leafdim[1] != res[0] || leafdim[2] != res[0] || leafcnt != cnt0[0])
+        FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: brick count or size disagrees with the topology", path);
+    // pools (PoolRead AL.cpp:985-993)
+    std::vector<std::vector<GNode>> hn((size_t)levels);
+    std::vector<std::vector<uint64_t>> hl((size_t)levels);
+    for (int l = 0; l < levels; l++) {
+        hn[l].resize((size_t)cnt0[l]);
+        f.r(hn[l].data(), (size_t)cnt0[l] * sizeof(GNode));
+    }
+    for (int l = 1; l < levels; l++) {
+        hl[l].resize((size_t)cnt1[l] * (size_t)(w1[l] / 8));
+        f.r(hl[l].data(), (size_t)cnt1[l] * (size_t)w1[l]);
+    }
+    if (!f.ok) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated pools", path);
+    // the topology state of a rebuild: counts, root, top level, and the
+    // cell -> node maps of the nodes reached from the top node
+    int64_t cnt[GVDBX_MAXLEV] = {};
+    for (int l = 0; l < levels; l++) cnt[l] = cnt0[l];
+    int tlev = 1;   // PrepareVDBPartially VG.cpp:3996-4012
+    for (int l = levels - 1; l >= 0; l--)
+        if (cnt[l] == 1) tlev = l;
+    if (tlev < 1 || tlev > depth) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: top level %d below the leaves' parents", path, tlev);
+    std::vector<std::vector<int32_t>> reach((size_t)levels);
+    reach[tlev].push_back(0);
+    for (int l = tlev; l >= 1; l--) {
+        const int64_t len = (int64_t)res[l] * res[l] * res[l];
+        for (int32_t k : reach[l]) {
+            const uint64_t lid = hn[l][k].mChildList;
+            if (lid == GVDBX_ID_UNDEFL || (int)((lid >> 8) & 0xFF) != l) continue;
+            const int64_t li = (int64_t)(lid >> 16);
+            if (li >= cnt1[l]) continue;
+            for (int64_t b = 0; b < len; b++) {
+                const uint64_t e = hl[l][(size_t)(li * len + b)];
+                if (e == GVDBX_ID_UNDEF64) continue;
+                const int64_t c = (int64_t)(int32_t)(uint32_t)(e >> 16);
+                if ((int)((e >> 8) & 0xFF) != l - 1 || c < 0 || c >= cnt[l - 1])
+                    FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: child list entry out of range at level %d", path, l);
+                reach[l - 1].push_back((int32_t)c);
+            }
+        }
+    }
+    if (reach[0].empty()) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: no leaf is reachable", path);
+    CellGrid grid[GVDBX_MAXLEV] = {};
+    for (int l = 0; l < depth; l++) {
+        if (reach[l].empty()) continue;
+        int lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+        for (int32_t k : reach[l])
+            for (int a = 0; a < 3; a++) {
+                const int c = (int)(hn[l][k].mPos[a] / v->range[l]);
+                lo[a] = std::min(lo[a], c);
+                hi[a] = std::max(hi[a], c);
+            }
+        grid[l] = CellGrid{lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    }
+    if (grid[0].count() > MAX_DENSE_CELLS)
+        FAIL(GVDBX_ERR_RANGE, "LoadVBX: %s: the leaves span %lld cells, beyond the dense-grid limit", path,
+             (long long)grid[0].count());
+    for (int l = 0; l < levels; l++) {
+        TRY(ensure(v, v->nodes[l], (size_t)std::max<int64_t>(cnt[l], 1) * sizeof(GNode)));
+        TRY(host_to_dev(v, v->nodes[l].p, hn[l].data(), (size_t)cnt[l] * sizeof(GNode)));
+        if (l >= 1) {
+            TRY(ensure(v, v->clist[l], std::max<size_t>(hl[l].size(), 1) * 8));
+            TRY(host_to_dev(v, v->clist[l].p, hl[l].data(), hl[l].size() * 8));
+        }
+    }
+    bool inactive = false;
+    for (const GNode& n : hn[0]) inactive = inactive || !n.mFlags;
+    for (int l = 0; l < depth; l++) {
+        std::vector<int32_t> idx((size_t)grid[l].count(), -1);
+        for (int32_t k : reach[l]) {
+            const GNode& n = hn[l][k];
+            idx[(size_t)grid[l].index((int)(n.mPos[0] / v->range[l]), (int)(n.mPos[1] / v->range[l]),
+                                      (int)(n.mPos[2] / v->range[l]))] = k;
+        }
+        TRY(ensure(v, v->idx[l], std::max<size_t>(idx.size(), 1) * 4));
+        TRY(host_to_dev(v, v->idx[l].p, idx.data(), idx.size() * 4));
+    }
+    v->depth = depth;
+    v->top_lev = tlev;
+    for (int a = 0; a < 3; a++) v->root_pos[a] = hn[depth][0].mPos[a];
+    for (int l = 0; l < GVDBX_MAXLEV; l++) {
+        v->cnt[l] = l < levels ? cnt[l] : 0;
+        v->nbase[l] = v->cnt[l];
+        v->grid[l] = l < depth ? grid[l] : CellGrid{};
+    }
+    v->has_topo = true;
+    v->has_atlas = false;
+    v->topo_gen++;
+    v->vis_valid = false;
+    v->topo_mode_used = GVDBX_TOPO_DENSE;
+    v->leaf_inactive = inactive;
+    v->force_full = false;
+    v->ncloud = 0;
+    TRY(compute_bounds(v));   // FinishTopology
+    TRY(resolve_bounds(v));
+    memcpy(v->pos_min, v->obj_min, sizeof(v->pos_min));   // no points were seen
+    memcpy(v->pos_max, v->obj_max, sizeof(v->pos_max));
+    // channels (DestroyChannels, AddChannel with the file's atlas size)
+    for (int c = 0; c < GVDBX_MAXCHAN; c++) {
+        free_buf(v->chan[c]);
+        v->has_chan[c] = false;
+    }
+    v->atlas_cap = 0;
+    std::vector<int> ctype((size_t)nch);
+    for (int c = 0; c < nch; c++) {
+        int type = 0, stride = 0;
+        f.r(&type, 4), f.r(&stride, 4);
+        if (!f.ok) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated channel %d", path, c);
+        if (stride != 4) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: channel %d has %d-byte voxels", path, c, stride);
+        TRY(gvdbx_add_channel(v, c, type, apron, 0, 0));
+        ctype[c] = type;
+        // the atlases follow one another; they are read after UpdateAtlas below
+        const long skip = (long)axisres[0] * axisres[1] * axisres[2] * 4;
+        if (fseek(f.fp, skip, SEEK_CUR) != 0) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated atlas", path);
+    }
+    if (nch == 0) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: no channel", path);
+    const int br = brick_res(v);
+    for (int a = 0; a < 3; a++)
+        if (axiscnt[a] < 1 || axisres[a] != axiscnt[a] * br)
+            FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: atlas of %dx%dx%d bricks, %d voxels wide", path, axiscnt[0], axiscnt[1],
+                 axiscnt[2], axisres[0]);
+    for (int a = 0; a < 3; a++) v->subdim[a] = axiscnt[a];
+    TRY(update_atlas(v));
+    // slot -> leaf of the active leaves, in leaf order (UpdateAtlas)
+    const int64_t slots = (int64_t)v->subdim[0] * v->subdim[1] * v->subdim[2];
+    std::vector<int32_t> slot_leaf((size_t)slots, -1);
+    int64_t rank = 0;
+    for (int64_t k = 0; k < cnt[0] && rank < slots; k++)
+        if (hn[0][k].mFlags) slot_leaf[(size_t)rank++] = (int32_t)k;
+    // rewind to the first atlas and read them in order, file atlas slots as
+    // UpdateAtlas assigns them
+    const long per = (long)axisres[0] * axisres[1] * axisres[2] * 4 + 8;
+    if (fseek(f.fp, -(long)nch * per, SEEK_CUR) != 0) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated atlas", path);
+    const int64_t bv = brick_vox(v);
+    const int64_t slice = (int64_t)axisres[0] * axisres[1];
+    const int64_t fz = axisres[2] / br;   // brick layers in the file
+    std::vector<uint32_t> bricks((size_t)(cnt[0] * bv), 0u), buf((size_t)slice);
+    for (int c = 0; c < nch; c++) {
+        int hdr[2];
+        f.r(hdr, 8);
+        std::fill(bricks.begin(), bricks.end(), 0u);
+        for (int z = 0; z < axisres[2]; z++) {
+            if (!f.r(buf.data(), (size_t)slice * 4)) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated atlas", path);
+            const int bz = z / br, zi = z % br;
+            if (bz >= fz) continue;
+            for (int by = 0; by < axiscnt[1]; by++)
+                for (int bx = 0; bx < axiscnt[0]; bx++) {
+                    const int64_t s = ((int64_t)bz * axiscnt[1] + by) * axiscnt[0] + bx;
+                    if (s >= slots || slot_leaf[(size_t)s] < 0) continue;
+                    uint32_t* dst = bricks.data() + (size_t)slot_leaf[(size_t)s] * bv + (size_t)zi * br * br;
+                    for (int y = 0; y < br; y++)
+                        memcpy(dst + (size_t)y * br, buf.data() + (size_t)(by * br + y) * axisres[0] + (size_t)bx * br,
+                               (size_t)br * 4);
+                }
+        }
+        TRY(host_to_dev(v, v->chan[c].p, bricks.data(), bricks.size() * 4));
+    }
+    (void)ctype;
+    return GVDBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gvdbx_set_transform(gvdbx_volume* v, const float* pretrans, const float* scale, const float* angs,
+                        const float* trans) {
+    TRY(check_vol(v));
+    if (!pretrans || !scale || !angs || !trans) FAIL(GVDBX_ERR_ARG, "SetTransform: null vector");
+    memcpy(v->xform, pretrans, 12);
+    memcpy(v->xform + 3, angs, 12);
+    memcpy(v->xform + 6, scale, 12);
+    memcpy(v->xform + 9, trans, 12);
+    return GVDBX_OK;
+}
+
+int gvdbx_get_transform(gvdbx_volume* v, float* pretrans, float* scale, float* angs, float* trans) {
+    TRY(check_vol(v));
+    if (!pretrans || !scale || !angs || !trans) FAIL(GVDBX_ERR_ARG, "GetTransform: null vector");
+    memcpy(pretrans, v->xform, 12);
+    memcpy(angs, v->xform + 3, 12);
+    memcpy(scale, v->xform + 6, 12);
+    memcpy(trans, v->xform + 9, 12);
+    return GVDBX_OK;
+}
+
+int gvdbx_save_vbx(gvdbx_volume* v, const char* path) {
+    TRY(check_vol(v));
+    if (!path) FAIL(GVDBX_ERR_ARG, "SaveVBX: null path");
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "SaveVBX: not available for batched clouds");
+    TRY(need_atlas(v, "SaveVBX"));
+    int nch = 0;
+    TRY(vbx_channels(v, "SaveVBX", &nch));
+    if (nch == 0) FAIL(GVDBX_ERR_STATE, "SaveVBX: no channel");
+    TRY(sync(v));
+    const int levels = v->levs, br = brick_res(v);
+    std::vector<std::vector<char>> hn((size_t)levels), hl((size_t)levels);
+    for (int l = 0; l < levels; l++) {
+        hn[l].resize((size_t)v->cnt[l] * sizeof(GNode));
+        TRY(dev_to_host(v, hn[l].data(), v->nodes[l].p, hn[l].size()));
+        if (l >= 1) {
+            hl[l].resize((size_t)(v->cnt[l] * (int64_t)v->res[l] * v->res[l] * v->res[l]) * 8);
+            TRY(dev_to_host(v, hl[l].data(), v->clist[l].p, hl[l].size()));
+        }
+    }
+    std::vector<GAtlasNode> amap((size_t)v->amap_cnt);
+    TRY(dev_to_host(v, amap.data(), v->amap.p, amap.size() * sizeof(GAtlasNode)));
+    VbxIO f;
+    f.fp = fopen(path, "wb");
+    if (!f.fp) FAIL(GVDBX_ERR_ARG, "SaveVBX: cannot open %s for writing", path);
+    const unsigned char major = 1, minor = 11;   // VG.cpp:31-32
+    const int num_grids = 1;
+    f.w(&major, 1), f.w(&minor, 1), f.w(v->xform, 48), f.w(&num_grids, 4);
+    const uint64_t grid_off = 2 + 48 + 4 + 8;
+    f.w(&grid_off, 8);
+    char name[256];
+    memset(name, 0, sizeof(name));
+    const unsigned char dtype = 'f', comps = 1, compress = 0, topotype = 2, layout = 0;
+    const float vsize[3] = {1, 1, 1};
+    const int leafcnt = (int)v->cnt[0], r0 = v->res[0], leafdim[3] = {r0, r0, r0}, apron = v->apron, reuse = 0;
+    const int axisres[3] = {v->subdim[0] * br, v->subdim[1] * br, v->subdim[2] * br};
+    const uint64_t atlas_sz = (uint64_t)4 * axisres[0] * (uint64_t)axisres[1] * axisres[2];
+    f.w(name, 256), f.w(&dtype, 1), f.w(&comps, 1), f.w(&compress, 1), f.w(vsize, 12), f.w(&leafcnt, 4);
+    f.w(leafdim, 12), f.w(&apron, 4), f.w(&nch, 4), f.w(&atlas_sz, 8), f.w(&topotype, 1), f.w(&reuse, 4);
+    f.w(&layout, 1), f.w(v->subdim, 12), f.w(axisres, 12);
+    const uint64_t root = (uint64_t)v->depth << 8;   // Elem(0, depth, 0)
+    f.w(&levels, 4), f.w(&root, 8);
+    for (int l = 0; l < levels; l++) {
+        const int rg = (int)v->range[l], range[3] = {rg, rg, rg}, c0 = (int)v->cnt[l], w0 = (int)sizeof(GNode);
+        const int c1 = l == 0 ? 0 : (int)v->cnt[l];   // one child list per node above the leaves
+        const int w1 = l == 0 ? 0 : 8 * v->res[l] * v->res[l] * v->res[l];
+        f.w(&v->logdim[l], 4), f.w(&v->res[l], 4), f.w(range, 12), f.w(&c0, 4), f.w(&w0, 4), f.w(&c1, 4), f.w(&w1, 4);
+    }
+    for (int l = 0; l < levels; l++) f.w(hn[l].data(), hn[l].size());
+    for (int l = 1; l < levels; l++) f.w(hl[l].data(), hl[l].size());
+    const int64_t bv = brick_vox(v), slice = (int64_t)axisres[0] * axisres[1];
+    std::vector<uint32_t> bricks((size_t)(v->cnt[0] * bv)), buf((size_t)slice);
+    for (int c = 0; c < nch; c++) {
+        TRY(dev_to_host(v, bricks.data(), v->chan[c].p, bricks.size() * 4));
+        const int type = v->chan_type[c], stride = 4;
+        f.w(&type, 4), f.w(&stride, 4);
+        for (int z = 0; z < axisres[2]; z++) {
+            const int bz = z / br, zi = z % br;
+            std::fill(buf.begin(), buf.end(), 0u);
+            for (int by = 0; by < v->subdim[1]; by++)
+                for (int bx = 0; bx < v->subdim[0]; bx++) {
+                    const int64_t s = ((int64_t)bz * v->subdim[1] + by) * v->subdim[0] + bx;
+                    if (s >= (int64_t)amap.size()) continue;
+                    const int32_t leaf = amap[(size_t)s].mLeafID;
+                    if (leaf < 0 || leaf >= v->cnt[0]) continue;
+                    const uint32_t* src = bricks.data() + (size_t)leaf * bv + (size_t)zi * br * br;
+                    for (int y = 0; y < br; y++)
+                        memcpy(buf.data() + (size_t)(by * br + y) * axisres[0] + (size_t)bx * br, src + (size_t)y * br,
+                               (size_t)br * 4);
+                }
+            f.w(buf.data(), (size_t)slice * 4);
+        }
+    }
+    if (!f.ok) FAIL(GVDBX_ERR_HIP, "SaveVBX: write to %s failed", path);
+    if (fclose(f.fp) != 0) {
+        f.fp = nullptr;
+        FAIL(GVDBX_ERR_HIP, "SaveVBX: closing %s failed", path);
+    }
+    f.fp = nullptr;
+    return GVDBX_OK;
+}
+
+int gvdbx_load_vbx(gvdbx_volume* v, const char* path) {
+    TRY(check_vol(v));
+    if (!path) FAIL(GVDBX_ERR_ARG, "LoadVBX: null path");
+    VbxIO f;
+    f.fp = fopen(path, "rb");
+    if (!f.fp) FAIL(GVDBX_ERR_ARG, "LoadVBX: unable to open %s", path);
+    unsigned char major = 0, minor = 0;
+    f.r(&major, 1), f.r(&minor, 1);
+    float xf[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
+    if ((major == 1 && minor >= 11) || major > 1) f.r(xf, 48);   // VG.cpp:562-569
+    int num_grids = 0;
+    f.r(&num_grids, 4);
+    unsigned char masks = (major == 1 && minor == 0) ? 1 : 0;   // VG.cpp:577-588
+    if (major >= 2) f.r(&masks, 1);
+    if (!f.ok || num_grids < 1 || num_grids > 4096) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: bad header", path);
+    if (masks) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: bitmask node records are not supported", path);
+    std::vector<uint64_t> offs((size_t)num_grids);
+    f.r(offs.data(), offs.size() * 8);
+    if (!f.ok) FAIL(GVDBX_ERR_ARG, "LoadVBX: %s: truncated grid table", path);
+    memcpy(v->xform, xf, sizeof(xf));
+    for (int g = 0; g < num_grids; g++) TRY(vbx_load_grid(v, f, path));   // each grid replaces the last
+    return GVDBX_OK;
+}
+
+}  // extern "C"
+

@@ -197,6 +197,23 @@ class VolumeGVDB:
     def RequestFullRebuild(self, tf):
         call("gvdbx_request_full_rebuild", self._h, 1 if tf else 0)
 
+    def SaveVBX(self, fname):
+        call("gvdbx_save_vbx", self._h, str(fname).encode())
+
+    def LoadVBX(self, fname):
+        call("gvdbx_load_vbx", self._h, str(fname).encode())
+        inf = self.info()
+        self.log2dims = tuple(int(inf.log2dim[l]) for l in range(inf.levs))
+        return True
+
+    def SetTransform(self, pretrans, scale, angs, trans):
+        call("gvdbx_set_transform", self._h, _v3(pretrans), _v3(scale), _v3(angs), _v3(trans))
+
+    def GetTransform(self):
+        out = [(C.c_float * 3)() for _ in range(4)]
+        call("gvdbx_get_transform", self._h, *out)
+        return tuple(tuple(float(x) for x in a) for a in out)   # pretrans, scale, angs, trans
+
     def FinishTopology(self, commit_pool=False, compute_bounds=True):
         call("gvdbx_finish_topology", self._h, int(commit_pool), int(compute_bounds))
 

@@ -23,7 +23,8 @@ class Info(C.Structure):
     _fields_ = [("levs", C.c_int), ("depth", C.c_int), ("top_lev", C.c_int), ("root_pos", C.c_int * 3),
                 ("node_cnt", C.c_int64 * 10), ("subdim", C.c_int * 3), ("brick_res", C.c_int), ("apron", C.c_int),
                 ("pos_min", C.c_float * 3), ("pos_max", C.c_float * 3), ("obj_min", C.c_float * 3),
-                ("obj_max", C.c_float * 3), ("sc_len", C.c_int64), ("topology_mode_used", C.c_int)]
+                ("obj_max", C.c_float * 3), ("sc_len", C.c_int64), ("topology_mode_used", C.c_int),
+                ("log2dim", C.c_int * 10)]
 
 
 class Camera(C.Structure):
@@ -61,6 +62,10 @@ _SIGS = {
     "gvdbx_set_point_weights": [_vp, _vp, _i],
     "gvdbx_rebuild_topology": [_vp, _i, _f, _f, _f, _f],
     "gvdbx_raytrace": [_vp, _i, _vp, _vp, _i64, _f],
+    "gvdbx_save_vbx": [_vp, C.c_char_p],
+    "gvdbx_load_vbx": [_vp, C.c_char_p],
+    "gvdbx_set_transform": [_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)],
+    "gvdbx_get_transform": [_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)],
     "gvdbx_accumulate_topology": [_vp, _i, _f, _f, _f, _f, _i],
     "gvdbx_request_full_rebuild": [_vp, _i],
     "gvdbx_finish_topology": [_vp, _i, _i],

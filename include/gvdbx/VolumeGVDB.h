@@ -115,6 +115,18 @@ public:
         check(gvdbx_accumulate_topology(v_, num_pnts, radius, orig.x, orig.y, orig.z, iDepth), "AccumulateTopology");
     }
     void RequestFullRebuild(bool tf) { check(gvdbx_request_full_rebuild(v_, tf ? 1 : 0), "RequestFullRebuild"); }
+    void SaveVBX(const std::string& fname) { check(gvdbx_save_vbx(v_, fname.c_str()), "SaveVBX"); }
+    bool LoadVBX(const std::string& fname, int force_maj = 0, int force_min = 0) {
+        if (force_maj > 0 || force_min > 0)
+            throw std::runtime_error("LoadVBX: forcing a file version is not supported");
+        check(gvdbx_load_vbx(v_, fname.c_str()), "LoadVBX");
+        return true;
+    }
+    void SetTransform(Vector3DF pretrans, Vector3DF scal, Vector3DF angs, Vector3DF trans) {
+        const float p[3] = {pretrans.x, pretrans.y, pretrans.z}, s[3] = {scal.x, scal.y, scal.z};
+        const float a[3] = {angs.x, angs.y, angs.z}, t[3] = {trans.x, trans.y, trans.z};
+        check(gvdbx_set_transform(v_, p, s, a, t), "SetTransform");
+    }
     void FinishTopology(bool commit = false, bool compute_bounds = true) {
         check(gvdbx_finish_topology(v_, commit, compute_bounds), "FinishTopology");
     }

@@ -63,6 +63,7 @@ typedef struct {
     float obj_min[3], obj_max[3];   /* ComputeBounds result (VG.cpp:1792-1816) */
     int64_t sc_len;                 /* last InsertPointsSubcell list length */
     int topology_mode_used;         /* gvdbx_topology_mode actually taken */
+    int log2dim[10];                /* Configure's log2 dimensions per level (0 beyond levs) */
 } gvdbx_info;
 
 /* ---- lifecycle ---- */
@@ -229,6 +230,27 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
  * channel the value outside every leaf is make_uchar4(b, b, b, b) with b
  * converted by saturating truncation (cuda_gvdb_operators.cuh:139-142) */
 int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
+
+/* ---- files and transform ---- */
+/* SaveVBX(std::string) VG.h:375 / VG.cpp:1626-1767: version 1.11, one float
+ * grid in atlas layout with GVDB topology (GVDB_FILESPEC.txt, in the field
+ * order SaveVBX writes: the channel count after the apron, the child-list
+ * pool count per level).  Atlas bricks no active leaf owns are written as
+ * zeros; the grid name is all zeros.  Channels must be 0..n-1. */
+int gvdbx_save_vbx(gvdbx_volume* v, const char* path);
+/* LoadVBX(std::string, int, int) VG.h:374 / VG.cpp:507-685: Configure from
+ * the file, the node and child-list pools, FinishTopology, the channels
+ * with the file's atlas size, UpdateAtlas; each active leaf takes the brick
+ * at the atlas slot UpdateAtlas gives it.  Files with bitmask node records
+ * (version 1.0, or 2.x with the flag) are refused; later grids replace
+ * earlier ones, as there. */
+int gvdbx_load_vbx(gvdbx_volume* v, const char* path);
+/* SetTransform(pretrans, scale, angs, trans) VG.h:672: kept with the volume
+ * and written to / read from VBX files; it changes nothing else here
+ * (cameras and points are in index space). */
+int gvdbx_set_transform(gvdbx_volume* v, const float* pretrans, const float* scale, const float* angs,
+                        const float* trans);
+int gvdbx_get_transform(gvdbx_volume* v, float* pretrans, float* scale, float* angs, float* trans);
 
 /* ---- rendering (RC.cuh, DDA.cuh, GEO.cuh, MOD.cu) ---- */
 enum gvdbx_shade {

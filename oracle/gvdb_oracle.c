@@ -46,6 +46,7 @@
  */
 #include "gvdb_oracle.h"
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1368,6 +1369,92 @@ int or_scatter_color(or_volume* v, const float* pos, int64_t n, int64_t stride, 
 /* ------------------------------------------------------------------ */
 /* readback                                                            */
 /* ------------------------------------------------------------------ */
+
+/* ------------------------------------------------------------------ */
+/* VBX file (SaveVBX VG.cpp:1626-1767, GVDB_FILESPEC.txt)              */
+/* ------------------------------------------------------------------ */
+
+static int wr(FILE* fp, const void* p, size_t n) { return fwrite(p, 1, n, fp) == n ? 0 : 1; }
+
+/* SaveVBX, version 1.11: header with the grid transform (pretrans, angles,
+ * scale, translation: the defaults (0,0,0), (0,0,0), (1,1,1), (0,0,0)), one
+ * grid of float scalars in atlas layout with GVDB topology: per level
+ * (log2 dim, res, range xyz, pool 0 count and width, pool 1 count and
+ * width), then the node pools and the child-list pools level by level, then
+ * per channel its type and stride and the 3-D atlas slice by slice.  Atlas
+ * bricks no active leaf owns are written as zeros; the grid name is zeros. */
+int or_save_vbx(or_volume* v, const char* path) {
+    if (!v || !v->configured || !path) return OR_ERR_ARG;
+    if (v->ncnt[0] == 0 || !v->amap) return OR_ERR_STATE;
+    int nch = 0;
+    while (nch < OR_MAXCHAN && v->has_chan[nch]) nch++;
+    for (int c = nch; c < OR_MAXCHAN; c++)
+        if (v->has_chan[c]) return OR_ERR_ARG;   /* channels must be 0..n-1 */
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return OR_ERR_ARG;
+    int err = 0;
+    const unsigned char major = 1, minor = 11;
+    const float xf[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
+    const int num_grids = 1;
+    err |= wr(fp, &major, 1) | wr(fp, &minor, 1) | wr(fp, xf, sizeof(xf)) | wr(fp, &num_grids, 4);
+    uint64_t grid_off = 2 + sizeof(xf) + 4 + 8;
+    err |= wr(fp, &grid_off, 8);
+    char name[256];
+    memset(name, 0, sizeof(name));
+    const char dtype = 'f', comps = 1, compress = 0, topotype = 2, layout = 0;
+    const float vsize[3] = {1, 1, 1};
+    const int leafcnt = (int)v->ncnt[0];
+    const int r0 = v->res[0];
+    const int leafdim[3] = {r0, r0, r0};
+    const int apron = v->apron;
+    const int br = brick_res(v);
+    const int axisres[3] = {v->subdim[0] * br, v->subdim[1] * br, v->subdim[2] * br};
+    const uint64_t atlas_sz = (uint64_t)4 * axisres[0] * (uint64_t)axisres[1] * axisres[2];
+    const int reuse = 0;
+    err |= wr(fp, name, 256) | wr(fp, &dtype, 1) | wr(fp, &comps, 1) | wr(fp, &compress, 1) | wr(fp, vsize, 12);
+    err |= wr(fp, &leafcnt, 4) | wr(fp, leafdim, 12) | wr(fp, &apron, 4) | wr(fp, &nch, 4) | wr(fp, &atlas_sz, 8);
+    err |= wr(fp, &topotype, 1) | wr(fp, &reuse, 4) | wr(fp, &layout, 1) | wr(fp, v->subdim, 12) | wr(fp, axisres, 12);
+    const int levels = v->levs;
+    const uint64_t root = (uint64_t)v->depth << 8;   /* Elem(0, depth, 0) */
+    err |= wr(fp, &levels, 4) | wr(fp, &root, 8);
+    for (int l = 0; l < levels; l++) {
+        const int ld = v->logdim[l], res = v->res[l];
+        const int rg = (int)v->range[l];
+        const int range[3] = {rg, rg, rg};
+        const int cnt0 = (int)v->ncnt[l], w0 = (int)sizeof(or_node);
+        const int cnt1 = l == 0 ? 0 : (int)v->ccnt[l];
+        const int w1 = l == 0 ? 0 : (int)(8 * ipow3(res));   /* pool 1 of level 0 is a placeholder (VG.cpp:2427) */
+        err |= wr(fp, &ld, 4) | wr(fp, &res, 4) | wr(fp, range, 12) | wr(fp, &cnt0, 4) | wr(fp, &w0, 4);
+        err |= wr(fp, &cnt1, 4) | wr(fp, &w1, 4);
+    }
+    for (int l = 0; l < levels; l++)
+        if (v->ncnt[l]) err |= wr(fp, v->nodes[l], (size_t)v->ncnt[l] * sizeof(or_node));
+    for (int l = 1; l < levels; l++)
+        if (v->ccnt[l]) err |= wr(fp, v->clist[l], (size_t)(v->ccnt[l] * ipow3(v->res[l])) * 8);
+    /* atlas: the voxels of every mapped brick, zeros elsewhere */
+    const int64_t slice = (int64_t)axisres[0] * axisres[1];
+    float* buf = (float*)malloc((size_t)slice * 4);
+    for (int c = 0; c < nch; c++) {
+        const int type = v->chan_type[c], stride = 4;
+        err |= wr(fp, &type, 4) | wr(fp, &stride, 4);
+        for (int z = 0; z < axisres[2]; z++) {
+            const int bz = z / br;
+            memset(buf, 0, (size_t)slice * 4);
+            for (int by = 0; by < v->subdim[1]; by++)
+                for (int bx = 0; bx < v->subdim[0]; bx++) {
+                    const int64_t slot = ((int64_t)bz * v->subdim[1] + by) * v->subdim[0] + bx;
+                    if (slot >= v->amap_cnt || v->amap[slot].mLeafID == (int32_t)OR_ID_UNDEFL) continue;
+                    for (int y = by * br; y < (by + 1) * br; y++)
+                        for (int x = bx * br; x < (bx + 1) * br; x++)
+                            buf[(int64_t)y * axisres[0] + x] = v->atlas[c][atlas_index(v, x, y, z)];
+                }
+            err |= wr(fp, buf, (size_t)slice * 4);
+        }
+    }
+    free(buf);
+    if (fclose(fp) != 0) err = 1;
+    return err ? OR_ERR_STATE : OR_OK;
+}
 
 int or_get_info(or_volume* v, or_info* out) {
     if (!v || !out) return OR_ERR_ARG;

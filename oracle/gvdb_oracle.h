@@ -146,6 +146,8 @@ const or_node* or_nodes(or_volume* v, int lev);
 const uint64_t* or_child_lists(or_volume* v, int lev);
 const or_atlas_node* or_atlas_map(or_volume* v, int64_t* count);
 /* copy leaf k's full brick (res0+2a)^3, x fastest, into out */
+/* SaveVBX VG.cpp:1626-1767 (version 1.11, one float grid, atlas layout) */
+int or_save_vbx(or_volume* v, const char* path);
 /* test input: overwrite one leaf's padded brick (inverse of or_read_brick) */
 int or_write_brick(or_volume* v, int chan, int64_t leaf, const float* in);
 int or_read_brick(or_volume* v, int chan, int64_t leaf, float* out);

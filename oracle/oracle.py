@@ -146,6 +146,7 @@ def lib():
         L.or_atlas.restype = vp
         L.or_read_brick.argtypes = [vp, i32, i64, fp]
         L.or_write_brick.argtypes = [vp, i32, i64, fp]
+        L.or_save_vbx.argtypes = [vp, C.c_char_p]
         L.or_raytrace.argtypes = [vp, i32, C.POINTER(OrRenderParams), C.POINTER(OrScnRay), i64, f32]
         cp, rpp = C.POINTER(OrCamera), C.POINTER(OrRenderParams)
         L.or_render.argtypes = [vp, i32, cp, rpp, dp, fp, dp]
@@ -348,6 +349,10 @@ class OracleVolume:
             _chk(self._L.or_read_brick(self._h, chan, k, tmp.ctypes.data_as(C.POINTER(C.c_float))), "read_brick")
             out[k] = tmp.reshape(br, br, br)
         return out
+
+    def save_vbx(self, path):
+        """SaveVBX (VG.cpp:1626-1767)."""
+        _chk(self._L.or_save_vbx(self._h, str(path).encode()), "save_vbx")
 
     def set_bricks(self, bricks, chan=0):
         """Test input: overwrite every leaf's padded brick ((B, br, br, br), [z, y, x])."""

@@ -56,13 +56,16 @@ def build_pair(pts, radius, log2dims=(3, 3, 3, 3, 5), orig=(0.0, 0.0, 0.0)):
     return gv, ov, tp
 
 
-def assert_same_topology(gv, ov):
+def assert_same_topology(gv, ov, points_bbox=True):
+    """points_bbox=False skips pos_min / pos_max (the points' box, which a
+    loaded VBX file does not carry)."""
     gi, oi = gv.info(), ov.info()
     assert gi.depth == oi.depth
     assert gi.top_lev == oi.top_lev
     assert list(gi.root_pos) == list(oi.root_pos)
     assert list(gi.node_cnt) == list(oi.node_cnt)
-    assert list(gi.pos_min) == list(oi.pos_min) and list(gi.pos_max) == list(oi.pos_max)
+    if points_bbox:
+        assert list(gi.pos_min) == list(oi.pos_min) and list(gi.pos_max) == list(oi.pos_max)
     assert list(gi.obj_min) == list(oi.obj_min) and list(gi.obj_max) == list(oi.obj_max)
     assert list(gi.subdim) == list(oi.subdim)
     for l in range(gi.depth + 1):
