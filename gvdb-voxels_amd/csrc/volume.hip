@@ -85,6 +85,12 @@ int scratch_on(int dev, uint32_t** out) {
 
 }  // namespace
 
+struct gvdbx_graph;
+struct gvdbx_volume;
+namespace {
+void retire_graphs(gvdbx_volume* v);
+}  // namespace
+
 struct gvdbx_volume {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -133,6 +139,11 @@ struct gvdbx_volume {
     // grid transform of SetTransform (VG.cpp:5770-5795), in VBX file order:
     // pretranslation, Euler angles, scale, translation
     float xform[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
+    // bumped whenever a device buffer is (re)allocated: a captured frame
+    // graph holds the addresses it saw
+    uint64_t buf_gen = 0;
+    bool bins_force = false;      // ensure_bins sorts even when the bins look current (graph capture)
+    std::vector<gvdbx_graph*> graphs;   // live frame graphs of this volume (retired by gvdbx_destroy)
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
     int gather_split = GATHER_SPLIT_DEFAULT;
@@ -223,6 +234,7 @@ int ensure(gvdbx_volume* v, DevBuf& b, size_t bytes) {
     b.p = nullptr;
     b.cap = 0;
     size_t want = bytes + bytes / 4 + 256;
+    v->buf_gen++;
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -240,6 +252,7 @@ int ensure_keep(gvdbx_volume* v, DevBuf& b, size_t bytes, size_t keep) {
     TRY(sync(v));
     size_t want = bytes + bytes / 4 + 256;
     void* np = nullptr;
+    v->buf_gen++;
     hipError_t e = hipMalloc(&np, want);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -338,6 +351,7 @@ int grow_atlas(gvdbx_volume* v, int64_t bricks) {
     if (v->atlas_cap > 0) bricks = std::max(bricks, v->atlas_cap + v->atlas_cap / 4);
     int64_t bv = brick_vox(v);
     TRY(sync(v));
+    v->buf_gen++;
     for (int c = 0; c < GVDBX_MAXCHAN; c++) {
         if (!v->has_chan[c]) continue;
         void* np = nullptr;
@@ -425,7 +439,7 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     bool same = v->bins_valid && v->bins_gen == v->pos_gen && v->bins_n == n && v->bins_base == v->pos.base &&
                 v->bins_stride == v->pos.stride && v->bins_trans[0] == t[0] && v->bins_trans[1] == t[1] &&
                 v->bins_trans[2] == t[2] && !memcmp(&v->gbin, &gb, sizeof(gb));
-    if (same) return GVDBX_OK;
+    if (same && !v->bins_force) return GVDBX_OK;
     hipStream_t s = v->stream;
     const int64_t m = std::max<int64_t>(n, 1);
     const int64_t nh = radix_hist_entries(m);
@@ -1420,6 +1434,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
     if (!v) FAIL(GVDBX_ERR_ARG, "null volume handle");
     (void)hipSetDevice(v->device);
     (void)hipStreamSynchronize(v->stream);
+    retire_graphs(v);
     for (int c = 0; c < GVDBX_MAXCHAN; c++) free_buf(v->chan[c]);
     for (int l = 0; l <= GVDBX_MAXLEV; l++) {
         free_buf(v->nodes[l]);
@@ -1512,6 +1527,7 @@ int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filte
     int64_t cap = std::max(slots, v->atlas_cap);
     size_t bytes = (size_t)(cap * brick_vox(v)) * sizeof(float);
     free_buf(v->chan[chan]);
+    v->buf_gen++;
     hipError_t e = hipMalloc(&v->chan[chan].p, bytes);
     if (e != hipSuccess) {
         v->chan[chan].p = nullptr;
@@ -1967,8 +1983,11 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
     return check_launch("SplatCubic");
 }
 
-int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
-    TRY(check_vol(v));
+}  // extern "C"
+
+namespace {
+
+int update_apron(gvdbx_volume* v, int chan, float boundval) {
     TRY(check_chan(v, chan));
     if (v->apron == 0) return GVDBX_OK;   // VG.cpp:4420
     TRY(need_atlas(v, "UpdateApron"));
@@ -1985,6 +2004,15 @@ int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
                    v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval, bits};
     launch_apron(ap, v->stream);
     return check_launch("UpdateApron");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval) {
+    TRY(check_vol(v));
+    return update_apron(v, chan, boundval);
 }
 
 int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
@@ -2774,3 +2802,135 @@ int gvdbx_load_vbx(gvdbx_volume* v, const char* path) {
 
 }  // extern "C"
 
+// ---------------- frame graphs (hipGraph of one level-set frame) ----------------
+struct gvdbx_graph {
+    gvdbx_volume* v = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t topo_gen = 0, buf_gen = 0, pos_gen = 0;
+    const char* pos_base = nullptr;
+    DevBuf sorted, sorted2;   // the binning buffers as the captured sort left them
+};
+
+extern "C" {
+
+int gvdbx_frame_graph_create(gvdbx_volume* v, int num_pnts, int sc, float sc_radius, float radius, int chan,
+                             int accum, int apron_chan, float boundval, gvdbx_graph** out) {
+    TRY(check_vol(v));
+    if (!out) FAIL(GVDBX_ERR_ARG, "FrameGraph: null output");
+    *out = nullptr;
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "FrameGraph: not available for batched clouds");
+    if (v->clr) FAIL(GVDBX_ERR_ARG, "FrameGraph: point colours are not captured (SetPoints without clr)");
+    if (apron_chan >= 0) TRY(check_chan(v, apron_chan));
+    // one eager frame: validates the arguments and sizes every buffer
+    int64_t len = 0;
+    TRY(insert_subcell(v, sc, num_pnts, sc_radius, 0.0f, 0.0f, 0.0f, &len, false));
+    TRY(gather(v, 1, sc, radius, len, chan, -1, accum));
+    if (apron_chan >= 0) TRY(update_apron(v, apron_chan, boundval));
+    TRY(sync(v));
+    // the same frame captured on a stream of its own
+    hipStream_t cs = nullptr;
+    GVDBX_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipStream_t keep = v->stream;
+    const uint64_t bg = v->buf_gen;
+    v->stream = cs;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int st = e == hipSuccess ? GVDBX_OK : GVDBX_ERR_HIP;
+    if (st == GVDBX_OK) {
+        const float t0[3] = {0.0f, 0.0f, 0.0f};
+        v->bins_force = true;
+        st = ensure_bins(v, num_pnts, t0, false);
+        v->bins_force = false;
+        if (st == GVDBX_OK) st = gather(v, 1, sc, radius, len, chan, -1, accum);
+        if (st == GVDBX_OK && apron_chan >= 0) st = update_apron(v, apron_chan, boundval);
+    }
+    hipGraph_t graph = nullptr;
+    if (e == hipSuccess) {
+        const hipError_t e2 = hipStreamEndCapture(cs, &graph);
+        if (st == GVDBX_OK && e2 != hipSuccess) {
+            gvdbx_set_error("FrameGraph: capture failed: %s", hipGetErrorString(e2));
+            st = GVDBX_ERR_HIP;
+        }
+    } else {
+        gvdbx_set_error("FrameGraph: hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    v->stream = keep;
+    (void)hipStreamDestroy(cs);
+    if (st == GVDBX_OK && v->buf_gen != bg) {
+        gvdbx_set_error("FrameGraph: a buffer was reallocated during capture");
+        st = GVDBX_ERR_STATE;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (st == GVDBX_OK) {
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            gvdbx_set_error("FrameGraph: hipGraphInstantiate: %s", hipGetErrorString(e));
+            st = GVDBX_ERR_HIP;
+        }
+    }
+    if (st != GVDBX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return st;
+    }
+    gvdbx_graph* g = new gvdbx_graph;
+    g->v = v;
+    g->graph = graph;
+    g->exec = exec;
+    g->topo_gen = v->topo_gen;
+    g->buf_gen = v->buf_gen;
+    g->pos_gen = v->pos_gen;
+    g->pos_base = v->pos.base;
+    g->sorted = v->sorted;
+    g->sorted2 = v->sorted2;
+    v->graphs.push_back(g);
+    *out = g;
+    return GVDBX_OK;
+}
+
+int gvdbx_frame_graph_launch(gvdbx_graph* g) {
+    if (!g) FAIL(GVDBX_ERR_ARG, "FrameGraph: null graph");
+    if (!g->v) FAIL(GVDBX_ERR_STATE, "FrameGraph: its volume was destroyed");
+    gvdbx_volume* v = g->v;
+    TRY(check_vol(v));
+    if (v->topo_gen != g->topo_gen || v->buf_gen != g->buf_gen || v->pos.base != g->pos_base)
+        FAIL(GVDBX_ERR_STATE, "FrameGraph: the topology, a buffer or the point buffer changed since capture");
+    GVDBX_CHECK(hipGraphLaunch(g->exec, v->stream));
+    // the bins now hold the replayed points, in the buffers the capture used
+    v->sorted = g->sorted;
+    v->sorted2 = g->sorted2;
+    v->bins_valid = true;
+    v->sc_valid = true;
+    return GVDBX_OK;
+}
+
+int gvdbx_frame_graph_destroy(gvdbx_graph* g) {
+    if (!g) return GVDBX_OK;
+    if (gvdbx_volume* v = g->v) {
+        (void)hipStreamSynchronize(v->stream);
+        v->graphs.erase(std::remove(v->graphs.begin(), v->graphs.end(), g), v->graphs.end());
+    }
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return GVDBX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// gvdbx_destroy: the volume's graphs lose their executables and their
+// volume; their handles stay valid for gvdbx_frame_graph_destroy
+void retire_graphs(gvdbx_volume* v) {
+    for (gvdbx_graph* g : v->graphs) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        g->exec = nullptr;
+        g->graph = nullptr;
+        g->v = nullptr;
+    }
+    v->graphs.clear();
+}
+
+}  // namespace

@@ -93,6 +93,22 @@ def _ptr(x):
     return x.data_ptr()
 
 
+class FrameGraph:
+    """A captured level-set frame of a VolumeGVDB (gvdbx_frame_graph_*)."""
+
+    def __init__(self, h, vol):
+        self._h = h
+        self._vol = vol   # the graph replays into this volume
+
+    def launch(self):
+        call("gvdbx_frame_graph_launch", self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().gvdbx_frame_graph_destroy(self._h)
+            self._h = None
+
+
 class VolumeGVDB:
     """One sparse volume on one GPU (reference: VolumeGVDB)."""
 
@@ -196,6 +212,15 @@ class VolumeGVDB:
 
     def RequestFullRebuild(self, tf):
         call("gvdbx_request_full_rebuild", self._h, 1 if tf else 0)
+
+    def FrameGraph(self, num_pnts, subcell_size, sc_radius, radius, chan=0, accum=False, apron_chan=0,
+                   boundval=3.0):
+        """Captured level-set frame (gvdbx_frame_graph_create); returns a
+        FrameGraph whose launch() replays it on the volume's stream."""
+        h = C.c_void_p()
+        call("gvdbx_frame_graph_create", self._h, int(num_pnts), int(subcell_size), float(sc_radius), float(radius),
+             int(chan), int(accum), int(apron_chan), float(boundval), C.byref(h))
+        return FrameGraph(h, self)
 
     def SaveVBX(self, fname):
         call("gvdbx_save_vbx", self._h, str(fname).encode())

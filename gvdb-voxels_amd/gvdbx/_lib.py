@@ -63,6 +63,9 @@ _SIGS = {
     "gvdbx_rebuild_topology": [_vp, _i, _f, _f, _f, _f],
     "gvdbx_raytrace": [_vp, _i, _vp, _vp, _i64, _f],
     "gvdbx_save_vbx": [_vp, C.c_char_p],
+    "gvdbx_frame_graph_create": [_vp, _i, _i, _f, _f, _i, _i, _i, _f, C.POINTER(_vp)],
+    "gvdbx_frame_graph_launch": [_vp],
+    "gvdbx_frame_graph_destroy": [_vp],
     "gvdbx_load_vbx": [_vp, C.c_char_p],
     "gvdbx_set_transform": [_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)],
     "gvdbx_get_transform": [_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)],

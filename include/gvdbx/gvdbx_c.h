@@ -231,6 +231,26 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
  * converted by saturating truncation (cuda_gvdb_operators.cuh:139-142) */
 int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
 
+/* ---- frame graphs (no reference counterpart) ---- */
+/* A hipGraph of one level-set frame on the current topology, for callers
+ * that replay a fixed pipeline: depth frames accumulated into one topology,
+ * fluid frames between topology rebuilds.  Create runs the frame once
+ * (InsertPointsSubcell(sc, num_pnts, sc_radius), GatherLevelSet(radius,
+ * chan, accum), UpdateApron(apron_chan, boundval) unless apron_chan < 0),
+ * which sizes every buffer, then captures the same kernels (binning of the
+ * points, subcell classes, gather, apron) on a stream of its own: the graph
+ * holds no host step, allocation or memset.  Launch replays it on the
+ * volume's stream over the current contents of the point buffer given to
+ * SetPoints (same buffer, same count); it fails with GVDBX_ERR_STATE once
+ * the topology, the point buffer or any device buffer changed since capture.
+ * The list length of a replayed frame is not computed.  Point colours are
+ * refused. */
+typedef struct gvdbx_graph gvdbx_graph;
+int gvdbx_frame_graph_create(gvdbx_volume* v, int num_pnts, int sc, float sc_radius, float radius, int chan,
+                             int accum, int apron_chan, float boundval, gvdbx_graph** out);
+int gvdbx_frame_graph_launch(gvdbx_graph* g);
+int gvdbx_frame_graph_destroy(gvdbx_graph* g);
+
 /* ---- files and transform ---- */
 /* SaveVBX(std::string) VG.h:375 / VG.cpp:1626-1767: version 1.11, one float
  * grid in atlas layout with GVDB topology (GVDB_FILESPEC.txt, in the field
