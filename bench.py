@@ -258,6 +258,37 @@ def bench_depth_fusion(torch, gvdbx, dev, steps):
 
     tb = timed(batched, steps)
     tp = timed(per_frame, max(1, steps // 2))
+
+    # per-frame accumulation on the fixed topology of all frames, eager calls
+    # vs one captured frame graph replayed per frame (same kernels, same bits)
+    fbuf = torch.empty((per, 3), dtype=torch.float32, device=dev)
+    gvdbx.backproject_depth(dt, intr, pt, vox, origin, out=pts)
+    gv.SetPoints(pts)
+    gv.RebuildTopology(n, 2.0)
+    gv.FinishTopology(False, True)
+    gv.UpdateAtlas()
+    gvdbx.backproject_depth(dt[0:1], intr, pt[0:1], vox, origin, out=fbuf)
+    gv.SetPoints(fbuf)
+    graph = gv.FrameGraph(per, 4, 2.0, 1.0, chan=0, accum=True, apron_chan=-1)
+
+    def frames_eager():
+        gv.FillChannel(0, 3.0)
+        for f in range(F):
+            gvdbx.backproject_depth(dt[f:f + 1], intr, pt[f:f + 1], vox, origin, out=fbuf)
+            lf = gv.InsertPointsSubcell(4, per, 2.0)
+            gv.GatherLevelSet(4, per, 1.0, (0.0, 0.0, 0.0), lf, 0, accum=True)
+        gv.UpdateApron(0, 0.0)
+
+    def frames_graph():
+        gv.FillChannel(0, 3.0)
+        for f in range(F):
+            gvdbx.backproject_depth(dt[f:f + 1], intr, pt[f:f + 1], vox, origin, out=fbuf)
+            graph.launch()
+        gv.UpdateApron(0, 0.0)
+
+    te = timed(frames_eager, max(1, steps // 2))
+    tgr = timed(frames_graph, max(1, steps // 2))
+    del graph
     valid = int((torch.isfinite(dt) & (dt >= 0.5) & (dt <= 8.0)).sum().item())
     inf = gv.info()
     return {
@@ -268,6 +299,8 @@ def bench_depth_fusion(torch, gvdbx, dev, steps):
         "batched_ms": round(tb, 4), "per_frame_ms": round(tp, 4),
         "batched_Mpoints_per_s": round(valid / (tb / 1e3) / 1e6, 2),
         "batched_speedup": round(tp / tb, 3),
+        "fixed_topology_per_frame_eager_ms": round(te, 4), "fixed_topology_per_frame_graph_ms": round(tgr, 4),
+        "graph_speedup": round(te / tgr, 3),
     }
 
 
@@ -423,6 +456,7 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 24)
     ap.add_argument("--frames", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel configuration")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable splat+render leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched depth-fusion and batched-clouds legs")
     ap.add_argument("--only", choices=["diff", "depth", "clouds"], default=None,
@@ -509,6 +543,14 @@ def main():
 
     for i in range(args.warmup):
         step(i, False)
+    tune = None
+    if not args.no_autotune:
+        # the library's autotuner on the last warm-up frame: every candidate
+        # must reproduce the default's bits, the fastest is kept
+        tb = gv.AutotuneBinning(n, 4, 3.0, reps=2)
+        tg = gv.AutotuneGather(1, 4, 1.5, chan=0, reps=2)
+        tune = {"binning": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tb.items()},
+                "gather": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tg.items()}}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -561,7 +603,9 @@ def main():
         "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
                            "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
-        "gather_variant": gv.GetTuning("gather_variant"), "xcd_remap": gv.GetTuning("xcd_remap"),
+        "autotune": tune,
+        "tuning": {k: gv.GetTuning(k) for k in ("gather_variant", "gather_classes", "gather_split", "xcd_remap",
+                                                "bin_records", "sort_tiles")},
         "fwd_bwd_over_fwd": None,
     }
     if not args.no_diff:

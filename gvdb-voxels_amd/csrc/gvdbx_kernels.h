@@ -68,6 +68,8 @@ void launch_flag_scan_apply(FlagApplyBatch& fb, hipStream_t s);
 // a[0..na) = 0 and b[0..nb) = 0 in one launch
 void launch_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb, hipStream_t s);
 void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
+// out[0] = sum of p[i], out[1] = sum of p[i] * (i mod 65521 + 1) (mod 2^64)
+void launch_checksum(const uint32_t* p, int64_t n, unsigned long long* out, hipStream_t s);
 void launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);

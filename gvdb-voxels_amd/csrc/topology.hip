@@ -373,6 +373,25 @@ __global__ __launch_bounds__(256) void k_level_up(CellGrid gc, const uint8_t* ba
     }
 }
 
+// ---- position-sensitive checksum of 32-bit words (two 64-bit sums) ----
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* p, int64_t n, unsigned long long* out) {
+    unsigned long long a = 0, b = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long w = p[i];
+        a += w;
+        b += w * (unsigned long long)(i % 65521 + 1);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        b += __shfl_xor(b, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, a);
+        atomicAdd(out + 1, b);
+    }
+}
+
 // ---- fills ----
 template <class T>
 __global__ __launch_bounds__(256) void k_fill(T* p, int64_t n, T v) {
@@ -939,6 +958,11 @@ void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* part
                              hipStream_t s) {
     int64_t tiles = scan_tiles(n + 1);
     hipLaunchKernelGGL(k_count_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, cnt, n, partials, off);
+}
+
+void launch_checksum(const uint32_t* p, int64_t n, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, s, out, (int64_t)2, 0ull);
+    if (n > 0) hipLaunchKernelGGL(k_checksum, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, out);
 }
 
 void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s) {

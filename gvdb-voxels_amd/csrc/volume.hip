@@ -143,6 +143,11 @@ struct gvdbx_volume {
     // graph holds the addresses it saw
     uint64_t buf_gen = 0;
     bool bins_force = false;      // ensure_bins sorts even when the bins look current (graph capture)
+    // the bins RebuildTopology sorted may serve the next point-reading call
+    // once (the reference pipeline RebuildTopology -> InsertPointsSubcell reads
+    // the same points twice); every other call reads the points again, since
+    // the caller may have rewritten the buffer in place
+    bool bins_once = false;
     std::vector<gvdbx_graph*> graphs;   // live frame graphs of this volume (retired by gvdbx_destroy)
     int topo_mode = GVDBX_TOPO_AUTO, topo_mode_used = 0;
     int gather_variant = 1;
@@ -439,7 +444,9 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     bool same = v->bins_valid && v->bins_gen == v->pos_gen && v->bins_n == n && v->bins_base == v->pos.base &&
                 v->bins_stride == v->pos.stride && v->bins_trans[0] == t[0] && v->bins_trans[1] == t[1] &&
                 v->bins_trans[2] == t[2] && !memcmp(&v->gbin, &gb, sizeof(gb));
-    if (same && !v->bins_force) return GVDBX_OK;
+    const bool reuse = same && v->bins_once && !v->bins_force;
+    v->bins_once = false;
+    if (reuse) return GVDBX_OK;
     hipStream_t s = v->stream;
     const int64_t m = std::max<int64_t>(n, 1);
     const int64_t nh = radix_hist_entries(m);
@@ -762,6 +769,7 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     ReachParams rp{grid[0], r0, v->logdim[0], (extra && mask_ok) ? 1 : 0, radius, cloud_map(v)};
     bool reached = false;
     TRY(ensure_bins(v, n, orig, false, &rp, &reached));
+    v->bins_once = true;
     if (!reached)
         launch_reach_sorted(v->sorted.as<float4>(), v->bin_off.as<int32_t>() + v->gbin.count(), n, rp,
                             v->reach0.as<uint32_t>(), s);
@@ -2934,3 +2942,181 @@ void retire_graphs(gvdbx_volume* v) {
 }
 
 }  // namespace
+
+// ---------------- autotuner (tile shapes of the result-invariant knobs) ----------------
+namespace {
+
+struct GatherCfg {
+    int variant, classes, split, xcd;
+};
+
+int checksum_dev(gvdbx_volume* v, const void* p, int64_t words, unsigned long long out[2]) {
+    TRY(ensure(v, v->misc, 256));
+    unsigned long long* d = (unsigned long long*)((char*)v->misc.p + 128);
+    launch_checksum((const uint32_t*)p, words, d, v->stream);
+    TRY(check_launch("checksum"));
+    return dev_to_host(v, out, d, 16);
+}
+
+// elapsed ms of `reps` calls of fn on the volume's stream (HIP events)
+template <class F>
+int time_calls(gvdbx_volume* v, int reps, F fn, float* ms) {
+    hipEvent_t e0, e1;
+    GVDBX_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        FAIL(GVDBX_ERR_HIP, "hipEventCreate failed");
+    }
+    int st = GVDBX_OK;
+    (void)hipEventRecord(e0, v->stream);
+    for (int r = 0; r < reps && st == GVDBX_OK; r++) st = fn();
+    (void)hipEventRecord(e1, v->stream);
+    if (st == GVDBX_OK && hipEventSynchronize(e1) == hipSuccess) {
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        *ms = t / (float)reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gvdbx_autotune_gather(gvdbx_volume* v, int mode, int sc, float radius, int chan, int reps,
+                          gvdbx_tune_gather* out) {
+    TRY(check_vol(v));
+    if (!out || reps < 1 || (mode != 0 && mode != 1)) FAIL(GVDBX_ERR_ARG, "AutotuneGather: bad arguments");
+    if (v->clr) FAIL(GVDBX_ERR_ARG, "AutotuneGather: tune without point colours");
+    const char* what = "AutotuneGather";
+    TRY(need_atlas(v, what));
+    if (!v->sc_valid) FAIL(GVDBX_ERR_STATE, "%s: InsertPointsSubcell has not been called", what);
+    const GatherCfg keep{v->gather_variant, v->gather_classes, v->gather_split, v->xcd_remap};
+    auto apply = [&](const GatherCfg& c) {
+        v->gather_variant = c.variant;
+        v->gather_classes = c.classes;
+        v->gather_split = c.split;
+        v->xcd_remap = c.xcd;
+    };
+    auto run = [&]() -> int { return gather(v, mode, sc, radius, v->sc_len, chan, -1, 0); };
+    const int64_t words = v->cnt[0] * brick_vox(v);
+    // the configuration in place is the reference every candidate must reproduce
+    TRY(run());
+    unsigned long long ref[2];
+    TRY(checksum_dev(v, v->chan[chan].p, words, ref));
+    float ms0 = 0.0f;
+    TRY(time_calls(v, reps, run, &ms0));
+    std::vector<GatherCfg> cand;
+    for (int x = 0; x <= 1; x++) {
+        for (int sp : {0, 8, 16, 32, 64}) cand.push_back(GatherCfg{1, 1, sp, x});
+        for (int sp : {0, 32, 64, 128}) cand.push_back(GatherCfg{1, 0, sp, x});
+        cand.push_back(GatherCfg{0, 1, 0, x});
+        cand.push_back(GatherCfg{2, 1, 0, x});
+    }
+    GatherCfg best = keep;
+    float best_ms = ms0;
+    int tried = 0;
+    for (const GatherCfg& c : cand) {
+        apply(c);
+        int st = run();   // warm-up: sizes this candidate's buffers
+        unsigned long long sum[2] = {0, 0};
+        if (st == GVDBX_OK) st = checksum_dev(v, v->chan[chan].p, words, sum);
+        if (st == GVDBX_OK && (sum[0] != ref[0] || sum[1] != ref[1])) {
+            apply(keep);
+            FAIL(GVDBX_ERR_STATE, "%s: variant %d classes %d split %d xcd %d changes the result", what, c.variant,
+                 c.classes, c.split, c.xcd);
+        }
+        float ms = 0.0f;
+        if (st == GVDBX_OK) st = time_calls(v, reps, run, &ms);
+        if (st != GVDBX_OK) {
+            apply(keep);
+            return st;
+        }
+        tried++;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    apply(best);
+    TRY(run());   // leave the channel as the chosen configuration writes it (the same bits)
+    out->gather_variant = best.variant;
+    out->gather_classes = best.classes;
+    out->gather_split = best.split;
+    out->xcd_remap = best.xcd;
+    out->best_ms = best_ms;
+    out->default_ms = ms0;
+    out->candidates = tried;
+    return GVDBX_OK;
+}
+
+int gvdbx_autotune_binning(gvdbx_volume* v, int num_pnts, int sc, float radius, int reps, gvdbx_tune_binning* out) {
+    TRY(check_vol(v));
+    if (!out || reps < 1) FAIL(GVDBX_ERR_ARG, "AutotuneBinning: bad arguments");
+    const char* what = "AutotuneBinning";
+    if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "%s: not available for batched clouds", what);
+    TRY(need_atlas(v, what));
+    const int keep_rec = v->bin_records, keep_tiles = v->sort_tiles;
+    int64_t len0 = 0;
+    auto run = [&]() -> int {
+        v->bins_valid = false;
+        int64_t len = 0;
+        TRY(insert_subcell(v, sc, num_pnts, radius, 0.0f, 0.0f, 0.0f, &len, false));
+        return len == len0 ? GVDBX_OK : GVDBX_ERR_STATE;
+    };
+    auto bins_sum = [&](unsigned long long s[2]) -> int {
+        unsigned long long a[2], b[2];
+        int32_t total = 0;   // binned points (those outside the bin grid sort last, unbinned)
+        TRY(dev_to_host(v, &total, v->bin_off.as<int32_t>() + v->gbin.count(), 4));
+        TRY(checksum_dev(v, v->bin_off.p, v->gbin.count() + 1, a));
+        TRY(checksum_dev(v, v->sorted.p, (int64_t)total * 4, b));
+        s[0] = a[0] ^ (b[0] * 31);
+        s[1] = a[1] ^ (b[1] * 31);
+        return GVDBX_OK;
+    };
+    v->bins_valid = false;
+    TRY(insert_subcell(v, sc, num_pnts, radius, 0.0f, 0.0f, 0.0f, &len0, false));
+    unsigned long long ref[2];
+    TRY(bins_sum(ref));
+    float ms0 = 0.0f;
+    TRY(time_calls(v, reps, run, &ms0));
+    int best_rec = keep_rec, best_tiles = keep_tiles, tried = 0;
+    float best_ms = ms0;
+    for (int rec : {1, 0})
+        for (int tiles : {1, 2}) {
+            v->bin_records = rec;
+            v->sort_tiles = tiles;
+            int st = run();
+            unsigned long long sum[2] = {0, 0};
+            if (st == GVDBX_OK) st = bins_sum(sum);
+            if (st == GVDBX_OK && (sum[0] != ref[0] || sum[1] != ref[1])) st = GVDBX_ERR_STATE;
+            float ms = 0.0f;
+            if (st == GVDBX_OK) st = time_calls(v, reps, run, &ms);
+            if (st != GVDBX_OK) {
+                v->bin_records = keep_rec;
+                v->sort_tiles = keep_tiles;
+                v->bins_valid = false;
+                FAIL(st, "%s: records %d tiles %d changes the bins or the list length", what, rec, tiles);
+            }
+            tried++;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_rec = rec;
+                best_tiles = tiles;
+            }
+        }
+    v->bin_records = best_rec;
+    v->sort_tiles = best_tiles;
+    TRY(run());
+    out->bin_records = best_rec;
+    out->sort_tiles = best_tiles;
+    out->best_ms = best_ms;
+    out->default_ms = ms0;
+    out->candidates = tried;
+    return GVDBX_OK;
+}
+
+}  // extern "C"
+

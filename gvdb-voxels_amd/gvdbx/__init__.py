@@ -222,6 +222,22 @@ class VolumeGVDB:
              int(chan), int(accum), int(apron_chan), float(boundval), C.byref(h))
         return FrameGraph(h, self)
 
+    def AutotuneGather(self, mode, subcell_size, radius, chan=0, reps=3):
+        """gvdbx_autotune_gather: returns the chosen configuration and times."""
+        from ._lib import TuneGather
+        t = TuneGather()
+        call("gvdbx_autotune_gather", self._h, int(mode), int(subcell_size), float(radius), int(chan), int(reps),
+             C.byref(t))
+        return {k: getattr(t, k) for k, _ in TuneGather._fields_}
+
+    def AutotuneBinning(self, num_pnts, subcell_size, radius, reps=3):
+        """gvdbx_autotune_binning: returns the chosen configuration and times."""
+        from ._lib import TuneBinning
+        t = TuneBinning()
+        call("gvdbx_autotune_binning", self._h, int(num_pnts), int(subcell_size), float(radius), int(reps),
+             C.byref(t))
+        return {k: getattr(t, k) for k, _ in TuneBinning._fields_}
+
     def SaveVBX(self, fname):
         call("gvdbx_save_vbx", self._h, str(fname).encode())
 

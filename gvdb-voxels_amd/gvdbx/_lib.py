@@ -27,6 +27,16 @@ class Info(C.Structure):
                 ("log2dim", C.c_int * 10)]
 
 
+class TuneGather(C.Structure):
+    _fields_ = [("gather_variant", C.c_int), ("gather_classes", C.c_int), ("gather_split", C.c_int),
+                ("xcd_remap", C.c_int), ("best_ms", C.c_float), ("default_ms", C.c_float), ("candidates", C.c_int)]
+
+
+class TuneBinning(C.Structure):
+    _fields_ = [("bin_records", C.c_int), ("sort_tiles", C.c_int), ("best_ms", C.c_float), ("default_ms", C.c_float),
+                ("candidates", C.c_int)]
+
+
 class Camera(C.Structure):
     """gvdbx_camera: pixel (x,y), u=(x+0.5)/width, w=(y+0.5)/height, ray
     origin (u*org_u + w*org_v) + org, direction normalize((u*dir_u + w*dir_v) + dir)."""
@@ -63,6 +73,8 @@ _SIGS = {
     "gvdbx_rebuild_topology": [_vp, _i, _f, _f, _f, _f],
     "gvdbx_raytrace": [_vp, _i, _vp, _vp, _i64, _f],
     "gvdbx_save_vbx": [_vp, C.c_char_p],
+    "gvdbx_autotune_gather": [_vp, _i, _i, _f, _i, _i, _vp],
+    "gvdbx_autotune_binning": [_vp, _i, _i, _f, _i, _vp],
     "gvdbx_frame_graph_create": [_vp, _i, _i, _f, _f, _i, _i, _i, _f, C.POINTER(_vp)],
     "gvdbx_frame_graph_launch": [_vp],
     "gvdbx_frame_graph_destroy": [_vp],

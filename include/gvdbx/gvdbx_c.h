@@ -231,6 +231,32 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
  * converted by saturating truncation (cuda_gvdb_operators.cuh:139-142) */
 int gvdbx_update_apron(gvdbx_volume* v, int chan, float boundval);
 
+/* ---- autotuner (no reference counterpart) ---- */
+/* Times the kernel configurations that leave the result unchanged on the
+ * caller's own data, checks that each reproduces the configuration in
+ * place bit for bit (a position-sensitive checksum of the output), keeps
+ * the fastest and leaves it set.
+ * Gather: GatherLevelSet (mode 1) or GatherDensity (mode 0) over the
+ * current subcell lists into chan, across gather_variant {0, 1, 2},
+ * gather_classes {0, 1}, gather_split {0, 8, 16, 32, 64, 128} and
+ * xcd_remap {0, 1} (22 candidates); reps timed calls each.
+ * Binning: InsertPointsSubcell(sc, num_pnts, radius) with the bins rebuilt
+ * every call, across bin_records {0, 1} x sort_tiles {1, 2}; the bins, their
+ * order and the list length must not change. */
+typedef struct {
+    int gather_variant, gather_classes, gather_split, xcd_remap;   /* chosen */
+    float best_ms, default_ms;   /* per call: chosen, and the configuration in place before */
+    int candidates;
+} gvdbx_tune_gather;
+typedef struct {
+    int bin_records, sort_tiles;   /* chosen */
+    float best_ms, default_ms;
+    int candidates;
+} gvdbx_tune_binning;
+int gvdbx_autotune_gather(gvdbx_volume* v, int mode, int sc, float radius, int chan, int reps,
+                          gvdbx_tune_gather* out);
+int gvdbx_autotune_binning(gvdbx_volume* v, int num_pnts, int sc, float radius, int reps, gvdbx_tune_binning* out);
+
 /* ---- frame graphs (no reference counterpart) ---- */
 /* A hipGraph of one level-set frame on the current topology, for callers
  * that replay a fixed pipeline: depth frames accumulated into one topology,

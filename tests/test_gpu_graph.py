@@ -129,3 +129,30 @@ def test_frame_graph_outlives_its_volume():
         graph.launch()
     assert e.value.code == 2
     del graph
+
+
+def test_points_rewritten_in_place_are_read_again():
+    """InsertPointsSubcell reads the point buffer at call time, as the
+    reference does: rewriting the buffer in place (no SetPoints) between
+    frames on a fixed topology gives the new frame's lists and level set."""
+    import gvdbx as G
+    dev = _dev()
+    n = 50000
+    frames = _frames(n, 2, seed=9)
+    union = np.concatenate(frames)
+    ga, _ = _vol(G, union, 3.0)
+    gb, _ = _vol(G, union, 3.0)
+    buf = torch.from_numpy(frames[0]).to(dev)
+    ga.SetPoints(buf)
+    la0 = ga.InsertPointsSubcell(4, n, 3.0)
+    ga.GatherLevelSet(4, n, 1.5, (0, 0, 0), la0, 0)
+    buf.copy_(torch.from_numpy(frames[1]).to(dev))   # in place
+    la1 = ga.InsertPointsSubcell(4, n, 3.0)
+    ga.GatherLevelSet(4, n, 1.5, (0, 0, 0), la1, 0)
+    t1 = torch.from_numpy(frames[1]).to(dev)
+    gb.SetPoints(t1)
+    lb1 = gb.InsertPointsSubcell(4, n, 3.0)
+    gb.GatherLevelSet(4, n, 1.5, (0, 0, 0), lb1, 0)
+    torch.cuda.synchronize()
+    assert la1 == lb1 and la0 != la1
+    np.testing.assert_array_equal(ga.bricks(0), gb.bricks(0))
