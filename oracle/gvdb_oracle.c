@@ -42,7 +42,13 @@
  *      FP32 sum of c/255 in atomic list order, then the same division);
  *   D9 ScatterDensity without colour averaging writes the colour of the
  *      highest-numbered point of each voxel (reference: racy surface write,
- *      PT.cuh:570).
+ *      PT.cuh:570);
+ *   D10 the incremental update keeps the tree's depth and root: points
+ *      outside the root are refused with no change (reference: out-of-range
+ *      child-list reads), box cells outside the root are not emitted (D3);
+ *   D11 tricubic taps are formed in brick-local coordinates and clamp to the
+ *      padded brick (reference: floor(p + atlas offset) in FP32, reading on
+ *      into the neighbouring atlas brick).
  */
 #include "gvdb_oracle.h"
 #include <math.h>
