@@ -75,6 +75,22 @@ def measured_traffic(kernel_profile):
     return measured_value(kernel_profile, "traffic_bytes_per_launch")
 
 
+def stage_counters(profile):
+    """Per-stage kernel time and copy-calibrated HBM bytes of the frame from a
+    committed rocprofv3 summary (its first table: stage, kernel ms, FETCH GB,
+    WRITE GB, ...), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    out = {}
+    try:
+        for line in open(path):
+            c = [x.strip() for x in line.strip().strip("|").split("|")]
+            if len(c) == 6 and c[0] in ("topology", "subcell", "gather", "apron", "frame"):
+                out[c[0]] = {"kernel_ms": float(c[1]), "fetch_GB": float(c[2]), "write_GB": float(c[3])}
+    except (OSError, ValueError):
+        return None
+    return out or None
+
+
 def valu_ceiling(stream):
     """Vector-issue ceilings measured on this GPU (libgvdbx_calib.so): wave64
     VALU instructions per second of eight independent FMA chains, and of the
@@ -603,6 +619,9 @@ def main():
         "frame_roofline": {"alg_bytes": fb, "achieved_GBps": round(fb / (ms / 1e3) / 1e9, 2),
                            "frac": round(fb / (ms / 1e3) / HBM_PEAK, 5)},
         "stages_ms": {k: round(v, 4) for k, v in stage.items()},
+        # HBM bytes per stage beside the stage times (steady-state counter runs of this workload)
+        "stage_traffic": stage_counters("config2_frame_stage_counters.md") if n == 1 << 24 else None,
+        "stage_traffic_source": "profiles/config2_frame_stage_counters.md" if n == 1 << 24 else None,
         "autotune": tune,
         "tuning": {k: gv.GetTuning(k) for k in ("gather_variant", "gather_classes", "gather_split", "xcd_remap",
                                                 "bin_records", "sort_tiles")},
