@@ -43,7 +43,8 @@ struct BinSortIn {
 __device__ __forceinline__ uint32_t rec_key(const float4& r, const BinSortIn& in) {
     if (!(r.x >= 0.0f)) return in.none;
     const int cx = gvdbx_cell(r.x, in.logr0), cy = gvdbx_cell(r.y, in.logr0), cz = gvdbx_cell(r.z, in.logr0);
-    return in.gb.contains(cx, cy, cz) ? (uint32_t)in.gb.index(cx, cy, cz) : in.none;
+    const int64_t b = in.gb.find(cx, cy, cz);
+    return b >= 0 ? (uint32_t)b : in.none;
 }
 
 template <bool FIRST>

@@ -52,8 +52,8 @@ __global__ __launch_bounds__(256) void k_gather_color(ColorGatherParams cp) {
         __syncthreads();
         for (int d = 0; d < 27; d++) {
             const int bx = cx + d % 3 - 1, by = cy + (d / 3) % 3 - 1, bz = cz + d / 9 - 1;
-            if (!cp.gb.contains(bx, by, bz)) continue;
-            const int64_t b = cp.gb.index(bx, by, bz);
+            const int64_t b = cp.gb.find(bx, by, bz);
+            if (b < 0) continue;
             const int32_t beg = cp.off[b], end = cp.off[b + 1];
             for (int32_t j = beg + (int32_t)threadIdx.x; j < end; j += (int32_t)blockDim.x) {
                 const float4 q = cp.sorted[j];
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(256) void k_scatter_color(ColorScatterParams cp) {
     const int r0 = cp.res0, nvox = r0 * r0 * r0;
     const GNode nd = cp.leaves[k];
     const int cx = nd.mPos[0] >> cp.logr0, cy = nd.mPos[1] >> cp.logr0, cz = nd.mPos[2] >> cp.logr0;
-    if (!cp.gb.contains(cx, cy, cz)) return;
-    const int64_t b = cp.gb.index(cx, cy, cz);
+    const int64_t b = cp.gb.find(cx, cy, cz);
+    if (b < 0) return;
     const int32_t beg = cp.off[b], end = cp.off[b + 1];
     if (beg == end) return;
     const int br = cp.brick_res, a = cp.apron;

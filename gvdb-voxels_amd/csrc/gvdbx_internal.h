@@ -30,20 +30,81 @@ struct alignas(16) GAtlasNode {
 };
 static_assert(sizeof(GAtlasNode) == 16, "atlas map entry must be 16 bytes");
 
-// A dense cell grid over one tree level: cell (x,y,z) of the level (node
-// corner / range) maps to index ((x-ox)*ny + (y-oy))*nz + (z-oz), which is
-// the reference's sorted key order (x most significant).
+// Cells of one tree level (node corner / range) in the box [ox, ox+nx) x
+// [oy, oy+ny) x [oz, oz+nz), numbered in the reference's sorted key order
+// (x most significant).
+//
+// Dense form (ht == null): every cell of the box, cell (x,y,z) at index
+// ((x-ox)*ny + (y-oy))*nz + (z-oz).
+//
+// Sparse form (ht != null): only the ncell cells listed in `keys`, sorted,
+// each a packed key ((x-ox) << (by+bz)) | ((y-oy) << bz) | (z-oz) with by, bz
+// the bits of ny, nz (so packed order is key order); a cell's index is its
+// rank in `keys`, found through an open-addressing hash table `ht` of
+// hmask+1 slots {key lo, key hi, rank, 0} (empty slots hold key ~0, load
+// factor <= 1/2).  Used when the box holds more cells than dense per-cell
+// arrays can cover.
 struct CellGrid {
     int ox, oy, oz;
     int nx, ny, nz;
-    __host__ __device__ int64_t count() const { return (int64_t)nx * ny * nz; }
-    __host__ __device__ bool contains(int x, int y, int z) const {
+    const uint4* ht;
+    const uint64_t* keys;
+    int64_t ncell;
+    uint32_t hmask;
+    int by, bz;
+    int pad_;
+    __host__ __device__ bool sparse() const { return ht != nullptr; }
+    __host__ __device__ int64_t count() const { return ht ? ncell : (int64_t)nx * ny * nz; }
+    __host__ __device__ bool in_box(int x, int y, int z) const {
         return x >= ox && y >= oy && z >= oz && x < ox + nx && y < oy + ny && z < oz + nz;
     }
-    __host__ __device__ int64_t index(int x, int y, int z) const {
+    __host__ __device__ uint64_t pack(int x, int y, int z) const {
+        return ((uint64_t)(x - ox) << (by + bz)) | ((uint64_t)(y - oy) << bz) | (uint64_t)(z - oz);
+    }
+    __host__ __device__ static uint32_t slot_of(uint64_t k) {
+        return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 32);
+    }
+    // index of (x,y,z) in the dense form (the cell must lie in the box)
+    __host__ __device__ int64_t dense_index(int x, int y, int z) const {
         return ((int64_t)(x - ox) * ny + (y - oy)) * nz + (z - oz);
     }
+    // index of cell (x,y,z), -1 when the grid does not hold it
+    __device__ __forceinline__ int64_t find(int x, int y, int z) const {
+        if (!in_box(x, y, z)) return -1;
+        if (!ht) return dense_index(x, y, z);
+        const uint64_t k = pack(x, y, z);
+        for (uint32_t s = slot_of(k) & hmask;; s = (s + 1) & hmask) {
+            const uint4 e = ht[s];
+            const uint64_t ek = (uint64_t)e.x | ((uint64_t)e.y << 32);
+            if (ek == k) return (int64_t)e.z;
+            if (ek == ~0ull) return -1;
+        }
+    }
+    // cell of index i (0 <= i < count())
+    __device__ __forceinline__ void cell(int64_t i, int& x, int& y, int& z) const {
+        if (!ht) {
+            z = (int)(i % nz) + oz;
+            y = (int)((i / nz) % ny) + oy;
+            x = (int)(i / ((int64_t)nz * ny)) + ox;
+            return;
+        }
+        unpack(keys[i], x, y, z);
+    }
+    // cell of a packed key
+    __host__ __device__ __forceinline__ void unpack(uint64_t k, int& x, int& y, int& z) const {
+        z = (int)(k & ((1ull << bz) - 1)) + oz;
+        y = (int)((k >> bz) & ((1ull << by) - 1)) + oy;
+        x = (int)(k >> (by + bz)) + ox;
+    }
 };
+static_assert(sizeof(CellGrid) == 64, "CellGrid has no padding (grids are compared with memcmp)");
+
+// Bits needed to hold 0 .. n-1.
+__host__ __device__ __forceinline__ int gvdbx_bits_for(int64_t n) {
+    int b = 0;
+    while (((int64_t)1 << b) < n) b++;
+    return b;
+}
 
 // Point input as the reference DataPtr handles it: base + offset + stride.
 struct PointSrc {

@@ -113,6 +113,24 @@ void launch_atlas_assign(GNode* leaves, int64_t nleaf, int sx, int sy, int brick
                          GAtlasNode* amap, int64_t nmap, hipStream_t s);
 void launch_leaf_bounds(const GNode* leaves, int64_t nleaf, int range0, int32_t* out6, hipStream_t s);
 
+// ---------------- sparse cell grids (sparse.hip) ----------------
+// packed home cell (in box) of every valid point, `none` otherwise
+void launch_sp_point_keys(PointSrc src, int64_t n, float tx, float ty, float tz, CellGrid box, int logr0,
+                          uint64_t none, uint64_t* keys, hipStream_t s);
+// flag[i] = 1 at the head of each run of equal sorted keys other than `none`
+void launch_sp_heads(const uint64_t* keys, int64_t n, uint64_t none, int32_t* flag, hipStream_t s);
+// out[off[i]] = keys[i] where flag[i]
+void launch_sp_compact(const uint64_t* keys, const int32_t* flag, const int32_t* off, int64_t n, uint64_t* out,
+                       hipStream_t s);
+// (hi-lo+1)^3 outputs per input cell: the cells at offsets [lo, hi]^3, packed in go or `none`
+void launch_sp_dilate(const uint64_t* in, int64_t m, CellGrid gi, int lo, int hi, CellGrid go, uint64_t none,
+                      uint64_t* out, hipStream_t s);
+// parent (>> logres) of every input cell, packed in gp or `none`
+void launch_sp_parents(const uint64_t* in, int64_t m, CellGrid gc, int logres, CellGrid gp, uint64_t none,
+                       uint64_t* out, hipStream_t s);
+// hash table of m distinct keys (rank i for keys[i]); ht pre-filled with ~0
+void launch_sp_hash(const uint64_t* keys, int64_t m, uint4* ht, uint32_t hmask, hipStream_t s);
+
 // ---------------- point binning ----------------
 struct BinParams {
     CellGrid gb;          // binning grid (leaf grid grown by one cell)

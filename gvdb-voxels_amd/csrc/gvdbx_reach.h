@@ -39,8 +39,9 @@ __device__ __forceinline__ void reach_point(const float (&w)[3], int idx, const 
 #pragma unroll
         for (int a = 0; a < 3; a++) sh[a] = rp.cm.cell_off[3 * cl + a];
     }
-    if (!rp.g0.contains(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2])) return;
-    key = (int)rp.g0.index(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
+    const int64_t hc = rp.g0.find(c[0] + sh[0], c[1] + sh[1], c[2] + sh[2]);
+    if (hc < 0) return;
+    key = (int)hc;
     m = 1u << 13;
     if (rp.use_mask) {
         int lo[3], hi[3];

@@ -34,10 +34,11 @@ inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
 
 __global__ __launch_bounds__(256) void k_remap_idx(CellGrid from, const int32_t* src, CellGrid to, int32_t* dst) {
     const int64_t n = to.count();
-    const int64_t yz = (int64_t)to.ny * to.nz;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i / yz) + to.ox, y = (int)((i / to.nz) % to.ny) + to.oy, z = (int)(i % to.nz) + to.oz;
-        dst[i] = (src && from.contains(x, y, z)) ? src[from.index(x, y, z)] : -1;
+        int x, y, z;
+        to.cell(i, x, y, z);
+        const int64_t f = src ? from.find(x, y, z) : -1;
+        dst[i] = f >= 0 ? src[f] : -1;
     }
 }
 
@@ -69,8 +70,8 @@ __global__ __launch_bounds__(256) void k_incre_mark(PointSrc src, int64_t n, flo
             for (int x = lo[0]; x <= hi[0]; x++)
                 for (int y = lo[1]; y <= hi[1]; y++)
                     for (int z = lo[2]; z <= hi[2]; z++) {
-                        if (!g.contains(x, y, z)) continue;   // the grid holds every touched cell
-                        const int64_t c = g.index(x, y, z);
+                        const int64_t c = g.find(x, y, z);
+                        if (c < 0) continue;   // the grid holds every touched cell
                         const int32_t id = im.idx[lev][c];
                         if (id < 0) im.newf[lev][c] = 1;
                         else if (lev == 0 && id < im.nleaf) im.marker0[id] = 1;
@@ -111,8 +112,9 @@ __global__ __launch_bounds__(256) void k_incre_delink(DelinkParams dp) {
             int pc[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) pc[a] = (int)(nd.mPos[a] / dp.range_p);
-            if (!dp.gp.contains(pc[0], pc[1], pc[2])) continue;
-            p = dp.idx_p[dp.gp.index(pc[0], pc[1], pc[2])];
+            const int64_t pcell = dp.gp.find(pc[0], pc[1], pc[2]);
+            if (pcell < 0) continue;
+            p = dp.idx_p[pcell];
         }
         if (p < 0 || p >= dp.nparent) continue;
         const GNode& pn = dp.parent_nodes[p];

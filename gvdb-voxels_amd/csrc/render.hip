@@ -1090,7 +1090,8 @@ __global__ __launch_bounds__(256) void k_apron_adjoint(ApronParams ap, float* gr
         int d = threadIdx.x;
         int nx = cx + d % 3 - 1, ny = cy + (d / 3) % 3 - 1, nz = cz + d / 9 - 1;
         int32_t leaf = -1;
-        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        const int64_t ncell = (nx >= 0 && ny >= 0 && nz >= 0) ? ap.g0.find(nx, ny, nz) : -1;
+        if (ncell >= 0) leaf = ap.idx0[ncell];
         nbl[d] = (leaf >= 0 && leaf < ap.nleaf) ? leaf : -1;
     }
     __syncthreads();
@@ -1146,7 +1147,8 @@ __global__ __launch_bounds__(256) void k_apron_adjoint10(ApronParams ap, float* 
     if (lane < 27) {
         const int cx = nd.mPos[0] >> 3, cy = nd.mPos[1] >> 3, cz = nd.mPos[2] >> 3;
         const int nx = cx + lane % 3 - 1, ny = cy + (lane / 3) % 3 - 1, nz = cz + lane / 9 - 1;
-        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        const int64_t ncell = (nx >= 0 && ny >= 0 && nz >= 0) ? ap.g0.find(nx, ny, nz) : -1;
+        if (ncell >= 0) leaf = ap.idx0[ncell];
         if (leaf >= ap.nleaf) leaf = -1;
     }
     int nbl[27];
@@ -1217,8 +1219,9 @@ __global__ __launch_bounds__(256) void k_splat_backward(SplatBackParams p) {
                     const float d2 = gvdbx_d2(dx, dy, dz);
                     if (!(d2 < r2)) continue;
                     const int cx = x >> p.logr0, cy = y >> p.logr0, cz = z >> p.logr0;
-                    if (!p.g0.contains(cx + sh[0], cy + sh[1], cz + sh[2])) continue;
-                    const int32_t leaf = p.idx0[p.g0.index(cx + sh[0], cy + sh[1], cz + sh[2])];
+                    const int64_t lcell = p.g0.find(cx + sh[0], cy + sh[1], cz + sh[2]);
+                    if (lcell < 0) continue;
+                    const int32_t leaf = p.idx0[lcell];
                     if (leaf < 0 || leaf >= p.nleaf) continue;
                     const int lx = x - (cx << p.logr0) + 1, ly = y - (cy << p.logr0) + 1, lz = z - (cz << p.logr0) + 1;
                     const float g = p.grad[(int64_t)leaf * nv + (lz * br + ly) * br + lx];

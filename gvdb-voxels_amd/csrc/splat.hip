@@ -102,8 +102,8 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
         for (int dcy = -1; dcy <= 1; dcy++)
             for (int dcz = -1; dcz <= 1; dcz++) {
                 int bx = cx + dcx, by = cy + dcy, bz = cz + dcz;
-                if (!gp.gb.contains(bx, by, bz)) continue;
-                int64_t ci = gp.gb.index(bx, by, bz);
+                const int64_t ci = gp.gb.find(bx, by, bz);
+                if (ci < 0) continue;
                 int beg = gp.off[ci], end = gp.off[ci + 1];
                 for (int j0 = beg; j0 < end; j0 += 64) {
                     int j = j0 + lane;
@@ -231,8 +231,8 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
         for (int dcy = -1; dcy <= 1; dcy++)
             for (int dcz = -1; dcz <= 1; dcz++) {
                 int bx = cx + dcx, by = cy + dcy, bz = cz + dcz;
-                if (!gp.gb.contains(bx, by, bz)) continue;
-                int64_t ci = gp.gb.index(bx, by, bz);
+                const int64_t ci = gp.gb.find(bx, by, bz);
+                if (ci < 0) continue;
                 int beg = gp.off[ci], end = gp.off[ci + 1];
                 for (int j0 = beg; j0 < end; j0 += 64) {
                     int j = j0 + lane;
@@ -643,8 +643,8 @@ __device__ __forceinline__ bool skip_chunks(ChunkIter& it, const GatherParams& g
     while (++it.d < 27) {
         const int nb = NB_ORDER[it.d];
         int bx = cx + nb / 9 - 1, by = cy + (nb / 3) % 3 - 1, bz = cz + nb % 3 - 1;
-        if (!gp.gb.contains(bx, by, bz)) continue;
-        int64_t ci = gp.gb.index(bx, by, bz);
+        const int64_t ci = gp.gb.find(bx, by, bz);
+        if (ci < 0) continue;
         int beg = gp.off[ci], end = gp.off[ci + 1];
         if (beg < end) {
             const int nch = (end - beg + 63) >> 6;
@@ -695,8 +695,8 @@ __device__ __forceinline__ int build_runs(const GatherParams& gp, int cx, int cy
     if (lane < 27) {
         const int nb = NB_ORDER[lane];
         const int ox = nb / 9 - 1, oy = (nb / 3) % 3 - 1, oz = nb % 3 - 1;
-        if (gp.gb.contains(cx + ox, cy + oy, cz + oz)) {
-            const int64_t ci = gp.gb.index(cx + ox, cy + oy, cz + oz);
+        const int64_t ci = gp.gb.find(cx + ox, cy + oy, cz + oz);
+        if (ci >= 0) {
 #pragma unroll
             for (int k = 0; k < 9; k++) c[k] = gp.coff[ci * 8 + k];
             if (c[8] > c[0]) want = class_mask(ox, oy, oz, gp.cboth[ci]);
@@ -1051,8 +1051,8 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
         } else {
             if (lane < 27) {
                 const int bx = g.cx + lane / 9 - 1, by = g.cy + (lane / 3) % 3 - 1, bz = g.cz + lane % 3 - 1;
-                if (gp.gb.contains(bx, by, bz)) {
-                    const int64_t ci = gp.gb.index(bx, by, bz);
+                const int64_t ci = gp.gb.find(bx, by, bz);
+                if (ci >= 0) {
                     nch = (gp.off[ci + 1] - gp.off[ci] + 63) >> 6;
                 }
             }
@@ -1176,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_splat_src(const float4* sorted, int64_t
 __device__ __forceinline__ int64_t nb_bin(const CellGrid& gb, int d, int cx, int cy, int cz) {
     const int nb = NB_ORDER[d];
     const int bx = cx + nb / 9 - 1, by = cy + (nb / 3) % 3 - 1, bz = cz + nb % 3 - 1;
-    return gb.contains(bx, by, bz) ? gb.index(bx, by, bz) : -1;
+    return gb.find(bx, by, bz);
 }
 
 // ---- cell push ----
@@ -1224,8 +1224,8 @@ __global__ __launch_bounds__(64) void k_splat_cell(SplatParams sp, const int32_t
     const int end = (int)min((int64_t)sp.off[c + 1], (int64_t)beg + chunk);
     for (int i = lane; i < G3; i += 64) grid[i] = 0.0f;
     // cell coordinates; batched clouds: shift to the cloud's own voxel space
-    const int64_t nyz = (int64_t)sp.gb.ny * sp.gb.nz;
-    int cx = (int)(c / nyz) + sp.gb.ox, cy = (int)((c / sp.gb.nz) % sp.gb.ny) + sp.gb.oy, cz = (int)(c % sp.gb.nz) + sp.gb.oz;
+    int cx, cy, cz;
+    sp.gb.cell(c, cx, cy, cz);
     if (sp.cm.ncloud > 0) {
         const int cl = sp.cm.cloud_of_point(__float_as_int(sp.bins[beg].w));
         cx -= sp.cm.cell_off[3 * cl];
@@ -1402,12 +1402,13 @@ __global__ __launch_bounds__(64) void k_splat_cell_backward(SplatBackParams p, C
     const int sub = b - run_off[c];
     const int beg = off[c] + sub * chunk;
     const int end = (int)min((int64_t)off[c + 1], (int64_t)beg + chunk);
-    const int64_t nyz = (int64_t)gb.ny * gb.nz;
-    const int gcx = (int)(c / nyz) + gb.ox, gcy = (int)((c / gb.nz) % gb.ny) + gb.oy, gcz = (int)(c % gb.nz) + gb.oz;
+    int gcx, gcy, gcz;
+    gb.cell(c, gcx, gcy, gcz);
     if (lane < 27) {
         const int nx = gcx + lane / 9 - 1, ny = gcy + (lane / 3) % 3 - 1, nz = gcz + lane % 3 - 1;
         int32_t leaf = -1;
-        if (p.g0.contains(nx, ny, nz)) leaf = p.idx0[p.g0.index(nx, ny, nz)];
+        const int64_t ncell = p.g0.find(nx, ny, nz);
+        if (ncell >= 0) leaf = p.idx0[ncell];
         nbleaf[lane] = (leaf >= 0 && leaf < p.nleaf) ? leaf : -1;
     }
     __syncthreads();
@@ -1643,7 +1644,8 @@ __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
     if (lane < 27) {
         const int cx = nd.mPos[0] >> 3, cy = nd.mPos[1] >> 3, cz = nd.mPos[2] >> 3;
         const int nx = cx + lane % 3 - 1, ny = cy + (lane / 3) % 3 - 1, nz = cz + lane / 9 - 1;
-        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        const int64_t ncell = (nx >= 0 && ny >= 0 && nz >= 0) ? ap.g0.find(nx, ny, nz) : -1;
+        if (ncell >= 0) leaf = ap.idx0[ncell];
         if (leaf >= ap.nleaf) leaf = -1;
     }
     // voxels are moved as 4-byte words: float or packed uchar4 channels alike
@@ -1682,7 +1684,8 @@ __global__ __launch_bounds__(256) void k_apron(ApronParams ap) {
         int d = threadIdx.x;
         int nx = cx + d % 3 - 1, ny = cy + (d / 3) % 3 - 1, nz = cz + d / 9 - 1;
         int32_t leaf = -1;
-        if (nx >= 0 && ny >= 0 && nz >= 0 && ap.g0.contains(nx, ny, nz)) leaf = ap.idx0[ap.g0.index(nx, ny, nz)];
+        const int64_t ncell = (nx >= 0 && ny >= 0 && nz >= 0) ? ap.g0.find(nx, ny, nz) : -1;
+        if (ncell >= 0) leaf = ap.idx0[ncell];
         nbl[d] = (leaf >= 0 && leaf < ap.nleaf) ? leaf : -1;
     }
     __syncthreads();
@@ -1714,10 +1717,11 @@ __global__ __launch_bounds__(256) void k_insert_points(InsertParams ip) {
         int i = __float_as_int(p.w);
         int c[3] = {gvdbx_cell(p.x, ip.logr0), gvdbx_cell(p.y, ip.logr0), gvdbx_cell(p.z, ip.logr0)};
         int32_t leaf = -1;
-        if (ip.g0.contains(c[0], c[1], c[2])) leaf = ip.idx0[ip.g0.index(c[0], c[1], c[2])];
+        const int64_t lcell = ip.g0.find(c[0], c[1], c[2]);
+        if (lcell >= 0) leaf = ip.idx0[lcell];
         if (leaf >= 0) {
             ip.pnode[i] = leaf;
-            ip.poff[i] = (int32_t)(j - ip.off[ip.gb.index(c[0], c[1], c[2])]);
+            ip.poff[i] = (int32_t)(j - ip.off[ip.gb.find(c[0], c[1], c[2])]);   // a leaf's cell is a bin
         }
     }
 }
@@ -1747,8 +1751,7 @@ __global__ __launch_bounds__(128) void k_scatter_density(ScatterParams sp) {
     float* brick = sp.chan + k * (int64_t)br * br * br;
     int c[3] = {nd.mPos[0] / r0, nd.mPos[1] / r0, nd.mPos[2] / r0};
     int beg = 0, end = 0;
-    if (sp.gb.contains(c[0], c[1], c[2])) {
-        int64_t ci = sp.gb.index(c[0], c[1], c[2]);
+    if (const int64_t ci = sp.gb.find(c[0], c[1], c[2]); ci >= 0) {
         beg = sp.off[ci];
         end = sp.off[ci + 1];
     }

@@ -137,10 +137,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_flag_apply(FlagApplyBatch fb) 
             else if (v[k].b) node = (int32_t)(np.node_off + nbase + run.b);
             if (!np.keep_idx || node >= 0) np.idx[i] = node;
             if (node >= 0) {
-                int64_t yz = (int64_t)g.ny * g.nz;
-                int cx = (int)(i / yz) + g.ox;
-                int cy = (int)((i / g.nz) % g.ny) + g.oy;
-                int cz = (int)(i % g.nz) + g.oz;
+                int cx, cy, cz;
+                g.cell(i, cx, cy, cz);
                 GNode nd;
                 nd.mLev = (uint8_t)np.lev;
                 nd.mFlags = 1;
@@ -322,7 +320,7 @@ __global__ __launch_bounds__(256) void k_mark_boxes(PointSrc src, int64_t n, flo
         for (int x = lo[0]; x <= hi[0]; x++)
             for (int y = lo[1]; y <= hi[1]; y++)
                 for (int z = lo[2]; z <= hi[2]; z++)
-                    if (g0.contains(x, y, z)) all0[g0.index(x, y, z)] = 1;
+                    if (const int64_t c = g0.find(x, y, z); c >= 0) all0[c] = 1;
     }
 }
 
@@ -335,17 +333,15 @@ __global__ __launch_bounds__(256) void k_expand_reach(CellGrid g0, const uint32_
         if (!m) continue;
         all0[i] = 1;
         if (generic) continue;
-        int64_t yz = (int64_t)g0.ny * g0.nz;
-        int cx = (int)(i / yz) + g0.ox, cy = (int)((i / g0.nz) % g0.ny) + g0.oy, cz = (int)(i % g0.nz) + g0.oz;
+        int cx, cy, cz;
+        g0.cell(i, cx, cy, cz);
         m &= ~(1u << 13);
         while (m) {
             int b = __ffs(m) - 1;
             m &= m - 1;
             int dx = b % 3 - 1, dy = (b / 3) % 3 - 1, dz = b / 9 - 1;
-            if (g0.contains(cx + dx, cy + dy, cz + dz)) {
-                const int64_t t = g0.index(cx + dx, cy + dy, cz + dz);
-                if (!allowed || allowed[t] >= 0) all0[t] = 1;
-            }
+            const int64_t t = g0.find(cx + dx, cy + dy, cz + dz);
+            if (t >= 0 && (!allowed || allowed[t] >= 0)) all0[t] = 1;
         }
     }
 }
@@ -355,8 +351,8 @@ __global__ __launch_bounds__(256) void k_level_up(CellGrid gc, const uint8_t* ba
     int64_t n = gp.count();
     int res = 1 << logres;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t yz = (int64_t)gp.ny * gp.nz;
-        int px = (int)(i / yz) + gp.ox, py = (int)((i / gp.nz) % gp.ny) + gp.oy, pz = (int)(i % gp.nz) + gp.oz;
+        int px, py, pz;
+        gp.cell(i, px, py, pz);
         int x0 = max(px * res, gc.ox), x1 = min(px * res + res, gc.ox + gc.nx);
         int y0 = max(py * res, gc.oy), y1 = min(py * res + res, gc.oy + gc.ny);
         int z0 = max(pz * res, gc.oz), z1 = min(pz * res + res, gc.oz + gc.nz);
@@ -364,7 +360,8 @@ __global__ __launch_bounds__(256) void k_level_up(CellGrid gc, const uint8_t* ba
         for (int x = x0; x < x1 && !(b && a); x++)
             for (int y = y0; y < y1; y++)
                 for (int z = z0; z < z1; z++) {
-                    int64_t ci = gc.index(x, y, z);
+                    const int64_t ci = gc.find(x, y, z);
+                    if (ci < 0) continue;   // a sparse grid lists only some cells
                     b |= base_c[ci];
                     a |= all_c[ci];
                 }
@@ -407,8 +404,9 @@ __device__ __forceinline__ void link_one(const LinkParams& lp, int64_t k) {
         int pc[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) pc[a] = (int)(c.mPos[a] / lp.range_p);
-        if (!lp.gp.contains(pc[0], pc[1], pc[2])) return;
-        p = lp.idx_p[lp.gp.index(pc[0], pc[1], pc[2])];
+        const int64_t pcell = lp.gp.find(pc[0], pc[1], pc[2]);
+        if (pcell < 0) return;
+        p = lp.idx_p[pcell];
         if (p < 0 || p >= lp.nparent) return;
     }
     const GNode& pn = lp.parent_nodes[p];
@@ -575,8 +573,9 @@ __device__ __forceinline__ unsigned long long subcell_count(const float4& p, con
         near = near && c0[a] >= hc[a] - 1 && c1[a] <= hc[a] + 1;
     }
     unsigned long long cnt = 0;
-    if (near && bp.g0.contains(hc[0], hc[1], hc[2])) {
-        const uint32_t m = bp.nbmask[bp.g0.index(hc[0], hc[1], hc[2])];
+    const int64_t hcell = near ? bp.g0.find(hc[0], hc[1], hc[2]) : -1;
+    if (hcell >= 0) {
+        const uint32_t m = bp.nbmask[hcell];
         // n[a][d]: the box's subcells along axis a inside cell hc[a] + d - 1
         // (the subcells_in_cell count: box and cell ends are multiples of
         // sc, so b - a + sc over sc is the count, <= 0 when empty); at most
@@ -614,8 +613,9 @@ __device__ __forceinline__ unsigned long long subcell_count(const float4& p, con
             if (ny == 0) continue;
             for (int cz = c0[2]; cz <= c1[2]; cz++) {
                 const int nz = subcells_in_cell(lo[2], hi[2], cz, r0, sc, bp.sc_shift);
-                if (nz == 0 || !bp.g0.contains(cx, cy, cz)) continue;
-                const int64_t ci = bp.g0.index(cx, cy, cz);
+                if (nz == 0) continue;
+                const int64_t ci = bp.g0.find(cx, cy, cz);
+                if (ci < 0) continue;
                 if ((bp.nbmask[ci] >> 13) & 1u) cnt += (unsigned long long)(nx * ny * nz);   // the cell itself
             }
         }
@@ -672,7 +672,8 @@ __global__ __launch_bounds__(256) void k_bin_keys(PointSrc src, int64_t n, float
 #pragma unroll
                 for (int a = 0; a < 3; a++) c[a] += cm.cell_off[3 * cl + a];
             }
-            if (gb.contains(c[0], c[1], c[2])) k = (uint32_t)gb.index(c[0], c[1], c[2]);
+            const int64_t b = gb.find(c[0], c[1], c[2]);
+            if (b >= 0) k = (uint32_t)b;
         }
         key[i] = k;
         if (val) val[i] = (uint32_t)i;
@@ -742,12 +743,13 @@ __global__ __launch_bounds__(256) void k_leaf_nbmask(const int32_t* idx0, CellGr
     const int64_t n = g0.count();
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0ull;   // a counter the next pass adds into
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
-        const int z = (int)(c % g0.nz) + g0.oz, y = (int)((c / g0.nz) % g0.ny) + g0.oy;
-        const int x = (int)(c / ((int64_t)g0.nz * g0.ny)) + g0.ox;
+        int x, y, z;
+        g0.cell(c, x, y, z);
         uint32_t m = 0;
         for (int d = 0; d < 27; d++) {
             const int nx = x + d % 3 - 1, ny = y + (d / 3) % 3 - 1, nz = z + d / 9 - 1;
-            if (g0.contains(nx, ny, nz) && idx0[g0.index(nx, ny, nz)] >= 0) m |= 1u << d;
+            const int64_t nc = g0.find(nx, ny, nz);
+            if (nc >= 0 && idx0[nc] >= 0) m |= 1u << d;
         }
         mask[c] = m;
     }
@@ -829,7 +831,7 @@ __global__ __launch_bounds__(256) void k_paint_clouds(CellGrid g0, const int32_t
     const int64_t nb = nx * ny * nz;
     for (int64_t t = c0 + threadIdx.x; t < min(c0 + (int64_t)CELL_CHUNK, nb); t += blockDim.x) {
         const int x = b[0] + (int)(t / (ny * nz)), y = b[1] + (int)((t / nz) % ny), z = b[2] + (int)(t % nz);
-        if (g0.contains(x, y, z)) cell_cloud[g0.index(x, y, z)] = cl;
+        if (const int64_t c = g0.find(x, y, z); c >= 0) cell_cloud[c] = cl;
     }
 }
 
@@ -837,7 +839,8 @@ __global__ __launch_bounds__(256) void k_leaf_clouds(const GNode* leaves, int64_
                                                      const int32_t* cell_cloud, int32_t* leaf_cloud) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nleaf; k += (int64_t)gridDim.x * blockDim.x) {
         const int x = leaves[k].mPos[0] >> logr0, y = leaves[k].mPos[1] >> logr0, z = leaves[k].mPos[2] >> logr0;
-        leaf_cloud[k] = g0.contains(x, y, z) ? cell_cloud[g0.index(x, y, z)] : -1;
+        const int64_t c = g0.find(x, y, z);
+        leaf_cloud[k] = c >= 0 ? cell_cloud[c] : -1;
     }
 }
 

@@ -171,6 +171,11 @@ struct gvdbx_volume {
     DevBuf gcctr, gcunit, gcbig, gcucnt;            // Gather*: units of large bins (counters, units, bins, counts)
     DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
+    // sparse topology: per level the sorted cell keys and their hash table,
+    // the same for the bin grid, and construction scratch
+    DevBuf sg_keys[GVDBX_MAXLEV], sg_ht[GVDBX_MAXLEV], sg_bkeys, sg_bht;
+    DevBuf sg_k1, sg_k2, sg_v1, sg_v2, sg_flag, sg_off, sg_part;
+    CellGrid gb_sparse = {};   // bin grid of a sparse topology
     bool bins_valid = false;
     int64_t bins_n = -1;
     int64_t pts_valid = -1;   // valid points of the last RebuildTopology (-1: unknown)
@@ -438,7 +443,8 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     (void)stable;   // sorted bins are always in ascending point order
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
     const CellGrid& g0 = v->grid[0];
-    CellGrid gb = {g0.ox - 1, g0.oy - 1, g0.oz - 1, g0.nx + 2, g0.ny + 2, g0.nz + 2};
+    // the leaf grid grown by one cell (a sparse topology lists its bin cells)
+    const CellGrid gb = g0.sparse() ? v->gb_sparse : CellGrid{g0.ox - 1, g0.oy - 1, g0.oz - 1, g0.nx + 2, g0.ny + 2, g0.nz + 2};
     int64_t nb = gb.count();
     if (nb >= (int64_t)UINT32_MAX) FAIL(GVDBX_ERR_RANGE, "bin grid exceeds 32-bit cell keys");
     bool same = v->bins_valid && v->bins_gen == v->pos_gen && v->bins_n == n && v->bins_base == v->pos.base &&
@@ -579,6 +585,131 @@ int frame_of(const gvdbx_volume* v, const BBoxAccum& bb, float radius, const cha
     return GVDBX_OK;
 }
 
+// ---- sparse cell grids (sparse.hip) ----
+// Sorts the n keys in sg_k1 (packed keys of `bits` bits; `none` = 2^bits
+// sorts last) with the LSD radix sort, flags the head of every run, and
+// stores the distinct keys, in order, in `out` (their ranks are the prefix
+// sum of the flags).
+int sg_sort_unique(gvdbx_volume* v, int64_t n, int bits, DevBuf& out, int64_t* count) {
+    hipStream_t s = v->stream;
+    *count = 0;
+    if (n <= 0) return ensure(v, out, 8);
+    const uint64_t none = 1ull << bits;
+    const int64_t nh = radix_hist_entries(n);
+    const int64_t st = scan_tiles(n + 1);
+    TRY(ensure(v, v->sg_k2, (size_t)n * 8));
+    TRY(ensure(v, v->sg_v1, (size_t)n * 4));
+    TRY(ensure(v, v->sg_v2, (size_t)n * 4));
+    TRY(ensure(v, v->rs_hist, (size_t)nh * 4));
+    TRY(ensure(v, v->rs_off, (size_t)(nh + 1) * 4));
+    TRY(ensure(v, v->rs_part, (size_t)(scan_tiles(nh + 1) + 1) * 4));
+    TRY(ensure(v, v->sg_flag, (size_t)(n + 1) * 4));
+    TRY(ensure(v, v->sg_off, (size_t)(n + 1) * 4));
+    TRY(ensure(v, v->sg_part, (size_t)(st + 1) * 4));
+    TRY(ensure_host(v, 4096));
+    const int flip = radix_sort_pairs(v->sg_k1.as<uint64_t>(), v->sg_v1.as<uint32_t>(), v->sg_k2.as<uint64_t>(),
+                                      v->sg_v2.as<uint32_t>(), n, 0, bits + 1, v->rs_hist.as<int32_t>(),
+                                      v->rs_off.as<int32_t>(), v->rs_part.as<uint32_t>(), s, true);
+    const uint64_t* sk = flip ? v->sg_k2.as<uint64_t>() : v->sg_k1.as<uint64_t>();
+    int32_t* flag = v->sg_flag.as<int32_t>();
+    int32_t* off = v->sg_off.as<int32_t>();
+    uint32_t* part = v->sg_part.as<uint32_t>();
+    launch_sp_heads(sk, n, none, flag, s);
+    launch_count_scan_reduce(flag, n, part, part + st, s);
+    launch_count_scan_apply(flag, n, part, off, s);
+    TRY(check_launch("sparse cell keys"));
+    GVDBX_CHECK(hipMemcpyAsync(v->host, part + st, 4, hipMemcpyDeviceToHost, s));
+    TRY(sync(v));
+    uint32_t m;
+    memcpy(&m, v->host, 4);
+    *count = m;
+    TRY(ensure(v, out, (size_t)std::max<int64_t>(m, 1) * 8));
+    launch_sp_compact(sk, flag, off, n, out.as<uint64_t>(), s);
+    return check_launch("sparse cell keys");
+}
+
+// Hash table of the `count` keys in `keys` (rank i for key i) into ht; sets
+// the grid's sparse fields.
+int sg_table(gvdbx_volume* v, DevBuf& keys, int64_t count, DevBuf& ht, CellGrid& g) {
+    uint32_t cap = 64;
+    while ((int64_t)cap < 2 * count) cap <<= 1;
+    TRY(ensure(v, ht, (size_t)cap * 16));
+    launch_fill_u64(ht.as<uint64_t>(), (int64_t)cap * 2, ~0ull, v->stream);
+    launch_sp_hash(keys.as<uint64_t>(), count, ht.as<uint4>(), cap - 1, v->stream);
+    TRY(check_launch("sparse cell table"));
+    g.ht = ht.as<uint4>();
+    g.keys = keys.as<uint64_t>();
+    g.ncell = count;
+    g.hmask = cap - 1;
+    return GVDBX_OK;
+}
+
+// Sparse grids for a topology whose level-0 box is too large for dense
+// per-cell arrays.  The boxes stay those of the dense grids; only the listed
+// cells get per-cell storage.  Level 0 lists the points' home cells dilated
+// by the extra-brick reach (a superset of every leaf the topology can make:
+// a point's box of gvdbCalcExtraBrickId, cuda_gvdb_particles.cuh:493-494,
+// spans cells home - ceil(ceil(r)/r0) .. home + floor((r0 - 1 + ceil(2r))/r0));
+// each level above lists the parents of the one below; the bin grid lists
+// the level-0 cells dilated by one more cell (every bin a leaf's
+// neighbourhood reads).
+int build_sparse_grids(gvdbx_volume* v, int64_t n, const float orig[3], float radius, bool extra, int depth,
+                       CellGrid* grid, CellGrid& gb) {
+    hipStream_t s = v->stream;
+    const int r0 = v->res[0];
+    int bits[GVDBX_MAXLEV];
+    for (int l = 0; l < depth; l++) {
+        CellGrid& g = grid[l];
+        g.by = gvdbx_bits_for(g.ny);
+        g.bz = gvdbx_bits_for(g.nz);
+        bits[l] = gvdbx_bits_for(g.nx) + g.by + g.bz;
+    }
+    gb = CellGrid{grid[0].ox - 1, grid[0].oy - 1, grid[0].oz - 1, grid[0].nx + 2, grid[0].ny + 2, grid[0].nz + 2};
+    gb.by = gvdbx_bits_for(gb.ny);
+    gb.bz = gvdbx_bits_for(gb.nz);
+    const int bitsb = gvdbx_bits_for(gb.nx) + gb.by + gb.bz;
+    const int64_t lim = ((int64_t)1 << 31) - 1;
+    // home cells of the points
+    TRY(ensure(v, v->sg_k1, (size_t)std::max<int64_t>(n, 1) * 8));
+    launch_sp_point_keys(v->pos, n, orig[0], orig[1], orig[2], grid[0], v->logdim[0], 1ull << bits[0],
+                         v->sg_k1.as<uint64_t>(), s);
+    int64_t nhome = 0;
+    TRY(sg_sort_unique(v, n, bits[0], v->sg_keys[0], &nhome));
+    // level 0: dilated by the extra-brick reach
+    int64_t nc[GVDBX_MAXLEV] = {};
+    nc[0] = nhome;
+    if (extra) {
+        const int cr = (int)ceilf(radius), c2r = (int)ceilf(2.0f * radius);
+        const int mlo = (cr + r0 - 1) / r0, mhi = (r0 - 1 + c2r) / r0;
+        const int w = mlo + mhi + 1;
+        const int64_t nd = nhome * w * w * w;
+        if (nd > lim) FAIL(GVDBX_ERR_RANGE, "RebuildTopology: %lld candidate cells exceed the sparse-grid limit", (long long)nd);
+        TRY(ensure(v, v->sg_k1, (size_t)std::max<int64_t>(nd, 1) * 8));
+        launch_sp_dilate(v->sg_keys[0].as<uint64_t>(), nhome, grid[0], -mlo, mhi, grid[0], 1ull << bits[0],
+                         v->sg_k1.as<uint64_t>(), s);
+        TRY(sg_sort_unique(v, nd, bits[0], v->sg_keys[0], &nc[0]));
+    }
+    // bin grid: one more cell around
+    {
+        const int64_t nd = nc[0] * 27;
+        if (nd > lim) FAIL(GVDBX_ERR_RANGE, "RebuildTopology: %lld bin cells exceed the sparse-grid limit", (long long)nd);
+        TRY(ensure(v, v->sg_k1, (size_t)std::max<int64_t>(nd, 1) * 8));
+        launch_sp_dilate(v->sg_keys[0].as<uint64_t>(), nc[0], grid[0], -1, 1, gb, 1ull << bitsb,
+                         v->sg_k1.as<uint64_t>(), s);
+    }
+    int64_t nbin = 0;
+    TRY(sg_sort_unique(v, nc[0] * 27, bitsb, v->sg_bkeys, &nbin));
+    // levels above: parents
+    for (int l = 1; l < depth; l++) {
+        TRY(ensure(v, v->sg_k1, (size_t)std::max<int64_t>(nc[l - 1], 1) * 8));
+        launch_sp_parents(v->sg_keys[l - 1].as<uint64_t>(), nc[l - 1], grid[l - 1], v->logdim[l], grid[l],
+                          1ull << bits[l], v->sg_k1.as<uint64_t>(), s);
+        TRY(sg_sort_unique(v, nc[l - 1], bits[l], v->sg_keys[l], &nc[l]));
+    }
+    for (int l = 0; l < depth; l++) TRY(sg_table(v, v->sg_keys[l], nc[l], v->sg_ht[l], grid[l]));
+    return sg_table(v, v->sg_bkeys, nbin, v->sg_bht, gb);
+}
+
 int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[3]) {
     if (!v->configured) FAIL(GVDBX_ERR_STATE, "Configure has not been called");
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "SetPoints has not been called");
@@ -716,8 +847,16 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
         g.ny = (int)(hi_v[1] / rg) - g.oy + 1;
         g.nz = (int)(hi_v[2] / rg) - g.oz + 1;
     }
-    if (grid[0].count() > MAX_DENSE_CELLS)
-        FAIL(GVDBX_ERR_RANGE, "RebuildTopology: %lld level-0 cells exceed the dense-grid limit", (long long)grid[0].count());
+    // cell grids: dense over the box, or sparse (the cells the points can
+    // reach, found by sorting cell keys) when the box is too large or asked for
+    const bool sparse = v->ncloud == 0 && (v->topo_mode == GVDBX_TOPO_RADIX ||
+                                           (v->topo_mode == GVDBX_TOPO_AUTO && grid[0].count() > MAX_DENSE_CELLS));
+    if (!sparse && grid[0].count() > MAX_DENSE_CELLS)
+        FAIL(GVDBX_ERR_RANGE, "RebuildTopology: %lld level-0 cells exceed the dense-grid limit%s",
+             (long long)grid[0].count(), v->ncloud ? " (batched clouds)" : " (topology mode DENSE)");
+    CellGrid gb_sparse = {};
+    if (sparse) TRY(build_sparse_grids(v, n, orig, radius, extra, depth, grid, gb_sparse));
+    v->gb_sparse = gb_sparse;
     const int32_t* allowed = nullptr;
     if (v->ncloud > 0) {
         // cell -> cloud over every cloud's cell box; extras only land inside their cloud's box
@@ -882,7 +1021,7 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     v->has_topo = true;
     v->has_atlas = false;
     v->sc_valid = false;
-    v->topo_mode_used = GVDBX_TOPO_DENSE;
+    v->topo_mode_used = sparse ? GVDBX_TOPO_RADIX : GVDBX_TOPO_DENSE;
     v->leaf_inactive = false;
     v->force_full = false;
     return GVDBX_OK;
@@ -903,6 +1042,13 @@ int incremental_topology(gvdbx_volume* v, int64_t n, float radius, const float o
     if (!v->has_pos) FAIL(GVDBX_ERR_STATE, "%s: SetPoints has not been called", what);
     if (!v->has_topo || v->cnt[0] == 0) FAIL(GVDBX_ERR_STATE, "%s: no topology to update", what);
     if (v->ncloud > 0) FAIL(GVDBX_ERR_STATE, "%s: no incremental update for batched clouds", what);
+    if (v->grid[0].sparse()) {   // the update grows dense grids: a sparse topology is rebuilt in full
+        if (full) {
+            *full = true;
+            return GVDBX_OK;
+        }
+        FAIL(GVDBX_ERR_STATE, "%s: no incremental update of a sparse (radix-sorted) topology", what);
+    }
     if (v->top_lev != v->depth) FAIL(GVDBX_ERR_STATE, "%s: the tree's top level is below its root", what);
     hipStream_t s = v->stream;
     const int depth = v->depth;
@@ -1453,8 +1599,11 @@ int gvdbx_destroy(gvdbx_volume* v) {
         free_buf(v->idx2[l]);
         free_buf(v->flag_base[l]);
         free_buf(v->flag_all[l]);
+        free_buf(v->sg_keys[l]);
+        free_buf(v->sg_ht[l]);
     }
-    DevBuf* bufs[] = {&v->inc_mark, &v->atl_flag, &v->atl_rank, &v->reach0, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
+    DevBuf* bufs[] = {&v->sg_bkeys, &v->sg_bht, &v->sg_k1, &v->sg_k2, &v->sg_v1, &v->sg_v2, &v->sg_flag, &v->sg_off, &v->sg_part,
+                      &v->inc_mark, &v->atl_flag, &v->atl_rank, &v->reach0, &v->nbmask, &v->partials, &v->totals, &v->bbox, &v->misc, &v->amap,
                       &v->bin_off, &v->sorted, &v->sorted2, &v->bkey, &v->bval, &v->bkey2, &v->bval2, &v->rs_hist,
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
@@ -1573,7 +1722,6 @@ int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value) {
 int gvdbx_set_topology_mode(gvdbx_volume* v, int mode) {
     TRY(check_vol(v));
     if (mode < GVDBX_TOPO_AUTO || mode > GVDBX_TOPO_RADIX) FAIL(GVDBX_ERR_ARG, "unknown topology mode %d", mode);
-    if (mode == GVDBX_TOPO_RADIX) FAIL(GVDBX_ERR_ARG, "radix topology mode is not available in this build");
     v->topo_mode = mode;
     return GVDBX_OK;
 }
@@ -2588,7 +2736,7 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
         std::vector<int32_t> idx((size_t)grid[l].count(), -1);
         for (int32_t k : reach[l]) {
             const GNode& n = hn[l][k];
-            idx[(size_t)grid[l].index((int)(n.mPos[0] / v->range[l]), (int)(n.mPos[1] / v->range[l]),
+            idx[(size_t)grid[l].dense_index((int)(n.mPos[0] / v->range[l]), (int)(n.mPos[1] / v->range[l]),
                                       (int)(n.mPos[2] / v->range[l]))] = k;
         }
         TRY(ensure(v, v->idx[l], std::max<size_t>(idx.size(), 1) * 4));

@@ -42,9 +42,16 @@ enum gvdbx_status {
  * voxel in the same brick layout (uchar4: x in the lowest byte) */
 enum gvdbx_dtype { GVDBX_UCHAR4 = 2, GVDBX_FLOAT = 3 };
 
+/* How RebuildTopology finds the cells of each tree level.  DENSE: per-cell
+ * arrays over the points' box (fast; GVDBX_ERR_RANGE beyond 2^28 level-0
+ * cells).  RADIX: the occupied cells only, found by an LSD radix sort of
+ * 64-bit cell keys with run-head dedup (the reference's RadixSortByByte +
+ * FindUnique path), looked up through a hash table; any box within the
+ * 16-bit-per-axis key range.  AUTO: DENSE when the box fits, else RADIX.
+ * Batched clouds always use DENSE (their packed space is compact); an
+ * incremental update of a RADIX topology rebuilds it in full. */
 enum gvdbx_topology_mode {
-    GVDBX_TOPO_AUTO = 0,       /* dense cell grids (RADIX is not available: AUTO
-                                  fails with GVDBX_ERR_RANGE beyond 2^28 level-0 cells) */
+    GVDBX_TOPO_AUTO = 0,
     GVDBX_TOPO_DENSE = 1,
     GVDBX_TOPO_RADIX = 2
 };
