@@ -2719,9 +2719,12 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
             }
         grid[l] = CellGrid{lo[0], lo[1], lo[2], hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
     }
-    if (grid[0].count() > MAX_DENSE_CELLS)
-        FAIL(GVDBX_ERR_RANGE, "LoadVBX: %s: the leaves span %lld cells, beyond the dense-grid limit", path,
-             (long long)grid[0].count());
+    // dense grids over the nodes' boxes, or sparse ones listing the node cells
+    const bool sparse = v->topo_mode == GVDBX_TOPO_RADIX ||
+                        (v->topo_mode == GVDBX_TOPO_AUTO && grid[0].count() > MAX_DENSE_CELLS);
+    if (!sparse && grid[0].count() > MAX_DENSE_CELLS)
+        FAIL(GVDBX_ERR_RANGE, "LoadVBX: %s: the leaves span %lld cells, beyond the dense-grid limit (topology mode DENSE)",
+             path, (long long)grid[0].count());
     for (int l = 0; l < levels; l++) {
         TRY(ensure(v, v->nodes[l], (size_t)std::max<int64_t>(cnt[l], 1) * sizeof(GNode)));
         TRY(host_to_dev(v, v->nodes[l].p, hn[l].data(), (size_t)cnt[l] * sizeof(GNode)));
@@ -2732,7 +2735,8 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
     }
     bool inactive = false;
     for (const GNode& n : hn[0]) inactive = inactive || !n.mFlags;
-    for (int l = 0; l < depth; l++) {
+    CellGrid gb_sparse = {};
+    for (int l = 0; l < depth && !sparse; l++) {
         std::vector<int32_t> idx((size_t)grid[l].count(), -1);
         for (int32_t k : reach[l]) {
             const GNode& n = hn[l][k];
@@ -2742,6 +2746,55 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
         TRY(ensure(v, v->idx[l], std::max<size_t>(idx.size(), 1) * 4));
         TRY(host_to_dev(v, v->idx[l].p, idx.data(), idx.size() * 4));
     }
+    if (sparse) {
+        // per level the node cells in key order (a later node of one cell
+        // replaces an earlier one, as the dense map does); the bin grid
+        // lists the leaf cells dilated by one
+        std::vector<uint64_t> leaf_keys;
+        for (int l = 0; l < depth; l++) {
+            CellGrid& g = grid[l];
+            g.by = gvdbx_bits_for(g.ny);
+            g.bz = gvdbx_bits_for(g.nz);
+            std::vector<std::pair<uint64_t, int32_t>> kv;
+            for (int32_t k : reach[l]) {
+                const GNode& n = hn[l][k];
+                kv.push_back({g.pack((int)(n.mPos[0] / v->range[l]), (int)(n.mPos[1] / v->range[l]),
+                                     (int)(n.mPos[2] / v->range[l])), k});
+            }
+            std::stable_sort(kv.begin(), kv.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            std::vector<uint64_t> keys;
+            std::vector<int32_t> idx;
+            for (size_t i = 0; i < kv.size(); i++) {
+                if (i + 1 < kv.size() && kv[i + 1].first == kv[i].first) continue;
+                keys.push_back(kv[i].first);
+                idx.push_back(kv[i].second);
+            }
+            TRY(ensure(v, v->sg_keys[l], std::max<size_t>(keys.size(), 1) * 8));
+            TRY(host_to_dev(v, v->sg_keys[l].p, keys.data(), keys.size() * 8));
+            TRY(ensure(v, v->idx[l], std::max<size_t>(idx.size(), 1) * 4));
+            TRY(host_to_dev(v, v->idx[l].p, idx.data(), idx.size() * 4));
+            TRY(sg_table(v, v->sg_keys[l], (int64_t)keys.size(), v->sg_ht[l], g));
+            if (l == 0) leaf_keys = keys;
+        }
+        const CellGrid& g0 = grid[0];
+        CellGrid gb{g0.ox - 1, g0.oy - 1, g0.oz - 1, g0.nx + 2, g0.ny + 2, g0.nz + 2};
+        gb.by = gvdbx_bits_for(gb.ny);
+        gb.bz = gvdbx_bits_for(gb.nz);
+        std::vector<uint64_t> bk;
+        bk.reserve(leaf_keys.size() * 27);
+        for (uint64_t k : leaf_keys) {
+            int x, y, z;
+            g0.unpack(k, x, y, z);
+            for (int d = 0; d < 27; d++) bk.push_back(gb.pack(x + d / 9 - 1, y + (d / 3) % 3 - 1, z + d % 3 - 1));
+        }
+        std::sort(bk.begin(), bk.end());
+        bk.erase(std::unique(bk.begin(), bk.end()), bk.end());
+        TRY(ensure(v, v->sg_bkeys, std::max<size_t>(bk.size(), 1) * 8));
+        TRY(host_to_dev(v, v->sg_bkeys.p, bk.data(), bk.size() * 8));
+        TRY(sg_table(v, v->sg_bkeys, (int64_t)bk.size(), v->sg_bht, gb));
+        gb_sparse = gb;
+    }
+    v->gb_sparse = gb_sparse;
     v->depth = depth;
     v->top_lev = tlev;
     for (int a = 0; a < 3; a++) v->root_pos[a] = hn[depth][0].mPos[a];
@@ -2754,7 +2807,7 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
     v->has_atlas = false;
     v->topo_gen++;
     v->vis_valid = false;
-    v->topo_mode_used = GVDBX_TOPO_DENSE;
+    v->topo_mode_used = sparse ? GVDBX_TOPO_RADIX : GVDBX_TOPO_DENSE;
     v->leaf_inactive = inactive;
     v->force_full = false;
     v->ncloud = 0;
