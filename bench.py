@@ -480,6 +480,8 @@ def main():
     ap.add_argument("--gather-variant", type=int, default=None,
                     help="0: LDS-staged candidates, 1: ballot/readlane walk (default), 2: point-centric LDS minima")
     ap.add_argument("--xcd-remap", type=int, default=None, help="1: contiguous brick runs per XCD (default 0)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a kernel tuning knob (gvdbx_set_tuning) before the run; repeatable")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     ap.add_argument("--calibrate-valu", action="store_true",
@@ -526,6 +528,9 @@ def main():
         gv.SetTuning("gather_variant", args.gather_variant)
     if args.xcd_remap is not None:
         gv.SetTuning("xcd_remap", args.xcd_remap)
+    for kv in args.tune:
+        k, _, val = kv.partition("=")
+        gv.SetTuning(k, int(val))
     torch.cuda.synchronize()
 
     ev = {}
@@ -623,7 +628,7 @@ def main():
         "stage_traffic": stage_counters("config2_frame_stage_counters.md") if n == 1 << 24 else None,
         "stage_traffic_source": "profiles/config2_frame_stage_counters.md" if n == 1 << 24 else None,
         "autotune": tune,
-        "tuning": {k: gv.GetTuning(k) for k in ("gather_variant", "gather_classes", "gather_split", "xcd_remap",
+        "tuning": {k: gv.GetTuning(k) for k in ("gather_variant", "gather_classes", "gather_split", "xcd_remap", "apron_order",
                                                 "bin_records", "sort_tiles")},
         "fwd_bwd_over_fwd": None,
     }

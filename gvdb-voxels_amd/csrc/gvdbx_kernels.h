@@ -257,6 +257,7 @@ struct ApronParams {
     int brick_res;
     float boundval;
     uint32_t bound_bits;      // the word stored outside every leaf (float bits or packed uchar4)
+    int order;                // 8^3 bricks, dense g0: 1 = bricks taken in cell order with x fastest, 0 = leaf order
 };
 void launch_apron(ApronParams ap, hipStream_t s);
 

@@ -1630,11 +1630,28 @@ __constant__ Apron10Tab APRON10 = make_apron10();
 // loaded before any is stored, so 8 reads per lane are in flight at once.
 // Each XCD takes a contiguous run of bricks, so the neighbour faces a brick
 // reads were mostly brought into the same L2 by its neighbours.
+//
+// The x faces are the costly reads: one word in each 40-byte row of the
+// neighbour, so a face touches nearly every line of that brick, while a z
+// face is one contiguous 320-byte run.  In leaf (key) order the x neighbours
+// are a whole y-z slab of cells apart, more bricks than an XCD's L2 holds;
+// with ORDER the waves walk the dense level-0 grid with x fastest (empty
+// cells exit at once), so a brick's x neighbours were read by the waves just
+// before it and its far neighbours are the z ones, whose faces are compact.
+template <bool ORDER>
 __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
     const int lane = threadIdx.x & 63;
     const int64_t blk = gvdbx_xcd_item(blockIdx.x, nblk);
     if (blk < 0) return;
-    const int64_t k = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int64_t k = blk * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if constexpr (ORDER) {
+        const CellGrid& g = ap.g0;
+        if (k >= g.count()) return;
+        const int64_t xy = (int64_t)g.nx * g.ny;
+        const int z = (int)(k / xy), y = (int)((k / g.nx) % g.ny), x = (int)(k % g.nx);
+        k = ap.idx0[((int64_t)x * g.ny + y) * g.nz + z];
+        if (k < 0) return;
+    }
     if (k >= ap.nleaf) return;
     int iv[8];
 #pragma unroll
@@ -1998,8 +2015,11 @@ void launch_splat_src(const float4* sorted, int64_t n, const int32_t* count, con
 
 void launch_apron(ApronParams ap, hipStream_t s) {
     if (ap.nleaf <= 0) return;
-    if (ap.brick_res == 10 && ap.range0 == 8)
-        hipLaunchKernelGGL(k_apron10, dim3((unsigned)gvdbx_xcd_blocks((ap.nleaf + 3) / 4)), dim3(256), 0, s, ap,
+    if (ap.brick_res == 10 && ap.range0 == 8 && ap.order && !ap.g0.sparse()) {
+        const int64_t nb = (ap.g0.count() + 3) / 4;
+        hipLaunchKernelGGL(k_apron10<true>, dim3((unsigned)gvdbx_xcd_blocks(nb)), dim3(256), 0, s, ap, nb);
+    } else if (ap.brick_res == 10 && ap.range0 == 8)
+        hipLaunchKernelGGL(k_apron10<false>, dim3((unsigned)gvdbx_xcd_blocks((ap.nleaf + 3) / 4)), dim3(256), 0, s, ap,
                            (ap.nleaf + 3) / 4);
     else if (ap.brick_res == 10)
         hipLaunchKernelGGL(k_apron<10>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);

@@ -158,6 +158,7 @@ struct gvdbx_volume {
     int splat_variant = 1;
     int sort_tiles = 0;   // point binning sort: 0 auto (wide from 2^23 valid points), 1 narrow, 2 wide
     int bin_records = 1;  // 1: binning sort moves point records; 0: sorts (cell, index) pairs, then gathers
+    int apron_order = 1;  // UpdateApron: 1 = bricks in cell order with x fastest (dense grids), 0 = leaf order
     // atlas map
     bool has_atlas = false;
     DevBuf amap;
@@ -1771,6 +1772,11 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->bins_valid = false;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "apron_order")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "apron_order must be 0 or 1");
+        v->apron_order = value;
+        return GVDBX_OK;
+    }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1807,6 +1813,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "bin_records")) {
         *value = v->bin_records;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "apron_order")) {
+        *value = v->apron_order;
         return GVDBX_OK;
     }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
@@ -2157,7 +2167,7 @@ int update_apron(gvdbx_volume* v, int chan, float boundval) {
         memcpy(&bits, &boundval, 4);
     }
     ApronParams ap{v->nodes[0].as<GNode>(), v->cnt[0], v->grid[0], v->idx[0].as<int32_t>(), v->res[0],
-                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval, bits};
+                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval, bits, v->apron_order};
     launch_apron(ap, v->stream);
     return check_launch("UpdateApron");
 }

@@ -523,3 +523,25 @@ def test_fp16_subcell_parity(log2dims, sc, R, split):
     with pytest.raises(G.GvdbxError) as e:
         gv.GatherLevelSet_FP16(sc, len(pts), 1.5, (0, 0, 0), gl, 0, 1)
     assert e.value.code == 1
+
+
+def test_apron_orders_bit_identical():
+    """UpdateApron walking bricks in leaf order and in cell order (x
+    fastest) writes the same words, for both boundary values."""
+    _log("[gpu] apron orders")
+    rng = np.random.default_rng(31)
+    pts = (rng.random((30000, 3)) * [90, 70, 80] + [20, 12, 30]).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 3.0)
+    gl = gv.InsertPointsSubcell(4, len(pts), 3.0)
+    gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), gl, 0)
+    for bv in (3.0, 0.0):
+        out = []
+        for order in (1, 0):
+            gv.SetTuning("apron_order", order)
+            gv.UpdateApron(0, bv)
+            out.append(gv.bricks())
+        np.testing.assert_array_equal(out[0], out[1])
+    ov.insert_points_subcell(4, pts, 3.0)
+    ov.gather_level_set(1.5)
+    ov.update_apron(0, 0.0)
+    np.testing.assert_array_equal(out[0], ov.bricks())
