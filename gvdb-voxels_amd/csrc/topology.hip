@@ -346,27 +346,21 @@ __global__ __launch_bounds__(256) void k_expand_reach(CellGrid g0, const uint32_
     }
 }
 
+// Level-l flags from level l-1: every flagged child cell marks its parent
+// (the parents' flags are cleared first; concurrent stores write the same 1).
+// One thread per child cell, so no thread loops over a parent's children.
 __global__ __launch_bounds__(256) void k_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c,
                                                   CellGrid gp, int logres, uint8_t* base_p, uint8_t* all_p) {
-    int64_t n = gp.count();
-    int res = 1 << logres;
+    const int64_t n = gc.count();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int px, py, pz;
-        gp.cell(i, px, py, pz);
-        int x0 = max(px * res, gc.ox), x1 = min(px * res + res, gc.ox + gc.nx);
-        int y0 = max(py * res, gc.oy), y1 = min(py * res + res, gc.oy + gc.ny);
-        int z0 = max(pz * res, gc.oz), z1 = min(pz * res + res, gc.oz + gc.nz);
-        uint8_t b = 0, a = 0;
-        for (int x = x0; x < x1 && !(b && a); x++)
-            for (int y = y0; y < y1; y++)
-                for (int z = z0; z < z1; z++) {
-                    const int64_t ci = gc.find(x, y, z);
-                    if (ci < 0) continue;   // a sparse grid lists only some cells
-                    b |= base_c[ci];
-                    a |= all_c[ci];
-                }
-        base_p[i] = b;
-        all_p[i] = a | b;
+        const uint8_t b = base_c[i], a = all_c[i];
+        if (!(a | b)) continue;
+        int x, y, z;
+        gc.cell(i, x, y, z);
+        const int64_t p = gp.find(x >> logres, y >> logres, z >> logres);
+        if (p < 0) continue;
+        if (b) base_p[p] = 1;
+        all_p[p] = 1;
     }
 }
 
@@ -920,7 +914,9 @@ void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, ui
 
 void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
                      uint8_t* base_p, uint8_t* all_p, hipStream_t s) {
-    hipLaunchKernelGGL(k_level_up, dim3(grid_for(gp.count(), 256)), dim3(256), 0, s, gc, base_c, all_c, gp, logres_p,
+    launch_fill_u8(base_p, gp.count(), 0, s);
+    launch_fill_u8(all_p, gp.count(), 0, s);
+    hipLaunchKernelGGL(k_level_up, dim3(grid_for(gc.count(), 256)), dim3(256), 0, s, gc, base_c, all_c, gp, logres_p,
                        base_p, all_p);
 }
 
