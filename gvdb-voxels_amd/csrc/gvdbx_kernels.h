@@ -32,7 +32,8 @@ void launch_mark_boxes(PointSrc src, int64_t n, float tx, float ty, float tz, Ce
 // allowed (optional): per g0 cell, >= 0 where extra bricks may be created
 void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, uint8_t* all0, int generic,
                          const int32_t* allowed, hipStream_t s);
-// level l flags from level l-1 flags (children gather)
+// level l flags from level l-1 flags (every flagged child marks its parent;
+// level l's flags must be clear)
 void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
                      uint8_t* base_p, uint8_t* all_p, hipStream_t s);
 
@@ -66,7 +67,13 @@ struct FlagApplyBatch {
 };
 void launch_flag_scan_apply(FlagApplyBatch& fb, hipStream_t s);
 // a[0..na) = 0 and b[0..nb) = 0 in one launch
-void launch_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb, hipStream_t s);
+// clears up to 2 * GVDBX_MAXLEV + 2 buffers (sizes in bytes) in one launch
+struct ZeroBatch {
+    uint8_t* p[2 * GVDBX_MAXLEV + 2];
+    int64_t bytes[2 * GVDBX_MAXLEV + 2];
+    int n;
+};
+void launch_zero_batch(const ZeroBatch& zb, hipStream_t s);
 void launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
 // out[0] = sum of p[i], out[1] = sum of p[i] * (i mod 65521 + 1) (mod 2^64)
 void launch_checksum(const uint32_t* p, int64_t n, unsigned long long* out, hipStream_t s);
@@ -180,7 +187,11 @@ struct GatherParams {
     const int32_t* coff;      // variant 1, optional: class starts of launch_gather_classes (sorted is then
                               // its reordered copy, packed ranges in .w; box unused)
     const uint8_t* cboth;     // with coff: per bin, axes along which a point reaches past both faces
+    int push;                 // 8^3 bricks, apron 1: also write boundary voxels into the neighbours' aprons
+    const int32_t* bin_leaf;  // push: per gb cell, its leaf or -1 (launch_bin_leaf)
 };
+// per gb cell: the leaf at that cell (idx0 over g0, leaves < nleaf) or -1
+void launch_bin_leaf(CellGrid gb, CellGrid g0, const int32_t* idx0, int64_t nleaf, int32_t* out, hipStream_t s);
 // Per bin entry (the first *total), the membership range clipped to the
 // cutoff along each axis, as offsets from floor(p): bits 10a..10a+4 hold
 // floor(p) + 1 - lo, bits 10a+5..10a+9 hold hi - floor(p); bit 31: empty on
@@ -258,6 +269,8 @@ struct ApronParams {
     float boundval;
     uint32_t bound_bits;      // the word stored outside every leaf (float bits or packed uchar4)
     int order;                // 8^3 bricks, dense g0: 1 = bricks taken in cell order with x fastest, 0 = leaf order
+    int pushed;               // 8^3 bricks: the apron voxels that have a neighbour leaf are current (pushing
+                              // gather); only the boundary word is written, where no leaf is
 };
 void launch_apron(ApronParams ap, hipStream_t s);
 

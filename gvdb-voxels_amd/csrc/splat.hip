@@ -999,20 +999,63 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
 
 // Brick values of a tile from its minima: sqrt is correctly rounded, so the
 // stored value is a function of min |p-v|^2 alone.
-__device__ __forceinline__ void gather_store(const GatherParams& gp, const GTile& g, int td, const uint32_t (&m)[8]) {
+//
+// gp.push (8^3 bricks, apron 1, one tile per brick): the tile's boundary
+// voxels are also written into the aprons of the neighbour leaves that hold
+// them as apron voxels (faces, edges and corners), the words UpdateApron
+// would copy there (neighbours found as k_apron10 finds them).  Every apron
+// voxel that has a neighbour leaf then holds that leaf's new value, and
+// UpdateApron only has to write the boundary value where no leaf is.  (The
+// x faces cost the most: one word per 40-byte row of the neighbour; copying
+// them in UpdateApron instead measured no faster.)
+//
+// push_leaves: lane d < 27 gets the leaf at direction d = (dz+1)*9 + (dy+1)*3
+// + (dx+1) from the tile's cell (-1 where none); taken before the walk, so
+// only this one register stays live across it.
+__device__ __forceinline__ int32_t push_leaves(const GatherParams& gp, const GTile& g) {
+    const int lane = threadIdx.x;
+    int32_t nl = -1;
+    if (gp.push && lane < 27) {
+        const int64_t ci = gp.gb.find(g.cx + lane % 3 - 1, g.cy + (lane / 3) % 3 - 1, g.cz + lane / 9 - 1);
+        if (ci >= 0) nl = gp.bin_leaf[ci];
+    }
+    return nl;
+}
+
+__device__ __forceinline__ void gather_store(const GatherParams& gp, const GTile& g, int td, const uint32_t (&m)[8],
+                                             int32_t nl) {
     const int lane = threadIdx.x;
     const int lx = lane % td, ly = lane / td;
     if (ly >= td) return;
     const int br = gp.brick_res, a = gp.apron;
     float* brick = gp.chan + g.k * br * br * br;
     const float r = gp.radius;
+    float val[8];
 #pragma unroll
     for (int z = 0; z < 8; z++) {
         if (z < td) {
             int64_t vi = ((int64_t)(g.toz + z + a) * br + (g.toy + ly + a)) * br + (g.tox + lx + a);
             float init = gp.accum ? brick[vi] : (gp.mode == 1 ? 3.0f : 0.0f);
             float c = sqrtf(__uint_as_float(m[z]));
-            brick[vi] = gp.mode == 1 ? fminf(init, c - r) : fmaxf(init, r - c);
+            val[z] = gp.mode == 1 ? fminf(init, c - r) : fmaxf(init, r - c);
+            brick[vi] = val[z];
+        }
+    }
+    if (!gp.push) return;
+#pragma unroll 1
+    for (int d = 0; d < 27; d++) {
+        if (d == 13) continue;
+        const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+        const int32_t n = __shfl(nl, d);   // every lane takes part (td == 8)
+        const bool col = (dx == 0 || lx == (dx < 0 ? 0 : 7)) && (dy == 0 || ly == (dy < 0 ? 0 : 7));
+        if (n < 0 || !col) continue;
+        float* nb = gp.chan + (int64_t)n * 1000;
+        const int px = dx < 0 ? 9 : (dx > 0 ? 0 : lx + 1), py = dy < 0 ? 9 : (dy > 0 ? 0 : ly + 1);
+        if (dz == 0) {
+#pragma unroll
+            for (int z = 0; z < 8; z++) nb[((z + 1) * 10 + py) * 10 + px] = val[z];
+        } else {
+            nb[((dz < 0 ? 9 : 0) * 10 + py) * 10 + px] = val[dz < 0 ? 0 : 7];
         }
     }
 }
@@ -1042,6 +1085,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
     const int lane = threadIdx.x;
     const int td = tile_dim(res0);
     const GTile g = gather_tile(gp, res0, tiles_per_brick, tile);
+    const int32_t nl = push_leaves(gp, g);
     int ns = 1, slot = -1;
     auto plan = [&](int, int total) {
         if (!SPLIT) return 1;
@@ -1081,7 +1125,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
     for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
     gather_walk<HALVES, CLS>(gp, g, td, 0, 1, plan, m);
     if (!SPLIT || ns == 1) {
-        gather_store(gp, g, td, m);
+        gather_store(gp, g, td, m, nl);
     } else if (slot >= 0) {
         const bool lane_ok = lane / td < td;
         uint32_t* at = gs.acc + (int64_t)slot * 512 + lane;
@@ -1127,7 +1171,7 @@ __global__ __launch_bounds__(64) void k_gather_finish(GatherParams gp, int res0,
         uint32_t m[8];
 #pragma unroll
         for (int z = 0; z < 8; z++) m[z] = acc[(int64_t)sl * 512 + z * 64 + lane];
-        gather_store(gp, g, td, m);
+        gather_store(gp, g, td, m, push_leaves(gp, g));
     }
 }
 
@@ -1638,7 +1682,11 @@ __constant__ Apron10Tab APRON10 = make_apron10();
 // with ORDER the waves walk the dense level-0 grid with x fastest (empty
 // cells exit at once), so a brick's x neighbours were read by the waves just
 // before it and its far neighbours are the z ones, whose faces are compact.
-template <bool ORDER>
+//
+// PUSHED: a pushing gather (gather_store) has already written every apron
+// voxel that has a neighbour leaf; only the boundary word is written, where
+// no leaf is (most bricks write nothing).
+template <bool ORDER, bool PUSHED>
 __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
     const int lane = threadIdx.x & 63;
     const int64_t blk = gvdbx_xcd_item(blockIdx.x, nblk);
@@ -1665,6 +1713,7 @@ __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
         if (ncell >= 0) leaf = ap.idx0[ncell];
         if (leaf >= ap.nleaf) leaf = -1;
     }
+    if (PUSHED && !__ballot(lane < 27 && lane != 13 && leaf < 0)) return;   // every neighbour pushed
     // voxels are moved as 4-byte words: float or packed uchar4 channels alike
     const uint32_t* chan = (const uint32_t*)ap.chan;
     uint32_t v[8];
@@ -1676,6 +1725,7 @@ __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
         const int dy = y == 0 ? -1 : (y == 9 ? 1 : 0);
         const int dz = z == 0 ? -1 : (z == 9 ? 1 : 0);
         const int nl = __shfl(leaf, (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));   // every lane takes part
+        if (PUSHED && nl >= 0) iv[u] = -1;   // written by the gather
         // interior voxel of the neighbour holding world voxel mPos + (x,y,z) - 1
         v[u] = ap.bound_bits;
         if (iv[u] >= 0 && nl >= 0) v[u] = chan[(int64_t)nl * 1000 + i - 8 * (dz * 100 + dy * 10 + dx)];
@@ -1684,6 +1734,20 @@ __global__ __launch_bounds__(256) void k_apron10(ApronParams ap, int64_t nblk) {
 #pragma unroll
     for (int u = 0; u < 8; u++)
         if (iv[u] >= 0) brick[iv[u]] = v[u];
+}
+
+// Per bin cell: the leaf whose cell it is (as k_apron10 finds neighbours:
+// idx0 over the level-0 grid, leaves < nleaf), or -1.
+__global__ __launch_bounds__(256) void k_bin_leaf(CellGrid gb, CellGrid g0, const int32_t* idx0, int64_t nleaf,
+                                                  int32_t* out) {
+    const int64_t n = gb.count();
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        int x, y, z;
+        gb.cell(c, x, y, z);
+        const int64_t f = (x >= 0 && y >= 0 && z >= 0) ? g0.find(x, y, z) : -1;
+        const int32_t k = f >= 0 ? idx0[f] : -1;
+        out[c] = k < nleaf ? k : -1;
+    }
 }
 
 // BR > 0: padded brick size known at compile time; BR == 0: runtime size.
@@ -2015,16 +2079,24 @@ void launch_splat_src(const float4* sorted, int64_t n, const int32_t* count, con
 
 void launch_apron(ApronParams ap, hipStream_t s) {
     if (ap.nleaf <= 0) return;
-    if (ap.brick_res == 10 && ap.range0 == 8 && ap.order && !ap.g0.sparse()) {
-        const int64_t nb = (ap.g0.count() + 3) / 4;
-        hipLaunchKernelGGL(k_apron10<true>, dim3((unsigned)gvdbx_xcd_blocks(nb)), dim3(256), 0, s, ap, nb);
-    } else if (ap.brick_res == 10 && ap.range0 == 8)
-        hipLaunchKernelGGL(k_apron10<false>, dim3((unsigned)gvdbx_xcd_blocks((ap.nleaf + 3) / 4)), dim3(256), 0, s, ap,
-                           (ap.nleaf + 3) / 4);
+    const bool ord = ap.order && !ap.g0.sparse();
+    const int64_t nb = ord ? (ap.g0.count() + 3) / 4 : (ap.nleaf + 3) / 4;
+    const dim3 g10((unsigned)gvdbx_xcd_blocks(nb));
+    if (ap.brick_res == 10 && ap.range0 == 8) {
+        if (ord && ap.pushed) hipLaunchKernelGGL((k_apron10<true, true>), g10, dim3(256), 0, s, ap, nb);
+        else if (ord) hipLaunchKernelGGL((k_apron10<true, false>), g10, dim3(256), 0, s, ap, nb);
+        else if (ap.pushed) hipLaunchKernelGGL((k_apron10<false, true>), g10, dim3(256), 0, s, ap, nb);
+        else hipLaunchKernelGGL((k_apron10<false, false>), g10, dim3(256), 0, s, ap, nb);
+    }
     else if (ap.brick_res == 10)
         hipLaunchKernelGGL(k_apron<10>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
     else
         hipLaunchKernelGGL(k_apron<0>, dim3((unsigned)ap.nleaf), dim3(256), 0, s, ap);
+}
+
+void launch_bin_leaf(CellGrid gb, CellGrid g0, const int32_t* idx0, int64_t nleaf, int32_t* out, hipStream_t s) {
+    if (gb.count() > 0)
+        hipLaunchKernelGGL(k_bin_leaf, dim3(grid_for(gb.count(), 256)), dim3(256), 0, s, gb, g0, idx0, nleaf, out);
 }
 
 void launch_insert_points(InsertParams ip, hipStream_t s) {

@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_expand_reach(CellGrid g0, const uint32_
 }
 
 // Level-l flags from level l-1: every flagged child cell marks its parent
-// (the parents' flags are cleared first; concurrent stores write the same 1).
+// (the parents' flags start cleared; concurrent stores write the same 1).
 // One thread per child cell, so no thread loops over a parent's children.
 __global__ __launch_bounds__(256) void k_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c,
                                                   CellGrid gp, int logres, uint8_t* base_p, uint8_t* all_p) {
@@ -914,8 +914,6 @@ void launch_expand_reach(CellGrid g0, const uint32_t* reach0, uint8_t* base0, ui
 
 void launch_level_up(CellGrid gc, const uint8_t* base_c, const uint8_t* all_c, CellGrid gp, int logres_p,
                      uint8_t* base_p, uint8_t* all_p, hipStream_t s) {
-    launch_fill_u8(base_p, gp.count(), 0, s);
-    launch_fill_u8(all_p, gp.count(), 0, s);
     hipLaunchKernelGGL(k_level_up, dim3(grid_for(gc.count(), 256)), dim3(256), 0, s, gc, base_c, all_c, gp, logres_p,
                        base_p, all_p);
 }
@@ -935,16 +933,21 @@ void launch_flag_scan_apply(FlagApplyBatch& fb, hipStream_t s) {
         hipLaunchKernelGGL(k_flag_apply, dim3((unsigned)fb.tile_start[fb.n]), dim3(SCAN_THREADS), 0, s, fb);
 }
 
-__global__ __launch_bounds__(256) void k_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < max(na, nb); i += (int64_t)gridDim.x * blockDim.x) {
-        if (i < na) a[i] = 0;
-        if (i < nb) b[i] = 0;
+// Clears every buffer of the batch (byte counts; 4-byte words where aligned).
+__global__ __launch_bounds__(256) void k_zero_batch(ZeroBatch zb) {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (int64_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < zb.n; k++) {
+        uint8_t* p = zb.p[k];
+        const int64_t nb = zb.bytes[k], nw = ((uintptr_t)p & 3) ? 0 : nb >> 2;
+        for (int64_t i = t0; i < nw; i += st) ((uint32_t*)p)[i] = 0u;
+        for (int64_t i = nw * 4 + t0; i < nb; i += st) p[i] = 0;
     }
 }
 
-void launch_zero_i32_u8(int32_t* a, int64_t na, uint8_t* b, int64_t nb, hipStream_t s) {
-    const int64_t n = std::max(na, nb);
-    if (n > 0) hipLaunchKernelGGL(k_zero_i32_u8, dim3(grid_for(n, 256)), dim3(256), 0, s, a, na, b, nb);
+void launch_zero_batch(const ZeroBatch& zb, hipStream_t s) {
+    int64_t mx = 0;
+    for (int k = 0; k < zb.n; k++) mx = std::max(mx, zb.bytes[k] / 4 + 1);
+    if (zb.n > 0) hipLaunchKernelGGL(k_zero_batch, dim3(grid_for(mx, 256)), dim3(256), 0, s, zb);
 }
 
 void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials, uint32_t* total, hipStream_t s) {

@@ -159,6 +159,13 @@ struct gvdbx_volume {
     int sort_tiles = 0;   // point binning sort: 0 auto (wide from 2^23 valid points), 1 narrow, 2 wide
     int bin_records = 1;  // 1: binning sort moves point records; 0: sorts (cell, index) pairs, then gathers
     int apron_order = 1;  // UpdateApron: 1 = bricks in cell order with x fastest (dense grids), 0 = leaf order
+    int apron_push = 1;   // level-set / density gathers on 8^3 bricks also fill the neighbours' aprons
+    // per channel: the apron voxels that have a neighbour leaf hold its current
+    // values (a pushing gather was the last write, on topology apron_push_gen);
+    // chan_raw: the caller holds the device pointer and may write any voxel
+    bool apron_pushed[GVDBX_MAXCHAN] = {};
+    uint64_t apron_push_gen[GVDBX_MAXCHAN] = {};
+    bool chan_raw[GVDBX_MAXCHAN] = {};
     // atlas map
     bool has_atlas = false;
     DevBuf amap;
@@ -170,6 +177,7 @@ struct gvdbx_volume {
     DevBuf gbox;                                    // Gather*: packed ranges per bin entry
     DevBuf gcls_pts, gcoff, gcboth;                 // Gather*: bin entries by reach class, class starts, both-face axes
     DevBuf gcctr, gcunit, gcbig, gcucnt;            // Gather*: units of large bins (counters, units, bins, counts)
+    DevBuf bin_leaf;                                // Gather* with apron push: leaf of each bin cell
     DevBuf sp_runs, sp_runoff, sp_runcell, sp_part, sp_scan;   // SplatCubic cell push: runs and their grids
     CellGrid gbin = {};
     // sparse topology: per level the sorted cell keys and their hash table,
@@ -333,6 +341,11 @@ void free_buf(DevBuf& b) {
     b.cap = 0;
 }
 
+// Every write of a channel other than a pushing gather leaves its aprons stale.
+void chan_written(gvdbx_volume* v, int chan) {
+    if (chan >= 0 && chan < GVDBX_MAXCHAN) v->apron_pushed[chan] = false;
+}
+
 int brick_res(const gvdbx_volume* v) { return v->res[0] + 2 * std::max(v->apron, 0); }
 int64_t brick_vox(const gvdbx_volume* v) {
     int64_t br = brick_res(v);
@@ -365,6 +378,7 @@ int grow_atlas(gvdbx_volume* v, int64_t bricks) {
     v->buf_gen++;
     for (int c = 0; c < GVDBX_MAXCHAN; c++) {
         if (!v->has_chan[c]) continue;
+        chan_written(v, c);
         void* np = nullptr;
         size_t bytes = (size_t)(bricks * bv) * sizeof(float);
         hipError_t e = hipMalloc(&np, bytes);
@@ -901,7 +915,17 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     }
     TRY(ensure(v, v->partials, (size_t)(ptot + 2) * sizeof(U2)));
     TRY(ensure(v, v->totals, (size_t)GVDBX_MAXLEV * sizeof(U2)));
-    launch_zero_i32_u8((int32_t*)v->reach0.p, G0, v->flag_all[0].as<uint8_t>(), G0, s);
+    {
+        // reach and level-0 extra flags; the flags of the levels above (level-up only sets them)
+        ZeroBatch zb{};
+        zb.p[zb.n] = v->reach0.as<uint8_t>(), zb.bytes[zb.n++] = G0 * 4;
+        zb.p[zb.n] = v->flag_all[0].as<uint8_t>(), zb.bytes[zb.n++] = G0;
+        for (int l = 1; l < depth; l++) {
+            zb.p[zb.n] = v->flag_base[l].as<uint8_t>(), zb.bytes[zb.n++] = grid[l].count();
+            zb.p[zb.n] = v->flag_all[l].as<uint8_t>(), zb.bytes[zb.n++] = grid[l].count();
+        }
+        launch_zero_batch(zb, s);
+    }
     // bin the points by level-0 cell (kept for InsertPointsSubcell / InsertPoints),
     // then derive occupancy and extra-brick reach from the cell-sorted points
     v->grid[0] = grid[0];
@@ -1362,6 +1386,7 @@ int gather_color(gvdbx_volume* v, int chan_clr, int accum) {
     cp.sc_shift = __builtin_ctz((unsigned)v->sc_size);
     cp.R = v->sc_radius;
     cp.accum = accum ? 1 : 0;
+    chan_written(v, chan_clr);
     launch_gather_color(cp, v->stream);
     return check_launch("gather colour");
 }
@@ -1412,6 +1437,16 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
     gp.variant = (noalloc || fp16) ? 1 : v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
+    // 8^3 bricks with apron 1: the walk's store also fills the neighbours' aprons
+    gp.push = (v->apron_push && gp.variant == 1 && gp.brick_res == 10 && gp.apron == 1 && v->res[0] == 8 &&
+               !v->chan_raw[chan]) ? 1 : 0;
+    if (gp.push) {
+        TRY(ensure(v, v->bin_leaf, (size_t)std::max<int64_t>(gp.gb.count(), 1) * 4));
+        launch_bin_leaf(gp.gb, v->grid[0], v->idx[0].as<int32_t>(), v->cnt[0], v->bin_leaf.as<int32_t>(), v->stream);
+        gp.bin_leaf = v->bin_leaf.as<int32_t>();
+    }
+    v->apron_pushed[chan] = gp.push != 0;
+    v->apron_push_gen[chan] = v->topo_gen;
     if (fp16 && (int)std::ceil(gp.box_radius) + sc > 28)
         FAIL(GVDBX_ERR_RANGE, "%s: subcell radius + subcell size beyond 28 voxels", what);
     if (gp.variant == 1 && (int)std::ceil(gp.box_radius) + sc <= 29 && v->bins_n > 0) {
@@ -1609,7 +1644,7 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->rs_off, &v->rs_part, &v->pnode, &v->poff,
                       &v->vis_key, &v->vis_rec, &v->vis_idx, &v->vis_tkey, &v->vis_tidx,
                       &v->vis_hist, &v->vis_off, &v->vis_part, &v->vis_seg, &v->vis_cnt,
-                      &v->splat_src, &v->gbox, &v->gcls_pts, &v->gcoff, &v->gcboth, &v->gcctr, &v->gcunit, &v->gcbig, &v->gcucnt, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
+                      &v->splat_src, &v->gbox, &v->bin_leaf, &v->gcls_pts, &v->gcoff, &v->gcboth, &v->gcctr, &v->gcunit, &v->gcbig, &v->gcucnt, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
                       &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
                       &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
@@ -1693,6 +1728,8 @@ int gvdbx_add_channel(gvdbx_volume* v, int chan, int dtype, int apron, int filte
         FAIL(GVDBX_ERR_NOMEM, "AddChannel: hipMalloc(%zu) failed", bytes);
     }
     v->chan[chan].cap = bytes;
+    chan_written(v, chan);
+    v->chan_raw[chan] = false;
     launch_fill_f32(v->chan[chan].as<float>(), cap * brick_vox(v), 0.0f, v->stream);
     v->has_chan[chan] = true;
     v->chan_type[chan] = dtype;
@@ -1713,9 +1750,11 @@ int gvdbx_fill_channel(gvdbx_volume* v, int chan, float value) {
             const uint32_t c = sat_u8(value * 255.0f);
             w = c | c << 8 | c << 16 | 255u << 24;
         }
+        chan_written(v, chan);
         launch_fill_i32(v->chan[chan].as<int32_t>(), v->atlas_cap * brick_vox(v), (int32_t)w, v->stream);
         return check_launch("FillChannel");
     }
+    chan_written(v, chan);
     launch_fill_f32(v->chan[chan].as<float>(), v->atlas_cap * brick_vox(v), value, v->stream);
     return check_launch("FillChannel");
 }
@@ -1777,6 +1816,11 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->apron_order = value;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "apron_push")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "apron_push must be 0 or 1");
+        v->apron_push = value;
+        return GVDBX_OK;
+    }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1817,6 +1861,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "apron_order")) {
         *value = v->apron_order;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "apron_push")) {
+        *value = v->apron_push;
         return GVDBX_OK;
     }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
@@ -2064,11 +2112,13 @@ int gvdbx_scatter_density(gvdbx_volume* v, int num_pnts, float radius, float amp
     ScatterParams sp{v->nodes[0].as<GNode>(), v->cnt[0], v->gbin, v->bin_off.as<int32_t>(), v->sorted.as<float4>(),
                      v->chan[0].as<float>(), brick_res(v), std::max(v->apron, 0), v->res[0], radius, expand ? 1 : 0};
     if (sp.apron == 0 && expand) FAIL(GVDBX_ERR_ARG, "ScatterDensity: expand needs an apron");
+    chan_written(v, 0);
     launch_scatter_density(sp, v->stream);
     if (v->clr) {
         ColorScatterParams cp{v->nodes[0].as<GNode>(), v->cnt[0], v->gbin, v->bin_off.as<int32_t>(),
                               v->sorted.as<float4>(), v->clr, v->clr_stride, v->chan[cchan].as<uint32_t>(),
                               brick_res(v), std::max(v->apron, 0), v->res[0], v->logdim[0], avg_color ? 1 : 0};
+        chan_written(v, cchan);
         launch_scatter_color(cp, v->stream);
     }
     return check_launch("ScatterDensity");
@@ -2123,6 +2173,7 @@ int gvdbx_splat_cubic(gvdbx_volume* v, int num_pnts, float radius, float tx, flo
     sp.weights = v->wts;
     sp.wstride = v->wstride;
     sp.chan = v->chan[chan].as<float>();
+    chan_written(v, chan);
     sp.brick_res = brick_res(v);
     sp.apron = std::max(v->apron, 0);
     sp.accum = accum ? 1 : 0;
@@ -2166,8 +2217,10 @@ int update_apron(gvdbx_volume* v, int chan, float boundval) {
     } else {
         memcpy(&bits, &boundval, 4);
     }
+    const bool pushed = v->apron_pushed[chan] && v->apron_push_gen[chan] == v->topo_gen && !v->chan_raw[chan];
     ApronParams ap{v->nodes[0].as<GNode>(), v->cnt[0], v->grid[0], v->idx[0].as<int32_t>(), v->res[0],
-                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval, bits, v->apron_order};
+                   v->logdim[0], v->chan[chan].as<float>(), brick_res(v), boundval, bits, v->apron_order,
+                   pushed ? 1 : 0};
     launch_apron(ap, v->stream);
     return check_launch("UpdateApron");
 }
@@ -2583,6 +2636,8 @@ int gvdbx_channel_device_ptr(gvdbx_volume* v, int chan, float** out_dev) {
     TRY(check_chan(v, chan));
     if (!out_dev) FAIL(GVDBX_ERR_ARG, "null output");
     *out_dev = v->chan[chan].as<float>();
+    v->chan_raw[chan] = true;   // the caller may write any voxel from now on
+    chan_written(v, chan);
     return GVDBX_OK;
 }
 
@@ -2883,6 +2938,7 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
                                (size_t)br * 4);
                 }
         }
+        chan_written(v, c);
         TRY(host_to_dev(v, v->chan[c].p, bricks.data(), bricks.size() * 4));
     }
     (void)ctype;

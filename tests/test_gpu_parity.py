@@ -545,3 +545,90 @@ def test_apron_orders_bit_identical():
     ov.gather_level_set(1.5)
     ov.update_apron(0, 0.0)
     np.testing.assert_array_equal(out[0], ov.bricks())
+
+
+@pytest.mark.parametrize("case", ["levelset", "density", "accum", "split", "fp16", "radix", "unclassed"])
+def test_apron_push_equals_full_apron(case):
+    """A gather on 8^3 bricks also writes its boundary voxels into the
+    neighbours' aprons, and UpdateApron then only writes the boundary value
+    where no neighbour leaf is: after UpdateApron the bricks equal those of
+    the full apron copy (apron_push 0) bit for bit, for every gather form,
+    and the oracle's."""
+    import gvdbx as G
+    _log(f"[gpu] apron push {case}")
+    rng = np.random.default_rng(41)
+    pts = np.abs(np.concatenate([rng.normal(c, 7.0, (9000, 3)) for c in ([40, 40, 40], [90, 60, 50], [60, 110, 80])])
+                 ).astype(np.float32)
+    out = []
+    for push in (1, 0):
+        gv = G.VolumeGVDB(0)
+        gv.configure((3, 3, 3, 3, 5))
+        gv.AddChannel(0, G.T_FLOAT, 1)
+        if case == "radix":
+            gv.SetTopologyMode(G.TOPO_RADIX)
+        tp = torch.from_numpy(pts).to(_dev())
+        gv.SetPoints(tp)
+        gv.RebuildTopology(len(pts), 3.0)
+        gv.FinishTopology(False, True)
+        gv.UpdateAtlas()
+        gv.SetTuning("apron_push", push)
+        if case == "split":
+            gv.SetTuning("gather_split", 1)
+        if case == "unclassed":
+            gv.SetTuning("gather_classes", 0)
+        if case == "fp16":
+            ln = gv.InsertPointsSubcell_FP16(4, len(pts), 3.0)
+            gv.GatherLevelSet_FP16(4, len(pts), 1.5, (0, 0, 0), ln, 0)
+        else:
+            ln = gv.InsertPointsSubcell(4, len(pts), 3.0)
+            if case == "density":
+                gv.GatherDensity(4, len(pts), 1.5, (0, 0, 0), ln, 0)
+            elif case == "accum":
+                gv.FillChannel(0, 0.75)
+                gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), ln, 0, -1, True)
+            else:
+                gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), ln, 0)
+        gv.UpdateApron(0, 3.0 if case != "density" else 0.0)
+        b1 = gv.bricks()
+        gv.UpdateApron(0, -2.0)   # another boundary value on the same pushed aprons
+        out.append((b1, gv.bricks()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    if case == "levelset":
+        ov = O.OracleVolume((3, 3, 3, 3, 5))
+        ov.add_channel(0, 1)
+        ov.rebuild_topology(pts, 3.0)
+        ov.update_atlas()
+        ov.insert_points_subcell(4, pts, 3.0)
+        ov.gather_level_set(1.5)
+        ov.update_apron(0, 3.0)
+        np.testing.assert_array_equal(out[0][0], ov.bricks())
+
+
+def test_apron_after_other_writes_copies_again():
+    """A channel written after a pushing gather (splat, fill) gets the full
+    apron copy from UpdateApron, as the reference does."""
+    import gvdbx as G
+    _log("[gpu] apron after other writes")
+    rng = np.random.default_rng(42)
+    pts = (rng.random((20000, 3)) * [80, 60, 70] + 20).astype(np.float32)
+    w = rng.uniform(0.1, 1.0, len(pts)).astype(np.float32)
+    gv, ov, tp = build_pair(pts, 3.0)
+    gv.SetPointWeights(torch.from_numpy(w).to(tp.device))
+    ln = gv.InsertPointsSubcell(4, len(pts), 3.0)
+    gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), ln, 0)
+    gv.SplatCubic(len(pts), 2.0, (0, 0, 0), 0, True)   # accumulates onto the level set
+    gv.UpdateApron(0, 3.0)
+    got = gv.bricks()
+    gv.SetTuning("apron_push", 0)
+    gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), ln, 0)
+    gv.SplatCubic(len(pts), 2.0, (0, 0, 0), 0, True)
+    gv.UpdateApron(0, 3.0)
+    np.testing.assert_array_equal(got, gv.bricks())
+    gv.SetTuning("apron_push", 1)
+    gv.GatherLevelSet(4, len(pts), 1.5, (0, 0, 0), ln, 0)
+    gv.FillChannel(0, 0.5)
+    gv.UpdateApron(0, 3.0)
+    ov.fill_channel(0, 0.5)
+    ov.update_apron(0, 3.0)
+    np.testing.assert_array_equal(gv.bricks(), ov.bricks())
