@@ -1438,8 +1438,9 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.variant = (noalloc || fp16) ? 1 : v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
     // 8^3 bricks with apron 1: the walk's store also fills the neighbours' aprons
+    // (not for the allocation-free call: its leaf table would be one more buffer)
     gp.push = (v->apron_push && gp.variant == 1 && gp.brick_res == 10 && gp.apron == 1 && v->res[0] == 8 &&
-               !v->chan_raw[chan]) ? 1 : 0;
+               !v->chan_raw[chan] && !noalloc) ? 1 : 0;
     if (gp.push) {
         TRY(ensure(v, v->bin_leaf, (size_t)std::max<int64_t>(gp.gb.count(), 1) * 4));
         launch_bin_leaf(gp.gb, v->grid[0], v->idx[0].as<int32_t>(), v->cnt[0], v->bin_leaf.as<int32_t>(), v->stream);
