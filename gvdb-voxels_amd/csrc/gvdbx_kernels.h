@@ -187,6 +187,7 @@ struct GatherParams {
     const int32_t* coff;      // variant 1, optional: class starts of launch_gather_classes (sorted is then
                               // its reordered copy, packed ranges in .w; box unused)
     const uint8_t* cboth;     // with coff: per bin, axes along which a point reaches past both faces
+    int quarters;             // 8-voxel tiles with classes: z pruned per 2-voxel quarter instead of per half
     int push;                 // 8^3 bricks, apron 1: also write boundary voxels into the neighbours' aprons
     const int32_t* bin_leaf;  // push: per gb cell, its leaf or -1 (launch_bin_leaf)
 };

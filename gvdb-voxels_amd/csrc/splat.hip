@@ -810,7 +810,7 @@ __device__ __forceinline__ uint64_t column_mask(int xlo, int xhi, int ylo, int y
 // stride overlaps that load.  CLS: the candidates are the class runs that
 // can reach the tile (k_gather_classes), packed into full chunks; otherwise
 // every point of the 27 bins, bin by bin.
-template <bool HALVES, bool CLS, typename Plan>
+template <bool HALVES, bool CLS, typename Plan, bool QUART = false>
 __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile& g, int td_rt, int s, int ns,
                                             Plan&& plan, uint32_t (&m)[8]) {
     const int td = HALVES ? 8 : td_rt;   // HALVES: 8-voxel tiles
@@ -921,7 +921,21 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                     if (a < 2) {
                         rlo[a] = lr, rhi[a] = hr;
                     } else {
-                        if (HALVES) {
+                        if (HALVES && QUART) {
+                            // z: membership (whole 4-voxel halves, sc >= 4), then
+                            // per 2-voxel quarter the test below for halves
+                            lr = max(lo - T0[a], 0);
+                            hr = min(hi - T0[a], td - 1);
+                            uint32_t qm = 0;
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                const float eq = fmaxf(fmaxf((float)(T0[a] + 2 * q) - pw[a], pw[a] - (float)(T0[a] + 2 * q + 1)), 0.0f);
+                                const bool in = 2 * q + 1 >= lr && 2 * q <= hr && !(__builtin_fmaf(eq, eq, db2xy) > bound);
+                                qm |= in ? 1u << q : 0u;
+                            }
+                            ok = ok && qm != 0;
+                            pk = qm;
+                        } else if (HALVES) {
                             // z: membership only (voxels beyond the cutoff cannot change the result)
                             lr = max(lo - T0[a], 0);
                             hr = min(hi - T0[a], td - 1);
@@ -936,7 +950,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                             if (__builtin_fmaf(e1, e1, db2xy) > bound) hr = min(hr, 3);
                             ok = ok && lr <= hr;
                         }
-                        pk = (uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3);
+                        if (!(HALVES && QUART)) pk = (uint32_t)(lr & 7) | ((uint32_t)(hr & 7) << 3);
                     }
                 }
                 ok = ok && db2 <= bound;
@@ -958,7 +972,19 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                     const float dx = vx - px, dy = vy - py;
                     const float dxy = __builtin_fmaf(dy, dy, dx * dx);
                     if (dxy <= lbound) {
-                        if (HALVES) {
+                        if (HALVES && QUART) {
+                            // q: the 2-voxel z quarters the candidate can lower
+#pragma unroll
+                            for (int qq = 0; qq < 4; qq++) {
+                                if ((q >> qq) & 1u) {
+#pragma unroll
+                                    for (int z = 2 * qq; z < 2 * qq + 2; z++) {
+                                        const float dz = vz[z] - pz;
+                                        m[z] = min(m[z], __float_as_uint(__builtin_fmaf(dz, dz, dxy)));
+                                    }
+                                }
+                            }
+                        } else if (HALVES) {
                             // with sc >= 4 the z membership inside an 8-voxel tile is
                             // one or both 4-voxel halves
                             if (zlo < 4) {
@@ -1076,7 +1102,7 @@ struct GatherSplit {
     uint32_t* acc;
 };
 
-template <bool HALVES, bool SPLIT, bool CLS>
+template <bool HALVES, bool SPLIT, bool CLS, bool QUART = false>
 __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int tiles_per_brick, int64_t nleaf_tiles,
                                                   GatherSplit gs) {
     const int64_t tile = gvdbx_xcd_item(blockIdx.x, nleaf_tiles, gp.xcd_remap);
@@ -1123,7 +1149,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
     uint32_t m[8];
 #pragma unroll
     for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
-    gather_walk<HALVES, CLS>(gp, g, td, 0, 1, plan, m);
+    gather_walk<HALVES, CLS, decltype(plan)&, QUART>(gp, g, td, 0, 1, plan, m);
     if (!SPLIT || ns == 1) {
         gather_store(gp, g, td, m, nl);
     } else if (slot >= 0) {
@@ -1136,7 +1162,7 @@ __global__ __launch_bounds__(64) void k_gather_rl(GatherParams gp, int res0, int
 
 // Persistent walk over the items beyond the first of each split tile
 // (counts[0] of them; the scratch tiles hold item 0's minima).
-template <bool HALVES, bool CLS>
+template <bool HALVES, bool CLS, bool QUART = false>
 __global__ __launch_bounds__(64) void k_gather_items(GatherParams gp, int res0, int tiles_per_brick,
                                                      const int4* items, int64_t item_cap, const int32_t* counts,
                                                      uint32_t* acc, int64_t slot_cap) {
@@ -1149,7 +1175,8 @@ __global__ __launch_bounds__(64) void k_gather_items(GatherParams gp, int res0, 
         uint32_t m[8];
 #pragma unroll
         for (int z = 0; z < 8; z++) m[z] = 0x7F800000u;
-        gather_walk<HALVES, CLS>(gp, g, td, w.y, w.z, [](int n, int) { return n; }, m);
+        auto same = [](int n, int) { return n; };
+        gather_walk<HALVES, CLS, decltype(same)&, QUART>(gp, g, td, w.y, w.z, same, m);
         if (lane / td < td && w.w >= 0 && w.w < slot_cap) {
             uint32_t* at = acc + (int64_t)w.w * 512 + lane;
 #pragma unroll
@@ -1892,7 +1919,9 @@ void launch_gather(GatherParams gp, hipStream_t s) {
     else if (gp.variant == 2)
         hipLaunchKernelGGL(k_gather_pc, grid, dim3(64), 0, s, gp, res0, tiles, nblk);
     else if (gp.coff) {
-        if (halves)
+        if (halves && gp.quarters)
+            hipLaunchKernelGGL((k_gather_rl<true, false, true, true>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
+        else if (halves)
             hipLaunchKernelGGL((k_gather_rl<true, false, true>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
         else
             hipLaunchKernelGGL((k_gather_rl<false, false, true>), grid, dim3(64), 0, s, gp, res0, tiles, nblk, GatherSplit{});
@@ -1954,7 +1983,9 @@ void launch_gather_split(GatherParams gp, int split, int4* items, int64_t item_c
     const GatherSplit gs{split, items, item_cap, counts, slot_tile, slot_cap, acc};
     const dim3 grid((unsigned)gvdbx_xcd_blocks(ntiles));
     const bool cls = gp.coff != nullptr;
-    if (halves && cls)
+    if (halves && cls && gp.quarters)
+        hipLaunchKernelGGL((k_gather_rl<true, true, true, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
+    else if (halves && cls)
         hipLaunchKernelGGL((k_gather_rl<true, true, true>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
     else if (halves)
         hipLaunchKernelGGL((k_gather_rl<true, true, false>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
@@ -1964,7 +1995,10 @@ void launch_gather_split(GatherParams gp, int split, int4* items, int64_t item_c
         hipLaunchKernelGGL((k_gather_rl<false, true, false>), grid, dim3(64), 0, s, gp, res0, tiles, ntiles, gs);
     const int waves = gather_waves();
     const dim3 gi((unsigned)std::max<int64_t>(1, std::min<int64_t>(item_cap, waves)));
-    if (halves && cls)
+    if (halves && cls && gp.quarters)
+        hipLaunchKernelGGL((k_gather_items<true, true, true>), gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap,
+                           counts, acc, slot_cap);
+    else if (halves && cls)
         hipLaunchKernelGGL((k_gather_items<true, true>), gi, dim3(64), 0, s, gp, res0, tiles, items, item_cap, counts,
                            acc, slot_cap);
     else if (halves)

@@ -160,6 +160,7 @@ struct gvdbx_volume {
     int bin_records = 1;  // 1: binning sort moves point records; 0: sorts (cell, index) pairs, then gathers
     int apron_order = 1;  // UpdateApron: 1 = bricks in cell order with x fastest (dense grids), 0 = leaf order
     int apron_push = 1;   // level-set / density gathers on 8^3 bricks also fill the neighbours' aprons
+    int gather_quarters = 1;  // 1: the walk prunes z per 2-voxel quarter (8-voxel tiles with reach classes)
     // per channel: the apron voxels that have a neighbour leaf hold its current
     // values (a pushing gather was the last write, on topology apron_push_gen);
     // chan_raw: the caller holds the device pointer and may write any voxel
@@ -1437,6 +1438,7 @@ int gather(gvdbx_volume* v, int mode, int sc, float radius, int64_t sc_len, int 
     gp.cutoff = accum ? 1.0e7f : (mode == 1 ? 3.0f + radius + 1.0e-3f : radius + 1.0e-3f);
     gp.variant = (noalloc || fp16) ? 1 : v->gather_variant;
     gp.xcd_remap = v->xcd_remap;
+    gp.quarters = v->gather_quarters;
     // 8^3 bricks with apron 1: the walk's store also fills the neighbours' aprons
     // (not for the allocation-free call: its leaf table would be one more buffer)
     gp.push = (v->apron_push && gp.variant == 1 && gp.brick_res == 10 && gp.apron == 1 && v->res[0] == 8 &&
@@ -1822,6 +1824,11 @@ int gvdbx_set_tuning(gvdbx_volume* v, const char* key, int value) {
         v->apron_push = value;
         return GVDBX_OK;
     }
+    if (!strcmp(key, "gather_quarters")) {
+        if (value < 0 || value > 1) FAIL(GVDBX_ERR_ARG, "gather_quarters must be 0 or 1");
+        v->gather_quarters = value;
+        return GVDBX_OK;
+    }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -1866,6 +1873,10 @@ int gvdbx_get_tuning(gvdbx_volume* v, const char* key, int* value) {
     }
     if (!strcmp(key, "apron_push")) {
         *value = v->apron_push;
+        return GVDBX_OK;
+    }
+    if (!strcmp(key, "gather_quarters")) {
+        *value = v->gather_quarters;
         return GVDBX_OK;
     }
     FAIL(GVDBX_ERR_ARG, "unknown tuning key '%s'", key);

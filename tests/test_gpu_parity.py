@@ -632,3 +632,35 @@ def test_apron_after_other_writes_copies_again():
     ov.fill_channel(0, 0.5)
     ov.update_apron(0, 3.0)
     np.testing.assert_array_equal(gv.bricks(), ov.bricks())
+
+
+@pytest.mark.parametrize("mode,R,r,sc,accum,split", [(1, 3.0, 1.5, 4, False, 16), (0, 4.0, 2.0, 4, False, 16),
+                                                     (1, 3.0, 1.5, 4, True, 0), (1, 2.0, 1.0, 4, False, 1),
+                                                     (0, 3.0, 0.5, 4, False, 0)])
+def test_gather_quarters_bit_identical(mode, R, r, sc, accum, split):
+    """z pruned per 2-voxel quarter instead of per 4-voxel half: the same bits
+    (the oracle's) for both gathers, accumulate and split tiles."""
+    _log(f"[gpu] gather quarters mode={mode} R={R} sc={sc} accum={accum} split={split}")
+    rng = np.random.default_rng(int(R * 7 + sc + split))
+    pts = np.abs(np.concatenate([rng.normal(c, 6.0, (8000, 3)) for c in ([40, 40, 40], [80, 60, 50], [50, 90, 70])])
+                 ).astype(np.float32)
+    gv, ov, tp = build_pair(pts, R)
+    gl = gv.InsertPointsSubcell(sc, len(pts), R)
+    ov.insert_points_subcell(sc, pts, R)
+    gv.SetTuning("gather_split", split)
+    out = []
+    for q in (0, 1):
+        gv.SetTuning("gather_quarters", q)
+        gv.FillChannel(0, 1.25)
+        if mode == 1:
+            gv.GatherLevelSet(sc, len(pts), r, (0, 0, 0), gl, 0, -1, accum)
+        else:
+            gv.GatherDensity(sc, len(pts), r, (0, 0, 0), gl, 0, -1, accum)
+        out.append(gv.bricks()[:, 1:9, 1:9, 1:9])
+    np.testing.assert_array_equal(out[0], out[1])
+    ov.fill_channel(0, 1.25)
+    if mode == 1:
+        ov.gather_level_set(r, accum=accum)
+    else:
+        ov.gather_density(r, accum=accum)
+    np.testing.assert_array_equal(out[1], ov.bricks()[:, 1:9, 1:9, 1:9])
