@@ -480,6 +480,8 @@ def main():
     ap.add_argument("--gather-variant", type=int, default=None,
                     help="0: LDS-staged candidates, 1: ballot/readlane walk (default), 2: point-centric LDS minima")
     ap.add_argument("--xcd-remap", type=int, default=None, help="1: contiguous brick runs per XCD (default 0)")
+    ap.add_argument("--topology", choices=["auto", "dense", "radix"], default="auto",
+                    help="topology builder (gvdbx_set_topology_mode) of the config-2 frame")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="set a kernel tuning knob (gvdbx_set_tuning) before the run; repeatable")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -531,6 +533,7 @@ def main():
     for kv in args.tune:
         k, _, val = kv.partition("=")
         gv.SetTuning(k, int(val))
+    gv.SetTopologyMode({"auto": gvdbx.TOPO_AUTO, "dense": gvdbx.TOPO_DENSE, "radix": gvdbx.TOPO_RADIX}[args.topology])
     torch.cuda.synchronize()
 
     ev = {}
@@ -629,7 +632,8 @@ def main():
         "stage_traffic_source": "profiles/config2_frame_stage_counters.md" if n == 1 << 24 else None,
         "autotune": tune,
         "tuning": {k: gv.GetTuning(k) for k in ("gather_variant", "gather_classes", "gather_split", "xcd_remap", "apron_order",
-                                                "bin_records", "sort_tiles")},
+                                                "apron_push", "gather_quarters", "bin_records", "sort_tiles")},
+        "topology_mode": {1: "dense", 2: "radix"}.get(inf.topology_mode_used, "?"),
         "fwd_bwd_over_fwd": None,
     }
     if not args.no_diff:

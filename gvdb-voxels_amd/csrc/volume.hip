@@ -473,8 +473,10 @@ int ensure_bins(gvdbx_volume* v, int64_t n, const float t[3], bool stable, const
     const int64_t m = std::max<int64_t>(n, 1);
     const int64_t nh = radix_hist_entries(m);
     TRY(ensure(v, v->bin_off, (size_t)(nb + 1) * 4));
-    if (v->ncloud == 0 && v->bin_records) {
-        // one cloud: the sort moves the point records (no random point reads)
+    if (v->ncloud == 0 && v->bin_records && !gb.sparse()) {
+        // one cloud: the sort moves the point records (no random point reads;
+        // each pass finds a record's bin again, which a sparse grid makes a
+        // hash probe, so sparse grids sort (bin, index) pairs instead)
         TRY(ensure(v, v->bkey, (size_t)m * 4));
         TRY(ensure(v, v->rs_hist, (size_t)nh * 4));
         TRY(ensure(v, v->rs_off, (size_t)(nh + 1) * 4));
