@@ -408,6 +408,7 @@ struct RenderScene {
     const float4* lut;
     float finestep;                      // SCN_FINESTEP (SHADE_TRICUBIC)
     float slice_pnt[3], slice_norm[3];   // SCN_SLICE_PNT / SCN_SLICE_NORM (sections)
+    const uchar4* clr;                   // SetColorChannel: the uchar4 channel colouring Render, or null
 };
 // per (ray, brick) segment entry state of a recorded EA raycast
 // Segment checkpoints of a recording EA raycast.  Each wave buffers its

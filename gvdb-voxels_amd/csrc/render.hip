@@ -197,7 +197,26 @@ enum { MODE_DEEP_LUT = 0, MODE_DEEP_LINEAR = 1, MODE_SURF_TRI = 2, MODE_SURF_LS 
 struct RayState {
     float clr[4];
     V3 hit, norm;
+    float hclr[4];   // the colour channel at a surface hit (Render's shading)
 };
+
+// getColorF RC.cuh:206-209 with make_float4(uchar4) cuda_math.cuh:221-224:
+// the colour channel's uchar4 at voxel floor(p) of the leaf (brick-local p,
+// interior from 0), each byte / 255.  The reference truncates p + the
+// brick's atlas offset in FP32; floor(p) is the same voxel except where
+// that sum rounds up to the next integer.
+__device__ __forceinline__ void color_at(const RenderTree& tv, const RenderScene& sc, int64_t leaf, V3 p,
+                                         float (&out)[4]) {
+    const int br = tv.brick_res, ap = (br - tv.res[0]) >> 1;
+    const int qx = min(max((int)floorf(p.x) + ap, 0), br - 1);
+    const int qy = min(max((int)floorf(p.y) + ap, 0), br - 1);
+    const int qz = min(max((int)floorf(p.z) + ap, 0), br - 1);
+    const uchar4 c = sc.clr[leaf * (int64_t)(br * br * br) + (qz * br + qy) * br + qx];
+    out[0] = (float)c.x / 255.0f;
+    out[1] = (float)c.y / 255.0f;
+    out[2] = (float)c.z / 255.0f;
+    out[3] = (float)c.w / 255.0f;
+}
 
 
 // getGradient RC.cuh:132-142 (sign -1) / getGradientLevelSet RC.cuh:146-157 (+1)
@@ -338,9 +357,18 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
                     val = make_float4(s, s, s, s);
                 const float om = ea_one_minus(ek, val.w, ea_atten(ek, val.w));
                 val.w = ea_atten(ek, val.w);
-                st.clr[0] += val.x * st.clr[3] * om * sc.albedo;
-                st.clr[1] += val.y * st.clr[3] * om * sc.albedo;
-                st.clr[2] += val.z * st.clr[3] * om * sc.albedo;
+                if (MODE == MODE_DEEP_LUT && sc.clr) {
+                    // RGB from the colour channel at the sample (RC.cuh:519)
+                    float h[4];
+                    color_at(tv, sc, leaf, p, h);
+                    st.clr[0] += val.x * st.clr[3] * om * sc.albedo * h[0];
+                    st.clr[1] += val.y * st.clr[3] * om * sc.albedo * h[1];
+                    st.clr[2] += val.z * st.clr[3] * om * sc.albedo * h[2];
+                } else {
+                    st.clr[0] += val.x * st.clr[3] * om * sc.albedo;
+                    st.clr[1] += val.y * st.clr[3] * om * sc.albedo;
+                    st.clr[2] += val.z * st.clr[3] * om * sc.albedo;
+                }
                 st.clr[3] *= val.w;
             }
             p = add(p, wpt);
@@ -385,6 +413,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
                 st.norm = v3(fabsf(fc.x) == m ? copysignf(1.0f, fc.x) : 0.0f, fabsf(fc.y) == m ? copysignf(1.0f, fc.y) : 0.0f,
                              fabsf(fc.z) == m ? copysignf(1.0f, fc.z) : 0.0f);
                 st.hit = hit;
+                if (sc.clr) color_at(tv, sc, leaf, v3((float)d.p[0], (float)d.p[1], (float)d.p[2]), st.hclr);
                 return;
             }
             d.next();
@@ -402,6 +431,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
                 p = add(p, scale(dir, -vy * sc.finestep));
                 st.hit = add(p, vmin);
                 st.norm = gradient_cubic<BR>(B, br, p);
+                if (sc.clr) color_at(tv, sc, leaf, p, st.hclr);
                 return;
             }
             p = add(p, scale(dir, sc.step));
@@ -414,6 +444,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
             if (trilinear<BR>(B, br, p) >= sc.thresh) {
                 st.hit = add(p, vmin);
                 st.norm = gradient<BR>(B, br, p, false);
+                if (sc.clr) color_at(tv, sc, leaf, p, st.hclr);
                 return;
             }
             p = add(p, scale(dir, sc.step));
@@ -425,6 +456,7 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
             if (trilinear<BR>(B, br, p) < sc.thresh) {
                 st.hit = add(p, vmin);
                 st.norm = gradient<BR>(B, br, p, true);
+                if (sc.clr) color_at(tv, sc, leaf, p, st.hclr);
                 return;
             }
             p = add(p, scale(dir, sc.step));
@@ -610,7 +642,8 @@ __global__ __launch_bounds__(256) void k_render_volume(RenderTree tv, RenderCam 
 }
 
 // gvdbRaySurfaceTrilinear / gvdbRayLevelSet MOD.cu:101-181 with
-// performPhongShading MOD.cu:38-57 (no colour channel)
+// performPhongShading MOD.cu:38-57: the hit's colour (1 without a colour
+// channel) times diffuse + ambient
 template <int MODE, int BR>
 __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba,
                                                         uchar4* rgba8) {
@@ -622,6 +655,7 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
     st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
     st.hit = MODE == MODE_SURF_LS ? v3(0, 0, GVDBX_NOHIT) : v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT);
     st.norm = v3(0, 0, 0);
+    st.hclr[0] = st.hclr[1] = st.hclr[2] = st.hclr[3] = 1.0f;
     VisitRec none{};
     ray_cast<MODE, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
     float4 o;
@@ -644,7 +678,7 @@ __global__ __launch_bounds__(256) void k_render_surface(RenderTree tv, RenderCam
             if (s2.hit.z != GVDBX_NOHIT) diff = (float)((double)diff * (1.0 - (double)sc.shadow_amt));
         }
         const float sh = diff + amb;
-        o = make_float4(sh, sh, sh, 1.0f);
+        o = make_float4(st.hclr[0] * sh, st.hclr[1] * sh, st.hclr[2] * sh, 1.0f);
     }
     const int64_t px = (int64_t)y * cam.width + x;
     if (rgba) rgba[px] = o;
@@ -765,9 +799,15 @@ __global__ __launch_bounds__(256) void k_render_section3d(RenderTree tv, RenderC
             const int br = BR > 0 ? BR : tv.brick_res;
             const int zero[3] = {0, 0, 0};
             const V3 vmin = node_pos(tv.nodes[0], leaf, zero);
-            t = trilinear<BR>(tv.chan + leaf * (int64_t)(br * br * br), br, sub(pos, vmin));
+            const V3 pl = sub(pos, vmin);
+            t = trilinear<BR>(tv.chan + leaf * (int64_t)(br * br * br), br, pl);
             const float4 c = transfer_lut(sc, t);
             clr[0] = c.x; clr[1] = c.y; clr[2] = c.z; clr[3] = c.w;
+            if (sc.clr) {   // MOD.cu:249-252: times the colour there (alpha times 1)
+                float h[4];
+                color_at(tv, sc, leaf, pl, h);
+                clr[0] *= h[0]; clr[1] *= h[1]; clr[2] *= h[2];
+            }
         } else {
             t = 0;
         }
@@ -776,15 +816,18 @@ __global__ __launch_bounds__(256) void k_render_section3d(RenderTree tv, RenderC
     st.clr[0] = st.clr[1] = st.clr[2] = st.clr[3] = 1.0f;
     st.hit = v3(GVDBX_NOHIT, GVDBX_NOHIT, GVDBX_NOHIT);
     st.norm = v3(0, 0, 0);
+    st.hclr[0] = st.hclr[1] = st.hclr[2] = st.hclr[3] = 1.0f;
     VisitRec none{};
     ray_cast<MODE_SURF_TRI, BR, false>(tv, sc, cam, pos, dir, st, 0, none);
     float o[4];
     if (st.hit.z != GVDBX_NOHIT) {
         const V3 ldir = normalize(sub(v3(sc.light[0], sc.light[1], sc.light[2]), st.hit));
         const float ds = (t > sc.thresh) ? 1.0f : (float)(0.8 * (double)fmaxf(0.0f, dot(st.norm, ldir)));
-        const float a = 1.0f * ds;
 #pragma unroll
-        for (int k = 0; k < 4; k++) o[k] = a + clr[3] * (clr[k] - a);
+        for (int k = 0; k < 4; k++) {
+            const float a = st.hclr[k] * ds;   // the surface colour (1 without a colour channel)
+            o[k] = a + clr[3] * (clr[k] - a);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = sc.backclr[k] + clr[3] * (clr[k] - sc.backclr[k]);

@@ -161,6 +161,7 @@ struct gvdbx_volume {
     int apron_order = 1;  // UpdateApron: 1 = bricks in cell order with x fastest (dense grids), 0 = leaf order
     int apron_push = 1;   // level-set / density gathers on 8^3 bricks also fill the neighbours' aprons
     int gather_quarters = 1;  // 1: the walk prunes z per 2-voxel quarter (8-voxel tiles with reach classes)
+    int clr_chan = -1;        // SetColorChannel: the uchar4 channel colouring Render, -1 none
     // per channel: the apron voxels that have a neighbour leaf hold its current
     // values (a pushing gather was the last write, on topology apron_push_gen);
     // chan_raw: the caller holds the device pointer and may write any voxel
@@ -1592,6 +1593,7 @@ int make_render(gvdbx_volume* v, int chan, const gvdbx_camera* cam, const gvdbx_
     sc.finestep = rp->finestep;
     memcpy(sc.slice_pnt, rp->slice_pnt, sizeof(sc.slice_pnt));
     memcpy(sc.slice_norm, rp->slice_norm, sizeof(sc.slice_norm));
+    sc.clr = nullptr;   // Render sets it (Raytrace and the EA raycast take no colour)
     return GVDBX_OK;
 }
 
@@ -1704,6 +1706,25 @@ int gvdbx_set_channel_default(gvdbx_volume* v, int sx, int sy, int sz) {
     v->def_axis[0] = sx;
     v->def_axis[1] = sy;
     v->def_axis[2] = sz;
+    return GVDBX_OK;
+}
+
+int gvdbx_set_color_channel(gvdbx_volume* v, int chan) {
+    TRY(check_vol(v));
+    // the reference stores a uchar: -1 and 255 (CHAN_UNDEF) both mean none
+    if (chan == -1 || chan == 255) {
+        v->clr_chan = -1;
+        return GVDBX_OK;
+    }
+    if (chan < 0 || chan >= GVDBX_MAXCHAN) FAIL(GVDBX_ERR_ARG, "SetColorChannel: channel %d out of range", chan);
+    v->clr_chan = chan;
+    return GVDBX_OK;
+}
+
+int gvdbx_get_color_channel(gvdbx_volume* v, int* chan) {
+    TRY(check_vol(v));
+    if (!chan) FAIL(GVDBX_ERR_ARG, "GetColorChannel: null output");
+    *chan = v->clr_chan;
     return GVDBX_OK;
 }
 
@@ -2273,6 +2294,10 @@ int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, 
         if (((uintptr_t)rp->lut_dev & 15) != 0) FAIL(GVDBX_ERR_ARG, "Render: transfer function must be 16-byte aligned");
         if (shade == GVDBX_SHADE_TRICUBIC && !(rp->finestep > 0.0f && isfinite(rp->finestep)))
             FAIL(GVDBX_ERR_ARG, "Render: SHADE_TRICUBIC needs finestep > 0");
+        if (v->clr_chan >= 0) {
+            TRY(check_chan_type(v, v->clr_chan, GVDBX_UCHAR4, "Render (colour channel)"));
+            sc.clr = v->chan[v->clr_chan].as<uchar4>();
+        }
     }
     if (!rgba_dev && !rgba8_dev) return GVDBX_OK;
     launch_render(shade, tv, rc, sc, (float4*)rgba_dev, (uchar4*)rgba8_dev, v->stream);
@@ -2899,6 +2924,7 @@ int vbx_load_grid(gvdbx_volume* v, VbxIO& f, const char* path) {
         free_buf(v->chan[c]);
         v->has_chan[c] = false;
     }
+    v->clr_chan = -1;   // DestroyChannels VG.cpp:2474-2484
     v->atlas_cap = 0;
     std::vector<int> ctype((size_t)nch);
     for (int c = 0; c < nch; c++) {

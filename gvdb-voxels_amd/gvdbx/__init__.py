@@ -308,6 +308,16 @@ class VolumeGVDB:
         call("gvdbx_update_apron", self._h, int(chan), float(boundval))
 
     # ------------------------------------------------------------ rendering
+    def SetColorChannel(self, chan):
+        """SetColorChannel(uchar chan) (VG.cpp:686-690): the uchar4 channel
+        that colours Render; -1 or 255 for none."""
+        call("gvdbx_set_color_channel", self._h, int(chan))
+
+    def GetColorChannel(self):
+        out = C.c_int(0)
+        call("gvdbx_get_color_channel", self._h, C.byref(out))
+        return out.value
+
     def Render(self, shade, chan, cam, rp, rgba=None, rgba8=None):
         """Render(shade, chan, buf) (VG.cpp:4336-4381).  Returns (rgba float32
         (H,W,4), rgba8 uint8 (H,W,4)) device tensors."""

@@ -96,6 +96,8 @@ _SIGS = {
     "gvdbx_splat_cubic": [_vp, _i, _f, _f, _f, _f, _i, _i],
     "gvdbx_update_apron": [_vp, _i, _f],
     "gvdbx_render": [_vp, _i, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp],
+    "gvdbx_set_color_channel": [_vp, _i],
+    "gvdbx_get_color_channel": [_vp, C.POINTER(C.c_int)],
     "gvdbx_raycast_ea": [_vp, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp, _i],
     "gvdbx_raycast_ea_backward": [_vp, _i, C.POINTER(Camera), C.POINTER(RenderParams), _vp, _vp, _vp, _vp, _vp],
     "gvdbx_apron_adjoint": [_vp, _vp],

@@ -192,6 +192,8 @@ public:
                 float* rgba_dev, uint8_t* rgba8_dev = nullptr) {
         check(gvdbx_render(v_, shade, chan, &cam, &rp, rgba_dev, rgba8_dev), "Render");
     }
+    // SetColorChannel(uchar) VG.cpp:686-690: 255 (CHAN_UNDEF) for none
+    void SetColorChannel(unsigned char chan) { check(gvdbx_set_color_channel(v_, chan), "SetColorChannel"); }
     void Raytrace(gvdbx_scn_ray* rays_dev, int64_t num_rays, unsigned char chan, const gvdbx_render_params& rp,
                   float bias) {
         check(gvdbx_raytrace(v_, chan, &rp, rays_dev, num_rays, bias), "Raytrace");

@@ -359,13 +359,23 @@ typedef struct {
  * ReadRenderBuf, every shading of the reference's switch: gvdbRayDeep
  * (SHADE_VOLUME), gvdbRaySurfaceVoxel / Trilinear / Tricubic, gvdbRayLevelSet,
  * gvdbRayEmptySkip, gvdbSection2D / 3D (MOD.cu:60-293) and SHADE_OFF (the
- * image cleared).  No colour channel is applied.  rgba_dev (optional): 4
+ * image cleared).  With a colour channel set (gvdbx_set_color_channel) the
+ * surface shadings colour their hits, SHADE_VOLUME its samples and
+ * SHADE_SECTION3D its slice and surface by that channel's uchar4 / 255 at
+ * the voxel holding the point, as the reference does.  rgba_dev (optional): 4
  * floats per pixel, the colour before quantisation; rgba8_dev (optional):
  * the reference's uchar4 output (alpha 255 for EMPTYSKIP and the sections,
  * as there).  Rows of width pixels, y-major.  Tricubic taps past a padded
  * brick clamp to it (the reference reads into the neighbouring atlas brick). */
 int gvdbx_render(gvdbx_volume* v, int shade, int chan, const gvdbx_camera* cam, const gvdbx_render_params* rp,
                  float* rgba_dev, uint8_t* rgba8_dev);
+/* SetColorChannel(uchar chan) VG.h / VG.cpp:686-690: the uchar4 channel that
+ * colours Render (getColorF RC.cuh:206-209); -1 or 255 (CHAN_UNDEF): none,
+ * the default, and what loading a VBX file (DestroyChannels) resets it to.
+ * Render fails with GVDBX_ERR_ARG while it names no uchar4 channel. */
+int gvdbx_set_color_channel(gvdbx_volume* v, int chan);
+/* mVDBInfo.clr_chan: the channel set above, -1 for none */
+int gvdbx_get_color_channel(gvdbx_volume* v, int* chan);
 /* Raytrace(DataPtr rays, uchar chan, char, int, float bias) VG.h:342 /
  * VG.cpp:4384-4408 (gvdbRaytrace MOD.cu:207-219): for every ray record in
  * device memory the trilinear surface hit (orig + t dir, first sample >=

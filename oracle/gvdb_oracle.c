@@ -1563,10 +1563,29 @@ typedef struct {
     double clr64[4];
     int64_t hit_leaf;
     f3 hit_p;
+    /* colour channel (SetColorChannel): its atlas words, or NULL; the colour
+     * the surface brick functions return with a hit (getColorF, 0..255) */
+    const uint32_t* C;
+    float hclr[4];
     /* optional sample recording (EA adjoint) */
     or_sample* rec;
     int64_t nrec, caprec;
 } ray_ctx;
+
+/* getColorF RC.cuh:206-209 with make_float4(uchar4) cuda_math.cuh:221-224:
+ * the uchar4 of the colour channel at voxel floor(p) (brick-local p, the
+ * interior starting at 0), each byte / 255 in FP32, x from the lowest byte.  The reference truncates p + the brick's atlas
+ * offset in FP32; floor(p) is the same voxel except where that sum rounds
+ * up to the next integer. */
+static void clr_at(const ray_ctx* c, int64_t leaf, f3 p, float out[4]) {
+    const or_node* nd = &c->v->nodes[0][leaf];
+    int br = brick_res(c->v), a = c->v->apron;
+    int q[3] = {(int)floorf(p.x) + a, (int)floorf(p.y) + a, (int)floorf(p.z) + a};
+    for (int k = 0; k < 3; k++) q[k] = q[k] < 0 ? 0 : (q[k] >= br ? br - 1 : q[k]);
+    uint32_t w = c->C[atlas_index(c->v, (int64_t)nd->mValue[0] - a + q[0], (int64_t)nd->mValue[1] - a + q[1],
+                                  (int64_t)nd->mValue[2] - a + q[2])];
+    for (int k = 0; k < 4; k++) out[k] = (float)((w >> (8 * k)) & 255u) / 255.0f;
+}
 
 static float tri_at32(const ray_ctx* c, const or_node* nd, int i, int j, int k) {
     int br = brick_res(c->v);
@@ -1733,7 +1752,7 @@ static int transfer_index(const or_render_params* rp, float s) {
     return (int)(qc * 16300.0f);
 }
 
-/* rayDeepBrick RC.cuh:485-533 (no depth buffer, no colour channel).
+/* rayDeepBrick RC.cuh:485-533 (no depth buffer).
  * M_DEEP_LUT uses the transfer function; M_DEEP_LINEAR is the transfer
  * s -> (s,s,s,s) with no final clamp (the differentiable form). */
 static void brick_deep(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
@@ -1766,11 +1785,15 @@ static void brick_deep(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
                 for (int k = 0; k < 4; k++) { val[k] = s; v64[k] = s64; }
             }
             record_sample(c, leaf, p, li);
+            /* RGB from the colour channel (RC.cuh:519), else 1 */
+            float h[4] = {1, 1, 1, 1};
+            if (c->C && c->mode == M_DEEP_LUT) clr_at(c, leaf, p, h);
             val[3] = expf(rp->extinct * val[3] * step);
-            for (int k = 0; k < 3; k++) c->clr[k] += val[k] * c->clr[3] * (1 - val[3]) * rp->albedo;
+            for (int k = 0; k < 3; k++) c->clr[k] += val[k] * c->clr[3] * (1 - val[3]) * rp->albedo * h[k];
             c->clr[3] *= val[3];
             double a64 = exp((double)rp->extinct * v64[3] * (double)step);
-            for (int k = 0; k < 3; k++) c->clr64[k] += v64[k] * c->clr64[3] * (1.0 - a64) * (double)rp->albedo;
+            for (int k = 0; k < 3; k++)
+                c->clr64[k] += v64[k] * c->clr64[3] * (1.0 - a64) * (double)rp->albedo * (double)h[k];
             c->clr64[3] *= a64;
         }
         p = f3add(p, wpt);
@@ -1830,6 +1853,7 @@ static void brick_surf_tri(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
             c->norm = gradient32(c, leaf, p, -1);
             c->hit_leaf = leaf;
             c->hit_p = p;
+            if (c->C) clr_at(c, leaf, p, c->hclr);
             return;
         }
         p = f3add(p, f3scale(dir, rp->step));
@@ -1853,6 +1877,7 @@ static void brick_surf_ls(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
             c->norm = gradient32(c, leaf, p, +1);
             c->hit_leaf = leaf;
             c->hit_p = p;
+            if (c->C) clr_at(c, leaf, p, c->hclr);
             return;
         }
         p = f3add(p, f3scale(dir, rp->step));
@@ -1893,6 +1918,7 @@ static void brick_surf_voxel(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
             c->hit = hit;
             c->hit_leaf = leaf;
             c->hit_p = f3sub(hit, vmin);
+            if (c->C) clr_at(c, leaf, f3m((float)d.p[0], (float)d.p[1], (float)d.p[2]), c->hclr);
             return;
         }
         hdda_next(&d);
@@ -1963,6 +1989,7 @@ static void brick_surf_cubic(ray_ctx* c, int64_t leaf, f3 t, f3 pos, f3 dir) {
             c->norm = gradient_cubic32(c, leaf, p);
             c->hit_leaf = leaf;
             c->hit_p = p;
+            if (c->C) clr_at(c, leaf, p, c->hclr);
             return;
         }
         p = f3add(p, f3scale(dir, rp->step));
@@ -2056,12 +2083,16 @@ static void camera_ray(const or_camera* cam, int x, int y, f3* pos, f3* dir) {
 
 static int render_check(or_volume* v, int chan, const or_camera* cam, const or_render_params* rp) {
     if (!v || !cam || !rp || chan < 0 || chan >= OR_MAXCHAN || !v->has_chan[chan]) return OR_ERR_ARG;
+    if (rp->use_clr && (rp->clr_chan < 0 || rp->clr_chan >= OR_MAXCHAN || !v->has_chan[rp->clr_chan] ||
+                        v->chan_type[rp->clr_chan] != OR_T_UCHAR4))
+        return OR_ERR_ARG;
     if (cam->width <= 0 || cam->height <= 0 || !(rp->step > 0)) return OR_ERR_ARG;
     if (!v->amap || v->apron != 1) return OR_ERR_STATE;
     return OR_OK;
 }
 
-/* performPhongShading MOD.cu:38-57 (colour (1,1,1,1)); FP32 and FP64 */
+/* performPhongShading MOD.cu:38-57: the hit's colour (1,1,1 without a
+ * colour channel) times diffuse + ambient; FP32 and FP64 */
 static void phong(ray_ctx* c, f3 pos_unused, float out[4], double out64[4]) {
     (void)pos_unused;
     const or_render_params* rp = c->rp;
@@ -2103,9 +2134,11 @@ static void phong(ray_ctx* c, f3 pos_unused, float out[4], double out64[4]) {
         }
     }
     float sh = diff + amb;
-    out[0] = out[1] = out[2] = sh;
+    for (int k = 0; k < 3; k++) {
+        out[k] = c->hclr[k] * sh;
+        out64[k] = (double)c->hclr[k] * (fp64 ? diff64 + 0.1 : (double)sh);
+    }
     out[3] = 1.0f;
-    out64[0] = out64[1] = out64[2] = fp64 ? diff64 + 0.1 : (double)sh;
     out64[3] = 1.0;
 }
 
@@ -2153,8 +2186,14 @@ static void section3d(ray_ctx* c, f3 pos, f3 dir, float o[4]) {
         int64_t leaf = point_leaf(c->v, w);
         if (leaf >= 0) {
             const or_node* nd = &c->v->nodes[0][leaf];
-            t = tri32(c, leaf, f3m(wpos.x - (float)nd->mPos[0], wpos.y - (float)nd->mPos[1], wpos.z - (float)nd->mPos[2]));
+            f3 pl = f3m(wpos.x - (float)nd->mPos[0], wpos.y - (float)nd->mPos[1], wpos.z - (float)nd->mPos[2]);
+            t = tri32(c, leaf, pl);
             lut_at(rp, t, clr);
+            if (c->C) {   /* MOD.cu:249-252: times the colour there (alpha times 1) */
+                float h[4];
+                clr_at(c, leaf, pl, h);
+                for (int k = 0; k < 3; k++) clr[k] *= h[k];
+            }
         } else {
             t = 0;
         }
@@ -2164,12 +2203,13 @@ static void section3d(ray_ctx* c, f3 pos, f3 dir, float o[4]) {
     s.hit = f3m(OR_NOHIT, OR_NOHIT, OR_NOHIT);
     s.norm = f3m(0, 0, 0);
     s.clr[0] = s.clr[1] = s.clr[2] = s.clr[3] = 1;
+    for (int k = 0; k < 4; k++) s.hclr[k] = 1.0f;
     ray_cast(&s, wpos, dir);
     if (s.hit.z != OR_NOHIT) {
         f3 ldir = f3norm(f3sub(f3m(rp->light[0], rp->light[1], rp->light[2]), s.hit));
         float ds = (t > rp->thresh) ? 1.0f : (float)(0.8 * (double)fmaxf(0.0f, f3dot(s.norm, ldir)));
         for (int k = 0; k < 4; k++) {
-            float a = 1.0f * ds;
+            float a = s.hclr[k] * ds;   /* the surface colour (1 without a colour channel) */
             o[k] = a + clr[3] * (clr[k] - a);
         }
     } else {
@@ -2204,6 +2244,8 @@ int or_render(or_volume* v, int chan, const or_camera* cam, const or_render_para
             ray_ctx c;
             memset(&c, 0, sizeof(c));
             c.v = v; c.A = v->atlas[chan]; c.B64 = bricks64; c.rp = rp; c.mode = mode;
+            c.C = rp->use_clr ? (const uint32_t*)v->atlas[rp->clr_chan] : NULL;
+            for (int k = 0; k < 4; k++) c.hclr[k] = 1.0f;
             f3 pos, dir;
             camera_ray(cam, x, y, &pos, &dir);
             float o32[4];

@@ -194,6 +194,7 @@ typedef struct {
     const float* lut;   /* transfer function, 16384 x float4 (SHADE_VOLUME, sections) */
     float finestep;     /* SCN_FINESTEP [0.1] (SHADE_TRICUBIC) */
     float slice_pnt[3], slice_norm[3];   /* SCN_SLICE_PNT / SCN_SLICE_NORM (sections) */
+    int use_clr, clr_chan;   /* SetColorChannel VG.cpp:686-690: use_clr != 0 colours or_render from uchar4 channel clr_chan */
 } or_render_params;
 
 /* ScnRay (cuda_gvdb_scene.cuh:66-74): 64-byte ray record of Raytrace */

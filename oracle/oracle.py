@@ -40,7 +40,7 @@ class OrRenderParams(C.Structure):
                 ("vmax", C.c_float), ("epsilon", C.c_float), ("max_iter", C.c_int), ("backclr", C.c_float * 4),
                 ("light", C.c_float * 3), ("shadow_amt", C.c_float), ("shadow_bias", C.c_float),
                 ("lut", C.POINTER(C.c_float)), ("finestep", C.c_float), ("slice_pnt", C.c_float * 3),
-                ("slice_norm", C.c_float * 3)]
+                ("slice_norm", C.c_float * 3), ("use_clr", C.c_int), ("clr_chan", C.c_int)]
 
 
 class OrScnRay(C.Structure):
@@ -85,6 +85,8 @@ def make_params(rp):
     p.finestep = float(rp.get("finestep", 0.1))
     p.slice_pnt = (C.c_float * 3)(*[float(x) for x in rp.get("slice_pnt", (0, 0, 0))])
     p.slice_norm = (C.c_float * 3)(*[float(x) for x in rp.get("slice_norm", (0, 0, 1))])
+    clr = rp.get("clr_chan")   # SetColorChannel: a uchar4 channel colours or_render
+    p.use_clr, p.clr_chan = (0, 0) if clr is None or int(clr) < 0 else (1, int(clr))
     lut = rp.get("lut")
     p._lut_keep = None
     if lut is not None:
