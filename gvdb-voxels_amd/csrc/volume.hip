@@ -3091,14 +3091,22 @@ int gvdbx_save_vbx(gvdbx_volume* v, const char* path) {
     return GVDBX_OK;
 }
 
-int gvdbx_load_vbx(gvdbx_volume* v, const char* path) {
+int gvdbx_load_vbx(gvdbx_volume* v, const char* path) { return gvdbx_load_vbx_version(v, path, 0, 0); }
+
+int gvdbx_load_vbx_version(gvdbx_volume* v, const char* path, int force_maj, int force_min) {
     TRY(check_vol(v));
     if (!path) FAIL(GVDBX_ERR_ARG, "LoadVBX: null path");
+    if (force_maj < 0 || force_maj > 255 || force_min < 0 || force_min > 255)
+        FAIL(GVDBX_ERR_ARG, "LoadVBX: forced version %d.%d out of range", force_maj, force_min);
     VbxIO f;
     f.fp = fopen(path, "rb");
     if (!f.fp) FAIL(GVDBX_ERR_ARG, "LoadVBX: unable to open %s", path);
     unsigned char major = 0, minor = 0;
     f.r(&major, 1), f.r(&minor, 1);
+    if (force_maj > 0 || force_min > 0) {   // VG.cpp:551-554: read the file as that version
+        major = (unsigned char)force_maj;
+        minor = (unsigned char)force_min;
+    }
     float xf[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
     if ((major == 1 && minor >= 11) || major > 1) f.r(xf, 48);   // VG.cpp:562-569
     int num_grids = 0;

@@ -241,8 +241,10 @@ class VolumeGVDB:
     def SaveVBX(self, fname):
         call("gvdbx_save_vbx", self._h, str(fname).encode())
 
-    def LoadVBX(self, fname):
-        call("gvdbx_load_vbx", self._h, str(fname).encode())
+    def LoadVBX(self, fname, force_maj=0, force_min=0):
+        """LoadVBX (VG.cpp:507-685); force_maj / force_min > 0 read the file
+        as that version whatever its header says."""
+        call("gvdbx_load_vbx_version", self._h, str(fname).encode(), int(force_maj), int(force_min))
         inf = self.info()
         self.log2dims = tuple(int(inf.log2dim[l]) for l in range(inf.levs))
         return True

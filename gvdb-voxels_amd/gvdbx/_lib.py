@@ -79,6 +79,7 @@ _SIGS = {
     "gvdbx_frame_graph_launch": [_vp],
     "gvdbx_frame_graph_destroy": [_vp],
     "gvdbx_load_vbx": [_vp, C.c_char_p],
+    "gvdbx_load_vbx_version": [_vp, C.c_char_p, _i, _i],
     "gvdbx_set_transform": [_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)],
     "gvdbx_get_transform": [_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)],
     "gvdbx_accumulate_topology": [_vp, _i, _f, _f, _f, _f, _i],

@@ -117,9 +117,7 @@ public:
     void RequestFullRebuild(bool tf) { check(gvdbx_request_full_rebuild(v_, tf ? 1 : 0), "RequestFullRebuild"); }
     void SaveVBX(const std::string& fname) { check(gvdbx_save_vbx(v_, fname.c_str()), "SaveVBX"); }
     bool LoadVBX(const std::string& fname, int force_maj = 0, int force_min = 0) {
-        if (force_maj > 0 || force_min > 0)
-            throw std::runtime_error("LoadVBX: forcing a file version is not supported");
-        check(gvdbx_load_vbx(v_, fname.c_str()), "LoadVBX");
+        check(gvdbx_load_vbx_version(v_, fname.c_str(), force_maj, force_min), "LoadVBX");
         return true;
     }
     void SetTransform(Vector3DF pretrans, Vector3DF scal, Vector3DF angs, Vector3DF trans) {

@@ -298,6 +298,11 @@ int gvdbx_save_vbx(gvdbx_volume* v, const char* path);
  * (version 1.0, or 2.x with the flag) are refused; later grids replace
  * earlier ones, as there. */
 int gvdbx_load_vbx(gvdbx_volume* v, const char* path);
+/* LoadVBX(fname, force_maj, force_min) VG.h:374 / VG.cpp:507-554: as above,
+ * reading the file as version force_maj.force_min (which decides whether a
+ * transform and a bitmask flag follow the version bytes) when either is > 0,
+ * whatever its header says. */
+int gvdbx_load_vbx_version(gvdbx_volume* v, const char* path, int force_maj, int force_min);
 /* SetTransform(pretrans, scale, angs, trans) VG.h:672: kept with the volume
  * and written to / read from VBX files; it changes nothing else here
  * (cameras and points are in index space). */

@@ -134,3 +134,31 @@ def test_vbx_transform_and_errors(tmp_path):
         G.VolumeGVDB(0).LoadVBX(tmp_path / "missing.vbx")
     with pytest.raises(G.GvdbxError):
         G.VolumeGVDB(0).SaveVBX(tmp_path / "empty.vbx")   # nothing to save
+
+
+def test_vbx_forced_version(tmp_path):
+    """LoadVBX(fname, force_maj, force_min) (VG.cpp:551-554): a file whose
+    version bytes are wrong loads as the version forced; without it the
+    header is read as version 1.0 (bitmask node records) and refused."""
+    import gvdbx as G
+    _log("[gpu] vbx forced version")
+    gv, ov, keep = _build(False)
+    f = tmp_path / "x.vbx"
+    gv.SaveVBX(f)
+    data = f.read_bytes()
+    assert data[:2] == bytes([1, 11])
+    relabel = tmp_path / "relabel.vbx"
+    relabel.write_bytes(bytes([1, 0]) + data[2:])   # 1.0: no transform, bitmask node records
+    with pytest.raises(G.GvdbxError):
+        G.VolumeGVDB(0).LoadVBX(relabel)
+    lv = G.VolumeGVDB(0)
+    lv.LoadVBX(relabel, 1, 11)
+    torch.cuda.synchronize()
+    assert_same_topology(lv, ov, points_bbox=False)
+    for c in (0, 1):
+        np.testing.assert_array_equal(lv.bricks(c).view(np.uint32), ov.bricks(c).view(np.uint32))
+    f2 = tmp_path / "again.vbx"
+    lv.SaveVBX(f2)
+    assert f2.read_bytes() == data
+    with pytest.raises(G.GvdbxError):
+        G.VolumeGVDB(0).LoadVBX(f, 300, 0)   # not a version byte
