@@ -875,12 +875,30 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
             // may reach changes none of them if db2 exceeds that max (fma is
             // monotone, so every reached voxel's |p-v|^2 >= db2).  Same per lane.
             uint32_t lmax = 0;
-            if (lane_ok) {
+            float qbound[4];   // HALVES && QUART: the bound of each 2-voxel z quarter alone
+            uint32_t wmax;
+            if constexpr (HALVES && QUART) {
+                // per quarter: the max over the tile of its voxels' minima (a
+                // quarter far from the tile's worst voxels prunes harder)
+                uint32_t qm[4];
 #pragma unroll
-                for (int z = 0; z < 8; z++)
-                    if (z < td) lmax = max(lmax, m[z]);
+                for (int q = 0; q < 4; q++) qm[q] = lane_ok ? max(m[2 * q], m[2 * q + 1]) : 0u;
+                lmax = max(max(qm[0], qm[1]), max(qm[2], qm[3]));
+                wmax = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t wq = wave_max_u32(qm[q]);
+                    wmax = max(wmax, wq);
+                    qbound[q] = fminf(cut2, __uint_as_float(wq));
+                }
+            } else {
+                if (lane_ok) {
+#pragma unroll
+                    for (int z = 0; z < 8; z++)
+                        if (z < td) lmax = max(lmax, m[z]);
+                }
+                wmax = wave_max_u32(lmax);
             }
-            const uint32_t wmax = wave_max_u32(lmax);
             const float bound = fminf(cut2, __uint_as_float(wmax));
             const float lbound = fminf(cut2, __uint_as_float(lmax));
             // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
@@ -930,7 +948,7 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
 #pragma unroll
                             for (int q = 0; q < 4; q++) {
                                 const float eq = fmaxf(fmaxf((float)(T0[a] + 2 * q) - pw[a], pw[a] - (float)(T0[a] + 2 * q + 1)), 0.0f);
-                                const bool in = 2 * q + 1 >= lr && 2 * q <= hr && !(__builtin_fmaf(eq, eq, db2xy) > bound);
+                                const bool in = 2 * q + 1 >= lr && 2 * q <= hr && !(__builtin_fmaf(eq, eq, db2xy) > qbound[q]);
                                 qm |= in ? 1u << q : 0u;
                             }
                             ok = ok && qm != 0;
