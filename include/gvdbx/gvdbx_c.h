@@ -187,7 +187,9 @@ int gvdbx_gather_level_set(gvdbx_volume* v, int subcell_size, int num_pnts, floa
 /* GatherLevelSet that allocates nothing (no reference counterpart): for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
- * the same result bit for bit as gvdbx_gather_level_set; its only scratch,
+ * the same interior voxels bit for bit as gvdbx_gather_level_set (which on
+ * 8^3 bricks also fills the neighbours' aprons; this one leaves them to
+ * gvdbx_update_apron, after which the bricks agree); its only scratch,
  * one 32-bit word per binned point, is a static device array of
  * GVDBX_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until
  * the process ends: more than GVDBX_NOALLOC_SCRATCH_BYTES / 4 points (the

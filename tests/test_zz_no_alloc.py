@@ -1,7 +1,8 @@
 """gvdbx_gather_level_set_noalloc (the GatherLevelSet that allocates nothing)
-against gvdbx_gather_level_set through the C-ABI: the same bits on two calls
-in a row, the second on a smaller point set, fresh and accumulated, and a
-clear error beyond the scratch limit."""
+against gvdbx_gather_level_set through the C-ABI: the same interior bits,
+and the same bricks after UpdateApron, on two calls in a row, the second on
+a smaller point set, fresh and accumulated, and a clear error beyond the
+scratch limit."""
 import ctypes as C
 import sys
 
@@ -45,12 +46,19 @@ def test_noalloc_gather_matches_gather():
         gv.FillChannel(0, 2.0)
         gv.GatherLevelSet(4, n, 1.5, (0, 0, 0), ln, 0, -1, accum)
         ref = gv.bricks(0)
+        gv.UpdateApron(0, 3.0)
+        ref_apron = gv.bricks(0)
         gv.FillChannel(0, 2.0)
         code = L.gvdbx_gather_level_set_noalloc(gv._h, 4, n, C.c_float(1.5), C.c_float(0), C.c_float(0),
                                                  C.c_float(0), C.c_int64(ln), 0, -1, int(accum))
         assert code == 0, G.last_error()
         got = gv.bricks(0)
-        np.testing.assert_array_equal(got, ref)
+        # the gather defines the interior voxels (the default one also fills
+        # the neighbours' aprons; the allocation-free one leaves them to
+        # UpdateApron), and after UpdateApron the whole bricks agree
+        np.testing.assert_array_equal(got[:, 1:9, 1:9, 1:9], ref[:, 1:9, 1:9, 1:9])
+        gv.UpdateApron(0, 3.0)
+        np.testing.assert_array_equal(gv.bricks(0), ref_apron)
         assert (ref[:, 1:9, 1:9, 1:9] < 3.0).any()
         gv.close()
 
