@@ -43,6 +43,13 @@ struct BinSortIn {
 __device__ __forceinline__ uint32_t rec_key(const float4& r, const BinSortIn& in) {
     if (!(r.x >= 0.0f)) return in.none;
     const int cx = gvdbx_cell(r.x, in.logr0), cy = gvdbx_cell(r.y, in.logr0), cz = gvdbx_cell(r.z, in.logr0);
+    if (!in.gb.sparse()) {
+        // a dense grid holds at most 2^28 cells: 32-bit index arithmetic is
+        // exact, and one unsigned compare per axis covers both box faces
+        const uint32_t ux = (uint32_t)(cx - in.gb.ox), uy = (uint32_t)(cy - in.gb.oy), uz = (uint32_t)(cz - in.gb.oz);
+        if (ux >= (uint32_t)in.gb.nx || uy >= (uint32_t)in.gb.ny || uz >= (uint32_t)in.gb.nz) return in.none;
+        return (ux * (uint32_t)in.gb.ny + uy) * (uint32_t)in.gb.nz + uz;
+    }
     const int64_t b = in.gb.find(cx, cy, cz);
     return b >= 0 ? (uint32_t)b : in.none;
 }
