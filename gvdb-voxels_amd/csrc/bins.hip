@@ -46,7 +46,8 @@ __device__ __forceinline__ uint32_t rec_key(const float4& r, const BinSortIn& in
     if (!in.gb.sparse()) {
         // a dense grid holds at most 2^28 cells: 32-bit index arithmetic is
         // exact, and one unsigned compare per axis covers both box faces
-        const uint32_t ux = (uint32_t)(cx - in.gb.ox), uy = (uint32_t)(cy - in.gb.oy), uz = (uint32_t)(cz - in.gb.oz);
+        const uint32_t ux = (uint32_t)cx - (uint32_t)in.gb.ox, uy = (uint32_t)cy - (uint32_t)in.gb.oy,
+                       uz = (uint32_t)cz - (uint32_t)in.gb.oz;
         if (ux >= (uint32_t)in.gb.nx || uy >= (uint32_t)in.gb.ny || uz >= (uint32_t)in.gb.nz) return in.none;
         return (ux * (uint32_t)in.gb.ny + uy) * (uint32_t)in.gb.nz + uz;
     }

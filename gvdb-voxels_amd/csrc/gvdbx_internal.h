@@ -70,8 +70,15 @@ struct CellGrid {
     }
     // index of cell (x,y,z), -1 when the grid does not hold it
     __device__ __forceinline__ int64_t find(int x, int y, int z) const {
+        if (!ht) {
+            // dense grids hold at most 2^28 cells (larger boxes take the sparse
+            // form), so 32-bit index arithmetic is exact; one unsigned compare
+            // per axis covers both faces of the box
+            const uint32_t ux = (uint32_t)x - (uint32_t)ox, uy = (uint32_t)y - (uint32_t)oy, uz = (uint32_t)z - (uint32_t)oz;
+            if (ux >= (uint32_t)nx || uy >= (uint32_t)ny || uz >= (uint32_t)nz) return -1;
+            return (int64_t)((ux * (uint32_t)ny + uy) * (uint32_t)nz + uz);
+        }
         if (!in_box(x, y, z)) return -1;
-        if (!ht) return dense_index(x, y, z);
         const uint64_t k = pack(x, y, z);
         for (uint32_t s = slot_of(k) & hmask;; s = (s + 1) & hmask) {
             const uint4 e = ht[s];
