@@ -89,10 +89,11 @@ struct CellGrid {
     }
     // cell of index i (0 <= i < count())
     __device__ __forceinline__ void cell(int64_t i, int& x, int& y, int& z) const {
-        if (!ht) {
-            z = (int)(i % nz) + oz;
-            y = (int)((i / nz) % ny) + oy;
-            x = (int)(i / ((int64_t)nz * ny)) + ox;
+        if (!ht) {   // i < 2^28 (dense grids): 32-bit divisions
+            const uint32_t u = (uint32_t)i, q = u / (uint32_t)nz;
+            z = (int)(u - q * (uint32_t)nz) + oz;
+            y = (int)(q % (uint32_t)ny) + oy;
+            x = (int)(q / (uint32_t)ny) + ox;
             return;
         }
         unpack(keys[i], x, y, z);
