@@ -975,6 +975,12 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 if (ok) cm = column_mask(rlo[0], rhi[0], rlo[1], rhi[1], td);
             }
             uint64_t mask = __ballot(ok);
+            // Below a bound of R^2 the column masks are implied: a column
+            // outside a candidate's range along x or y is a non-member, more
+            // than R from it along that axis (its |p-v|^2 >= fl(R*R) > every
+            // lane bound, so the test below drops it), or beyond the cutoff
+            // (which leaves the stored value as it is).
+            const bool use_cols = !(bound < R * R);
             while (mask) {
                 const int b = __builtin_ctzll(mask);
                 mask &= mask - 1;
@@ -982,8 +988,10 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                 const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.y), b));
                 const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.z), b));
                 const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pk, b);
-                const uint64_t cols = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cm, b) |
-                                      ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cm >> 32), b) << 32);
+                uint64_t cols = ~0ull;
+                if (use_cols)
+                    cols = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)cm, b) |
+                           ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cm >> 32), b) << 32);
                 const int zlo = q & 7, zhi = (q >> 3) & 7;
                 // the lanes whose column the candidate reaches, straight from the mask
                 if (__builtin_amdgcn_inverse_ballot_w64(cols)) {
