@@ -901,6 +901,12 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
             }
             const float bound = fminf(cut2, __uint_as_float(wmax));
             const float lbound = fminf(cut2, __uint_as_float(lmax));
+            // Below a bound of R^2 the column masks are implied: a column
+            // outside a candidate's range along x or y is a non-member, more
+            // than R from it along that axis (its |p-v|^2 >= fl(R*R) > every
+            // lane bound, so the lane test of the walk drops it), or beyond
+            // the cutoff (which leaves the stored value as it is).
+            const bool use_cols = !(bound < R * R);
             // Filter this chunk.  Membership = subcell box (cuda_gvdb_particles.cuh:237-242)
             // as a voxel range; a candidate must also come within the cutoff of the tile.
             // The packed range is the membership range clipped to the tile (z) and to
@@ -972,15 +978,9 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                     }
                 }
                 ok = ok && db2 <= bound;
-                if (ok) cm = column_mask(rlo[0], rhi[0], rlo[1], rhi[1], td);
+                if (ok && use_cols) cm = column_mask(rlo[0], rhi[0], rlo[1], rhi[1], td);
             }
             uint64_t mask = __ballot(ok);
-            // Below a bound of R^2 the column masks are implied: a column
-            // outside a candidate's range along x or y is a non-member, more
-            // than R from it along that axis (its |p-v|^2 >= fl(R*R) > every
-            // lane bound, so the test below drops it), or beyond the cutoff
-            // (which leaves the stored value as it is).
-            const bool use_cols = !(bound < R * R);
             while (mask) {
                 const int b = __builtin_ctzll(mask);
                 mask &= mask - 1;
