@@ -63,8 +63,8 @@ __global__ __launch_bounds__(256) void k_gather_color(ColorGatherParams cp) {
 #pragma unroll
                 for (int t = 0; t < 3; t++) {
                     // subcell indices of the leaf inside the box
-                    const int blo = gvdbx_cdiv((int)(pw[t] - cp.R), cp.sc_shift) * sc;
-                    const int bhi = gvdbx_cdiv((int)(pw[t] + 1 + cp.R), cp.sc_shift) * sc;
+                    const int blo = gvdbx_cdiv_mul((int)(pw[t] - cp.R), cp.sc_shift);
+                    const int bhi = gvdbx_cdiv_mul((int)(pw[t] + 1 + cp.R), cp.sc_shift);
                     lo[t] = max(blo - nd.mPos[t], 0) >> cp.sc_shift;
                     const int h = bhi - nd.mPos[t];
                     hi[t] = h < 0 ? -1 : min(h >> cp.sc_shift, nsd - 1);

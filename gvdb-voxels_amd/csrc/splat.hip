@@ -118,8 +118,8 @@ __global__ __launch_bounds__(64) void k_gather(GatherParams gp, int res0, int ti
 #pragma unroll
                         for (int a = 0; a < 3; a++) {
                             // subcell box (cuda_gvdb_particles.cuh:237-242) as a voxel range
-                            int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
-                            int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                            int lo = gvdbx_cdiv_mul((int)(pw[a] - R), scs);
+                            int hi = gvdbx_cdiv_mul((int)(pw[a] + 1 + R), scs) + sc - 1;
                             lo = max(lo, (int)ceilf(pw[a] - cut));
                             hi = min(hi, (int)floorf(pw[a] + cut));
                             lo_r[a] = max(lo - T0[a], 0);
@@ -247,8 +247,8 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
 #pragma unroll
                         for (int a = 0; a < 3; a++) {
                             // subcell box (cuda_gvdb_particles.cuh:237-242) as a voxel range
-                            int lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
-                            int hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                            int lo = gvdbx_cdiv_mul((int)(pw[a] - R), scs);
+                            int hi = gvdbx_cdiv_mul((int)(pw[a] + 1 + R), scs) + sc - 1;
                             lo = max(lo, (int)ceilf(pw[a] - cut));
                             hi = min(hi, (int)floorf(pw[a] + cut));
                             lo_r[a] = max(lo - T0[a], 0);
@@ -302,8 +302,8 @@ __global__ __launch_bounds__(64) void k_gather_pc(GatherParams gp, int res0, int
 // Subcell membership range of a point along one axis clipped to the cutoff
 // (cuda_gvdb_particles.cuh:237-242), the arithmetic of the gather's filter.
 __device__ __forceinline__ void gather_range(float p, float R, int sc, int scs, float cut, int& lo_c, int& hi_c) {
-    const int lo = gvdbx_cdiv((int)(p - R), scs) * sc;
-    const int hi = gvdbx_cdiv((int)(p + 1 + R), scs) * sc + sc - 1;
+    const int lo = gvdbx_cdiv_mul((int)(p - R), scs);
+    const int hi = gvdbx_cdiv_mul((int)(p + 1 + R), scs) + sc - 1;
     lo_c = max(lo, (int)ceilf(p - cut));
     hi_c = min(hi, (int)floorf(p + cut));
 }
@@ -321,8 +321,8 @@ __device__ __forceinline__ uint32_t gather_box_word2(const float (&pm)[3], const
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         // membership alone (gather_range's box), then the cutoff from pc
-        const int lo = gvdbx_cdiv((int)(pm[a] - R), scs) * sc;
-        const int hi = gvdbx_cdiv((int)(pm[a] + 1 + R), scs) * sc + sc - 1;
+        const int lo = gvdbx_cdiv_mul((int)(pm[a] - R), scs);
+        const int hi = gvdbx_cdiv_mul((int)(pm[a] + 1 + R), scs) + sc - 1;
         lo_c[a] = max(lo, (int)ceilf(pc[a] - cut));
         hi_c[a] = min(hi, (int)floorf(pc[a] + cut));
         const int f = (int)floorf(pc[a]);
@@ -932,8 +932,8 @@ __device__ __forceinline__ void gather_walk(const GatherParams& gp, const GTile&
                         hi_c = f + (int)((curb >> (10 * a + 5)) & 31u);
                         lo = lo_c, hi = hi_c;
                     } else {
-                        lo = gvdbx_cdiv((int)(pw[a] - R), scs) * sc;
-                        hi = gvdbx_cdiv((int)(pw[a] + 1 + R), scs) * sc + sc - 1;
+                        lo = gvdbx_cdiv_mul((int)(pw[a] - R), scs);
+                        hi = gvdbx_cdiv_mul((int)(pw[a] + 1 + R), scs) + sc - 1;
                         lo_c = max(lo, (int)ceilf(pw[a] - cut));
                         hi_c = min(hi, (int)floorf(pw[a] + cut));
                     }

@@ -559,8 +559,8 @@ __device__ __forceinline__ unsigned long long subcell_count(const float4& p, con
     bool near = true;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        lo[a] = gvdbx_cdiv((int)(w[a] - bp.radius), bp.sc_shift) * sc;
-        hi[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), bp.sc_shift) * sc;
+        lo[a] = gvdbx_cdiv_mul((int)(w[a] - bp.radius), bp.sc_shift);
+        hi[a] = gvdbx_cdiv_mul((int)(w[a] + 1 + bp.radius), bp.sc_shift);
         c0[a] = max(lo[a], 0) >> bp.logr0;
         c1[a] = hi[a] >> bp.logr0;   // hi < 0: empty range (c1 < c0 or no subcell counted)
         hc[a] = gvdbx_cell(w[a], bp.logr0);
@@ -594,7 +594,7 @@ __device__ __forceinline__ unsigned long long subcell_count(const float4& p, con
             for (int dy = 0; dy < 3; dy++) {
                 // the row's three leaf bits as bytes, dotted with the x counts
                 const uint32_t bits = (m >> (dz * 9 + dy * 3)) & 7u;
-                const uint32_t row = __builtin_amdgcn_udot4(nx, (bits * 0x4081u) & 0x010101u, 0u, false);
+                const uint32_t row = __builtin_amdgcn_udot4(nx, (bits | (bits << 7) | (bits << 14)) & 0x010101u, 0u, false);
                 c32 += __umul24(row, __umul24(n[1][dy], n[2][dz]));
             }
         return c32;
