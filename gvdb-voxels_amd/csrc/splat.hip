@@ -363,11 +363,12 @@ __global__ __launch_bounds__(256) void k_gather_box(const float4* sorted, const 
 // of its bin (level-0 cell).  A tile in cell c needs, from the bin at c + o,
 // only the classes that can reach c: bit a set where o_a = -1, bit a clear
 // where o_a = +1 (unless a point of that bin reaches past both faces along
-// a), anything where o_a = 0.  Each bin's points are reordered by class
-// (stably) into csorted as (x, y, z, packed ranges), the class starts go to
-// coff[8 ci + k] (coff[8 nbin] = the total) and the axes with points
-// reaching past both faces to both[ci].  The order inside a bin does not
-// change a gather: its result is a minimum.
+// a), anything where o_a = 0.  Each bin's points are grouped by class into
+// csorted as (x, y, z, packed ranges), the class starts go to coff[8 ci + k]
+// (coff[8 nbin] = the total) and the axes with points reaching past both
+// faces to both[ci].  The order inside a bin does not change a gather: its
+// result is a minimum (bins of at most 256 entries are grouped with LDS
+// atomics, in no fixed order inside a class; larger ones stably).
 constexpr int CLS_UNROLL = 4;   // chunks of 64 entries loaded at once
 
 // Entries j0 + 64 u + lane (u < CLS_UNROLL) of a bin ending at `end`: the
@@ -470,6 +471,7 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
                                                        int2* units, int4* bigs) {
     const int lane = threadIdx.x;
     const int64_t nbin = gb.count();
+    __shared__ int32_t lcnt[8];
     for (int64_t ci = blockIdx.x; ci < nbin; ci += gridDim.x) {
         const int beg = off[ci], end = off[ci + 1];
         if (end - beg > CLS_SPAN) {
@@ -489,9 +491,20 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
         int base[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) base[k] = 0;
+        // counted and placed with LDS atomics: the order inside a class is
+        // not kept (the walks take minima, which no order changes)
+        const int nu = class_chunks(beg, end);
         if (end > beg) {
             class_load(sorted, beg, end, logr0, R, sc, scs, cut, qz, q, cls, bb);
-            class_count(cls, class_chunks(beg, end), base);
+            if (lane < 8) lcnt[lane] = 0;
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < CLS_UNROLL; u++)
+                if (u < nu && cls[u] >= 0) atomicAdd(&lcnt[cls[u]], 1);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 8; k++) base[k] = lcnt[k];
+            __syncthreads();
         }
         int acc = beg;
 #pragma unroll
@@ -503,7 +516,17 @@ __global__ __launch_bounds__(64) void k_gather_classes(const float4* sorted, con
         }
         const uint32_t bb_all = wave_or3(bb);
         if (lane == 0) both[ci] = (uint8_t)bb_all;
-        if (end > beg) class_place(q, cls, class_chunks(beg, end), base, csorted);
+        if (end > beg) {
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) lcnt[k] = base[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < CLS_UNROLL; u++)
+                if (u < nu && cls[u] >= 0) csorted[atomicAdd(&lcnt[cls[u]], 1)] = q[u];
+            __syncthreads();
+        }
     }
     if (blockIdx.x == 0 && lane == 0) coff[nbin * 8] = off[nbin];
 }
