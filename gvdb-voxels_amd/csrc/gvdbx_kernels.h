@@ -421,10 +421,11 @@ struct VisitRec {
     float4* rec;         // (t handed to the brick function, T and I at entry, ray id bits)
     uint32_t* count;     // count[0]: records, count[1]: chunks, one 8-byte-aligned pair (may exceed the caps: then rerun)
     int64_t cap;
-    uint64_t* chunk_tag;
+    uint64_t* chunk_tag;   // (global wave index << 32) | records the wave wrote in earlier chunks
     uint32_t* chunk_base;
     uint32_t* chunk_size;
     int64_t chunk_cap;
+    uint32_t* wave_recs;   // per global wave: records written (zero-filled by the host)
 };
 void launch_render(int shade, RenderTree tv, RenderCam cam, RenderScene sc, float4* rgba, uchar4* rgba8,
                    hipStream_t s);
@@ -456,13 +457,12 @@ struct EABackParams {
 };
 // per leaf [begin, end) of the leaf-sorted keys
 void launch_segments(const uint32_t* key, int64_t n, int32_t* seg, int64_t nleaf, hipStream_t s);
-// chunk i of the tag-sorted order (original chunk order[i]) is copied to
-// dst_off[i]: key_out = its records' leaves, val_out = their record indices
-void launch_chunk_order(const uint32_t* order, int64_t nchunk, const uint32_t* chunk_base, const uint32_t* chunk_size,
-                        const int32_t* dst_off, const uint32_t* key, uint32_t* key_out, uint32_t* val_out,
+// chunk c (tag[c] = wave << 32 | records before it in its wave) is copied to
+// wave_off[wave] + that offset: key_out = its records' leaves, val_out =
+// their record indices
+void launch_chunk_order(const uint64_t* tag, int64_t nchunk, const uint32_t* chunk_base, const uint32_t* chunk_size,
+                        const int32_t* wave_off, const uint32_t* key, uint32_t* key_out, uint32_t* val_out,
                         hipStream_t s);
-// out[i] = size[order[i]]
-void launch_gather_sizes(const uint32_t* order, int64_t n, const uint32_t* size, int32_t* out, hipStream_t s);
 void launch_ea_backward(EABackParams p, hipStream_t s);
 
 struct SplatBackParams {

@@ -468,14 +468,17 @@ __device__ __forceinline__ void brick_fn(const RenderTree& tv, const RenderScene
 // of up to VIS_BUF, so a wave takes one slot range (one global atomic) per
 // chunk instead of one per march step.  Where a chunk lands depends on which
 // wave flushes first, so each chunk is also described by a tag (global wave
-// index, the wave's chunk number): the backward orders chunks by tag, which
-// makes the order of the records, and of every sum over them, fixed.
+// index, records the wave wrote before it), and every wave leaves its record
+// total in wave_recs: the backward places chunk c at (exclusive scan of
+// wave_recs)[wave] + tag's record offset, which lays the records out in
+// (wave, march) order whichever wave wrote first, and so fixes the order of
+// every sum over them.
 constexpr int VIS_BUF = 448;   // ~9 KB per wave, 4 waves per block: 4 blocks fit a CU, as many as the registers allow
 struct VisitLds {
     uint32_t key[VIS_BUF];
     float4 rec[VIS_BUF];
     int cnt;
-    int seq;       // chunks written so far
+    int pre;       // records written so far in earlier chunks
     uint32_t wave; // global wave index (chunk tags)
 };
 
@@ -493,13 +496,13 @@ __device__ __forceinline__ void flush_visits(VisitLds& vl, const VisitRec& vr, i
             atomicAdd((unsigned long long*)vr.count, (unsigned long long)cnt | (1ull << 32));
         base = (uint32_t)old;
         const uint32_t cid = (uint32_t)(old >> 32);
-        const int seq = __hip_atomic_load(&vl.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const int pre = __hip_atomic_load(&vl.pre, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         if ((int64_t)cid < vr.chunk_cap) {
-            vr.chunk_tag[cid] = ((uint64_t)vl.wave << 32) | (uint32_t)seq;
+            vr.chunk_tag[cid] = ((uint64_t)vl.wave << 32) | (uint32_t)pre;
             vr.chunk_base[cid] = base;
             vr.chunk_size[cid] = (uint32_t)cnt;
         }
-        __hip_atomic_store(&vl.seq, seq + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_store(&vl.pre, pre + cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
     base = __builtin_amdgcn_readlane(base, leader);
     __builtin_amdgcn_wave_barrier();
@@ -873,7 +876,7 @@ __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam
     VisitLds& vl = vbuf[RECORD ? threadIdx.x / 64 : 0];
     if (RECORD && __lane_id() == 0) {
         set_visit_count(vl, 0);
-        __hip_atomic_store(&vl.seq, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __hip_atomic_store(&vl.pre, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         vl.wave = (uint32_t)((((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 4 +
                              threadIdx.x / 64);
     }
@@ -893,7 +896,11 @@ __global__ __launch_bounds__(256) void k_raycast_ea(RenderTree tv, RenderCam cam
     T[ray] = st.clr[3];
     if (RECORD) {
         const int cnt = visit_count(vl);
-        if (cnt > 0) flush_visits(vl, vr, cnt, __ballot(1));
+        const uint64_t act = __ballot(1);
+        if (cnt > 0) flush_visits(vl, vr, cnt, act);
+        // waves whose lanes all left early keep the zero the host filled in
+        if (__lane_id() == __ffsll((unsigned long long)act) - 1)
+            vr.wave_recs[vl.wave] = (uint32_t)__hip_atomic_load(&vl.pre, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
 }
 
@@ -907,25 +914,21 @@ __global__ void k_segments(const uint32_t* key, int64_t n, int32_t* seg, int64_t
     }
 }
 
-// One wave per chunk, in tag order: its records go to consecutive places.
-__global__ __launch_bounds__(64) void k_chunk_order(const uint32_t* order, int64_t nchunk, const uint32_t* chunk_base,
-                                                    const uint32_t* chunk_size, const int32_t* dst_off,
+// One wave per chunk: its records go to consecutive places from its wave's
+// start (wave_off, the exclusive scan of the waves' record totals) plus the
+// records its wave wrote before it (the tag's low word).
+__global__ __launch_bounds__(64) void k_chunk_order(const uint64_t* tag, int64_t nchunk, const uint32_t* chunk_base,
+                                                    const uint32_t* chunk_size, const int32_t* wave_off,
                                                     const uint32_t* key, uint32_t* key_out, uint32_t* val_out) {
-    const int64_t i = blockIdx.x;
-    if (i >= nchunk) return;
-    const uint32_t c = order[i];
+    const int64_t c = blockIdx.x;
+    if (c >= nchunk) return;
+    const uint64_t t = tag[c];
     const uint32_t base = chunk_base[c], size = chunk_size[c];
-    const int64_t dst = dst_off[i];
+    const int64_t dst = (int64_t)(uint32_t)wave_off[t >> 32] + (uint32_t)t;
     for (uint32_t j = threadIdx.x; j < size; j += 64) {
         key_out[dst + j] = key[base + j];
         val_out[dst + j] = base + j;
     }
-}
-
-__global__ __launch_bounds__(256) void k_gather_sizes(const uint32_t* order, int64_t n, const uint32_t* size,
-                                                      int32_t* out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (int32_t)size[order[i]];
 }
 
 // Move the four corner sums a cell shares with its neighbour along axis A:
@@ -1349,16 +1352,12 @@ void launch_segments(const uint32_t* key, int64_t n, int32_t* seg, int64_t nleaf
     if (n > 0) hipLaunchKernelGGL(k_segments, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, seg, nleaf);
 }
 
-void launch_chunk_order(const uint32_t* order, int64_t nchunk, const uint32_t* chunk_base, const uint32_t* chunk_size,
-                        const int32_t* dst_off, const uint32_t* key, uint32_t* key_out, uint32_t* val_out,
+void launch_chunk_order(const uint64_t* tag, int64_t nchunk, const uint32_t* chunk_base, const uint32_t* chunk_size,
+                        const int32_t* wave_off, const uint32_t* key, uint32_t* key_out, uint32_t* val_out,
                         hipStream_t s) {
     if (nchunk > 0)
-        hipLaunchKernelGGL(k_chunk_order, dim3((unsigned)nchunk), dim3(64), 0, s, order, nchunk, chunk_base, chunk_size,
-                           dst_off, key, key_out, val_out);
-}
-
-void launch_gather_sizes(const uint32_t* order, int64_t n, const uint32_t* size, int32_t* out, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_gather_sizes, dim3(grid_for(n, 256)), dim3(256), 0, s, order, n, size, out);
+        hipLaunchKernelGGL(k_chunk_order, dim3((unsigned)nchunk), dim3(64), 0, s, tag, nchunk, chunk_base, chunk_size,
+                           wave_off, key, key_out, val_out);
 }
 
 void launch_ea_backward(EABackParams p, hipStream_t s) {

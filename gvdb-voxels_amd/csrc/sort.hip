@@ -51,9 +51,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const K* keys, int64_t n
         const int64_t i = t0 + r * RS_THREADS + threadIdx.x;
         kr[r] = i < n ? keys[i] : (K)0;
     }
+    // Keys often come in runs (segments of neighbouring rays, points near in
+    // index): lanes holding equal digits would all hit one LDS word, and
+    // such atomics serialize.  Each run of equal digits over consecutive
+    // lanes adds its length once, from its first lane.
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int r = 0; r < ITEMS; r++)
-        if (t0 + r * RS_THREADS + threadIdx.x < n) atomicAdd(&cnt[(uint32_t)(kr[r] >> shift) & mask], 1);
+    for (int r = 0; r < ITEMS; r++) {
+        const bool ok = t0 + r * RS_THREADS + threadIdx.x < n;
+        const uint32_t d = (uint32_t)(kr[r] >> shift) & mask;
+        const uint32_t prev = (uint32_t)__shfl_up((int)d, 1);
+        const uint64_t okm = __ballot(ok);
+        const bool head = ok && (lane == 0 || d != prev);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+            // the run ends at the next head or at the last valid lane (valid
+            // lanes are a prefix of the wave)
+            const uint64_t after = heads & ~((2ull << lane) - 1ull);
+            const int end = after ? __ffsll((unsigned long long)after) - 1 : __popcll(okm);
+            atomicAdd(&cnt[d], end - lane);
+        }
+    }
     __syncthreads();
     for (int b = threadIdx.x; b < B; b += RS_THREADS) hist[(int64_t)b * ntiles + blockIdx.x] = cnt[b];
 }

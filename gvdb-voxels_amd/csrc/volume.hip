@@ -206,9 +206,9 @@ struct gvdbx_volume {
     int64_t npnode = 0;
     // recorded emission-absorption segments (gvdbx_raycast_ea with record)
     DevBuf vis_key, vis_rec, vis_idx, vis_tkey, vis_tidx, vis_hist, vis_off, vis_part, vis_seg, vis_cnt;
-    DevBuf vis_lkey, vis_ctag, vis_ctag2, vis_cbase, vis_csize, vis_cidx, vis_cidx2, vis_coff;   // checkpoint chunks
+    DevBuf vis_lkey, vis_ctag, vis_cbase, vis_csize, vis_coff;   // checkpoint chunks
     int64_t vis_nchunk = 0, vis_ccap = 0;
-    int vis_wave_bits = 0;
+    int64_t vis_nwaves = 0;   // waves of the recorded launch (their record totals follow vis_cnt's counters)
     int64_t vis_n = 0, vis_cap = 0;
     int vis_leaf_bits = 0;
     bool vis_valid = false, vis_sorted = false, vis_in_tmp = false;
@@ -1654,8 +1654,8 @@ int gvdbx_destroy(gvdbx_volume* v) {
                       &v->splat_src, &v->gbox, &v->bin_leaf, &v->gcls_pts, &v->gcoff, &v->gcboth, &v->gcctr, &v->gcunit, &v->gcbig, &v->gcucnt, &v->gs_items, &v->gs_counts, &v->gs_slots, &v->gs_acc,
                       &v->sp_runs, &v->sp_runoff, &v->sp_runcell, &v->sp_part, &v->sp_scan,
                       &v->c_ptoff, &v->c_celloff, &v->c_boxes, &v->c_tiles, &v->c_cellcloud, &v->c_leafcloud,
-                      &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag, &v->vis_ctag2,
-                      &v->vis_cbase, &v->vis_csize, &v->vis_cidx, &v->vis_cidx2, &v->vis_coff};
+                      &v->c_bbox, &v->c_bounds, &v->c_cams, &v->vis_lkey, &v->vis_ctag,
+                      &v->vis_cbase, &v->vis_csize, &v->vis_coff};
     for (DevBuf* b : bufs) free_buf(*b);
     if (v->host) (void)hipHostFree(v->host);
     if (v->pin) (void)hipHostFree(v->pin);
@@ -2376,12 +2376,14 @@ int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bo
     if (rays >= ((int64_t)1 << 32)) FAIL(GVDBX_ERR_RANGE, "%s: too many rays", what);
     const int lb = bits_for(v->cnt[0]);
     // one checkpoint chunk per flush of a wave's buffer: every chunk but a
-    // wave's last holds more than VIS_BUF/2 = 64 records
+    // wave's last holds more than 64 records
     const int64_t nwaves = (int64_t)((rc.width + 15) / 16) * ((rc.height + 15) / 16) * (batched ? ncam : 1) * 4;
     if (nwaves >= ((int64_t)1 << 32)) FAIL(GVDBX_ERR_RANGE, "%s: too many rays", what);
     v->vis_valid = false;
     int64_t cap = std::max<int64_t>(v->vis_cap, std::max<int64_t>(rays * 8, 1 << 20));
-    TRY(ensure(v, v->vis_cnt, 16));
+    // vis_cnt: record and chunk counters (8 B), padding, then one record
+    // total per wave
+    TRY(ensure(v, v->vis_cnt, (size_t)(4 + nwaves) * 4));
     TRY(ensure_host(v, 4096));
     for (int attempt = 0; attempt < 2; attempt++) {
         if (cap >= ((int64_t)1 << 31)) FAIL(GVDBX_ERR_RANGE, "%s: more than 2^31 segment checkpoints", what);
@@ -2391,7 +2393,7 @@ int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bo
         TRY(ensure(v, v->vis_ctag, (size_t)ccap * 8));
         TRY(ensure(v, v->vis_cbase, (size_t)ccap * 4));
         TRY(ensure(v, v->vis_csize, (size_t)ccap * 4));
-        launch_fill_i32(v->vis_cnt.as<int32_t>(), 2, 0, s);
+        launch_fill_i32(v->vis_cnt.as<int32_t>(), 4 + nwaves, 0, s);
         vr.key = v->vis_key.as<uint32_t>();
         vr.rec = v->vis_rec.as<float4>();
         vr.count = v->vis_cnt.as<uint32_t>();
@@ -2400,6 +2402,7 @@ int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bo
         vr.chunk_base = v->vis_cbase.as<uint32_t>();
         vr.chunk_size = v->vis_csize.as<uint32_t>();
         vr.chunk_cap = ccap;
+        vr.wave_recs = v->vis_cnt.as<uint32_t>() + 4;
         launch_raycast_ea(tv, rc, dcams, ncam, sc, I_dev, T_dev, 1, vr, s);
         TRY(check_launch(what));
         GVDBX_CHECK(hipMemcpyAsync(v->host, v->vis_cnt.p, 8, hipMemcpyDeviceToHost, s));
@@ -2411,7 +2414,7 @@ int raycast_ea(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int ncam, bo
             v->vis_cap = cap;
             v->vis_ccap = ccap;
             v->vis_leaf_bits = lb;
-            v->vis_wave_bits = bits_for(nwaves);
+            v->vis_nwaves = nwaves;
             v->vis_w = rc.width;
             v->vis_h = rc.height;
             v->vis_ncam = batched ? ncam : 0;
@@ -2449,37 +2452,29 @@ int raycast_ea_backward(gvdbx_volume* v, int chan, const gvdbx_camera* cams, int
     hipStream_t s = v->stream;
     const int64_t n = v->vis_n, nleaf = v->cnt[0];
     if (!v->vis_sorted) {
-        // Checkpoints grouped by leaf in an order fixed by the data: chunks
-        // ordered by their tag (wave, sequence), records in chunk order, then
-        // a stable sort by leaf.  Within a leaf the segments are then in
-        // (wave, chunk, position) order whichever wave wrote first.
-        const int64_t m = v->vis_nchunk;
-        const int64_t nh = radix_hist_entries(std::max(n, m));
-        TRY(ensure(v, v->vis_ctag2, (size_t)std::max<int64_t>(m, 1) * 8));
-        TRY(ensure(v, v->vis_cidx, (size_t)std::max<int64_t>(m, 1) * 4));
-        TRY(ensure(v, v->vis_cidx2, (size_t)std::max<int64_t>(m, 1) * 4));
-        TRY(ensure(v, v->vis_coff, (size_t)(m + 1) * 4 + 4 * m));
+        // Checkpoints grouped by leaf in an order fixed by the data: records
+        // laid out in (wave, march) order (each chunk placed at its wave's
+        // start, from a scan of the waves' record totals, plus the records
+        // its wave wrote before it), then a stable sort by leaf.  Within a
+        // leaf the segments are then in (wave, march) order whichever wave
+        // wrote first.
+        const int64_t m = v->vis_nchunk, nw = v->vis_nwaves;
+        const int64_t nh = radix_hist_entries(n);
+        TRY(ensure(v, v->vis_coff, (size_t)(nw + 1) * 4));
         TRY(ensure(v, v->vis_lkey, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_idx, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_tkey, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_tidx, (size_t)std::max<int64_t>(n, 1) * 4));
         TRY(ensure(v, v->vis_hist, (size_t)nh * 4));
         TRY(ensure(v, v->vis_off, (size_t)(nh + 1) * 4));
-        TRY(ensure(v, v->vis_part, (size_t)(scan_tiles(std::max(nh, m) + 1) + 1) * 4));
-        const int cflip = radix_sort_pairs(v->vis_ctag.as<uint64_t>(), v->vis_cidx.as<uint32_t>(),
-                                           v->vis_ctag2.as<uint64_t>(), v->vis_cidx2.as<uint32_t>(), m, 0,
-                                           32 + v->vis_wave_bits, v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(),
-                                           v->vis_part.as<uint32_t>(), s, true);   // values: chunk indices
-        const uint32_t* corder = cflip ? v->vis_cidx2.as<uint32_t>() : v->vis_cidx.as<uint32_t>();
-        int32_t* csz = v->vis_coff.as<int32_t>() + (m + 1);
-        int32_t* coff = v->vis_coff.as<int32_t>();
+        TRY(ensure(v, v->vis_part, (size_t)(scan_tiles(std::max(nh, nw) + 1) + 1) * 4));
+        const int32_t* wrec = v->vis_cnt.as<int32_t>() + 4;
+        int32_t* woff = v->vis_coff.as<int32_t>();
         uint32_t* spart = v->vis_part.as<uint32_t>();
-        const int64_t stiles = scan_tiles(m + 1);
-        launch_gather_sizes(corder, m, v->vis_csize.as<uint32_t>(), csz, s);
-        launch_count_scan_reduce(csz, m, spart, spart + stiles, s);
-        launch_count_scan_apply(csz, m, spart, coff, s);
-        launch_chunk_order(corder, m, v->vis_cbase.as<uint32_t>(), v->vis_csize.as<uint32_t>(), coff,
-                           v->vis_key.as<uint32_t>(), v->vis_lkey.as<uint32_t>(), v->vis_idx.as<uint32_t>(), s);
+        launch_count_scan_reduce(wrec, nw, spart, spart + scan_tiles(nw + 1), s);
+        launch_count_scan_apply(wrec, nw, spart, woff, s);
+        launch_chunk_order(v->vis_ctag.as<uint64_t>(), m, v->vis_cbase.as<uint32_t>(), v->vis_csize.as<uint32_t>(),
+                           woff, v->vis_key.as<uint32_t>(), v->vis_lkey.as<uint32_t>(), v->vis_idx.as<uint32_t>(), s);
         const int flip = radix_sort_pairs32(v->vis_lkey.as<uint32_t>(), v->vis_idx.as<uint32_t>(),
                                             v->vis_tkey.as<uint32_t>(), v->vis_tidx.as<uint32_t>(), n, 0,
                                             v->vis_leaf_bits, v->vis_hist.as<int32_t>(), v->vis_off.as<int32_t>(),
