@@ -555,49 +555,76 @@ __device__ __forceinline__ int subcells_in_cell(int lo, int hi, int c, int r0, i
 // box is tested (bit 13 of its own mask: the cell itself).
 __device__ __forceinline__ unsigned long long subcell_count(const float4& p, const BinParams& bp, int r0, int sc) {
     const float w[3] = {p.x, p.y, p.z};
-    int lo[3], hi[3], c0[3], c1[3], hc[3];
+    // In subcell units the box's subcells along axis a are L[a] .. H1[a] - 1
+    // (subcells >= 0 only; empty when H1 <= L) and the home cell's first
+    // subcell is Q[a]: the ends are those of the box [cdiv(w - r) * sc,
+    // cdiv(w + 1 + r) * sc] divided by sc, exactly (both are multiples of sc).
+    const int s = bp.sc_shift, ks = bp.logr0 - s, q = 1 << ks;
+    int L[3], H1[3], Q[3], hc[3];
     bool near = true;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        lo[a] = gvdbx_cdiv_mul((int)(w[a] - bp.radius), bp.sc_shift);
-        hi[a] = gvdbx_cdiv_mul((int)(w[a] + 1 + bp.radius), bp.sc_shift);
-        c0[a] = max(lo[a], 0) >> bp.logr0;
-        c1[a] = hi[a] >> bp.logr0;   // hi < 0: empty range (c1 < c0 or no subcell counted)
+        L[a] = max((int)(w[a] - bp.radius), 0) >> s;
+        H1[a] = gvdbx_cdiv((int)(w[a] + 1 + bp.radius), s) + 1;
         hc[a] = gvdbx_cell(w[a], bp.logr0);
-        near = near && c0[a] >= hc[a] - 1 && c1[a] <= hc[a] + 1;
+        Q[a] = hc[a] * q;
+        near = near && L[a] >= Q[a] - q && H1[a] <= Q[a] + 2 * q;
     }
-    unsigned long long cnt = 0;
     const int64_t hcell = near ? bp.g0.find(hc[0], hc[1], hc[2]) : -1;
     if (hcell >= 0) {
         const uint32_t m = bp.nbmask[hcell];
         // n[a][d]: the box's subcells along axis a inside cell hc[a] + d - 1
-        // (the subcells_in_cell count: box and cell ends are multiples of
-        // sc, so b - a + sc over sc is the count, <= 0 when empty); at most
-        // r0 / sc <= 32 each (bricks are at most 32^3), so they pack into
-        // bytes and every 24-bit product below stays exact
+        // (its overlap with subcells [Q + (d-1) q, Q + d q); the box lies in
+        // these three cells, so the middle one is the rest); at most
+        // q = r0 / sc <= 32 each (bricks are at most 32^3), so they pack
+        // into bytes and every 24-bit product below stays exact
         int n[3][3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const int lo0 = max(lo[a], 0), base = hc[a] << bp.logr0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const int cs = base + (d - 1) * r0;
-                const int first = max(lo0, cs), last = min(hi[a], cs + r0 - sc);
-                n[a][d] = max((last - first + sc) >> bp.sc_shift, 0);
-            }
+            const int n0 = max(min(H1[a], Q[a]) - L[a], 0), n2 = max(H1[a] - max(L[a], Q[a] + q), 0);
+            n[a][0] = n0;
+            n[a][2] = n2;
+            n[a][1] = max(H1[a] - L[a], 0) - n0 - n2;
         }
         const uint32_t nx = (uint32_t)n[0][0] | (uint32_t)n[0][1] << 8 | (uint32_t)n[0][2] << 16;
+        // a row's three leaf bits spread to bytes: b * 0x4081 puts bit k at
+        // bit 8k (no carries for b < 8)
+        auto row_bits = [&](int dy, int dz) {
+            return __umul24((m >> (dz * 9 + dy * 3)) & 7u, 0x4081u) & 0x010101u;
+        };
         uint32_t c32 = 0;
+        if (q <= 15) {
+            // x counts times the y count as bytes (<= 225: no carries), one
+            // dot product per row, one multiply per z layer
+            uint32_t P[3];
 #pragma unroll
-        for (int dz = 0; dz < 3; dz++)
+            for (int dy = 0; dy < 3; dy++) P[dy] = __umul24(nx, (uint32_t)n[1][dy]);
 #pragma unroll
-            for (int dy = 0; dy < 3; dy++) {
-                // the row's three leaf bits as bytes, dotted with the x counts
-                const uint32_t bits = (m >> (dz * 9 + dy * 3)) & 7u;
-                const uint32_t row = __builtin_amdgcn_udot4(nx, (bits | (bits << 7) | (bits << 14)) & 0x010101u, 0u, false);
-                c32 += __umul24(row, __umul24(n[1][dy], n[2][dz]));
+            for (int dz = 0; dz < 3; dz++) {
+                uint32_t sz = 0;
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++) sz = __builtin_amdgcn_udot4(P[dy], row_bits(dy, dz), sz, false);
+                c32 += __umul24(sz, (uint32_t)n[2][dz]);
             }
+        } else {
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++) {
+                    const uint32_t row = __builtin_amdgcn_udot4(nx, row_bits(dy, dz), 0u, false);
+                    c32 += __umul24(row, __umul24(n[1][dy], n[2][dz]));
+                }
+        }
         return c32;
+    }
+    unsigned long long cnt = 0;
+    int lo[3], hi[3], c0[3], c1[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = gvdbx_cdiv_mul((int)(w[a] - bp.radius), s);
+        hi[a] = gvdbx_cdiv_mul((int)(w[a] + 1 + bp.radius), s);
+        c0[a] = max(lo[a], 0) >> bp.logr0;
+        c1[a] = hi[a] >> bp.logr0;   // hi < 0: empty range (c1 < c0 or no subcell counted)
     }
     for (int cx = c0[0]; cx <= c1[0]; cx++) {
         const int nx = subcells_in_cell(lo[0], hi[0], cx, r0, sc, bp.sc_shift);
