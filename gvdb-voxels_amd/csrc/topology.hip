@@ -175,6 +175,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_count_reduce(const int32_t* cn
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
+// Each thread scans SCAN_ITEMS consecutive counts.  VEC (both arrays 16-byte
+// aligned): whole groups of four move as one 16-byte load or store, the
+// groups at the end element by element.
+template <bool VEC>
 __global__ __launch_bounds__(SCAN_THREADS) void k_count_apply(const int32_t* cnt, int64_t n,
                                                                 const uint32_t* partials, int32_t* off) {
     __shared__ uint32_t lds[4];
@@ -182,19 +186,36 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_count_apply(const int32_t* cnt
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        int64_t i = t0 + k;
-        v[k] = i < n ? (uint32_t)cnt[i] : 0u;
-        s += v[k];
+    for (int g = 0; g < SCAN_ITEMS; g += 4) {
+        const int64_t i = t0 + g;
+        if (VEC && i + 3 < n) {
+            const int4 x = *reinterpret_cast<const int4*>(cnt + i);
+            v[g] = (uint32_t)x.x, v[g + 1] = (uint32_t)x.y, v[g + 2] = (uint32_t)x.z, v[g + 3] = (uint32_t)x.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[g + k] = i + k < n ? (uint32_t)cnt[i + k] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) s += v[g + k];
     }
     uint32_t tot;
     uint32_t run = block_excl_scan<uint32_t, 4>(s, lds, tot) + partials[blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        int64_t i = t0 + k;
-        if (i < n) off[i] = (int32_t)run;
-        if (i == n) off[i] = (int32_t)run;
-        run += v[k];
+    for (int g = 0; g < SCAN_ITEMS; g += 4) {
+        const int64_t i = t0 + g;
+        int32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            o[k] = (int32_t)run;
+            run += v[g + k];
+        }
+        if (VEC && i + 3 < n) {
+            *reinterpret_cast<int4*>(off + i) = make_int4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i + k <= n) off[i + k] = o[k];   // off[n] = the total
+        }
     }
 }
 
@@ -986,7 +1007,10 @@ void launch_count_scan_reduce(const int32_t* cnt, int64_t n, uint32_t* partials,
 void launch_count_scan_apply(const int32_t* cnt, int64_t n, const uint32_t* partials, int32_t* off,
                              hipStream_t s) {
     int64_t tiles = scan_tiles(n + 1);
-    hipLaunchKernelGGL(k_count_apply, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, cnt, n, partials, off);
+    if (((uintptr_t)cnt | (uintptr_t)off) % 16 == 0)
+        hipLaunchKernelGGL(k_count_apply<true>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, cnt, n, partials, off);
+    else
+        hipLaunchKernelGGL(k_count_apply<false>, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, s, cnt, n, partials, off);
 }
 
 void launch_checksum(const uint32_t* p, int64_t n, unsigned long long* out, hipStream_t s) {
