@@ -1025,14 +1025,16 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
 #pragma unroll
         for (int m = 0; m < 8; m++) acc[m] = 0.0f;
         int cx = 0, cy = 0, cz = 0;
+        int cb = 0;   // brick index of the current cell's corner 0
         bool have = false;
         auto flush = [&](int mask_keep, int axis_bit) {
 #pragma unroll
             for (int m = 0; m < 8; m++) {
                 if (((m >> axis_bit) & 1) == mask_keep) continue;
                 if (acc[m] != 0.0f) {
-                    // the cell is inside the padded brick (samples stay in the interior range)
-                    const int vi = __mul24(cz + (m >> 2), BR * BR) + __mul24(cy + ((m >> 1) & 1), BR) + (cx + (m & 1));
+                    // the cell is inside the padded brick (samples stay in the interior range);
+                    // corner m is corner 0 plus a constant (an immediate offset of the LDS access)
+                    const int vi = cb + ((m >> 2) * BR * BR + ((m >> 1) & 1) * BR + (m & 1));
                     // relaxed atomics: plain LDS byte stores and loads the compiler may not fold
                     __hip_atomic_store(&claim[vi], (uint8_t)lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     if (__hip_atomic_load(&claim[vi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) == (uint8_t)lane)
@@ -1073,6 +1075,7 @@ __global__ __launch_bounds__(256) void k_ea_backward(EABackParams p) {
                         else shift_corners<2>(acc, keep != 0);
                     }
                 }
+                if (!have || !(ix == cx && iy == cy && iz == cz)) cb = __mul24(iz, BR * BR) + __mul24(iy, BR) + ix;
                 cx = ix, cy = iy, cz = iz;
                 have = true;
                 // gradient only (no decision depends on it): fused products
