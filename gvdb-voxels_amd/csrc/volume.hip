@@ -745,8 +745,10 @@ int rebuild_topology(gvdbx_volume* v, int64_t n, float radius, const float orig[
     if (v->ncloud == 0) {
         TRY(ensure(v, v->bbox, sizeof(BBoxAccum) * (1 + (size_t)bbox_parts(std::max<int64_t>(n, 1)))));
         TRY(ensure_host(v, 4096));
-        launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
+        // the bounding-box pass writes every field of the accumulator; only an
+        // empty point set needs the initial (empty) box
         if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), v->bbox.as<BBoxAccum>() + 1, s);
+        else launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
         TRY(check_launch("bounding box"));
         GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
         TRY(sync(v));
@@ -1085,8 +1087,10 @@ int incremental_topology(gvdbx_volume* v, int64_t n, float radius, const float o
     // GetBoundingBox (VG.cpp:5404-5467)
     TRY(ensure(v, v->bbox, sizeof(BBoxAccum) * (1 + (size_t)bbox_parts(std::max<int64_t>(n, 1)))));
     TRY(ensure_host(v, 4096));
-    launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
+    // the bounding-box pass writes every field of the accumulator; only an
+    // empty point set needs the initial (empty) box
     if (n > 0) launch_bbox(v->pos, n, orig[0], orig[1], orig[2], v->bbox.as<BBoxAccum>(), v->bbox.as<BBoxAccum>() + 1, s);
+    else launch_bbox_init(v->bbox.as<BBoxAccum>(), s);
     TRY(check_launch("bounding box"));
     GVDBX_CHECK(hipMemcpyAsync(v->host, v->bbox.p, sizeof(BBoxAccum), hipMemcpyDeviceToHost, s));
     TRY(sync(v));
